@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Throughput of the DEM time step on one MI355X.
+
+Workload (default): 1,000,000 polydisperse FP64 spheres, radii U[0.8, 1.2] mm,
+triply periodic cube at solid fraction 0.30 (random sequential addition),
+v ~ N(0, 0.1^2) m/s, no gravity, Hertz + Coulomb contact (E* 1e7 Pa,
+G* = E*/4, en 0.85, mu 0.4), cell edge 2 * 1.2 * r_max, dt 5e-6 s.
+
+One step = one kick-drift-kick time step of every particle (binning, contact
+forces, integration).  value = particles * timed steps / seconds of the timed
+region, state resident in HBM.
+
+Prints one JSON line with the throughput, the roofline of the dominant kernel
+(the fused force + integrate kernel, timed with HIP events on its stream) and a
+CPU baseline (the oracle restatement of the reference's CPU step, one thread,
+on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s at 1M spheres; batched calibration gradients/s vs plain"
+HBM_PEAK_GBS = 8000.0
+# compulsory HBM bytes per particle-step (x, v, w, q, r read; x, v, w, q
+# written; cell key and permutation) and per contact (one contact record
+# written and read): the algorithmic traffic of one step
+BYTES_PER_PARTICLE_STEP = 232.0
+BYTES_PER_CONTACT = 112.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_force_periodic_hertz_1M_traffic.txt")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cached_workload(n, seed):
+    from pacific_amd import workloads
+    cache_dir = os.path.join(os.environ.get("TMPDIR", "/tmp"), "gdem_cache")
+    os.makedirs(cache_dir, exist_ok=True)
+    path = os.path.join(cache_dir, f"periodic_hertz_{n}_s{seed}.npz")
+    wl = None
+    if os.path.exists(path):
+        try:
+            z = np.load(path)
+            wl = workloads.periodic_hertz(16, seed)  # structure; arrays replaced below
+            r = z["radius"]
+            wl["particles"] = workloads._particles(z["x"], z["v"], r)
+            wl["size"] = tuple(z["size"])
+            wl["cell_size"] = float(z["cell_size"])
+            wl["name"] = f"periodic_hertz_{n}"
+        except Exception:
+            wl = None
+    if wl is None:
+        log(f"generating {n} spheres by random sequential addition (seed {seed})")
+        t = time.time()
+        wl = workloads.periodic_hertz(n, seed)
+        log(f"generated in {time.time() - t:.1f} s")
+        p = wl["particles"]
+        np.savez(path, x=p["x"], v=p["v"], radius=p["radius"], size=np.array(wl["size"]),
+                 cell_size=wl["cell_size"])
+    return wl
+
+
+def read_traffic():
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(wl, steps, warm=2):
+    """The oracle (restatement of the reference CPU step), one thread."""
+    from oracle import py_oracle
+    o = py_oracle.Oracle(wl)
+    o.prime()
+    o.step(warm, wl["dt"])
+    t = time.perf_counter()
+    o.step(steps, wl["dt"])
+    el = time.perf_counter() - t
+    n = len(wl["particles"]["radius"])
+    return n * steps / el, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the DEM step is not sharded "
+                                   "across GPUs (see DESIGN.md, multi-GPU design)"}))
+        sys.exit(2)
+
+    from pacific_amd import gdem
+    wl = cached_workload(args.n, args.seed)
+    n = len(wl["particles"]["radius"])
+    dt = wl["dt"]
+    sysg = gdem.build_system(wl)
+    nc, edge = sysg.grid()
+    log(f"grid {nc}, edge {edge[0]:.4e} m, L {wl['size'][0]:.5f} m")
+    sysg.prime()
+    sysg.synchronize()
+    sysg.step(args.warmup, dt)
+    sysg.synchronize()
+    log(f"warmup {args.warmup} steps done")
+
+    # timed region: K steps, no instrumentation
+    sysg.reset_counters()
+    sysg.synchronize()
+    t0 = time.perf_counter()
+    sysg.step(args.steps, dt)
+    sysg.synchronize()
+    el = time.perf_counter() - t0
+    cnt = sysg.counters()
+    mean_contacts = cnt["contacts"] / max(cnt["force_evals"], 1)
+    value = n * args.steps / el
+    log(f"timed {args.steps} steps in {el:.4f} s: {value:.4e} particle-steps/s, "
+        f"mean contacts {mean_contacts:.0f}")
+
+    # force-kernel duration with HIP events on the context's stream
+    ksteps = min(args.steps, 200)
+    sysg.reset_counters()
+    sysg.set_kernel_timing(True)
+    sysg.step(ksteps, dt)
+    sysg.synchronize()
+    kc = sysg.counters()
+    sysg.set_kernel_timing(False)
+    k_ms = kc["force_kernel_ms"] / max(kc["timed_launches"], 1)
+    k_contacts = kc["contacts"] / max(kc["force_evals"], 1)
+    alg_bytes = BYTES_PER_PARTICLE_STEP * n + BYTES_PER_CONTACT * k_contacts
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
+    step_bytes = BYTES_PER_PARTICLE_STEP * n + BYTES_PER_CONTACT * mean_contacts
+    step_achieved = step_bytes * args.steps / el / 1e9
+    log(f"force kernel {k_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s algorithmic")
+    sysg.close()
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"CPU baseline: oracle, 1 thread, {args.cpu_steps} steps of the same workload")
+        cv, cel = cpu_baseline(wl, args.cpu_steps)
+        log(f"CPU {cv:.4e} particle-steps/s ({cel:.1f} s)")
+        cpu = {"value": cv, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{n} spheres of the same workload, {args.cpu_steps} steps after 2 "
+                         f"warm-up steps, single thread ({cel:.1f} s)"}
+
+    traffic = read_traffic()
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"periodic_hertz_{n}", "n_particles": n, "solid_fraction": 0.30,
+                   "radius_mm": [0.8, 1.2], "contact": "Hertz+Coulomb", "dt": dt,
+                   "cell_factor": 1.2, "grid": nc, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_force<MODE_MOVE>", "kernel_us": k_ms * 1e3,
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "mean_contacts": k_contacts},
+        "step_roofline": {"achieved": step_achieved, "unit": "GB/s",
+                          "frac": step_achieved / HBM_PEAK_GBS},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,175 @@
+/*
+ * gdem.h -- C ABI of the MI355X-native Grains3D DEM time step (libgdem.so).
+ *
+ * One context = one HIP device + one stream + the device-resident particle
+ * state.  The reference is object-per-particle C++ (Grains3D); its plugin seams
+ * are batched here, one call per phase of Grains::Simulation
+ * (src/Grains3D/Grains/Base/src/Grains.cpp:344-364):
+ *
+ *   reference call (src/Grains3D/Grains/...)            C entry point
+ *   --------------------------------------------------  ------------------------
+ *   LinkedCell::set (Base/include/LinkedCell.hh:249)    gdem_create
+ *   AllComponents / Particle construction               gdem_set_particles
+ *   LinkedCell::Link(Obstacle*) (LinkedCell.cpp:1371)   gdem_add_wall
+ *   ContactBuilderFactory::define + Hooke/Hertz ctor    gdem_set_contact_model
+ *     (Component/src/ContactBuilderFactory.cpp:81-192)
+ *   AppCollision::LinkUpdate (Base/include/AppCollision.hh:90)
+ *                                                        gdem_link_update
+ *   App::ComputeForces (Base/include/App.hh:38-39) with  gdem_compute_forces
+ *     AllComponents::InitializeForces and every
+ *     ContactForceModel::computeForces
+ *     (Component/include/ContactForceModel.hh:66-68)
+ *   TimeIntegrator::Move via AllComponents::Move         gdem_move
+ *     (TimeIntegration/include/TimeIntegrator.hh:47-57)
+ *   TimeIntegrator::advanceVelocity                      gdem_advance_velocity
+ *     (TimeIntegrator.hh:59-65)
+ *   Grains::do_before_time_stepping a0 (Grains.cpp:169-174)
+ *                                                        gdem_prime
+ *   Grains::Simulation loop body (Grains.cpp:286-452)   gdem_step
+ *   Particle / kinematics getters                        gdem_get_state
+ *   C-ABI precedent: InterfaceCBasilisk/include/InterfaceGrains3DBasilisk.h:11-33
+ *
+ * Conventions
+ *   - Return 0 on success, else one of GDEM_ERR_*; the exception the reference
+ *     would throw maps to CONTACT (ContactError, RigidBodyWithCrust.cpp:624-630),
+ *     MOTION (MotionError, Particle.cpp:447-457) or SIMULATION
+ *     (SimulationError, LinkedCell.cpp:1506-1531).  gdem_last_error names the
+ *     offending ids (smallest particle id among the offenders; a wall is -1-k).
+ *   - Host pointers are borrowed for the duration of the call; the context owns
+ *     all device memory.  Particle arrays are in particle-id order; vectors are
+ *     xyz-interleaved (3 doubles per particle), quaternions (x, y, z, w).
+ *   - Calls are ordered on the context's stream.  gdem_step, gdem_move,
+ *     gdem_advance_velocity and gdem_compute_forces return without waiting;
+ *     errors raised on the device are reported by the next synchronizing call
+ *     (gdem_synchronize, gdem_get_*).  A context is not thread-safe.
+ *   - Arithmetic is IEEE FP64 without FMA contraction; the result of a call
+ *     does not depend on launch shape or run (no floating-point atomics).
+ */
+#ifndef GDEM_H
+#define GDEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GDEM_OK 0
+#define GDEM_ERR_CONTACT 1
+#define GDEM_ERR_MOTION 2
+#define GDEM_ERR_SIMULATION 3
+#define GDEM_ERR_HIP 4
+#define GDEM_ERR_ARG 5
+#define GDEM_ERR_CAPACITY 6
+
+#define GDEM_MODEL_HOOKE 1
+#define GDEM_MODEL_HERTZ 2
+
+#define GDEM_MAX_MATERIALS 32
+#define GDEM_MAX_WALLS 64
+
+typedef struct gdem_ctx gdem_ctx;
+
+typedef struct {
+  double origin[3];     /* domain origin (Grains.cpp:545-579) */
+  double size[3];       /* domain size */
+  int32_t periodic[3];  /* 1 = periodic (needs >= 3 cells in that direction) */
+  double cell_size;     /* requested cell edge = 2 * CellSizeFactor * max radius
+                           (Grains.cpp:1013-1022, 2522-2526) */
+  double gravity[3];    /* GrainsExec::m_vgravity */
+  int32_t device;       /* HIP device ordinal */
+  int32_t reserved;
+} gdem_config;
+
+/* LinkedCell::set (Base/src/LinkedCell.cpp:47-419) + the context. */
+int gdem_create(const gdem_config* cfg, gdem_ctx** out);
+int gdem_destroy(gdem_ctx* ctx);
+
+/* Spheres (Particle(RigidBodyWithCrust*, density, mat, type), Particle.cpp:179-232;
+   kinematics setters ParticleKinematics.cpp:170-222).  w is the world angular
+   velocity; mass, inertia and equivalent radius are derived as Particle.cpp:158-164.
+   Replaces any previous particle set. */
+int gdem_set_particles(gdem_ctx* ctx, int64_t n, const double* x, const double* v,
+                       const double* w, const double* q, const double* radius,
+                       const double* density, const double* crust,
+                       const int32_t* material);
+
+/* Static box obstacle (SimpleObstacle with a Box convex; Box(LX,LY,LZ) uses
+   full edges, here half extents).  basis is the row-major rotation of its
+   Transform, centre its origin. */
+int gdem_add_wall(gdem_ctx* ctx, const double* basis, const double* centre,
+                  const double* half_extent, double crust, int32_t material);
+
+/* HookeContactForceModel(map) (HookeContactForceModel.cpp:11-20):
+     kind GDEM_MODEL_HOOKE, p = {kn, en, etat, muc, kr}
+   HertzContactForceModel(map) (HertzContactForceModel.cpp:12-22):
+     kind GDEM_MODEL_HERTZ, p = {Es, en, Gs, muc, kr}
+   kr must be 0 (no rolling resistance). */
+int gdem_set_contact_model(gdem_ctx* ctx, int32_t mat_a, int32_t mat_b, int32_t kind,
+                           const double* p);
+
+/* AppCollision::LinkUpdate -> LinkedCell::LinkUpdate (LinkedCell.cpp:1452-1547):
+   counting sort of the particles by Morton cell key. */
+int gdem_link_update(gdem_ctx* ctx);
+
+/* Grains::computeParticlesForceAndTorque (Grains.cpp:459-482): weight, then
+   LinkedCell::ComputeForces (LinkedCell.cpp:1019-1103) with every contact
+   force (Hooke/Hertz + Coulomb) accumulated per particle in a fixed order.
+   Re-bins first if positions changed since the last binning. */
+int gdem_compute_forces(gdem_ctx* ctx);
+
+/* AllComponents::Move (AllComponents.cpp:151-166) -> ParticleKinematics::Move
+   (ParticleKinematics.cpp:62-104) -> SecondOrderLeapFrog::Move
+   (SecondOrderLeapFrog.cpp:44-67). */
+int gdem_move(gdem_ctx* ctx, double dt_vel, double dt_disp);
+
+/* AllComponents::advanceParticlesVelocity (AllComponents.cpp:224-231) ->
+   SecondOrderLeapFrog::advanceVelocity (SecondOrderLeapFrog.cpp:74-94). */
+int gdem_advance_velocity(gdem_ctx* ctx, double dt_vel);
+
+/* Link + a0 (Grains::do_before_time_stepping, Grains.cpp:169-174). */
+int gdem_prime(gdem_ctx* ctx);
+
+/* nsteps of the kick-drift-kick loop (Grains.cpp:344-364), fused: each step is
+   one binning pass plus one force kernel whose tail applies the closing
+   half-kick and the next step's kick and drift.  Primes first if needed. */
+int gdem_step(gdem_ctx* ctx, int32_t nsteps, double dt);
+
+/* Waits for the stream; returns the first deferred device error. */
+int gdem_synchronize(gdem_ctx* ctx);
+
+/* Particle state in id order (any pointer may be NULL).  F and T are the
+   torsor of the last force evaluation (Torsor, Component/src/Torsor.cpp). */
+int gdem_get_state(gdem_ctx* ctx, double* x, double* v, double* w, double* q,
+                   double* F, double* T);
+
+/* Contact list of the next force evaluations, kept for the last one.
+   capacity_per_particle bounds the contacts a particle can be the first body
+   of; 0 disables recording.  Records are in canonical order: sorted by (id0,
+   id1) with id0 the first body of the pair as the reference orders it
+   (LinkedCell.cpp:1034-1060: same cell -> lower id first here; otherwise the
+   body whose half stencil holds the other's cell).  A wall partner is -1-k. */
+int gdem_set_contact_recording(gdem_ctx* ctx, int32_t capacity_per_particle);
+int gdem_get_contacts(gdem_ctx* ctx, int64_t capacity, int64_t* n_out, int32_t* id0,
+                      int32_t* id1, double* point, double* overlap_vec,
+                      double* overlap, double* force);
+
+int gdem_last_error(gdem_ctx* ctx, int32_t* code, int32_t* id0, int32_t* id1);
+
+/* Counters since the last reset: force evaluations, contacts (pairs with
+   overlap < 0, walls included) summed over them, and the event-timed total of
+   the force kernel in ms when timing is on (events on the context's stream). */
+int gdem_set_kernel_timing(gdem_ctx* ctx, int32_t on);
+int gdem_get_counters(gdem_ctx* ctx, int64_t* force_evals, int64_t* contacts,
+                      double* force_kernel_ms, int64_t* timed_launches);
+int gdem_reset_counters(gdem_ctx* ctx);
+
+/* Grid actually used: cells per direction and edge lengths. */
+int gdem_get_grid(gdem_ctx* ctx, int32_t* nc, double* edge);
+
+const char* gdem_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,910 @@
+/*
+ * gdem_oracle.c -- CPU restatement of the Grains3D sphere DEM step.
+ *
+ * TEST INFRASTRUCTURE ONLY (the correctness oracle and the timed CPU
+ * baseline); see gdem_oracle.h.  Built with -O2 -std=c99 -ffp-contract=off.
+ *
+ * All paths below are relative to src/Grains3D/Grains/ of the reference.
+ * Every expression keeps the reference's operand order so that rounding is
+ * identical: a*b*c is (a*b)*c, "double * Vector3" multiplies each component,
+ * cross products and norms follow Vector3.cpp / Group3.cpp.
+ */
+#include "gdem_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* Basic.hh:22-29 */
+#define ORC_PI 3.14159265358979323846
+#define ORC_EPSILON 1.0e-10
+
+typedef struct { double x, y, z; } v3;
+
+static inline v3 mk(double x, double y, double z) { v3 r = {x, y, z}; return r; }
+static inline v3 ld(const double* p) { return mk(p[0], p[1], p[2]); }
+static inline void st(double* p, v3 a) { p[0] = a.x; p[1] = a.y; p[2] = a.z; }
+/* Group3.cpp:149-196 */
+static inline v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 scl(v3 a, double d) { return mk(a.x * d, a.y * d, a.z * d); }
+static inline v3 dvd(v3 a, double d) { return mk(a.x / d, a.y / d, a.z / d); }
+static inline v3 neg(v3 a) { return mk(-a.x, -a.y, -a.z); }
+static inline double dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+/* Vector3.cpp:114-119 */
+static inline v3 cross(v3 a, v3 b) {
+  return mk(a.y * b.z - a.z * b.y, -a.x * b.z + a.z * b.x, a.x * b.y - a.y * b.x);
+}
+/* Vector3.cpp:147-151 */
+static inline double norm(v3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+/* Group3.cpp:390-395 */
+static inline v3 round3(v3 a) {
+  return mk(fabs(a.x) < ORC_EPSILON ? 0. : a.x, fabs(a.y) < ORC_EPSILON ? 0. : a.y,
+            fabs(a.z) < ORC_EPSILON ? 0. : a.z);
+}
+
+/* Quaternion (vector part v, scalar w), Quaternion.hh:250-251 */
+typedef struct { v3 v; double w; } quat;
+static inline quat qmk(v3 v, double w) { quat q; q.v = v; q.w = w; return q; }
+/* Quaternion.cpp:74-77 */
+static inline quat qconj(quat q) { return qmk(neg(q.v), q.w); }
+/* Quaternion.cpp:95-100 ([0,lhs] x q, reached through operator,(Vector3,Quaternion)) */
+static inline quat qleftvec(quat q, v3 lhs) {
+  double tmp = dot(neg(lhs), q.v);
+  v3 vt = add(cross(lhs, q.v), scl(lhs, q.w));
+  return qmk(vt, tmp);
+}
+/* Quaternion.cpp:108-113 */
+static inline v3 qmulv(quat p, quat r) {
+  return add(add(cross(p.v, r.v), scl(r.v, p.w)), scl(p.v, r.w));
+}
+/* Quaternion.cpp:442-448 */
+static inline quat qmul(quat p, quat r) {
+  double tmp = (p.w * r.w) - dot(p.v, r.v);
+  v3 vt = add(add(cross(p.v, r.v), scl(r.v, p.w)), scl(p.v, r.w));
+  return qmk(vt, tmp);
+}
+
+typedef struct {
+  int kind;
+  double p[5];
+  double beta;      /* HookeContactForceModel.cpp:19 / HertzContactForceModel.cpp:20 */
+  double m2sqrt56;  /* HertzContactForceModel.cpp:21 */
+} model_t;
+
+typedef struct {
+  double basis[9], origin[3], ext[3];
+  double inv_basis[9], inv_origin[3];
+  double bb_centre[3], bb_ext[3];
+  double crust, req, corners[8][3];
+  int material;
+} wall_t;
+
+struct orc_sys {
+  orc_config cfg;
+  int nc[3];
+  double edge[3], lc_origin[3];
+  int ncell;
+  int n;
+  double *x, *v, *w, *wbf, *q, *basis, *F, *T;
+  double *r, *crust, *mass, *inertia, *req;
+  int32_t* mat;
+  int nwalls;
+  wall_t walls[ORC_MAX_WALLS];
+  model_t models[ORC_MAX_MATERIALS][ORC_MAX_MATERIALS];
+  /* cell lists: doubly linked, head insertion (Cell.cpp:90-93, push_front) */
+  int *head, *next, *prev, *cell;
+  int stencil[13][3];
+  /* contact log */
+  int log_cap, log_n;
+  int32_t *log_id0, *log_id1;
+  double *log_pt, *log_ov, *log_d, *log_f;
+  int64_t ncand;
+  /* error */
+  int err_code;
+  int32_t err_id0, err_id1;
+  double err_value, err_limit;
+};
+
+static int set_error(orc_sys* s, int code, int id0, int id1, double val, double lim) {
+  if (s->err_code == ORC_OK) {
+    s->err_code = code;
+    s->err_id0 = id0;
+    s->err_id1 = id1;
+    s->err_value = val;
+    s->err_limit = lim;
+  }
+  return code;
+}
+
+/* ---------------------------------------------------------------------- */
+/* LinkedCell::set (Base/src/LinkedCell.cpp:47-119) without ghost layers:
+   n_d = (int)((L_d + EPSILON) / h), edge_d = L_d / n_d. */
+int orc_create(const orc_config* cfg, orc_sys** out) {
+  orc_sys* s = (orc_sys*)calloc(1, sizeof(orc_sys));
+  if (!s) return ORC_ERR_ARG;
+  s->cfg = *cfg;
+  for (int d = 0; d < 3; ++d) {
+    int n = 1;
+    double e = cfg->size[d];
+    if (cfg->cell_size > ORC_EPSILON) {
+      n = (int)((cfg->size[d] + ORC_EPSILON) / cfg->cell_size);
+      if (!n) n = 1;
+      e = cfg->size[d] / n;
+    }
+    if (cfg->periodic[d] && n < 3) { free(s); return ORC_ERR_ARG; }
+    s->nc[d] = n;
+    s->edge[d] = e;
+    s->lc_origin[d] = cfg->origin[d];
+  }
+  s->ncell = s->nc[0] * s->nc[1] * s->nc[2];
+  /* LinkedCell.cpp:965-967, offsets (di, dj, dk) per column */
+  static const int cel[3][13] = {{-1, 0, +1, +1, -1, 0, +1, -1, 0, +1, -1, 0, +1},
+                                 {0, 0, 0, 0, +1, +1, +1, +1, +1, +1, +1, +1, +1},
+                                 {+1, +1, +1, 0, +1, +1, +1, 0, 0, 0, -1, -1, -1}};
+  for (int j = 0; j < 13; ++j)
+    for (int d = 0; d < 3; ++d) s->stencil[j][d] = cel[d][j];
+  s->head = (int*)malloc(sizeof(int) * (size_t)s->ncell);
+  for (int c = 0; c < s->ncell; ++c) s->head[c] = -1;
+  *out = s;
+  return ORC_OK;
+}
+
+static void free_particles(orc_sys* s) {
+  free(s->x); free(s->v); free(s->w); free(s->wbf); free(s->q); free(s->basis);
+  free(s->F); free(s->T); free(s->r); free(s->crust); free(s->mass);
+  free(s->inertia); free(s->req); free(s->mat); free(s->next); free(s->prev);
+  free(s->cell);
+}
+
+void orc_destroy(orc_sys* s) {
+  if (!s) return;
+  free_particles(s);
+  free(s->head);
+  free(s->log_id0); free(s->log_id1); free(s->log_pt); free(s->log_ov);
+  free(s->log_d); free(s->log_f);
+  free(s);
+}
+
+/* Sphere.cpp:152-155 (volume), Sphere.cpp:61-70 (inertia), Particle.cpp:158-164
+   (m = rho V, I *= rho), Component.cpp:205-208 (equivalent radius),
+   ParticleKinematics.cpp:197-205 (w_bf from w at setup). */
+int orc_set_particles(orc_sys* s, int n, const double* x, const double* v,
+                      const double* w, const double* q, const double* radius,
+                      const double* density, const double* crust,
+                      const int32_t* material) {
+  if (n < 0) return ORC_ERR_ARG;
+  free_particles(s);
+  s->n = n;
+  size_t n3 = (size_t)n * 3;
+  s->x = (double*)malloc(sizeof(double) * n3);
+  s->v = (double*)malloc(sizeof(double) * n3);
+  s->w = (double*)malloc(sizeof(double) * n3);
+  s->wbf = (double*)malloc(sizeof(double) * n3);
+  s->q = (double*)malloc(sizeof(double) * (size_t)n * 4);
+  s->basis = (double*)malloc(sizeof(double) * (size_t)n * 9);
+  s->F = (double*)calloc(n3, sizeof(double));
+  s->T = (double*)calloc(n3, sizeof(double));
+  s->r = (double*)malloc(sizeof(double) * (size_t)n);
+  s->crust = (double*)malloc(sizeof(double) * (size_t)n);
+  s->mass = (double*)malloc(sizeof(double) * (size_t)n);
+  s->inertia = (double*)malloc(sizeof(double) * (size_t)n);
+  s->req = (double*)malloc(sizeof(double) * (size_t)n);
+  s->mat = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  s->next = (int*)malloc(sizeof(int) * (size_t)n);
+  s->prev = (int*)malloc(sizeof(int) * (size_t)n);
+  s->cell = (int*)malloc(sizeof(int) * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    double rr = radius[i], rho = density[i];
+    double vol = 4.0 * ORC_PI * pow(rr, 3.) / 3.0;
+    double in0 = 8.0 * ORC_PI * pow(rr, 5.) / 15.0;
+    s->r[i] = rr;
+    s->crust[i] = crust[i];
+    s->mass[i] = rho * vol;
+    s->inertia[i] = in0 * rho;
+    s->req[i] = pow((0.75 / ORC_PI) * vol, 1. / 3.);
+    s->mat[i] = material ? material[i] : 0;
+    memcpy(s->x + 3 * i, x + 3 * i, sizeof(double) * 3);
+    memcpy(s->v + 3 * i, v + 3 * i, sizeof(double) * 3);
+    memcpy(s->w + 3 * i, w + 3 * i, sizeof(double) * 3);
+    memcpy(s->q + 4 * i, q + 4 * i, sizeof(double) * 4);
+    quat qq = qmk(mk(q[4 * i], q[4 * i + 1], q[4 * i + 2]), q[4 * i + 3]);
+    quat pc = qconj(qq);
+    st(s->wbf + 3 * i, qmulv(pc, qleftvec(qq, ld(w + 3 * i))));
+    /* basis = rotation matrix of q (orientation given at insertion) */
+    double qx = qq.v.x, qy = qq.v.y, qz = qq.v.z, qw = qq.w;
+    double* b = s->basis + 9 * i;
+    b[0] = 1. - 2. * (qy * qy + qz * qz); b[1] = 2. * (qx * qy - qz * qw); b[2] = 2. * (qx * qz + qy * qw);
+    b[3] = 2. * (qx * qy + qz * qw); b[4] = 1. - 2. * (qx * qx + qz * qz); b[5] = 2. * (qy * qz - qx * qw);
+    b[6] = 2. * (qx * qz - qy * qw); b[7] = 2. * (qy * qz + qx * qw); b[8] = 1. - 2. * (qx * qx + qy * qy);
+    s->cell[i] = -1;
+    s->next[i] = s->prev[i] = -1;
+  }
+  for (int c = 0; c < s->ncell; ++c) s->head[c] = -1;
+  return ORC_OK;
+}
+
+/* Transform::setToInverseTransform (Geometry/src/Transform.cpp:71-77) for a
+   rotation basis; Convex::bbox (Convex.cpp:29-41) + BBox::setValue
+   (BBox.cpp:235-239) for the wall bounding box; Box::setCornersFaces
+   (Box.cpp:92-111); Box::getVolume (Box.cpp:273-279). */
+int orc_add_wall(orc_sys* s, const double* basis, const double* centre,
+                 const double* half_extent, double crust, int material) {
+  if (s->nwalls >= ORC_MAX_WALLS || material < 0 || material >= ORC_MAX_MATERIALS)
+    return ORC_ERR_ARG;
+  wall_t* wl = &s->walls[s->nwalls++];
+  memset(wl, 0, sizeof(*wl));
+  memcpy(wl->basis, basis, sizeof(double) * 9);
+  memcpy(wl->origin, centre, sizeof(double) * 3);
+  memcpy(wl->ext, half_extent, sizeof(double) * 3);
+  wl->crust = crust;
+  wl->material = material;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) wl->inv_basis[3 * i + j] = basis[3 * j + i];
+  for (int i = 0; i < 3; ++i) {
+    v3 row = mk(wl->inv_basis[3 * i], wl->inv_basis[3 * i + 1], wl->inv_basis[3 * i + 2]);
+    wl->inv_origin[i] = dot(neg(row), ld(centre));
+  }
+  double e0 = half_extent[0], e1 = half_extent[1], e2 = half_extent[2];
+  double mn[3], mx[3];
+  for (int i = 0; i < 3; ++i) {
+    v3 row = mk(basis[3 * i], basis[3 * i + 1], basis[3 * i + 2]);
+    v3 nrow = neg(row);
+    /* Box::support (Box.cpp:307-316) */
+    v3 sm = norm(nrow) < ORC_EPSILON ? mk(0., 0., 0.)
+                                      : mk(nrow.x < 0. ? -e0 : e0, nrow.y < 0. ? -e1 : e1,
+                                           nrow.z < 0. ? -e2 : e2);
+    v3 sp = norm(row) < ORC_EPSILON ? mk(0., 0., 0.)
+                                     : mk(row.x < 0. ? -e0 : e0, row.y < 0. ? -e1 : e1,
+                                          row.z < 0. ? -e2 : e2);
+    mn[i] = centre[i] + dot(row, sm);
+    mx[i] = centre[i] + dot(row, sp);
+  }
+  for (int i = 0; i < 3; ++i) {
+    wl->bb_ext[i] = (mx[i] - mn[i]) / 2.;
+    wl->bb_centre[i] = mn[i] + wl->bb_ext[i];
+  }
+  static const int sg[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1},
+                               {-1, 1, -1}, {1, 1, -1}, {1, 1, 1}, {-1, 1, 1}};
+  for (int k = 0; k < 8; ++k)
+    for (int d = 0; d < 3; ++d) wl->corners[k][d] = sg[k][d] < 0 ? -half_extent[d] : half_extent[d];
+  double vol = 8.0 * e0 * e1 * e2;
+  wl->req = pow((0.75 / ORC_PI) * vol, 1. / 3.);
+  return ORC_OK;
+}
+
+/* HookeContactForceModel ctor (Component/src/HookeContactForceModel.cpp:11-20),
+   HertzContactForceModel ctor (HertzContactForceModel.cpp:12-22);
+   ContactBuilderFactory::contactForceModel (ContactBuilderFactory.cpp:22-26)
+   keys a model by the OR of two material bits, i.e. an unordered pair. */
+int orc_set_contact_model(orc_sys* s, int a, int b, int kind, const double* p) {
+  if (a < 0 || b < 0 || a >= ORC_MAX_MATERIALS || b >= ORC_MAX_MATERIALS) return ORC_ERR_ARG;
+  if (kind != ORC_MODEL_HOOKE && kind != ORC_MODEL_HERTZ) return ORC_ERR_ARG;
+  model_t m;
+  memset(&m, 0, sizeof(m));
+  m.kind = kind;
+  memcpy(m.p, p, sizeof(double) * 5);
+  double en = p[1];
+  m.beta = log(en) / sqrt(ORC_PI * ORC_PI + log(en) * log(en));
+  m.m2sqrt56 = -2. * sqrt(5. / 6.);
+  s->models[a][b] = m;
+  s->models[b][a] = m;
+  return ORC_OK;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Cell::GetCell (Base/src/Cell.cpp:172-184).  Returns the cell number
+   j*nk*ni + k*ni + i (LinkedCell.cpp:1165-1168) or -1 outside the grid. */
+static int get_cell(const orc_sys* s, const double* p, int* ijk) {
+  int id[3];
+  for (int d = 0; d < 3; ++d) {
+    id[d] = (int)floor((p[d] - s->lc_origin[d]) / s->edge[d]);
+    if (s->cfg.periodic[d]) {
+      if (id[d] >= s->nc[d]) id[d] = s->nc[d] - 1;
+      if (id[d] < 0) id[d] = 0;
+    }
+    if (id[d] < 0 || id[d] >= s->nc[d]) return -1;
+  }
+  if (ijk) { ijk[0] = id[0]; ijk[1] = id[1]; ijk[2] = id[2]; }
+  return id[1] * s->nc[2] * s->nc[0] + id[2] * s->nc[0] + id[0];
+}
+
+static void cell_push_front(orc_sys* s, int c, int p) {
+  s->prev[p] = -1;
+  s->next[p] = s->head[c];
+  if (s->head[c] >= 0) s->prev[s->head[c]] = p;
+  s->head[c] = p;
+  s->cell[p] = c;
+}
+
+static void cell_remove(orc_sys* s, int p) {
+  int c = s->cell[p];
+  if (s->prev[p] >= 0) s->next[s->prev[p]] = s->next[p];
+  else s->head[c] = s->next[p];
+  if (s->next[p] >= 0) s->prev[s->next[p]] = s->prev[p];
+  s->next[p] = s->prev[p] = -1;
+  s->cell[p] = -1;
+}
+
+/* LinkedCell::Link(Particle*) (LinkedCell.cpp:1352-1363), called at insertion
+   in particle order. */
+int orc_link(orc_sys* s) {
+  for (int c = 0; c < s->ncell; ++c) s->head[c] = -1;
+  for (int p = 0; p < s->n; ++p) {
+    int c = get_cell(s, s->x + 3 * p, NULL);
+    if (c < 0) return set_error(s, ORC_ERR_SIMULATION, p, -1, 0., 0.);
+    cell_push_front(s, c, p);
+  }
+  return ORC_OK;
+}
+
+/* LinkedCell::LinkUpdate / LinkUpdateActiveParticle (LinkedCell.cpp:1452-1547) */
+int orc_link_update(orc_sys* s) {
+  for (int p = 0; p < s->n; ++p) {
+    int c = get_cell(s, s->x + 3 * p, NULL);
+    if (c < 0) return set_error(s, ORC_ERR_SIMULATION, p, -1, 0., 0.);
+    if (c != s->cell[p]) {
+      cell_remove(s, p);
+      cell_push_front(s, c, p);
+    }
+  }
+  return ORC_OK;
+}
+
+/* ---------------------------------------------------------------------- */
+/* ClosestPointSPHERESPHERE (Geometry/src/RigidBodyWithCrust.cpp:596-640) */
+int orc_sphere_sphere(const double* a, double ra, double cra, const double* b,
+                      double rb, double crb, double* point, double* ovec,
+                      double* dist) {
+  v3 A = ld(a), B = ld(b);
+  v3 AB = sub(B, A);
+  double distance = norm(AB) - (ra + rb);
+  *dist = distance;
+  if (distance > 0.) return 0;
+  if (-distance >= cra + crb) return -1;
+  v3 c = add(A, dvd(scl(AB, ra + 0.5 * distance), norm(AB)));
+  v3 ov = scl(dvd(AB, norm(AB)), distance);
+  st(point, c);
+  st(ovec, ov);
+  return 1;
+}
+
+/* Box::ContactCornerSPHERE (Box.cpp:725-742) */
+static v3 box_corner(const wall_t* wl, v3 g, double r, int k, double* overlap) {
+  v3 cp = mk(0., 0., 0.);
+  v3 dist = sub(g, ld(wl->corners[k]));
+  double nd = norm(dist);
+  if (nd < r) {
+    *overlap = nd - r;
+    cp = sub(g, scl(dist, 1. + 0.5 * *overlap / nd));
+  }
+  return cp;
+}
+
+/* Box::ContactEdgeSPHERE (Box.cpp:750-769) */
+static v3 box_edge(const wall_t* wl, v3 g, double r, int k, int dir, double* overlap) {
+  v3 cp = mk(0., 0., 0.);
+  double pr[3] = {g.x, g.y, g.z};
+  pr[dir] = wl->corners[k][dir];
+  v3 dist = sub(ld(pr), ld(wl->corners[k]));
+  double nd = norm(dist);
+  if (nd < r) {
+    *overlap = nd - r;
+    cp = sub(g, scl(dist, 1. + 0.5 * *overlap / nd));
+  }
+  return cp;
+}
+
+/* Box::IntersectionPointSPHERE (Box.cpp:466-716), 27-region classification */
+static v3 box_sphere(const wall_t* wl, v3 g, double r, double* overlap) {
+  const double ex = wl->ext[0], ey = wl->ext[1], ez = wl->ext[2];
+  v3 cp = mk(0., 0., 0.);
+  double nd;
+  *overlap = 1.;
+  if (g.x > ex) {
+    if (g.y > ey) {
+      if (g.z > ez) cp = box_corner(wl, g, r, 6, overlap);
+      else if (g.z < -ez) cp = box_corner(wl, g, r, 5, overlap);
+      else cp = box_edge(wl, g, r, 5, 2, overlap);
+    } else if (g.y < -ey) {
+      if (g.z > ez) cp = box_corner(wl, g, r, 2, overlap);
+      else if (g.z < -ez) cp = box_corner(wl, g, r, 1, overlap);
+      else cp = box_edge(wl, g, r, 1, 2, overlap);
+    } else {
+      if (g.z > ez) cp = box_edge(wl, g, r, 6, 1, overlap);
+      else if (g.z < -ez) cp = box_edge(wl, g, r, 1, 1, overlap);
+      else {
+        nd = g.x - ex;
+        if (nd < r) { *overlap = nd - r; cp = mk(ex + 0.5 * *overlap, g.y, g.z); }
+      }
+    }
+  } else if (g.x < -ex) {
+    if (g.y > ey) {
+      if (g.z > ez) cp = box_corner(wl, g, r, 7, overlap);
+      else if (g.z < -ez) cp = box_corner(wl, g, r, 4, overlap);
+      else cp = box_edge(wl, g, r, 4, 2, overlap);
+    } else if (g.y < -ey) {
+      if (g.z > ez) cp = box_corner(wl, g, r, 3, overlap);
+      else if (g.z < -ez) cp = box_corner(wl, g, r, 0, overlap);
+      else cp = box_edge(wl, g, r, 0, 2, overlap);
+    } else {
+      if (g.z > ez) cp = box_edge(wl, g, r, 3, 1, overlap);
+      else if (g.z < -ez) cp = box_edge(wl, g, r, 0, 1, overlap);
+      else {
+        nd = -ex - g.x;
+        if (nd < r) { *overlap = nd - r; cp = mk(-ex - 0.5 * *overlap, g.y, g.z); }
+      }
+    }
+  } else {
+    if (g.y > ey) {
+      if (g.z > ez) cp = box_edge(wl, g, r, 6, 0, overlap);
+      else if (g.z < -ez) cp = box_edge(wl, g, r, 4, 0, overlap);
+      else {
+        nd = g.y - ey;
+        if (nd < r) { *overlap = nd - r; cp = mk(g.x, ey + 0.5 * *overlap, g.z); }
+      }
+    } else if (g.y < -ey) {
+      if (g.z > ez) cp = box_edge(wl, g, r, 3, 0, overlap);
+      else if (g.z < -ez) cp = box_edge(wl, g, r, 0, 0, overlap);
+      else {
+        nd = -ey - g.y;
+        if (nd < r) { *overlap = nd - r; cp = mk(g.x, -ey - 0.5 * *overlap, g.z); }
+      }
+    } else {
+      if (g.z > ez) {
+        nd = g.z - ez;
+        if (nd < r) { *overlap = nd - r; cp = mk(g.x, g.y, ez + 0.5 * *overlap); }
+      } else if (g.z < -ez) {
+        nd = -ez - g.z;
+        if (nd < r) { *overlap = nd - r; cp = mk(g.x, g.y, -ez - 0.5 * *overlap); }
+      } else {
+        *overlap = -10000.;
+      }
+    }
+  }
+  return cp;
+}
+
+/* Transform::operator() (Transform.cpp:312-317) */
+static v3 xform(const double* basis, const double* origin, v3 p) {
+  return mk(dot(mk(basis[0], basis[1], basis[2]), p) + origin[0],
+            dot(mk(basis[3], basis[4], basis[5]), p) + origin[1],
+            dot(mk(basis[6], basis[7], basis[8]), p) + origin[2]);
+}
+
+/* ClosestPointSPHEREBOX, sphere first (RigidBodyWithCrust.cpp:647-683) */
+static int sphere_wall(const wall_t* wl, v3 A, double r, double cr, v3* point,
+                       v3* ov, double* dist) {
+  double overlap = 0.;
+  v3 local = xform(wl->inv_basis, wl->inv_origin, A);
+  v3 cp = box_sphere(wl, local, r, &overlap);
+  *dist = overlap;
+  if (overlap < 0.) {
+    if (-overlap >= cr + wl->crust) return -1;
+    v3 contact = xform(wl->basis, wl->origin, cp);
+    v3 AB = sub(contact, A);
+    *ov = scl(AB, overlap / norm(AB));
+    *point = contact;
+    return 1;
+  }
+  return 0;
+}
+
+int orc_sphere_box(const double* centre, double r, double cr,
+                   const double* basis, const double* origin,
+                   const double* half_extent, double crb, double* point,
+                   double* ovec, double* dist) {
+  orc_sys tmp;
+  memset(&tmp, 0, sizeof(tmp));
+  orc_add_wall(&tmp, basis, origin, half_extent, crb, 0);
+  v3 pt = mk(0., 0., 0.), ov = mk(0., 0., 0.);
+  int rc = sphere_wall(&tmp.walls[0], ld(centre), r, cr, &pt, &ov, dist);
+  st(point, pt);
+  st(ovec, ov);
+  return rc;
+}
+
+/* Velocity at a point: ParticleKinematics::Velocity (ParticleKinematics.cpp:229-232)
+   via Particle::getVelocityAtPoint (Particle.cpp:662-666) */
+static inline v3 vel_at(v3 x, v3 v, v3 w, v3 pt) { return add(v, cross(w, sub(pt, x))); }
+
+/* HookeContactForceModel::performForcesCalculus (HookeContactForceModel.cpp:44-114)
+   and HertzContactForceModel::performForcesCalculus (HertzContactForceModel.cpp:46-114);
+   returns delFN + delFT (kr = 0: no rolling moment). */
+static v3 contact_force(const model_t* m, v3 pt, v3 pen, double odist, v3 x0, v3 v0,
+                        v3 w0, double m0, double req0, v3 x1, v3 v1, v3 w1, double m1,
+                        double req1) {
+  v3 normal = dvd(pen, norm(pen));
+  normal = round3(normal);
+  v3 tmpV = sub(vel_at(x0, v0, w0, pt), vel_at(x1, v1, w1, pt));
+  v3 v_n = scl(normal, dot(tmpV, normal));
+  v3 v_t = sub(tmpV, v_n);
+  v3 delFN, delFT;
+  double normFN, muc;
+  if (m->kind == ORC_MODEL_HOOKE) {
+    double kn = m->p[0], etat_in = m->p[2];
+    muc = m->p[3];
+    delFN = scl(pen, kn);
+    double avmass = 1. / (1. / m0 + 1. / m1);
+    double gamman = -2. * m->beta * sqrt(avmass * kn);
+    delFN = sub(delFN, scl(v_n, gamman));
+    normFN = norm(delFN);
+    double etat = etat_in == -1. ? -m->beta * sqrt(kn / avmass) : etat_in;
+    double gammat = 2. * etat * avmass;
+    delFT = scl(v_t, -gammat);
+  } else {
+    double Es = m->p[0], Gs = m->p[2];
+    muc = m->p[3];
+    double deltan = -odist;
+    double Req = 1. / (1. / req0 + 1. / req1);
+    double avmass = 1. / (1. / m0 + 1. / m1);
+    double sq = sqrt(Req * deltan);
+    double Sn = 2. * Es * sq;
+    double St = 8. * Gs * sq;
+    double kn = (4. / 3.) * Es * sq;
+    delFN = scl(pen, kn);
+    double gamman = m->m2sqrt56 * m->beta * sqrt(avmass * Sn);
+    delFN = sub(delFN, scl(v_n, gamman));
+    normFN = norm(delFN);
+    double gammat = m->m2sqrt56 * m->beta * sqrt(avmass * St);
+    delFT = scl(v_t, -gammat);
+  }
+  double fn = muc * normFN;
+  double ft = norm(delFT);
+  if (fn < ft) {
+    double nvt = norm(v_t);
+    v3 tangent = mk(0., 0., 0.);
+    if (nvt > ORC_EPSILON) tangent = dvd(neg(v_t), nvt);
+    delFT = scl(tangent, fn);
+  }
+  return add(delFN, delFT);
+}
+
+void orc_contact_force(int kind, const double* p, const double* point,
+                       const double* ovec, double dist, const double* x0,
+                       const double* v0, const double* w0, double m0,
+                       double req0, const double* x1, const double* v1,
+                       const double* w1, double m1, double req1, double* f) {
+  model_t m;
+  memset(&m, 0, sizeof(m));
+  m.kind = kind;
+  memcpy(m.p, p, sizeof(double) * 5);
+  m.beta = log(p[1]) / sqrt(ORC_PI * ORC_PI + log(p[1]) * log(p[1]));
+  m.m2sqrt56 = -2. * sqrt(5. / 6.);
+  v3 r = contact_force(&m, ld(point), ld(ovec), dist, ld(x0), ld(v0), ld(w0), m0, req0,
+                       ld(x1), ld(v1), ld(w1), m1, req1);
+  st(f, r);
+}
+
+static void log_contact(orc_sys* s, int id0, int id1, v3 pt, v3 ov, double d, v3 f) {
+  if (s->log_n < s->log_cap) {
+    int k = s->log_n;
+    s->log_id0[k] = id0;
+    s->log_id1[k] = id1;
+    st(s->log_pt + 3 * k, pt);
+    st(s->log_ov + 3 * k, ov);
+    s->log_d[k] = d;
+    st(s->log_f + 3 * k, f);
+  }
+  s->log_n++;
+}
+
+/* Particle::InterAction (Component/src/Particle.cpp:478-546) for two spheres,
+   HookeContactForceModel::computeForces (HookeContactForceModel.cpp:121-163)
+   and Torsor::addForce(point, f) (Torsor.cpp:184-189) on both bodies.
+   shift is the periodic vector applied to the second body (minimum image). */
+static int interact(orc_sys* s, int p0, int p1, v3 shift) {
+  v3 A = ld(s->x + 3 * p0);
+  v3 B = add(ld(s->x + 3 * p1), shift);
+  double a[3], b[3], pt[3], ov[3], dist;
+  st(a, A);
+  st(b, B);
+  s->ncand++;
+  int rc = orc_sphere_sphere(a, s->r[p0], s->crust[p0], b, s->r[p1], s->crust[p1], pt,
+                             ov, &dist);
+  if (rc < 0)
+    return set_error(s, ORC_ERR_CONTACT, p0, p1, -dist, s->crust[p0] + s->crust[p1]);
+  if (rc == 0 || !(dist < 0.)) return ORC_OK;
+  const model_t* m = &s->models[s->mat[p0]][s->mat[p1]];
+  if (m->kind == ORC_MODEL_NONE) return set_error(s, ORC_ERR_ARG, p0, p1, 0., 0.);
+  v3 P = ld(pt);
+  v3 f = contact_force(m, P, ld(ov), dist, A, ld(s->v + 3 * p0), ld(s->w + 3 * p0),
+                       s->mass[p0], s->req[p0], B, ld(s->v + 3 * p1), ld(s->w + 3 * p1),
+                       s->mass[p1], s->req[p1]);
+  /* coef = 1: body 0 gets coef*(delFN+delFT); body 1 gets coef*(-delFN-delFT),
+     which is -(delFN+delFT) bit for bit */
+  v3 f0 = scl(f, 1.);
+  v3 f1 = neg(f);
+  st(s->F + 3 * p0, add(ld(s->F + 3 * p0), f0));
+  st(s->T + 3 * p0, add(ld(s->T + 3 * p0), cross(sub(P, A), f0)));
+  st(s->F + 3 * p1, add(ld(s->F + 3 * p1), f1));
+  st(s->T + 3 * p1, add(ld(s->T + 3 * p1), cross(sub(P, B), f1)));
+  log_contact(s, p0, p1, P, ld(ov), dist, f0);
+  return ORC_OK;
+}
+
+/* SimpleObstacle::InterAction (Component/src/SimpleObstacle.cpp:222-271):
+   BBox::InZone pre-test (BBox.cpp:166-171), sphere-box closest point, force
+   with the particle as first body and a fixed obstacle (mass 1e20,
+   Component.cpp:16, zero velocity, Obstacle.cpp:128-133) as second body. */
+static int interact_wall(orc_sys* s, int p, int wi) {
+  const wall_t* wl = &s->walls[wi];
+  const double* xp = s->x + 3 * p;
+  double r = s->r[p];
+  if (!(fabs(wl->bb_centre[0] - xp[0]) <= wl->bb_ext[0] + r &&
+        fabs(wl->bb_centre[1] - xp[1]) <= wl->bb_ext[1] + r &&
+        fabs(wl->bb_centre[2] - xp[2]) <= wl->bb_ext[2] + r))
+    return ORC_OK;
+  v3 A = ld(xp), pt = mk(0., 0., 0.), ov = mk(0., 0., 0.);
+  double dist;
+  int rc = sphere_wall(wl, A, r, s->crust[p], &pt, &ov, &dist);
+  if (rc < 0) return set_error(s, ORC_ERR_CONTACT, p, -1 - wi, -dist, s->crust[p] + wl->crust);
+  if (rc == 0 || !(dist < 0.)) return ORC_OK;
+  const model_t* m = &s->models[s->mat[p]][wl->material];
+  if (m->kind == ORC_MODEL_NONE) return set_error(s, ORC_ERR_ARG, p, -1 - wi, 0., 0.);
+  v3 zero = mk(0., 0., 0.);
+  v3 f = contact_force(m, pt, ov, dist, A, ld(s->v + 3 * p), ld(s->w + 3 * p), s->mass[p],
+                       s->req[p], ld(wl->origin), zero, zero, 1.e20, wl->req);
+  st(s->F + 3 * p, add(ld(s->F + 3 * p), f));
+  st(s->T + 3 * p, add(ld(s->T + 3 * p), cross(sub(pt, A), f)));
+  log_contact(s, p, -1 - wi, pt, ov, dist, f);
+  return ORC_OK;
+}
+
+/* Grains::computeParticlesForceAndTorque (Base/src/Grains.cpp:459-482):
+   AllComponents::InitializeForces (AllComponents.cpp:259-270) sets each torsor
+   to the weight m (1 - rho_f / rho) g (Particle.cpp:414-419, rho_f = 0) at the
+   centre; then LinkedCell::ComputeForces (LinkedCell.cpp:1019-1103): for each
+   particle, the particles after it in its own cell list, then the particles of
+   its contact-neighbour cells in list order (the 13 cells were added with
+   push_front, Cell.cpp:100-103, so they are visited in reverse stencil order),
+   then the obstacles. */
+int orc_compute_forces(orc_sys* s) {
+  const double* g = s->cfg.gravity;
+  s->log_n = 0;
+  s->ncand = 0;
+  for (int p = 0; p < s->n; ++p) {
+    double m = s->mass[p] * (1. - 0. / 1.);
+    s->F[3 * p] = g[0] * m;
+    s->F[3 * p + 1] = g[1] * m;
+    s->F[3 * p + 2] = g[2] * m;
+    s->T[3 * p] = s->T[3 * p + 1] = s->T[3 * p + 2] = 0.;
+  }
+  const int ni = s->nc[0];
+  for (int p = 0; p < s->n; ++p) {
+    int ijk[3];
+    int c = get_cell(s, s->x + 3 * p, ijk);
+    if (c < 0) return set_error(s, ORC_ERR_SIMULATION, p, -1, 0., 0.);
+    v3 zero = mk(0., 0., 0.);
+    for (int q = s->next[p]; q >= 0; q = s->next[q]) {
+      int rc = interact(s, p, q, zero);
+      if (rc) return rc;
+    }
+    for (int jj = 12; jj >= 0; --jj) {
+      int t[3];
+      double sh[3] = {0., 0., 0.};
+      int ok = 1;
+      for (int d = 0; d < 3; ++d) {
+        t[d] = ijk[d] + s->stencil[jj][d];
+        int n = s->nc[d];
+        if (t[d] < 0 || t[d] >= n) {
+          if (!s->cfg.periodic[d]) { ok = 0; break; }
+          if (t[d] < 0) { t[d] += n; sh[d] = -s->cfg.size[d]; }
+          else { t[d] -= n; sh[d] = s->cfg.size[d]; }
+        }
+      }
+      if (!ok) continue;
+      int cn = t[1] * s->nc[2] * ni + t[2] * ni + t[0];
+      for (int q = s->head[cn]; q >= 0; q = s->next[q]) {
+        int rc = interact(s, p, q, mk(sh[0], sh[1], sh[2]));
+        if (rc) return rc;
+      }
+    }
+    for (int wi = 0; wi < s->nwalls; ++wi) {
+      int rc = interact_wall(s, p, wi);
+      if (rc) return rc;
+    }
+  }
+  return ORC_OK;
+}
+
+/* ParticleKinematicsSphere::computeAngularAccelerationBodyFixed
+   (Component/src/ParticleKinematicsSphere.cpp:47-55), coupling factor 1 */
+static inline v3 ang_acc(v3 tq, double I) {
+  return mk(tq.x / (1. * I), tq.y / (1. * I), tq.z / (1. * I));
+}
+
+/* SSPRK3 angular update of SecondOrderLeapFrog (TimeIntegration/src/SecondOrderLeapFrog.cpp:56-66) */
+static inline v3 ssprk3(v3 tq, v3 wbf, double I, double dt) {
+  /* a sphere's angular acceleration does not depend on w, so K1 = K2 = K3 */
+  v3 K1 = scl(ang_acc(tq, I), dt);
+  v3 K2 = scl(ang_acc(tq, I), dt);
+  v3 K3 = scl(ang_acc(tq, I), dt);
+  return add(wbf, dvd(add(add(K1, K2), scl(K3, 4.)), 6.));
+}
+
+/* ParticleKinematics::Move (Component/src/ParticleKinematics.cpp:62-104) with
+   SecondOrderLeapFrog::Move (SecondOrderLeapFrog.cpp:44-67) and
+   Transform::composeLeftByRotation (Geometry/src/Transform.cpp:140-156).
+   Returns |dx| (Particle::Move compares it with the crust, Particle.cpp:437-460). */
+static double move_one(orc_sys* s, int p, double dtv, double dtd) {
+  quat q = qmk(mk(s->q[4 * p], s->q[4 * p + 1], s->q[4 * p + 2]), s->q[4 * p + 3]);
+  quat pc = qconj(q);
+  v3 tq = qmulv(pc, qleftvec(q, ld(s->T + 3 * p)));
+  double mcf = s->mass[p] * 1.;
+  v3 v = add(ld(s->v + 3 * p), scl(dvd(ld(s->F + 3 * p), mcf), dtv));
+  v3 dx = scl(v, dtd);
+  v3 wbf = ssprk3(tq, ld(s->wbf + 3 * p), s->inertia[p], dtv);
+  v3 w = qmulv(q, qleftvec(pc, wbf));
+  v3 avg = qmulv(q, qleftvec(pc, wbf));
+  st(s->v + 3 * p, v);
+  st(s->wbf + 3 * p, wbf);
+  st(s->w + 3 * p, w);
+  /* Component::Translate (Component.cpp:376-379) */
+  st(s->x + 3 * p, add(ld(s->x + 3 * p), dx));
+  double nOmega = norm(avg);
+  quat dq;
+  if (nOmega > ORC_EPSILON) {
+    double c = cos(nOmega * dtd / 2.);
+    double sn = sin(nOmega * dtd / 2.);
+    v3 t = scl(avg, sn * 1. / nOmega);
+    dq = qmk(t, c);
+  } else {
+    dq = qmk(mk(0., 0., 0.), 1.);
+  }
+  quat qn = qmul(dq, q);
+  s->q[4 * p] = qn.v.x; s->q[4 * p + 1] = qn.v.y; s->q[4 * p + 2] = qn.v.z; s->q[4 * p + 3] = qn.w;
+  double* mm = s->basis + 9 * p;
+  double qx = dq.v.x, qy = dq.v.y, qz = dq.v.z, qw = dq.w;
+  double nb[9];
+  for (int i = 0; i < 3; ++i) {
+    double px = qy * mm[6 + i] - qz * mm[3 + i] + qw * mm[0 + i];
+    double py = qz * mm[0 + i] - qx * mm[6 + i] + qw * mm[3 + i];
+    double pz = qx * mm[3 + i] - qy * mm[0 + i] + qw * mm[6 + i];
+    double pw = -qx * mm[0 + i] - qy * mm[3 + i] - qz * mm[6 + i];
+    nb[0 + i] = qy * pz - qz * py - pw * qx + qw * px;
+    nb[3 + i] = qz * px - qx * pz - pw * qy + qw * py;
+    nb[6 + i] = qx * py - qy * px - pw * qz + qw * pz;
+  }
+  memcpy(mm, nb, sizeof(nb));
+  return norm(dx);
+}
+
+/* AllComponents::Move (Component/src/AllComponents.cpp:151-166); then the
+   periodic wrap that stands in for the clone update of
+   LinkedCell::updateDestroyPeriodicClones (LinkedCell.cpp:2008-2126). */
+int orc_move(orc_sys* s, double dtv, double dtd) {
+  for (int p = 0; p < s->n; ++p) {
+    double depl = move_one(s, p, dtv, dtd);
+    if (depl > s->crust[p]) return set_error(s, ORC_ERR_MOTION, p, -1, depl, s->crust[p]);
+    for (int d = 0; d < 3; ++d) {
+      if (!s->cfg.periodic[d]) continue;
+      double lo = s->cfg.origin[d], L = s->cfg.size[d];
+      double* xd = s->x + 3 * p + d;
+      if (*xd >= lo + L) *xd -= L;
+      else if (*xd < lo) *xd += L;
+    }
+  }
+  return ORC_OK;
+}
+
+/* ParticleKinematics::advanceVelocity (ParticleKinematics.cpp:111-128) with
+   SecondOrderLeapFrog::advanceVelocity (SecondOrderLeapFrog.cpp:74-94) */
+int orc_advance_velocity(orc_sys* s, double dtv) {
+  for (int p = 0; p < s->n; ++p) {
+    quat q = qmk(mk(s->q[4 * p], s->q[4 * p + 1], s->q[4 * p + 2]), s->q[4 * p + 3]);
+    quat pc = qconj(q);
+    v3 tq = qmulv(pc, qleftvec(q, ld(s->T + 3 * p)));
+    double mcf = s->mass[p] * 1.;
+    v3 v = add(ld(s->v + 3 * p), scl(dvd(ld(s->F + 3 * p), mcf), dtv));
+    v3 wbf = ssprk3(tq, ld(s->wbf + 3 * p), s->inertia[p], dtv);
+    st(s->v + 3 * p, v);
+    st(s->wbf + 3 * p, wbf);
+    st(s->w + 3 * p, qmulv(q, qleftvec(pc, wbf)));
+  }
+  return ORC_OK;
+}
+
+/* Grains::do_before_time_stepping (Base/src/Grains.cpp:95-179): particles are
+   linked at insertion, then a0 is computed for the leap-frog (:169-174). */
+int orc_prime(orc_sys* s) {
+  s->err_code = ORC_OK;
+  int rc = orc_link(s);
+  if (rc) return rc;
+  return orc_compute_forces(s);
+}
+
+/* Grains::Simulation kick-drift-kick body (Base/src/Grains.cpp:344-364) */
+int orc_step(orc_sys* s, int nsteps, double dt) {
+  for (int k = 0; k < nsteps; ++k) {
+    int rc = orc_move(s, 0.5 * dt, dt);
+    if (rc) return rc;
+    rc = orc_link_update(s);
+    if (rc) return rc;
+    rc = orc_compute_forces(s);
+    if (rc) return rc;
+    rc = orc_advance_velocity(s, 0.5 * dt);
+    if (rc) return rc;
+  }
+  return ORC_OK;
+}
+
+int orc_get_state(const orc_sys* s, double* x, double* v, double* w, double* q,
+                  double* F, double* T) {
+  size_t n3 = sizeof(double) * 3 * (size_t)s->n;
+  if (x) memcpy(x, s->x, n3);
+  if (v) memcpy(v, s->v, n3);
+  if (w) memcpy(w, s->w, n3);
+  if (q) memcpy(q, s->q, sizeof(double) * 4 * (size_t)s->n);
+  if (F) memcpy(F, s->F, n3);
+  if (T) memcpy(T, s->T, n3);
+  return ORC_OK;
+}
+
+int orc_set_state(orc_sys* s, const double* x, const double* v, const double* w,
+                  const double* q, const double* F, const double* T) {
+  size_t n3 = sizeof(double) * 3 * (size_t)s->n;
+  if (x) memcpy(s->x, x, n3);
+  if (v) memcpy(s->v, v, n3);
+  if (q) memcpy(s->q, q, sizeof(double) * 4 * (size_t)s->n);
+  if (w) {
+    memcpy(s->w, w, n3);
+    for (int i = 0; i < s->n; ++i) {
+      quat qq = qmk(mk(s->q[4 * i], s->q[4 * i + 1], s->q[4 * i + 2]), s->q[4 * i + 3]);
+      st(s->wbf + 3 * i, qmulv(qconj(qq), qleftvec(qq, ld(s->w + 3 * i))));
+    }
+  }
+  if (F) memcpy(s->F, F, n3);
+  if (T) memcpy(s->T, T, n3);
+  return ORC_OK;
+}
+
+int orc_get_grid(const orc_sys* s, int* nc, double* edge, double* origin) {
+  for (int d = 0; d < 3; ++d) {
+    if (nc) nc[d] = s->nc[d];
+    if (edge) edge[d] = s->edge[d];
+    if (origin) origin[d] = s->lc_origin[d];
+  }
+  return ORC_OK;
+}
+
+void orc_enable_contact_log(orc_sys* s, int cap) {
+  free(s->log_id0); free(s->log_id1); free(s->log_pt); free(s->log_ov);
+  free(s->log_d); free(s->log_f);
+  s->log_cap = cap;
+  s->log_n = 0;
+  s->log_id0 = (int32_t*)malloc(sizeof(int32_t) * (size_t)(cap > 0 ? cap : 1));
+  s->log_id1 = (int32_t*)malloc(sizeof(int32_t) * (size_t)(cap > 0 ? cap : 1));
+  s->log_pt = (double*)malloc(sizeof(double) * 3 * (size_t)(cap > 0 ? cap : 1));
+  s->log_ov = (double*)malloc(sizeof(double) * 3 * (size_t)(cap > 0 ? cap : 1));
+  s->log_d = (double*)malloc(sizeof(double) * (size_t)(cap > 0 ? cap : 1));
+  s->log_f = (double*)malloc(sizeof(double) * 3 * (size_t)(cap > 0 ? cap : 1));
+}
+
+int orc_get_contacts(const orc_sys* s, int cap, int* n_out, int32_t* id0, int32_t* id1,
+                     double* point, double* ovec, double* overlap, double* force) {
+  *n_out = s->log_n;
+  int n = s->log_n < s->log_cap ? s->log_n : s->log_cap;
+  if (n > cap) n = cap;
+  for (int k = 0; k < n; ++k) {
+    if (id0) id0[k] = s->log_id0[k];
+    if (id1) id1[k] = s->log_id1[k];
+    if (point) memcpy(point + 3 * k, s->log_pt + 3 * k, sizeof(double) * 3);
+    if (ovec) memcpy(ovec + 3 * k, s->log_ov + 3 * k, sizeof(double) * 3);
+    if (overlap) overlap[k] = s->log_d[k];
+    if (force) memcpy(force + 3 * k, s->log_f + 3 * k, sizeof(double) * 3);
+  }
+  return s->log_n > s->log_cap ? ORC_ERR_ARG : ORC_OK;
+}
+
+int orc_last_error(const orc_sys* s, int* code, int32_t* id0, int32_t* id1,
+                   double* value, double* limit) {
+  if (code) *code = s->err_code;
+  if (id0) *id0 = s->err_id0;
+  if (id1) *id1 = s->err_id1;
+  if (value) *value = s->err_value;
+  if (limit) *limit = s->err_limit;
+  return s->err_code;
+}
+
+int64_t orc_candidates(const orc_sys* s) { return s->ncand; }

@@ -1,0 +1,625 @@
+// gdem_capi.hip -- the C ABI declared in include/gdem.h.
+//
+// The host side owns one HIP stream per context and orchestrates the phases of
+// Grains::Simulation (src/Grains3D/Grains/Base/src/Grains.cpp:344-364) as
+// kernel launches; nothing here computes physics on the CPU except the
+// per-particle constants the reference also derives once at construction
+// (mass, inertia, equivalent radius: Component/src/Particle.cpp:158-164).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <utility>
+#include <vector>
+
+#include "../../include/gdem.h"
+#include "gdem_internal.h"
+
+#pragma clang fp contract(off)
+
+using namespace gdem;
+
+struct gdem_ctx {
+  gdem_config cfg{};
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  Grid grid{};
+  int32_t* d_rank = nullptr;
+  int64_t n = 0;
+  State buf[2]{};
+  int cur = 0;
+  double4* d_F = nullptr;
+  double4* d_T = nullptr;
+  Bins bins{};
+  Model h_models[kMaxMaterials * kMaxMaterials]{};
+  Model* d_models = nullptr;
+  Wall h_walls[kMaxWalls]{};
+  Wall* d_walls = nullptr;
+  int nwalls = 0;
+  unsigned long long* d_err = nullptr;
+  unsigned long long* d_counters = nullptr;  // [0] contacts
+  bool primed = false, binned = false;
+  int32_t rec_cap = 0;
+  ContactRec* d_rec = nullptr;
+  int32_t* d_rec_count = nullptr;
+  bool rec_valid = false;
+  // staging for gdem_get_state
+  double* d_stage = nullptr;
+  size_t stage_len = 0;
+  // counters / timing
+  int64_t force_evals = 0;
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+  size_t ev_used = 0;
+  double force_ms = 0.;
+  int64_t timed = 0;
+  // last error
+  int32_t err_code = 0, err_id0 = -1, err_id1 = -1;
+};
+
+#define HIP_TRY(expr)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      fprintf(stderr, "gdem: %s failed: %s\n", #expr, hipGetErrorString(e_)); \
+      return GDEM_ERR_HIP;                                                   \
+    }                                                                        \
+  } while (0)
+
+static int set_err(gdem_ctx* c, int code, int id0, int id1) {
+  c->err_code = code;
+  c->err_id0 = id0;
+  c->err_id1 = id1;
+  return code;
+}
+
+static void free_state(State& s) {
+  hipFree(s.pos_r); hipFree(s.vel_m); hipFree(s.om_req); hipFree(s.wbf_I); hipFree(s.quat);
+  hipFree(s.aux);
+  s = State{};
+}
+
+static int alloc_state(State& s, int64_t n) {
+  size_t b4 = sizeof(double4) * (size_t)std::max<int64_t>(n, 1);
+  HIP_TRY(hipMalloc(&s.pos_r, b4));
+  HIP_TRY(hipMalloc(&s.vel_m, b4));
+  HIP_TRY(hipMalloc(&s.om_req, b4));
+  HIP_TRY(hipMalloc(&s.wbf_I, b4));
+  HIP_TRY(hipMalloc(&s.quat, b4));
+  HIP_TRY(hipMalloc(&s.aux, sizeof(Aux) * (size_t)std::max<int64_t>(n, 1)));
+  return GDEM_OK;
+}
+
+// Morton (Z-order) rank of each cell, bits interleaved i, j, k from the least
+// significant end; cells are binned and swept in this order.
+static std::vector<int32_t> morton_rank(const int32_t nc[3]) {
+  auto spread = [](uint64_t v) {
+    uint64_t r = 0;
+    for (int b = 0; b < 21; ++b) r |= ((v >> b) & 1ull) << (3 * b);
+    return r;
+  };
+  const int64_t ncell = (int64_t)nc[0] * nc[1] * nc[2];
+  std::vector<std::pair<uint64_t, int32_t>> codes;
+  codes.reserve((size_t)ncell);
+  for (int j = 0; j < nc[1]; ++j)
+    for (int k = 0; k < nc[2]; ++k)
+      for (int i = 0; i < nc[0]; ++i) {
+        uint64_t code = spread((uint64_t)i) | (spread((uint64_t)j) << 1) | (spread((uint64_t)k) << 2);
+        codes.emplace_back(code, (int32_t)(j * nc[2] * nc[0] + k * nc[0] + i));
+      }
+  std::sort(codes.begin(), codes.end());
+  std::vector<int32_t> rank((size_t)ncell);
+  for (size_t r = 0; r < codes.size(); ++r) rank[(size_t)codes[r].second] = (int32_t)r;
+  return rank;
+}
+
+extern "C" const char* gdem_version(void) { return "gdem 0.1 (gfx950, fp64)"; }
+
+extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
+  if (!cfg || !out) return GDEM_ERR_ARG;
+  *out = nullptr;
+  gdem_ctx* c = new gdem_ctx();
+  c->cfg = *cfg;
+  c->dev = cfg->device;
+  // LinkedCell::set (Base/src/LinkedCell.cpp:47-119), no ghost layers
+  for (int d = 0; d < 3; ++d) {
+    int n = 1;
+    double e = cfg->size[d];
+    if (!(cfg->size[d] > 0.)) { delete c; return GDEM_ERR_ARG; }
+    if (cfg->cell_size > kEps) {
+      n = (int)((cfg->size[d] + kEps) / cfg->cell_size);
+      if (!n) n = 1;
+      e = cfg->size[d] / n;
+    }
+    if (cfg->periodic[d] && n < 3) { delete c; return GDEM_ERR_ARG; }
+    c->grid.nc[d] = n;
+    c->grid.edge[d] = e;
+    c->grid.origin[d] = cfg->origin[d];
+    c->grid.size[d] = cfg->size[d];
+    c->grid.periodic[d] = cfg->periodic[d] ? 1 : 0;
+  }
+  int64_t ncell = (int64_t)c->grid.nc[0] * c->grid.nc[1] * c->grid.nc[2];
+  if (ncell > 8 * 1024 * 1024) { delete c; return GDEM_ERR_ARG; }
+  c->grid.ncell = (int32_t)ncell;
+  if (hipSetDevice(c->dev) != hipSuccess) { delete c; return GDEM_ERR_HIP; }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return GDEM_ERR_HIP; }
+  std::vector<int32_t> rank = morton_rank(c->grid.nc);
+  const int nb = (int)((ncell + 1023) / 1024);
+  c->bins.nbsum = nb;
+  if (hipMalloc(&c->d_rank, sizeof(int32_t) * ncell) != hipSuccess ||
+      hipMalloc(&c->bins.cnt, sizeof(int32_t) * ncell) != hipSuccess ||
+      hipMalloc(&c->bins.local, sizeof(int32_t) * ncell) != hipSuccess ||
+      hipMalloc(&c->bins.bsum, sizeof(int32_t) * nb) != hipSuccess ||
+      hipMalloc(&c->bins.cell_start, sizeof(int32_t) * (ncell + 1)) != hipSuccess ||
+      hipMalloc(&c->d_models, sizeof(c->h_models)) != hipSuccess ||
+      hipMalloc(&c->d_walls, sizeof(c->h_walls)) != hipSuccess ||
+      hipMalloc(&c->d_err, sizeof(unsigned long long) * kErrWords) != hipSuccess ||
+      hipMalloc(&c->d_counters, sizeof(unsigned long long) * 2) != hipSuccess) {
+    gdem_destroy(c);
+    return GDEM_ERR_HIP;
+  }
+  hipMemcpy(c->d_rank, rank.data(), sizeof(int32_t) * ncell, hipMemcpyHostToDevice);
+  hipMemset(c->bins.cnt, 0, sizeof(int32_t) * ncell);
+  hipMemset(c->d_err, 0xff, sizeof(unsigned long long) * kErrWords);
+  hipMemset(c->d_counters, 0, sizeof(unsigned long long) * 2);
+  hipMemcpy(c->d_models, c->h_models, sizeof(c->h_models), hipMemcpyHostToDevice);
+  if (hipDeviceSynchronize() != hipSuccess) { gdem_destroy(c); return GDEM_ERR_HIP; }
+  *out = c;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_destroy(gdem_ctx* c) {
+  if (!c) return GDEM_OK;
+  hipSetDevice(c->dev);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_state(c->buf[0]);
+  free_state(c->buf[1]);
+  hipFree(c->d_F); hipFree(c->d_T);
+  hipFree(c->bins.key); hipFree(c->bins.slot); hipFree(c->bins.perm);
+  hipFree(c->bins.cnt); hipFree(c->bins.local); hipFree(c->bins.bsum); hipFree(c->bins.cell_start);
+  hipFree(c->d_rank); hipFree(c->d_models); hipFree(c->d_walls); hipFree(c->d_err);
+  hipFree(c->d_counters); hipFree(c->d_rec); hipFree(c->d_rec_count); hipFree(c->d_stage);
+  for (auto& e : c->ev_pool) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_set_particles(gdem_ctx* c, int64_t n, const double* x, const double* v,
+                                  const double* w, const double* q, const double* radius,
+                                  const double* density, const double* crust,
+                                  const int32_t* material) {
+  if (!c || n < 0 || n > 0x7fffffff) return GDEM_ERR_ARG;
+  if (n > 0 && (!x || !v || !w || !q || !radius || !density || !crust)) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  free_state(c->buf[0]);
+  free_state(c->buf[1]);
+  hipFree(c->d_F); hipFree(c->d_T);
+  hipFree(c->bins.key); hipFree(c->bins.slot); hipFree(c->bins.perm);
+  hipFree(c->d_rec); hipFree(c->d_rec_count);
+  c->d_F = c->d_T = nullptr;
+  c->bins.key = c->bins.slot = c->bins.perm = nullptr;
+  c->d_rec = nullptr;
+  c->d_rec_count = nullptr;
+  c->n = n;
+  const size_t nn = (size_t)std::max<int64_t>(n, 1);
+  if (alloc_state(c->buf[0], n) || alloc_state(c->buf[1], n)) return GDEM_ERR_HIP;
+  HIP_TRY(hipMalloc(&c->d_F, sizeof(double4) * nn));
+  HIP_TRY(hipMalloc(&c->d_T, sizeof(double4) * nn));
+  HIP_TRY(hipMalloc(&c->bins.key, sizeof(int32_t) * nn));
+  HIP_TRY(hipMalloc(&c->bins.slot, sizeof(int32_t) * nn));
+  HIP_TRY(hipMalloc(&c->bins.perm, sizeof(int32_t) * nn));
+  HIP_TRY(hipMemset(c->d_F, 0, sizeof(double4) * nn));
+  HIP_TRY(hipMemset(c->d_T, 0, sizeof(double4) * nn));
+  std::vector<double4> pr(nn), vm(nn), wr(nn), wb(nn), qq(nn);
+  std::vector<Aux> ax(nn);
+  for (int64_t i = 0; i < n; ++i) {
+    const double rr = radius[i], rho = density[i];
+    if (!(rr > 0.) || !(rho > 0.) || !(crust[i] > 0.)) return GDEM_ERR_ARG;
+    int32_t mat = material ? material[i] : 0;
+    if (mat < 0 || mat >= kMaxMaterials) return GDEM_ERR_ARG;
+    // Sphere::getVolume (Geometry/src/Sphere.cpp:152-155), Sphere::BuildInertia
+    // (:61-70), Particle.cpp:158-164, Component::getEquivalentSphereRadius
+    // (Component/src/Component.cpp:205-208)
+    const double vol = 4.0 * kPi * pow(rr, 3.) / 3.0;
+    const double in0 = 8.0 * kPi * pow(rr, 5.) / 15.0;
+    const double mass = rho * vol;
+    const double inertia = in0 * rho;
+    const double req = pow((0.75 / kPi) * vol, 1. / 3.);
+    const quat qi = qmk(mk(q[4 * i], q[4 * i + 1], q[4 * i + 2]), q[4 * i + 3]);
+    // ParticleKinematics::setAngularVelocity (ParticleKinematics.cpp:197-205)
+    const v3 wbf = qmulv(qconj(qi), qleftvec(qi, mk(w[3 * i], w[3 * i + 1], w[3 * i + 2])));
+    pr[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], rr);
+    vm[i] = make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], mass);
+    wr[i] = make_double4(w[3 * i], w[3 * i + 1], w[3 * i + 2], req);
+    wb[i] = make_double4(wbf.x, wbf.y, wbf.z, inertia);
+    qq[i] = make_double4(q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]);
+    ax[i].id = (int32_t)i;
+    ax[i].mat = mat;
+    ax[i].crust = crust[i];
+  }
+  State& s = c->buf[0];
+  c->cur = 0;
+  const size_t b4 = sizeof(double4) * (size_t)n;
+  if (n > 0) {
+    HIP_TRY(hipMemcpy(s.pos_r, pr.data(), b4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s.vel_m, vm.data(), b4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s.om_req, wr.data(), b4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s.wbf_I, wb.data(), b4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s.quat, qq.data(), b4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s.aux, ax.data(), sizeof(Aux) * (size_t)n, hipMemcpyHostToDevice));
+  }
+  if (c->rec_cap > 0) {
+    HIP_TRY(hipMalloc(&c->d_rec, sizeof(ContactRec) * nn * (size_t)c->rec_cap));
+    HIP_TRY(hipMalloc(&c->d_rec_count, sizeof(int32_t) * nn));
+  }
+  c->primed = false;
+  c->binned = false;
+  c->rec_valid = false;
+  launch_keys(s, (int32_t)n, c->grid, c->d_rank, c->bins.key, c->d_err, c->stream);
+  HIP_TRY(hipGetLastError());
+  return GDEM_OK;
+}
+
+extern "C" int gdem_add_wall(gdem_ctx* c, const double* basis, const double* centre,
+                             const double* half, double crust, int32_t material) {
+  if (!c || !basis || !centre || !half) return GDEM_ERR_ARG;
+  if (c->nwalls >= kMaxWalls || material < 0 || material >= kMaxMaterials) return GDEM_ERR_ARG;
+  Wall wl{};
+  memcpy(wl.basis, basis, sizeof(wl.basis));
+  memcpy(wl.origin, centre, sizeof(wl.origin));
+  memcpy(wl.ext, half, sizeof(wl.ext));
+  wl.crust = crust;
+  wl.material = material;
+  // Transform::setToInverseTransform (Geometry/src/Transform.cpp:71-77), rotation basis
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) wl.inv_basis[3 * i + j] = basis[3 * j + i];
+  for (int i = 0; i < 3; ++i) {
+    v3 row = mk(wl.inv_basis[3 * i], wl.inv_basis[3 * i + 1], wl.inv_basis[3 * i + 2]);
+    wl.inv_origin[i] = dot(neg(row), mk(centre[0], centre[1], centre[2]));
+  }
+  // Convex::bbox (Geometry/src/Convex.cpp:29-41) with Box::support (Box.cpp:307-316),
+  // BBox::setValue (BBox.cpp:235-239)
+  const double e0 = half[0], e1 = half[1], e2 = half[2];
+  double mn[3], mx[3];
+  for (int i = 0; i < 3; ++i) {
+    v3 row = mk(basis[3 * i], basis[3 * i + 1], basis[3 * i + 2]);
+    v3 nrow = neg(row);
+    v3 sm = norm(nrow) < kEps ? mk(0., 0., 0.)
+                              : mk(nrow.x < 0. ? -e0 : e0, nrow.y < 0. ? -e1 : e1, nrow.z < 0. ? -e2 : e2);
+    v3 sp = norm(row) < kEps ? mk(0., 0., 0.)
+                             : mk(row.x < 0. ? -e0 : e0, row.y < 0. ? -e1 : e1, row.z < 0. ? -e2 : e2);
+    mn[i] = centre[i] + dot(row, sm);
+    mx[i] = centre[i] + dot(row, sp);
+  }
+  for (int i = 0; i < 3; ++i) {
+    wl.bb_ext[i] = (mx[i] - mn[i]) / 2.;
+    wl.bb_centre[i] = mn[i] + wl.bb_ext[i];
+  }
+  // Box::getVolume (Box.cpp:273-279), equivalent-sphere radius of the obstacle
+  const double vol = 8.0 * e0 * e1 * e2;
+  wl.req = pow((0.75 / kPi) * vol, 1. / 3.);
+  c->h_walls[c->nwalls++] = wl;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipMemcpyAsync(c->d_walls, c->h_walls, sizeof(Wall) * c->nwalls, hipMemcpyHostToDevice,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GDEM_OK;
+}
+
+extern "C" int gdem_set_contact_model(gdem_ctx* c, int32_t a, int32_t b, int32_t kind,
+                                      const double* p) {
+  if (!c || !p || a < 0 || b < 0 || a >= kMaxMaterials || b >= kMaxMaterials) return GDEM_ERR_ARG;
+  if (kind != GDEM_MODEL_HOOKE && kind != GDEM_MODEL_HERTZ) return GDEM_ERR_ARG;
+  if (p[4] != 0.) return GDEM_ERR_ARG;  // rolling resistance not supported
+  if (!(p[1] > 0. && p[1] <= 1.)) return GDEM_ERR_ARG;
+  Model m{};
+  m.kind = kind;
+  memcpy(m.p, p, sizeof(m.p));
+  // HookeContactForceModel.cpp:19, HertzContactForceModel.cpp:20-21
+  m.beta = log(p[1]) / sqrt(kPi * kPi + log(p[1]) * log(p[1]));
+  m.m2sqrt56 = -2. * sqrt(5. / 6.);
+  c->h_models[a * kMaxMaterials + b] = m;
+  c->h_models[b * kMaxMaterials + a] = m;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipMemcpyAsync(c->d_models, c->h_models, sizeof(c->h_models), hipMemcpyHostToDevice,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GDEM_OK;
+}
+
+static int do_binning(gdem_ctx* c) {
+  launch_binning(c->bins, (int32_t)c->n, c->grid.ncell, c->stream);
+  HIP_TRY(hipGetLastError());
+  c->binned = true;
+  return GDEM_OK;
+}
+
+static std::pair<hipEvent_t, hipEvent_t>* next_event_pair(gdem_ctx* c) {
+  if (c->ev_used == c->ev_pool.size()) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return nullptr;
+    c->ev_pool.emplace_back(a, b);
+  }
+  return &c->ev_pool[c->ev_used++];
+}
+
+static int do_force(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft) {
+  if (!c->binned) {
+    int rc = do_binning(c);
+    if (rc) return rc;
+  }
+  ForceArgs a{};
+  a.in = c->buf[c->cur];
+  a.out = c->buf[1 - c->cur];
+  a.perm = c->bins.perm;
+  a.cell_start = c->bins.cell_start;
+  a.rank = c->d_rank;
+  a.grid = c->grid;
+  a.models = c->d_models;
+  a.walls = c->d_walls;
+  a.nwalls = c->nwalls;
+  a.n = (int32_t)c->n;
+  for (int d = 0; d < 3; ++d) a.gravity[d] = c->cfg.gravity[d];
+  a.dt_vel = dtv;
+  a.dt_disp = dtd;
+  a.F = write_ft ? c->d_F : nullptr;
+  a.T = write_ft ? c->d_T : nullptr;
+  a.key_out = c->bins.key;
+  a.err = c->d_err;
+  a.contacts = c->d_counters;
+  const bool record = c->rec_cap > 0 && c->d_rec != nullptr;
+  a.rec = c->d_rec;
+  a.rec_count = c->d_rec_count;
+  a.rec_cap = c->rec_cap;
+  const int nblk = (int)((c->n + 255) / 256);
+  a.nblk8 = ((nblk + 7) / 8) * 8;
+  std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
+  if (c->timing) {
+    ev = next_event_pair(c);
+    if (ev) HIP_TRY(hipEventRecord(ev->first, c->stream));
+  }
+  launch_force(mode, record, a, c->stream);
+  HIP_TRY(hipGetLastError());
+  if (ev) HIP_TRY(hipEventRecord(ev->second, c->stream));
+  c->cur = 1 - c->cur;
+  c->binned = false;
+  c->force_evals++;
+  if (record) c->rec_valid = true;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_link_update(gdem_ctx* c) {
+  if (!c) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  return do_binning(c);
+}
+
+extern "C" int gdem_compute_forces(gdem_ctx* c) {
+  if (!c) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  return do_force(c, MODE_FORCE, 0., 0., true);
+}
+
+extern "C" int gdem_move(gdem_ctx* c, double dtv, double dtd) {
+  if (!c) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  launch_move(c->buf[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid, c->d_rank, dtv, dtd,
+              c->bins.key, c->d_err, c->stream);
+  HIP_TRY(hipGetLastError());
+  c->binned = false;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_advance_velocity(gdem_ctx* c, double dtv) {
+  if (!c) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  launch_advance(c->buf[c->cur], c->d_F, c->d_T, (int32_t)c->n, dtv, c->stream);
+  HIP_TRY(hipGetLastError());
+  return GDEM_OK;
+}
+
+extern "C" int gdem_prime(gdem_ctx* c) {
+  if (!c) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  int rc = do_binning(c);
+  if (rc) return rc;
+  rc = do_force(c, MODE_FORCE, 0., 0., true);
+  if (rc) return rc;
+  c->primed = true;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_step(gdem_ctx* c, int32_t nsteps, double dt) {
+  if (!c || nsteps < 0 || !(dt > 0.)) return GDEM_ERR_ARG;
+  if (nsteps == 0) return GDEM_OK;
+  HIP_TRY(hipSetDevice(c->dev));
+  int rc;
+  if (!c->primed) {
+    rc = gdem_prime(c);
+    if (rc) return rc;
+  }
+  // step 1's kick + drift; afterwards each force kernel closes its step and
+  // opens the next one
+  rc = gdem_move(c, 0.5 * dt, dt);
+  if (rc) return rc;
+  for (int32_t k = 0; k < nsteps; ++k) {
+    const bool last = (k == nsteps - 1);
+    rc = do_force(c, last ? MODE_KICK : MODE_MOVE, 0.5 * dt, dt, last);
+    if (rc) return rc;
+  }
+  return GDEM_OK;
+}
+
+static int collect_errors(gdem_ctx* c) {
+  unsigned long long h[kErrWords];
+  HIP_TRY(hipMemcpy(h, c->d_err, sizeof(h), hipMemcpyDeviceToHost));
+  int code = GDEM_OK;
+  if (h[kErrMotion] != kNoErr) code = set_err(c, GDEM_ERR_MOTION, (int32_t)h[kErrMotion], -1);
+  else if (h[kErrSim] != kNoErr) code = set_err(c, GDEM_ERR_SIMULATION, (int32_t)h[kErrSim], -1);
+  else if (h[kErrContact] != kNoErr)
+    code = set_err(c, GDEM_ERR_CONTACT, (int32_t)(h[kErrContact] >> 32),
+                   (int32_t)(uint32_t)(h[kErrContact] & 0xffffffffull));
+  else if (h[kErrCapacity] != kNoErr)
+    code = set_err(c, GDEM_ERR_CAPACITY, (int32_t)h[kErrCapacity], -1);
+  if (code != GDEM_OK) {
+    HIP_TRY(hipMemset(c->d_err, 0xff, sizeof(h)));
+  }
+  return code;
+}
+
+extern "C" int gdem_synchronize(gdem_ctx* c) {
+  if (!c) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return collect_errors(c);
+}
+
+extern "C" int gdem_get_state(gdem_ctx* c, double* x, double* v, double* w, double* q, double* F,
+                              double* T) {
+  if (!c) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  const size_t n = (size_t)c->n;
+  const size_t need = n * 19;
+  if (c->stage_len < need) {
+    hipFree(c->d_stage);
+    c->d_stage = nullptr;
+    HIP_TRY(hipMalloc(&c->d_stage, sizeof(double) * std::max<size_t>(need, 1)));
+    c->stage_len = need;
+  }
+  double* dx = c->d_stage;
+  double* dv = dx + 3 * n;
+  double* dw = dv + 3 * n;
+  double* dq = dw + 3 * n;
+  double* dF = dq + 4 * n;
+  double* dT = dF + 3 * n;
+  launch_unpermute(c->buf[c->cur], c->d_F, c->d_T, (int32_t)n, x ? dx : nullptr,
+                   v ? dv : nullptr, w ? dw : nullptr, q ? dq : nullptr, F ? dF : nullptr,
+                   T ? dT : nullptr, c->stream);
+  HIP_TRY(hipGetLastError());
+  if (n > 0) {
+    if (x) HIP_TRY(hipMemcpyAsync(x, dx, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    if (v) HIP_TRY(hipMemcpyAsync(v, dv, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    if (w) HIP_TRY(hipMemcpyAsync(w, dw, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    if (q) HIP_TRY(hipMemcpyAsync(q, dq, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, c->stream));
+    if (F) HIP_TRY(hipMemcpyAsync(F, dF, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    if (T) HIP_TRY(hipMemcpyAsync(T, dT, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return collect_errors(c);
+}
+
+extern "C" int gdem_set_contact_recording(gdem_ctx* c, int32_t cap) {
+  if (!c || cap < 0 || cap > 256) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  hipFree(c->d_rec);
+  hipFree(c->d_rec_count);
+  c->d_rec = nullptr;
+  c->d_rec_count = nullptr;
+  c->rec_cap = cap;
+  c->rec_valid = false;
+  if (cap > 0 && c->n > 0) {
+    HIP_TRY(hipMalloc(&c->d_rec, sizeof(ContactRec) * (size_t)c->n * (size_t)cap));
+    HIP_TRY(hipMalloc(&c->d_rec_count, sizeof(int32_t) * (size_t)c->n));
+  }
+  return GDEM_OK;
+}
+
+extern "C" int gdem_get_contacts(gdem_ctx* c, int64_t capacity, int64_t* n_out, int32_t* id0,
+                                 int32_t* id1, double* point, double* ovec, double* overlap,
+                                 double* force) {
+  if (!c || !n_out) return GDEM_ERR_ARG;
+  *n_out = 0;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  int rc = collect_errors(c);
+  if (rc) return rc;
+  if (!c->rec_valid || c->n == 0) return GDEM_OK;
+  const size_t n = (size_t)c->n, cap = (size_t)c->rec_cap;
+  std::vector<int32_t> cnt(n);
+  std::vector<ContactRec> rec(n * cap);
+  HIP_TRY(hipMemcpy(cnt.data(), c->d_rec_count, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(rec.data(), c->d_rec, sizeof(ContactRec) * n * cap, hipMemcpyDeviceToHost));
+  std::vector<ContactRec> all;
+  for (size_t t = 0; t < n; ++t)
+    for (int32_t k = 0; k < cnt[t] && (size_t)k < cap; ++k) all.push_back(rec[t * cap + k]);
+  std::sort(all.begin(), all.end(), [](const ContactRec& a, const ContactRec& b) {
+    if (a.id0 != b.id0) return a.id0 < b.id0;
+    // walls (negative ids) after particles, in wall order
+    const int64_t ka = a.id1 >= 0 ? a.id1 : (int64_t)0x7fffffff - a.id1;
+    const int64_t kb = b.id1 >= 0 ? b.id1 : (int64_t)0x7fffffff - b.id1;
+    return ka < kb;
+  });
+  *n_out = (int64_t)all.size();
+  const size_t m = std::min<size_t>(all.size(), (size_t)std::max<int64_t>(capacity, 0));
+  for (size_t k = 0; k < m; ++k) {
+    const ContactRec& r = all[k];
+    if (id0) id0[k] = r.id0;
+    if (id1) id1[k] = r.id1;
+    if (point) memcpy(point + 3 * k, r.point, sizeof(r.point));
+    if (ovec) memcpy(ovec + 3 * k, r.ovec, sizeof(r.ovec));
+    if (overlap) overlap[k] = r.dist;
+    if (force) memcpy(force + 3 * k, r.force, sizeof(r.force));
+  }
+  return (int64_t)all.size() > capacity ? GDEM_ERR_CAPACITY : GDEM_OK;
+}
+
+extern "C" int gdem_last_error(gdem_ctx* c, int32_t* code, int32_t* id0, int32_t* id1) {
+  if (!c) return GDEM_ERR_ARG;
+  if (code) *code = c->err_code;
+  if (id0) *id0 = c->err_id0;
+  if (id1) *id1 = c->err_id1;
+  return c->err_code;
+}
+
+extern "C" int gdem_set_kernel_timing(gdem_ctx* c, int32_t on) {
+  if (!c) return GDEM_ERR_ARG;
+  c->timing = on != 0;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_get_counters(gdem_ctx* c, int64_t* force_evals, int64_t* contacts,
+                                 double* force_kernel_ms, int64_t* timed_launches) {
+  if (!c) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (size_t k = 0; k < c->ev_used; ++k) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev_pool[k].first, c->ev_pool[k].second));
+    c->force_ms += ms;
+    c->timed++;
+  }
+  c->ev_used = 0;
+  unsigned long long h[2];
+  HIP_TRY(hipMemcpy(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost));
+  if (force_evals) *force_evals = c->force_evals;
+  if (contacts) *contacts = (int64_t)h[0];
+  if (force_kernel_ms) *force_kernel_ms = c->force_ms;
+  if (timed_launches) *timed_launches = c->timed;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_reset_counters(gdem_ctx* c) {
+  if (!c) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemset(c->d_counters, 0, sizeof(unsigned long long) * 2));
+  c->force_evals = 0;
+  c->force_ms = 0.;
+  c->timed = 0;
+  c->ev_used = 0;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_get_grid(gdem_ctx* c, int32_t* nc, double* edge) {
+  if (!c) return GDEM_ERR_ARG;
+  for (int d = 0; d < 3; ++d) {
+    if (nc) nc[d] = c->grid.nc[d];
+    if (edge) edge[d] = c->grid.edge[d];
+  }
+  return GDEM_OK;
+}

@@ -1,0 +1,312 @@
+// gdem_device.h -- FP64 device arithmetic of the Grains3D DEM step.
+//
+// Every helper keeps the operand order of the reference C++ so that results
+// round exactly as the CPU path does (the library is built with
+// -ffp-contract=off; no FMA is formed).  Paths are relative to
+// src/Grains3D/Grains/ of the reference.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace gdem {
+
+constexpr double kPi = 3.14159265358979323846;  // Base/include/Basic.hh:22
+constexpr double kEps = 1.0e-10;                // Basic.hh:26 (EPSILON)
+constexpr int kMaxMaterials = 32;
+constexpr int kMaxWalls = 64;
+
+struct v3 {
+  double x, y, z;
+};
+
+__host__ __device__ __forceinline__ v3 mk(double x, double y, double z) {
+  v3 r;
+  r.x = x;
+  r.y = y;
+  r.z = z;
+  return r;
+}
+// Geometry/src/Group3.cpp:149-196
+__host__ __device__ __forceinline__ v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ __forceinline__ v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ __forceinline__ v3 scl(v3 a, double d) { return mk(a.x * d, a.y * d, a.z * d); }
+__host__ __device__ __forceinline__ v3 dvd(v3 a, double d) { return mk(a.x / d, a.y / d, a.z / d); }
+__host__ __device__ __forceinline__ v3 neg(v3 a) { return mk(-a.x, -a.y, -a.z); }
+__host__ __device__ __forceinline__ double dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// Geometry/src/Vector3.cpp:114-119
+__host__ __device__ __forceinline__ v3 cross(v3 a, v3 b) {
+  return mk(a.y * b.z - a.z * b.y, -a.x * b.z + a.z * b.x, a.x * b.y - a.y * b.x);
+}
+// Vector3.cpp:147-151 (IEEE sqrt, correctly rounded)
+__host__ __device__ __forceinline__ double norm(v3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+// Group3.cpp:390-395
+__host__ __device__ __forceinline__ v3 round3(v3 a) {
+  return mk(fabs(a.x) < kEps ? 0. : a.x, fabs(a.y) < kEps ? 0. : a.y,
+            fabs(a.z) < kEps ? 0. : a.z);
+}
+
+struct quat {
+  v3 v;
+  double w;
+};
+__host__ __device__ __forceinline__ quat qmk(v3 v, double w) {
+  quat q;
+  q.v = v;
+  q.w = w;
+  return q;
+}
+// Geometry/src/Quaternion.cpp:74-77
+__host__ __device__ __forceinline__ quat qconj(quat q) { return qmk(neg(q.v), q.w); }
+// Quaternion.cpp:95-100: [0, lhs] x q (operator,(Vector3, Quaternion), :468-472)
+__host__ __device__ __forceinline__ quat qleftvec(quat q, v3 lhs) {
+  double tmp = dot(neg(lhs), q.v);
+  v3 vt = add(cross(lhs, q.v), scl(lhs, q.w));
+  return qmk(vt, tmp);
+}
+// Quaternion.cpp:108-113
+__host__ __device__ __forceinline__ v3 qmulv(quat p, quat r) {
+  return add(add(cross(p.v, r.v), scl(r.v, p.w)), scl(p.v, r.w));
+}
+// Quaternion.cpp:442-448
+__host__ __device__ __forceinline__ quat qmul(quat p, quat r) {
+  double tmp = (p.w * r.w) - dot(p.v, r.v);
+  v3 vt = add(add(cross(p.v, r.v), scl(r.v, p.w)), scl(p.v, r.w));
+  return qmk(vt, tmp);
+}
+
+// Contact model parameters (one entry per unordered material pair).
+struct Model {
+  int32_t kind;  // 0 none, 1 Hooke, 2 Hertz
+  int32_t pad;
+  double p[5];     // Hooke {kn, en, etat, muc, kr}; Hertz {Es, en, Gs, muc, kr}
+  double beta;     // HookeContactForceModel.cpp:19
+  double m2sqrt56; // HertzContactForceModel.cpp:21
+};
+
+// Static box obstacle.
+struct Wall {
+  double basis[9], origin[3], ext[3];
+  double inv_basis[9], inv_origin[3];
+  double bb_centre[3], bb_ext[3];
+  double crust, req;
+  int32_t material, pad;
+};
+
+// Velocity of a rigid body at a point: ParticleKinematics::Velocity
+// (Component/src/ParticleKinematics.cpp:229-232).
+__host__ __device__ __forceinline__ v3 vel_at(v3 x, v3 v, v3 w, v3 pt) { return add(v, cross(w, sub(pt, x))); }
+
+// performForcesCalculus of HookeContactForceModel (HookeContactForceModel.cpp:44-114)
+// and HertzContactForceModel (HertzContactForceModel.cpp:46-114), kr = 0.
+// Returns delFN + delFT, the force on body 0.
+__host__ __device__ __forceinline__ v3 contact_force(const Model& m, v3 pt, v3 pen, double odist, v3 x0,
+                                            v3 v0, v3 w0, double m0, double req0, v3 x1, v3 v1,
+                                            v3 w1, double m1, double req1) {
+  v3 normal = dvd(pen, norm(pen));
+  normal = round3(normal);
+  v3 tmpV = sub(vel_at(x0, v0, w0, pt), vel_at(x1, v1, w1, pt));
+  v3 v_n = scl(normal, dot(tmpV, normal));
+  v3 v_t = sub(tmpV, v_n);
+  v3 delFN, delFT;
+  double normFN, muc;
+  if (m.kind == 1) {
+    const double kn = m.p[0], etat_in = m.p[2];
+    muc = m.p[3];
+    delFN = scl(pen, kn);
+    double avmass = 1. / (1. / m0 + 1. / m1);
+    double gamman = -2. * m.beta * sqrt(avmass * kn);
+    delFN = sub(delFN, scl(v_n, gamman));
+    normFN = norm(delFN);
+    double etat = etat_in == -1. ? -m.beta * sqrt(kn / avmass) : etat_in;
+    double gammat = 2. * etat * avmass;
+    delFT = scl(v_t, -gammat);
+  } else {
+    const double Es = m.p[0], Gs = m.p[2];
+    muc = m.p[3];
+    double deltan = -odist;
+    double Req = 1. / (1. / req0 + 1. / req1);
+    double avmass = 1. / (1. / m0 + 1. / m1);
+    double sq = sqrt(Req * deltan);
+    double Sn = 2. * Es * sq;
+    double St = 8. * Gs * sq;
+    double kn = (4. / 3.) * Es * sq;
+    delFN = scl(pen, kn);
+    double gamman = m.m2sqrt56 * m.beta * sqrt(avmass * Sn);
+    delFN = sub(delFN, scl(v_n, gamman));
+    normFN = norm(delFN);
+    double gammat = m.m2sqrt56 * m.beta * sqrt(avmass * St);
+    delFT = scl(v_t, -gammat);
+  }
+  double fn = muc * normFN;
+  double ft = norm(delFT);
+  if (fn < ft) {
+    double nvt = norm(v_t);
+    v3 tangent = mk(0., 0., 0.);
+    if (nvt > kEps) tangent = dvd(neg(v_t), nvt);
+    delFT = scl(tangent, fn);
+  }
+  return add(delFN, delFT);
+}
+
+// ClosestPointSPHERESPHERE (Geometry/src/RigidBodyWithCrust.cpp:596-640).
+// 1 = contact candidate (dist <= 0), 0 = apart, -1 = ContactError.
+__host__ __device__ __forceinline__ int sphere_sphere(v3 A, double rA, double crA, v3 B, double rB,
+                                             double crB, v3& c, v3& ov, double& dist) {
+  v3 AB = sub(B, A);
+  double nAB = norm(AB);
+  dist = nAB - (rA + rB);
+  if (dist > 0.) return 0;
+  if (-dist >= crA + crB) return -1;
+  c = add(A, dvd(scl(AB, rA + 0.5 * dist), nAB));
+  ov = scl(dvd(AB, nAB), dist);
+  return 1;
+}
+
+// Transform::operator() (Geometry/src/Transform.cpp:312-317)
+__host__ __device__ __forceinline__ v3 xform(const double* b, const double* o, v3 p) {
+  return mk(dot(mk(b[0], b[1], b[2]), p) + o[0], dot(mk(b[3], b[4], b[5]), p) + o[1],
+            dot(mk(b[6], b[7], b[8]), p) + o[2]);
+}
+
+// Box corners, Box::setCornersFaces (Geometry/src/Box.cpp:92-111)
+__host__ __device__ __forceinline__ v3 box_corner_pt(const Wall& wl, int k) {
+  const int sx = (k == 1 || k == 2 || k == 5 || k == 6) ? 1 : -1;
+  const int sy = (k >= 4) ? 1 : -1;
+  const int sz = (k == 2 || k == 3 || k == 6 || k == 7) ? 1 : -1;
+  return mk(sx < 0 ? -wl.ext[0] : wl.ext[0], sy < 0 ? -wl.ext[1] : wl.ext[1],
+            sz < 0 ? -wl.ext[2] : wl.ext[2]);
+}
+
+// Box::ContactCornerSPHERE (Box.cpp:725-742)
+__host__ __device__ __forceinline__ v3 box_corner(const Wall& wl, v3 g, double r, int k, double& overlap) {
+  v3 cp = mk(0., 0., 0.);
+  v3 d = sub(g, box_corner_pt(wl, k));
+  double nd = norm(d);
+  if (nd < r) {
+    overlap = nd - r;
+    cp = sub(g, scl(d, 1. + 0.5 * overlap / nd));
+  }
+  return cp;
+}
+
+// Box::ContactEdgeSPHERE (Box.cpp:750-769)
+__host__ __device__ __forceinline__ v3 box_edge(const Wall& wl, v3 g, double r, int k, int dir,
+                                       double& overlap) {
+  v3 cp = mk(0., 0., 0.);
+  v3 c = box_corner_pt(wl, k);
+  v3 pr = g;
+  if (dir == 0) pr.x = c.x;
+  else if (dir == 1) pr.y = c.y;
+  else pr.z = c.z;
+  v3 d = sub(pr, c);
+  double nd = norm(d);
+  if (nd < r) {
+    overlap = nd - r;
+    cp = sub(g, scl(d, 1. + 0.5 * overlap / nd));
+  }
+  return cp;
+}
+
+// Box::IntersectionPointSPHERE (Box.cpp:466-716)
+__host__ __device__ inline v3 box_sphere(const Wall& wl, v3 g, double r, double& overlap) {
+  const double ex = wl.ext[0], ey = wl.ext[1], ez = wl.ext[2];
+  v3 cp = mk(0., 0., 0.);
+  double nd;
+  overlap = 1.;
+  if (g.x > ex) {
+    if (g.y > ey) {
+      if (g.z > ez) cp = box_corner(wl, g, r, 6, overlap);
+      else if (g.z < -ez) cp = box_corner(wl, g, r, 5, overlap);
+      else cp = box_edge(wl, g, r, 5, 2, overlap);
+    } else if (g.y < -ey) {
+      if (g.z > ez) cp = box_corner(wl, g, r, 2, overlap);
+      else if (g.z < -ez) cp = box_corner(wl, g, r, 1, overlap);
+      else cp = box_edge(wl, g, r, 1, 2, overlap);
+    } else {
+      if (g.z > ez) cp = box_edge(wl, g, r, 6, 1, overlap);
+      else if (g.z < -ez) cp = box_edge(wl, g, r, 1, 1, overlap);
+      else {
+        nd = g.x - ex;
+        if (nd < r) { overlap = nd - r; cp = mk(ex + 0.5 * overlap, g.y, g.z); }
+      }
+    }
+  } else if (g.x < -ex) {
+    if (g.y > ey) {
+      if (g.z > ez) cp = box_corner(wl, g, r, 7, overlap);
+      else if (g.z < -ez) cp = box_corner(wl, g, r, 4, overlap);
+      else cp = box_edge(wl, g, r, 4, 2, overlap);
+    } else if (g.y < -ey) {
+      if (g.z > ez) cp = box_corner(wl, g, r, 3, overlap);
+      else if (g.z < -ez) cp = box_corner(wl, g, r, 0, overlap);
+      else cp = box_edge(wl, g, r, 0, 2, overlap);
+    } else {
+      if (g.z > ez) cp = box_edge(wl, g, r, 3, 1, overlap);
+      else if (g.z < -ez) cp = box_edge(wl, g, r, 0, 1, overlap);
+      else {
+        nd = -ex - g.x;
+        if (nd < r) { overlap = nd - r; cp = mk(-ex - 0.5 * overlap, g.y, g.z); }
+      }
+    }
+  } else {
+    if (g.y > ey) {
+      if (g.z > ez) cp = box_edge(wl, g, r, 6, 0, overlap);
+      else if (g.z < -ez) cp = box_edge(wl, g, r, 4, 0, overlap);
+      else {
+        nd = g.y - ey;
+        if (nd < r) { overlap = nd - r; cp = mk(g.x, ey + 0.5 * overlap, g.z); }
+      }
+    } else if (g.y < -ey) {
+      if (g.z > ez) cp = box_edge(wl, g, r, 3, 0, overlap);
+      else if (g.z < -ez) cp = box_edge(wl, g, r, 0, 0, overlap);
+      else {
+        nd = -ey - g.y;
+        if (nd < r) { overlap = nd - r; cp = mk(g.x, -ey - 0.5 * overlap, g.z); }
+      }
+    } else {
+      if (g.z > ez) {
+        nd = g.z - ez;
+        if (nd < r) { overlap = nd - r; cp = mk(g.x, g.y, ez + 0.5 * overlap); }
+      } else if (g.z < -ez) {
+        nd = -ez - g.z;
+        if (nd < r) { overlap = nd - r; cp = mk(g.x, g.y, -ez - 0.5 * overlap); }
+      } else {
+        overlap = -10000.;
+      }
+    }
+  }
+  return cp;
+}
+
+// ClosestPointSPHEREBOX with the sphere first (RigidBodyWithCrust.cpp:647-683).
+__host__ __device__ __forceinline__ int sphere_wall(const Wall& wl, v3 A, double r, double cr, v3& point,
+                                           v3& ov, double& dist) {
+  double overlap = 0.;
+  v3 local = xform(wl.inv_basis, wl.inv_origin, A);
+  v3 cp = box_sphere(wl, local, r, overlap);
+  dist = overlap;
+  if (overlap < 0.) {
+    if (-overlap >= cr + wl.crust) return -1;
+    v3 contact = xform(wl.basis, wl.origin, cp);
+    v3 AB = sub(contact, A);
+    ov = scl(AB, overlap / norm(AB));
+    point = contact;
+    return 1;
+  }
+  return 0;
+}
+
+// SSPRK3 update of the body-frame angular velocity for a sphere
+// (TimeIntegration/src/SecondOrderLeapFrog.cpp:56-66 with
+// ParticleKinematicsSphere::computeAngularAccelerationBodyFixed,
+// Component/src/ParticleKinematicsSphere.cpp:47-55; coupling factor 1).
+__host__ __device__ __forceinline__ v3 ssprk3_sphere(v3 tq, v3 wbf, double I, double dt) {
+  v3 a = mk(tq.x / (1. * I), tq.y / (1. * I), tq.z / (1. * I));
+  v3 K1 = scl(a, dt);
+  v3 K2 = scl(a, dt);
+  v3 K3 = scl(a, dt);
+  return add(wbf, dvd(add(add(K1, K2), scl(K3, 4.)), 6.));
+}
+
+}  // namespace gdem

@@ -1,0 +1,99 @@
+// gdem_internal.h -- device buffers, launch parameters and launchers shared by
+// the kernels (gdem_kernels.hip) and the C ABI (gdem_capi.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gdem_device.h"
+
+namespace gdem {
+
+// Per-particle record kept next to the state: id, material, crust.
+struct Aux {
+  int32_t id;
+  int32_t mat;
+  double crust;
+};
+
+// One copy of the particle state, structure of double4 arrays, in the order of
+// the last binning (sorted by Morton cell key, ties by previous position).
+struct State {
+  double4* pos_r;   // x, y, z, radius
+  double4* vel_m;   // v, mass
+  double4* om_req;  // world angular velocity, equivalent-sphere radius
+  double4* wbf_I;   // body-frame angular velocity, principal inertia
+  double4* quat;    // orientation (x, y, z, w)
+  Aux* aux;
+};
+
+struct Grid {
+  double origin[3], edge[3], size[3];
+  int32_t nc[3];
+  int32_t periodic[3];
+  int32_t ncell;
+};
+
+// One recorded contact (first body's view).
+struct ContactRec {
+  int32_t id0, id1;
+  double point[3], ovec[3], dist, force[3];
+};
+
+// Device error words, min-reduced with integer atomics so the reported
+// offender does not depend on scheduling.
+enum { kErrContact = 0, kErrMotion = 1, kErrSim = 2, kErrCapacity = 3, kErrWords = 4 };
+constexpr unsigned long long kNoErr = ~0ull;
+
+struct ForceArgs {
+  State in, out;
+  const int32_t* perm;
+  const int32_t* cell_start;
+  const int32_t* rank;
+  Grid grid;
+  const Model* models;
+  const Wall* walls;
+  int32_t nwalls;
+  int32_t n;
+  double gravity[3];
+  double dt_vel, dt_disp;
+  double4* F;  // written when non-null
+  double4* T;
+  int32_t* key_out;  // next keys (MODE_MOVE)
+  unsigned long long* err;
+  unsigned long long* contacts;  // contact counter
+  ContactRec* rec;               // recording (nullable)
+  int32_t* rec_count;
+  int32_t rec_cap;
+  int32_t nblk8;  // blocks padded to a multiple of 8 (XCD-aware mapping)
+};
+
+enum { MODE_FORCE = 0, MODE_KICK = 1, MODE_MOVE = 2 };
+
+struct Bins {
+  int32_t* key;         // Morton rank of each particle's cell (storage order)
+  int32_t* slot;        // arrival slot within the cell (count pass)
+  int32_t* cnt;         // per-cell count (zeroed by the scan)
+  int32_t* local;       // block-local exclusive scan
+  int32_t* bsum;        // block sums, scanned in place
+  int32_t* perm;        // sorted slot -> storage index
+  int32_t* cell_start;  // ncell + 1
+  int32_t nbsum;
+};
+
+// Launchers (all asynchronous on the given stream).
+void launch_keys(const State& s, int32_t n, const Grid& g, const int32_t* rank, int32_t* key,
+                 unsigned long long* err, hipStream_t st);
+void launch_binning(Bins& b, int32_t n, int32_t ncell, hipStream_t st);
+void launch_force(int mode, bool record, const ForceArgs& a, hipStream_t st);
+void launch_move(const State& s, const double4* F, const double4* T, int32_t n, const Grid& g,
+                 const int32_t* rank, double dt_vel, double dt_disp, int32_t* key,
+                 unsigned long long* err, hipStream_t st);
+void launch_advance(const State& s, const double4* F, const double4* T, int32_t n,
+                    double dt_vel, hipStream_t st);
+void launch_unpermute(const State& s, const double4* F, const double4* T, int32_t n,
+                      double* x, double* v, double* w, double* q, double* f, double* t,
+                      hipStream_t st);
+void launch_copy_state(const State& src, const State& dst, const int32_t* perm, int32_t n,
+                       hipStream_t st);
+
+}  // namespace gdem

@@ -1,0 +1,603 @@
+// gdem_kernels.hip -- CDNA4 (gfx950) kernels of the DEM step.
+//
+//   binning : keys -> count (int atomics) -> block scan -> scatter -> per-cell
+//             sort by previous slot (deterministic counting sort; replaces
+//             LinkedCell::LinkUpdate, Base/src/LinkedCell.cpp:1452-1547)
+//   force   : one thread per particle in cell order; 27-cell sweep; every
+//             contact is evaluated with the reference's body order (first body
+//             = the one whose half stencil holds the other, LinkedCell.cpp:
+//             958-984, 1034-1060) so both partners compute bit-identical pair
+//             forces; each thread reduces its own contact segment in a fixed
+//             order (no floating-point atomics) and its tail applies the
+//             leap-frog kicks and drift (SecondOrderLeapFrog.cpp:44-94).
+//   move / advance : stand-alone integrator phases for the per-phase ABI.
+#include <hip/hip_runtime.h>
+
+#include "gdem_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace gdem {
+
+constexpr int kBlock = 256;
+constexpr int kScanBlock = 1024;
+
+__device__ __forceinline__ unsigned long long pack_ids(int32_t a, int32_t b) {
+  return ((unsigned long long)(uint32_t)a << 32) | (unsigned long long)(uint32_t)b;
+}
+
+// Cell::GetCell (Base/src/Cell.cpp:172-184); periodic indices are clamped to
+// the grid after the wrap of the position.  Returns false outside the grid.
+__device__ __forceinline__ bool cell_of(const Grid& g, v3 x, int* ijk) {
+  const double p[3] = {x.x, x.y, x.z};
+  bool ok = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double t = (p[d] - g.origin[d]) / g.edge[d];
+    int id;
+    if (!(t == t) || t > 2.0e9 || t < -2.0e9) {
+      ok = false;
+      id = 0;
+    } else {
+      id = (int)floor(t);
+      if (g.periodic[d]) {
+        if (id >= g.nc[d]) id = g.nc[d] - 1;
+        if (id < 0) id = 0;
+      } else if (id < 0 || id >= g.nc[d]) {
+        ok = false;
+      }
+    }
+    ijk[d] = id;
+  }
+  return ok;
+}
+
+// LinkedCell::getCellNumber (LinkedCell.cpp:1165-1168)
+__device__ __forceinline__ int cell_linear(const Grid& g, int i, int j, int k) {
+  return j * g.nc[2] * g.nc[0] + k * g.nc[0] + i;
+}
+
+__device__ __forceinline__ int logical_index(int nblk8) {
+  // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+  // contiguous range of cells so that neighbour reads stay in its own L2.
+  const int b = blockIdx.x;
+  const int per = nblk8 >> 3;
+  const int lb = (b & 7) * per + (b >> 3);
+  return lb * blockDim.x + threadIdx.x;
+}
+
+// ---------------------------------------------------------------- keys
+__global__ void __launch_bounds__(kBlock) k_keys(State s, int n, Grid g, const int32_t* rank,
+                                                 int32_t* key, unsigned long long* err) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double4 p = s.pos_r[i];
+  int ijk[3];
+  if (!cell_of(g, mk(p.x, p.y, p.z), ijk)) {
+    atomicMin(&err[kErrSim], (unsigned long long)(uint32_t)s.aux[i].id);
+    key[i] = 0;
+    return;
+  }
+  key[i] = rank[cell_linear(g, ijk[0], ijk[1], ijk[2])];
+}
+
+// ---------------------------------------------------------------- binning
+__global__ void __launch_bounds__(kBlock) k_count(const int32_t* key, int32_t* slot, int32_t* cnt,
+                                                  int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  slot[i] = atomicAdd(&cnt[key[i]], 1);
+}
+
+template <int BS>
+__device__ __forceinline__ int block_exclusive_scan(int v, int* smem, int& total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) smem[wid] = x;
+  __syncthreads();
+  if (wid == 0) {
+    int w = (lane < BS / 64) ? smem[lane] : 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      int y = __shfl_up(w, o, 64);
+      if (lane >= o) w += y;
+    }
+    if (lane < BS / 64) smem[lane] = w;
+  }
+  __syncthreads();
+  int prefix = wid > 0 ? smem[wid - 1] : 0;
+  total = smem[BS / 64 - 1];
+  return prefix + x - v;
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_local(int32_t* cnt, int32_t* local,
+                                                           int32_t* bsum, int ncell) {
+  __shared__ int smem[kScanBlock / 64];
+  int c = blockIdx.x * kScanBlock + threadIdx.x;
+  int v = c < ncell ? cnt[c] : 0;
+  int total;
+  int ex = block_exclusive_scan<kScanBlock>(v, smem, total);
+  if (c < ncell) {
+    local[c] = ex;
+    cnt[c] = 0;  // ready for the next count pass (no memset node in graphs)
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// Exclusive scan of the block sums by one workgroup (nb <= 8 * 1024).
+__global__ void __launch_bounds__(kScanBlock) k_scan_bsum(int32_t* bsum, int nb) {
+  __shared__ int smem[kScanBlock / 64];
+  const int per = (nb + kScanBlock - 1) / kScanBlock;
+  const int beg = threadIdx.x * per;
+  int loc[8];
+  int s = 0;
+  for (int k = 0; k < per; ++k) {
+    int idx = beg + k;
+    loc[k] = idx < nb ? bsum[idx] : 0;
+    s += loc[k];
+  }
+  int total;
+  int ex = block_exclusive_scan<kScanBlock>(s, smem, total);
+  for (int k = 0; k < per; ++k) {
+    int idx = beg + k;
+    if (idx < nb) bsum[idx] = ex;
+    ex += loc[k];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_scatter(const int32_t* key, const int32_t* slot,
+                                                    const int32_t* local, const int32_t* bsum,
+                                                    int32_t* perm, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int k = key[i];
+  perm[local[k] + bsum[k / kScanBlock] + slot[i]] = i;
+}
+
+// Orders each cell by previous slot, which makes the permutation independent of
+// the order in which the count pass's atomics were served.
+__global__ void __launch_bounds__(kBlock) k_cellsort(const int32_t* local, const int32_t* bsum,
+                                                     int32_t* perm, int32_t* cell_start,
+                                                     int ncell, int n) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int beg = local[c] + bsum[c / kScanBlock];
+  int end = (c + 1 < ncell) ? local[c + 1] + bsum[(c + 1) / kScanBlock] : n;
+  for (int a = beg + 1; a < end; ++a) {
+    int v = perm[a];
+    int b = a - 1;
+    while (b >= beg && perm[b] > v) {
+      perm[b + 1] = perm[b];
+      --b;
+    }
+    perm[b + 1] = v;
+  }
+  cell_start[c] = beg;
+  if (c == ncell - 1) cell_start[ncell] = n;
+}
+
+// ---------------------------------------------------------------- force
+// Which of two bodies in different cells is the reference's first body: the
+// one whose 13-cell half stencil (LinkedCell.cpp:965-967) holds the other's
+// cell, i.e. a positive offset in (j, k, i) lexicographic order.
+__device__ __forceinline__ bool in_half_stencil(int di, int dj, int dk) {
+  return dj > 0 || (dj == 0 && (dk > 0 || (dk == 0 && di > 0)));
+}
+
+template <int MODE, bool RECORD>
+__global__ void __launch_bounds__(kBlock) k_force(ForceArgs a) {
+  __shared__ unsigned long long sh_cnt[kBlock / 64];
+  const int t = logical_index(a.nblk8);
+  int ncont = 0;
+  if (t < a.n) {
+    const int s = a.perm[t];
+    const double4 pr = a.in.pos_r[s];
+    const double4 vm = a.in.vel_m[s];
+    const double4 wr = a.in.om_req[s];
+    const Aux ax = a.in.aux[s];
+    const v3 X = mk(pr.x, pr.y, pr.z);
+    const v3 V = mk(vm.x, vm.y, vm.z);
+    const v3 W = mk(wr.x, wr.y, wr.z);
+    const double r = pr.w, m = vm.w, req = wr.w;
+    int ijk[3];
+    cell_of(a.grid, X, ijk);
+    // a particle outside the grid was already reported (SimulationError) by
+    // the pass that computed its key; clamp so every index below stays in range
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ijk[d] = min(max(ijk[d], 0), a.grid.nc[d] - 1);
+    // InitializeForce: weight m (1 - 0/rho) g at the centre (Particle.cpp:414-419, 1409-1416)
+    const double mw = m * (1. - 0. / 1.);
+    v3 F = mk(a.gravity[0] * mw, a.gravity[1] * mw, a.gravity[2] * mw);
+    v3 T = mk(0., 0., 0.);
+    int nrec = 0;
+    for (int dk = -1; dk <= 1; ++dk) {
+      int kk = ijk[2] + dk;
+      double shz = 0.;
+      if (kk < 0 || kk >= a.grid.nc[2]) {
+        if (!a.grid.periodic[2]) continue;
+        if (kk < 0) { kk += a.grid.nc[2]; shz = -a.grid.size[2]; }
+        else { kk -= a.grid.nc[2]; shz = a.grid.size[2]; }
+      }
+      for (int dj = -1; dj <= 1; ++dj) {
+        int jj = ijk[1] + dj;
+        double shy = 0.;
+        if (jj < 0 || jj >= a.grid.nc[1]) {
+          if (!a.grid.periodic[1]) continue;
+          if (jj < 0) { jj += a.grid.nc[1]; shy = -a.grid.size[1]; }
+          else { jj -= a.grid.nc[1]; shy = a.grid.size[1]; }
+        }
+        for (int di = -1; di <= 1; ++di) {
+          int ii = ijk[0] + di;
+          double shx = 0.;
+          if (ii < 0 || ii >= a.grid.nc[0]) {
+            if (!a.grid.periodic[0]) continue;
+            if (ii < 0) { ii += a.grid.nc[0]; shx = -a.grid.size[0]; }
+            else { ii -= a.grid.nc[0]; shx = a.grid.size[0]; }
+          }
+          const bool same = (di == 0 && dj == 0 && dk == 0);
+          const bool stencil_first = in_half_stencil(di, dj, dk);
+          const int rk = a.rank[cell_linear(a.grid, ii, jj, kk)];
+          const int beg = a.cell_start[rk], end = a.cell_start[rk + 1];
+          for (int u = beg; u < end; ++u) {
+            if (u == t) continue;
+            const int s2 = a.perm[u];
+            const double4 p2 = a.in.pos_r[s2];
+            // cheap reject, looser than the exact test below
+            const v3 Bn = mk(p2.x + shx, p2.y + shy, p2.z + shz);
+            const v3 dd = sub(Bn, X);
+            const double d2 = dot(dd, dd);
+            const double rs = r + p2.w;
+            if (d2 > rs * rs * (1. + 1e-9)) continue;
+            const Aux ax2 = a.in.aux[s2];
+            const bool first = same ? (ax.id < ax2.id) : stencil_first;
+            // canonical pair: A = first body, B = second body moved to the
+            // first body's periodic image
+            v3 A, B;
+            double rA, rB, crA, crB;
+            if (first) {
+              A = X; B = Bn; rA = r; rB = p2.w; crA = ax.crust; crB = ax2.crust;
+            } else {
+              A = mk(p2.x, p2.y, p2.z);
+              B = mk(X.x + (-shx), X.y + (-shy), X.z + (-shz));
+              rA = p2.w; rB = r; crA = ax2.crust; crB = ax.crust;
+            }
+            v3 c, ov;
+            double dist;
+            const int rc = sphere_sphere(A, rA, crA, B, rB, crB, c, ov, dist);
+            if (rc < 0) {
+              atomicMin(&a.err[kErrContact], first ? pack_ids(ax.id, ax2.id) : pack_ids(ax2.id, ax.id));
+              continue;
+            }
+            if (rc == 0 || !(dist < 0.)) continue;
+            const double4 vm2 = a.in.vel_m[s2];
+            const double4 wr2 = a.in.om_req[s2];
+            const v3 V2 = mk(vm2.x, vm2.y, vm2.z), W2 = mk(wr2.x, wr2.y, wr2.z);
+            const Model md = a.models[(first ? ax.mat : ax2.mat) * kMaxMaterials +
+                                      (first ? ax2.mat : ax.mat)];
+            v3 f;
+            if (first)
+              f = contact_force(md, c, ov, dist, A, V, W, m, req, B, V2, W2, vm2.w, wr2.w);
+            else
+              f = contact_force(md, c, ov, dist, A, V2, W2, vm2.w, wr2.w, B, V, W, m, req);
+            if (first) {
+              F = add(F, f);
+              T = add(T, cross(sub(c, A), f));
+              ++ncont;
+              if (RECORD) {
+                if (nrec < a.rec_cap) {
+                  ContactRec& cr = a.rec[(size_t)t * a.rec_cap + nrec];
+                  cr.id0 = ax.id; cr.id1 = ax2.id;
+                  cr.point[0] = c.x; cr.point[1] = c.y; cr.point[2] = c.z;
+                  cr.ovec[0] = ov.x; cr.ovec[1] = ov.y; cr.ovec[2] = ov.z;
+                  cr.dist = dist;
+                  cr.force[0] = f.x; cr.force[1] = f.y; cr.force[2] = f.z;
+                } else {
+                  atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax.id);
+                }
+                ++nrec;
+              }
+            } else {
+              const v3 fn = neg(f);
+              F = add(F, fn);
+              T = add(T, cross(sub(c, B), fn));
+            }
+          }
+        }
+      }
+    }
+    // SimpleObstacle::InterAction (Component/src/SimpleObstacle.cpp:222-271)
+    for (int wi = 0; wi < a.nwalls; ++wi) {
+      const Wall& wl = a.walls[wi];
+      if (!(fabs(wl.bb_centre[0] - X.x) <= wl.bb_ext[0] + r &&
+            fabs(wl.bb_centre[1] - X.y) <= wl.bb_ext[1] + r &&
+            fabs(wl.bb_centre[2] - X.z) <= wl.bb_ext[2] + r))
+        continue;
+      v3 c = mk(0., 0., 0.), ov = mk(0., 0., 0.);
+      double dist;
+      const int rc = sphere_wall(wl, X, r, ax.crust, c, ov, dist);
+      if (rc < 0) {
+        atomicMin(&a.err[kErrContact], pack_ids(ax.id, -1 - wi));
+        continue;
+      }
+      if (rc == 0 || !(dist < 0.)) continue;
+      const Model md = a.models[ax.mat * kMaxMaterials + wl.material];
+      const v3 zero = mk(0., 0., 0.);
+      const v3 f = contact_force(md, c, ov, dist, X, V, W, m, req,
+                                 mk(wl.origin[0], wl.origin[1], wl.origin[2]), zero, zero, 1.e20,
+                                 wl.req);
+      F = add(F, f);
+      T = add(T, cross(sub(c, X), f));
+      ++ncont;
+      if (RECORD) {
+        if (nrec < a.rec_cap) {
+          ContactRec& cr = a.rec[(size_t)t * a.rec_cap + nrec];
+          cr.id0 = ax.id; cr.id1 = -1 - wi;
+          cr.point[0] = c.x; cr.point[1] = c.y; cr.point[2] = c.z;
+          cr.ovec[0] = ov.x; cr.ovec[1] = ov.y; cr.ovec[2] = ov.z;
+          cr.dist = dist;
+          cr.force[0] = f.x; cr.force[1] = f.y; cr.force[2] = f.z;
+        } else {
+          atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax.id);
+        }
+        ++nrec;
+      }
+    }
+    if (RECORD) a.rec_count[t] = nrec;
+
+    // ---- tail
+    const double4 wb = a.in.wbf_I[s];
+    const double4 qq = a.in.quat[s];
+    v3 Xo = X, Vo = V, Wo = W;
+    v3 WBF = mk(wb.x, wb.y, wb.z);
+    quat q = qmk(mk(qq.x, qq.y, qq.z), qq.w);
+    int key = a.rank[cell_linear(a.grid, ijk[0], ijk[1], ijk[2])];
+    if (MODE != MODE_FORCE) {
+      const double I = wb.w;
+      const quat pc = qconj(q);
+      // torque in body frame (ParticleKinematics.cpp:115-117)
+      const v3 tq = qmulv(pc, qleftvec(q, T));
+      const double mcf = m * 1.;
+      // closing half kick, SecondOrderLeapFrog::advanceVelocity (:74-94)
+      Vo = add(V, scl(dvd(F, mcf), a.dt_vel));
+      WBF = ssprk3_sphere(tq, WBF, I, a.dt_vel);
+      Wo = qmulv(q, qleftvec(pc, WBF));
+      if (MODE == MODE_MOVE) {
+        // next step's kick and drift, SecondOrderLeapFrog::Move (:44-67) and
+        // ParticleKinematics::Move (:62-104); q and T unchanged -> same tq
+        Vo = add(Vo, scl(dvd(F, mcf), a.dt_vel));
+        const v3 dx = scl(Vo, a.dt_disp);
+        WBF = ssprk3_sphere(tq, WBF, I, a.dt_vel);
+        Wo = qmulv(q, qleftvec(pc, WBF));
+        const v3 avg = Wo;
+        Xo = add(X, dx);
+        const double nOmega = norm(avg);
+        quat dq;
+        if (nOmega > kEps) {
+          const double cc = cos(nOmega * a.dt_disp / 2.);
+          const double sn = sin(nOmega * a.dt_disp / 2.);
+          dq = qmk(scl(avg, sn * 1. / nOmega), cc);
+        } else {
+          dq = qmk(mk(0., 0., 0.), 1.);
+        }
+        q = qmul(dq, q);
+        const double depl = norm(dx);
+        if (depl > ax.crust) atomicMin(&a.err[kErrMotion], (unsigned long long)(uint32_t)ax.id);
+        double xo[3] = {Xo.x, Xo.y, Xo.z};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          if (!a.grid.periodic[d]) continue;
+          const double lo = a.grid.origin[d], L = a.grid.size[d];
+          if (xo[d] >= lo + L) xo[d] -= L;
+          else if (xo[d] < lo) xo[d] += L;
+        }
+        Xo = mk(xo[0], xo[1], xo[2]);
+        int nijk[3];
+        if (cell_of(a.grid, Xo, nijk)) {
+          key = a.rank[cell_linear(a.grid, nijk[0], nijk[1], nijk[2])];
+        } else {
+          key = 0;
+          atomicMin(&a.err[kErrSim], (unsigned long long)(uint32_t)ax.id);
+        }
+      }
+    }
+    a.out.pos_r[t] = make_double4(Xo.x, Xo.y, Xo.z, r);
+    a.out.vel_m[t] = make_double4(Vo.x, Vo.y, Vo.z, m);
+    a.out.om_req[t] = make_double4(Wo.x, Wo.y, Wo.z, req);
+    a.out.wbf_I[t] = make_double4(WBF.x, WBF.y, WBF.z, wb.w);
+    a.out.quat[t] = make_double4(q.v.x, q.v.y, q.v.z, q.w);
+    a.out.aux[t] = ax;
+    a.key_out[t] = key;
+    if (a.F) {
+      a.F[t] = make_double4(F.x, F.y, F.z, 0.);
+      a.T[t] = make_double4(T.x, T.y, T.z, 0.);
+    }
+  }
+  // contact counter: wave sum, then one integer atomic per workgroup
+  unsigned long long c = (unsigned long long)ncont;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0) sh_cnt[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tot = 0;
+    for (int w = 0; w < kBlock / 64; ++w) tot += sh_cnt[w];
+    if (tot) atomicAdd(a.contacts, tot);
+  }
+}
+
+// ---------------------------------------------------------------- integrator phases
+__global__ void __launch_bounds__(kBlock) k_move(State s, const double4* Fa, const double4* Ta,
+                                                 int n, Grid g, const int32_t* rank, double dtv,
+                                                 double dtd, int32_t* key,
+                                                 unsigned long long* err) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 pr = s.pos_r[i], vm = s.vel_m[i], wr = s.om_req[i], wb = s.wbf_I[i],
+                qq = s.quat[i];
+  const double4 f4 = Fa[i], t4 = Ta[i];
+  const Aux ax = s.aux[i];
+  const v3 F = mk(f4.x, f4.y, f4.z), T = mk(t4.x, t4.y, t4.z);
+  quat q = qmk(mk(qq.x, qq.y, qq.z), qq.w);
+  const quat pc = qconj(q);
+  const v3 tq = qmulv(pc, qleftvec(q, T));
+  const double mcf = vm.w * 1.;
+  const v3 V = add(mk(vm.x, vm.y, vm.z), scl(dvd(F, mcf), dtv));
+  const v3 dx = scl(V, dtd);
+  const v3 WBF = ssprk3_sphere(tq, mk(wb.x, wb.y, wb.z), wb.w, dtv);
+  const v3 W = qmulv(q, qleftvec(pc, WBF));
+  const v3 avg = W;
+  v3 X = add(mk(pr.x, pr.y, pr.z), dx);
+  const double nOmega = norm(avg);
+  quat dq;
+  if (nOmega > kEps) {
+    const double cc = cos(nOmega * dtd / 2.);
+    const double sn = sin(nOmega * dtd / 2.);
+    dq = qmk(scl(avg, sn * 1. / nOmega), cc);
+  } else {
+    dq = qmk(mk(0., 0., 0.), 1.);
+  }
+  q = qmul(dq, q);
+  if (norm(dx) > ax.crust) atomicMin(&err[kErrMotion], (unsigned long long)(uint32_t)ax.id);
+  double xo[3] = {X.x, X.y, X.z};
+  for (int d = 0; d < 3; ++d) {
+    if (!g.periodic[d]) continue;
+    const double lo = g.origin[d], L = g.size[d];
+    if (xo[d] >= lo + L) xo[d] -= L;
+    else if (xo[d] < lo) xo[d] += L;
+  }
+  X = mk(xo[0], xo[1], xo[2]);
+  int ijk[3];
+  if (cell_of(g, X, ijk)) {
+    key[i] = rank[cell_linear(g, ijk[0], ijk[1], ijk[2])];
+  } else {
+    key[i] = 0;
+    atomicMin(&err[kErrSim], (unsigned long long)(uint32_t)ax.id);
+  }
+  s.pos_r[i] = make_double4(X.x, X.y, X.z, pr.w);
+  s.vel_m[i] = make_double4(V.x, V.y, V.z, vm.w);
+  s.om_req[i] = make_double4(W.x, W.y, W.z, wr.w);
+  s.wbf_I[i] = make_double4(WBF.x, WBF.y, WBF.z, wb.w);
+  s.quat[i] = make_double4(q.v.x, q.v.y, q.v.z, q.w);
+}
+
+__global__ void __launch_bounds__(kBlock) k_advance(State s, const double4* Fa, const double4* Ta,
+                                                    int n, double dtv) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 vm = s.vel_m[i], wb = s.wbf_I[i], qq = s.quat[i];
+  const double4 f4 = Fa[i], t4 = Ta[i];
+  const v3 F = mk(f4.x, f4.y, f4.z), T = mk(t4.x, t4.y, t4.z);
+  const quat q = qmk(mk(qq.x, qq.y, qq.z), qq.w);
+  const quat pc = qconj(q);
+  const v3 tq = qmulv(pc, qleftvec(q, T));
+  const double mcf = vm.w * 1.;
+  const v3 V = add(mk(vm.x, vm.y, vm.z), scl(dvd(F, mcf), dtv));
+  const v3 WBF = ssprk3_sphere(tq, mk(wb.x, wb.y, wb.z), wb.w, dtv);
+  const v3 W = qmulv(q, qleftvec(pc, WBF));
+  const double req = s.om_req[i].w;
+  s.vel_m[i] = make_double4(V.x, V.y, V.z, vm.w);
+  s.wbf_I[i] = make_double4(WBF.x, WBF.y, WBF.z, wb.w);
+  s.om_req[i] = make_double4(W.x, W.y, W.z, req);
+}
+
+__global__ void __launch_bounds__(kBlock) k_unpermute(State s, const double4* Fa,
+                                                      const double4* Ta, int n, double* x,
+                                                      double* v, double* w, double* q,
+                                                      double* f, double* t) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t id = (size_t)s.aux[i].id;
+  if (x) { double4 a = s.pos_r[i]; x[3 * id] = a.x; x[3 * id + 1] = a.y; x[3 * id + 2] = a.z; }
+  if (v) { double4 a = s.vel_m[i]; v[3 * id] = a.x; v[3 * id + 1] = a.y; v[3 * id + 2] = a.z; }
+  if (w) { double4 a = s.om_req[i]; w[3 * id] = a.x; w[3 * id + 1] = a.y; w[3 * id + 2] = a.z; }
+  if (q) {
+    double4 a = s.quat[i];
+    q[4 * id] = a.x; q[4 * id + 1] = a.y; q[4 * id + 2] = a.z; q[4 * id + 3] = a.w;
+  }
+  if (f) { double4 a = Fa[i]; f[3 * id] = a.x; f[3 * id + 1] = a.y; f[3 * id + 2] = a.z; }
+  if (t) { double4 a = Ta[i]; t[3 * id] = a.x; t[3 * id + 1] = a.y; t[3 * id + 2] = a.z; }
+}
+
+__global__ void __launch_bounds__(kBlock) k_copy_state(State src, State dst, const int32_t* perm,
+                                                       int n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int s = perm[t];
+  dst.pos_r[t] = src.pos_r[s];
+  dst.vel_m[t] = src.vel_m[s];
+  dst.om_req[t] = src.om_req[s];
+  dst.wbf_I[t] = src.wbf_I[s];
+  dst.quat[t] = src.quat[s];
+  dst.aux[t] = src.aux[s];
+}
+
+// ---------------------------------------------------------------- launchers
+static inline int nblocks(int n, int bs) { return (n + bs - 1) / bs; }
+
+void launch_keys(const State& s, int32_t n, const Grid& g, const int32_t* rank, int32_t* key,
+                 unsigned long long* err, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_keys, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, s, n, g, rank, key, err);
+}
+
+void launch_binning(Bins& b, int32_t n, int32_t ncell, hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_count, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, b.key, b.slot, b.cnt, n);
+  const int nb = nblocks(ncell, kScanBlock);
+  hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kScanBlock), 0, st, b.cnt, b.local, b.bsum, ncell);
+  hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(kScanBlock), 0, st, b.bsum, nb);
+  if (n > 0)
+    hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, b.key, b.slot,
+                       b.local, b.bsum, b.perm, n);
+  hipLaunchKernelGGL(k_cellsort, dim3(nblocks(ncell, kBlock)), dim3(kBlock), 0, st, b.local,
+                     b.bsum, b.perm, b.cell_start, ncell, n);
+}
+
+void launch_force(int mode, bool record, const ForceArgs& a, hipStream_t st) {
+  if (a.n <= 0) return;
+  dim3 grid(a.nblk8), block(kBlock);
+#define GDEM_LAUNCH_FORCE(M, R) hipLaunchKernelGGL((k_force<M, R>), grid, block, 0, st, a)
+  if (record) {
+    if (mode == MODE_FORCE) GDEM_LAUNCH_FORCE(MODE_FORCE, true);
+    else if (mode == MODE_KICK) GDEM_LAUNCH_FORCE(MODE_KICK, true);
+    else GDEM_LAUNCH_FORCE(MODE_MOVE, true);
+  } else {
+    if (mode == MODE_FORCE) GDEM_LAUNCH_FORCE(MODE_FORCE, false);
+    else if (mode == MODE_KICK) GDEM_LAUNCH_FORCE(MODE_KICK, false);
+    else GDEM_LAUNCH_FORCE(MODE_MOVE, false);
+  }
+#undef GDEM_LAUNCH_FORCE
+}
+
+void launch_move(const State& s, const double4* F, const double4* T, int32_t n, const Grid& g,
+                 const int32_t* rank, double dt_vel, double dt_disp, int32_t* key,
+                 unsigned long long* err, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_move, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, s, F, T, n, g, rank,
+                     dt_vel, dt_disp, key, err);
+}
+
+void launch_advance(const State& s, const double4* F, const double4* T, int32_t n,
+                    double dt_vel, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_advance, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, s, F, T, n, dt_vel);
+}
+
+void launch_unpermute(const State& s, const double4* F, const double4* T, int32_t n, double* x,
+                      double* v, double* w, double* q, double* f, double* t, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_unpermute, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, s, F, T, n, x,
+                     v, w, q, f, t);
+}
+
+void launch_copy_state(const State& src, const State& dst, const int32_t* perm, int32_t n,
+                       hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy_state, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, src, dst, perm, n);
+}
+
+}  // namespace gdem

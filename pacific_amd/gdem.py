@@ -1,0 +1,262 @@
+"""ctypes binding of libgdem.so (include/gdem.h).
+
+Python mirror of the batched plugin seams of Grains3D: one ``GdemSystem`` is
+one ``Grains`` application restricted to its step (LinkedCell binning,
+ContactForceModel forces, SecondOrderLeapFrog integration).  Errors the device
+raises come back as the reference's exception types (ContactError,
+MotionError, SimulationError; src/Grains3D/Grains/Base/include/Error.hh:19,75,129).
+
+There is no CPU fallback: if the HIP library is missing the import of this
+module fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libgdem.so")
+
+OK, ERR_CONTACT, ERR_MOTION, ERR_SIMULATION, ERR_HIP, ERR_ARG, ERR_CAPACITY = range(7)
+MODEL_HOOKE, MODEL_HERTZ = 1, 2
+
+
+class GdemError(RuntimeError):
+    def __init__(self, code, msg, id0=-1, id1=-1):
+        super().__init__(f"{msg} (code {code}, ids {id0}, {id1})")
+        self.code, self.id0, self.id1 = code, id0, id1
+
+
+class ContactError(GdemError):
+    """Overlap reached the sum of the crust thicknesses (RigidBodyWithCrust.cpp:624-630)."""
+
+
+class MotionError(GdemError):
+    """Displacement over one step exceeded the crust (Particle.cpp:447-457)."""
+
+
+class SimulationError(GdemError):
+    """Particle left the linked-cell grid (LinkedCell.cpp:1506-1531)."""
+
+
+class CapacityError(GdemError):
+    """A contact buffer was too small."""
+
+
+_ERRORS = {ERR_CONTACT: ContactError, ERR_MOTION: MotionError,
+           ERR_SIMULATION: SimulationError, ERR_CAPACITY: CapacityError}
+
+
+class _Config(C.Structure):
+    _fields_ = [("origin", C.c_double * 3), ("size", C.c_double * 3),
+                ("periodic", C.c_int32 * 3), ("cell_size", C.c_double),
+                ("gravity", C.c_double * 3), ("device", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libgdem.so once; raises OSError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"{path} is missing: build it with __graft_entry__.build()")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    d, i32, i64 = C.c_double, C.c_int32, C.c_int64
+    vp = C.c_void_p
+    sig = {
+        "gdem_create": (C.c_int, [P(_Config), P(vp)]),
+        "gdem_destroy": (C.c_int, [vp]),
+        "gdem_set_particles": (C.c_int, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "gdem_add_wall": (C.c_int, [vp, vp, vp, vp, d, i32]),
+        "gdem_set_contact_model": (C.c_int, [vp, i32, i32, i32, vp]),
+        "gdem_link_update": (C.c_int, [vp]),
+        "gdem_compute_forces": (C.c_int, [vp]),
+        "gdem_move": (C.c_int, [vp, d, d]),
+        "gdem_advance_velocity": (C.c_int, [vp, d]),
+        "gdem_prime": (C.c_int, [vp]),
+        "gdem_step": (C.c_int, [vp, i32, d]),
+        "gdem_synchronize": (C.c_int, [vp]),
+        "gdem_get_state": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+        "gdem_set_contact_recording": (C.c_int, [vp, i32]),
+        "gdem_get_contacts": (C.c_int, [vp, i64, P(i64), vp, vp, vp, vp, vp, vp]),
+        "gdem_last_error": (C.c_int, [vp, P(i32), P(i32), P(i32)]),
+        "gdem_set_kernel_timing": (C.c_int, [vp, i32]),
+        "gdem_get_counters": (C.c_int, [vp, P(i64), P(i64), P(d), P(i64)]),
+        "gdem_reset_counters": (C.c_int, [vp]),
+        "gdem_get_grid": (C.c_int, [vp, P(i32), P(d)]),
+        "gdem_version": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall",
+            "gdem_set_contact_model", "gdem_link_update", "gdem_compute_forces",
+            "gdem_move", "gdem_advance_velocity", "gdem_prime", "gdem_step",
+            "gdem_synchronize", "gdem_get_state", "gdem_set_contact_recording",
+            "gdem_get_contacts", "gdem_last_error", "gdem_set_kernel_timing",
+            "gdem_get_counters", "gdem_reset_counters", "gdem_get_grid", "gdem_version"]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, shape):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(shape))
+
+
+class GdemSystem:
+    """One DEM system on one GPU (see include/gdem.h for the phase mapping)."""
+
+    def __init__(self, origin, size, periodic=(0, 0, 0), cell_size=0.0,
+                 gravity=(0.0, 0.0, 0.0), device=0):
+        self.lib = load_library()
+        cfg = _Config()
+        for d in range(3):
+            cfg.origin[d] = float(origin[d])
+            cfg.size[d] = float(size[d])
+            cfg.periodic[d] = int(bool(periodic[d]))
+            cfg.gravity[d] = float(gravity[d])
+        cfg.cell_size = float(cell_size)
+        cfg.device = int(device)
+        h = C.c_void_p()
+        self._check(self.lib.gdem_create(C.byref(cfg), C.byref(h)), "gdem_create")
+        self.h = h
+        self.n = 0
+
+    # -- lifecycle
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gdem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc == OK:
+            return
+        id0 = id1 = -1
+        if getattr(self, "h", None):
+            code, a, b = C.c_int32(), C.c_int32(), C.c_int32()
+            self.lib.gdem_last_error(self.h, C.byref(code), C.byref(a), C.byref(b))
+            id0, id1 = a.value, b.value
+        raise _ERRORS.get(rc, GdemError)(rc, what, id0, id1)
+
+    # -- setup
+    def set_particles(self, x, v, w, q, radius, density, crust, material=None):
+        n = len(radius)
+        self._keep = [_f64(x, (n, 3)), _f64(v, (n, 3)), _f64(w, (n, 3)), _f64(q, (n, 4)),
+                      _f64(radius, (n,)), _f64(density, (n,)), _f64(crust, (n,))]
+        mat = None if material is None else np.ascontiguousarray(material, dtype=np.int32)
+        self._check(self.lib.gdem_set_particles(self.h, n, *[_ptr(a) for a in self._keep],
+                                                _ptr(mat)), "gdem_set_particles")
+        self.n = n
+
+    def add_wall(self, basis, centre, half_extent, crust, material):
+        self._check(self.lib.gdem_add_wall(self.h, _ptr(_f64(basis, (9,))), _ptr(_f64(centre, (3,))),
+                                           _ptr(_f64(half_extent, (3,))), float(crust), int(material)),
+                    "gdem_add_wall")
+
+    def set_contact_model(self, mat_a, mat_b, kind, params):
+        p = _f64(params, (5,))
+        self._check(self.lib.gdem_set_contact_model(self.h, int(mat_a), int(mat_b), int(kind), _ptr(p)),
+                    "gdem_set_contact_model")
+
+    # -- phases
+    def link_update(self):
+        self._check(self.lib.gdem_link_update(self.h), "gdem_link_update")
+
+    def compute_forces(self):
+        self._check(self.lib.gdem_compute_forces(self.h), "gdem_compute_forces")
+
+    def move(self, dt_vel, dt_disp):
+        self._check(self.lib.gdem_move(self.h, float(dt_vel), float(dt_disp)), "gdem_move")
+
+    def advance_velocity(self, dt_vel):
+        self._check(self.lib.gdem_advance_velocity(self.h, float(dt_vel)), "gdem_advance_velocity")
+
+    def prime(self):
+        self._check(self.lib.gdem_prime(self.h), "gdem_prime")
+
+    def step(self, nsteps, dt):
+        self._check(self.lib.gdem_step(self.h, int(nsteps), float(dt)), "gdem_step")
+
+    def synchronize(self):
+        self._check(self.lib.gdem_synchronize(self.h), "gdem_synchronize")
+
+    # -- results
+    def get_state(self):
+        n = self.n
+        out = {k: np.empty((n, 3)) for k in ("x", "v", "w", "F", "T")}
+        out["q"] = np.empty((n, 4))
+        self._check(self.lib.gdem_get_state(self.h, _ptr(out["x"]), _ptr(out["v"]), _ptr(out["w"]),
+                                            _ptr(out["q"]), _ptr(out["F"]), _ptr(out["T"])),
+                    "gdem_get_state")
+        return out
+
+    def set_contact_recording(self, cap):
+        self._check(self.lib.gdem_set_contact_recording(self.h, int(cap)), "gdem_set_contact_recording")
+
+    def get_contacts(self):
+        nout = C.c_int64()
+        rc = self.lib.gdem_get_contacts(self.h, 0, C.byref(nout), None, None, None, None, None, None)
+        if rc not in (OK, ERR_CAPACITY):
+            self._check(rc, "gdem_get_contacts")
+        m = nout.value
+        res = {"id0": np.empty(m, np.int32), "id1": np.empty(m, np.int32),
+               "point": np.empty((m, 3)), "ovec": np.empty((m, 3)),
+               "overlap": np.empty(m), "force": np.empty((m, 3))}
+        self._check(self.lib.gdem_get_contacts(self.h, m, C.byref(nout), _ptr(res["id0"]),
+                                               _ptr(res["id1"]), _ptr(res["point"]), _ptr(res["ovec"]),
+                                               _ptr(res["overlap"]), _ptr(res["force"])),
+                    "gdem_get_contacts")
+        return res
+
+    def set_kernel_timing(self, on):
+        self._check(self.lib.gdem_set_kernel_timing(self.h, int(bool(on))), "gdem_set_kernel_timing")
+
+    def counters(self):
+        fe, ct, tl = C.c_int64(), C.c_int64(), C.c_int64()
+        ms = C.c_double()
+        self._check(self.lib.gdem_get_counters(self.h, C.byref(fe), C.byref(ct), C.byref(ms), C.byref(tl)),
+                    "gdem_get_counters")
+        return {"force_evals": fe.value, "contacts": ct.value, "force_kernel_ms": ms.value,
+                "timed_launches": tl.value}
+
+    def reset_counters(self):
+        self._check(self.lib.gdem_reset_counters(self.h), "gdem_reset_counters")
+
+    def grid(self):
+        nc = (C.c_int32 * 3)()
+        e = (C.c_double * 3)()
+        self._check(self.lib.gdem_get_grid(self.h, nc, e), "gdem_get_grid")
+        return list(nc), list(e)
+
+
+def build_system(wl, device=0):
+    """Create a GdemSystem from a workload dict (pacific_amd.workloads)."""
+    s = GdemSystem(wl["origin"], wl["size"], wl["periodic"], wl["cell_size"], wl["gravity"], device)
+    for (a, b, kind, p) in wl["models"]:
+        s.set_contact_model(a, b, kind, p)
+    for wdef in wl["walls"]:
+        s.add_wall(*wdef)
+    p = wl["particles"]
+    s.set_particles(p["x"], p["v"], p["w"], p["q"], p["radius"], p["density"], p["crust"],
+                    p["material"])
+    return s
