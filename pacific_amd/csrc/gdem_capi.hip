@@ -26,13 +26,17 @@ struct gdem_ctx {
   int dev = 0;
   hipStream_t stream = nullptr;
   Grid grid{};
-  int32_t* d_rank = nullptr;
+  int32_t* d_row_rank = nullptr;
   int64_t n = 0;
   State buf[2]{};
   int cur = 0;
   double4* d_F = nullptr;
   double4* d_T = nullptr;
   Bins bins{};
+  int2* d_out_list = nullptr;
+  int2* d_in_list = nullptr;
+  int32_t* d_out_count = nullptr;
+  int32_t* d_in_count = nullptr;
   Model h_models[kMaxMaterials * kMaxMaterials]{};
   Model* d_models = nullptr;
   Wall h_walls[kMaxWalls]{};
@@ -76,8 +80,8 @@ static int set_err(gdem_ctx* c, int code, int id0, int id1) {
 }
 
 static void free_state(State& s) {
-  hipFree(s.pos_r); hipFree(s.vel_m); hipFree(s.om_req); hipFree(s.wbf_I); hipFree(s.quat);
-  hipFree(s.aux);
+  (void)hipFree(s.pos_r); (void)hipFree(s.vel_m); (void)hipFree(s.om_req); (void)hipFree(s.wbf_I); (void)hipFree(s.quat);
+  (void)hipFree(s.aux);
   s = State{};
 }
 
@@ -92,25 +96,22 @@ static int alloc_state(State& s, int64_t n) {
   return GDEM_OK;
 }
 
-// Morton (Z-order) rank of each cell, bits interleaved i, j, k from the least
-// significant end; cells are binned and swept in this order.
-static std::vector<int32_t> morton_rank(const int32_t nc[3]) {
+// Rank of each cell row (j, k) in a 2-D Morton (Z) order of (j, k); the cell
+// key is row_rank[j * nk + k] * ni + i, so the cells of a row along x are
+// consecutive in the sorted order and neighbouring rows stay close.
+static std::vector<int32_t> row_morton_rank(const int32_t nc[3]) {
   auto spread = [](uint64_t v) {
     uint64_t r = 0;
-    for (int b = 0; b < 21; ++b) r |= ((v >> b) & 1ull) << (3 * b);
+    for (int b = 0; b < 31; ++b) r |= ((v >> b) & 1ull) << (2 * b);
     return r;
   };
-  const int64_t ncell = (int64_t)nc[0] * nc[1] * nc[2];
   std::vector<std::pair<uint64_t, int32_t>> codes;
-  codes.reserve((size_t)ncell);
+  codes.reserve((size_t)nc[1] * nc[2]);
   for (int j = 0; j < nc[1]; ++j)
     for (int k = 0; k < nc[2]; ++k)
-      for (int i = 0; i < nc[0]; ++i) {
-        uint64_t code = spread((uint64_t)i) | (spread((uint64_t)j) << 1) | (spread((uint64_t)k) << 2);
-        codes.emplace_back(code, (int32_t)(j * nc[2] * nc[0] + k * nc[0] + i));
-      }
+      codes.emplace_back(spread((uint64_t)k) | (spread((uint64_t)j) << 1), (int32_t)(j * nc[2] + k));
   std::sort(codes.begin(), codes.end());
-  std::vector<int32_t> rank((size_t)ncell);
+  std::vector<int32_t> rank(codes.size());
   for (size_t r = 0; r < codes.size(); ++r) rank[(size_t)codes[r].second] = (int32_t)r;
   return rank;
 }
@@ -145,10 +146,11 @@ extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
   c->grid.ncell = (int32_t)ncell;
   if (hipSetDevice(c->dev) != hipSuccess) { delete c; return GDEM_ERR_HIP; }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return GDEM_ERR_HIP; }
-  std::vector<int32_t> rank = morton_rank(c->grid.nc);
+  std::vector<int32_t> rank = row_morton_rank(c->grid.nc);
+  const int64_t nrows = (int64_t)c->grid.nc[1] * c->grid.nc[2];
   const int nb = (int)((ncell + 1023) / 1024);
   c->bins.nbsum = nb;
-  if (hipMalloc(&c->d_rank, sizeof(int32_t) * ncell) != hipSuccess ||
+  if (hipMalloc(&c->d_row_rank, sizeof(int32_t) * nrows) != hipSuccess ||
       hipMalloc(&c->bins.cnt, sizeof(int32_t) * ncell) != hipSuccess ||
       hipMalloc(&c->bins.local, sizeof(int32_t) * ncell) != hipSuccess ||
       hipMalloc(&c->bins.bsum, sizeof(int32_t) * nb) != hipSuccess ||
@@ -160,11 +162,11 @@ extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
     gdem_destroy(c);
     return GDEM_ERR_HIP;
   }
-  hipMemcpy(c->d_rank, rank.data(), sizeof(int32_t) * ncell, hipMemcpyHostToDevice);
-  hipMemset(c->bins.cnt, 0, sizeof(int32_t) * ncell);
-  hipMemset(c->d_err, 0xff, sizeof(unsigned long long) * kErrWords);
-  hipMemset(c->d_counters, 0, sizeof(unsigned long long) * 2);
-  hipMemcpy(c->d_models, c->h_models, sizeof(c->h_models), hipMemcpyHostToDevice);
+  (void)hipMemcpy(c->d_row_rank, rank.data(), sizeof(int32_t) * nrows, hipMemcpyHostToDevice);
+  (void)hipMemset(c->bins.cnt, 0, sizeof(int32_t) * ncell);
+  (void)hipMemset(c->d_err, 0xff, sizeof(unsigned long long) * kErrWords);
+  (void)hipMemset(c->d_counters, 0, sizeof(unsigned long long) * 2);
+  (void)hipMemcpy(c->d_models, c->h_models, sizeof(c->h_models), hipMemcpyHostToDevice);
   if (hipDeviceSynchronize() != hipSuccess) { gdem_destroy(c); return GDEM_ERR_HIP; }
   *out = c;
   return GDEM_OK;
@@ -172,17 +174,18 @@ extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
 
 extern "C" int gdem_destroy(gdem_ctx* c) {
   if (!c) return GDEM_OK;
-  hipSetDevice(c->dev);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->dev);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_state(c->buf[0]);
   free_state(c->buf[1]);
-  hipFree(c->d_F); hipFree(c->d_T);
-  hipFree(c->bins.key); hipFree(c->bins.slot); hipFree(c->bins.perm);
-  hipFree(c->bins.cnt); hipFree(c->bins.local); hipFree(c->bins.bsum); hipFree(c->bins.cell_start);
-  hipFree(c->d_rank); hipFree(c->d_models); hipFree(c->d_walls); hipFree(c->d_err);
-  hipFree(c->d_counters); hipFree(c->d_rec); hipFree(c->d_rec_count); hipFree(c->d_stage);
-  for (auto& e : c->ev_pool) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
-  if (c->stream) hipStreamDestroy(c->stream);
+  (void)hipFree(c->d_F); (void)hipFree(c->d_T);
+  (void)hipFree(c->bins.key); (void)hipFree(c->bins.slot); (void)hipFree(c->bins.perm); (void)hipFree(c->bins.pos);
+  (void)hipFree(c->d_out_list); (void)hipFree(c->d_in_list); (void)hipFree(c->d_out_count); (void)hipFree(c->d_in_count);
+  (void)hipFree(c->bins.cnt); (void)hipFree(c->bins.local); (void)hipFree(c->bins.bsum); (void)hipFree(c->bins.cell_start);
+  (void)hipFree(c->d_row_rank); (void)hipFree(c->d_models); (void)hipFree(c->d_walls); (void)hipFree(c->d_err);
+  (void)hipFree(c->d_counters); (void)hipFree(c->d_rec); (void)hipFree(c->d_rec_count); (void)hipFree(c->d_stage);
+  for (auto& e : c->ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return GDEM_OK;
 }
@@ -197,11 +200,15 @@ extern "C" int gdem_set_particles(gdem_ctx* c, int64_t n, const double* x, const
   HIP_TRY(hipStreamSynchronize(c->stream));
   free_state(c->buf[0]);
   free_state(c->buf[1]);
-  hipFree(c->d_F); hipFree(c->d_T);
-  hipFree(c->bins.key); hipFree(c->bins.slot); hipFree(c->bins.perm);
-  hipFree(c->d_rec); hipFree(c->d_rec_count);
+  (void)hipFree(c->d_F); (void)hipFree(c->d_T);
+  (void)hipFree(c->bins.key); (void)hipFree(c->bins.slot); (void)hipFree(c->bins.perm); (void)hipFree(c->bins.pos);
+  (void)hipFree(c->d_out_list); (void)hipFree(c->d_in_list); (void)hipFree(c->d_out_count); (void)hipFree(c->d_in_count);
+  (void)hipFree(c->d_rec); (void)hipFree(c->d_rec_count);
   c->d_F = c->d_T = nullptr;
   c->bins.key = c->bins.slot = c->bins.perm = nullptr;
+  c->bins.pos = nullptr;
+  c->d_out_list = c->d_in_list = nullptr;
+  c->d_out_count = c->d_in_count = nullptr;
   c->d_rec = nullptr;
   c->d_rec_count = nullptr;
   c->n = n;
@@ -212,6 +219,12 @@ extern "C" int gdem_set_particles(gdem_ctx* c, int64_t n, const double* x, const
   HIP_TRY(hipMalloc(&c->bins.key, sizeof(int32_t) * nn));
   HIP_TRY(hipMalloc(&c->bins.slot, sizeof(int32_t) * nn));
   HIP_TRY(hipMalloc(&c->bins.perm, sizeof(int32_t) * nn));
+  HIP_TRY(hipMalloc(&c->bins.pos, sizeof(double4) * nn));
+  HIP_TRY(hipMalloc(&c->d_out_list, sizeof(int2) * nn * kListCap));
+  HIP_TRY(hipMalloc(&c->d_in_list, sizeof(int2) * nn * kListCap));
+  HIP_TRY(hipMalloc(&c->d_out_count, sizeof(int32_t) * nn));
+  HIP_TRY(hipMalloc(&c->d_in_count, sizeof(int32_t) * nn));
+  HIP_TRY(hipMemset(c->d_in_count, 0, sizeof(int32_t) * nn));
   HIP_TRY(hipMemset(c->d_F, 0, sizeof(double4) * nn));
   HIP_TRY(hipMemset(c->d_T, 0, sizeof(double4) * nn));
   std::vector<double4> pr(nn), vm(nn), wr(nn), wb(nn), qq(nn);
@@ -259,7 +272,7 @@ extern "C" int gdem_set_particles(gdem_ctx* c, int64_t n, const double* x, const
   c->primed = false;
   c->binned = false;
   c->rec_valid = false;
-  launch_keys(s, (int32_t)n, c->grid, c->d_rank, c->bins.key, c->d_err, c->stream);
+  launch_keys(s, (int32_t)n, c->grid, c->d_row_rank, c->bins.key, c->d_err, c->stream);
   HIP_TRY(hipGetLastError());
   return GDEM_OK;
 }
@@ -332,7 +345,7 @@ extern "C" int gdem_set_contact_model(gdem_ctx* c, int32_t a, int32_t b, int32_t
 }
 
 static int do_binning(gdem_ctx* c) {
-  launch_binning(c->bins, (int32_t)c->n, c->grid.ncell, c->stream);
+  launch_binning(c->bins, c->buf[c->cur], (int32_t)c->n, c->grid.ncell, c->stream);
   HIP_TRY(hipGetLastError());
   c->binned = true;
   return GDEM_OK;
@@ -352,20 +365,39 @@ static int do_force(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft
     int rc = do_binning(c);
     if (rc) return rc;
   }
+  const int nblk = (int)((c->n + 255) / 256);
+  const int nblk8 = ((nblk + 7) / 8) * 8;
+  DetectArgs d{};
+  d.pos = c->bins.pos;
+  d.in = c->buf[c->cur];
+  d.perm = c->bins.perm;
+  d.cell_start = c->bins.cell_start;
+  d.row_rank = c->d_row_rank;
+  d.grid = c->grid;
+  d.n = (int32_t)c->n;
+  d.out_list = c->d_out_list;
+  d.out_count = c->d_out_count;
+  d.in_list = c->d_in_list;
+  d.in_count = c->d_in_count;
+  d.err = c->d_err;
+  d.nblk8 = nblk8;
   ForceArgs a{};
   a.in = c->buf[c->cur];
   a.out = c->buf[1 - c->cur];
   a.perm = c->bins.perm;
-  a.cell_start = c->bins.cell_start;
-  a.rank = c->d_rank;
+  a.row_rank = c->d_row_rank;
   a.grid = c->grid;
   a.models = c->d_models;
   a.walls = c->d_walls;
   a.nwalls = c->nwalls;
   a.n = (int32_t)c->n;
-  for (int d = 0; d < 3; ++d) a.gravity[d] = c->cfg.gravity[d];
+  for (int dd = 0; dd < 3; ++dd) a.gravity[dd] = c->cfg.gravity[dd];
   a.dt_vel = dtv;
   a.dt_disp = dtd;
+  a.out_list = c->d_out_list;
+  a.out_count = c->d_out_count;
+  a.in_list = c->d_in_list;
+  a.in_count = c->d_in_count;
   a.F = write_ft ? c->d_F : nullptr;
   a.T = write_ft ? c->d_T : nullptr;
   a.key_out = c->bins.key;
@@ -375,13 +407,13 @@ static int do_force(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft
   a.rec = c->d_rec;
   a.rec_count = c->d_rec_count;
   a.rec_cap = c->rec_cap;
-  const int nblk = (int)((c->n + 255) / 256);
-  a.nblk8 = ((nblk + 7) / 8) * 8;
+  a.nblk8 = nblk8;
   std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
   if (c->timing) {
     ev = next_event_pair(c);
     if (ev) HIP_TRY(hipEventRecord(ev->first, c->stream));
   }
+  launch_detect(d, c->stream);
   launch_force(mode, record, a, c->stream);
   HIP_TRY(hipGetLastError());
   if (ev) HIP_TRY(hipEventRecord(ev->second, c->stream));
@@ -407,7 +439,7 @@ extern "C" int gdem_compute_forces(gdem_ctx* c) {
 extern "C" int gdem_move(gdem_ctx* c, double dtv, double dtd) {
   if (!c) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
-  launch_move(c->buf[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid, c->d_rank, dtv, dtd,
+  launch_move(c->buf[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid, c->d_row_rank, dtv, dtd,
               c->bins.key, c->d_err, c->stream);
   HIP_TRY(hipGetLastError());
   c->binned = false;
@@ -485,7 +517,7 @@ extern "C" int gdem_get_state(gdem_ctx* c, double* x, double* v, double* w, doub
   const size_t n = (size_t)c->n;
   const size_t need = n * 19;
   if (c->stage_len < need) {
-    hipFree(c->d_stage);
+    (void)hipFree(c->d_stage);
     c->d_stage = nullptr;
     HIP_TRY(hipMalloc(&c->d_stage, sizeof(double) * std::max<size_t>(need, 1)));
     c->stage_len = need;
@@ -516,8 +548,8 @@ extern "C" int gdem_set_contact_recording(gdem_ctx* c, int32_t cap) {
   if (!c || cap < 0 || cap > 256) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  hipFree(c->d_rec);
-  hipFree(c->d_rec_count);
+  (void)hipFree(c->d_rec);
+  (void)hipFree(c->d_rec_count);
   c->d_rec = nullptr;
   c->d_rec_count = nullptr;
   c->rec_cap = cap;

@@ -16,7 +16,7 @@ struct Aux {
 };
 
 // One copy of the particle state, structure of double4 arrays, in the order of
-// the last binning (sorted by Morton cell key, ties by previous position).
+// the last binning (sorted by cell key, ties by previous position).
 struct State {
   double4* pos_r;   // x, y, z, radius
   double4* vel_m;   // v, mass
@@ -26,6 +26,8 @@ struct State {
   Aux* aux;
 };
 
+// Cell key = row_rank[j * nk + k] * ni + i: cells of one row along x are
+// consecutive, rows follow a 2-D Morton order of (j, k).
 struct Grid {
   double origin[3], edge[3], size[3];
   int32_t nc[3];
@@ -39,16 +41,38 @@ struct ContactRec {
   double point[3], ovec[3], dist, force[3];
 };
 
+// Maximum contacts one particle can detect (outgoing) or receive (incoming)
+// per step; exceeding it raises GDEM_ERR_CAPACITY.
+constexpr int kListCap = 24;
+
 // Device error words, min-reduced with integer atomics so the reported
 // offender does not depend on scheduling.
 enum { kErrContact = 0, kErrMotion = 1, kErrSim = 2, kErrCapacity = 3, kErrWords = 4 };
 constexpr unsigned long long kNoErr = ~0ull;
 
+// A contact-list entry: partner slot and a code packing the periodic shift to
+// apply to the partner ((s+1) in 2 bits per axis) and whether this particle is
+// the first body of the pair (bit 6).
+struct DetectArgs {
+  const double4* pos;     // positions + radius in sorted order
+  State in;               // state in storage order (aux via perm)
+  const int32_t* perm;
+  const int32_t* cell_start;
+  const int32_t* row_rank;
+  Grid grid;
+  int32_t n;
+  int2* out_list;         // [n][kListCap]
+  int32_t* out_count;     // [n]
+  int2* in_list;          // [n][kListCap]
+  int32_t* in_count;      // [n], zero on entry
+  unsigned long long* err;
+  int32_t nblk8;
+};
+
 struct ForceArgs {
   State in, out;
   const int32_t* perm;
-  const int32_t* cell_start;
-  const int32_t* rank;
+  const int32_t* row_rank;
   Grid grid;
   const Model* models;
   const Wall* walls;
@@ -56,9 +80,13 @@ struct ForceArgs {
   int32_t n;
   double gravity[3];
   double dt_vel, dt_disp;
+  const int2* out_list;
+  const int32_t* out_count;
+  const int2* in_list;
+  int32_t* in_count;  // read, then cleared for the next detection
   double4* F;  // written when non-null
   double4* T;
-  int32_t* key_out;  // next keys (MODE_MOVE)
+  int32_t* key_out;
   unsigned long long* err;
   unsigned long long* contacts;  // contact counter
   ContactRec* rec;               // recording (nullable)
@@ -70,30 +98,30 @@ struct ForceArgs {
 enum { MODE_FORCE = 0, MODE_KICK = 1, MODE_MOVE = 2 };
 
 struct Bins {
-  int32_t* key;         // Morton rank of each particle's cell (storage order)
+  int32_t* key;         // cell key of each particle (storage order)
   int32_t* slot;        // arrival slot within the cell (count pass)
   int32_t* cnt;         // per-cell count (zeroed by the scan)
   int32_t* local;       // block-local exclusive scan
   int32_t* bsum;        // block sums, scanned in place
   int32_t* perm;        // sorted slot -> storage index
   int32_t* cell_start;  // ncell + 1
+  double4* pos;         // sorted copy of pos_r
   int32_t nbsum;
 };
 
 // Launchers (all asynchronous on the given stream).
-void launch_keys(const State& s, int32_t n, const Grid& g, const int32_t* rank, int32_t* key,
+void launch_keys(const State& s, int32_t n, const Grid& g, const int32_t* row_rank, int32_t* key,
                  unsigned long long* err, hipStream_t st);
-void launch_binning(Bins& b, int32_t n, int32_t ncell, hipStream_t st);
+void launch_binning(Bins& b, const State& s, int32_t n, int32_t ncell, hipStream_t st);
+void launch_detect(const DetectArgs& a, hipStream_t st);
 void launch_force(int mode, bool record, const ForceArgs& a, hipStream_t st);
 void launch_move(const State& s, const double4* F, const double4* T, int32_t n, const Grid& g,
-                 const int32_t* rank, double dt_vel, double dt_disp, int32_t* key,
+                 const int32_t* row_rank, double dt_vel, double dt_disp, int32_t* key,
                  unsigned long long* err, hipStream_t st);
 void launch_advance(const State& s, const double4* F, const double4* T, int32_t n,
                     double dt_vel, hipStream_t st);
 void launch_unpermute(const State& s, const double4* F, const double4* T, int32_t n,
                       double* x, double* v, double* w, double* q, double* f, double* t,
                       hipStream_t st);
-void launch_copy_state(const State& src, const State& dst, const int32_t* perm, int32_t n,
-                       hipStream_t st);
 
 }  // namespace gdem

@@ -1,15 +1,18 @@
 // gdem_kernels.hip -- CDNA4 (gfx950) kernels of the DEM step.
 //
-//   binning : keys -> count (int atomics) -> block scan -> scatter -> per-cell
-//             sort by previous slot (deterministic counting sort; replaces
-//             LinkedCell::LinkUpdate, Base/src/LinkedCell.cpp:1452-1547)
-//   force   : one thread per particle in cell order; 27-cell sweep; every
-//             contact is evaluated with the reference's body order (first body
-//             = the one whose half stencil holds the other, LinkedCell.cpp:
-//             958-984, 1034-1060) so both partners compute bit-identical pair
-//             forces; each thread reduces its own contact segment in a fixed
-//             order (no floating-point atomics) and its tail applies the
-//             leap-frog kicks and drift (SecondOrderLeapFrog.cpp:44-94).
+//   binning : keys -> count (wave-aggregated int atomics) -> block scan ->
+//             scatter -> per-cell sort by previous slot + sorted position copy
+//             (deterministic counting sort; replaces LinkedCell::LinkUpdate,
+//             Base/src/LinkedCell.cpp:1452-1547)
+//   detect  : one thread per particle in cell order sweeps the reference's
+//             13-cell half stencil (LinkedCell.cpp:958-984) as 5 contiguous
+//             slot ranges (cells of a row along x are consecutive) and appends
+//             each contact to both partners' lists
+//   force   : one thread per particle reduces its own contact segment in a
+//             fixed order -- every pair force is evaluated with the
+//             reference's body order so both partners get bit-identical
+//             values, no floating-point atomics -- then its tail applies the
+//             leap-frog kicks and drift (SecondOrderLeapFrog.cpp:44-94)
 //   move / advance : stand-alone integrator phases for the per-phase ABI.
 #include <hip/hip_runtime.h>
 
@@ -24,6 +27,14 @@ constexpr int kScanBlock = 1024;
 
 __device__ __forceinline__ unsigned long long pack_ids(int32_t a, int32_t b) {
   return ((unsigned long long)(uint32_t)a << 32) | (unsigned long long)(uint32_t)b;
+}
+
+__device__ __forceinline__ int pack_code(int sx, int sy, int sz, bool first) {
+  return (sx + 1) | ((sy + 1) << 2) | ((sz + 1) << 4) | (first ? 64 : 0);
+}
+
+__device__ __forceinline__ int mirror_code(int code) {
+  return pack_code(1 - (code & 3), 1 - ((code >> 2) & 3), 1 - ((code >> 4) & 3), !(code & 64));
 }
 
 // Cell::GetCell (Base/src/Cell.cpp:172-184); periodic indices are clamped to
@@ -52,9 +63,8 @@ __device__ __forceinline__ bool cell_of(const Grid& g, v3 x, int* ijk) {
   return ok;
 }
 
-// LinkedCell::getCellNumber (LinkedCell.cpp:1165-1168)
-__device__ __forceinline__ int cell_linear(const Grid& g, int i, int j, int k) {
-  return j * g.nc[2] * g.nc[0] + k * g.nc[0] + i;
+__device__ __forceinline__ int row_base(const Grid& g, const int32_t* row_rank, int j, int k) {
+  return row_rank[j * g.nc[2] + k] * g.nc[0];
 }
 
 __device__ __forceinline__ int logical_index(int nblk8) {
@@ -67,7 +77,7 @@ __device__ __forceinline__ int logical_index(int nblk8) {
 }
 
 // ---------------------------------------------------------------- keys
-__global__ void __launch_bounds__(kBlock) k_keys(State s, int n, Grid g, const int32_t* rank,
+__global__ void __launch_bounds__(kBlock) k_keys(State s, int n, Grid g, const int32_t* row_rank,
                                                  int32_t* key, unsigned long long* err) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -78,15 +88,29 @@ __global__ void __launch_bounds__(kBlock) k_keys(State s, int n, Grid g, const i
     key[i] = 0;
     return;
   }
-  key[i] = rank[cell_linear(g, ijk[0], ijk[1], ijk[2])];
+  key[i] = row_base(g, row_rank, ijk[1], ijk[2]) + ijk[0];
 }
 
 // ---------------------------------------------------------------- binning
+// Arrival slot of each particle in its cell.  Keys arrive almost sorted (the
+// storage order is the previous step's cell order), so equal keys of
+// neighbouring lanes are merged into one atomic per run.
 __global__ void __launch_bounds__(kBlock) k_count(const int32_t* key, int32_t* slot, int32_t* cnt,
                                                   int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  slot[i] = atomicAdd(&cnt[key[i]], 1);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int k = i < n ? key[i] : -1;
+  const int kprev = __shfl_up(k, 1, 64);
+  const bool head = (lane == 0) || (k != kprev);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  const int head_lane = 63 - __clzll(heads & upto);
+  const unsigned long long after = heads & ~upto;
+  const int next_head = after ? __ffsll((long long)after) - 1 : 64;
+  int base = 0;
+  if (head && k >= 0) base = atomicAdd(&cnt[k], next_head - lane);
+  base = __shfl(base, head_lane, 64);
+  if (i < n) slot[i] = base + (lane - head_lane);
 }
 
 template <int BS>
@@ -160,9 +184,11 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const int32_t* key, const in
 }
 
 // Orders each cell by previous slot, which makes the permutation independent of
-// the order in which the count pass's atomics were served.
+// the order in which the count pass's atomics were served, and writes the
+// sorted copy of the positions the detection sweep reads.
 __global__ void __launch_bounds__(kBlock) k_cellsort(const int32_t* local, const int32_t* bsum,
                                                      int32_t* perm, int32_t* cell_start,
+                                                     const double4* pos_r, double4* pos_sorted,
                                                      int ncell, int n) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
@@ -177,17 +203,189 @@ __global__ void __launch_bounds__(kBlock) k_cellsort(const int32_t* local, const
     }
     perm[b + 1] = v;
   }
+  for (int a = beg; a < end; ++a) pos_sorted[a] = pos_r[perm[a]];
   cell_start[c] = beg;
   if (c == ncell - 1) cell_start[ncell] = n;
 }
 
-// ---------------------------------------------------------------- force
-// Which of two bodies in different cells is the reference's first body: the
-// one whose 13-cell half stencil (LinkedCell.cpp:965-967) holds the other's
-// cell, i.e. a positive offset in (j, k, i) lexicographic order.
-__device__ __forceinline__ bool in_half_stencil(int di, int dj, int dk) {
-  return dj > 0 || (dj == 0 && (dk > 0 || (dk == 0 && di > 0)));
+// ---------------------------------------------------------------- detect
+struct Detector {
+  const DetectArgs& a;
+  int t;
+  v3 X;
+  double r;
+  bool have_aux;
+  Aux ax;
+  int nout;
+
+  __device__ __forceinline__ Detector(const DetectArgs& a_, int t_, v3 X_, double r_)
+      : a(a_), t(t_), X(X_), r(r_), have_aux(false), ax{}, nout(0) {}
+
+  // One candidate u with periodic shift s (applied to u).  Cross-cell
+  // candidates of the half stencil have this particle as the first body; a
+  // same-cell pair is ordered by id.
+  __device__ __forceinline__ void test(int u, int sx, int sy, int sz, bool same) {
+    const double4 p2 = a.pos[u];
+    const double shx = sx * a.grid.size[0], shy = sy * a.grid.size[1], shz = sz * a.grid.size[2];
+    const v3 Bn = mk(p2.x + shx, p2.y + shy, p2.z + shz);
+    const v3 dd = sub(Bn, X);
+    const double d2 = dot(dd, dd);
+    const double rs = r + p2.w;
+    if (d2 > rs * rs * (1. + 1e-9)) return;  // looser than the exact test
+    if (!have_aux) {
+      ax = a.in.aux[a.perm[t]];
+      have_aux = true;
+    }
+    const Aux ax2 = a.in.aux[a.perm[u]];
+    const bool first = same ? (ax.id < ax2.id) : true;
+    v3 A, B, c, ov;
+    double rA, rB, crA, crB, dist;
+    if (first) {
+      A = X; B = Bn; rA = r; rB = p2.w; crA = ax.crust; crB = ax2.crust;
+    } else {
+      A = mk(p2.x, p2.y, p2.z);
+      B = mk(X.x + (-shx), X.y + (-shy), X.z + (-shz));
+      rA = p2.w; rB = r; crA = ax2.crust; crB = ax.crust;
+    }
+    const int rc = sphere_sphere(A, rA, crA, B, rB, crB, c, ov, dist);
+    if (rc < 0) {
+      atomicMin(&a.err[kErrContact], first ? pack_ids(ax.id, ax2.id) : pack_ids(ax2.id, ax.id));
+      return;
+    }
+    if (rc == 0 || !(dist < 0.)) return;
+    const int code = pack_code(sx, sy, sz, first);
+    if (nout < kListCap) {
+      a.out_list[(size_t)t * kListCap + nout] = make_int2(u, code);
+    } else {
+      atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax.id);
+    }
+    ++nout;
+    const int kin = atomicAdd(&a.in_count[u], 1);
+    if (kin < kListCap) {
+      a.in_list[(size_t)u * kListCap + kin] = make_int2(t, mirror_code(code));
+    } else {
+      atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax2.id);
+    }
+  }
+
+  __device__ __forceinline__ void range(int beg, int end, int sx, int sy, int sz, bool same) {
+    for (int u = beg; u < end; ++u) test(u, sx, sy, sz, same);
+  }
+};
+
+__global__ void __launch_bounds__(kBlock) k_detect(DetectArgs a) {
+  const int t = logical_index(a.nblk8);
+  if (t >= a.n) return;
+  const double4 me = a.pos[t];
+  const v3 X = mk(me.x, me.y, me.z);
+  int ijk[3];
+  cell_of(a.grid, X, ijk);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) ijk[d] = min(max(ijk[d], 0), a.grid.nc[d] - 1);
+  const int i = ijk[0], j = ijk[1], k = ijk[2];
+  const int ni = a.grid.nc[0];
+  const bool px = a.grid.periodic[0] != 0;
+  Detector det(a, t, X, me.w);
+  const int32_t* cs = a.cell_start;
+  // own row: the rest of this cell (same cell, after self), then cell i+1
+  {
+    const int rb = row_base(a.grid, a.row_rank, j, k);
+    det.range(t + 1, cs[rb + i + 1], 0, 0, 0, true);
+    if (i + 1 < ni) det.range(cs[rb + i + 1], cs[rb + i + 2], 0, 0, 0, false);
+    else if (px) det.range(cs[rb], cs[rb + 1], 1, 0, 0, false);
+  }
+  // rows (j, k+1), (j+1, k-1), (j+1, k), (j+1, k+1): cells i-1 .. i+1
+  const int drow[4][2] = {{0, 1}, {1, -1}, {1, 0}, {1, 1}};
+  for (int q = 0; q < 4; ++q) {
+    int jj = j + drow[q][0], kk = k + drow[q][1];
+    int sy = 0, sz = 0;
+    if (jj >= a.grid.nc[1]) {
+      if (!a.grid.periodic[1]) continue;
+      jj -= a.grid.nc[1];
+      sy = 1;
+    }
+    if (kk < 0 || kk >= a.grid.nc[2]) {
+      if (!a.grid.periodic[2]) continue;
+      if (kk < 0) { kk += a.grid.nc[2]; sz = -1; }
+      else { kk -= a.grid.nc[2]; sz = 1; }
+    }
+    const int rb = row_base(a.grid, a.row_rank, jj, kk);
+    const int lo = max(i - 1, 0), hi = min(i + 1, ni - 1);
+    if (px && i - 1 < 0) det.range(cs[rb + ni - 1], cs[rb + ni], -1, sy, sz, false);
+    det.range(cs[rb + lo], cs[rb + hi + 1], 0, sy, sz, false);
+    if (px && i + 1 >= ni) det.range(cs[rb], cs[rb + 1], 1, sy, sz, false);
+  }
+  a.out_count[t] = det.nout;
 }
+
+// ---------------------------------------------------------------- force
+template <bool RECORD>
+struct Reducer {
+  const ForceArgs& a;
+  int t;
+  v3 X, V, W;
+  double r, m, req;
+  Aux ax;
+  v3 F, T;
+  int ncont, nrec;
+
+  // One contact of this particle's segment: partner slot p, code (shift to the
+  // partner's image, first-body flag).  The pair is evaluated with A = first
+  // body and B = second body moved to the first body's image, exactly as the
+  // partner's thread does.
+  __device__ __forceinline__ void contact(int p, int code) {
+    const int sx = (code & 3) - 1, sy = ((code >> 2) & 3) - 1, sz = ((code >> 4) & 3) - 1;
+    const bool first = (code & 64) != 0;
+    const double shx = sx * a.grid.size[0], shy = sy * a.grid.size[1], shz = sz * a.grid.size[2];
+    const int s2 = a.perm[p];
+    const double4 p2 = a.in.pos_r[s2];
+    const double4 vm2 = a.in.vel_m[s2];
+    const double4 wr2 = a.in.om_req[s2];
+    const Aux ax2 = a.in.aux[s2];
+    const v3 V2 = mk(vm2.x, vm2.y, vm2.z), W2 = mk(wr2.x, wr2.y, wr2.z);
+    v3 A, B, c, ov;
+    double rA, rB, crA, crB, dist;
+    if (first) {
+      A = X; B = mk(p2.x + shx, p2.y + shy, p2.z + shz);
+      rA = r; rB = p2.w; crA = ax.crust; crB = ax2.crust;
+    } else {
+      A = mk(p2.x, p2.y, p2.z);
+      B = mk(X.x + (-shx), X.y + (-shy), X.z + (-shz));
+      rA = p2.w; rB = r; crA = ax2.crust; crB = ax.crust;
+    }
+    sphere_sphere(A, rA, crA, B, rB, crB, c, ov, dist);
+    const Model md = a.models[(first ? ax.mat : ax2.mat) * kMaxMaterials + (first ? ax2.mat : ax.mat)];
+    v3 f;
+    if (first)
+      f = contact_force(md, c, ov, dist, A, V, W, m, req, B, V2, W2, vm2.w, wr2.w);
+    else
+      f = contact_force(md, c, ov, dist, A, V2, W2, vm2.w, wr2.w, B, V, W, m, req);
+    if (first) {
+      F = add(F, f);
+      T = add(T, cross(sub(c, A), f));
+      ++ncont;
+      if (RECORD) record(ax.id, ax2.id, c, ov, dist, f);
+    } else {
+      const v3 fn = neg(f);
+      F = add(F, fn);
+      T = add(T, cross(sub(c, B), fn));
+    }
+  }
+
+  __device__ __forceinline__ void record(int id0, int id1, v3 c, v3 ov, double dist, v3 f) {
+    if (nrec < a.rec_cap) {
+      ContactRec& cr = a.rec[(size_t)t * a.rec_cap + nrec];
+      cr.id0 = id0; cr.id1 = id1;
+      cr.point[0] = c.x; cr.point[1] = c.y; cr.point[2] = c.z;
+      cr.ovec[0] = ov.x; cr.ovec[1] = ov.y; cr.ovec[2] = ov.z;
+      cr.dist = dist;
+      cr.force[0] = f.x; cr.force[1] = f.y; cr.force[2] = f.z;
+    } else {
+      atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)id0);
+    }
+    ++nrec;
+  }
+};
 
 template <int MODE, bool RECORD>
 __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a) {
@@ -199,163 +397,82 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a) {
     const double4 pr = a.in.pos_r[s];
     const double4 vm = a.in.vel_m[s];
     const double4 wr = a.in.om_req[s];
-    const Aux ax = a.in.aux[s];
-    const v3 X = mk(pr.x, pr.y, pr.z);
-    const v3 V = mk(vm.x, vm.y, vm.z);
-    const v3 W = mk(wr.x, wr.y, wr.z);
-    const double r = pr.w, m = vm.w, req = wr.w;
-    int ijk[3];
-    cell_of(a.grid, X, ijk);
-    // a particle outside the grid was already reported (SimulationError) by
-    // the pass that computed its key; clamp so every index below stays in range
-#pragma unroll
-    for (int d = 0; d < 3; ++d) ijk[d] = min(max(ijk[d], 0), a.grid.nc[d] - 1);
+    const double4 wb = a.in.wbf_I[s];
+    const double4 qq = a.in.quat[s];
+    Reducer<RECORD> R{a, t};
+    R.ax = a.in.aux[s];
+    R.X = mk(pr.x, pr.y, pr.z);
+    R.V = mk(vm.x, vm.y, vm.z);
+    R.W = mk(wr.x, wr.y, wr.z);
+    R.r = pr.w;
+    R.m = vm.w;
+    R.req = wr.w;
     // InitializeForce: weight m (1 - 0/rho) g at the centre (Particle.cpp:414-419, 1409-1416)
-    const double mw = m * (1. - 0. / 1.);
-    v3 F = mk(a.gravity[0] * mw, a.gravity[1] * mw, a.gravity[2] * mw);
-    v3 T = mk(0., 0., 0.);
-    int nrec = 0;
-    for (int dk = -1; dk <= 1; ++dk) {
-      int kk = ijk[2] + dk;
-      double shz = 0.;
-      if (kk < 0 || kk >= a.grid.nc[2]) {
-        if (!a.grid.periodic[2]) continue;
-        if (kk < 0) { kk += a.grid.nc[2]; shz = -a.grid.size[2]; }
-        else { kk -= a.grid.nc[2]; shz = a.grid.size[2]; }
+    const double mw = R.m * (1. - 0. / 1.);
+    R.F = mk(a.gravity[0] * mw, a.gravity[1] * mw, a.gravity[2] * mw);
+    R.T = mk(0., 0., 0.);
+    R.ncont = 0;
+    R.nrec = 0;
+    // contact segment: detected contacts in sweep order, then the contacts
+    // other particles detected, by increasing partner slot
+    const int nout = min(a.out_count[t], kListCap);
+    const int nin = min(a.in_count[t], kListCap);
+    a.in_count[t] = 0;
+    const int2* ol = a.out_list + (size_t)t * kListCap;
+    for (int e = 0; e < nout; ++e) {
+      const int2 en = ol[e];
+      R.contact(en.x, en.y);
+    }
+    const int2* il = a.in_list + (size_t)t * kListCap;
+    int last = -1;
+    for (int e = 0; e < nin; ++e) {
+      int best = 0x7fffffff, code = 0;
+      for (int f = 0; f < nin; ++f) {
+        const int2 en = il[f];
+        if (en.x > last && en.x < best) { best = en.x; code = en.y; }
       }
-      for (int dj = -1; dj <= 1; ++dj) {
-        int jj = ijk[1] + dj;
-        double shy = 0.;
-        if (jj < 0 || jj >= a.grid.nc[1]) {
-          if (!a.grid.periodic[1]) continue;
-          if (jj < 0) { jj += a.grid.nc[1]; shy = -a.grid.size[1]; }
-          else { jj -= a.grid.nc[1]; shy = a.grid.size[1]; }
-        }
-        for (int di = -1; di <= 1; ++di) {
-          int ii = ijk[0] + di;
-          double shx = 0.;
-          if (ii < 0 || ii >= a.grid.nc[0]) {
-            if (!a.grid.periodic[0]) continue;
-            if (ii < 0) { ii += a.grid.nc[0]; shx = -a.grid.size[0]; }
-            else { ii -= a.grid.nc[0]; shx = a.grid.size[0]; }
-          }
-          const bool same = (di == 0 && dj == 0 && dk == 0);
-          const bool stencil_first = in_half_stencil(di, dj, dk);
-          const int rk = a.rank[cell_linear(a.grid, ii, jj, kk)];
-          const int beg = a.cell_start[rk], end = a.cell_start[rk + 1];
-          for (int u = beg; u < end; ++u) {
-            if (u == t) continue;
-            const int s2 = a.perm[u];
-            const double4 p2 = a.in.pos_r[s2];
-            // cheap reject, looser than the exact test below
-            const v3 Bn = mk(p2.x + shx, p2.y + shy, p2.z + shz);
-            const v3 dd = sub(Bn, X);
-            const double d2 = dot(dd, dd);
-            const double rs = r + p2.w;
-            if (d2 > rs * rs * (1. + 1e-9)) continue;
-            const Aux ax2 = a.in.aux[s2];
-            const bool first = same ? (ax.id < ax2.id) : stencil_first;
-            // canonical pair: A = first body, B = second body moved to the
-            // first body's periodic image
-            v3 A, B;
-            double rA, rB, crA, crB;
-            if (first) {
-              A = X; B = Bn; rA = r; rB = p2.w; crA = ax.crust; crB = ax2.crust;
-            } else {
-              A = mk(p2.x, p2.y, p2.z);
-              B = mk(X.x + (-shx), X.y + (-shy), X.z + (-shz));
-              rA = p2.w; rB = r; crA = ax2.crust; crB = ax.crust;
-            }
-            v3 c, ov;
-            double dist;
-            const int rc = sphere_sphere(A, rA, crA, B, rB, crB, c, ov, dist);
-            if (rc < 0) {
-              atomicMin(&a.err[kErrContact], first ? pack_ids(ax.id, ax2.id) : pack_ids(ax2.id, ax.id));
-              continue;
-            }
-            if (rc == 0 || !(dist < 0.)) continue;
-            const double4 vm2 = a.in.vel_m[s2];
-            const double4 wr2 = a.in.om_req[s2];
-            const v3 V2 = mk(vm2.x, vm2.y, vm2.z), W2 = mk(wr2.x, wr2.y, wr2.z);
-            const Model md = a.models[(first ? ax.mat : ax2.mat) * kMaxMaterials +
-                                      (first ? ax2.mat : ax.mat)];
-            v3 f;
-            if (first)
-              f = contact_force(md, c, ov, dist, A, V, W, m, req, B, V2, W2, vm2.w, wr2.w);
-            else
-              f = contact_force(md, c, ov, dist, A, V2, W2, vm2.w, wr2.w, B, V, W, m, req);
-            if (first) {
-              F = add(F, f);
-              T = add(T, cross(sub(c, A), f));
-              ++ncont;
-              if (RECORD) {
-                if (nrec < a.rec_cap) {
-                  ContactRec& cr = a.rec[(size_t)t * a.rec_cap + nrec];
-                  cr.id0 = ax.id; cr.id1 = ax2.id;
-                  cr.point[0] = c.x; cr.point[1] = c.y; cr.point[2] = c.z;
-                  cr.ovec[0] = ov.x; cr.ovec[1] = ov.y; cr.ovec[2] = ov.z;
-                  cr.dist = dist;
-                  cr.force[0] = f.x; cr.force[1] = f.y; cr.force[2] = f.z;
-                } else {
-                  atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax.id);
-                }
-                ++nrec;
-              }
-            } else {
-              const v3 fn = neg(f);
-              F = add(F, fn);
-              T = add(T, cross(sub(c, B), fn));
-            }
-          }
-        }
-      }
+      R.contact(best, code);
+      last = best;
     }
     // SimpleObstacle::InterAction (Component/src/SimpleObstacle.cpp:222-271)
     for (int wi = 0; wi < a.nwalls; ++wi) {
       const Wall& wl = a.walls[wi];
-      if (!(fabs(wl.bb_centre[0] - X.x) <= wl.bb_ext[0] + r &&
-            fabs(wl.bb_centre[1] - X.y) <= wl.bb_ext[1] + r &&
-            fabs(wl.bb_centre[2] - X.z) <= wl.bb_ext[2] + r))
+      if (!(fabs(wl.bb_centre[0] - R.X.x) <= wl.bb_ext[0] + R.r &&
+            fabs(wl.bb_centre[1] - R.X.y) <= wl.bb_ext[1] + R.r &&
+            fabs(wl.bb_centre[2] - R.X.z) <= wl.bb_ext[2] + R.r))
         continue;
       v3 c = mk(0., 0., 0.), ov = mk(0., 0., 0.);
       double dist;
-      const int rc = sphere_wall(wl, X, r, ax.crust, c, ov, dist);
+      const int rc = sphere_wall(wl, R.X, R.r, R.ax.crust, c, ov, dist);
       if (rc < 0) {
-        atomicMin(&a.err[kErrContact], pack_ids(ax.id, -1 - wi));
+        atomicMin(&a.err[kErrContact], pack_ids(R.ax.id, -1 - wi));
         continue;
       }
       if (rc == 0 || !(dist < 0.)) continue;
-      const Model md = a.models[ax.mat * kMaxMaterials + wl.material];
+      const Model md = a.models[R.ax.mat * kMaxMaterials + wl.material];
       const v3 zero = mk(0., 0., 0.);
-      const v3 f = contact_force(md, c, ov, dist, X, V, W, m, req,
+      const v3 f = contact_force(md, c, ov, dist, R.X, R.V, R.W, R.m, R.req,
                                  mk(wl.origin[0], wl.origin[1], wl.origin[2]), zero, zero, 1.e20,
                                  wl.req);
-      F = add(F, f);
-      T = add(T, cross(sub(c, X), f));
-      ++ncont;
-      if (RECORD) {
-        if (nrec < a.rec_cap) {
-          ContactRec& cr = a.rec[(size_t)t * a.rec_cap + nrec];
-          cr.id0 = ax.id; cr.id1 = -1 - wi;
-          cr.point[0] = c.x; cr.point[1] = c.y; cr.point[2] = c.z;
-          cr.ovec[0] = ov.x; cr.ovec[1] = ov.y; cr.ovec[2] = ov.z;
-          cr.dist = dist;
-          cr.force[0] = f.x; cr.force[1] = f.y; cr.force[2] = f.z;
-        } else {
-          atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax.id);
-        }
-        ++nrec;
-      }
+      R.F = add(R.F, f);
+      R.T = add(R.T, cross(sub(c, R.X), f));
+      ++R.ncont;
+      if (RECORD) R.record(R.ax.id, -1 - wi, c, ov, dist, f);
     }
-    if (RECORD) a.rec_count[t] = nrec;
+    if (RECORD) a.rec_count[t] = R.nrec;
+    ncont = R.ncont;
+    const v3 F = R.F, T = R.T, X = R.X, V = R.V, W = R.W;
+    const double m = R.m;
 
     // ---- tail
-    const double4 wb = a.in.wbf_I[s];
-    const double4 qq = a.in.quat[s];
     v3 Xo = X, Vo = V, Wo = W;
     v3 WBF = mk(wb.x, wb.y, wb.z);
     quat q = qmk(mk(qq.x, qq.y, qq.z), qq.w);
-    int key = a.rank[cell_linear(a.grid, ijk[0], ijk[1], ijk[2])];
+    int ijk[3];
+    cell_of(a.grid, X, ijk);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ijk[d] = min(max(ijk[d], 0), a.grid.nc[d] - 1);
+    int key = row_base(a.grid, a.row_rank, ijk[1], ijk[2]) + ijk[0];
     if (MODE != MODE_FORCE) {
       const double I = wb.w;
       const quat pc = qconj(q);
@@ -386,7 +503,8 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a) {
         }
         q = qmul(dq, q);
         const double depl = norm(dx);
-        if (depl > ax.crust) atomicMin(&a.err[kErrMotion], (unsigned long long)(uint32_t)ax.id);
+        if (depl > R.ax.crust)
+          atomicMin(&a.err[kErrMotion], (unsigned long long)(uint32_t)R.ax.id);
         double xo[3] = {Xo.x, Xo.y, Xo.z};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -398,19 +516,19 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a) {
         Xo = mk(xo[0], xo[1], xo[2]);
         int nijk[3];
         if (cell_of(a.grid, Xo, nijk)) {
-          key = a.rank[cell_linear(a.grid, nijk[0], nijk[1], nijk[2])];
+          key = row_base(a.grid, a.row_rank, nijk[1], nijk[2]) + nijk[0];
         } else {
           key = 0;
-          atomicMin(&a.err[kErrSim], (unsigned long long)(uint32_t)ax.id);
+          atomicMin(&a.err[kErrSim], (unsigned long long)(uint32_t)R.ax.id);
         }
       }
     }
-    a.out.pos_r[t] = make_double4(Xo.x, Xo.y, Xo.z, r);
+    a.out.pos_r[t] = make_double4(Xo.x, Xo.y, Xo.z, R.r);
     a.out.vel_m[t] = make_double4(Vo.x, Vo.y, Vo.z, m);
-    a.out.om_req[t] = make_double4(Wo.x, Wo.y, Wo.z, req);
+    a.out.om_req[t] = make_double4(Wo.x, Wo.y, Wo.z, R.req);
     a.out.wbf_I[t] = make_double4(WBF.x, WBF.y, WBF.z, wb.w);
     a.out.quat[t] = make_double4(q.v.x, q.v.y, q.v.z, q.w);
-    a.out.aux[t] = ax;
+    a.out.aux[t] = R.ax;
     a.key_out[t] = key;
     if (a.F) {
       a.F[t] = make_double4(F.x, F.y, F.z, 0.);
@@ -431,8 +549,8 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a) {
 
 // ---------------------------------------------------------------- integrator phases
 __global__ void __launch_bounds__(kBlock) k_move(State s, const double4* Fa, const double4* Ta,
-                                                 int n, Grid g, const int32_t* rank, double dtv,
-                                                 double dtd, int32_t* key,
+                                                 int n, Grid g, const int32_t* row_rank,
+                                                 double dtv, double dtd, int32_t* key,
                                                  unsigned long long* err) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -472,7 +590,7 @@ __global__ void __launch_bounds__(kBlock) k_move(State s, const double4* Fa, con
   X = mk(xo[0], xo[1], xo[2]);
   int ijk[3];
   if (cell_of(g, X, ijk)) {
-    key[i] = rank[cell_linear(g, ijk[0], ijk[1], ijk[2])];
+    key[i] = row_base(g, row_rank, ijk[1], ijk[2]) + ijk[0];
   } else {
     key[i] = 0;
     atomicMin(&err[kErrSim], (unsigned long long)(uint32_t)ax.id);
@@ -522,29 +640,17 @@ __global__ void __launch_bounds__(kBlock) k_unpermute(State s, const double4* Fa
   if (t) { double4 a = Ta[i]; t[3 * id] = a.x; t[3 * id + 1] = a.y; t[3 * id + 2] = a.z; }
 }
 
-__global__ void __launch_bounds__(kBlock) k_copy_state(State src, State dst, const int32_t* perm,
-                                                       int n) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  int s = perm[t];
-  dst.pos_r[t] = src.pos_r[s];
-  dst.vel_m[t] = src.vel_m[s];
-  dst.om_req[t] = src.om_req[s];
-  dst.wbf_I[t] = src.wbf_I[s];
-  dst.quat[t] = src.quat[s];
-  dst.aux[t] = src.aux[s];
-}
-
 // ---------------------------------------------------------------- launchers
 static inline int nblocks(int n, int bs) { return (n + bs - 1) / bs; }
 
-void launch_keys(const State& s, int32_t n, const Grid& g, const int32_t* rank, int32_t* key,
+void launch_keys(const State& s, int32_t n, const Grid& g, const int32_t* row_rank, int32_t* key,
                  unsigned long long* err, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_keys, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, s, n, g, rank, key, err);
+  hipLaunchKernelGGL(k_keys, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, s, n, g, row_rank, key,
+                     err);
 }
 
-void launch_binning(Bins& b, int32_t n, int32_t ncell, hipStream_t st) {
+void launch_binning(Bins& b, const State& s, int32_t n, int32_t ncell, hipStream_t st) {
   if (n > 0)
     hipLaunchKernelGGL(k_count, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, b.key, b.slot, b.cnt, n);
   const int nb = nblocks(ncell, kScanBlock);
@@ -554,7 +660,12 @@ void launch_binning(Bins& b, int32_t n, int32_t ncell, hipStream_t st) {
     hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, b.key, b.slot,
                        b.local, b.bsum, b.perm, n);
   hipLaunchKernelGGL(k_cellsort, dim3(nblocks(ncell, kBlock)), dim3(kBlock), 0, st, b.local,
-                     b.bsum, b.perm, b.cell_start, ncell, n);
+                     b.bsum, b.perm, b.cell_start, (const double4*)s.pos_r, b.pos, ncell, n);
+}
+
+void launch_detect(const DetectArgs& a, hipStream_t st) {
+  if (a.n <= 0) return;
+  hipLaunchKernelGGL(k_detect, dim3(a.nblk8), dim3(kBlock), 0, st, a);
 }
 
 void launch_force(int mode, bool record, const ForceArgs& a, hipStream_t st) {
@@ -574,11 +685,11 @@ void launch_force(int mode, bool record, const ForceArgs& a, hipStream_t st) {
 }
 
 void launch_move(const State& s, const double4* F, const double4* T, int32_t n, const Grid& g,
-                 const int32_t* rank, double dt_vel, double dt_disp, int32_t* key,
+                 const int32_t* row_rank, double dt_vel, double dt_disp, int32_t* key,
                  unsigned long long* err, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_move, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, s, F, T, n, g, rank,
-                     dt_vel, dt_disp, key, err);
+  hipLaunchKernelGGL(k_move, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, s, F, T, n, g,
+                     row_rank, dt_vel, dt_disp, key, err);
 }
 
 void launch_advance(const State& s, const double4* F, const double4* T, int32_t n,
@@ -592,12 +703,6 @@ void launch_unpermute(const State& s, const double4* F, const double4* T, int32_
   if (n <= 0) return;
   hipLaunchKernelGGL(k_unpermute, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, s, F, T, n, x,
                      v, w, q, f, t);
-}
-
-void launch_copy_state(const State& src, const State& dst, const int32_t* perm, int32_t n,
-                       hipStream_t st) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_copy_state, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, src, dst, perm, n);
 }
 
 }  // namespace gdem
