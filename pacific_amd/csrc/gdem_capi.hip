@@ -27,6 +27,7 @@ struct gdem_ctx {
   hipStream_t stream = nullptr;
   Grid grid{};
   int32_t* d_row_rank = nullptr;
+  int32_t* d_row_jk = nullptr;
   int64_t n = 0;
   State buf[2]{};
   int cur = 0;
@@ -150,7 +151,10 @@ extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
   const int64_t nrows = (int64_t)c->grid.nc[1] * c->grid.nc[2];
   const int nb = (int)((ncell + 1023) / 1024);
   c->bins.nbsum = nb;
+  std::vector<int32_t> row_jk((size_t)nrows);
+  for (int64_t q = 0; q < nrows; ++q) row_jk[(size_t)rank[(size_t)q]] = (int32_t)q;
   if (hipMalloc(&c->d_row_rank, sizeof(int32_t) * nrows) != hipSuccess ||
+      hipMalloc(&c->d_row_jk, sizeof(int32_t) * nrows) != hipSuccess ||
       hipMalloc(&c->bins.cnt, sizeof(int32_t) * ncell) != hipSuccess ||
       hipMalloc(&c->bins.local, sizeof(int32_t) * ncell) != hipSuccess ||
       hipMalloc(&c->bins.bsum, sizeof(int32_t) * nb) != hipSuccess ||
@@ -163,6 +167,7 @@ extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
     return GDEM_ERR_HIP;
   }
   (void)hipMemcpy(c->d_row_rank, rank.data(), sizeof(int32_t) * nrows, hipMemcpyHostToDevice);
+  (void)hipMemcpy(c->d_row_jk, row_jk.data(), sizeof(int32_t) * nrows, hipMemcpyHostToDevice);
   (void)hipMemset(c->bins.cnt, 0, sizeof(int32_t) * ncell);
   (void)hipMemset(c->d_err, 0xff, sizeof(unsigned long long) * kErrWords);
   (void)hipMemset(c->d_counters, 0, sizeof(unsigned long long) * 2);
@@ -182,7 +187,7 @@ extern "C" int gdem_destroy(gdem_ctx* c) {
   (void)hipFree(c->bins.key); (void)hipFree(c->bins.slot); (void)hipFree(c->bins.perm); (void)hipFree(c->bins.pos);
   (void)hipFree(c->d_out_list); (void)hipFree(c->d_in_list); (void)hipFree(c->d_out_count); (void)hipFree(c->d_in_count);
   (void)hipFree(c->bins.cnt); (void)hipFree(c->bins.local); (void)hipFree(c->bins.bsum); (void)hipFree(c->bins.cell_start);
-  (void)hipFree(c->d_row_rank); (void)hipFree(c->d_models); (void)hipFree(c->d_walls); (void)hipFree(c->d_err);
+  (void)hipFree(c->d_row_rank); (void)hipFree(c->d_row_jk); (void)hipFree(c->d_models); (void)hipFree(c->d_walls); (void)hipFree(c->d_err);
   (void)hipFree(c->d_counters); (void)hipFree(c->d_rec); (void)hipFree(c->d_rec_count); (void)hipFree(c->d_stage);
   for (auto& e : c->ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -219,7 +224,7 @@ extern "C" int gdem_set_particles(gdem_ctx* c, int64_t n, const double* x, const
   HIP_TRY(hipMalloc(&c->bins.key, sizeof(int32_t) * nn));
   HIP_TRY(hipMalloc(&c->bins.slot, sizeof(int32_t) * nn));
   HIP_TRY(hipMalloc(&c->bins.perm, sizeof(int32_t) * nn));
-  HIP_TRY(hipMalloc(&c->bins.pos, sizeof(double4) * nn));
+  HIP_TRY(hipMalloc(&c->bins.pos, sizeof(float4) * nn));
   HIP_TRY(hipMalloc(&c->d_out_list, sizeof(int2) * nn * kListCap));
   HIP_TRY(hipMalloc(&c->d_in_list, sizeof(int2) * nn * kListCap));
   HIP_TRY(hipMalloc(&c->d_out_count, sizeof(int32_t) * nn));
@@ -369,6 +374,19 @@ static int do_force(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft
   const int nblk8 = ((nblk + 7) / 8) * 8;
   DetectArgs d{};
   d.pos = c->bins.pos;
+  // Pre-filter margin: coordinates lie within [lo, lo + L] (a particle outside
+  // a non-periodic grid is a SimulationError), a shifted image within L more.
+  // Rounding each to float moves it by at most 2^-24 of its magnitude; 2^-19 of
+  // (max |coordinate| + max L) bounds the error of the float distance test by
+  // a wide factor.
+  double mag = 0., lmax = 0.;
+  for (int dd = 0; dd < 3; ++dd) {
+    mag = std::max(mag, std::max(std::fabs(c->grid.origin[dd]),
+                                 std::fabs(c->grid.origin[dd] + c->grid.size[dd])));
+    lmax = std::max(lmax, c->grid.size[dd]);
+    d.sizef[dd] = (float)c->grid.size[dd];
+  }
+  d.margin = (float)(std::ldexp(1.0, -19) * (mag + lmax));
   d.in = c->buf[c->cur];
   d.perm = c->bins.perm;
   d.cell_start = c->bins.cell_start;
@@ -408,12 +426,12 @@ static int do_force(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft
   a.rec_count = c->d_rec_count;
   a.rec_cap = c->rec_cap;
   a.nblk8 = nblk8;
+  launch_detect(d, c->stream);
   std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
   if (c->timing) {
     ev = next_event_pair(c);
     if (ev) HIP_TRY(hipEventRecord(ev->first, c->stream));
   }
-  launch_detect(d, c->stream);
   launch_force(mode, record, a, c->stream);
   HIP_TRY(hipGetLastError());
   if (ev) HIP_TRY(hipEventRecord(ev->second, c->stream));

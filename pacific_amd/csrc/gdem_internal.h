@@ -54,11 +54,13 @@ constexpr unsigned long long kNoErr = ~0ull;
 // apply to the partner ((s+1) in 2 bits per axis) and whether this particle is
 // the first body of the pair (bit 6).
 struct DetectArgs {
-  const double4* pos;     // positions + radius in sorted order
+  const float4* pos;      // single-precision positions + radius, sorted order
+  float sizef[3];         // box size in single precision (periodic shifts)
+  float margin;           // pre-filter margin, bounds the float rounding
   State in;               // state in storage order (aux via perm)
   const int32_t* perm;
   const int32_t* cell_start;
-  const int32_t* row_rank;
+  const int32_t* row_rank;  // (j * nk + k) -> rank
   Grid grid;
   int32_t n;
   int2* out_list;         // [n][kListCap]
@@ -66,7 +68,7 @@ struct DetectArgs {
   int2* in_list;          // [n][kListCap]
   int32_t* in_count;      // [n], zero on entry
   unsigned long long* err;
-  int32_t nblk8;
+  int32_t nblk8;  // blocks padded to a multiple of 8 (XCD-aware mapping)
 };
 
 struct ForceArgs {
@@ -105,7 +107,7 @@ struct Bins {
   int32_t* bsum;        // block sums, scanned in place
   int32_t* perm;        // sorted slot -> storage index
   int32_t* cell_start;  // ncell + 1
-  double4* pos;         // sorted copy of pos_r
+  float4* pos;          // sorted single-precision copy of pos_r
   int32_t nbsum;
 };
 

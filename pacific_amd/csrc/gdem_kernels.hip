@@ -1,7 +1,7 @@
 // gdem_kernels.hip -- CDNA4 (gfx950) kernels of the DEM step.
 //
 //   binning : keys -> count (wave-aggregated int atomics) -> block scan ->
-//             scatter -> per-cell sort by previous slot + sorted position copy
+//             scatter -> per-cell sort by previous slot -> sorted position copy
 //             (deterministic counting sort; replaces LinkedCell::LinkUpdate,
 //             Base/src/LinkedCell.cpp:1452-1547)
 //   detect  : one thread per particle in cell order sweeps the reference's
@@ -184,11 +184,9 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const int32_t* key, const in
 }
 
 // Orders each cell by previous slot, which makes the permutation independent of
-// the order in which the count pass's atomics were served, and writes the
-// sorted copy of the positions the detection sweep reads.
+// the order in which the count pass's atomics were served.
 __global__ void __launch_bounds__(kBlock) k_cellsort(const int32_t* local, const int32_t* bsum,
                                                      int32_t* perm, int32_t* cell_start,
-                                                     const double4* pos_r, double4* pos_sorted,
                                                      int ncell, int n) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
@@ -203,40 +201,59 @@ __global__ void __launch_bounds__(kBlock) k_cellsort(const int32_t* local, const
     }
     perm[b + 1] = v;
   }
-  for (int a = beg; a < end; ++a) pos_sorted[a] = pos_r[perm[a]];
   cell_start[c] = beg;
   if (c == ncell - 1) cell_start[ncell] = n;
+}
+
+// Sorted single-precision copy of position + radius, read by the detection
+// sweep's conservative pre-filter only (every contact decision is made in
+// FP64 on the full-precision state).
+__global__ void __launch_bounds__(kBlock) k_gather_pos(const int32_t* perm, const double4* pos_r,
+                                                       float4* pos_sorted, int n) {
+  int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  const double4 p = pos_r[perm[u]];
+  pos_sorted[u] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
 }
 
 // ---------------------------------------------------------------- detect
 struct Detector {
   const DetectArgs& a;
   int t;
+  float4 mef;
+  bool have_me;
   v3 X;
   double r;
-  bool have_aux;
   Aux ax;
   int nout;
 
-  __device__ __forceinline__ Detector(const DetectArgs& a_, int t_, v3 X_, double r_)
-      : a(a_), t(t_), X(X_), r(r_), have_aux(false), ax{}, nout(0) {}
+  __device__ __forceinline__ Detector(const DetectArgs& a_, int t_, float4 mef_)
+      : a(a_), t(t_), mef(mef_), have_me(false), X{}, r(0.), ax{}, nout(0) {}
 
   // One candidate u with periodic shift s (applied to u).  Cross-cell
   // candidates of the half stencil have this particle as the first body; a
   // same-cell pair is ordered by id.
-  __device__ __forceinline__ void test(int u, int sx, int sy, int sz, bool same) {
-    const double4 p2 = a.pos[u];
+  __device__ __forceinline__ void test(const float4 p2f, int u, int sx, int sy, int sz, bool same) {
+    // single-precision pre-filter: rejects only pairs whose gap exceeds the
+    // margin, which bounds the rounding of the float copy
+    const float dxf = (p2f.x + sx * a.sizef[0]) - mef.x;
+    const float dyf = (p2f.y + sy * a.sizef[1]) - mef.y;
+    const float dzf = (p2f.z + sz * a.sizef[2]) - mef.z;
+    const float rsf = mef.w + p2f.w + a.margin;
+    if (dxf * dxf + dyf * dyf + dzf * dzf > rsf * rsf) return;
+    if (!have_me) {
+      const int s = a.perm[t];
+      const double4 me = a.in.pos_r[s];
+      X = mk(me.x, me.y, me.z);
+      r = me.w;
+      ax = a.in.aux[s];
+      have_me = true;
+    }
+    const int s2 = a.perm[u];
+    const double4 p2 = a.in.pos_r[s2];
+    const Aux ax2 = a.in.aux[s2];
     const double shx = sx * a.grid.size[0], shy = sy * a.grid.size[1], shz = sz * a.grid.size[2];
     const v3 Bn = mk(p2.x + shx, p2.y + shy, p2.z + shz);
-    const v3 dd = sub(Bn, X);
-    const double d2 = dot(dd, dd);
-    const double rs = r + p2.w;
-    if (d2 > rs * rs * (1. + 1e-9)) return;  // looser than the exact test
-    if (!have_aux) {
-      ax = a.in.aux[a.perm[t]];
-      have_aux = true;
-    }
-    const Aux ax2 = a.in.aux[a.perm[u]];
     const bool first = same ? (ax.id < ax2.id) : true;
     v3 A, B, c, ov;
     double rA, rB, crA, crB, dist;
@@ -269,23 +286,30 @@ struct Detector {
   }
 
   __device__ __forceinline__ void range(int beg, int end, int sx, int sy, int sz, bool same) {
-    for (int u = beg; u < end; ++u) test(u, sx, sy, sz, same);
+    int u = beg;
+    for (; u + 1 < end; u += 2) {
+      // issue both loads before either test
+      const float4 p0 = a.pos[u], p1 = a.pos[u + 1];
+      test(p0, u, sx, sy, sz, same);
+      test(p1, u + 1, sx, sy, sz, same);
+    }
+    if (u < end) test(a.pos[u], u, sx, sy, sz, same);
   }
 };
 
 __global__ void __launch_bounds__(kBlock) k_detect(DetectArgs a) {
   const int t = logical_index(a.nblk8);
   if (t >= a.n) return;
-  const double4 me = a.pos[t];
-  const v3 X = mk(me.x, me.y, me.z);
+  // the cell index comes from the full-precision position, as in the binning
+  const double4 me = a.in.pos_r[a.perm[t]];
   int ijk[3];
-  cell_of(a.grid, X, ijk);
+  cell_of(a.grid, mk(me.x, me.y, me.z), ijk);
 #pragma unroll
   for (int d = 0; d < 3; ++d) ijk[d] = min(max(ijk[d], 0), a.grid.nc[d] - 1);
   const int i = ijk[0], j = ijk[1], k = ijk[2];
   const int ni = a.grid.nc[0];
   const bool px = a.grid.periodic[0] != 0;
-  Detector det(a, t, X, me.w);
+  Detector det(a, t, a.pos[t]);
   const int32_t* cs = a.cell_start;
   // own row: the rest of this cell (same cell, after self), then cell i+1
   {
@@ -660,7 +684,10 @@ void launch_binning(Bins& b, const State& s, int32_t n, int32_t ncell, hipStream
     hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, b.key, b.slot,
                        b.local, b.bsum, b.perm, n);
   hipLaunchKernelGGL(k_cellsort, dim3(nblocks(ncell, kBlock)), dim3(kBlock), 0, st, b.local,
-                     b.bsum, b.perm, b.cell_start, (const double4*)s.pos_r, b.pos, ncell, n);
+                     b.bsum, b.perm, b.cell_start, ncell, n);
+  if (n > 0)
+    hipLaunchKernelGGL(k_gather_pos, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, b.perm,
+                       (const double4*)s.pos_r, b.pos, n);
 }
 
 void launch_detect(const DetectArgs& a, hipStream_t st) {

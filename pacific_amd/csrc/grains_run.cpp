@@ -1,0 +1,136 @@
+// grains_run.cpp -- command-line driver of the DEM step through the C++ host
+// classes (the role of src/Grains3D/Main/src/main.cpp, with a binary input in
+// place of the XML file).
+//
+//   grains_run <input.bin> <output.bin>
+//
+// Input (little endian): "GDEMIN01", int32 n, nwalls, nmodels, nsteps,
+// double dt, origin[3], size[3], int32 periodic[3], double cell_size,
+// gravity[3]; per model int32 kind, mat_a, mat_b, double p[5]; per wall
+// double basis[9], centre[3], half_extent[3], crust, int32 material; then
+// x[3n], v[3n], w[3n], q[4n], radius[n], density[n], crust[n], int32 material[n].
+// Output: "GDEMOUT1", int32 n, status (0 or the GDEM_ERR_* the step raised),
+// id0, id1, then x, v, w, q, F, T of the final state.
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+#include <vector>
+
+#include "../../include/gdem_grains.hpp"
+
+using namespace gdem::grains;
+
+template <class T>
+static bool rd(FILE* f, T* p, size_t n) {
+  return fread(p, sizeof(T), n, f) == n;
+}
+
+int main(int argc, char** argv) {
+  if (argc < 3) {
+    fprintf(stderr, "usage: grains_run <input.bin> <output.bin>\n");
+    return 2;
+  }
+  FILE* f = fopen(argv[1], "rb");
+  if (!f) return 2;
+  char magic[8];
+  int32_t n, nwalls, nmodels, nsteps, periodic[3];
+  double dt, origin[3], size[3], cell, grav[3];
+  if (!rd(f, magic, 8) || memcmp(magic, "GDEMIN01", 8) || !rd(f, &n, 1) || !rd(f, &nwalls, 1) ||
+      !rd(f, &nmodels, 1) || !rd(f, &nsteps, 1) || !rd(f, &dt, 1) || !rd(f, origin, 3) ||
+      !rd(f, size, 3) || !rd(f, periodic, 3) || !rd(f, &cell, 1) || !rd(f, grav, 3))
+    return 2;
+  for (int d = 0; d < 3; ++d) GrainsExec::m_vgravity[d] = grav[d];
+  ContactBuilderFactory::reset();
+  for (int m = 0; m < nmodels; ++m) {
+    int32_t kind, ma, mb;
+    double p[5];
+    if (!rd(f, &kind, 1) || !rd(f, &ma, 1) || !rd(f, &mb, 1) || !rd(f, p, 5)) return 2;
+    std::map<std::string, double> par;
+    std::unique_ptr<ContactForceModel> model;
+    if (kind == GDEM_MODEL_HOOKE) {
+      par = {{"kn", p[0]}, {"en", p[1]}, {"etat", p[2]}, {"muc", p[3]}, {"kr", p[4]}};
+      model.reset(new HookeContactForceModel(par));
+    } else {
+      par = {{"Es", p[0]}, {"en", p[1]}, {"Gs", p[2]}, {"muc", p[3]}, {"kr", p[4]}};
+      model.reset(new HertzContactForceModel(par));
+    }
+    // materials are named by index so that indices match the input file
+    for (int q = 0; q <= std::max(ma, mb); ++q)
+      ContactBuilderFactory::defineMaterial("mat" + std::to_string(q), false);
+    ContactBuilderFactory::define("mat" + std::to_string(ma), "mat" + std::to_string(mb),
+                                  std::move(model));
+  }
+  std::vector<BoxObstacle> walls((size_t)nwalls);
+  for (auto& w : walls) {
+    double half[3];
+    int32_t mat;
+    if (!rd(f, w.basis, 9) || !rd(f, w.centre, 3) || !rd(f, half, 3) || !rd(f, &w.crust, 1) ||
+        !rd(f, &mat, 1))
+      return 2;
+    for (int d = 0; d < 3; ++d) w.edges[d] = 2. * half[d];
+    w.material = "mat" + std::to_string(mat);
+    w.name = "wall";
+  }
+  std::vector<double> x(3 * n), v(3 * n), w(3 * n), q(4 * n), r(n), rho(n), cr(n);
+  std::vector<int32_t> mat(n);
+  if (!rd(f, x.data(), x.size()) || !rd(f, v.data(), v.size()) || !rd(f, w.data(), w.size()) ||
+      !rd(f, q.data(), q.size()) || !rd(f, r.data(), r.size()) || !rd(f, rho.data(), rho.size()) ||
+      !rd(f, cr.data(), cr.size()) || !rd(f, mat.data(), mat.size()))
+    return 2;
+  fclose(f);
+
+  std::vector<SphereParticle> ps((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    for (int d = 0; d < 3; ++d) {
+      ps[i].position[d] = x[3 * i + d];
+      ps[i].velocity[d] = v[3 * i + d];
+      ps[i].angular_velocity[d] = w[3 * i + d];
+    }
+    for (int d = 0; d < 4; ++d) ps[i].quaternion[d] = q[4 * i + d];
+    ps[i].radius = r[i];
+    ps[i].density = rho[i];
+    ps[i].crust = cr[i];
+    ps[i].material = "mat" + std::to_string(mat[i]);
+  }
+  const bool per[3] = {periodic[0] != 0, periodic[1] != 0, periodic[2] != 0};
+  LinkedCell lc(origin, size, per);
+  if (lc.set(cell)) {
+    fprintf(stderr, "LinkedCell::set failed\n");
+    return 3;
+  }
+  for (auto const& wl : walls) lc.Link(wl);
+  lc.Link(ps);
+  std::unique_ptr<TimeIntegrator> ti(TimeIntegratorBuilderFactory::create());
+  Grains app(lc, *ti);
+  int32_t status = 0, id0 = -1, id1 = -1;
+  try {
+    app.do_before_time_stepping();
+    app.Simulation(nsteps * dt, dt);
+  } catch (ContactError const& e) {
+    e.Message(std::cout);
+    status = GDEM_ERR_CONTACT;
+    id0 = e.id0;
+    id1 = e.id1;
+  } catch (MotionError const& e) {
+    e.Message(std::cout);
+    status = GDEM_ERR_MOTION;
+    id0 = e.id;
+  } catch (SimulationError const& e) {
+    e.Message(std::cout);
+    status = GDEM_ERR_SIMULATION;
+    id0 = e.id;
+  }
+  std::vector<double> X, V, W, Q, F, T;
+  app.getState(X, V, W, Q, F, T);
+  FILE* o = fopen(argv[2], "wb");
+  if (!o) return 2;
+  fwrite("GDEMOUT1", 1, 8, o);
+  fwrite(&n, sizeof(n), 1, o);
+  fwrite(&status, sizeof(status), 1, o);
+  fwrite(&id0, sizeof(id0), 1, o);
+  fwrite(&id1, sizeof(id1), 1, o);
+  for (auto* a : {&X, &V, &W, &Q, &F, &T}) fwrite(a->data(), sizeof(double), a->size(), o);
+  fclose(o);
+  printf("grains_run: %d particles, %d steps, status %d\n", n, nsteps, status);
+  return 0;
+}

@@ -1,0 +1,348 @@
+// grains_shim.cpp -- host classes of include/gdem_grains.hpp over the C ABI.
+#include "../../include/gdem_grains.hpp"
+
+#include <cmath>
+#include <sstream>
+
+namespace gdem {
+namespace grains {
+
+namespace {
+const double kPi = 3.14159265358979323846;
+
+std::string describe(const char* what, int a, int b, double t) {
+  std::ostringstream o;
+  o << what << " between components " << a << " and " << b << " at t=" << t;
+  return o.str();
+}
+
+struct Registry {
+  std::map<std::string, int> materials;  // material -> index (bit number)
+  std::map<std::pair<int, int>, std::unique_ptr<ContactForceModel>> models;
+};
+
+Registry& registry() {
+  static Registry r;
+  return r;
+}
+}  // namespace
+
+ContactError::ContactError(int a, int b, double t)
+    : std::runtime_error(describe("ContactError: overlap exceeds the crusts", a, b, t)),
+      id0(a), id1(b), time(t) {}
+void ContactError::Message(std::ostream& f) const { f << what() << std::endl; }
+
+MotionError::MotionError(int a, double t)
+    : std::runtime_error(describe("MotionError: displacement exceeds the crust", a, a, t)),
+      id(a), time(t) {}
+void MotionError::Message(std::ostream& f) const { f << what() << std::endl; }
+
+SimulationError::SimulationError(int a, double t)
+    : std::runtime_error(describe("SimulationError: particle outside the linked cell", a, a, t)),
+      id(a), time(t) {}
+void SimulationError::Message(std::ostream& f) const { f << what() << std::endl; }
+
+std::array<double, 3> GrainsExec::m_vgravity = {0., 0., 0.};
+std::string GrainsExec::m_TIScheme = "SecondOrderLeapFrog";
+
+// ------------------------------------------------------------ contact models
+ContactForceModel::ContactForceModel(std::map<std::string, double>&) {
+  for (double& p : m_p) p = 0.;
+}
+ContactForceModel::~ContactForceModel() = default;
+
+static double need(std::map<std::string, double>& p, const char* key) {
+  auto it = p.find(key);
+  if (it == p.end()) throw std::invalid_argument(std::string("missing contact parameter ") + key);
+  return it->second;
+}
+
+HookeContactForceModel::HookeContactForceModel(std::map<std::string, double>& p)
+    : ContactForceModel(p) {
+  m_p[0] = need(p, "kn");
+  m_p[1] = need(p, "en");
+  m_p[2] = p.count("etat") ? p["etat"] : -1.;
+  m_p[3] = need(p, "muc");
+  m_p[4] = p.count("kr") ? p["kr"] : 0.;
+}
+
+// HookeContactForceModel::computeAndWriteEstimates (HookeContactForceModel.cpp:208-230)
+double HookeContactForceModel::contactTime(double m0, double m1) const {
+  const double avmass = 1. / (1. / m0 + 1. / m1);
+  const double beta = log(m_p[1]) / sqrt(kPi * kPi + log(m_p[1]) * log(m_p[1]));
+  const double etan = -sqrt(m_p[0] / avmass) * beta;
+  const double omega0 = sqrt(m_p[0] / avmass);
+  const double theta = sqrt(pow(omega0, 2.) - pow(etan, 2.));
+  return kPi / theta;
+}
+
+// Maximum overlap of the damped oscillator: the first zero of the velocity,
+// solved by Newton from tc / 2 as computeDeltaMax does (:238-256).
+double HookeContactForceModel::maxOverlap(double m0, double m1, double v0) const {
+  const double avmass = 1. / (1. / m0 + 1. / m1);
+  const double beta = log(m_p[1]) / sqrt(kPi * kPi + log(m_p[1]) * log(m_p[1]));
+  const double eta = -sqrt(m_p[0] / avmass) * beta;
+  const double omega0 = sqrt(m_p[0] / avmass);
+  const double theta = sqrt(omega0 * omega0 - eta * eta);
+  const double tc = kPi / theta;
+  double f = 1., t0 = tc / (m_p[1] > 0.2 ? 2. : 100.);
+  for (int it = 0; it < 200 && fabs(f) > 1e-10; ++it) {
+    f = (v0 / theta) * exp(-eta * t0) * (-eta * sin(theta * t0) + theta * cos(theta * t0));
+    const double df = (v0 / theta) * exp(-eta * t0) *
+                      (eta * eta * sin(theta * t0) - 2. * eta * theta * cos(theta * t0) -
+                       theta * theta * sin(theta * t0));
+    t0 -= f / df;
+  }
+  return (v0 / theta) * exp(-eta * t0) * sin(theta * t0);
+}
+
+void HookeContactForceModel::computeAndWriteEstimates(double m0, double m1, double crust0,
+                                                      double crust1, double, double, double v0,
+                                                      double, std::ostream& OUT) const {
+  const double tc = contactTime(m0, m1);
+  const double dmax = maxOverlap(m0, m1, v0);
+  OUT << "  Maximum overlap allowed by crusts = " << crust0 + crust1 << "\n";
+  OUT << "  Collisional relative velocity = " << v0 << "\n";
+  OUT << "  Tc = " << tc << "  delta_max = " << dmax << "\n";
+  if (dmax > crust0 + crust1) OUT << "  WARNING: delta_max > maximum overlap allowed by crusts\n";
+}
+
+HertzContactForceModel::HertzContactForceModel(std::map<std::string, double>& p)
+    : ContactForceModel(p) {
+  m_p[0] = need(p, "Es");
+  m_p[1] = need(p, "en");
+  m_p[2] = p.count("Gs") ? p["Gs"] : m_p[0] / 4.;
+  m_p[3] = need(p, "muc");
+  m_p[4] = p.count("kr") ? p["kr"] : 0.;
+}
+
+// HertzContactForceModel::computeAndWriteEstimates (HertzContactForceModel.cpp:209-260):
+// RK2 integration of the normal motion until the overlap closes.
+void HertzContactForceModel::computeAndWriteEstimates(double m0, double m1, double crust0,
+                                                      double crust1, double req0, double req1,
+                                                      double v0, double dt,
+                                                      std::ostream& OUT) const {
+  const double Req = 1. / (1. / req0 + 1. / req1);
+  const double avmass = 1. / (1. / m0 + 1. / m1);
+  const double beta = log(m_p[1]) / sqrt(kPi * kPi + log(m_p[1]) * log(m_p[1]));
+  const double m2sqrt56 = -2. * sqrt(5. / 6.);
+  auto dvdt = [&](double dn, double v) {
+    if (dn <= 0.) return 0.;
+    const double sq = sqrt(Req * dn);
+    const double kn = (4. / 3.) * m_p[0] * sq;
+    const double gn = m2sqrt56 * beta * sqrt(avmass * 2. * m_p[0] * sq);
+    return -(kn * dn + gn * v) / avmass;
+  };
+  double t = 0., dn = 0., v = v0, dmax = -1e20;
+  while (dn >= 0. && t < 1.) {
+    const double k1d = v * dt, k1v = dvdt(dn, v) * dt;
+    const double k2d = (v + k1v / 2.) * dt, k2v = dvdt(dn + k1d / 2., v + k1v / 2.) * dt;
+    dn += k2d;
+    v += k2v;
+    t += dt;
+    dmax = std::max(dn, dmax);
+  }
+  OUT << "  Maximum overlap allowed by crusts = " << crust0 + crust1 << "\n";
+  OUT << "  Collisional relative velocity = " << v0 << "\n";
+  OUT << "  Tc = " << t << "  delta_max = " << dmax << "\n";
+  if (dmax > crust0 + crust1) OUT << "  WARNING: delta_max > maximum overlap allowed by crusts\n";
+}
+
+// ------------------------------------------------------------ factory
+void ContactBuilderFactory::defineMaterial(std::string const& material, bool) {
+  Registry& r = registry();
+  if (r.materials.count(material)) return;
+  const int idx = (int)r.materials.size();
+  if (idx >= GDEM_MAX_MATERIALS) throw std::length_error("too many materials");
+  r.materials[material] = idx;
+}
+
+void ContactBuilderFactory::define(std::string const& a, std::string const& b,
+                                   std::unique_ptr<ContactForceModel> model) {
+  defineMaterial(a, false);
+  defineMaterial(b, false);
+  int ia = materialIndex(a), ib = materialIndex(b);
+  registry().models[{std::min(ia, ib), std::max(ia, ib)}] = std::move(model);
+}
+
+ContactForceModel* ContactBuilderFactory::contactForceModel(std::string const& a,
+                                                            std::string const& b) {
+  int ia = materialIndex(a), ib = materialIndex(b);
+  auto it = registry().models.find({std::min(ia, ib), std::max(ia, ib)});
+  return it == registry().models.end() ? nullptr : it->second.get();
+}
+
+int ContactBuilderFactory::materialIndex(std::string const& material) {
+  auto it = registry().materials.find(material);
+  if (it == registry().materials.end()) throw std::invalid_argument("unknown material " + material);
+  return it->second;
+}
+
+bool ContactBuilderFactory::checkContactForceModelsExist(std::string& matA, std::string& matB) {
+  for (auto const& a : registry().materials)
+    for (auto const& b : registry().materials)
+      if (!contactForceModel(a.first, b.first)) {
+        matA = a.first;
+        matB = b.first;
+        return false;
+      }
+  return true;
+}
+
+void ContactBuilderFactory::eraseAllContactForceModels() { registry().models.clear(); }
+
+void ContactBuilderFactory::reset() {
+  registry().models.clear();
+  registry().materials.clear();
+}
+
+TimeIntegrator::~TimeIntegrator() = default;
+
+TimeIntegrator* TimeIntegratorBuilderFactory::create(std::string const& scheme) {
+  if (scheme == "SecondOrderLeapFrog") return new SecondOrderLeapFrog();
+  throw std::invalid_argument("time integration scheme " + scheme +
+                              " has no device implementation (SecondOrderLeapFrog only)");
+}
+
+// ------------------------------------------------------------ linked cell
+LinkedCell::LinkedCell(const double origin[3], const double size[3], const bool periodic[3],
+                       int device)
+    : m_device(device) {
+  for (int d = 0; d < 3; ++d) {
+    m_origin[d] = origin[d];
+    m_size[d] = size[d];
+    m_periodic[d] = periodic[d];
+  }
+}
+
+LinkedCell::~LinkedCell() {
+  if (m_ctx) gdem_destroy(m_ctx);
+}
+
+size_t LinkedCell::set(double cellsize, std::string const&) {
+  gdem_config cfg{};
+  for (int d = 0; d < 3; ++d) {
+    cfg.origin[d] = m_origin[d];
+    cfg.size[d] = m_size[d];
+    cfg.periodic[d] = m_periodic[d] ? 1 : 0;
+    cfg.gravity[d] = GrainsExec::m_vgravity[d];
+  }
+  cfg.cell_size = cellsize;
+  cfg.device = m_device;
+  if (m_ctx) gdem_destroy(m_ctx);
+  m_ctx = nullptr;
+  return gdem_create(&cfg, &m_ctx) == GDEM_OK ? 0 : 1;
+}
+
+void LinkedCell::push_models() {
+  for (auto const& kv : registry().models) {
+    const ContactForceModel* m = kv.second.get();
+    if (gdem_set_contact_model(m_ctx, kv.first.first, kv.first.second, m->kind(), m->parameters()))
+      throw std::invalid_argument("contact model " + m->name() + " rejected");
+  }
+}
+
+void LinkedCell::Link(std::vector<SphereParticle> const& ps) {
+  if (!m_ctx) throw std::logic_error("LinkedCell::set must be called first");
+  push_models();
+  const size_t n = ps.size();
+  std::vector<double> x(3 * n), v(3 * n), w(3 * n), q(4 * n), r(n), rho(n), cr(n);
+  std::vector<int32_t> mat(n);
+  for (size_t i = 0; i < n; ++i) {
+    for (int d = 0; d < 3; ++d) {
+      x[3 * i + d] = ps[i].position[d];
+      v[3 * i + d] = ps[i].velocity[d];
+      w[3 * i + d] = ps[i].angular_velocity[d];
+    }
+    for (int d = 0; d < 4; ++d) q[4 * i + d] = ps[i].quaternion[d];
+    r[i] = ps[i].radius;
+    rho[i] = ps[i].density;
+    cr[i] = ps[i].crust;
+    ContactBuilderFactory::defineMaterial(ps[i].material, false);
+    mat[i] = ContactBuilderFactory::materialIndex(ps[i].material);
+  }
+  if (gdem_set_particles(m_ctx, (int64_t)n, x.data(), v.data(), w.data(), q.data(), r.data(),
+                         rho.data(), cr.data(), mat.data()))
+    throw std::invalid_argument("particle set rejected");
+  m_n = n;
+}
+
+void LinkedCell::Link(BoxObstacle const& ob) {
+  if (!m_ctx) throw std::logic_error("LinkedCell::set must be called first");
+  ContactBuilderFactory::defineMaterial(ob.material, true);
+  const double half[3] = {fabs(ob.edges[0]) / 2., fabs(ob.edges[1]) / 2., fabs(ob.edges[2]) / 2.};
+  if (gdem_add_wall(m_ctx, ob.basis, ob.centre, half, ob.crust,
+                    ContactBuilderFactory::materialIndex(ob.material)))
+    throw std::invalid_argument("obstacle " + ob.name + " rejected");
+  push_models();
+}
+
+void LinkedCell::LinkUpdate(double, double) {
+  if (gdem_link_update(m_ctx)) throw std::runtime_error("gdem_link_update failed");
+}
+
+void LinkedCell::ComputeForces(double, double) {
+  if (gdem_compute_forces(m_ctx)) throw std::runtime_error("gdem_compute_forces failed");
+}
+
+void LinkedCell::getCells(int32_t nc[3], double edge[3]) const { gdem_get_grid(m_ctx, nc, edge); }
+
+// ------------------------------------------------------------ step driver
+Grains::Grains(LinkedCell& lc, TimeIntegrator const& ti) : m_lc(lc) {
+  if (ti.name() != "SecondOrderLeapFrog")
+    throw std::invalid_argument("only SecondOrderLeapFrog has a device implementation");
+}
+
+void Grains::check(int rc) const {
+  if (rc == GDEM_OK) return;
+  int32_t code = 0, a = -1, b = -1;
+  gdem_last_error(m_lc.context(), &code, &a, &b);
+  switch (rc) {
+    case GDEM_ERR_CONTACT: throw ContactError(a, b, m_time);
+    case GDEM_ERR_MOTION: throw MotionError(a, m_time);
+    case GDEM_ERR_SIMULATION: throw SimulationError(a, m_time);
+    default: throw std::runtime_error("gdem call failed with code " + std::to_string(rc));
+  }
+}
+
+void Grains::do_before_time_stepping() {
+  check(gdem_prime(m_lc.context()));
+  check(gdem_synchronize(m_lc.context()));
+  m_primed = true;
+}
+
+void Grains::Simulation(double time_interval, double dt) {
+  if (!m_primed) do_before_time_stepping();
+  const int32_t nsteps = (int32_t)llround(time_interval / dt);
+  check(gdem_step(m_lc.context(), nsteps, dt));
+  m_time += nsteps * dt;
+  check(gdem_synchronize(m_lc.context()));
+}
+
+void Grains::moveParticlesAndObstacles(double dtv, double dtd) {
+  check(gdem_move(m_lc.context(), dtv, dtd));
+  m_lc.LinkUpdate(m_time, dtd);
+}
+
+void Grains::computeParticlesForceAndTorque() { m_lc.ComputeForces(m_time, 0.); }
+
+void Grains::advanceParticlesVelocity(double dtv) {
+  check(gdem_advance_velocity(m_lc.context(), dtv));
+  check(gdem_synchronize(m_lc.context()));
+}
+
+void Grains::getState(std::vector<double>& x, std::vector<double>& v, std::vector<double>& w,
+                      std::vector<double>& q, std::vector<double>& F,
+                      std::vector<double>& T) const {
+  const size_t n = m_lc.numberOfParticles();
+  x.resize(3 * n);
+  v.resize(3 * n);
+  w.resize(3 * n);
+  q.resize(4 * n);
+  F.resize(3 * n);
+  T.resize(3 * n);
+  check(gdem_get_state(m_lc.context(), x.data(), v.data(), w.data(), q.data(), F.data(), T.data()));
+}
+
+}  // namespace grains
+}  // namespace gdem
