@@ -113,6 +113,48 @@ void orc_contact_force(int kind, const double* p, const double* point,
                        double req0, const double* x1, const double* v1,
                        const double* w1, double m1, double req1, double* f);
 
+/* ---- Convex particles (boxes, polyhedra) with the Johnson GJK ---------- */
+#define ORC_SHAPE_SPHERE 0
+#define ORC_SHAPE_BOX 1
+#define ORC_SHAPE_POLYHEDRON 2
+
+typedef struct orc_csys orc_csys;
+
+int orc_c_create(const orc_config* cfg, orc_csys** out);
+void orc_c_destroy(orc_csys* s);
+/* returns the shape index, or -1 */
+int orc_c_add_box_shape(orc_csys* s, const double* half_extent);
+int orc_c_add_polyhedron_shape(orc_csys* s, int nv, const double* verts, int nf,
+                               const int32_t* face_size, const int32_t* face_idx);
+int orc_c_shape_info(const orc_csys* s, int shape, double* vol, double* inertia,
+                     double* circ, double* bbox_ext);
+/* basis: row-major rotation of each particle's transform */
+int orc_c_set_particles(orc_csys* s, int n, const double* x, const double* v,
+                        const double* w, const double* q, const double* basis,
+                        const int32_t* shape, const double* density,
+                        const double* crust, const int32_t* material);
+int orc_c_add_wall(orc_csys* s, const double* basis, const double* centre,
+                   const double* half_extent, double crust, int material);
+int orc_c_set_contact_model(orc_csys* s, int mat_a, int mat_b, int kind,
+                            const double* p);
+int orc_c_prime(orc_csys* s);
+int orc_c_step(orc_csys* s, int nsteps, double dt);
+int orc_c_get_state(const orc_csys* s, double* x, double* v, double* w,
+                    double* q, double* basis, double* F, double* T);
+void orc_c_enable_contact_log(orc_csys* s, int capacity);
+int orc_c_get_contacts(const orc_csys* s, int capacity, int* n_out, int32_t* id0,
+                       int32_t* id1, double* point, double* overlap_vec,
+                       double* overlap, double* force);
+int orc_c_last_error(const orc_csys* s, int* code, int32_t* id0, int32_t* id1);
+int64_t orc_c_gjk_calls(const orc_csys* s);
+/* GJK between two shapes of this system under crust-scaled transforms:
+   world witness points and distance */
+int orc_c_closest_points(const orc_csys* s, int shape_a, const double* basis_a,
+                         const double* origin_a, const double* scale_a, int shape_b,
+                         const double* basis_b, const double* origin_b,
+                         const double* scale_b, double* pa, double* pb, int* nbIter,
+                         double* dist);
+
 #ifdef __cplusplus
 }
 #endif

@@ -212,3 +212,143 @@ def contact_force(kind, params, point, ovec, dist, x0, v0, w0, m0, req0, x1, v1,
                           _p(_f(x0, (3,))), _p(_f(v0, (3,))), _p(_f(w0, (3,))), m0, req0,
                           _p(_f(x1, (3,))), _p(_f(v1, (3,))), _p(_f(w1, (3,))), m1, req1, _p(f))
     return f
+
+
+# --------------------------------------------------------------------------- convex
+SHAPE_BOX, SHAPE_POLYHEDRON = 1, 2
+
+
+def _load_convex(lib):
+    if getattr(lib, "_convex_sig", False):
+        return lib
+    vp, d, i32 = C.c_void_p, C.c_double, C.c_int
+    P = C.POINTER
+    sig = {
+        "orc_c_create": (i32, [P(_Config), P(vp)]),
+        "orc_c_destroy": (None, [vp]),
+        "orc_c_add_box_shape": (i32, [vp, vp]),
+        "orc_c_add_polyhedron_shape": (i32, [vp, i32, vp, i32, vp, vp]),
+        "orc_c_shape_info": (i32, [vp, i32, vp, vp, vp, vp]),
+        "orc_c_set_particles": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "orc_c_add_wall": (i32, [vp, vp, vp, vp, d, i32]),
+        "orc_c_set_contact_model": (i32, [vp, i32, i32, i32, vp]),
+        "orc_c_prime": (i32, [vp]),
+        "orc_c_step": (i32, [vp, i32, d]),
+        "orc_c_get_state": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
+        "orc_c_enable_contact_log": (None, [vp, i32]),
+        "orc_c_get_contacts": (i32, [vp, i32, P(i32), vp, vp, vp, vp, vp, vp]),
+        "orc_c_last_error": (i32, [vp, P(i32), P(C.c_int32), P(C.c_int32)]),
+        "orc_c_gjk_calls": (C.c_int64, [vp]),
+        "orc_c_closest_points": (i32, [vp, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, P(i32), P(d)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    lib._convex_sig = True
+    return lib
+
+
+class ConvexOracle:
+    """Oracle of the convex (GJK) step built from a workload dict with
+    "shapes" (pacific_amd.workloads.convex_box)."""
+
+    def __init__(self, wl, log_capacity=0):
+        self.lib = _load_convex(load())
+        cfg = _Config()
+        for d in range(3):
+            cfg.origin[d] = wl["origin"][d]
+            cfg.size[d] = wl["size"][d]
+            cfg.periodic[d] = int(wl["periodic"][d])
+            cfg.gravity[d] = wl["gravity"][d]
+        cfg.cell_size = wl["cell_size"]
+        h = C.c_void_p()
+        rc = self.lib.orc_c_create(C.byref(cfg), C.byref(h))
+        if rc:
+            raise OracleError(rc, "orc_c_create", -1, -1)
+        self.h = h
+        for sh in wl["shapes"]:
+            self.add_shape(sh)
+        for (a, b, kind, p) in wl["models"]:
+            self._chk(self.lib.orc_c_set_contact_model(h, a, b, kind, _p(_f(p, (5,)))), "model")
+        for (basis, centre, half, crust, mat) in wl["walls"]:
+            self._chk(self.lib.orc_c_add_wall(h, _p(_f(basis, (9,))), _p(_f(centre, (3,))),
+                                              _p(_f(half, (3,))), crust, mat), "wall")
+        p = wl["particles"]
+        n = len(p["shape"])
+        self.n = n
+        keep = [_f(p["x"], (n, 3)), _f(p["v"], (n, 3)), _f(p["w"], (n, 3)), _f(p["q"], (n, 4)),
+                _f(p["basis"], (n, 9))]
+        shape = np.ascontiguousarray(p["shape"], dtype=np.int32)
+        dens, crust = _f(p["density"], (n,)), _f(p["crust"], (n,))
+        mat = np.ascontiguousarray(p["material"], dtype=np.int32)
+        self._chk(self.lib.orc_c_set_particles(h, n, *[_p(a) for a in keep], _p(shape), _p(dens),
+                                               _p(crust), _p(mat)), "particles")
+        if log_capacity:
+            self.lib.orc_c_enable_contact_log(h, int(log_capacity))
+
+    def add_shape(self, sh):
+        if sh[0] == "box":
+            return self.lib.orc_c_add_box_shape(self.h, _p(_f(sh[1], (3,))))
+        verts = _f(sh[1], (-1, 3))
+        faces = [list(f) for f in sh[2]]
+        sizes = np.array([len(f) for f in faces], np.int32)
+        idx = np.array([i for f in faces for i in f], np.int32)
+        return self.lib.orc_c_add_polyhedron_shape(self.h, len(verts), _p(verts), len(faces),
+                                                   _p(sizes), _p(idx))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_c_destroy(self.h)
+            self.h = None
+
+    def _chk(self, rc, what):
+        if rc:
+            code, a, b = C.c_int(), C.c_int32(), C.c_int32()
+            self.lib.orc_c_last_error(self.h, C.byref(code), C.byref(a), C.byref(b))
+            raise OracleError(rc, what, a.value, b.value)
+
+    def prime(self):
+        self._chk(self.lib.orc_c_prime(self.h), "prime")
+
+    def step(self, n, dt):
+        self._chk(self.lib.orc_c_step(self.h, int(n), float(dt)), "step")
+
+    def state(self):
+        n = self.n
+        out = {k: np.empty((n, 3)) for k in ("x", "v", "w", "F", "T")}
+        out["q"] = np.empty((n, 4))
+        out["basis"] = np.empty((n, 9))
+        self.lib.orc_c_get_state(self.h, _p(out["x"]), _p(out["v"]), _p(out["w"]), _p(out["q"]),
+                                 _p(out["basis"]), _p(out["F"]), _p(out["T"]))
+        return out
+
+    def contacts(self):
+        n = C.c_int()
+        self.lib.orc_c_get_contacts(self.h, 0, C.byref(n), None, None, None, None, None, None)
+        m = n.value
+        res = {"id0": np.empty(m, np.int32), "id1": np.empty(m, np.int32),
+               "point": np.empty((m, 3)), "ovec": np.empty((m, 3)), "overlap": np.empty(m),
+               "force": np.empty((m, 3))}
+        self.lib.orc_c_get_contacts(self.h, m, C.byref(n), _p(res["id0"]), _p(res["id1"]),
+                                    _p(res["point"]), _p(res["ovec"]), _p(res["overlap"]),
+                                    _p(res["force"]))
+        return res
+
+    def shape_info(self, k):
+        vol, circ = C.c_double(), C.c_double()
+        inertia, bb = np.zeros(6), np.zeros(3)
+        self.lib.orc_c_shape_info(self.h, k, C.byref(vol), _p(inertia), C.byref(circ), _p(bb))
+        return vol.value, inertia, circ.value, bb
+
+    def closest_points(self, sa, basis_a, origin_a, scale_a, sb, basis_b, origin_b, scale_b):
+        pa, pb = np.zeros(3), np.zeros(3)
+        it, d = C.c_int(), C.c_double()
+        self.lib.orc_c_closest_points(self.h, sa, _p(_f(basis_a, (9,))), _p(_f(origin_a, (3,))),
+                                      _p(_f(scale_a, (3,))), sb, _p(_f(basis_b, (9,))),
+                                      _p(_f(origin_b, (3,))), _p(_f(scale_b, (3,))), _p(pa),
+                                      _p(pb), C.byref(it), C.byref(d))
+        return d.value, pa, pb, it.value
+
+    def gjk_calls(self):
+        return self.lib.orc_c_gjk_calls(self.h)

@@ -144,3 +144,204 @@ void gdw_uniform(int n, double a, double b, int log_scale, uint64_t seed, double
     out[p] = log_scale ? exp(log(a) + (log(b) - log(a)) * u) : a + (b - a) * u;
   }
 }
+
+/* ---------------------------------------------------------------- convex */
+/* A convex polyhedron for the separating-axis test: vertices, face normals and
+   edge directions in its local frame. */
+typedef struct {
+  int nv, nf, ne;
+  double v[8][3], f[6][3], e[6][3];
+  double circ;
+} gdw_poly;
+
+static void rot_apply(const double* R, const double* a, double* out) {
+  for (int i = 0; i < 3; ++i) out[i] = R[3 * i] * a[0] + R[3 * i + 1] * a[1] + R[3 * i + 2] * a[2];
+}
+
+static void cross3(const double* a, const double* b, double* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* Separated along some axis by more than margin (margin < 0 tolerates an
+   overlap of |margin|). */
+static int sat_separated(const gdw_poly* A, const double* RA, const double* cA, const gdw_poly* B,
+                         const double* RB, const double* cB, double margin) {
+  double va[8][3], vb[8][3], axes[6 + 6 + 36][3];
+  int na = 0;
+  for (int i = 0; i < A->nv; ++i) {
+    rot_apply(RA, A->v[i], va[i]);
+    for (int d = 0; d < 3; ++d) va[i][d] += cA[d];
+  }
+  for (int i = 0; i < B->nv; ++i) {
+    rot_apply(RB, B->v[i], vb[i]);
+    for (int d = 0; d < 3; ++d) vb[i][d] += cB[d];
+  }
+  for (int i = 0; i < A->nf; ++i) rot_apply(RA, A->f[i], axes[na++]);
+  for (int i = 0; i < B->nf; ++i) rot_apply(RB, B->f[i], axes[na++]);
+  double ea[6][3], eb[6][3];
+  for (int i = 0; i < A->ne; ++i) rot_apply(RA, A->e[i], ea[i]);
+  for (int i = 0; i < B->ne; ++i) rot_apply(RB, B->e[i], eb[i]);
+  for (int i = 0; i < A->ne; ++i)
+    for (int j = 0; j < B->ne; ++j) cross3(ea[i], eb[j], axes[na++]);
+  for (int k = 0; k < na; ++k) {
+    double* n = axes[k];
+    double l = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (l < 1e-12) continue;
+    double mna = 1e300, mxa = -1e300, mnb = 1e300, mxb = -1e300;
+    for (int i = 0; i < A->nv; ++i) {
+      double p = (va[i][0] * n[0] + va[i][1] * n[1] + va[i][2] * n[2]) / l;
+      mna = p < mna ? p : mna;
+      mxa = p > mxa ? p : mxa;
+    }
+    for (int i = 0; i < B->nv; ++i) {
+      double p = (vb[i][0] * n[0] + vb[i][1] * n[1] + vb[i][2] * n[2]) / l;
+      mnb = p < mnb ? p : mnb;
+      mxb = p > mxb ? p : mxb;
+    }
+    if (mxa + margin < mnb || mxb + margin < mna) return 1;
+  }
+  return 0;
+}
+
+/* Build a SAT description: kind 0 = box (half extents h), kind 1 = polyhedron
+   with nv vertices and triangular faces (tri, 3 per face). */
+static void poly_box(gdw_poly* P, const double* h) {
+  memset(P, 0, sizeof(*P));
+  P->nv = 8;
+  for (int k = 0; k < 8; ++k) {
+    P->v[k][0] = (k & 1) ? h[0] : -h[0];
+    P->v[k][1] = (k & 2) ? h[1] : -h[1];
+    P->v[k][2] = (k & 4) ? h[2] : -h[2];
+  }
+  P->nf = 3;
+  P->ne = 3;
+  for (int d = 0; d < 3; ++d) {
+    P->f[d][d] = 1.0;
+    P->e[d][d] = 1.0;
+  }
+  P->circ = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+}
+
+static void poly_tri(gdw_poly* P, int nv, const double* verts, int nf, const int* tri) {
+  memset(P, 0, sizeof(*P));
+  P->nv = nv;
+  double r2 = 0.0;
+  for (int i = 0; i < nv; ++i) {
+    for (int d = 0; d < 3; ++d) P->v[i][d] = verts[3 * i + d];
+    double q = verts[3 * i] * verts[3 * i] + verts[3 * i + 1] * verts[3 * i + 1] +
+               verts[3 * i + 2] * verts[3 * i + 2];
+    r2 = q > r2 ? q : r2;
+  }
+  P->circ = sqrt(r2);
+  P->nf = nf;
+  for (int f = 0; f < nf; ++f) {
+    const double* a = P->v[tri[3 * f]];
+    const double* b = P->v[tri[3 * f + 1]];
+    const double* c = P->v[tri[3 * f + 2]];
+    double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, w[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    cross3(u, w, P->f[f]);
+  }
+  /* unique edges of the triangles */
+  int ne = 0, ea[12], eb[12];
+  for (int f = 0; f < nf; ++f)
+    for (int k = 0; k < 3; ++k) {
+      int a = tri[3 * f + k], b = tri[3 * f + (k + 1) % 3];
+      int lo = a < b ? a : b, hi = a < b ? b : a, dup = 0;
+      for (int m = 0; m < ne; ++m) dup |= (ea[m] == lo && eb[m] == hi);
+      if (!dup && ne < 6) {
+        ea[ne] = lo;
+        eb[ne] = hi;
+        for (int d = 0; d < 3; ++d) P->e[ne][d] = P->v[hi][d] - P->v[lo][d];
+        ++ne;
+      }
+    }
+  P->ne = ne;
+}
+
+static void quat_to_basis(const double* q, double* b) {
+  double x = q[0], y = q[1], z = q[2], w = q[3];
+  b[0] = 1. - 2. * (y * y + z * z); b[1] = 2. * (x * y - z * w); b[2] = 2. * (x * z + y * w);
+  b[3] = 2. * (x * y + z * w); b[4] = 1. - 2. * (x * x + z * z); b[5] = 2. * (y * z - x * w);
+  b[6] = 2. * (x * z - y * w); b[7] = 2. * (y * z + x * w); b[8] = 1. - 2. * (x * x + y * y);
+}
+
+/* Random sequential addition of n convex particles in the box [lo, hi]
+   (walls, no periodicity): type[i] = 0 box (half extents box_h), 1
+   tetrahedron (tet_v, 4 triangular faces tet_f).  Orientation q (Shoemake,
+   generated here), basis = R(q).  margin > 0 keeps a gap, margin < 0 allows an
+   overlap of |margin| along the least-overlap axis.  Returns 0, or -1 - p when
+   the attempts ran out while placing particle p. */
+int gdw_rsa_convex(int n, const int32_t* type, const double* box_h, const double* tet_v,
+                   const int32_t* tet_f, const double* lo, const double* hi, double margin,
+                   double sigma_v, uint64_t seed, long max_attempts, double* x, double* v,
+                   double* q, double* basis) {
+  gdw_poly P[2];
+  poly_box(&P[0], box_h);
+  int tri[12];
+  for (int k = 0; k < 12; ++k) tri[k] = tet_f[k];
+  poly_tri(&P[1], 4, tet_v, 4, tri);
+  double rmax = P[0].circ > P[1].circ ? P[0].circ : P[1].circ;
+  double h = 2.0 * rmax + (margin > 0 ? margin : 0.0);
+  int nc[3];
+  for (int d = 0; d < 3; ++d) {
+    nc[d] = (int)floor((hi[d] - lo[d]) / h);
+    if (nc[d] < 1) nc[d] = 1;
+  }
+  long ncell = (long)nc[0] * nc[1] * nc[2];
+  int* head = (int*)malloc(sizeof(int) * ncell);
+  int* next = (int*)malloc(sizeof(int) * (size_t)n);
+  if (!head || !next) { free(head); free(next); return -1; }
+  for (long c = 0; c < ncell; ++c) head[c] = -1;
+  gdw_random_quaternions(n, seed * 31 + 17, q);
+  for (int i = 0; i < n; ++i) quat_to_basis(q + 4 * i, basis + 9 * i);
+  uint64_t st = seed * 0xD1B54A32D192ED03ULL + 29;
+  long attempts = 0;
+  for (int p = 0; p < n; ++p) {
+    const gdw_poly* A = &P[type[p] ? 1 : 0];
+    for (;;) {
+      if (++attempts > max_attempts) { free(head); free(next); return -1 - p; }
+      double c[3];
+      int ic[3];
+      for (int d = 0; d < 3; ++d) {
+        double a = lo[d] + A->circ, b = hi[d] - A->circ;
+        c[d] = a + (b - a) * u01(&st);
+        ic[d] = (int)((c[d] - lo[d]) / ((hi[d] - lo[d]) / nc[d]));
+        if (ic[d] >= nc[d]) ic[d] = nc[d] - 1;
+      }
+      int ok = 1;
+      for (int dk = -1; dk <= 1 && ok; ++dk)
+        for (int dj = -1; dj <= 1 && ok; ++dj)
+          for (int di = -1; di <= 1 && ok; ++di) {
+            int a = ic[0] + di, b = ic[1] + dj, e = ic[2] + dk;
+            if (a < 0 || b < 0 || e < 0 || a >= nc[0] || b >= nc[1] || e >= nc[2]) continue;
+            for (int qq = head[((long)e * nc[1] + b) * nc[0] + a]; qq >= 0; qq = next[qq]) {
+              const gdw_poly* B = &P[type[qq] ? 1 : 0];
+              double d2 = 0.0;
+              for (int d = 0; d < 3; ++d) {
+                double dd = x[3 * qq + d] - c[d];
+                d2 += dd * dd;
+              }
+              double s = A->circ + B->circ + (margin > 0 ? margin : 0.0);
+              if (d2 > s * s) continue;
+              if (!sat_separated(A, basis + 9 * p, c, B, basis + 9 * qq, x + 3 * qq, margin)) {
+                ok = 0;
+                break;
+              }
+            }
+          }
+      if (!ok) continue;
+      for (int d = 0; d < 3; ++d) x[3 * p + d] = c[d];
+      long cc = ((long)ic[2] * nc[1] + ic[1]) * nc[0] + ic[0];
+      next[p] = head[cc];
+      head[cc] = p;
+      break;
+    }
+  }
+  for (int p = 0; p < n; ++p)
+    for (int d = 0; d < 3; ++d) v[3 * p + d] = sigma_v * normal01(&st);
+  free(head);
+  free(next);
+  return 0;
+}
