@@ -10,6 +10,10 @@
  *   --------------------------------------------------  ------------------------
  *   LinkedCell::set (Base/include/LinkedCell.hh:249)    gdem_create
  *   AllComponents / Particle construction               gdem_set_particles
+ *   ConvexBuilderFactory::create (Geometry/src/ConvexBuilderFactory.cpp:20-50)
+ *     for Box / Polyhedron                              gdem_add_box_shape,
+ *                                                        gdem_add_polyhedron_shape
+ *   Particle construction with a convex shape            gdem_set_convex_particles
  *   LinkedCell::Link(Obstacle*) (LinkedCell.cpp:1371)   gdem_add_wall
  *   ContactBuilderFactory::define + Hooke/Hertz ctor    gdem_set_contact_model
  *     (Component/src/ContactBuilderFactory.cpp:81-192)
@@ -67,6 +71,8 @@ extern "C" {
 
 #define GDEM_MAX_MATERIALS 32
 #define GDEM_MAX_WALLS 64
+#define GDEM_MAX_SHAPES 16
+#define GDEM_MAX_VERTICES 8
 
 typedef struct gdem_ctx gdem_ctx;
 
@@ -93,6 +99,38 @@ int gdem_set_particles(gdem_ctx* ctx, int64_t n, const double* x, const double* 
                        const double* w, const double* q, const double* radius,
                        const double* density, const double* crust,
                        const int32_t* material);
+
+/* Convex shapes, registered before gdem_set_convex_particles; shape ids count
+   from 0 in registration order.
+   Box(LX, LY, LZ) (Geometry/src/Box.cpp:27-33; half extents here): volume,
+   inertia and circumscribed radius as Box.cpp:181-229, 273-279. */
+int gdem_add_box_shape(gdem_ctx* ctx, const double* half_extent, int32_t* shape_id);
+
+/* Polyhedron from vertices and faces (Geometry/src/Polyhedron.cpp:324-365):
+   vertex adjacency from the face loops, volume and inertia from a fan of
+   tetrahedra (GrainsExec::computeVolumeInertiaContrib,
+   Base/src/GrainsExec.cpp:748-800), vertices about the centre of mass.
+   nv <= GDEM_MAX_VERTICES; face k has face_size[k] vertex indices, listed
+   consecutively in face_index. */
+int gdem_add_polyhedron_shape(gdem_ctx* ctx, int32_t nv, const double* vertices,
+                              int32_t nfaces, const int32_t* face_size,
+                              const int32_t* face_index, int32_t* shape_id);
+
+/* Convex particles (Particle(DOMNode*), Particle.cpp:179-232, with a
+   RigidBodyWithCrust over a registered shape): basis is the row-major
+   rotation of each particle's Transform (9 doubles), consistent with q.
+   Mass = density * V, inertia = density * I_shape, equivalent radius from V,
+   crust scaling per axis (extent - crust) / extent (RigidBodyWithCrust.cpp:51-80).
+   Replaces any previous particle set; the step then runs the GJK narrow phase
+   (Geometry/src/Convex.cpp:536-627) for every pair and wall. */
+int gdem_set_convex_particles(gdem_ctx* ctx, int64_t n, const double* x, const double* v,
+                              const double* w, const double* q, const double* basis,
+                              const int32_t* shape, const double* density,
+                              const double* crust, const int32_t* material);
+
+/* Row-major basis of every convex particle in id order (RigidBody::getTransform,
+   Geometry/include/RigidBody.hh:167). */
+int gdem_get_basis(gdem_ctx* ctx, double* basis);
 
 /* Static box obstacle (SimpleObstacle with a Box convex; Box(LX,LY,LZ) uses
    full edges, here half extents).  basis is the row-major rotation of its

@@ -147,6 +147,13 @@ int orc_c_get_contacts(const orc_csys* s, int capacity, int* n_out, int32_t* id0
                        double* overlap, double* force);
 int orc_c_last_error(const orc_csys* s, int* code, int32_t* id0, int32_t* id1);
 int64_t orc_c_gjk_calls(const orc_csys* s);
+/* one pair / one particle-wall contact on the current state, no accumulation:
+   1 contact (point, overlap vector, overlap, force on the first body), 0 none,
+   -1 ContactError */
+int orc_c_pair(orc_csys* s, int p0, int p1, const double* shift, double* point,
+               double* ovec, double* overlap, double* force);
+int orc_c_wall_pair(orc_csys* s, int p, int wall, double* point, double* ovec,
+                    double* overlap, double* force);
 /* GJK between two shapes of this system under crust-scaled transforms:
    world witness points and distance */
 int orc_c_closest_points(const orc_csys* s, int shape_a, const double* basis_a,

@@ -742,6 +742,29 @@ static int cinteract(orc_csys* s, int p0, int p1, v3 shift) {
   return ORC_OK;
 }
 
+/* One particle pair evaluated on the current state without accumulating:
+   closest point of (p0, p1 moved by shift) and the contact force on p0.
+   Returns 1 contact, 0 none, -1 ContactError. */
+int orc_c_pair(orc_csys* s, int p0, int p1, const double* shift, double* point, double* ovec,
+               double* overlap, double* force) {
+  body_t A = particle_body(s, p0, mk(0., 0., 0.)), B = particle_body(s, p1, ld(shift));
+  v3 c, ov;
+  double dist;
+  int64_t ng = 0;
+  int rc = closest_point(&A, &B, &c, &ov, &dist, &ng);
+  if (rc <= 0) return rc;
+  st(point, c);
+  st(ovec, ov);
+  *overlap = dist;
+  if (!(dist < 0.)) return 0;
+  const cmodel_t* m = &s->models[s->mat[p0]][s->mat[p1]];
+  v3 f = cforce(m, c, ov, dist, ld(A.origin), ld(s->v + 3 * p0), ld(s->w + 3 * p0), s->mass[p0],
+                s->req[p0], ld(B.origin), ld(s->v + 3 * p1), ld(s->w + 3 * p1), s->mass[p1],
+                s->req[p1]);
+  st(force, f);
+  return 1;
+}
+
 /* SimpleObstacle::InterAction (SimpleObstacle.cpp:222-271): the particle is
    the first body of ClosestPoint, the box wall the second */
 static int cinteract_wall(orc_csys* s, int p, int wi) {
@@ -779,6 +802,39 @@ static int cinteract_wall(orc_csys* s, int p, int wi) {
   st(s->T + 3 * p, add(ld(s->T + 3 * p), cross(sub(c, xa), f)));
   clog_contact(s, p, -1 - wi, c, ov, dist, f);
   return ORC_OK;
+}
+
+/* One particle-wall contact evaluated on the current state (no InZone test). */
+int orc_c_wall_pair(orc_csys* s, int p, int wi, double* point, double* ovec, double* overlap,
+                    double* force) {
+  const cwall_t* wl = &s->walls[wi];
+  body_t A = particle_body(s, p, mk(0., 0., 0.));
+  shape_t wsh;
+  memset(&wsh, 0, sizeof(wsh));
+  wsh.type = ORC_SHAPE_BOX;
+  memcpy(wsh.ext, wl->ext, sizeof(wsh.ext));
+  body_t B;
+  B.sh = &wsh;
+  B.basis = wl->basis;
+  memcpy(B.origin, wl->origin, sizeof(B.origin));
+  B.scale = wl->scale;
+  B.crust = wl->crust;
+  B.circ = wl->circ;
+  v3 c, ov;
+  double dist;
+  int64_t ng = 0;
+  int rc = closest_point(&A, &B, &c, &ov, &dist, &ng);
+  if (rc <= 0) return rc;
+  st(point, c);
+  st(ovec, ov);
+  *overlap = dist;
+  if (!(dist < 0.)) return 0;
+  const cmodel_t* m = &s->models[s->mat[p]][wl->material];
+  v3 zero = mk(0., 0., 0.);
+  v3 f = cforce(m, c, ov, dist, ld(s->x + 3 * p), ld(s->v + 3 * p), ld(s->w + 3 * p), s->mass[p],
+                s->req[p], ld(wl->origin), zero, zero, 1.e20, wl->req);
+  st(force, f);
+  return 1;
 }
 
 static int cget_cell(const orc_csys* s, const double* p, int* ijk) {

@@ -240,6 +240,8 @@ def _load_convex(lib):
         "orc_c_last_error": (i32, [vp, P(i32), P(C.c_int32), P(C.c_int32)]),
         "orc_c_gjk_calls": (C.c_int64, [vp]),
         "orc_c_closest_points": (i32, [vp, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, P(i32), P(d)]),
+        "orc_c_pair": (i32, [vp, i32, i32, vp, vp, vp, P(d), vp]),
+        "orc_c_wall_pair": (i32, [vp, i32, i32, vp, vp, P(d), vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -352,3 +354,18 @@ class ConvexOracle:
 
     def gjk_calls(self):
         return self.lib.orc_c_gjk_calls(self.h)
+
+    def pair(self, p0, p1, shift=(0.0, 0.0, 0.0)):
+        """(rc, point, overlap vector, overlap, force on p0) of one pair with
+        p0 as the first body, on the current state."""
+        pt, ov, f = np.zeros(3), np.zeros(3), np.zeros(3)
+        dist = C.c_double()
+        rc = self.lib.orc_c_pair(self.h, int(p0), int(p1), _p(_f(shift, (3,))), _p(pt), _p(ov),
+                                 C.byref(dist), _p(f))
+        return rc, pt, ov, dist.value, f
+
+    def wall_pair(self, p, wall):
+        pt, ov, f = np.zeros(3), np.zeros(3), np.zeros(3)
+        dist = C.c_double()
+        rc = self.lib.orc_c_wall_pair(self.h, int(p), int(wall), _p(pt), _p(ov), C.byref(dist), _p(f))
+        return rc, pt, ov, dist.value, f

@@ -62,6 +62,17 @@ struct gdem_ctx {
   int64_t timed = 0;
   // last error
   int32_t err_code = 0, err_id0 = -1, err_id1 = -1;
+  // convex particles: shapes, orientation state, stored contacts, wall shapes
+  bool convex = false;
+  CShape h_shapes[kMaxShapes]{};
+  int nshapes = 0;
+  CShape* d_shapes = nullptr;
+  CState cbuf[2]{};
+  CRec* d_out_rec = nullptr;
+  CShape h_wall_shapes[kMaxWalls]{};
+  double4 h_wall_sc[kMaxWalls]{};
+  CShape* d_wall_shapes = nullptr;
+  double4* d_wall_sc = nullptr;
 };
 
 #define HIP_TRY(expr)                                                        \
@@ -72,6 +83,8 @@ struct gdem_ctx {
       return GDEM_ERR_HIP;                                                   \
     }                                                                        \
   } while (0)
+
+static int collect_errors(gdem_ctx* c);
 
 static int set_err(gdem_ctx* c, int code, int id0, int id1) {
   c->err_code = code;
@@ -84,6 +97,20 @@ static void free_state(State& s) {
   (void)hipFree(s.pos_r); (void)hipFree(s.vel_m); (void)hipFree(s.om_req); (void)hipFree(s.wbf_I); (void)hipFree(s.quat);
   (void)hipFree(s.aux);
   s = State{};
+}
+
+static void free_cstate(CState& s) {
+  (void)hipFree(s.rot0); (void)hipFree(s.rot1); (void)hipFree(s.rot2); (void)hipFree(s.inert);
+  s = CState{};
+}
+
+static int alloc_cstate(CState& s, int64_t n) {
+  size_t b4 = sizeof(double4) * (size_t)std::max<int64_t>(n, 1);
+  HIP_TRY(hipMalloc(&s.rot0, b4));
+  HIP_TRY(hipMalloc(&s.rot1, b4));
+  HIP_TRY(hipMalloc(&s.rot2, b4));
+  HIP_TRY(hipMalloc(&s.inert, b4));
+  return GDEM_OK;
 }
 
 static int alloc_state(State& s, int64_t n) {
@@ -161,6 +188,9 @@ extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
       hipMalloc(&c->bins.cell_start, sizeof(int32_t) * (ncell + 1)) != hipSuccess ||
       hipMalloc(&c->d_models, sizeof(c->h_models)) != hipSuccess ||
       hipMalloc(&c->d_walls, sizeof(c->h_walls)) != hipSuccess ||
+      hipMalloc(&c->d_shapes, sizeof(c->h_shapes)) != hipSuccess ||
+      hipMalloc(&c->d_wall_shapes, sizeof(c->h_wall_shapes)) != hipSuccess ||
+      hipMalloc(&c->d_wall_sc, sizeof(c->h_wall_sc)) != hipSuccess ||
       hipMalloc(&c->d_err, sizeof(unsigned long long) * kErrWords) != hipSuccess ||
       hipMalloc(&c->d_counters, sizeof(unsigned long long) * 2) != hipSuccess) {
     gdem_destroy(c);
@@ -189,9 +219,65 @@ extern "C" int gdem_destroy(gdem_ctx* c) {
   (void)hipFree(c->bins.cnt); (void)hipFree(c->bins.local); (void)hipFree(c->bins.bsum); (void)hipFree(c->bins.cell_start);
   (void)hipFree(c->d_row_rank); (void)hipFree(c->d_row_jk); (void)hipFree(c->d_models); (void)hipFree(c->d_walls); (void)hipFree(c->d_err);
   (void)hipFree(c->d_counters); (void)hipFree(c->d_rec); (void)hipFree(c->d_rec_count); (void)hipFree(c->d_stage);
+  free_cstate(c->cbuf[0]);
+  free_cstate(c->cbuf[1]);
+  (void)hipFree(c->d_out_rec); (void)hipFree(c->d_shapes); (void)hipFree(c->d_wall_shapes); (void)hipFree(c->d_wall_sc);
   for (auto& e : c->ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+  return GDEM_OK;
+}
+
+// Frees the previous particle set and allocates the buffers of n particles
+// (both state copies, torsors, binning, contact lists, recording).
+static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  free_state(c->buf[0]);
+  free_state(c->buf[1]);
+  free_cstate(c->cbuf[0]);
+  free_cstate(c->cbuf[1]);
+  (void)hipFree(c->d_F); (void)hipFree(c->d_T);
+  (void)hipFree(c->bins.key); (void)hipFree(c->bins.slot); (void)hipFree(c->bins.perm); (void)hipFree(c->bins.pos);
+  (void)hipFree(c->d_out_list); (void)hipFree(c->d_in_list); (void)hipFree(c->d_out_count); (void)hipFree(c->d_in_count);
+  (void)hipFree(c->d_rec); (void)hipFree(c->d_rec_count); (void)hipFree(c->d_out_rec);
+  c->d_F = c->d_T = nullptr;
+  c->bins.key = c->bins.slot = c->bins.perm = nullptr;
+  c->bins.pos = nullptr;
+  c->d_out_list = c->d_in_list = nullptr;
+  c->d_out_count = c->d_in_count = nullptr;
+  c->d_rec = nullptr;
+  c->d_rec_count = nullptr;
+  c->d_out_rec = nullptr;
+  c->n = n;
+  c->convex = convex;
+  const size_t nn = (size_t)std::max<int64_t>(n, 1);
+  if (alloc_state(c->buf[0], n) || alloc_state(c->buf[1], n)) return GDEM_ERR_HIP;
+  if (convex) {
+    if (alloc_cstate(c->cbuf[0], n) || alloc_cstate(c->cbuf[1], n)) return GDEM_ERR_HIP;
+    HIP_TRY(hipMalloc(&c->d_out_rec, sizeof(CRec) * nn * kCListCap));
+  } else {
+    HIP_TRY(hipMalloc(&c->d_out_list, sizeof(int2) * nn * kListCap));
+  }
+  HIP_TRY(hipMalloc(&c->d_F, sizeof(double4) * nn));
+  HIP_TRY(hipMalloc(&c->d_T, sizeof(double4) * nn));
+  HIP_TRY(hipMalloc(&c->bins.key, sizeof(int32_t) * nn));
+  HIP_TRY(hipMalloc(&c->bins.slot, sizeof(int32_t) * nn));
+  HIP_TRY(hipMalloc(&c->bins.perm, sizeof(int32_t) * nn));
+  HIP_TRY(hipMalloc(&c->bins.pos, sizeof(float4) * nn));
+  HIP_TRY(hipMalloc(&c->d_in_list, sizeof(int2) * nn * kListCap));
+  HIP_TRY(hipMalloc(&c->d_out_count, sizeof(int32_t) * nn));
+  HIP_TRY(hipMalloc(&c->d_in_count, sizeof(int32_t) * nn));
+  HIP_TRY(hipMemset(c->d_in_count, 0, sizeof(int32_t) * nn));
+  HIP_TRY(hipMemset(c->d_F, 0, sizeof(double4) * nn));
+  HIP_TRY(hipMemset(c->d_T, 0, sizeof(double4) * nn));
+  if (c->rec_cap > 0) {
+    HIP_TRY(hipMalloc(&c->d_rec, sizeof(ContactRec) * nn * (size_t)c->rec_cap));
+    HIP_TRY(hipMalloc(&c->d_rec_count, sizeof(int32_t) * nn));
+  }
+  c->primed = false;
+  c->binned = false;
+  c->rec_valid = false;
   return GDEM_OK;
 }
 
@@ -201,37 +287,9 @@ extern "C" int gdem_set_particles(gdem_ctx* c, int64_t n, const double* x, const
                                   const int32_t* material) {
   if (!c || n < 0 || n > 0x7fffffff) return GDEM_ERR_ARG;
   if (n > 0 && (!x || !v || !w || !q || !radius || !density || !crust)) return GDEM_ERR_ARG;
-  HIP_TRY(hipSetDevice(c->dev));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  free_state(c->buf[0]);
-  free_state(c->buf[1]);
-  (void)hipFree(c->d_F); (void)hipFree(c->d_T);
-  (void)hipFree(c->bins.key); (void)hipFree(c->bins.slot); (void)hipFree(c->bins.perm); (void)hipFree(c->bins.pos);
-  (void)hipFree(c->d_out_list); (void)hipFree(c->d_in_list); (void)hipFree(c->d_out_count); (void)hipFree(c->d_in_count);
-  (void)hipFree(c->d_rec); (void)hipFree(c->d_rec_count);
-  c->d_F = c->d_T = nullptr;
-  c->bins.key = c->bins.slot = c->bins.perm = nullptr;
-  c->bins.pos = nullptr;
-  c->d_out_list = c->d_in_list = nullptr;
-  c->d_out_count = c->d_in_count = nullptr;
-  c->d_rec = nullptr;
-  c->d_rec_count = nullptr;
-  c->n = n;
+  int rc = alloc_particles(c, n, false);
+  if (rc) return rc;
   const size_t nn = (size_t)std::max<int64_t>(n, 1);
-  if (alloc_state(c->buf[0], n) || alloc_state(c->buf[1], n)) return GDEM_ERR_HIP;
-  HIP_TRY(hipMalloc(&c->d_F, sizeof(double4) * nn));
-  HIP_TRY(hipMalloc(&c->d_T, sizeof(double4) * nn));
-  HIP_TRY(hipMalloc(&c->bins.key, sizeof(int32_t) * nn));
-  HIP_TRY(hipMalloc(&c->bins.slot, sizeof(int32_t) * nn));
-  HIP_TRY(hipMalloc(&c->bins.perm, sizeof(int32_t) * nn));
-  HIP_TRY(hipMalloc(&c->bins.pos, sizeof(float4) * nn));
-  HIP_TRY(hipMalloc(&c->d_out_list, sizeof(int2) * nn * kListCap));
-  HIP_TRY(hipMalloc(&c->d_in_list, sizeof(int2) * nn * kListCap));
-  HIP_TRY(hipMalloc(&c->d_out_count, sizeof(int32_t) * nn));
-  HIP_TRY(hipMalloc(&c->d_in_count, sizeof(int32_t) * nn));
-  HIP_TRY(hipMemset(c->d_in_count, 0, sizeof(int32_t) * nn));
-  HIP_TRY(hipMemset(c->d_F, 0, sizeof(double4) * nn));
-  HIP_TRY(hipMemset(c->d_T, 0, sizeof(double4) * nn));
   std::vector<double4> pr(nn), vm(nn), wr(nn), wb(nn), qq(nn);
   std::vector<Aux> ax(nn);
   for (int64_t i = 0; i < n; ++i) {
@@ -270,16 +328,220 @@ extern "C" int gdem_set_particles(gdem_ctx* c, int64_t n, const double* x, const
     HIP_TRY(hipMemcpy(s.quat, qq.data(), b4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s.aux, ax.data(), sizeof(Aux) * (size_t)n, hipMemcpyHostToDevice));
   }
-  if (c->rec_cap > 0) {
-    HIP_TRY(hipMalloc(&c->d_rec, sizeof(ContactRec) * nn * (size_t)c->rec_cap));
-    HIP_TRY(hipMalloc(&c->d_rec_count, sizeof(int32_t) * nn));
-  }
-  c->primed = false;
-  c->binned = false;
-  c->rec_valid = false;
   launch_keys(s, (int32_t)n, c->grid, c->d_row_rank, c->bins.key, c->d_err, c->stream);
   HIP_TRY(hipGetLastError());
   return GDEM_OK;
+}
+
+// ---------------------------------------------------------------- convex shapes
+// Convex::bbox at identity (Geometry/src/Convex.cpp:29-41): half extent along
+// axis d = (support(e_d).d - support(-e_d).d) / 2.
+static void finish_shape(CShape& sh) {
+  for (int d = 0; d < 3; ++d) {
+    const v3 e = mk(d == 0 ? 1. : 0., d == 1 ? 1. : 0., d == 2 ? 1. : 0.);
+    const double mx = dot(e, support(sh, e)), mn = dot(e, support(sh, neg(e)));
+    sh.bbox_ext[d] = (mx - mn) / 2.;
+  }
+}
+
+// GrainsExec::computeVolumeInertiaContrib (Base/src/GrainsExec.cpp:748-800):
+// volume and inertia of the tetrahedron (0, A2, A3, A4) (F. Tonon, J. Math.
+// Stat. 1 (2004) 8-11), accumulated.
+static void tet_contrib(v3 A2, v3 A3, v3 A4, double& vol, double* in) {
+  const double x1 = 0., x2 = A2.x, x3 = A3.x, x4 = A4.x, y1 = 0., y2 = A2.y, y3 = A3.y,
+               y4 = A4.y, z1 = 0., z2 = A2.z, z3 = A3.z, z4 = A4.z;
+  const double det = fabs((x2 - x1) * (y3 - y1) * (z4 - z1) + (y2 - y1) * (z3 - z1) * (x4 - x1) +
+                          (z2 - z1) * (x3 - x1) * (y4 - y1) - (z2 - z1) * (y3 - y1) * (x4 - x1) -
+                          (x2 - x1) * (z3 - z1) * (y4 - y1) - (y2 - y1) * (x3 - x1) * (z4 - z1));
+  vol += det / 6.;
+  // each moment is one left-to-right sum, in the reference's term order
+  auto sq = [](double s, double a1, double a2, double a3, double a4) {
+    return s + a1 * a1 + a1 * a2 + a2 * a2 + a1 * a3 + a2 * a3 + a3 * a3 + a1 * a4 + a2 * a4 +
+           a3 * a4 + a4 * a4;
+  };
+  auto mixed = [](double a1, double a2, double a3, double a4, double b1, double b2, double b3,
+                  double b4) {
+    return 2. * a1 * b1 + a2 * b1 + a3 * b1 + a4 * b1 + a1 * b2 + 2. * a2 * b2 + a3 * b2 +
+           a4 * b2 + a1 * b3 + a2 * b3 + 2. * a3 * b3 + a4 * b3 + a1 * b4 + a2 * b4 + a3 * b4 +
+           2. * a4 * b4;
+  };
+  in[0] += det * sq(sq(0., y1, y2, y3, y4), z1, z2, z3, z4) / 60.;
+  in[1] -= det * mixed(x1, x2, x3, x4, z1, z2, z3, z4) / 120.;
+  in[2] -= det * mixed(x1, x2, x3, x4, y1, y2, y3, y4) / 120.;
+  in[3] += det * sq(sq(0., x1, x2, x3, x4), z1, z2, z3, z4) / 60.;
+  in[4] -= det * mixed(y1, y2, y3, y4, z1, z2, z3, z4) / 120.;
+  in[5] += det * sq(sq(0., x1, x2, x3, x4), y1, y2, y3, y4) / 60.;
+}
+
+extern "C" int gdem_add_box_shape(gdem_ctx* c, const double* half, int32_t* shape_id) {
+  if (!c || !half) return GDEM_ERR_ARG;
+  if (c->nshapes >= kMaxShapes) return GDEM_ERR_CAPACITY;
+  if (!(half[0] > 0. && half[1] > 0. && half[2] > 0.)) return GDEM_ERR_ARG;
+  CShape sh{};
+  sh.type = SHAPE_BOX;
+  for (int d = 0; d < 3; ++d) sh.ext[d] = half[d];
+  const double ex = half[0], ey = half[1], ez = half[2];
+  // Box::getVolume (Box.cpp:273-279), Box::BuildInertia (:181-205),
+  // Box::computeCircumscribedRadius (:226-229)
+  sh.vol = 8.0 * ex * ey * ez;
+  sh.inertia[0] = 8.0 * ex * ey * ez * (ey * ey + ez * ez) / 3.0;
+  sh.inertia[3] = 8.0 * ex * ey * ez * (ex * ex + ez * ez) / 3.0;
+  sh.inertia[5] = 8.0 * ex * ey * ez * (ey * ey + ex * ex) / 3.0;
+  sh.circ = norm(mk(ex, ey, ez));
+  finish_shape(sh);
+  c->h_shapes[c->nshapes] = sh;
+  if (shape_id) *shape_id = c->nshapes;
+  c->nshapes++;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipMemcpy(c->d_shapes, c->h_shapes, sizeof(CShape) * c->nshapes, hipMemcpyHostToDevice));
+  return GDEM_OK;
+}
+
+extern "C" int gdem_add_polyhedron_shape(gdem_ctx* c, int32_t nv, const double* verts,
+                                         int32_t nf, const int32_t* face_size,
+                                         const int32_t* face_idx, int32_t* shape_id) {
+  if (!c || !verts || !face_size || !face_idx) return GDEM_ERR_ARG;
+  if (c->nshapes >= kMaxShapes) return GDEM_ERR_CAPACITY;
+  if (nv < 4 || nv > kMaxVerts || nf < 4) return GDEM_ERR_ARG;
+  CShape sh{};
+  sh.type = SHAPE_POLYHEDRON;
+  sh.nv = nv;
+  for (int i = 0; i < nv; ++i)
+    for (int d = 0; d < 3; ++d) sh.v[i][d] = verts[3 * i + d];
+  auto V = [&](int i) { return mk(sh.v[i][0], sh.v[i][1], sh.v[i][2]); };
+  int off = 0;
+  for (int f = 0; f < nf; ++f) {
+    const int m = face_size[f];
+    if (m < 3) return GDEM_ERR_ARG;
+    const int32_t* fi = face_idx + off;
+    for (int j = 0; j < m; ++j)
+      if (fi[j] < 0 || fi[j] >= nv) return GDEM_ERR_ARG;
+    // adjacency: each vertex is followed by the next vertex of the face loop
+    for (int j = 0, k = m - 1; j < m; k = j++) {
+      const int a = fi[k];
+      if (sh.cob_n[a] >= kMaxVerts) return GDEM_ERR_ARG;
+      sh.cob[a][sh.cob_n[a]++] = fi[j];
+    }
+    if (m == 3) {
+      tet_contrib(V(fi[0]), V(fi[1]), V(fi[2]), sh.vol, sh.inertia);
+    } else {
+      v3 G = mk(0., 0., 0.);
+      for (int j = 0; j < m; ++j) G = add(G, V(fi[j]));
+      G = dvd(G, (double)m);
+      for (int j = 0, k = m - 1; j < m; k = j++) tet_contrib(G, V(fi[k]), V(fi[j]), sh.vol, sh.inertia);
+    }
+    off += m;
+  }
+  // Polyhedron.cpp:362-364: the support search starts at the first vertex
+  // with neighbours
+  sh.first_vertex = 0;
+  while (sh.first_vertex < nv && sh.cob_n[sh.first_vertex] == 0) ++sh.first_vertex;
+  if (sh.first_vertex == nv) return GDEM_ERR_ARG;
+  // Polyhedron::computeCircumscribedRadius (Polyhedron.cpp:201-210)
+  double ray = dot(V(0), V(0));
+  for (int i = 1; i < nv; ++i) {
+    const double d = dot(V(i), V(i));
+    if (d > ray) ray = d;
+  }
+  sh.circ = sqrt(ray);
+  finish_shape(sh);
+  c->h_shapes[c->nshapes] = sh;
+  if (shape_id) *shape_id = c->nshapes;
+  c->nshapes++;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipMemcpy(c->d_shapes, c->h_shapes, sizeof(CShape) * c->nshapes, hipMemcpyHostToDevice));
+  return GDEM_OK;
+}
+
+// Crust scaling of a body with the given bounding-box half extents
+// (RigidBodyWithCrust.cpp:51-80).
+static void crust_scale(const double* ext, double crust, double* sc) {
+  for (int d = 0; d < 3; ++d) sc[d] = ext[d] < kEps ? 1. : (ext[d] - crust) / ext[d];
+}
+
+extern "C" int gdem_set_convex_particles(gdem_ctx* c, int64_t n, const double* x, const double* v,
+                                         const double* w, const double* q, const double* basis,
+                                         const int32_t* shape, const double* density,
+                                         const double* crust, const int32_t* material) {
+  if (!c || n < 0 || n > 0x7fffffff) return GDEM_ERR_ARG;
+  if (n > 0 && (!x || !v || !w || !q || !basis || !shape || !density || !crust))
+    return GDEM_ERR_ARG;
+  for (int64_t i = 0; i < n; ++i) {
+    if (shape[i] < 0 || shape[i] >= c->nshapes) return GDEM_ERR_ARG;
+    if (!(density[i] > 0.) || !(crust[i] > 0.)) return GDEM_ERR_ARG;
+    const int32_t mat = material ? material[i] : 0;
+    if (mat < 0 || mat >= kMaxMaterials) return GDEM_ERR_ARG;
+  }
+  int rc = alloc_particles(c, n, true);
+  if (rc) return rc;
+  const size_t nn = (size_t)std::max<int64_t>(n, 1);
+  std::vector<double4> pr(nn), vm(nn), wr(nn), wb(nn), qq(nn), r0(nn), r1(nn), r2(nn), in3(nn);
+  std::vector<Aux> ax(nn);
+  for (int64_t i = 0; i < n; ++i) {
+    const CShape& sh = c->h_shapes[shape[i]];
+    const double rho = density[i];
+    const int32_t mat = material ? material[i] : 0;
+    // Particle.cpp:158-164, Component::getEquivalentSphereRadius (Component.cpp:205-208)
+    const double mass = rho * sh.vol;
+    const double req = pow((0.75 / kPi) * sh.vol, 1. / 3.);
+    double sc[3];
+    crust_scale(sh.bbox_ext, crust[i], sc);
+    const quat qi = qmk(mk(q[4 * i], q[4 * i + 1], q[4 * i + 2]), q[4 * i + 3]);
+    const v3 wbf = qmulv(qconj(qi), qleftvec(qi, mk(w[3 * i], w[3 * i + 1], w[3 * i + 2])));
+    const double* b = basis + 9 * i;
+    pr[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], sh.circ);
+    vm[i] = make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], mass);
+    wr[i] = make_double4(w[3 * i], w[3 * i + 1], w[3 * i + 2], req);
+    wb[i] = make_double4(wbf.x, wbf.y, wbf.z, 0.);
+    qq[i] = make_double4(q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]);
+    r0[i] = make_double4(b[0], b[1], b[2], sc[0]);
+    r1[i] = make_double4(b[3], b[4], b[5], sc[1]);
+    r2[i] = make_double4(b[6], b[7], b[8], sc[2]);
+    in3[i] = make_double4(sh.inertia[0] * rho, sh.inertia[3] * rho, sh.inertia[5] * rho, 0.);
+    ax[i].id = (int32_t)i;
+    ax[i].mat = mat | (shape[i] << 8);
+    ax[i].crust = crust[i];
+  }
+  State& s = c->buf[0];
+  CState& cs = c->cbuf[0];
+  c->cur = 0;
+  const size_t b4 = sizeof(double4) * (size_t)n;
+  if (n > 0) {
+    HIP_TRY(hipMemcpy(s.pos_r, pr.data(), b4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s.vel_m, vm.data(), b4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s.om_req, wr.data(), b4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s.wbf_I, wb.data(), b4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s.quat, qq.data(), b4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s.aux, ax.data(), sizeof(Aux) * (size_t)n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(cs.rot0, r0.data(), b4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(cs.rot1, r1.data(), b4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(cs.rot2, r2.data(), b4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(cs.inert, in3.data(), b4, hipMemcpyHostToDevice));
+  }
+  launch_keys(s, (int32_t)n, c->grid, c->d_row_rank, c->bins.key, c->d_err, c->stream);
+  HIP_TRY(hipGetLastError());
+  return GDEM_OK;
+}
+
+extern "C" int gdem_get_basis(gdem_ctx* c, double* basis) {
+  if (!c || !basis) return GDEM_ERR_ARG;
+  if (!c->convex) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  const size_t n = (size_t)c->n;
+  const size_t need = n * 9;
+  if (c->stage_len < need) {
+    (void)hipFree(c->d_stage);
+    c->d_stage = nullptr;
+    HIP_TRY(hipMalloc(&c->d_stage, sizeof(double) * std::max<size_t>(need, 1)));
+    c->stage_len = need;
+  }
+  launch_cunpermute(c->buf[c->cur], c->cbuf[c->cur], (int32_t)n, c->d_stage, c->stream);
+  HIP_TRY(hipGetLastError());
+  if (n > 0)
+    HIP_TRY(hipMemcpyAsync(basis, c->d_stage, sizeof(double) * need, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return collect_errors(c);
 }
 
 extern "C" int gdem_add_wall(gdem_ctx* c, const double* basis, const double* centre,
@@ -320,10 +582,23 @@ extern "C" int gdem_add_wall(gdem_ctx* c, const double* basis, const double* cen
   // Box::getVolume (Box.cpp:273-279), equivalent-sphere radius of the obstacle
   const double vol = 8.0 * e0 * e1 * e2;
   wl.req = pow((0.75 / kPi) * vol, 1. / 3.);
+  // the wall as a convex body for the GJK narrow phase of convex particles:
+  // Box shape, crust scaling of its extents, circumscribed radius
+  CShape wsh{};
+  wsh.type = SHAPE_BOX;
+  for (int d = 0; d < 3; ++d) wsh.ext[d] = half[d];
+  double wsc[3];
+  crust_scale(half, crust, wsc);
+  c->h_wall_shapes[c->nwalls] = wsh;
+  c->h_wall_sc[c->nwalls] = make_double4(wsc[0], wsc[1], wsc[2], norm(mk(e0, e1, e2)));
   c->h_walls[c->nwalls++] = wl;
   HIP_TRY(hipSetDevice(c->dev));
   HIP_TRY(hipMemcpyAsync(c->d_walls, c->h_walls, sizeof(Wall) * c->nwalls, hipMemcpyHostToDevice,
                          c->stream));
+  HIP_TRY(hipMemcpyAsync(c->d_wall_shapes, c->h_wall_shapes, sizeof(CShape) * c->nwalls,
+                         hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->d_wall_sc, c->h_wall_sc, sizeof(double4) * c->nwalls,
+                         hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return GDEM_OK;
 }
@@ -365,54 +640,59 @@ static std::pair<hipEvent_t, hipEvent_t>* next_event_pair(gdem_ctx* c) {
   return &c->ev_pool[c->ev_used++];
 }
 
-static int do_force(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft) {
-  if (!c->binned) {
-    int rc = do_binning(c);
-    if (rc) return rc;
-  }
-  const int nblk = (int)((c->n + 255) / 256);
-  const int nblk8 = ((nblk + 7) / 8) * 8;
-  DetectArgs d{};
-  d.pos = c->bins.pos;
-  // Pre-filter margin: coordinates lie within [lo, lo + L] (a particle outside
-  // a non-periodic grid is a SimulationError), a shifted image within L more.
-  // Rounding each to float moves it by at most 2^-24 of its magnitude; 2^-19 of
-  // (max |coordinate| + max L) bounds the error of the float distance test by
-  // a wide factor.
+// Pre-filter margin of the single-precision distance test: coordinates lie
+// within [lo, lo + L] (a particle outside a non-periodic grid is a
+// SimulationError), a shifted image within L more.  Rounding each to float
+// moves it by at most 2^-24 of its magnitude; 2^-19 of (max |coordinate| +
+// max L) bounds the error of the float distance test by a wide factor.
+static float prefilter_margin(const Grid& g, float* sizef) {
   double mag = 0., lmax = 0.;
   for (int dd = 0; dd < 3; ++dd) {
-    mag = std::max(mag, std::max(std::fabs(c->grid.origin[dd]),
-                                 std::fabs(c->grid.origin[dd] + c->grid.size[dd])));
-    lmax = std::max(lmax, c->grid.size[dd]);
-    d.sizef[dd] = (float)c->grid.size[dd];
+    mag = std::max(mag, std::max(std::fabs(g.origin[dd]), std::fabs(g.origin[dd] + g.size[dd])));
+    lmax = std::max(lmax, g.size[dd]);
+    sizef[dd] = (float)g.size[dd];
   }
-  d.margin = (float)(std::ldexp(1.0, -19) * (mag + lmax));
+  return (float)(std::ldexp(1.0, -19) * (mag + lmax));
+}
+
+static void launch_convex_step(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft,
+                               bool record, int nblk8) {
+  CDetectArgs d{};
+  d.pos = c->bins.pos;
+  d.margin = prefilter_margin(c->grid, d.sizef);
   d.in = c->buf[c->cur];
+  d.cin = c->cbuf[c->cur];
+  d.shapes = c->d_shapes;
   d.perm = c->bins.perm;
   d.cell_start = c->bins.cell_start;
   d.row_rank = c->d_row_rank;
   d.grid = c->grid;
   d.n = (int32_t)c->n;
-  d.out_list = c->d_out_list;
+  d.out_rec = c->d_out_rec;
   d.out_count = c->d_out_count;
   d.in_list = c->d_in_list;
   d.in_count = c->d_in_count;
   d.err = c->d_err;
   d.nblk8 = nblk8;
-  ForceArgs a{};
+  CForceArgs a{};
   a.in = c->buf[c->cur];
   a.out = c->buf[1 - c->cur];
+  a.cin = c->cbuf[c->cur];
+  a.cout = c->cbuf[1 - c->cur];
+  a.shapes = c->d_shapes;
   a.perm = c->bins.perm;
   a.row_rank = c->d_row_rank;
   a.grid = c->grid;
   a.models = c->d_models;
   a.walls = c->d_walls;
+  a.wall_shapes = c->d_wall_shapes;
+  a.wall_sc = c->d_wall_sc;
   a.nwalls = c->nwalls;
   a.n = (int32_t)c->n;
   for (int dd = 0; dd < 3; ++dd) a.gravity[dd] = c->cfg.gravity[dd];
   a.dt_vel = dtv;
   a.dt_disp = dtd;
-  a.out_list = c->d_out_list;
+  a.out_rec = c->d_out_rec;
   a.out_count = c->d_out_count;
   a.in_list = c->d_in_list;
   a.in_count = c->d_in_count;
@@ -421,18 +701,79 @@ static int do_force(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft
   a.key_out = c->bins.key;
   a.err = c->d_err;
   a.contacts = c->d_counters;
-  const bool record = c->rec_cap > 0 && c->d_rec != nullptr;
   a.rec = c->d_rec;
   a.rec_count = c->d_rec_count;
   a.rec_cap = c->rec_cap;
   a.nblk8 = nblk8;
-  launch_detect(d, c->stream);
-  std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
-  if (c->timing) {
-    ev = next_event_pair(c);
-    if (ev) HIP_TRY(hipEventRecord(ev->first, c->stream));
+  launch_cdetect(d, c->stream);
+  launch_cforce(mode, record, a, c->stream);
+}
+
+static int do_force(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft) {
+  if (!c->binned) {
+    int rc = do_binning(c);
+    if (rc) return rc;
   }
-  launch_force(mode, record, a, c->stream);
+  const int nblk = (int)((c->n + 255) / 256);
+  const int nblk8 = ((nblk + 7) / 8) * 8;
+  const bool record = c->rec_cap > 0 && c->d_rec != nullptr;
+  std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
+  if (c->convex) {
+    // events bracket detection + force: the GJK narrow phase runs in detection
+    if (c->timing) {
+      ev = next_event_pair(c);
+      if (ev) HIP_TRY(hipEventRecord(ev->first, c->stream));
+    }
+    launch_convex_step(c, mode, dtv, dtd, write_ft, record, nblk8);
+  } else {
+    DetectArgs d{};
+    d.pos = c->bins.pos;
+    d.margin = prefilter_margin(c->grid, d.sizef);
+    d.in = c->buf[c->cur];
+    d.perm = c->bins.perm;
+    d.cell_start = c->bins.cell_start;
+    d.row_rank = c->d_row_rank;
+    d.grid = c->grid;
+    d.n = (int32_t)c->n;
+    d.out_list = c->d_out_list;
+    d.out_count = c->d_out_count;
+    d.in_list = c->d_in_list;
+    d.in_count = c->d_in_count;
+    d.err = c->d_err;
+    d.nblk8 = nblk8;
+    ForceArgs a{};
+    a.in = c->buf[c->cur];
+    a.out = c->buf[1 - c->cur];
+    a.perm = c->bins.perm;
+    a.row_rank = c->d_row_rank;
+    a.grid = c->grid;
+    a.models = c->d_models;
+    a.walls = c->d_walls;
+    a.nwalls = c->nwalls;
+    a.n = (int32_t)c->n;
+    for (int dd = 0; dd < 3; ++dd) a.gravity[dd] = c->cfg.gravity[dd];
+    a.dt_vel = dtv;
+    a.dt_disp = dtd;
+    a.out_list = c->d_out_list;
+    a.out_count = c->d_out_count;
+    a.in_list = c->d_in_list;
+    a.in_count = c->d_in_count;
+    a.F = write_ft ? c->d_F : nullptr;
+    a.T = write_ft ? c->d_T : nullptr;
+    a.key_out = c->bins.key;
+    a.err = c->d_err;
+    a.contacts = c->d_counters;
+    a.rec = c->d_rec;
+    a.rec_count = c->d_rec_count;
+    a.rec_cap = c->rec_cap;
+    a.nblk8 = nblk8;
+    launch_detect(d, c->stream);
+    if (c->timing) {
+      ev = next_event_pair(c);
+      if (ev) HIP_TRY(hipEventRecord(ev->first, c->stream));
+    }
+    launch_force(mode, record, a, c->stream);
+  }
   HIP_TRY(hipGetLastError());
   if (ev) HIP_TRY(hipEventRecord(ev->second, c->stream));
   c->cur = 1 - c->cur;
@@ -457,8 +798,12 @@ extern "C" int gdem_compute_forces(gdem_ctx* c) {
 extern "C" int gdem_move(gdem_ctx* c, double dtv, double dtd) {
   if (!c) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
-  launch_move(c->buf[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid, c->d_row_rank, dtv, dtd,
-              c->bins.key, c->d_err, c->stream);
+  if (c->convex)
+    launch_cmove(c->buf[c->cur], c->cbuf[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid,
+                 c->d_row_rank, dtv, dtd, c->bins.key, c->d_err, c->stream);
+  else
+    launch_move(c->buf[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid, c->d_row_rank, dtv, dtd,
+                c->bins.key, c->d_err, c->stream);
   HIP_TRY(hipGetLastError());
   c->binned = false;
   return GDEM_OK;
@@ -467,7 +812,11 @@ extern "C" int gdem_move(gdem_ctx* c, double dtv, double dtd) {
 extern "C" int gdem_advance_velocity(gdem_ctx* c, double dtv) {
   if (!c) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
-  launch_advance(c->buf[c->cur], c->d_F, c->d_T, (int32_t)c->n, dtv, c->stream);
+  if (c->convex)
+    launch_cadvance(c->buf[c->cur], c->cbuf[c->cur], c->d_F, c->d_T, (int32_t)c->n, dtv,
+                    c->stream);
+  else
+    launch_advance(c->buf[c->cur], c->d_F, c->d_T, (int32_t)c->n, dtv, c->stream);
   HIP_TRY(hipGetLastError());
   return GDEM_OK;
 }

@@ -1,0 +1,355 @@
+// gdem_convex.h -- convex shapes, crust-shrunk transforms and the Johnson GJK
+// distance (FP64, reference operand order) for device and host.
+//
+// Paths are relative to src/Grains3D/Grains/.  Polyhedron::support hill-climbs
+// from a vertex it keeps between calls (Polyhedron.cpp:265-300); here each call
+// starts from the shape's first vertex with neighbours, so results do not
+// depend on call history (they differ only when two vertices tie within 1e-16).
+#pragma once
+#include "gdem_device.h"
+
+#pragma clang fp contract(off)
+
+namespace gdem {
+
+constexpr int kMaxShapes = 16;
+constexpr int kMaxVerts = 8;
+enum { SHAPE_SPHERE = 0, SHAPE_BOX = 1, SHAPE_POLYHEDRON = 2 };
+
+struct CShape {
+  int32_t type, nv, first_vertex, pad;
+  double ext[3];  // box half extents
+  double v[kMaxVerts][3];
+  int32_t cob_n[kMaxVerts];
+  int32_t cob[kMaxVerts][kMaxVerts];
+  double circ;         // circumscribed radius
+  double bbox_ext[3];  // half extents of the bounding box at identity
+  double vol;
+  double inertia[6];
+};
+
+// Box::support (Geometry/src/Box.cpp:307-316)
+__host__ __device__ inline v3 support_box(const CShape& sh, v3 v) {
+  double nrm = norm(v);
+  if (nrm < kEps) return mk(0., 0., 0.);
+  return mk(v.x < 0. ? -sh.ext[0] : sh.ext[0], v.y < 0. ? -sh.ext[1] : sh.ext[1],
+            v.z < 0. ? -sh.ext[2] : sh.ext[2]);
+}
+
+// Polyhedron::support (Polyhedron.cpp:265-300)
+__host__ __device__ inline v3 support_poly(const CShape& sh, v3 v) {
+  double nrm = norm(v);
+  if (!(nrm > kEps)) return mk(0., 0., 0.);
+  int curr = sh.first_vertex, last = -1;
+  double h = dot(mk(sh.v[curr][0], sh.v[curr][1], sh.v[curr][2]), v), d = 0.;
+  for (int guard = 0; guard < 64; ++guard) {
+    int i = 0;
+    const int n = sh.cob_n[curr];
+    while (i != n) {
+      const int c = sh.cob[curr][i];
+      if (c != last) {
+        d = dot(mk(sh.v[c][0], sh.v[c][1], sh.v[c][2]), v);
+        if (!(d - h < 1.e-16)) break;
+      }
+      ++i;
+    }
+    if (i == n) break;
+    last = curr;
+    curr = sh.cob[curr][i];
+    h = d;
+  }
+  return mk(sh.v[curr][0], sh.v[curr][1], sh.v[curr][2]);
+}
+
+// Sphere::support (Sphere.cpp:99-109), radius in ext[0]
+__host__ __device__ inline v3 support_sphere(const CShape& sh, v3 v) {
+  double s = norm(v);
+  if (s > kEps) {
+    double r = sh.ext[0] / s;
+    return mk(v.x * r, v.y * r, v.z * r);
+  }
+  return mk(0., 0., 0.);
+}
+
+__host__ __device__ inline v3 support(const CShape& sh, v3 v) {
+  if (sh.type == SHAPE_BOX) return support_box(sh, v);
+  if (sh.type == SHAPE_POLYHEDRON) return support_poly(sh, v);
+  return support_sphere(sh, v);
+}
+
+// Transform: row-major basis and origin (Transform.cpp:312-317)
+struct Xf {
+  double b[9];
+  double o[3];
+};
+
+__host__ __device__ inline v3 xf_apply(const Xf& t, v3 p) {
+  return mk(dot(mk(t.b[0], t.b[1], t.b[2]), p) + t.o[0],
+            dot(mk(t.b[3], t.b[4], t.b[5]), p) + t.o[1],
+            dot(mk(t.b[6], t.b[7], t.b[8]), p) + t.o[2]);
+}
+
+// Vector3 * Matrix (Matrix.cpp:527-530, Matrix::tdot :238-241)
+__host__ __device__ inline v3 vec_mat(v3 v, const double* m) {
+  return mk(m[0] * v.x + m[3] * v.y + m[6] * v.z, m[1] * v.x + m[4] * v.y + m[7] * v.z,
+            m[2] * v.x + m[5] * v.y + m[8] * v.z);
+}
+
+// getTransformWithCrust (RigidBodyWithCrust.cpp:749-760) with
+// Matrix::multiplyByScalingMatrix (Matrix.cpp:347-358)
+__host__ __device__ inline Xf crust_xf(const double* basis, v3 origin, const double* sc) {
+  Xf t;
+  for (int i = 0; i < 3; ++i) {
+    t.b[3 * i + 0] = basis[3 * i + 0] * sc[0];
+    t.b[3 * i + 1] = basis[3 * i + 1] * sc[1];
+    t.b[3 * i + 2] = basis[3 * i + 2] * sc[2];
+  }
+  t.o[0] = origin.x;
+  t.o[1] = origin.y;
+  t.o[2] = origin.z;
+  return t;
+}
+
+// ------------------------------------------------------------ Johnson GJK
+// Geometry/src/Convex.cpp:219-430 (sub-algorithm) and :536-627 (driver)
+struct Gjk {
+  v3 y[4];
+  double dp[4][4];
+  double det[16][4];
+};
+
+__host__ __device__ inline void gjk_compute_det(unsigned bits, unsigned last, unsigned last_bit,
+                                                unsigned all_bits, Gjk& g) {
+  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1)
+    if (bits & bit) g.dp[i][last] = g.dp[last][i] = dot(g.y[i], g.y[last]);
+  g.dp[last][last] = dot(g.y[last], g.y[last]);
+  g.det[last_bit][last] = 1.;
+  for (unsigned j = 0, sj = 1; j < 4; ++j, sj <<= 1) {
+    if (bits & sj) {
+      unsigned s2 = sj | last_bit;
+      g.det[s2][j] = g.dp[last][last] - g.dp[last][j];
+      g.det[s2][last] = g.dp[j][j] - g.dp[j][last];
+      for (unsigned k = 0, sk = 1; k < j; ++k, sk <<= 1) {
+        if (bits & sk) {
+          unsigned s3 = sk | s2;
+          g.det[s3][k] = g.det[s2][j] * (g.dp[j][j] - g.dp[j][k]) +
+                         g.det[s2][last] * (g.dp[last][j] - g.dp[last][k]);
+          g.det[s3][j] = g.det[sk | last_bit][k] * (g.dp[k][k] - g.dp[k][j]) +
+                         g.det[sk | last_bit][last] * (g.dp[last][k] - g.dp[last][j]);
+          g.det[s3][last] = g.det[sk | sj][k] * (g.dp[k][k] - g.dp[k][last]) +
+                            g.det[sk | sj][j] * (g.dp[j][k] - g.dp[j][last]);
+        }
+      }
+    }
+  }
+  if (all_bits == 15) {
+    g.det[15][0] = g.det[14][1] * (g.dp[1][1] - g.dp[1][0]) + g.det[14][2] * (g.dp[2][1] - g.dp[2][0]) +
+                   g.det[14][3] * (g.dp[3][1] - g.dp[3][0]);
+    g.det[15][1] = g.det[13][0] * (g.dp[0][0] - g.dp[0][1]) + g.det[13][2] * (g.dp[2][0] - g.dp[2][1]) +
+                   g.det[13][3] * (g.dp[3][0] - g.dp[3][1]);
+    g.det[15][2] = g.det[11][0] * (g.dp[0][0] - g.dp[0][2]) + g.det[11][1] * (g.dp[1][0] - g.dp[1][2]) +
+                   g.det[11][3] * (g.dp[3][0] - g.dp[3][2]);
+    g.det[15][3] = g.det[7][0] * (g.dp[0][0] - g.dp[0][3]) + g.det[7][1] * (g.dp[1][0] - g.dp[1][3]) +
+                   g.det[7][2] * (g.dp[2][0] - g.dp[2][3]);
+  }
+}
+
+__host__ __device__ inline bool gjk_valid(unsigned s, unsigned all_bits, const Gjk& g) {
+  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1) {
+    if (all_bits & bit) {
+      if (s & bit) {
+        if (g.det[s][i] <= 1.0e-20) return false;  // EPSILON3
+      } else if (g.det[s | bit][i] > 0) {
+        return false;
+      }
+    }
+  }
+  return true;
+}
+
+__host__ __device__ inline v3 gjk_vector(unsigned bits, const Gjk& g) {
+  double sum = 0.;
+  v3 v = mk(0., 0., 0.);
+  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1) {
+    if (bits & bit) {
+      sum += g.det[bits][i];
+      v = add(v, scl(g.y[i], g.det[bits][i]));
+    }
+  }
+  return scl(v, 1. / sum);
+}
+
+__host__ __device__ inline bool gjk_proper(unsigned s, const Gjk& g) {
+  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1)
+    if ((s & bit) && g.det[s][i] <= 1.0e-20) return false;
+  return true;
+}
+
+__host__ __device__ inline bool gjk_closest(unsigned& bits, unsigned last, unsigned last_bit,
+                                            unsigned all_bits, Gjk& g, v3& v) {
+  unsigned s;
+  gjk_compute_det(bits, last, last_bit, all_bits, g);
+  for (s = bits; s; --s) {
+    if ((s & bits) == s) {
+      if (gjk_valid(s | last_bit, all_bits, g)) {
+        bits = s | last_bit;
+        v = gjk_vector(bits, g);
+        return true;
+      }
+    }
+  }
+  if (gjk_valid(last_bit, all_bits, g)) {
+    bits = last_bit;
+    v = g.y[last];
+    return true;
+  }
+  double min_dist2 = __builtin_huge_val();
+  for (s = all_bits; s; --s) {
+    if ((s & all_bits) == s) {
+      if (gjk_proper(s, g)) {
+        v3 u = gjk_vector(s, g);
+        double dist2 = dot(u, u);
+        if (dist2 < min_dist2) {
+          min_dist2 = dist2;
+          bits = s;
+          v = u;
+        }
+      }
+    }
+  }
+  return false;
+}
+
+__host__ __device__ inline bool gjk_degenerate(unsigned all_bits, const v3* y, v3 w) {
+  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1)
+    if ((all_bits & bit) && y[i].x == w.x && y[i].y == w.y && y[i].z == w.z) return true;
+  return false;
+}
+
+// closest_points (Convex.cpp:536-627): tolerance EPSILON (GrainsExec.cpp:51),
+// no acceleration.  Returns the distance; pa, pb are the local witness points.
+__host__ __device__ inline double closest_points(const CShape& a, const CShape& b, const Xf& a2w,
+                                                 const Xf& b2w, v3& pa, v3& pb, int& nbIter) {
+  unsigned bits = 0, last = 0, last_bit = 0, all_bits = 0;
+  v3 p[4], q[4];
+  Gjk g;
+  for (int i = 0; i < 4; ++i) {
+    p[i] = q[i] = g.y[i] = mk(0., 0., 0.);
+    for (int j = 0; j < 4; ++j) g.dp[i][j] = 0.;
+  }
+  for (int s = 0; s < 16; ++s)
+    for (int j = 0; j < 4; ++j) g.det[s][j] = 0.;
+  double mu = 0.;
+  int numIterations = 0;
+  const double relError = kEps;
+  const double absError = 1.e-4 * relError;
+  const v3 zero = mk(0., 0., 0.);
+  v3 v = sub(xf_apply(a2w, support(a, zero)), xf_apply(b2w, support(b, zero)));
+  v3 w = v;
+  double dist = norm(v);
+  while (bits < 15 && dist > 1.0e-15 && numIterations < 1000) {
+    ++numIterations;
+    last = 0;
+    last_bit = 1;
+    while (bits & last_bit) {
+      ++last;
+      last_bit <<= 1;
+    }
+    const v3 d = v;
+    p[last] = support(a, vec_mat(neg(d), a2w.b));
+    q[last] = support(b, vec_mat(d, b2w.b));
+    w = sub(xf_apply(a2w, p[last]), xf_apply(b2w, q[last]));
+    mu = dist - dot(v, w) / dist;
+    if (mu < dist * relError || mu < absError) break;
+    if (gjk_degenerate(all_bits, g.y, w)) break;
+    g.y[last] = w;
+    all_bits = bits | last_bit;
+    if (!gjk_closest(bits, last, last_bit, all_bits, g, v)) break;
+    dist = norm(v);
+  }
+  // computePoints (Convex.cpp:314-337)
+  double sum = 0.;
+  pa = mk(0., 0., 0.);
+  pb = mk(0., 0., 0.);
+  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1) {
+    if (bits & bit) {
+      sum += g.det[bits][i];
+      pa = add(pa, scl(p[i], g.det[bits][i]));
+      pb = add(pb, scl(q[i], g.det[bits][i]));
+    }
+  }
+  const double s = 1. / sum;
+  pa = scl(pa, s);
+  pb = scl(pb, s);
+  nbIter = numIterations;
+  return dist;
+}
+
+// A body as seen by the narrow phase
+struct CBody {
+  const CShape* sh;
+  const double* basis;  // row-major
+  v3 origin;
+  const double* scale;  // crust scaling per axis
+  double crust, circ;
+};
+
+// RigidBodyWithCrust::ClosestPoint general case (RigidBodyWithCrust.cpp:274-348).
+// 1 = candidate (dist may be >= 0), 0 = no bounding-volume overlap,
+// -1 = ContactError (shrunk bodies overlap).
+__host__ __device__ inline int closest_point(const CBody& A, const CBody& B, v3& contact, v3& ov,
+                                             double& dist) {
+  const v3 gab = sub(A.origin, B.origin);
+  if (!(norm(gab) < A.circ + B.circ)) return 0;
+  const Xf a2w = crust_xf(A.basis, A.origin, A.scale);
+  const Xf b2w = crust_xf(B.basis, B.origin, B.scale);
+  v3 pa, pb;
+  int nit = 0;
+  double distance = closest_points(*A.sh, *B.sh, a2w, b2w, pa, pb, nit);
+  if (distance < kEps) return -1;
+  pa = xf_apply(a2w, pa);
+  pb = xf_apply(b2w, pb);
+  const v3 ba = sub(pa, pb);
+  contact = add(dvd(pa, 2.0), dvd(pb, 2.0));
+  v3 o = dvd(ba, distance);
+  o = round3(o);
+  ov = scl(o, A.crust + B.crust - distance);
+  distance -= A.crust + B.crust;
+  dist = distance;
+  return 1;
+}
+
+// ParticleKinematics3D::computeAngularAccelerationBodyFixed
+// (Component/src/ParticleKinematics3D.cpp:56-77), predictor mode
+__host__ __device__ inline v3 ang_acc3(v3 tq, v3 om, double I0, double I1, double I2) {
+  return mk(tq.x / (1. * I0) + om.y * om.z * (I1 - I2) / I0,
+            tq.y / (1. * I1) + om.z * om.x * (I2 - I0) / I1,
+            tq.z / (1. * I2) + om.x * om.y * (I0 - I1) / I2);
+}
+
+// SecondOrderLeapFrog SSPRK3 (TimeIntegration/src/SecondOrderLeapFrog.cpp:56-66)
+__host__ __device__ inline v3 ssprk3_3d(v3 tq, v3 wbf, double I0, double I1, double I2,
+                                        double dt) {
+  v3 K1 = scl(ang_acc3(tq, wbf, I0, I1, I2), dt);
+  v3 K2 = scl(ang_acc3(tq, add(wbf, K1), I0, I1, I2), dt);
+  v3 K3 = scl(ang_acc3(tq, add(wbf, scl(add(K1, K2), 0.25)), I0, I1, I2), dt);
+  return add(wbf, dvd(add(add(K1, K2), scl(K3, 4.)), 6.));
+}
+
+// Transform::composeLeftByRotation (Geometry/src/Transform.cpp:140-156)
+__host__ __device__ inline void compose_left_rotation(double* mm, quat dq) {
+  const double qx = dq.v.x, qy = dq.v.y, qz = dq.v.z, qw = dq.w;
+  double nb[9];
+  for (int i = 0; i < 3; ++i) {
+    double px = qy * mm[6 + i] - qz * mm[3 + i] + qw * mm[0 + i];
+    double py = qz * mm[0 + i] - qx * mm[6 + i] + qw * mm[3 + i];
+    double pz = qx * mm[3 + i] - qy * mm[0 + i] + qw * mm[6 + i];
+    double pw = -qx * mm[0 + i] - qy * mm[3 + i] - qz * mm[6 + i];
+    nb[0 + i] = qy * pz - qz * py - pw * qx + qw * px;
+    nb[3 + i] = qz * px - qx * pz - pw * qy + qw * py;
+    nb[6 + i] = qx * py - qy * px - pw * qz + qw * pz;
+  }
+  for (int k = 0; k < 9; ++k) mm[k] = nb[k];
+}
+
+}  // namespace gdem

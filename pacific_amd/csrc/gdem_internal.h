@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "gdem_convex.h"
 #include "gdem_device.h"
 
 namespace gdem {
@@ -98,6 +99,86 @@ struct ForceArgs {
 };
 
 enum { MODE_FORCE = 0, MODE_KICK = 1, MODE_MOVE = 2 };
+
+// ---- convex particles
+// Extra per-particle state: basis rows (row-major) with the crust scaling of
+// that axis in .w, and the principal inertia.  aux.mat holds material + 256 *
+// shape.
+struct CState {
+  double4* rot0;
+  double4* rot1;
+  double4* rot2;
+  double4* inert;  // I0, I3, I5, 0
+};
+
+// A contact found by the detection pass, stored once in the detecting
+// particle's list: partner slot, code (shift to the partner's image, whether
+// the detecting particle is the first body), contact point, overlap vector
+// and overlap distance of the pair with A = first body.
+struct CRec {
+  int32_t partner, code;
+  double c[3], ov[3], dist;
+};
+constexpr int kCListCap = 16;
+
+struct CDetectArgs {
+  const float4* pos;
+  State in;
+  CState cin;
+  const CShape* shapes;
+  const int32_t* perm;
+  const int32_t* cell_start;
+  const int32_t* row_rank;
+  Grid grid;
+  float sizef[3];
+  float margin;
+  int32_t n;
+  CRec* out_rec;       // [n][kCListCap]
+  int32_t* out_count;  // [n]
+  int2* in_list;       // [n][kCListCap]: (detecting slot, index in its list)
+  int32_t* in_count;   // [n], zero on entry
+  unsigned long long* err;
+  int32_t nblk8;
+};
+
+struct CForceArgs {
+  State in, out;
+  CState cin, cout;
+  const CShape* shapes;
+  const int32_t* perm;
+  const int32_t* row_rank;
+  Grid grid;
+  const Model* models;
+  const Wall* walls;
+  const CShape* wall_shapes;  // box shape of each wall
+  const double4* wall_sc;     // crust scaling per axis, circumscribed radius
+  int32_t nwalls;
+  int32_t n;
+  double gravity[3];
+  double dt_vel, dt_disp;
+  const CRec* out_rec;
+  const int32_t* out_count;
+  const int2* in_list;
+  int32_t* in_count;
+  double4* F;
+  double4* T;
+  int32_t* key_out;
+  unsigned long long* err;
+  unsigned long long* contacts;
+  ContactRec* rec;
+  int32_t* rec_count;
+  int32_t rec_cap;
+  int32_t nblk8;
+};
+
+void launch_cdetect(const CDetectArgs& a, hipStream_t st);
+void launch_cforce(int mode, bool record, const CForceArgs& a, hipStream_t st);
+void launch_cmove(const State& s, const CState& c, const double4* F, const double4* T, int32_t n,
+                  const Grid& g, const int32_t* row_rank, double dt_vel, double dt_disp,
+                  int32_t* key, unsigned long long* err, hipStream_t st);
+void launch_cadvance(const State& s, const CState& c, const double4* F, const double4* T,
+                     int32_t n, double dt_vel, hipStream_t st);
+void launch_cunpermute(const State& s, const CState& c, int32_t n, double* basis, hipStream_t st);
 
 struct Bins {
   int32_t* key;         // cell key of each particle (storage order)

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gdem_internal.h"
+#include "gdem_sweep.h"
 
 #pragma clang fp contract(off)
 
@@ -24,57 +25,6 @@ namespace gdem {
 
 constexpr int kBlock = 256;
 constexpr int kScanBlock = 1024;
-
-__device__ __forceinline__ unsigned long long pack_ids(int32_t a, int32_t b) {
-  return ((unsigned long long)(uint32_t)a << 32) | (unsigned long long)(uint32_t)b;
-}
-
-__device__ __forceinline__ int pack_code(int sx, int sy, int sz, bool first) {
-  return (sx + 1) | ((sy + 1) << 2) | ((sz + 1) << 4) | (first ? 64 : 0);
-}
-
-__device__ __forceinline__ int mirror_code(int code) {
-  return pack_code(1 - (code & 3), 1 - ((code >> 2) & 3), 1 - ((code >> 4) & 3), !(code & 64));
-}
-
-// Cell::GetCell (Base/src/Cell.cpp:172-184); periodic indices are clamped to
-// the grid after the wrap of the position.  Returns false outside the grid.
-__device__ __forceinline__ bool cell_of(const Grid& g, v3 x, int* ijk) {
-  const double p[3] = {x.x, x.y, x.z};
-  bool ok = true;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    double t = (p[d] - g.origin[d]) / g.edge[d];
-    int id;
-    if (!(t == t) || t > 2.0e9 || t < -2.0e9) {
-      ok = false;
-      id = 0;
-    } else {
-      id = (int)floor(t);
-      if (g.periodic[d]) {
-        if (id >= g.nc[d]) id = g.nc[d] - 1;
-        if (id < 0) id = 0;
-      } else if (id < 0 || id >= g.nc[d]) {
-        ok = false;
-      }
-    }
-    ijk[d] = id;
-  }
-  return ok;
-}
-
-__device__ __forceinline__ int row_base(const Grid& g, const int32_t* row_rank, int j, int k) {
-  return row_rank[j * g.nc[2] + k] * g.nc[0];
-}
-
-__device__ __forceinline__ int logical_index(int nblk8) {
-  // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
-  // contiguous range of cells so that neighbour reads stay in its own L2.
-  const int b = blockIdx.x;
-  const int per = nblk8 >> 3;
-  const int lb = (b & 7) * per + (b >> 3);
-  return lb * blockDim.x + threadIdx.x;
-}
 
 // ---------------------------------------------------------------- keys
 __global__ void __launch_bounds__(kBlock) k_keys(State s, int n, Grid g, const int32_t* row_rank,
@@ -303,42 +253,9 @@ __global__ void __launch_bounds__(kBlock) k_detect(DetectArgs a) {
   // the cell index comes from the full-precision position, as in the binning
   const double4 me = a.in.pos_r[a.perm[t]];
   int ijk[3];
-  cell_of(a.grid, mk(me.x, me.y, me.z), ijk);
-#pragma unroll
-  for (int d = 0; d < 3; ++d) ijk[d] = min(max(ijk[d], 0), a.grid.nc[d] - 1);
-  const int i = ijk[0], j = ijk[1], k = ijk[2];
-  const int ni = a.grid.nc[0];
-  const bool px = a.grid.periodic[0] != 0;
+  clamped_cell(a.grid, mk(me.x, me.y, me.z), ijk);
   Detector det(a, t, a.pos[t]);
-  const int32_t* cs = a.cell_start;
-  // own row: the rest of this cell (same cell, after self), then cell i+1
-  {
-    const int rb = row_base(a.grid, a.row_rank, j, k);
-    det.range(t + 1, cs[rb + i + 1], 0, 0, 0, true);
-    if (i + 1 < ni) det.range(cs[rb + i + 1], cs[rb + i + 2], 0, 0, 0, false);
-    else if (px) det.range(cs[rb], cs[rb + 1], 1, 0, 0, false);
-  }
-  // rows (j, k+1), (j+1, k-1), (j+1, k), (j+1, k+1): cells i-1 .. i+1
-  const int drow[4][2] = {{0, 1}, {1, -1}, {1, 0}, {1, 1}};
-  for (int q = 0; q < 4; ++q) {
-    int jj = j + drow[q][0], kk = k + drow[q][1];
-    int sy = 0, sz = 0;
-    if (jj >= a.grid.nc[1]) {
-      if (!a.grid.periodic[1]) continue;
-      jj -= a.grid.nc[1];
-      sy = 1;
-    }
-    if (kk < 0 || kk >= a.grid.nc[2]) {
-      if (!a.grid.periodic[2]) continue;
-      if (kk < 0) { kk += a.grid.nc[2]; sz = -1; }
-      else { kk -= a.grid.nc[2]; sz = 1; }
-    }
-    const int rb = row_base(a.grid, a.row_rank, jj, kk);
-    const int lo = max(i - 1, 0), hi = min(i + 1, ni - 1);
-    if (px && i - 1 < 0) det.range(cs[rb + ni - 1], cs[rb + ni], -1, sy, sz, false);
-    det.range(cs[rb + lo], cs[rb + hi + 1], 0, sy, sz, false);
-    if (px && i + 1 >= ni) det.range(cs[rb], cs[rb + 1], 1, sy, sz, false);
-  }
+  half_stencil_sweep(a.grid, a.row_rank, a.cell_start, t, ijk, det);
   a.out_count[t] = det.nout;
 }
 

@@ -92,6 +92,10 @@ def load_library(path: str = LIB_PATH):
         "gdem_reset_counters": (C.c_int, [vp]),
         "gdem_get_grid": (C.c_int, [vp, P(i32), P(d)]),
         "gdem_version": (C.c_char_p, []),
+        "gdem_add_box_shape": (C.c_int, [vp, vp, P(i32)]),
+        "gdem_add_polyhedron_shape": (C.c_int, [vp, i32, vp, i32, vp, vp, P(i32)]),
+        "gdem_set_convex_particles": (C.c_int, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "gdem_get_basis": (C.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -106,7 +110,9 @@ EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall"
             "gdem_move", "gdem_advance_velocity", "gdem_prime", "gdem_step",
             "gdem_synchronize", "gdem_get_state", "gdem_set_contact_recording",
             "gdem_get_contacts", "gdem_last_error", "gdem_set_kernel_timing",
-            "gdem_get_counters", "gdem_reset_counters", "gdem_get_grid", "gdem_version"]
+            "gdem_get_counters", "gdem_reset_counters", "gdem_get_grid", "gdem_version",
+            "gdem_add_box_shape", "gdem_add_polyhedron_shape", "gdem_set_convex_particles",
+            "gdem_get_basis"]
 
 
 def _ptr(a):
@@ -167,6 +173,44 @@ class GdemSystem:
         self._check(self.lib.gdem_set_particles(self.h, n, *[_ptr(a) for a in self._keep],
                                                 _ptr(mat)), "gdem_set_particles")
         self.n = n
+
+    def add_box_shape(self, half_extent):
+        """Box with the given half extents; returns its shape id."""
+        sid = C.c_int32()
+        self._check(self.lib.gdem_add_box_shape(self.h, _ptr(_f64(half_extent, (3,))), C.byref(sid)),
+                    "gdem_add_box_shape")
+        return sid.value
+
+    def add_polyhedron_shape(self, vertices, faces):
+        """Polyhedron from vertices (about the centre of mass) and faces (lists
+        of vertex indices, each loop oriented outwards); returns its shape id."""
+        v = _f64(vertices, (-1, 3))
+        size = np.array([len(f) for f in faces], np.int32)
+        idx = np.array([i for f in faces for i in f], np.int32)
+        sid = C.c_int32()
+        self._check(self.lib.gdem_add_polyhedron_shape(self.h, len(v), _ptr(v), len(size), _ptr(size),
+                                                       _ptr(idx), C.byref(sid)),
+                    "gdem_add_polyhedron_shape")
+        return sid.value
+
+    def set_convex_particles(self, x, v, w, q, basis, shape, density, crust, material=None):
+        n = len(shape)
+        self._keep = [_f64(x, (n, 3)), _f64(v, (n, 3)), _f64(w, (n, 3)), _f64(q, (n, 4)),
+                      _f64(basis, (n, 9))]
+        sh = np.ascontiguousarray(shape, dtype=np.int32)
+        rest = [_f64(density, (n,)), _f64(crust, (n,))]
+        self._keep += [sh] + rest
+        mat = None if material is None else np.ascontiguousarray(material, dtype=np.int32)
+        self._check(self.lib.gdem_set_convex_particles(
+            self.h, n, *[_ptr(a) for a in self._keep[:5]], _ptr(sh), *[_ptr(a) for a in rest],
+            _ptr(mat)), "gdem_set_convex_particles")
+        self.n = n
+
+    def get_basis(self):
+        """Row-major rotation of every convex particle, (n, 9)."""
+        b = np.empty((self.n, 9))
+        self._check(self.lib.gdem_get_basis(self.h, _ptr(b)), "gdem_get_basis")
+        return b
 
     def add_wall(self, basis, centre, half_extent, crust, material):
         self._check(self.lib.gdem_add_wall(self.h, _ptr(_f64(basis, (9,))), _ptr(_f64(centre, (3,))),
@@ -257,6 +301,17 @@ def build_system(wl, device=0):
     for wdef in wl["walls"]:
         s.add_wall(*wdef)
     p = wl["particles"]
-    s.set_particles(p["x"], p["v"], p["w"], p["q"], p["radius"], p["density"], p["crust"],
-                    p["material"])
+    if "shapes" in wl:
+        for sh in wl["shapes"]:
+            if sh[0] == "box":
+                s.add_box_shape(sh[1])
+            elif sh[0] == "polyhedron":
+                s.add_polyhedron_shape(sh[1], sh[2])
+            else:
+                raise ValueError(f"unknown shape {sh[0]}")
+        s.set_convex_particles(p["x"], p["v"], p["w"], p["q"], p["basis"], p["shape"],
+                               p["density"], p["crust"], p["material"])
+    else:
+        s.set_particles(p["x"], p["v"], p["w"], p["q"], p["radius"], p["density"], p["crust"],
+                        p["material"])
     return s

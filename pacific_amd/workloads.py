@@ -154,7 +154,8 @@ def tetrahedron_vertices(edge=TET_EDGE):
                      [0.0, 0.0, rcirc]])
 
 
-def convex_box(n=200_000, seed=3, steps=300, phi=0.20, height=0.20, overlap=0.0, cell_factor=1.0):
+def convex_box(n=200_000, seed=3, steps=300, phi=0.20, height=0.20, overlap=0.0, cell_factor=1.0,
+               sigma_v=0.0):
     """n convex particles, alternating cubes (edge 2 mm) and regular tetrahedra
     (edge 2.5 mm), uniform random orientations (Shoemake), random sequential
     addition (exact separating-axis overlap test) at solid fraction phi in a
@@ -179,7 +180,7 @@ def convex_box(n=200_000, seed=3, steps=300, phi=0.20, height=0.20, overlap=0.0,
     faces = np.array(TET_FACES, np.int32).reshape(-1)
     rc = _lib_convex().gdw_rsa_convex(n, _p(types), _p(np.array(CUBE_HALF)),
                                       _p(np.ascontiguousarray(tv)), _p(faces), _p(lo), _p(hi),
-                                      -overlap, 0.0, seed, 2000 * n + 100000, _p(x), _p(v), _p(q),
+                                      -overlap, sigma_v, seed, 2000 * n + 100000, _p(x), _p(v), _p(q),
                                       _p(basis))
     if rc != 0:
         raise RuntimeError(f"random sequential addition stopped at particle {-1 - rc} of {n}")
