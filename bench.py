@@ -68,6 +68,28 @@ def cached_workload(n, seed):
     return wl
 
 
+def cached_convex(n, seed):
+    from pacific_amd import workloads
+    cache_dir = os.path.join(os.environ.get("TMPDIR", "/tmp"), "gdem_cache")
+    os.makedirs(cache_dir, exist_ok=True)
+    path = os.path.join(cache_dir, f"convex_box_{n}_s{seed}.npz")
+    wl = workloads.convex_box(64, seed)  # structure; arrays replaced below
+    if os.path.exists(path):
+        z = np.load(path)
+        wl["particles"] = {k[2:]: z[k] for k in z.files if k.startswith("p_")}
+        wl["size"] = tuple(z["size"])
+        wl["walls"] = workloads.box_walls((0.0, 0.0, 0.0), wl["size"], 1)
+    else:
+        log(f"generating {n} convex particles by random sequential addition (seed {seed})")
+        t = time.time()
+        wl = workloads.convex_box(n, seed)
+        log(f"generated in {time.time() - t:.1f} s")
+        np.savez(path, size=np.array(wl["size"]),
+                 **{"p_" + k: v for k, v in wl["particles"].items()})
+    wl["name"] = f"convex_box_{n}"
+    return wl
+
+
 def read_traffic():
     if not os.path.exists(TRAFFIC_FILE):
         return None
@@ -80,24 +102,27 @@ def read_traffic():
 def cpu_baseline(wl, steps, warm=2):
     """The oracle (restatement of the reference CPU step), one thread."""
     from oracle import py_oracle
-    o = py_oracle.Oracle(wl)
+    o = py_oracle.ConvexOracle(wl) if "shapes" in wl else py_oracle.Oracle(wl)
     o.prime()
     o.step(warm, wl["dt"])
     t = time.perf_counter()
     o.step(steps, wl["dt"])
     el = time.perf_counter() - t
-    n = len(wl["particles"]["radius"])
+    n = len(wl["particles"]["x"])
     return n * steps / el, el
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None, help="500 (spheres) / 300 (convex)")
+    ap.add_argument("--warmup", type=int, default=None, help="20 (spheres) / 10 (convex)")
+    ap.add_argument("--workload", choices=["periodic_hertz", "convex_box"], default="periodic_hertz",
+                    help="periodic_hertz: 1M-sphere throughput case (default); convex_box: "
+                         "200k cubes + tetrahedra with GJK, a secondary line")
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--seed", type=int, default=None)
+    ap.add_argument("--cpu-steps", type=int, default=None, help="20 (spheres) / 5 (convex)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -106,12 +131,22 @@ def main():
         sys.exit(2)
 
     from pacific_amd import gdem
-    wl = cached_workload(args.n, args.seed)
-    n = len(wl["particles"]["radius"])
+    convex = args.workload == "convex_box"
+    if args.steps is None:
+        args.steps = 300 if convex else 500
+    if args.warmup is None:
+        args.warmup = 10 if convex else 20
+    if args.cpu_steps is None:
+        args.cpu_steps = 5 if convex else 20
+    if convex:
+        wl = cached_convex(args.n or 200_000, 3 if args.seed is None else args.seed)
+    else:
+        wl = cached_workload(args.n or 1_000_000, 2 if args.seed is None else args.seed)
+    n = len(wl["particles"]["x"])
     dt = wl["dt"]
     sysg = gdem.build_system(wl)
     nc, edge = sysg.grid()
-    log(f"grid {nc}, edge {edge[0]:.4e} m, L {wl['size'][0]:.5f} m")
+    log(f"grid {nc}, edge {edge[0]:.4e} m, size {tuple(round(v, 5) for v in wl['size'])} m")
     sysg.prime()
     sysg.synchronize()
     sysg.step(args.warmup, dt)
@@ -154,9 +189,23 @@ def main():
         cv, cel = cpu_baseline(wl, args.cpu_steps)
         log(f"CPU {cv:.4e} particle-steps/s ({cel:.1f} s)")
         cpu = {"value": cv, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{n} spheres of the same workload, {args.cpu_steps} steps after 2 "
+               "sample": f"{n} particles of the same workload, {args.cpu_steps} steps after 2 "
                          f"warm-up steps, single thread ({cel:.1f} s)"}
 
+    if convex:
+        out = {
+            "metric": "particle-steps/s at 200k convex particles (secondary line)", "value": value,
+            "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": wl["name"], "n_particles": n, "shapes": "cube 2 mm / tetrahedron 2.5 mm",
+                       "solid_fraction": 0.20, "contact": "Hooke+Coulomb, GJK", "dt": dt,
+                       "grid": nc, "parallelism": "single GPU"},
+            "detect_force_us": k_ms * 1e3, "mean_contacts": k_contacts,
+            "cpu_baseline": cpu,
+        }
+        print(json.dumps(out), flush=True)
+        return
     traffic = read_traffic()
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,

@@ -68,7 +68,11 @@ struct gdem_ctx {
   int nshapes = 0;
   CShape* d_shapes = nullptr;
   CState cbuf[2]{};
-  CRec* d_out_rec = nullptr;
+  int2* d_cand = nullptr;
+  CRec* d_crec = nullptr;
+  uint32_t* d_mask = nullptr;
+  int32_t* d_work = nullptr;
+  int32_t* d_nwork = nullptr;
   CShape h_wall_shapes[kMaxWalls]{};
   double4 h_wall_sc[kMaxWalls]{};
   CShape* d_wall_shapes = nullptr;
@@ -221,7 +225,8 @@ extern "C" int gdem_destroy(gdem_ctx* c) {
   (void)hipFree(c->d_counters); (void)hipFree(c->d_rec); (void)hipFree(c->d_rec_count); (void)hipFree(c->d_stage);
   free_cstate(c->cbuf[0]);
   free_cstate(c->cbuf[1]);
-  (void)hipFree(c->d_out_rec); (void)hipFree(c->d_shapes); (void)hipFree(c->d_wall_shapes); (void)hipFree(c->d_wall_sc);
+  (void)hipFree(c->d_cand); (void)hipFree(c->d_crec); (void)hipFree(c->d_mask); (void)hipFree(c->d_work);
+  (void)hipFree(c->d_nwork); (void)hipFree(c->d_shapes); (void)hipFree(c->d_wall_shapes); (void)hipFree(c->d_wall_sc);
   for (auto& e : c->ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -240,7 +245,9 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
   (void)hipFree(c->d_F); (void)hipFree(c->d_T);
   (void)hipFree(c->bins.key); (void)hipFree(c->bins.slot); (void)hipFree(c->bins.perm); (void)hipFree(c->bins.pos);
   (void)hipFree(c->d_out_list); (void)hipFree(c->d_in_list); (void)hipFree(c->d_out_count); (void)hipFree(c->d_in_count);
-  (void)hipFree(c->d_rec); (void)hipFree(c->d_rec_count); (void)hipFree(c->d_out_rec);
+  (void)hipFree(c->d_rec); (void)hipFree(c->d_rec_count);
+  (void)hipFree(c->d_cand); (void)hipFree(c->d_crec); (void)hipFree(c->d_mask); (void)hipFree(c->d_work);
+  (void)hipFree(c->d_nwork);
   c->d_F = c->d_T = nullptr;
   c->bins.key = c->bins.slot = c->bins.perm = nullptr;
   c->bins.pos = nullptr;
@@ -248,24 +255,34 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
   c->d_out_count = c->d_in_count = nullptr;
   c->d_rec = nullptr;
   c->d_rec_count = nullptr;
-  c->d_out_rec = nullptr;
+  c->d_cand = nullptr;
+  c->d_crec = nullptr;
+  c->d_mask = nullptr;
+  c->d_work = nullptr;
+  c->d_nwork = nullptr;
   c->n = n;
   c->convex = convex;
   const size_t nn = (size_t)std::max<int64_t>(n, 1);
   if (alloc_state(c->buf[0], n) || alloc_state(c->buf[1], n)) return GDEM_ERR_HIP;
   if (convex) {
     if (alloc_cstate(c->cbuf[0], n) || alloc_cstate(c->cbuf[1], n)) return GDEM_ERR_HIP;
-    HIP_TRY(hipMalloc(&c->d_out_rec, sizeof(CRec) * nn * kCListCap));
+    HIP_TRY(hipMalloc(&c->d_cand, sizeof(int2) * nn * kCandCap));
+    HIP_TRY(hipMalloc(&c->d_crec, sizeof(CRec) * nn * kCandCap));
+    HIP_TRY(hipMalloc(&c->d_mask, sizeof(uint32_t) * nn));
+    HIP_TRY(hipMalloc(&c->d_work, sizeof(int32_t) * nn * kCandCap));
+    HIP_TRY(hipMalloc(&c->d_nwork, sizeof(int32_t)));
+    HIP_TRY(hipMemset(c->d_nwork, 0, sizeof(int32_t)));
   } else {
     HIP_TRY(hipMalloc(&c->d_out_list, sizeof(int2) * nn * kListCap));
   }
+  const size_t in_cap = convex ? (size_t)kCandCap : (size_t)kListCap;
   HIP_TRY(hipMalloc(&c->d_F, sizeof(double4) * nn));
   HIP_TRY(hipMalloc(&c->d_T, sizeof(double4) * nn));
   HIP_TRY(hipMalloc(&c->bins.key, sizeof(int32_t) * nn));
   HIP_TRY(hipMalloc(&c->bins.slot, sizeof(int32_t) * nn));
   HIP_TRY(hipMalloc(&c->bins.perm, sizeof(int32_t) * nn));
   HIP_TRY(hipMalloc(&c->bins.pos, sizeof(float4) * nn));
-  HIP_TRY(hipMalloc(&c->d_in_list, sizeof(int2) * nn * kListCap));
+  HIP_TRY(hipMalloc(&c->d_in_list, sizeof(int2) * nn * in_cap));
   HIP_TRY(hipMalloc(&c->d_out_count, sizeof(int32_t) * nn));
   HIP_TRY(hipMalloc(&c->d_in_count, sizeof(int32_t) * nn));
   HIP_TRY(hipMemset(c->d_in_count, 0, sizeof(int32_t) * nn));
@@ -661,19 +678,37 @@ static void launch_convex_step(gdem_ctx* c, int mode, double dtv, double dtd, bo
   d.pos = c->bins.pos;
   d.margin = prefilter_margin(c->grid, d.sizef);
   d.in = c->buf[c->cur];
-  d.cin = c->cbuf[c->cur];
-  d.shapes = c->d_shapes;
   d.perm = c->bins.perm;
   d.cell_start = c->bins.cell_start;
   d.row_rank = c->d_row_rank;
   d.grid = c->grid;
   d.n = (int32_t)c->n;
-  d.out_rec = c->d_out_rec;
-  d.out_count = c->d_out_count;
-  d.in_list = c->d_in_list;
-  d.in_count = c->d_in_count;
+  d.cand = c->d_cand;
+  d.cand_count = c->d_out_count;
+  d.mask = c->d_mask;
+  d.work = c->d_work;
+  d.nwork = c->d_nwork;
+  d.walls = c->d_walls;
+  d.nwalls = c->nwalls;
   d.err = c->d_err;
   d.nblk8 = nblk8;
+  CGjkArgs g{};
+  g.in = c->buf[c->cur];
+  g.cin = c->cbuf[c->cur];
+  g.shapes = c->d_shapes;
+  g.perm = c->bins.perm;
+  g.grid = c->grid;
+  g.cand = c->d_cand;
+  g.work = c->d_work;
+  g.nwork = c->d_nwork;
+  g.walls = c->d_walls;
+  g.wall_shapes = c->d_wall_shapes;
+  g.wall_sc = c->d_wall_sc;
+  g.rec = c->d_crec;
+  g.mask = c->d_mask;
+  g.in_list = c->d_in_list;
+  g.in_count = c->d_in_count;
+  g.err = c->d_err;
   CForceArgs a{};
   a.in = c->buf[c->cur];
   a.out = c->buf[1 - c->cur];
@@ -685,27 +720,28 @@ static void launch_convex_step(gdem_ctx* c, int mode, double dtv, double dtd, bo
   a.grid = c->grid;
   a.models = c->d_models;
   a.walls = c->d_walls;
-  a.wall_shapes = c->d_wall_shapes;
-  a.wall_sc = c->d_wall_sc;
-  a.nwalls = c->nwalls;
   a.n = (int32_t)c->n;
   for (int dd = 0; dd < 3; ++dd) a.gravity[dd] = c->cfg.gravity[dd];
   a.dt_vel = dtv;
   a.dt_disp = dtd;
-  a.out_rec = c->d_out_rec;
-  a.out_count = c->d_out_count;
+  a.cand = c->d_cand;
+  a.rec = c->d_crec;
+  a.mask = c->d_mask;
   a.in_list = c->d_in_list;
   a.in_count = c->d_in_count;
+  a.nwork = c->d_nwork;
   a.F = write_ft ? c->d_F : nullptr;
   a.T = write_ft ? c->d_T : nullptr;
   a.key_out = c->bins.key;
   a.err = c->d_err;
   a.contacts = c->d_counters;
-  a.rec = c->d_rec;
+  a.crec = c->d_rec;
   a.rec_count = c->d_rec_count;
   a.rec_cap = c->rec_cap;
   a.nblk8 = nblk8;
   launch_cdetect(d, c->stream);
+  // a queue of about 4 candidates per particle is covered in one pass
+  launch_cgjk(g, (int32_t)std::min<int64_t>(4 * c->n, 0x7fffffff), c->stream);
   launch_cforce(mode, record, a, c->stream);
 }
 

@@ -19,6 +19,8 @@
 // Paths are relative to src/Grains3D/Grains/ of the reference.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "gdem_internal.h"
 #include "gdem_sweep.h"
 
@@ -47,22 +49,24 @@ __device__ __forceinline__ Orient load_orient(const CState& c, int s) {
 }
 
 // ---------------------------------------------------------------- detect
-struct CDetector {
+// Sweep: every candidate whose circumscribed sphere overlaps this particle's
+// (the first test of RigidBodyWithCrust::ClosestPoint) is queued; the GJK runs
+// in a second kernel with one lane per candidate, so lanes do not wait on
+// each other's narrow phase.
+struct CCandidates {
   const CDetectArgs& a;
   int t;
   float4 mef;
   bool have_me;
   v3 X;
   double circ;
-  Aux ax;
-  Orient me;
-  int nout;
+  int32_t id;
+  int ncand;
 
-  __device__ __forceinline__ CDetector(const CDetectArgs& a_, int t_, float4 mef_)
-      : a(a_), t(t_), mef(mef_), have_me(false), X{}, circ(0.), ax{}, me{}, nout(0) {}
+  __device__ __forceinline__ CCandidates(const CDetectArgs& a_, int t_, float4 mef_)
+      : a(a_), t(t_), mef(mef_), have_me(false), X{}, circ(0.), id(0), ncand(0) {}
 
   __device__ __forceinline__ void test(const float4 p2f, int u, int sx, int sy, int sz, bool same) {
-    // single-precision pre-filter of the circumscribed-sphere test
     const float dxf = (p2f.x + sx * a.sizef[0]) - mef.x;
     const float dyf = (p2f.y + sy * a.sizef[1]) - mef.y;
     const float dzf = (p2f.z + sz * a.sizef[2]) - mef.z;
@@ -73,53 +77,31 @@ struct CDetector {
       const double4 p = a.in.pos_r[s];
       X = mk(p.x, p.y, p.z);
       circ = p.w;
-      ax = a.in.aux[s];
-      me = load_orient(a.cin, s);
+      id = a.in.aux[s].id;
       have_me = true;
     }
     const int s2 = a.perm[u];
     const double4 p2 = a.in.pos_r[s2];
-    const Aux ax2 = a.in.aux[s2];
-    const Orient o2 = load_orient(a.cin, s2);
+    const int32_t id2 = a.in.aux[s2].id;
     const double shx = sx * a.grid.size[0], shy = sy * a.grid.size[1], shz = sz * a.grid.size[2];
-    // cross-cell candidates: this particle is the first body; same cell: lower id
-    const bool first = same ? (ax.id < ax2.id) : true;
-    CBody A, B;
+    const bool first = same ? (id < id2) : true;
+    // circumscribed spheres, first body A, second body B moved to A's image
+    v3 gab;
+    double cA, cB;
     if (first) {
-      A = CBody{&a.shapes[aux_shape(ax)], me.b, X, me.sc, ax.crust, circ};
-      B = CBody{&a.shapes[aux_shape(ax2)], o2.b, mk(p2.x + shx, p2.y + shy, p2.z + shz), o2.sc,
-                ax2.crust, p2.w};
+      gab = sub(X, mk(p2.x + shx, p2.y + shy, p2.z + shz));
+      cA = circ; cB = p2.w;
     } else {
-      A = CBody{&a.shapes[aux_shape(ax2)], o2.b, mk(p2.x, p2.y, p2.z), o2.sc, ax2.crust, p2.w};
-      B = CBody{&a.shapes[aux_shape(ax)], me.b, mk(X.x + (-shx), X.y + (-shy), X.z + (-shz)),
-                me.sc, ax.crust, circ};
+      gab = sub(mk(p2.x, p2.y, p2.z), mk(X.x + (-shx), X.y + (-shy), X.z + (-shz)));
+      cA = p2.w; cB = circ;
     }
-    v3 c, ov;
-    double dist = 0.;
-    const int rc = closest_point(A, B, c, ov, dist);
-    if (rc < 0) {
-      atomicMin(&a.err[kErrContact], first ? pack_ids(ax.id, ax2.id) : pack_ids(ax2.id, ax.id));
+    if (!(norm(gab) < cA + cB)) return;
+    if (ncand >= kCandCap) {
+      atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)id);
       return;
     }
-    if (rc == 0 || !(dist < 0.)) return;
-    if (nout >= kCListCap) {
-      atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax.id);
-      ++nout;
-      return;
-    }
-    CRec& r = a.out_rec[(size_t)t * kCListCap + nout];
-    r.partner = u;
-    r.code = pack_code(sx, sy, sz, first);
-    r.c[0] = c.x; r.c[1] = c.y; r.c[2] = c.z;
-    r.ov[0] = ov.x; r.ov[1] = ov.y; r.ov[2] = ov.z;
-    r.dist = dist;
-    const int kin = atomicAdd(&a.in_count[u], 1);
-    if (kin < kCListCap) {
-      a.in_list[(size_t)u * kCListCap + kin] = make_int2(t, nout);
-    } else {
-      atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax2.id);
-    }
-    ++nout;
+    a.cand[(size_t)t * kCandCap + ncand] = make_int2(u, pack_code(sx, sy, sz, first));
+    ++ncand;
   }
 
   __device__ __forceinline__ void range(int beg, int end, int sx, int sy, int sz, bool same) {
@@ -129,13 +111,120 @@ struct CDetector {
 
 __global__ void __launch_bounds__(kCBlock) k_cdetect(CDetectArgs a) {
   const int t = logical_index(a.nblk8);
-  if (t >= a.n) return;
-  const double4 me = a.in.pos_r[a.perm[t]];
-  int ijk[3];
-  clamped_cell(a.grid, mk(me.x, me.y, me.z), ijk);
-  CDetector det(a, t, a.pos[t]);
-  half_stencil_sweep(a.grid, a.row_rank, a.cell_start, t, ijk, det);
-  a.out_count[t] = det.nout;
+  int nc = 0;
+  if (t < a.n) {
+    const double4 me = a.in.pos_r[a.perm[t]];
+    int ijk[3];
+    clamped_cell(a.grid, mk(me.x, me.y, me.z), ijk);
+    CCandidates det(a, t, a.pos[t]);
+    half_stencil_sweep(a.grid, a.row_rank, a.cell_start, t, ijk, det);
+    nc = det.ncand;
+    // SimpleObstacle::InterAction's InZone test (Component/src/SimpleObstacle.cpp:222-271)
+    for (int wi = 0; wi < a.nwalls; ++wi) {
+      const Wall& wl = a.walls[wi];
+      if (!(fabs(wl.bb_centre[0] - me.x) <= wl.bb_ext[0] + me.w &&
+            fabs(wl.bb_centre[1] - me.y) <= wl.bb_ext[1] + me.w &&
+            fabs(wl.bb_centre[2] - me.z) <= wl.bb_ext[2] + me.w))
+        continue;
+      if (nc >= kCandCap) {
+        atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)a.in.aux[a.perm[t]].id);
+        break;
+      }
+      a.cand[(size_t)t * kCandCap + nc] = make_int2(wi, kWallCode);
+      ++nc;
+    }
+    a.cand_count[t] = nc;
+    a.mask[t] = 0u;
+  }
+  // queue the candidates: one atomic per wave
+  const int lane = threadIdx.x & 63;
+  int incl = nc;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  const int total = __shfl(incl, 63, 64);
+  int base = 0;
+  if (lane == 63 && total > 0) base = atomicAdd(a.nwork, total);
+  base = __shfl(base, 63, 64);
+  const int off = base + incl - nc;
+  for (int j = 0; j < nc; ++j) a.work[off + j] = t * kCandCap + j;
+}
+
+// Narrow phase of one candidate: RigidBodyWithCrust::ClosestPoint
+// (Component/src/RigidBodyWithCrust.cpp:274-348) between the crust-shrunk
+// bodies, first body A at its position, second body B moved to A's image.
+__global__ void __launch_bounds__(kCBlock) k_cgjk(CGjkArgs a) {
+  const int nw = *a.nwork;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nw; k += gridDim.x * blockDim.x) {
+    const int item = a.work[k];
+    const int t = item / kCandCap, j = item - t * kCandCap;
+    const int2 cd = a.cand[item];
+    const int u = cd.x, code = cd.y;
+    if (code == kWallCode) {
+      // the particle is the first body, the wall box the second
+      const int s1 = a.perm[t];
+      const double4 p1 = a.in.pos_r[s1];
+      const Aux ax1 = a.in.aux[s1];
+      const Orient o1 = load_orient(a.cin, s1);
+      const Wall& wl = a.walls[u];
+      const double4 wsc = a.wall_sc[u];
+      const double wscale[3] = {wsc.x, wsc.y, wsc.z};
+      const CBody A{&a.shapes[aux_shape(ax1)], o1.b, mk(p1.x, p1.y, p1.z), o1.sc, ax1.crust, p1.w};
+      const CBody B{&a.wall_shapes[u], wl.basis, mk(wl.origin[0], wl.origin[1], wl.origin[2]),
+                    wscale, wl.crust, wsc.w};
+      v3 c = mk(0., 0., 0.), ov = mk(0., 0., 0.);
+      double dist = 0.;
+      const int rc = closest_point(A, B, c, ov, dist);
+      if (rc < 0) {
+        atomicMin(&a.err[kErrContact], pack_ids(ax1.id, -1 - u));
+        continue;
+      }
+      if (rc == 0 || !(dist < 0.)) continue;
+      CRec& r = a.rec[item];
+      r.c[0] = c.x; r.c[1] = c.y; r.c[2] = c.z;
+      r.ov[0] = ov.x; r.ov[1] = ov.y; r.ov[2] = ov.z;
+      r.dist = dist;
+      atomicOr(&a.mask[t], 1u << j);
+      continue;
+    }
+    const int sx = (code & 3) - 1, sy = ((code >> 2) & 3) - 1, sz = ((code >> 4) & 3) - 1;
+    const bool first = (code & 64) != 0;
+    const double shx = sx * a.grid.size[0], shy = sy * a.grid.size[1], shz = sz * a.grid.size[2];
+    const int s1 = a.perm[t], s2 = a.perm[u];
+    const double4 p1 = a.in.pos_r[s1], p2 = a.in.pos_r[s2];
+    const Aux ax1 = a.in.aux[s1], ax2 = a.in.aux[s2];
+    const Orient o1 = load_orient(a.cin, s1), o2 = load_orient(a.cin, s2);
+    CBody A, B;
+    if (first) {
+      A = CBody{&a.shapes[aux_shape(ax1)], o1.b, mk(p1.x, p1.y, p1.z), o1.sc, ax1.crust, p1.w};
+      B = CBody{&a.shapes[aux_shape(ax2)], o2.b, mk(p2.x + shx, p2.y + shy, p2.z + shz), o2.sc,
+                ax2.crust, p2.w};
+    } else {
+      A = CBody{&a.shapes[aux_shape(ax2)], o2.b, mk(p2.x, p2.y, p2.z), o2.sc, ax2.crust, p2.w};
+      B = CBody{&a.shapes[aux_shape(ax1)], o1.b, mk(p1.x + (-shx), p1.y + (-shy), p1.z + (-shz)),
+                o1.sc, ax1.crust, p1.w};
+    }
+    v3 c, ov;
+    double dist = 0.;
+    const int rc = closest_point(A, B, c, ov, dist);
+    if (rc < 0) {
+      atomicMin(&a.err[kErrContact], first ? pack_ids(ax1.id, ax2.id) : pack_ids(ax2.id, ax1.id));
+      continue;
+    }
+    if (rc == 0 || !(dist < 0.)) continue;
+    CRec& r = a.rec[item];
+    r.c[0] = c.x; r.c[1] = c.y; r.c[2] = c.z;
+    r.ov[0] = ov.x; r.ov[1] = ov.y; r.ov[2] = ov.z;
+    r.dist = dist;
+    atomicOr(&a.mask[t], 1u << j);
+    const int kin = atomicAdd(&a.in_count[u], 1);
+    if (kin < kCandCap) {
+      a.in_list[(size_t)u * kCandCap + kin] = make_int2(t, j);
+    } else {
+      atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax2.id);
+    }
+  }
 }
 
 // ---------------------------------------------------------------- force
@@ -150,10 +239,9 @@ struct CReducer {
   int ncont, nrec;
 
   // One stored contact.  det_me: this particle detected it (else the partner
-  // slot `other` did).  The record's code gives the shift applied to the
+  // slot `other` did).  The candidate's code gives the shift applied to the
   // detected particle and whether the detecting particle is the first body.
-  __device__ __forceinline__ void contact(const CRec& r, int other, bool det_me) {
-    const int code = r.code;
+  __device__ __forceinline__ void contact(const CRec& r, int code, int other, bool det_me) {
     const int sx = (code & 3) - 1, sy = ((code >> 2) & 3) - 1, sz = ((code >> 4) & 3) - 1;
     const bool first_det = (code & 64) != 0;
     const double shx = sx * a.grid.size[0], shy = sy * a.grid.size[1], shz = sz * a.grid.size[2];
@@ -197,9 +285,25 @@ struct CReducer {
     }
   }
 
+  // Contact with wall wi (SimpleObstacle::InterAction, SimpleObstacle.cpp:222-271)
+  __device__ __forceinline__ void wall(const CRec& r, int wi) {
+    const Wall& wl = a.walls[wi];
+    const v3 c = mk(r.c[0], r.c[1], r.c[2]);
+    const v3 ov = mk(r.ov[0], r.ov[1], r.ov[2]);
+    const Model md = a.models[aux_material(ax) * kMaxMaterials + wl.material];
+    const v3 zero = mk(0., 0., 0.);
+    const v3 f = contact_force(md, c, ov, r.dist, X, V, W, m, req,
+                               mk(wl.origin[0], wl.origin[1], wl.origin[2]), zero, zero, 1.e20,
+                               wl.req);
+    F = add(F, f);
+    T = add(T, cross(sub(c, X), f));
+    ++ncont;
+    if (RECORD) record(ax.id, -1 - wi, c, ov, r.dist, f);
+  }
+
   __device__ __forceinline__ void record(int id0, int id1, v3 c, v3 ov, double dist, v3 f) {
     if (nrec < a.rec_cap) {
-      ContactRec& cr = a.rec[(size_t)t * a.rec_cap + nrec];
+      ContactRec& cr = a.crec[(size_t)t * a.rec_cap + nrec];
       cr.id0 = id0; cr.id1 = id1;
       cr.point[0] = c.x; cr.point[1] = c.y; cr.point[2] = c.z;
       cr.ovec[0] = ov.x; cr.ovec[1] = ov.y; cr.ovec[2] = ov.z;
@@ -263,14 +367,22 @@ __global__ void __launch_bounds__(kCBlock) k_cforce(CForceArgs a) {
     R.T = mk(0., 0., 0.);
     R.ncont = 0;
     R.nrec = 0;
-    // own detections in sweep order, then the received ones by increasing
-    // (detecting slot, record index)
-    const int nout = min(a.out_count[t], kCListCap);
-    const int nin = min(a.in_count[t], kCListCap);
+    if (t == 0) *a.nwork = 0;  // the candidate queue of the next step starts empty
+    // own contacts in sweep order, then the received ones by increasing
+    // (detecting slot, candidate index)
+    uint32_t mk_bits = a.mask[t];
+    while (mk_bits) {
+      const int j = __ffs(mk_bits) - 1;
+      mk_bits &= mk_bits - 1u;
+      const int2 cd = a.cand[(size_t)t * kCandCap + j];
+      if (cd.y == kWallCode)
+        R.wall(a.rec[(size_t)t * kCandCap + j], cd.x);
+      else
+        R.contact(a.rec[(size_t)t * kCandCap + j], cd.y, cd.x, true);
+    }
+    const int nin = min(a.in_count[t], kCandCap);
     a.in_count[t] = 0;
-    const CRec* ol = a.out_rec + (size_t)t * kCListCap;
-    for (int e = 0; e < nout; ++e) R.contact(ol[e], ol[e].partner, true);
-    const int2* il = a.in_list + (size_t)t * kCListCap;
+    const int2* il = a.in_list + (size_t)t * kCandCap;
     long long last = -1;
     for (int e = 0; e < nin; ++e) {
       long long best = 0x7fffffffffffffffll;
@@ -281,42 +393,11 @@ __global__ void __launch_bounds__(kCBlock) k_cforce(CForceArgs a) {
       }
       if (best == 0x7fffffffffffffffll) break;
       const int src = (int)(best >> 8), idx = (int)(best & 255);
-      R.contact(a.out_rec[(size_t)src * kCListCap + idx], src, false);
+      const size_t item = (size_t)src * kCandCap + idx;
+      R.contact(a.rec[item], a.cand[item].y, src, false);
       last = best;
     }
     Orient me = load_orient(a.cin, s);
-    const CShape* sh = &a.shapes[aux_shape(R.ax)];
-    // SimpleObstacle::InterAction (Component/src/SimpleObstacle.cpp:222-271):
-    // the particle is the first body, the wall box the second
-    for (int wi = 0; wi < a.nwalls; ++wi) {
-      const Wall& wl = a.walls[wi];
-      if (!(fabs(wl.bb_centre[0] - R.X.x) <= wl.bb_ext[0] + circ &&
-            fabs(wl.bb_centre[1] - R.X.y) <= wl.bb_ext[1] + circ &&
-            fabs(wl.bb_centre[2] - R.X.z) <= wl.bb_ext[2] + circ))
-        continue;
-      const double4 wsc = a.wall_sc[wi];
-      const double wscale[3] = {wsc.x, wsc.y, wsc.z};
-      const CBody A{sh, me.b, R.X, me.sc, R.ax.crust, circ};
-      const CBody B{&a.wall_shapes[wi], wl.basis, mk(wl.origin[0], wl.origin[1], wl.origin[2]),
-                    wscale, wl.crust, wsc.w};
-      v3 c = mk(0., 0., 0.), ov = mk(0., 0., 0.);
-      double dist = 0.;
-      const int rc = closest_point(A, B, c, ov, dist);
-      if (rc < 0) {
-        atomicMin(&a.err[kErrContact], pack_ids(R.ax.id, -1 - wi));
-        continue;
-      }
-      if (rc == 0 || !(dist < 0.)) continue;
-      const Model md = a.models[aux_material(R.ax) * kMaxMaterials + wl.material];
-      const v3 zero = mk(0., 0., 0.);
-      const v3 f = contact_force(md, c, ov, dist, R.X, R.V, R.W, R.m, R.req,
-                                 mk(wl.origin[0], wl.origin[1], wl.origin[2]), zero, zero, 1.e20,
-                                 wl.req);
-      R.F = add(R.F, f);
-      R.T = add(R.T, cross(sub(c, R.X), f));
-      ++R.ncont;
-      if (RECORD) R.record(R.ax.id, -1 - wi, c, ov, dist, f);
-    }
     if (RECORD) a.rec_count[t] = R.nrec;
     ncont = R.ncont;
     const v3 F = R.F, T = R.T, X = R.X, V = R.V, W = R.W;
@@ -470,6 +551,13 @@ static inline int cblocks(int n) { return (n + kCBlock - 1) / kCBlock; }
 void launch_cdetect(const CDetectArgs& a, hipStream_t st) {
   if (a.n <= 0) return;
   hipLaunchKernelGGL(k_cdetect, dim3(a.nblk8), dim3(kCBlock), 0, st, a);
+}
+
+void launch_cgjk(const CGjkArgs& a, int32_t max_work, hipStream_t st) {
+  if (max_work <= 0) return;
+  // grid-stride over the queue; the grid covers a typical queue in one pass
+  const int blocks = std::min(std::max(cblocks(max_work), 1), 16384);
+  hipLaunchKernelGGL(k_cgjk, dim3(blocks), dim3(kCBlock), 0, st, a);
 }
 
 void launch_cforce(int mode, bool record, const CForceArgs& a, hipStream_t st) {

@@ -111,21 +111,21 @@ struct CState {
   double4* inert;  // I0, I3, I5, 0
 };
 
-// A contact found by the detection pass, stored once in the detecting
-// particle's list: partner slot, code (shift to the partner's image, whether
-// the detecting particle is the first body), contact point, overlap vector
-// and overlap distance of the pair with A = first body.
+// A candidate pair of the detection sweep (circumscribed spheres overlap):
+// slot j of the detecting particle t holds (partner slot, code) with the code
+// of the sphere path, or (wall index, kWallCode) for a wall whose bounding box
+// grown by the circumscribed radius holds the particle's centre.  The GJK pass evaluates every candidate once; a contact
+// sets bit j of the detecting particle's mask and fills record (t, j).
 struct CRec {
-  int32_t partner, code;
   double c[3], ov[3], dist;
+  int32_t pad[2];
 };
-constexpr int kCListCap = 16;
+constexpr int kCandCap = 32;
+constexpr int kWallCode = 128;
 
 struct CDetectArgs {
   const float4* pos;
   State in;
-  CState cin;
-  const CShape* shapes;
   const int32_t* perm;
   const int32_t* cell_start;
   const int32_t* row_rank;
@@ -133,12 +133,34 @@ struct CDetectArgs {
   float sizef[3];
   float margin;
   int32_t n;
-  CRec* out_rec;       // [n][kCListCap]
-  int32_t* out_count;  // [n]
-  int2* in_list;       // [n][kCListCap]: (detecting slot, index in its list)
-  int32_t* in_count;   // [n], zero on entry
+  int2* cand;          // [n][kCandCap]
+  int32_t* cand_count; // [n]
+  uint32_t* mask;      // [n], cleared here
+  int32_t* work;       // candidate queue: t * kCandCap + j
+  int32_t* nwork;      // queue length (zero on entry)
+  const Wall* walls;
+  int32_t nwalls;
   unsigned long long* err;
   int32_t nblk8;
+};
+
+struct CGjkArgs {
+  State in;
+  CState cin;
+  const CShape* shapes;
+  const int32_t* perm;
+  Grid grid;
+  const int2* cand;
+  const int32_t* work;
+  const int32_t* nwork;
+  const Wall* walls;
+  const CShape* wall_shapes;  // box shape of each wall
+  const double4* wall_sc;     // crust scaling per axis, circumscribed radius
+  CRec* rec;           // [n][kCandCap]
+  uint32_t* mask;
+  int2* in_list;       // [n][kCandCap]: (detecting slot, candidate index)
+  int32_t* in_count;   // [n], zero on entry
+  unsigned long long* err;
 };
 
 struct CForceArgs {
@@ -150,27 +172,27 @@ struct CForceArgs {
   Grid grid;
   const Model* models;
   const Wall* walls;
-  const CShape* wall_shapes;  // box shape of each wall
-  const double4* wall_sc;     // crust scaling per axis, circumscribed radius
-  int32_t nwalls;
   int32_t n;
   double gravity[3];
   double dt_vel, dt_disp;
-  const CRec* out_rec;
-  const int32_t* out_count;
+  const int2* cand;
+  const CRec* rec;
+  const uint32_t* mask;
   const int2* in_list;
   int32_t* in_count;
+  int32_t* nwork;      // reset to zero for the next step
   double4* F;
   double4* T;
   int32_t* key_out;
   unsigned long long* err;
   unsigned long long* contacts;
-  ContactRec* rec;
+  ContactRec* crec;
   int32_t* rec_count;
   int32_t rec_cap;
   int32_t nblk8;
 };
 
+void launch_cgjk(const CGjkArgs& a, int32_t max_work, hipStream_t st);
 void launch_cdetect(const CDetectArgs& a, hipStream_t st);
 void launch_cforce(int mode, bool record, const CForceArgs& a, hipStream_t st);
 void launch_cmove(const State& s, const CState& c, const double4* F, const double4* T, int32_t n,
