@@ -207,6 +207,56 @@ int gdem_get_grid(gdem_ctx* ctx, int32_t* nc, double* edge);
 
 const char* gdem_version(void);
 
+/* ------------------------------------------------------------------------
+ * Batched systems: many independent small sphere systems (calibration
+ * sweeps) advanced by one persistent launch per call.  There is no batch in
+ * the reference, which runs one Grains application per system
+ * (Base/src/Grains.cpp:286-452); each system here is that application's step
+ * with its own domain, grid, walls and Hooke/Hertz parameters, and is
+ * bit-identical to the same system advanced alone by gdem_step.  Particle
+ * material is 0, wall material 1.
+ * ------------------------------------------------------------------------ */
+typedef struct gdem_batch gdem_batch;
+
+typedef struct {
+  gdem_config cfg;            /* domain, grid request, gravity (device unused) */
+  int64_t n;                  /* particles */
+  const double *x, *v, *w, *q, *radius, *density, *crust;  /* as gdem_set_particles */
+  int32_t kind;               /* GDEM_MODEL_HOOKE or GDEM_MODEL_HERTZ */
+  int32_t nwalls;             /* box walls (<= GDEM_MAX_WALLS) */
+  double pp[5];               /* particle-particle parameters (gdem_set_contact_model) */
+  double pw[5];               /* particle-wall parameters */
+  const double* wall_basis;   /* nwalls x 9, row-major rotation */
+  const double* wall_centre;  /* nwalls x 3 */
+  const double* wall_half;    /* nwalls x 3, half extents */
+  double wall_crust;
+} gdem_batch_system;
+
+int gdem_batch_create(int32_t device, gdem_batch** out);
+int gdem_batch_destroy(gdem_batch* b);
+/* Copies the system's arrays; ids count from 0 in call order. */
+int gdem_batch_add_system(gdem_batch* b, const gdem_batch_system* sys, int32_t* sys_id);
+/* Uploads every system; no system can be added afterwards. */
+int gdem_batch_commit(gdem_batch* b);
+/* Workgroup size (64, 128, 256 or 512) and workgroup count (0 = fill the
+   device) of the persistent launch; results do not depend on either. */
+int gdem_batch_set_launch(gdem_batch* b, int32_t block_size, int32_t workgroups);
+/* Link + a0 of every system (gdem_prime). */
+int gdem_batch_prime(gdem_batch* b);
+/* nsteps of every system (gdem_step semantics), one persistent launch. */
+int gdem_batch_step(gdem_batch* b, int32_t nsteps, double dt);
+int gdem_batch_synchronize(gdem_batch* b);
+/* State of one system in particle-id order (any pointer may be NULL). */
+int gdem_batch_get_state(gdem_batch* b, int32_t sys, double* x, double* v, double* w,
+                         double* q, double* F, double* T);
+/* First failing system and its error (code, ids as gdem_last_error). */
+int gdem_batch_last_error(gdem_batch* b, int32_t* sys, int32_t* code, int32_t* id0,
+                          int32_t* id1);
+int gdem_batch_set_kernel_timing(gdem_batch* b, int32_t on);
+int gdem_batch_get_counters(gdem_batch* b, int64_t* steps, int64_t* contacts,
+                            double* kernel_ms, int64_t* launches);
+int gdem_batch_info(gdem_batch* b, int32_t* nsys, int64_t* nparticles);
+
 #ifdef __cplusplus
 }
 #endif

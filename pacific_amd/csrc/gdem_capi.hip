@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/gdem.h"
+#include "gdem_host.h"
 #include "gdem_internal.h"
 
 #pragma clang fp contract(off)
@@ -128,26 +129,6 @@ static int alloc_state(State& s, int64_t n) {
   return GDEM_OK;
 }
 
-// Rank of each cell row (j, k) in a 2-D Morton (Z) order of (j, k); the cell
-// key is row_rank[j * nk + k] * ni + i, so the cells of a row along x are
-// consecutive in the sorted order and neighbouring rows stay close.
-static std::vector<int32_t> row_morton_rank(const int32_t nc[3]) {
-  auto spread = [](uint64_t v) {
-    uint64_t r = 0;
-    for (int b = 0; b < 31; ++b) r |= ((v >> b) & 1ull) << (2 * b);
-    return r;
-  };
-  std::vector<std::pair<uint64_t, int32_t>> codes;
-  codes.reserve((size_t)nc[1] * nc[2]);
-  for (int j = 0; j < nc[1]; ++j)
-    for (int k = 0; k < nc[2]; ++k)
-      codes.emplace_back(spread((uint64_t)k) | (spread((uint64_t)j) << 1), (int32_t)(j * nc[2] + k));
-  std::sort(codes.begin(), codes.end());
-  std::vector<int32_t> rank(codes.size());
-  for (size_t r = 0; r < codes.size(); ++r) rank[(size_t)codes[r].second] = (int32_t)r;
-  return rank;
-}
-
 extern "C" const char* gdem_version(void) { return "gdem 0.1 (gfx950, fp64)"; }
 
 extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
@@ -156,26 +137,8 @@ extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
   gdem_ctx* c = new gdem_ctx();
   c->cfg = *cfg;
   c->dev = cfg->device;
-  // LinkedCell::set (Base/src/LinkedCell.cpp:47-119), no ghost layers
-  for (int d = 0; d < 3; ++d) {
-    int n = 1;
-    double e = cfg->size[d];
-    if (!(cfg->size[d] > 0.)) { delete c; return GDEM_ERR_ARG; }
-    if (cfg->cell_size > kEps) {
-      n = (int)((cfg->size[d] + kEps) / cfg->cell_size);
-      if (!n) n = 1;
-      e = cfg->size[d] / n;
-    }
-    if (cfg->periodic[d] && n < 3) { delete c; return GDEM_ERR_ARG; }
-    c->grid.nc[d] = n;
-    c->grid.edge[d] = e;
-    c->grid.origin[d] = cfg->origin[d];
-    c->grid.size[d] = cfg->size[d];
-    c->grid.periodic[d] = cfg->periodic[d] ? 1 : 0;
-  }
-  int64_t ncell = (int64_t)c->grid.nc[0] * c->grid.nc[1] * c->grid.nc[2];
-  if (ncell > 8 * 1024 * 1024) { delete c; return GDEM_ERR_ARG; }
-  c->grid.ncell = (int32_t)ncell;
+  if (!make_grid(*cfg, c->grid)) { delete c; return GDEM_ERR_ARG; }
+  const int64_t ncell = c->grid.ncell;
   if (hipSetDevice(c->dev) != hipSuccess) { delete c; return GDEM_ERR_HIP; }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return GDEM_ERR_HIP; }
   std::vector<int32_t> rank = row_morton_rank(c->grid.nc);
@@ -314,25 +277,8 @@ extern "C" int gdem_set_particles(gdem_ctx* c, int64_t n, const double* x, const
     if (!(rr > 0.) || !(rho > 0.) || !(crust[i] > 0.)) return GDEM_ERR_ARG;
     int32_t mat = material ? material[i] : 0;
     if (mat < 0 || mat >= kMaxMaterials) return GDEM_ERR_ARG;
-    // Sphere::getVolume (Geometry/src/Sphere.cpp:152-155), Sphere::BuildInertia
-    // (:61-70), Particle.cpp:158-164, Component::getEquivalentSphereRadius
-    // (Component/src/Component.cpp:205-208)
-    const double vol = 4.0 * kPi * pow(rr, 3.) / 3.0;
-    const double in0 = 8.0 * kPi * pow(rr, 5.) / 15.0;
-    const double mass = rho * vol;
-    const double inertia = in0 * rho;
-    const double req = pow((0.75 / kPi) * vol, 1. / 3.);
-    const quat qi = qmk(mk(q[4 * i], q[4 * i + 1], q[4 * i + 2]), q[4 * i + 3]);
-    // ParticleKinematics::setAngularVelocity (ParticleKinematics.cpp:197-205)
-    const v3 wbf = qmulv(qconj(qi), qleftvec(qi, mk(w[3 * i], w[3 * i + 1], w[3 * i + 2])));
-    pr[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], rr);
-    vm[i] = make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], mass);
-    wr[i] = make_double4(w[3 * i], w[3 * i + 1], w[3 * i + 2], req);
-    wb[i] = make_double4(wbf.x, wbf.y, wbf.z, inertia);
-    qq[i] = make_double4(q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]);
-    ax[i].id = (int32_t)i;
-    ax[i].mat = mat;
-    ax[i].crust = crust[i];
+    sphere_particle(x + 3 * i, v + 3 * i, w + 3 * i, q + 4 * i, rr, rho, crust[i], mat,
+                    (int32_t)i, pr[i], vm[i], wr[i], wb[i], qq[i], ax[i]);
   }
   State& s = c->buf[0];
   c->cur = 0;
@@ -565,40 +511,8 @@ extern "C" int gdem_add_wall(gdem_ctx* c, const double* basis, const double* cen
                              const double* half, double crust, int32_t material) {
   if (!c || !basis || !centre || !half) return GDEM_ERR_ARG;
   if (c->nwalls >= kMaxWalls || material < 0 || material >= kMaxMaterials) return GDEM_ERR_ARG;
-  Wall wl{};
-  memcpy(wl.basis, basis, sizeof(wl.basis));
-  memcpy(wl.origin, centre, sizeof(wl.origin));
-  memcpy(wl.ext, half, sizeof(wl.ext));
-  wl.crust = crust;
-  wl.material = material;
-  // Transform::setToInverseTransform (Geometry/src/Transform.cpp:71-77), rotation basis
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) wl.inv_basis[3 * i + j] = basis[3 * j + i];
-  for (int i = 0; i < 3; ++i) {
-    v3 row = mk(wl.inv_basis[3 * i], wl.inv_basis[3 * i + 1], wl.inv_basis[3 * i + 2]);
-    wl.inv_origin[i] = dot(neg(row), mk(centre[0], centre[1], centre[2]));
-  }
-  // Convex::bbox (Geometry/src/Convex.cpp:29-41) with Box::support (Box.cpp:307-316),
-  // BBox::setValue (BBox.cpp:235-239)
+  const Wall wl = make_wall(basis, centre, half, crust, material);
   const double e0 = half[0], e1 = half[1], e2 = half[2];
-  double mn[3], mx[3];
-  for (int i = 0; i < 3; ++i) {
-    v3 row = mk(basis[3 * i], basis[3 * i + 1], basis[3 * i + 2]);
-    v3 nrow = neg(row);
-    v3 sm = norm(nrow) < kEps ? mk(0., 0., 0.)
-                              : mk(nrow.x < 0. ? -e0 : e0, nrow.y < 0. ? -e1 : e1, nrow.z < 0. ? -e2 : e2);
-    v3 sp = norm(row) < kEps ? mk(0., 0., 0.)
-                             : mk(row.x < 0. ? -e0 : e0, row.y < 0. ? -e1 : e1, row.z < 0. ? -e2 : e2);
-    mn[i] = centre[i] + dot(row, sm);
-    mx[i] = centre[i] + dot(row, sp);
-  }
-  for (int i = 0; i < 3; ++i) {
-    wl.bb_ext[i] = (mx[i] - mn[i]) / 2.;
-    wl.bb_centre[i] = mn[i] + wl.bb_ext[i];
-  }
-  // Box::getVolume (Box.cpp:273-279), equivalent-sphere radius of the obstacle
-  const double vol = 8.0 * e0 * e1 * e2;
-  wl.req = pow((0.75 / kPi) * vol, 1. / 3.);
   // the wall as a convex body for the GJK narrow phase of convex particles:
   // Box shape, crust scaling of its extents, circumscribed radius
   CShape wsh{};
@@ -623,15 +537,8 @@ extern "C" int gdem_add_wall(gdem_ctx* c, const double* basis, const double* cen
 extern "C" int gdem_set_contact_model(gdem_ctx* c, int32_t a, int32_t b, int32_t kind,
                                       const double* p) {
   if (!c || !p || a < 0 || b < 0 || a >= kMaxMaterials || b >= kMaxMaterials) return GDEM_ERR_ARG;
-  if (kind != GDEM_MODEL_HOOKE && kind != GDEM_MODEL_HERTZ) return GDEM_ERR_ARG;
-  if (p[4] != 0.) return GDEM_ERR_ARG;  // rolling resistance not supported
-  if (!(p[1] > 0. && p[1] <= 1.)) return GDEM_ERR_ARG;
-  Model m{};
-  m.kind = kind;
-  memcpy(m.p, p, sizeof(m.p));
-  // HookeContactForceModel.cpp:19, HertzContactForceModel.cpp:20-21
-  m.beta = log(p[1]) / sqrt(kPi * kPi + log(p[1]) * log(p[1]));
-  m.m2sqrt56 = -2. * sqrt(5. / 6.);
+  if (!model_params_ok(kind, p)) return GDEM_ERR_ARG;
+  const Model m = make_model(kind, p);
   c->h_models[a * kMaxMaterials + b] = m;
   c->h_models[b * kMaxMaterials + a] = m;
   HIP_TRY(hipSetDevice(c->dev));
@@ -655,21 +562,6 @@ static std::pair<hipEvent_t, hipEvent_t>* next_event_pair(gdem_ctx* c) {
     c->ev_pool.emplace_back(a, b);
   }
   return &c->ev_pool[c->ev_used++];
-}
-
-// Pre-filter margin of the single-precision distance test: coordinates lie
-// within [lo, lo + L] (a particle outside a non-periodic grid is a
-// SimulationError), a shifted image within L more.  Rounding each to float
-// moves it by at most 2^-24 of its magnitude; 2^-19 of (max |coordinate| +
-// max L) bounds the error of the float distance test by a wide factor.
-static float prefilter_margin(const Grid& g, float* sizef) {
-  double mag = 0., lmax = 0.;
-  for (int dd = 0; dd < 3; ++dd) {
-    mag = std::max(mag, std::max(std::fabs(g.origin[dd]), std::fabs(g.origin[dd] + g.size[dd])));
-    lmax = std::max(lmax, g.size[dd]);
-    sizef[dd] = (float)g.size[dd];
-  }
-  return (float)(std::ldexp(1.0, -19) * (mag + lmax));
 }
 
 static void launch_convex_step(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft,
@@ -784,6 +676,7 @@ static int do_force(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft
     a.row_rank = c->d_row_rank;
     a.grid = c->grid;
     a.models = c->d_models;
+    a.model_stride = kMaxMaterials;
     a.walls = c->d_walls;
     a.nwalls = c->nwalls;
     a.n = (int32_t)c->n;

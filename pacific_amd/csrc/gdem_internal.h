@@ -77,7 +77,8 @@ struct ForceArgs {
   const int32_t* perm;
   const int32_t* row_rank;
   Grid grid;
-  const Model* models;
+  const Model* models;   // pair table, models[mat_a * model_stride + mat_b]
+  int32_t model_stride;
   const Wall* walls;
   int32_t nwalls;
   int32_t n;

@@ -1,0 +1,378 @@
+// gdem_sphere_step.h -- per-particle bodies of the sphere detection and force
+// kernels, shared by the single-system kernels (gdem_kernels.hip) and the
+// batched persistent kernel (gdem_batch.hip), so both compute the same
+// operations in the same order.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "gdem_internal.h"
+#include "gdem_sweep.h"
+
+#pragma clang fp contract(off)
+
+namespace gdem {
+
+// ---------------------------------------------------------------- detect
+struct Detector {
+  const DetectArgs& a;
+  int t;
+  float4 mef;
+  bool have_me;
+  v3 X;
+  double r;
+  Aux ax;
+  int nout;
+
+  __device__ __forceinline__ Detector(const DetectArgs& a_, int t_, float4 mef_)
+      : a(a_), t(t_), mef(mef_), have_me(false), X{}, r(0.), ax{}, nout(0) {}
+
+  // One candidate u with periodic shift s (applied to u).  Cross-cell
+  // candidates of the half stencil have this particle as the first body; a
+  // same-cell pair is ordered by id.
+  __device__ __forceinline__ void test(const float4 p2f, int u, int sx, int sy, int sz, bool same) {
+    // single-precision pre-filter: rejects only pairs whose gap exceeds the
+    // margin, which bounds the rounding of the float copy
+    const float dxf = (p2f.x + sx * a.sizef[0]) - mef.x;
+    const float dyf = (p2f.y + sy * a.sizef[1]) - mef.y;
+    const float dzf = (p2f.z + sz * a.sizef[2]) - mef.z;
+    const float rsf = mef.w + p2f.w + a.margin;
+    if (dxf * dxf + dyf * dyf + dzf * dzf > rsf * rsf) return;
+    if (!have_me) {
+      const int s = a.perm[t];
+      const double4 me = a.in.pos_r[s];
+      X = mk(me.x, me.y, me.z);
+      r = me.w;
+      ax = a.in.aux[s];
+      have_me = true;
+    }
+    const int s2 = a.perm[u];
+    const double4 p2 = a.in.pos_r[s2];
+    const Aux ax2 = a.in.aux[s2];
+    const double shx = sx * a.grid.size[0], shy = sy * a.grid.size[1], shz = sz * a.grid.size[2];
+    const v3 Bn = mk(p2.x + shx, p2.y + shy, p2.z + shz);
+    const bool first = same ? (ax.id < ax2.id) : true;
+    v3 A, B, c, ov;
+    double rA, rB, crA, crB, dist;
+    if (first) {
+      A = X; B = Bn; rA = r; rB = p2.w; crA = ax.crust; crB = ax2.crust;
+    } else {
+      A = mk(p2.x, p2.y, p2.z);
+      B = mk(X.x + (-shx), X.y + (-shy), X.z + (-shz));
+      rA = p2.w; rB = r; crA = ax2.crust; crB = ax.crust;
+    }
+    const int rc = sphere_sphere(A, rA, crA, B, rB, crB, c, ov, dist);
+    if (rc < 0) {
+      atomicMin(&a.err[kErrContact], first ? pack_ids(ax.id, ax2.id) : pack_ids(ax2.id, ax.id));
+      return;
+    }
+    if (rc == 0 || !(dist < 0.)) return;
+    const int code = pack_code(sx, sy, sz, first);
+    if (nout < kListCap) {
+      a.out_list[(size_t)t * kListCap + nout] = make_int2(u, code);
+    } else {
+      atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax.id);
+    }
+    ++nout;
+    const int kin = atomicAdd(&a.in_count[u], 1);
+    if (kin < kListCap) {
+      a.in_list[(size_t)u * kListCap + kin] = make_int2(t, mirror_code(code));
+    } else {
+      atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax2.id);
+    }
+  }
+
+  __device__ __forceinline__ void range(int beg, int end, int sx, int sy, int sz, bool same) {
+    int u = beg;
+    for (; u + 1 < end; u += 2) {
+      // issue both loads before either test
+      const float4 p0 = a.pos[u], p1 = a.pos[u + 1];
+      test(p0, u, sx, sy, sz, same);
+      test(p1, u + 1, sx, sy, sz, same);
+    }
+    if (u < end) test(a.pos[u], u, sx, sy, sz, same);
+  }
+};
+
+// Detection sweep of one particle (storage slot t of the sorted order).
+__device__ __forceinline__ void detect_particle(const DetectArgs& a, int t) {
+  // the cell index comes from the full-precision position, as in the binning
+  const double4 me = a.in.pos_r[a.perm[t]];
+  int ijk[3];
+  clamped_cell(a.grid, mk(me.x, me.y, me.z), ijk);
+  Detector det(a, t, a.pos[t]);
+  half_stencil_sweep(a.grid, a.row_rank, a.cell_start, t, ijk, det);
+  a.out_count[t] = det.nout;
+}
+
+// ---------------------------------------------------------------- force
+template <bool RECORD>
+struct Reducer {
+  const ForceArgs& a;
+  int t;
+  v3 X, V, W;
+  double r, m, req;
+  Aux ax;
+  v3 F, T;
+  int ncont, nrec;
+
+  // One contact of this particle's segment: partner slot p, code (shift to the
+  // partner's image, first-body flag).  The pair is evaluated with A = first
+  // body and B = second body moved to the first body's image, exactly as the
+  // partner's thread does.
+  __device__ __forceinline__ void contact(int p, int code) {
+    const int sx = (code & 3) - 1, sy = ((code >> 2) & 3) - 1, sz = ((code >> 4) & 3) - 1;
+    const bool first = (code & 64) != 0;
+    const double shx = sx * a.grid.size[0], shy = sy * a.grid.size[1], shz = sz * a.grid.size[2];
+    const int s2 = a.perm[p];
+    const double4 p2 = a.in.pos_r[s2];
+    const double4 vm2 = a.in.vel_m[s2];
+    const double4 wr2 = a.in.om_req[s2];
+    const Aux ax2 = a.in.aux[s2];
+    const v3 V2 = mk(vm2.x, vm2.y, vm2.z), W2 = mk(wr2.x, wr2.y, wr2.z);
+    v3 A, B, c, ov;
+    double rA, rB, crA, crB, dist;
+    if (first) {
+      A = X; B = mk(p2.x + shx, p2.y + shy, p2.z + shz);
+      rA = r; rB = p2.w; crA = ax.crust; crB = ax2.crust;
+    } else {
+      A = mk(p2.x, p2.y, p2.z);
+      B = mk(X.x + (-shx), X.y + (-shy), X.z + (-shz));
+      rA = p2.w; rB = r; crA = ax2.crust; crB = ax.crust;
+    }
+    sphere_sphere(A, rA, crA, B, rB, crB, c, ov, dist);
+    const Model md = a.models[(first ? ax.mat : ax2.mat) * a.model_stride + (first ? ax2.mat : ax.mat)];
+    v3 f;
+    if (first)
+      f = contact_force(md, c, ov, dist, A, V, W, m, req, B, V2, W2, vm2.w, wr2.w);
+    else
+      f = contact_force(md, c, ov, dist, A, V2, W2, vm2.w, wr2.w, B, V, W, m, req);
+    if (first) {
+      F = add(F, f);
+      T = add(T, cross(sub(c, A), f));
+      ++ncont;
+      if (RECORD) record(ax.id, ax2.id, c, ov, dist, f);
+    } else {
+      const v3 fn = neg(f);
+      F = add(F, fn);
+      T = add(T, cross(sub(c, B), fn));
+    }
+  }
+
+  __device__ __forceinline__ void record(int id0, int id1, v3 c, v3 ov, double dist, v3 f) {
+    if (nrec < a.rec_cap) {
+      ContactRec& cr = a.rec[(size_t)t * a.rec_cap + nrec];
+      cr.id0 = id0; cr.id1 = id1;
+      cr.point[0] = c.x; cr.point[1] = c.y; cr.point[2] = c.z;
+      cr.ovec[0] = ov.x; cr.ovec[1] = ov.y; cr.ovec[2] = ov.z;
+      cr.dist = dist;
+      cr.force[0] = f.x; cr.force[1] = f.y; cr.force[2] = f.z;
+    } else {
+      atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)id0);
+    }
+    ++nrec;
+  }
+};
+
+// Force, torque and integrator tail of one particle (storage slot t of the
+// sorted order).  Returns the number of contacts it is the first body of.
+template <int MODE, bool RECORD>
+__device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
+  const int s = a.perm[t];
+  const double4 pr = a.in.pos_r[s];
+  const double4 vm = a.in.vel_m[s];
+  const double4 wr = a.in.om_req[s];
+  const double4 wb = a.in.wbf_I[s];
+  const double4 qq = a.in.quat[s];
+  Reducer<RECORD> R{a, t};
+  R.ax = a.in.aux[s];
+  R.X = mk(pr.x, pr.y, pr.z);
+  R.V = mk(vm.x, vm.y, vm.z);
+  R.W = mk(wr.x, wr.y, wr.z);
+  R.r = pr.w;
+  R.m = vm.w;
+  R.req = wr.w;
+  // InitializeForce: weight m (1 - 0/rho) g at the centre (Particle.cpp:414-419, 1409-1416)
+  const double mw = R.m * (1. - 0. / 1.);
+  R.F = mk(a.gravity[0] * mw, a.gravity[1] * mw, a.gravity[2] * mw);
+  R.T = mk(0., 0., 0.);
+  R.ncont = 0;
+  R.nrec = 0;
+  // contact segment: detected contacts in sweep order, then the contacts
+  // other particles detected, by increasing partner slot
+  const int nout = min(a.out_count[t], kListCap);
+  const int nin = min(a.in_count[t], kListCap);
+  a.in_count[t] = 0;
+  const int2* ol = a.out_list + (size_t)t * kListCap;
+  for (int e = 0; e < nout; ++e) {
+    const int2 en = ol[e];
+    R.contact(en.x, en.y);
+  }
+  const int2* il = a.in_list + (size_t)t * kListCap;
+  int last = -1;
+  for (int e = 0; e < nin; ++e) {
+    int best = 0x7fffffff, code = 0;
+    for (int f = 0; f < nin; ++f) {
+      const int2 en = il[f];
+      if (en.x > last && en.x < best) { best = en.x; code = en.y; }
+    }
+    R.contact(best, code);
+    last = best;
+  }
+  // SimpleObstacle::InterAction (Component/src/SimpleObstacle.cpp:222-271)
+  for (int wi = 0; wi < a.nwalls; ++wi) {
+    const Wall& wl = a.walls[wi];
+    if (!(fabs(wl.bb_centre[0] - R.X.x) <= wl.bb_ext[0] + R.r &&
+          fabs(wl.bb_centre[1] - R.X.y) <= wl.bb_ext[1] + R.r &&
+          fabs(wl.bb_centre[2] - R.X.z) <= wl.bb_ext[2] + R.r))
+      continue;
+    v3 c = mk(0., 0., 0.), ov = mk(0., 0., 0.);
+    double dist;
+    const int rc = sphere_wall(wl, R.X, R.r, R.ax.crust, c, ov, dist);
+    if (rc < 0) {
+      atomicMin(&a.err[kErrContact], pack_ids(R.ax.id, -1 - wi));
+      continue;
+    }
+    if (rc == 0 || !(dist < 0.)) continue;
+    const Model md = a.models[R.ax.mat * a.model_stride + wl.material];
+    const v3 zero = mk(0., 0., 0.);
+    const v3 f = contact_force(md, c, ov, dist, R.X, R.V, R.W, R.m, R.req,
+                               mk(wl.origin[0], wl.origin[1], wl.origin[2]), zero, zero, 1.e20,
+                               wl.req);
+    R.F = add(R.F, f);
+    R.T = add(R.T, cross(sub(c, R.X), f));
+    ++R.ncont;
+    if (RECORD) R.record(R.ax.id, -1 - wi, c, ov, dist, f);
+  }
+  if (RECORD) a.rec_count[t] = R.nrec;
+  const v3 F = R.F, T = R.T, X = R.X, V = R.V, W = R.W;
+  const double m = R.m;
+
+  // ---- tail
+  v3 Xo = X, Vo = V, Wo = W;
+  v3 WBF = mk(wb.x, wb.y, wb.z);
+  quat q = qmk(mk(qq.x, qq.y, qq.z), qq.w);
+  int ijk[3];
+  cell_of(a.grid, X, ijk);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) ijk[d] = min(max(ijk[d], 0), a.grid.nc[d] - 1);
+  int key = row_base(a.grid, a.row_rank, ijk[1], ijk[2]) + ijk[0];
+  if (MODE != MODE_FORCE) {
+    const double I = wb.w;
+    const quat pc = qconj(q);
+    // torque in body frame (ParticleKinematics.cpp:115-117)
+    const v3 tq = qmulv(pc, qleftvec(q, T));
+    const double mcf = m * 1.;
+    // closing half kick, SecondOrderLeapFrog::advanceVelocity (:74-94)
+    Vo = add(V, scl(dvd(F, mcf), a.dt_vel));
+    WBF = ssprk3_sphere(tq, WBF, I, a.dt_vel);
+    Wo = qmulv(q, qleftvec(pc, WBF));
+    if (MODE == MODE_MOVE) {
+      // next step's kick and drift, SecondOrderLeapFrog::Move (:44-67) and
+      // ParticleKinematics::Move (:62-104); q and T unchanged -> same tq
+      Vo = add(Vo, scl(dvd(F, mcf), a.dt_vel));
+      const v3 dx = scl(Vo, a.dt_disp);
+      WBF = ssprk3_sphere(tq, WBF, I, a.dt_vel);
+      Wo = qmulv(q, qleftvec(pc, WBF));
+      const v3 avg = Wo;
+      Xo = add(X, dx);
+      const double nOmega = norm(avg);
+      quat dq;
+      if (nOmega > kEps) {
+        const double cc = cos(nOmega * a.dt_disp / 2.);
+        const double sn = sin(nOmega * a.dt_disp / 2.);
+        dq = qmk(scl(avg, sn * 1. / nOmega), cc);
+      } else {
+        dq = qmk(mk(0., 0., 0.), 1.);
+      }
+      q = qmul(dq, q);
+      const double depl = norm(dx);
+      if (depl > R.ax.crust)
+        atomicMin(&a.err[kErrMotion], (unsigned long long)(uint32_t)R.ax.id);
+      double xo[3] = {Xo.x, Xo.y, Xo.z};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (!a.grid.periodic[d]) continue;
+        const double lo = a.grid.origin[d], L = a.grid.size[d];
+        if (xo[d] >= lo + L) xo[d] -= L;
+        else if (xo[d] < lo) xo[d] += L;
+      }
+      Xo = mk(xo[0], xo[1], xo[2]);
+      int nijk[3];
+      if (cell_of(a.grid, Xo, nijk)) {
+        key = row_base(a.grid, a.row_rank, nijk[1], nijk[2]) + nijk[0];
+      } else {
+        key = 0;
+        atomicMin(&a.err[kErrSim], (unsigned long long)(uint32_t)R.ax.id);
+      }
+    }
+  }
+  a.out.pos_r[t] = make_double4(Xo.x, Xo.y, Xo.z, R.r);
+  a.out.vel_m[t] = make_double4(Vo.x, Vo.y, Vo.z, m);
+  a.out.om_req[t] = make_double4(Wo.x, Wo.y, Wo.z, R.req);
+  a.out.wbf_I[t] = make_double4(WBF.x, WBF.y, WBF.z, wb.w);
+  a.out.quat[t] = make_double4(q.v.x, q.v.y, q.v.z, q.w);
+  a.out.aux[t] = R.ax;
+  a.key_out[t] = key;
+  if (a.F) {
+    a.F[t] = make_double4(F.x, F.y, F.z, 0.);
+    a.T[t] = make_double4(T.x, T.y, T.z, 0.);
+  }
+  return R.ncont;
+}
+
+// SecondOrderLeapFrog::Move (TimeIntegration/src/SecondOrderLeapFrog.cpp:44-67)
+// via ParticleKinematics::Move (Component/src/ParticleKinematics.cpp:62-104) of
+// the particle in storage slot i, in place; writes its new cell key.
+__device__ __forceinline__ void move_particle(const State& s, const double4* Fa, const double4* Ta,
+                                              int i, const Grid& g, const int32_t* row_rank,
+                                              double dtv, double dtd, int32_t* key,
+                                              unsigned long long* err) {
+  const double4 pr = s.pos_r[i], vm = s.vel_m[i], wr = s.om_req[i], wb = s.wbf_I[i],
+                qq = s.quat[i];
+  const double4 f4 = Fa[i], t4 = Ta[i];
+  const Aux ax = s.aux[i];
+  const v3 F = mk(f4.x, f4.y, f4.z), T = mk(t4.x, t4.y, t4.z);
+  quat q = qmk(mk(qq.x, qq.y, qq.z), qq.w);
+  const quat pc = qconj(q);
+  const v3 tq = qmulv(pc, qleftvec(q, T));
+  const double mcf = vm.w * 1.;
+  const v3 V = add(mk(vm.x, vm.y, vm.z), scl(dvd(F, mcf), dtv));
+  const v3 dx = scl(V, dtd);
+  const v3 WBF = ssprk3_sphere(tq, mk(wb.x, wb.y, wb.z), wb.w, dtv);
+  const v3 W = qmulv(q, qleftvec(pc, WBF));
+  const v3 avg = W;
+  v3 X = add(mk(pr.x, pr.y, pr.z), dx);
+  const double nOmega = norm(avg);
+  quat dq;
+  if (nOmega > kEps) {
+    const double cc = cos(nOmega * dtd / 2.);
+    const double sn = sin(nOmega * dtd / 2.);
+    dq = qmk(scl(avg, sn * 1. / nOmega), cc);
+  } else {
+    dq = qmk(mk(0., 0., 0.), 1.);
+  }
+  q = qmul(dq, q);
+  if (norm(dx) > ax.crust) atomicMin(&err[kErrMotion], (unsigned long long)(uint32_t)ax.id);
+  double xo[3] = {X.x, X.y, X.z};
+  for (int d = 0; d < 3; ++d) {
+    if (!g.periodic[d]) continue;
+    const double lo = g.origin[d], L = g.size[d];
+    if (xo[d] >= lo + L) xo[d] -= L;
+    else if (xo[d] < lo) xo[d] += L;
+  }
+  X = mk(xo[0], xo[1], xo[2]);
+  int ijk[3];
+  if (cell_of(g, X, ijk)) {
+    key[i] = row_base(g, row_rank, ijk[1], ijk[2]) + ijk[0];
+  } else {
+    key[i] = 0;
+    atomicMin(&err[kErrSim], (unsigned long long)(uint32_t)ax.id);
+  }
+  s.pos_r[i] = make_double4(X.x, X.y, X.z, pr.w);
+  s.vel_m[i] = make_double4(V.x, V.y, V.z, vm.w);
+  s.om_req[i] = make_double4(W.x, W.y, W.z, wr.w);
+  s.wbf_I[i] = make_double4(WBF.x, WBF.y, WBF.z, wb.w);
+  s.quat[i] = make_double4(q.v.x, q.v.y, q.v.z, q.w);
+}
+
+}  // namespace gdem

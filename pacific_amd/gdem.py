@@ -56,6 +56,16 @@ class _Config(C.Structure):
                 ("reserved", C.c_int32)]
 
 
+class _BatchSystem(C.Structure):
+    _fields_ = [("cfg", _Config), ("n", C.c_int64),
+                ("x", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p), ("q", C.c_void_p),
+                ("radius", C.c_void_p), ("density", C.c_void_p), ("crust", C.c_void_p),
+                ("kind", C.c_int32), ("nwalls", C.c_int32),
+                ("pp", C.c_double * 5), ("pw", C.c_double * 5),
+                ("wall_basis", C.c_void_p), ("wall_centre", C.c_void_p),
+                ("wall_half", C.c_void_p), ("wall_crust", C.c_double)]
+
+
 _lib = None
 
 
@@ -96,6 +106,19 @@ def load_library(path: str = LIB_PATH):
         "gdem_add_polyhedron_shape": (C.c_int, [vp, i32, vp, i32, vp, vp, P(i32)]),
         "gdem_set_convex_particles": (C.c_int, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "gdem_get_basis": (C.c_int, [vp, vp]),
+        "gdem_batch_create": (C.c_int, [i32, P(vp)]),
+        "gdem_batch_destroy": (C.c_int, [vp]),
+        "gdem_batch_add_system": (C.c_int, [vp, P(_BatchSystem), P(i32)]),
+        "gdem_batch_commit": (C.c_int, [vp]),
+        "gdem_batch_set_launch": (C.c_int, [vp, i32, i32]),
+        "gdem_batch_prime": (C.c_int, [vp]),
+        "gdem_batch_step": (C.c_int, [vp, i32, d]),
+        "gdem_batch_synchronize": (C.c_int, [vp]),
+        "gdem_batch_get_state": (C.c_int, [vp, i32, vp, vp, vp, vp, vp, vp]),
+        "gdem_batch_last_error": (C.c_int, [vp, P(i32), P(i32), P(i32), P(i32)]),
+        "gdem_batch_set_kernel_timing": (C.c_int, [vp, i32]),
+        "gdem_batch_get_counters": (C.c_int, [vp, P(i64), P(i64), P(d), P(i64)]),
+        "gdem_batch_info": (C.c_int, [vp, P(i32), P(i64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -112,7 +135,10 @@ EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall"
             "gdem_get_contacts", "gdem_last_error", "gdem_set_kernel_timing",
             "gdem_get_counters", "gdem_reset_counters", "gdem_get_grid", "gdem_version",
             "gdem_add_box_shape", "gdem_add_polyhedron_shape", "gdem_set_convex_particles",
-            "gdem_get_basis"]
+            "gdem_get_basis", "gdem_batch_create", "gdem_batch_destroy", "gdem_batch_add_system",
+            "gdem_batch_commit", "gdem_batch_set_launch", "gdem_batch_prime", "gdem_batch_step",
+            "gdem_batch_synchronize", "gdem_batch_get_state", "gdem_batch_last_error",
+            "gdem_batch_set_kernel_timing", "gdem_batch_get_counters", "gdem_batch_info"]
 
 
 def _ptr(a):
@@ -315,3 +341,125 @@ def build_system(wl, device=0):
         s.set_particles(p["x"], p["v"], p["w"], p["q"], p["radius"], p["density"], p["crust"],
                         p["material"])
     return s
+
+
+class GdemBatch:
+    """Many independent sphere systems advanced by one persistent launch per
+    call (include/gdem.h, gdem_batch_*).  Each system is a workload dict with
+    one particle material (0) and walls of material 1, as
+    pacific_amd.workloads.batch_systems builds them."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        self._check(self.lib.gdem_batch_create(int(device), C.byref(h)), "gdem_batch_create")
+        self.h = h
+        self.sizes = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gdem_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc == OK:
+            return
+        sysid = id0 = id1 = -1
+        if getattr(self, "h", None):
+            s, code, a, b = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+            self.lib.gdem_batch_last_error(self.h, C.byref(s), C.byref(code), C.byref(a), C.byref(b))
+            sysid, id0, id1 = s.value, a.value, b.value
+        err = _ERRORS.get(rc, GdemError)(rc, f"{what} (system {sysid})", id0, id1)
+        err.system = sysid
+        raise err
+
+    def add_system(self, wl):
+        p = wl["particles"]
+        n = len(p["radius"])
+        bs = _BatchSystem()
+        for d in range(3):
+            bs.cfg.origin[d] = float(wl["origin"][d])
+            bs.cfg.size[d] = float(wl["size"][d])
+            bs.cfg.periodic[d] = int(bool(wl["periodic"][d]))
+            bs.cfg.gravity[d] = float(wl["gravity"][d])
+        bs.cfg.cell_size = float(wl["cell_size"])
+        keep = [_f64(p["x"], (n, 3)), _f64(p["v"], (n, 3)), _f64(p["w"], (n, 3)),
+                _f64(p["q"], (n, 4)), _f64(p["radius"], (n,)), _f64(p["density"], (n,)),
+                _f64(p["crust"], (n,))]
+        bs.n = n
+        bs.x, bs.v, bs.w, bs.q, bs.radius, bs.density, bs.crust = [a.ctypes.data for a in keep]
+        models = {(a, b): (kind, prm) for (a, b, kind, prm) in wl["models"]}
+        kind, pp = models[(0, 0)]
+        kind_w, pw = models.get((0, 1), models[(0, 0)])
+        if kind_w != kind:
+            raise ValueError("particle-particle and particle-wall models must be of one kind")
+        bs.kind = int(kind)
+        for k in range(5):
+            bs.pp[k] = float(pp[k])
+            bs.pw[k] = float(pw[k])
+        walls = wl["walls"]
+        bs.nwalls = len(walls)
+        if walls:
+            wb = _f64([w[0] for w in walls], (len(walls), 9))
+            wc = _f64([w[1] for w in walls], (len(walls), 3))
+            wh = _f64([w[2] for w in walls], (len(walls), 3))
+            keep += [wb, wc, wh]
+            bs.wall_basis, bs.wall_centre, bs.wall_half = wb.ctypes.data, wc.ctypes.data, wh.ctypes.data
+            bs.wall_crust = float(walls[0][3])
+        sid = C.c_int32()
+        self._check(self.lib.gdem_batch_add_system(self.h, C.byref(bs), C.byref(sid)),
+                    "gdem_batch_add_system")
+        self.sizes.append(n)
+        return sid.value
+
+    def commit(self):
+        self._check(self.lib.gdem_batch_commit(self.h), "gdem_batch_commit")
+
+    def set_launch(self, block_size, workgroups=0):
+        self._check(self.lib.gdem_batch_set_launch(self.h, int(block_size), int(workgroups)),
+                    "gdem_batch_set_launch")
+
+    def prime(self):
+        self._check(self.lib.gdem_batch_prime(self.h), "gdem_batch_prime")
+
+    def step(self, nsteps, dt):
+        self._check(self.lib.gdem_batch_step(self.h, int(nsteps), float(dt)), "gdem_batch_step")
+
+    def synchronize(self):
+        self._check(self.lib.gdem_batch_synchronize(self.h), "gdem_batch_synchronize")
+
+    def get_state(self, sys):
+        n = self.sizes[sys]
+        out = {k: np.empty((n, 3)) for k in ("x", "v", "w", "F", "T")}
+        out["q"] = np.empty((n, 4))
+        self._check(self.lib.gdem_batch_get_state(self.h, int(sys), _ptr(out["x"]), _ptr(out["v"]),
+                                                  _ptr(out["w"]), _ptr(out["q"]), _ptr(out["F"]),
+                                                  _ptr(out["T"])), "gdem_batch_get_state")
+        return out
+
+    def set_kernel_timing(self, on):
+        self._check(self.lib.gdem_batch_set_kernel_timing(self.h, int(bool(on))),
+                    "gdem_batch_set_kernel_timing")
+
+    def counters(self):
+        st, ct, ln = C.c_int64(), C.c_int64(), C.c_int64()
+        ms = C.c_double()
+        self._check(self.lib.gdem_batch_get_counters(self.h, C.byref(st), C.byref(ct), C.byref(ms),
+                                                     C.byref(ln)), "gdem_batch_get_counters")
+        return {"steps": st.value, "contacts": ct.value, "kernel_ms": ms.value, "launches": ln.value}
+
+
+def build_batch(systems, device=0, block_size=256, workgroups=0):
+    """A committed GdemBatch holding every workload dict of systems."""
+    b = GdemBatch(device)
+    for wl in systems:
+        b.add_system(wl)
+    b.commit()
+    b.set_launch(block_size, workgroups)
+    return b

@@ -206,3 +206,48 @@ def _lib_convex():
         lib.gdw_rsa_convex.argtypes = [i32, vp, vp, vp, vp, vp, vp, d, d, u64, C.c_long, vp, vp, vp, vp]
         lib._convex_ready = True
     return lib
+
+
+# ------------------------------------------------------------------ batched
+def batch_system(n, seed, aspect, kn, en, mu, cell_factor=1.0, spacing=2.3e-3, jitter=4.0e-5,
+                 sigma_v=0.05):
+    """One small settling system: n spheres, r ~ U[0.9, 1.1] mm, on a jittered
+    cubic lattice (spacing 2.3 mm) in a closed box whose height is about
+    `aspect` times its width, six walls, gravity -9.81 z, Hooke (kn, en, mu)
+    for particle-particle and particle-wall contacts, dt 1e-5 s."""
+    import math
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(0.9e-3, 1.1e-3, n)
+    nx = max(1, math.ceil((n / aspect) ** (1.0 / 3.0)))
+    nz = max(1, math.ceil(n / (nx * nx)))
+    lo = np.zeros(3)
+    hi = np.array([nx + 0.5, nx + 0.5, nz + 0.5]) * spacing
+    x = np.empty((n, 3))
+    v = np.empty((n, 3))
+    rc = _lib().gdw_lattice(n, spacing, _p(lo), _p(hi), jitter, sigma_v, int(seed), _p(x), _p(v))
+    if rc < 0:
+        raise ValueError("lattice too small")
+    hooke = (float(kn), float(en), -1.0, float(mu), 0.0)
+    parts = _particles(x, v, r)
+    return {"name": f"batch_system_{n}_s{seed}", "origin": tuple(lo), "size": tuple(hi - lo),
+            "periodic": (0, 0, 0), "cell_size": 2.0 * cell_factor * r.max(),
+            "gravity": (0.0, 0.0, -9.81), "dt": 1.0e-5, "steps": 1000,
+            "models": [(0, 0, MODEL_HOOKE, hooke), (0, 1, MODEL_HOOKE, hooke)],
+            "walls": box_walls(lo, hi, 1), "particles": parts}
+
+
+def batch_systems(nsys=4096, seed=4, nmin=64, nmax=2048, kn_range=(1.0e4, 2.0e5), cell_factor=1.0):
+    """Independent settling systems of a calibration sweep: particle count
+    U{nmin..nmax}, aspect ratio U[1, 4], kn log-uniform in kn_range, en
+    U[0.5, 0.95], mu U[0.1, 0.8] per system.  The upper end of kn stays at
+    2e5 N/m: with 1 mm spheres (m* ~ 5e-6 kg) the leap-frog step of 1e-5 s is
+    unstable above kn ~ 2.1e5 N/m (omega dt > 2)."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(nmin, nmax + 1, nsys)
+    aspect = rng.uniform(1.0, 4.0, nsys)
+    kn = np.exp(rng.uniform(np.log(kn_range[0]), np.log(kn_range[1]), nsys))
+    en = rng.uniform(0.5, 0.95, nsys)
+    mu = rng.uniform(0.1, 0.8, nsys)
+    seeds = rng.integers(1, 2 ** 62, nsys)
+    return [batch_system(int(ns[s]), int(seeds[s]), aspect[s], kn[s], en[s], mu[s], cell_factor)
+            for s in range(nsys)]

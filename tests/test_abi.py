@@ -43,3 +43,23 @@ def test_create_rejects_bad_config_without_device_work():
     h = ctypes.c_void_p()
     assert lib.gdem_create(ctypes.byref(cfg), ctypes.byref(h)) == gdem.ERR_ARG
     assert lib.gdem_create(None, ctypes.byref(h)) == gdem.ERR_ARG
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """ctypes mirrors of gdem_config and gdem_batch_system have the C layout
+    (sizes and field offsets from a program compiled against the header)."""
+    import subprocess
+    src = tmp_path / "layout.c"
+    src.write_text(
+        "#include <stdio.h>\n#include <stddef.h>\n#include \"gdem.h\"\n"
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(gdem_config),"
+        " offsetof(gdem_config, cell_size), offsetof(gdem_config, device),"
+        " sizeof(gdem_batch_system), offsetof(gdem_batch_system, kind),"
+        " offsetof(gdem_batch_system, pp), offsetof(gdem_batch_system, wall_crust));return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)])
+    got = [int(v) for v in subprocess.check_output([str(exe)]).split()]
+    B = gdem._BatchSystem
+    assert got == [ctypes.sizeof(gdem._Config), gdem._Config.cell_size.offset,
+                   gdem._Config.device.offset, ctypes.sizeof(B), B.kind.offset, B.pp.offset,
+                   B.wall_crust.offset]
