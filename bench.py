@@ -112,14 +112,98 @@ def cpu_baseline(wl, steps, warm=2):
     return n * steps / el, el
 
 
+def run_batch(args):
+    """Batched calibration-sweep systems: one persistent launch for all of
+    them, against looping the single-system step over a sample of them."""
+    from pacific_amd import autotune, gdem, workloads
+    steps = args.steps or 1000
+    warm = 10 if args.warmup is None else args.warmup
+    t = time.time()
+    systems = workloads.batch_systems(args.systems, seed=4 if args.seed is None else args.seed)
+    ntot = sum(len(w["particles"]["radius"]) for w in systems)
+    dt = systems[0]["dt"]
+    log(f"{len(systems)} systems, {ntot} particles, generated in {time.time() - t:.1f} s")
+    cache = os.path.join(os.environ.get("TMPDIR", "/tmp"), "gdem_autotune.json")
+    tuning = autotune.tune(systems, dt, sample=min(256, len(systems)), steps=20, cache=cache, log=log)
+    log(f"autotune: workgroup {tuning['block_size']}, cell factor {tuning['cell_factor']}")
+    tuned = autotune.with_cell_factor(systems, tuning["cell_factor"])
+    b = gdem.build_batch(tuned, block_size=tuning["block_size"])
+    b.prime()
+    b.step(warm, dt)
+    b.synchronize()
+    log(f"warmup {warm} steps done")
+    b.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    b.step(steps, dt)
+    b.synchronize()
+    el = time.perf_counter() - t0
+    cnt = b.counters()
+    value = ntot * steps / el
+    log(f"timed {steps} steps in {el:.3f} s (kernel {cnt['kernel_ms']:.1f} ms): "
+        f"{value:.4e} particle-steps/s")
+    b.close()
+    # the plain single-system step looped over a sample of the systems
+    sample = autotune.sample_systems(tuned, 32)
+    loop_steps = min(steps, 100)
+    tot_ps, tot_t = 0, 0.0
+    for wl in sample:
+        s1 = gdem.build_system(wl)
+        s1.prime()
+        s1.step(5, dt)
+        s1.synchronize()
+        t1 = time.perf_counter()
+        s1.step(loop_steps, dt)
+        s1.synchronize()
+        tot_t += time.perf_counter() - t1
+        tot_ps += len(wl["particles"]["radius"]) * loop_steps
+        s1.close()
+    loop_rate = tot_ps / tot_t
+    log(f"looped single-system step: {loop_rate:.4e} particle-steps/s (32 systems x {loop_steps} steps)")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import py_oracle
+        cs = args.cpu_steps or 1000
+        cps, cel = 0, 0.0
+        for wl in sample:
+            o = py_oracle.Oracle(wl)
+            o.prime()
+            t2 = time.perf_counter()
+            o.step(cs, dt)
+            cel += time.perf_counter() - t2
+            cps += len(wl["particles"]["radius"]) * cs
+        cpu = {"value": cps / cel, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"the same 32 systems ({sum(len(w['particles']['radius']) for w in sample)} "
+                         f"spheres) x {cs} steps, one after the other, single thread ({cel:.1f} s)"}
+        log(f"CPU {cpu['value']:.4e} particle-steps/s ({cel:.1f} s)")
+    out = {
+        "metric": "particle-steps/s, 4,096 batched settling systems (secondary line)",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": el / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batch_{len(systems)}", "systems": len(systems),
+                   "n_particles": ntot, "particles_per_system": [64, 2048],
+                   "contact": "Hooke+Coulomb, per-system kn/en/mu", "dt": dt,
+                   "block_size": tuning["block_size"], "cell_factor": tuning["cell_factor"],
+                   "parallelism": "single GPU, one persistent launch"},
+        "looped_single_system": {"value": loop_rate, "sample": f"32 systems x {loop_steps} steps"},
+        "speedup_vs_looped": value / loop_rate,
+        "autotune": tuning["table"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="500 (spheres) / 300 (convex)")
     ap.add_argument("--warmup", type=int, default=None, help="20 (spheres) / 10 (convex)")
-    ap.add_argument("--workload", choices=["periodic_hertz", "convex_box"], default="periodic_hertz",
+    ap.add_argument("--workload", choices=["periodic_hertz", "convex_box", "batch"],
+                    default="periodic_hertz",
                     help="periodic_hertz: 1M-sphere throughput case (default); convex_box: "
-                         "200k cubes + tetrahedra with GJK, a secondary line")
+                         "200k cubes + tetrahedra with GJK; batch: 4,096 small settling "
+                         "systems in one persistent launch (secondary lines)")
+    ap.add_argument("--systems", type=int, default=4096, help="batch: number of systems")
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=None, help="20 (spheres) / 5 (convex)")
@@ -131,6 +215,9 @@ def main():
         sys.exit(2)
 
     from pacific_amd import gdem
+    if args.workload == "batch":
+        run_batch(args)
+        return
     convex = args.workload == "convex_box"
     if args.steps is None:
         args.steps = 300 if convex else 500

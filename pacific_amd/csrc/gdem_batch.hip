@@ -123,6 +123,23 @@ struct BatchRunner {
 
   __device__ __forceinline__ unsigned long long* err() const { return a.err + (size_t)s * kErrWords; }
 
+  // True once any error of this system is raised: the reference stops at the
+  // exception, so the system is not advanced further (its state is then not
+  // meaningful).  Read with agent-scope atomic loads (the words are written by
+  // atomics) and broadcast through LDS, so the branch is uniform.
+  __device__ __forceinline__ bool failed(int* flag) {
+    if (threadIdx.x == 0) {
+      unsigned long long all = kNoErr;
+      for (int k = 0; k < kErrWords; ++k)
+        all &= __hip_atomic_load(&err()[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      *flag = all != kNoErr;
+    }
+    __syncthreads();
+    const bool f = *flag != 0;
+    __syncthreads();
+    return f;
+  }
+
   // Counting sort by cell key of the current copy (as launch_binning).
   __device__ __forceinline__ void bin(const State& cur) {
     const int n = sy.n, nc = sy.grid.ncell;
@@ -266,6 +283,7 @@ struct BatchRunner {
                       0.5 * dt, dt, a.key + sy.off, err());
       __syncthreads();
       for (int k = 0; k < a.nsteps; ++k) {
+        if (failed(smem)) break;
         bin(sst[cur]);
         detect(sst[cur]);
         if (k == a.nsteps - 1) force<MODE_KICK>(sst[cur], sst[cur ^ 1], 0.5 * dt, dt, true);

@@ -208,15 +208,14 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
     R.contact(en.x, en.y);
   }
   const int2* il = a.in_list + (size_t)t * kListCap;
-  int last = -1;
+  long long last = -1;
   for (int e = 0; e < nin; ++e) {
-    int best = 0x7fffffff, code = 0;
-    for (int f = 0; f < nin; ++f) {
-      const int2 en = il[f];
-      if (en.x > last && en.x < best) { best = en.x; code = en.y; }
-    }
-    R.contact(best, code);
-    last = best;
+    long long key;
+    const int f = next_incoming(il, nin, last, key);
+    if (f < 0) break;
+    const int2 en = il[f];
+    R.contact(en.x, en.y);
+    last = key;
   }
   // SimpleObstacle::InterAction (Component/src/SimpleObstacle.cpp:222-271)
   for (int wi = 0; wi < a.nwalls; ++wi) {

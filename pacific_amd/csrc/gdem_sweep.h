@@ -101,6 +101,28 @@ __device__ __forceinline__ void half_stencil_sweep(const Grid& g, const int32_t*
   }
 }
 
+// Next entry of an incoming contact list in the order (partner slot, code,
+// list position), after the entry of key `last` (-1 before the first): returns
+// its position and key, or -1 when every entry has been taken.  The position
+// breaks ties, so an entry that appears twice (a particle swept from a cell it
+// is not binned in, after it left the grid) is still visited once per copy and
+// the walk never runs past the list.
+__host__ __device__ __forceinline__ int next_incoming(const int2* il, int nin, long long last,
+                                                      long long& key) {
+  long long best = 0x7fffffffffffffffll;
+  int bi = -1;
+  for (int f = 0; f < nin; ++f) {
+    const int2 en = il[f];
+    const long long k = ((long long)en.x << 12) | ((long long)(en.y & 127) << 5) | (long long)f;
+    if (k > last && k < best) {
+      best = k;
+      bi = f;
+    }
+  }
+  key = best;
+  return bi;
+}
+
 // Cell of a position clamped to the grid: the cell whose half stencil a
 // particle sweeps (a particle outside a non-periodic grid is reported by the
 // binning as a SimulationError).

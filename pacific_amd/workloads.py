@@ -236,12 +236,13 @@ def batch_system(n, seed, aspect, kn, en, mu, cell_factor=1.0, spacing=2.3e-3, j
             "walls": box_walls(lo, hi, 1), "particles": parts}
 
 
-def batch_systems(nsys=4096, seed=4, nmin=64, nmax=2048, kn_range=(1.0e4, 2.0e5), cell_factor=1.0):
+def batch_systems(nsys=4096, seed=4, nmin=64, nmax=2048, kn_range=(1.0e4, 1.0e5), cell_factor=1.0):
     """Independent settling systems of a calibration sweep: particle count
     U{nmin..nmax}, aspect ratio U[1, 4], kn log-uniform in kn_range, en
     U[0.5, 0.95], mu U[0.1, 0.8] per system.  The upper end of kn stays at
-    2e5 N/m: with 1 mm spheres (m* ~ 5e-6 kg) the leap-frog step of 1e-5 s is
-    unstable above kn ~ 2.1e5 N/m (omega dt > 2)."""
+    1e5 N/m: for two 0.9 mm spheres (m* = 3.8e-6 kg) the leap-frog step of
+    1e-5 s is unstable above kn = 4 m* / dt^2 = 1.5e5 N/m (omega dt > 2), and
+    systems drawn above it blow up within a few hundred steps."""
     rng = np.random.default_rng(seed)
     ns = rng.integers(nmin, nmax + 1, nsys)
     aspect = rng.uniform(1.0, 4.0, nsys)
