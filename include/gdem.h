@@ -257,6 +257,43 @@ int gdem_batch_get_counters(gdem_batch* b, int64_t* steps, int64_t* contacts,
                             double* kernel_ms, int64_t* launches);
 int gdem_batch_info(gdem_batch* b, int32_t* nsys, int64_t* nparticles);
 
+/* ------------------------------------------------------------------------
+ * Differentiated calibration: loss = mean square error of particle positions
+ * at observation steps against target trajectories, and its gradient with
+ * respect to each system's Hooke parameters (kn, en, mu; etat = -1, the same
+ * set for particle-particle and particle-wall contacts) and to the initial
+ * velocities, by a hand-written adjoint of the contact-force and leap-frog
+ * kernels (checkpoints every K steps, recomputation per window, the contact
+ * set of each step held as the forward found it).  There is no counterpart in
+ * the reference, which has no derivatives; the forward is its step
+ * (Base/src/Grains.cpp:344-364) for spheres.  Systems are described as for
+ * gdem_batch_add_system (pw must equal pp, kind Hooke, pp[2] = -1).  Arrays
+ * over particles concatenate the systems in add order, ids within.
+ * ------------------------------------------------------------------------ */
+typedef struct gdem_diff gdem_diff;
+
+int gdem_diff_create(int32_t device, gdem_diff** out);
+int gdem_diff_destroy(gdem_diff* d);
+int gdem_diff_add_system(gdem_diff* d, const gdem_batch_system* sys, int32_t* sys_id);
+/* nsteps of dt, checkpoints every checkpoint_every steps, observation steps
+   (strictly increasing, in 1..nsteps). */
+int gdem_diff_commit(gdem_diff* d, int32_t nsteps, double dt, int32_t checkpoint_every,
+                     int32_t nobs, const int32_t* obs_steps);
+int gdem_diff_set_launch(gdem_diff* d, int32_t block_size, int32_t workgroups);
+/* theta: nsys x (kn, en, mu); v0: total particles x 3.  NULL keeps a value. */
+int gdem_diff_set_parameters(gdem_diff* d, const double* theta, const double* v0);
+/* Target positions, nobs x total particles x 3. */
+int gdem_diff_set_targets(gdem_diff* d, const double* target);
+/* Positions at the observation steps of the current parameters (nobs x total x 3). */
+int gdem_diff_record(gdem_diff* d, double* xobs);
+/* Loss of every system (nsys). */
+int gdem_diff_loss(gdem_diff* d, double* loss);
+/* Loss, d loss / d theta (nsys x 3) and d loss / d v0 (total x 3); any
+   pointer may be NULL. */
+int gdem_diff_gradient(gdem_diff* d, double* loss, double* grad_theta, double* grad_v0);
+int gdem_diff_last_error(gdem_diff* d, int32_t* sys, int32_t* code, int32_t* id0, int32_t* id1);
+int gdem_diff_info(gdem_diff* d, int32_t* nsys, int64_t* nparticles);
+
 #ifdef __cplusplus
 }
 #endif

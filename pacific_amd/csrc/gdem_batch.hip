@@ -584,6 +584,7 @@ static int batch_launch(gdem_batch* b, int prime, int32_t nsteps, double dt) {
   a.walls = b->d_walls;
   a.err = b->d_err;
   a.contacts = b->d_contacts;
+  (void)hipGetLastError();  // an earlier failed call must not be reported as this launch's
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, b->stream, b->d_next);
   if (b->timing) {
     if (!b->ev0) {

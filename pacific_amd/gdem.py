@@ -119,6 +119,18 @@ def load_library(path: str = LIB_PATH):
         "gdem_batch_set_kernel_timing": (C.c_int, [vp, i32]),
         "gdem_batch_get_counters": (C.c_int, [vp, P(i64), P(i64), P(d), P(i64)]),
         "gdem_batch_info": (C.c_int, [vp, P(i32), P(i64)]),
+        "gdem_diff_create": (C.c_int, [i32, P(vp)]),
+        "gdem_diff_destroy": (C.c_int, [vp]),
+        "gdem_diff_add_system": (C.c_int, [vp, P(_BatchSystem), P(i32)]),
+        "gdem_diff_commit": (C.c_int, [vp, i32, d, i32, i32, vp]),
+        "gdem_diff_set_launch": (C.c_int, [vp, i32, i32]),
+        "gdem_diff_set_parameters": (C.c_int, [vp, vp, vp]),
+        "gdem_diff_set_targets": (C.c_int, [vp, vp]),
+        "gdem_diff_record": (C.c_int, [vp, vp]),
+        "gdem_diff_loss": (C.c_int, [vp, vp]),
+        "gdem_diff_gradient": (C.c_int, [vp, vp, vp, vp]),
+        "gdem_diff_last_error": (C.c_int, [vp, P(i32), P(i32), P(i32), P(i32)]),
+        "gdem_diff_info": (C.c_int, [vp, P(i32), P(i64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -138,7 +150,11 @@ EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall"
             "gdem_get_basis", "gdem_batch_create", "gdem_batch_destroy", "gdem_batch_add_system",
             "gdem_batch_commit", "gdem_batch_set_launch", "gdem_batch_prime", "gdem_batch_step",
             "gdem_batch_synchronize", "gdem_batch_get_state", "gdem_batch_last_error",
-            "gdem_batch_set_kernel_timing", "gdem_batch_get_counters", "gdem_batch_info"]
+            "gdem_batch_set_kernel_timing", "gdem_batch_get_counters", "gdem_batch_info",
+            "gdem_diff_create", "gdem_diff_destroy", "gdem_diff_add_system", "gdem_diff_commit",
+            "gdem_diff_set_launch", "gdem_diff_set_parameters", "gdem_diff_set_targets",
+            "gdem_diff_record", "gdem_diff_loss", "gdem_diff_gradient", "gdem_diff_last_error",
+            "gdem_diff_info"]
 
 
 def _ptr(a):
@@ -343,6 +359,43 @@ def build_system(wl, device=0):
     return s
 
 
+def _batch_system(wl):
+    """gdem_batch_system record of a workload dict (and the arrays it points to)."""
+    p = wl["particles"]
+    n = len(p["radius"])
+    bs = _BatchSystem()
+    for d in range(3):
+        bs.cfg.origin[d] = float(wl["origin"][d])
+        bs.cfg.size[d] = float(wl["size"][d])
+        bs.cfg.periodic[d] = int(bool(wl["periodic"][d]))
+        bs.cfg.gravity[d] = float(wl["gravity"][d])
+    bs.cfg.cell_size = float(wl["cell_size"])
+    keep = [_f64(p["x"], (n, 3)), _f64(p["v"], (n, 3)), _f64(p["w"], (n, 3)),
+            _f64(p["q"], (n, 4)), _f64(p["radius"], (n,)), _f64(p["density"], (n,)),
+            _f64(p["crust"], (n,))]
+    bs.n = n
+    bs.x, bs.v, bs.w, bs.q, bs.radius, bs.density, bs.crust = [a.ctypes.data for a in keep]
+    models = {(a, b): (kind, prm) for (a, b, kind, prm) in wl["models"]}
+    kind, pp = models[(0, 0)]
+    kind_w, pw = models.get((0, 1), models[(0, 0)])
+    if kind_w != kind:
+        raise ValueError("particle-particle and particle-wall models must be of one kind")
+    bs.kind = int(kind)
+    for k in range(5):
+        bs.pp[k] = float(pp[k])
+        bs.pw[k] = float(pw[k])
+    walls = wl["walls"]
+    bs.nwalls = len(walls)
+    if walls:
+        wb = _f64([w[0] for w in walls], (len(walls), 9))
+        wc = _f64([w[1] for w in walls], (len(walls), 3))
+        wh = _f64([w[2] for w in walls], (len(walls), 3))
+        keep += [wb, wc, wh]
+        bs.wall_basis, bs.wall_centre, bs.wall_half = wb.ctypes.data, wc.ctypes.data, wh.ctypes.data
+        bs.wall_crust = float(walls[0][3])
+    return bs, keep
+
+
 class GdemBatch:
     """Many independent sphere systems advanced by one persistent launch per
     call (include/gdem.h, gdem_batch_*).  Each system is a workload dict with
@@ -380,42 +433,11 @@ class GdemBatch:
         raise err
 
     def add_system(self, wl):
-        p = wl["particles"]
-        n = len(p["radius"])
-        bs = _BatchSystem()
-        for d in range(3):
-            bs.cfg.origin[d] = float(wl["origin"][d])
-            bs.cfg.size[d] = float(wl["size"][d])
-            bs.cfg.periodic[d] = int(bool(wl["periodic"][d]))
-            bs.cfg.gravity[d] = float(wl["gravity"][d])
-        bs.cfg.cell_size = float(wl["cell_size"])
-        keep = [_f64(p["x"], (n, 3)), _f64(p["v"], (n, 3)), _f64(p["w"], (n, 3)),
-                _f64(p["q"], (n, 4)), _f64(p["radius"], (n,)), _f64(p["density"], (n,)),
-                _f64(p["crust"], (n,))]
-        bs.n = n
-        bs.x, bs.v, bs.w, bs.q, bs.radius, bs.density, bs.crust = [a.ctypes.data for a in keep]
-        models = {(a, b): (kind, prm) for (a, b, kind, prm) in wl["models"]}
-        kind, pp = models[(0, 0)]
-        kind_w, pw = models.get((0, 1), models[(0, 0)])
-        if kind_w != kind:
-            raise ValueError("particle-particle and particle-wall models must be of one kind")
-        bs.kind = int(kind)
-        for k in range(5):
-            bs.pp[k] = float(pp[k])
-            bs.pw[k] = float(pw[k])
-        walls = wl["walls"]
-        bs.nwalls = len(walls)
-        if walls:
-            wb = _f64([w[0] for w in walls], (len(walls), 9))
-            wc = _f64([w[1] for w in walls], (len(walls), 3))
-            wh = _f64([w[2] for w in walls], (len(walls), 3))
-            keep += [wb, wc, wh]
-            bs.wall_basis, bs.wall_centre, bs.wall_half = wb.ctypes.data, wc.ctypes.data, wh.ctypes.data
-            bs.wall_crust = float(walls[0][3])
+        bs, keep = _batch_system(wl)
         sid = C.c_int32()
         self._check(self.lib.gdem_batch_add_system(self.h, C.byref(bs), C.byref(sid)),
                     "gdem_batch_add_system")
-        self.sizes.append(n)
+        self.sizes.append(bs.n)
         return sid.value
 
     def commit(self):
@@ -463,3 +485,83 @@ def build_batch(systems, device=0, block_size=256, workgroups=0):
     b.commit()
     b.set_launch(block_size, workgroups)
     return b
+
+
+class GdemDiff:
+    """Loss and adjoint gradient of calibration trajectories (include/gdem.h,
+    gdem_diff_*): systems as for GdemBatch, Hooke with one (kn, en, mu) per
+    system."""
+
+    def __init__(self, systems, nsteps, dt, checkpoint_every, obs_steps, device=0, block_size=256,
+                 workgroups=0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        self._check(self.lib.gdem_diff_create(int(device), C.byref(h)), "gdem_diff_create")
+        self.h = h
+        self.sizes = []
+        for wl in systems:
+            bs, keep = _batch_system(wl)
+            sid = C.c_int32()
+            self._check(self.lib.gdem_diff_add_system(self.h, C.byref(bs), C.byref(sid)),
+                        "gdem_diff_add_system")
+            self.sizes.append(bs.n)
+        obs = np.ascontiguousarray(obs_steps, dtype=np.int32)
+        self.nobs = len(obs)
+        self._check(self.lib.gdem_diff_commit(self.h, int(nsteps), float(dt), int(checkpoint_every),
+                                              self.nobs, _ptr(obs)), "gdem_diff_commit")
+        self._check(self.lib.gdem_diff_set_launch(self.h, int(block_size), int(workgroups)),
+                    "gdem_diff_set_launch")
+        self.nsys = len(self.sizes)
+        self.ntot = int(sum(self.sizes))
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gdem_diff_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc == OK:
+            return
+        sysid = id0 = id1 = -1
+        if getattr(self, "h", None):
+            s, code, a, b = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+            self.lib.gdem_diff_last_error(self.h, C.byref(s), C.byref(code), C.byref(a), C.byref(b))
+            sysid, id0, id1 = s.value, a.value, b.value
+        err = _ERRORS.get(rc, GdemError)(rc, f"{what} (system {sysid})", id0, id1)
+        err.system = sysid
+        raise err
+
+    def set_parameters(self, theta=None, v0=None):
+        th = None if theta is None else _f64(theta, (self.nsys, 3))
+        vv = None if v0 is None else _f64(v0, (self.ntot, 3))
+        self._check(self.lib.gdem_diff_set_parameters(self.h, _ptr(th), _ptr(vv)),
+                    "gdem_diff_set_parameters")
+
+    def set_targets(self, target):
+        t = _f64(target, (self.nobs, self.ntot, 3))
+        self._check(self.lib.gdem_diff_set_targets(self.h, _ptr(t)), "gdem_diff_set_targets")
+
+    def record(self):
+        out = np.empty((self.nobs, self.ntot, 3))
+        self._check(self.lib.gdem_diff_record(self.h, _ptr(out)), "gdem_diff_record")
+        return out
+
+    def loss(self):
+        out = np.empty(self.nsys)
+        self._check(self.lib.gdem_diff_loss(self.h, _ptr(out)), "gdem_diff_loss")
+        return out
+
+    def gradient(self):
+        loss = np.empty(self.nsys)
+        gth = np.empty((self.nsys, 3))
+        gv0 = np.empty((self.ntot, 3))
+        self._check(self.lib.gdem_diff_gradient(self.h, _ptr(loss), _ptr(gth), _ptr(gv0)),
+                    "gdem_diff_gradient")
+        return loss, gth, gv0

@@ -252,3 +252,43 @@ def batch_systems(nsys=4096, seed=4, nmin=64, nmax=2048, kn_range=(1.0e4, 1.0e5)
     seeds = rng.integers(1, 2 ** 62, nsys)
     return [batch_system(int(ns[s]), int(seeds[s]), aspect[s], kn[s], en[s], mu[s], cell_factor)
             for s in range(nsys)]
+
+
+# ------------------------------------------------------------------ calibration
+def calibration_system(n=1000, seed=1, spacing=2.1e-3, jitter=4.0e-5, sigma_v=0.2,
+                       theta=(5.0e4, 0.8, 0.4)):
+    """One calibration system: n spheres of radius 1 mm on a jittered cubic
+    lattice (spacing 2.1 mm, so neighbours collide within the first
+    milliseconds), v ~ N(0, sigma_v^2), in a closed box of aspect ratio about
+    2, gravity -9.81 z, Hooke (kn, en, mu) = theta for particle-particle and
+    particle-wall contacts, dt 1e-5 s."""
+    import math
+    nx = max(1, math.ceil((n / 2.0) ** (1.0 / 3.0)))
+    nz = max(1, math.ceil(n / (nx * nx)))
+    lo = np.zeros(3)
+    hi = np.array([nx + 0.5, nx + 0.5, nz + 0.5]) * spacing
+    x = np.empty((n, 3))
+    v = np.empty((n, 3))
+    rc = _lib().gdw_lattice(n, spacing, _p(lo), _p(hi), jitter, sigma_v, int(seed), _p(x), _p(v))
+    if rc < 0:
+        raise ValueError("lattice too small")
+    r = np.full(n, 1.0e-3)
+    hooke = (float(theta[0]), float(theta[1]), -1.0, float(theta[2]), 0.0)
+    return {"name": f"calibration_{n}_s{seed}", "origin": tuple(lo), "size": tuple(hi - lo),
+            "periodic": (0, 0, 0), "cell_size": 2.0 * r.max(), "gravity": (0.0, 0.0, -9.81),
+            "dt": 1.0e-5, "steps": 400,
+            "models": [(0, 0, MODEL_HOOKE, hooke), (0, 1, MODEL_HOOKE, hooke)],
+            "walls": box_walls(lo, hi, 1), "particles": _particles(x, v, r)}
+
+
+def calibration_systems(nsys=512, n=1000, seed=5, kn_range=(1.0e4, 1.0e5)):
+    """Systems of a calibration run and their hidden parameters (kn, en, mu),
+    drawn as for batch_systems: kn log-uniform in kn_range, en U[0.5, 0.95],
+    mu U[0.1, 0.8]."""
+    rng = np.random.default_rng(seed)
+    kn = np.exp(rng.uniform(np.log(kn_range[0]), np.log(kn_range[1]), nsys))
+    en = rng.uniform(0.5, 0.95, nsys)
+    mu = rng.uniform(0.1, 0.8, nsys)
+    seeds = rng.integers(1, 2 ** 62, nsys)
+    theta = np.stack([kn, en, mu], 1)
+    return [calibration_system(n, int(seeds[s]), theta=theta[s]) for s in range(nsys)], theta
