@@ -193,12 +193,93 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def run_calibration(args, nsys=None):
+    """Batched adjoint gradients of calibration trajectories against central
+    finite differences looped over systems with the single-system step."""
+    from pacific_amd import gdem, workloads
+    nsys = nsys or (args.systems if args.systems != 4096 else 512)
+    nsteps, K, obs, dt = 400, 50, (100, 200, 300, 400), 1.0e-5
+    t = time.time()
+    systems, hidden = workloads.calibration_systems(nsys, n=1000, seed=5)
+    v0 = np.concatenate([w["particles"]["v"] for w in systems])
+    log(f"{nsys} calibration systems of 1000 spheres generated in {time.time() - t:.1f} s")
+    eng = gdem.GdemDiff(systems, nsteps, dt, K, obs)
+    eng.set_parameters(hidden, v0)
+    target = eng.record()
+    eng.set_targets(target)
+    guess = np.array([1.2, 0.9, 1.1])
+    theta = hidden * guess[None, :]
+    eng.set_parameters(theta, v0)
+    eng.gradient()  # warm-up
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loss, gth, gv0 = eng.gradient()
+    tg = (time.perf_counter() - t0) / reps
+    rate = nsys / tg
+    log(f"adjoint: {tg:.3f} s per launch for {nsys} systems: {rate:.1f} gradients/s")
+    # accuracy against central differences of the same loss (h = 1e-6 relative)
+    worst = 0.0
+    for s in range(2):
+        for c in range(3):
+            h = 1e-6 * theta[s, c]
+            tp, tm = theta.copy(), theta.copy()
+            tp[s, c] += h
+            tm[s, c] -= h
+            eng.set_parameters(tp)
+            lp = eng.loss()[s]
+            eng.set_parameters(tm)
+            lm = eng.loss()[s]
+            fd = (lp - lm) / (2 * h)
+            worst = max(worst, abs(gth[s, c] - fd) / max(abs(fd), 1e-300))
+    eng.set_parameters(theta)
+    log(f"adjoint vs central differences: max relative error {worst:.2e}")
+    eng.close()
+    # the single-system step looped over systems, gradient by central differences
+    nloop = 4
+    t1 = time.perf_counter()
+    for s in range(nloop):
+        wl = systems[s]
+        for c in range(3):
+            for sign in (1.0, -1.0):
+                th = theta[s].copy()
+                th[c] += sign * 1e-6 * theta[s, c]
+                w = dict(wl)
+                hooke = (th[0], th[1], -1.0, th[2], 0.0)
+                w["models"] = [(0, 0, gdem.MODEL_HOOKE, hooke), (0, 1, gdem.MODEL_HOOKE, hooke)]
+                one = gdem.build_system(w)
+                last = 0
+                for step in obs:
+                    one.step(step - last, dt)
+                    last = step
+                    one.get_state()
+                one.close()
+    tl = (time.perf_counter() - t1) / nloop
+    loop_rate = 1.0 / tl
+    log(f"looped single-system step with central differences: {tl:.3f} s per gradient "
+        f"({loop_rate:.2f} gradients/s)")
+    return {
+        "metric": "batched calibration gradients/s (secondary line)", "value": rate,
+        "unit": "gradients/s", "n_gpus": 1, "steps": nsteps, "warmup": 1,
+        "ms_per_step": tg / nsteps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"calibration_{nsys}x1000", "systems": nsys, "particles_per_system": 1000,
+                   "window": nsteps, "checkpoint_every": K, "observations": list(obs), "dt": dt,
+                   "gradient": "d loss / d (kn, en, mu) and d loss / d v0 per system",
+                   "parallelism": "single GPU, one persistent launch"},
+        "fd_max_rel_error": worst,
+        "looped_single_system_fd": {"value": loop_rate, "unit": "gradients/s",
+                                    "sample": f"{nloop} systems, 6 runs of {nsteps} steps each"},
+        "speedup_vs_looped": rate / loop_rate,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="500 (spheres) / 300 (convex)")
     ap.add_argument("--warmup", type=int, default=None, help="20 (spheres) / 10 (convex)")
-    ap.add_argument("--workload", choices=["periodic_hertz", "convex_box", "batch"],
+    ap.add_argument("--workload", choices=["periodic_hertz", "convex_box", "batch", "calibration"],
                     default="periodic_hertz",
                     help="periodic_hertz: 1M-sphere throughput case (default); convex_box: "
                          "200k cubes + tetrahedra with GJK; batch: 4,096 small settling "
@@ -217,6 +298,9 @@ def main():
     from pacific_amd import gdem
     if args.workload == "batch":
         run_batch(args)
+        return
+    if args.workload == "calibration":
+        print(json.dumps(run_calibration(args)), flush=True)
         return
     convex = args.workload == "convex_box"
     if args.steps is None:

@@ -173,6 +173,13 @@ int gdem_prime(gdem_ctx* ctx);
    half-kick and the next step's kick and drift.  Primes first if needed. */
 int gdem_step(gdem_ctx* ctx, int32_t nsteps, double dt);
 
+/* Replay of gdem_step's fused steps from a hipGraph captured on first use
+   (one graph per state-buffer parity and dt; default on).  Steps run
+   eagerly while contact recording or kernel timing is on.  Results are
+   bit-identical either way.  No counterpart in the reference (its loop is
+   host code, Grains.cpp:286-452). */
+int gdem_set_graph(gdem_ctx* ctx, int32_t on);
+
 /* Waits for the stream; returns the first deferred device error. */
 int gdem_synchronize(gdem_ctx* ctx);
 

@@ -78,6 +78,12 @@ struct gdem_ctx {
   double4 h_wall_sc[kMaxWalls]{};
   CShape* d_wall_shapes = nullptr;
   double4* d_wall_sc = nullptr;
+  // captured fused step (one graph per state-buffer parity)
+  bool use_graph = true;
+  int64_t graph_gen = 0;
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  double graph_dt[2] = {0., 0.};
+  int64_t graph_made[2] = {-1, -1};
 };
 
 #define HIP_TRY(expr)                                                        \
@@ -191,6 +197,8 @@ extern "C" int gdem_destroy(gdem_ctx* c) {
   (void)hipFree(c->d_cand); (void)hipFree(c->d_crec); (void)hipFree(c->d_mask); (void)hipFree(c->d_work);
   (void)hipFree(c->d_nwork); (void)hipFree(c->d_shapes); (void)hipFree(c->d_wall_shapes); (void)hipFree(c->d_wall_sc);
   for (auto& e : c->ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+  for (auto& g : c->graph_exec)
+    if (g) (void)hipGraphExecDestroy(g);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return GDEM_OK;
@@ -225,6 +233,7 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
   c->d_nwork = nullptr;
   c->n = n;
   c->convex = convex;
+  c->graph_gen++;
   const size_t nn = (size_t)std::max<int64_t>(n, 1);
   if (alloc_state(c->buf[0], n) || alloc_state(c->buf[1], n)) return GDEM_ERR_HIP;
   if (convex) {
@@ -523,6 +532,7 @@ extern "C" int gdem_add_wall(gdem_ctx* c, const double* basis, const double* cen
   c->h_wall_shapes[c->nwalls] = wsh;
   c->h_wall_sc[c->nwalls] = make_double4(wsc[0], wsc[1], wsc[2], norm(mk(e0, e1, e2)));
   c->h_walls[c->nwalls++] = wl;
+  c->graph_gen++;
   HIP_TRY(hipSetDevice(c->dev));
   HIP_TRY(hipMemcpyAsync(c->d_walls, c->h_walls, sizeof(Wall) * c->nwalls, hipMemcpyHostToDevice,
                          c->stream));
@@ -637,10 +647,13 @@ static void launch_convex_step(gdem_ctx* c, int mode, double dtv, double dtd, bo
   launch_cforce(mode, record, a, c->stream);
 }
 
-static int do_force(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft) {
+// Kernels of one force evaluation on the current copy (binning first when
+// positions changed); no host bookkeeping, so the sequence can be captured.
+static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft,
+                          std::pair<hipEvent_t, hipEvent_t>** evp) {
   if (!c->binned) {
-    int rc = do_binning(c);
-    if (rc) return rc;
+    launch_binning(c->bins, c->buf[c->cur], (int32_t)c->n, c->grid.ncell, c->stream);
+    HIP_TRY(hipGetLastError());
   }
   const int nblk = (int)((c->n + 255) / 256);
   const int nblk8 = ((nblk + 7) / 8) * 8;
@@ -705,10 +718,57 @@ static int do_force(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft
   }
   HIP_TRY(hipGetLastError());
   if (ev) HIP_TRY(hipEventRecord(ev->second, c->stream));
+  *evp = ev;
+  return GDEM_OK;
+}
+
+static void after_force(gdem_ctx* c) {
   c->cur = 1 - c->cur;
   c->binned = false;
   c->force_evals++;
-  if (record) c->rec_valid = true;
+  if (c->rec_cap > 0 && c->d_rec != nullptr) c->rec_valid = true;
+}
+
+static int do_force(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft) {
+  std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
+  const int rc = force_launches(c, mode, dtv, dtd, write_ft, &ev);
+  if (rc) return rc;
+  after_force(c);
+  return GDEM_OK;
+}
+
+// One fused step (binning, detection, force kernel with MODE_MOVE) replayed
+// from a graph captured on first use for the current buffer parity and dt.
+// Only used without contact recording and kernel timing; the captured
+// kernels clear their own counters (no memset nodes).
+static int graph_force(gdem_ctx* c, double dt) {
+  const int p = c->cur;
+  if (!c->graph_exec[p] || c->graph_dt[p] != dt || c->graph_made[p] != c->graph_gen) {
+    if (c->graph_exec[p]) {
+      HIP_TRY(hipGraphExecDestroy(c->graph_exec[p]));
+      c->graph_exec[p] = nullptr;
+    }
+    const bool binned = c->binned;
+    c->binned = false;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
+    const int rc = force_launches(c, MODE_MOVE, 0.5 * dt, dt, false, &ev);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    c->binned = binned;
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    HIP_TRY(e);
+    const hipError_t ei = hipGraphInstantiate(&c->graph_exec[p], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_TRY(ei);
+    c->graph_dt[p] = dt;
+    c->graph_made[p] = c->graph_gen;
+  }
+  HIP_TRY(hipGraphLaunch(c->graph_exec[p], c->stream));
+  after_force(c);
   return GDEM_OK;
 }
 
@@ -774,11 +834,19 @@ extern "C" int gdem_step(gdem_ctx* c, int32_t nsteps, double dt) {
   // opens the next one
   rc = gdem_move(c, 0.5 * dt, dt);
   if (rc) return rc;
+  const bool graph = c->use_graph && !c->timing && c->rec_cap == 0;
   for (int32_t k = 0; k < nsteps; ++k) {
     const bool last = (k == nsteps - 1);
-    rc = do_force(c, last ? MODE_KICK : MODE_MOVE, 0.5 * dt, dt, last);
+    if (!last && graph) rc = graph_force(c, dt);
+    else rc = do_force(c, last ? MODE_KICK : MODE_MOVE, 0.5 * dt, dt, last);
     if (rc) return rc;
   }
+  return GDEM_OK;
+}
+
+extern "C" int gdem_set_graph(gdem_ctx* c, int32_t on) {
+  if (!c) return GDEM_ERR_ARG;
+  c->use_graph = on != 0;
   return GDEM_OK;
 }
 
@@ -850,6 +918,7 @@ extern "C" int gdem_set_contact_recording(gdem_ctx* c, int32_t cap) {
   c->d_rec_count = nullptr;
   c->rec_cap = cap;
   c->rec_valid = false;
+  c->graph_gen++;
   if (cap > 0 && c->n > 0) {
     HIP_TRY(hipMalloc(&c->d_rec, sizeof(ContactRec) * (size_t)c->n * (size_t)cap));
     HIP_TRY(hipMalloc(&c->d_rec_count, sizeof(int32_t) * (size_t)c->n));

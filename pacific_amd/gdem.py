@@ -92,6 +92,7 @@ def load_library(path: str = LIB_PATH):
         "gdem_advance_velocity": (C.c_int, [vp, d]),
         "gdem_prime": (C.c_int, [vp]),
         "gdem_step": (C.c_int, [vp, i32, d]),
+        "gdem_set_graph": (C.c_int, [vp, i32]),
         "gdem_synchronize": (C.c_int, [vp]),
         "gdem_get_state": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
         "gdem_set_contact_recording": (C.c_int, [vp, i32]),
@@ -142,7 +143,7 @@ def load_library(path: str = LIB_PATH):
 
 EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall",
             "gdem_set_contact_model", "gdem_link_update", "gdem_compute_forces",
-            "gdem_move", "gdem_advance_velocity", "gdem_prime", "gdem_step",
+            "gdem_move", "gdem_advance_velocity", "gdem_prime", "gdem_step", "gdem_set_graph",
             "gdem_synchronize", "gdem_get_state", "gdem_set_contact_recording",
             "gdem_get_contacts", "gdem_last_error", "gdem_set_kernel_timing",
             "gdem_get_counters", "gdem_reset_counters", "gdem_get_grid", "gdem_version",
@@ -285,6 +286,9 @@ class GdemSystem:
 
     def synchronize(self):
         self._check(self.lib.gdem_synchronize(self.h), "gdem_synchronize")
+
+    def set_graph(self, on):
+        self._check(self.lib.gdem_set_graph(self.h, int(bool(on))), "gdem_set_graph")
 
     # -- results
     def get_state(self):
