@@ -298,6 +298,11 @@ int gdem_diff_loss(gdem_diff* d, double* loss);
 /* Loss, d loss / d theta (nsys x 3) and d loss / d v0 (total x 3); any
    pointer may be NULL. */
 int gdem_diff_gradient(gdem_diff* d, double* loss, double* grad_theta, double* grad_v0);
+/* Forward-mode directional derivative of every system's loss along
+   (dtheta: nsys x 3 in (kn, en, mu), dv0: total x 3), the contact set held
+   as in the gradient: an independent check of gdem_diff_gradient. */
+int gdem_diff_directional(gdem_diff* d, const double* dtheta, const double* dv0, double* loss,
+                          double* dloss);
 int gdem_diff_last_error(gdem_diff* d, int32_t* sys, int32_t* code, int32_t* id0, int32_t* id1);
 int gdem_diff_info(gdem_diff* d, int32_t* nsys, int64_t* nparticles);
 

@@ -50,7 +50,7 @@ struct DiffSys {
   double kn, en, mu, beta, dbeta;  // beta(en) and d beta / d en
 };
 
-enum { DIFF_LOSS = 0, DIFF_GRAD = 1, DIFF_RECORD = 2 };
+enum { DIFF_LOSS = 0, DIFF_GRAD = 1, DIFF_RECORD = 2, DIFF_TANGENT = 3 };
 
 struct DiffArgs {
   const DiffSys* sys;
@@ -81,6 +81,8 @@ struct DiffArgs {
   double4 *gX, *gV, *gW;                        // contact parts of one step's adjoint
   double* loss;    // per system
   double* gtheta;  // per system: d loss / d (kn, en, mu)
+  const double* dtheta;  // tangent mode: direction in (kn, en, mu) per system
+  double* dloss;         // tangent mode: directional derivative per system
   double4* gv0;    // d loss / d v0, id order
   unsigned long long* err;
 };
@@ -143,6 +145,30 @@ __device__ __forceinline__ double wall_dir(const Wall& wl, v3 x, v3 v, v3 w, dou
   return dot(aF, mk(FA.x.d, FA.y.d, FA.z.d)) + dot(aT, mk(TA.x.d, TA.y.d, TA.z.d));
 }
 
+// Tangent (Jacobian-vector product) of one contact: inputs carry tangents
+// tin, parameters (kn, beta, mu) tangents tp; returns the tangents of the
+// force and torque on A (a_side) or on B.
+struct PairTan {
+  v3 xA, vA, wA, xB, vB, wB;
+};
+__device__ __forceinline__ bool pair_jvp(const PairIn& in, const PairTan& tin, double kn, double beta,
+                                         double mu, v3 tp, bool a_side, v3& dF, v3& dT) {
+  auto dv = [](v3 a, v3 t) { return mk3(Dual{a.x, t.x}, Dual{a.y, t.y}, Dual{a.z, t.z}); };
+  HookeParams<Dual> p;
+  p.kn = Dual{kn, tp.x};
+  p.beta = Dual{beta, tp.y};
+  p.mu = Dual{mu, tp.z};
+  V3<Dual> FA, TA, FB, TB;
+  if (!pair_torsors(dv(in.xA, tin.xA), dv(in.vA, tin.vA), dv(in.wA, tin.wA), in.rA, in.crA, in.mA,
+                    dv(in.xB, tin.xB), dv(in.vB, tin.vB), dv(in.wB, tin.wB), in.rB, in.crB, in.mB, p,
+                    FA, TA, FB, TB))
+    return false;
+  const V3<Dual> F = a_side ? FA : FB, T = a_side ? TA : TB;
+  dF = mk(F.x.d, F.y.d, F.z.d);
+  dT = mk(T.x.d, T.y.d, T.z.d);
+  return true;
+}
+
 template <int BS>
 __device__ __forceinline__ double block_sum(double v, double* red) {
   // fixed-order tree over the workgroup
@@ -193,6 +219,7 @@ struct DiffRunner {
   double* red;
   DetectArgs* sd;
   double lsum;      // this thread's share of the loss
+  double dlsum;     // tangent mode: share of the directional derivative
   v3 gth;           // this thread's share of d loss / d (kn, en, mu)
 
   __device__ __forceinline__ unsigned long long* err() const { return a.err + (size_t)s * kErrWords; }
@@ -460,8 +487,97 @@ struct DiffRunner {
     __syncthreads();
   }
 
+  // ---- forward-mode tangents (DIFF_TANGENT): tx, tv, tw, tF, tT live in
+  // the adjoint arrays ax, av, aw, aF, aT
+  __device__ __forceinline__ v3 dtheta_beta() const {
+    // tangent of (kn, beta(en), mu) for the system's direction in (kn, en, mu)
+    const double* d = a.dtheta + 3 * s;
+    return mk(d[0], sy.dbeta * d[1], d[2]);
+  }
+
+  __device__ __forceinline__ void tangent_forces() {
+    const int n = sy.n;
+    const v3 tp = dtheta_beta();
+    const double4 *tx = a.ax + sy.off, *tv = a.av + sy.off, *tw = a.aw + sy.off;
+    for (int t = threadIdx.x; t < n; t += BS) {
+      const int p = a.perm[sy.off + t];
+      v3 dF = mk(0., 0., 0.), dT = mk(0., 0., 0.);
+      auto one = [&](int u, int code) {
+        bool meA;
+        v3 shift;
+        const int q = decode(u, code, meA, shift);
+        const PairIn in = pair_in(p, q, meA, shift);
+        const int A = meA ? p : q, B = meA ? q : p;
+        PairTan tin;
+        tin.xA = ld3(tx[A]); tin.vA = ld3(tv[A]); tin.wA = ld3(tw[A]);
+        tin.xB = ld3(tx[B]); tin.vB = ld3(tv[B]); tin.wB = ld3(tw[B]);
+        v3 f, tq;
+        if (pair_jvp(in, tin, sy.kn, sy.beta, sy.mu, tp, meA, f, tq)) {
+          dF = add(dF, f);
+          dT = add(dT, tq);
+        }
+      };
+      const int nout = min(a.out_count[sy.off + t], kListCap);
+      const int2* ol = a.out_list + (size_t)(sy.off + t) * kListCap;
+      for (int e = 0; e < nout; ++e) one(ol[e].x, ol[e].y);
+      const int nin = min(a.in_count[sy.off + t], kListCap);
+      const int2* il = a.in_list + (size_t)(sy.off + t) * kListCap;
+      for (int e = 0; e < nin; ++e) one(il[e].x, il[e].y);
+      const double4 pp = P()[p];
+      const v3 x = ld3(X()[p]), vv = ld3(V()[p]), ww = ld3(W()[p]);
+      const v3 dx = ld3(tx[p]), dvv = ld3(tv[p]), dww = ld3(tw[p]);
+      auto dv = [](v3 a0, v3 t0) { return mk3(Dual{a0.x, t0.x}, Dual{a0.y, t0.y}, Dual{a0.z, t0.z}); };
+      HookeParams<Dual> hp;
+      hp.kn = Dual{sy.kn, tp.x};
+      hp.beta = Dual{sy.beta, tp.y};
+      hp.mu = Dual{sy.mu, tp.z};
+      for (int wi = 0; wi < sy.nwalls; ++wi) {
+        const Wall& wl = a.walls[sy.wall_off + wi];
+        if (!(fabs(wl.bb_centre[0] - x.x) <= wl.bb_ext[0] + pp.x &&
+              fabs(wl.bb_centre[1] - x.y) <= wl.bb_ext[1] + pp.x &&
+              fabs(wl.bb_centre[2] - x.z) <= wl.bb_ext[2] + pp.x))
+          continue;
+        V3<Dual> FA, TA;
+        if (!wall_torsor(wl, dv(x, dx), dv(vv, dvv), dv(ww, dww), pp.x, pp.w, pp.y, hp, FA, TA)) continue;
+        dF = add(dF, mk(FA.x.d, FA.y.d, FA.z.d));
+        dT = add(dT, mk(TA.x.d, TA.y.d, TA.z.d));
+      }
+      a.aF[sy.off + p] = st3(dF, 0.);
+      a.aT[sy.off + p] = st3(dT, 0.);
+    }
+    __syncthreads();
+  }
+
+  // tangent of the first kick and the drift (uses the tangents of F, T at n)
+  __device__ __forceinline__ void tangent_kick_drift() {
+    const double hdt = 0.5 * a.dt;
+    for (int i = threadIdx.x; i < sy.n; i += BS) {
+      const double4 pp = P()[i];
+      const size_t g = sy.off + i;
+      const v3 tvh = add(ld3(a.av[g]), scl(ld3(a.aF[g]), hdt / pp.y));
+      const v3 twh = add(ld3(a.aw[g]), scl(ld3(a.aT[g]), hdt / pp.z));
+      a.ax[g] = st3(add(ld3(a.ax[g]), scl(tvh, a.dt)), 0.);
+      a.av[g] = st3(tvh, 0.);
+      a.aw[g] = st3(twh, 0.);
+    }
+    __syncthreads();
+  }
+
+  __device__ __forceinline__ void tangent_kick() {
+    const double hdt = 0.5 * a.dt;
+    for (int i = threadIdx.x; i < sy.n; i += BS) {
+      const double4 pp = P()[i];
+      const size_t g = sy.off + i;
+      a.av[g] = st3(add(ld3(a.av[g]), scl(ld3(a.aF[g]), hdt / pp.y)), 0.);
+      a.aw[g] = st3(add(ld3(a.aw[g]), scl(ld3(a.aT[g]), hdt / pp.z)), 0.);
+    }
+    __syncthreads();
+  }
+
   // One forward step n -> n+1; optionally stores the force inputs in window slot j.
   __device__ __forceinline__ void step(int n1, int j) {
+    const bool tan = a.mode == DIFF_TANGENT;
+    if (tan) tangent_kick_drift();
     kick_drift();
     bin_and_detect();
     if (j >= 0) {
@@ -471,7 +587,9 @@ struct DiffRunner {
         a.win_w[(size_t)j * a.ntot + sy.off + i] = W()[i];
       }
     }
+    if (tan) tangent_forces();  // reads in_count before forces() clears it
     forces();
+    if (tan) tangent_kick();
     kick();
     const int k = obs_index(n1);
     if (k >= 0 && j < 0) {
@@ -483,6 +601,10 @@ struct DiffRunner {
           const double4 tg = a.target[(size_t)k * a.ntot + sy.off + i];
           const double dx = x.x - tg.x, dy = x.y - tg.y, dz = x.z - tg.z;
           lsum += dx * dx + dy * dy + dz * dz;
+          if (tan) {
+            const double4 t = a.ax[sy.off + i];
+            dlsum += 2. * (dx * t.x + dy * t.y + dz * t.z);
+          }
         }
       }
     }
@@ -617,10 +739,23 @@ struct DiffRunner {
 
   __device__ __forceinline__ void run() {
     lsum = 0.;
+    dlsum = 0.;
     gth = mk(0., 0., 0.);
     const double lscale = 1. / ((double)a.nobs * 3. * (double)sy.n);
     initial_state();
     bin_and_detect();
+    if (a.mode == DIFF_TANGENT) {
+      // tangents of the initial state: positions fixed, velocities along the
+      // given direction (in gv0), angular velocities fixed
+      for (int i = threadIdx.x; i < sy.n; i += BS) {
+        const double4 z = make_double4(0., 0., 0., 0.);
+        a.ax[sy.off + i] = z;
+        a.av[sy.off + i] = a.gv0[sy.off + i];
+        a.aw[sy.off + i] = z;
+      }
+      __syncthreads();
+      tangent_forces();
+    }
     forces();
     if (a.mode == DIFF_GRAD) save(a.ck_x, a.ck_v, a.ck_w, a.ck_F, a.ck_T);
     bool ok = true;
@@ -635,6 +770,10 @@ struct DiffRunner {
     if (ok && failed()) ok = false;
     const double L = block_sum<BS>(lsum, red) * lscale;
     if (threadIdx.x == 0) a.loss[s] = ok ? L : __builtin_nan("");
+    if (a.mode == DIFF_TANGENT) {
+      const double dL = block_sum<BS>(dlsum, red) * lscale;
+      if (threadIdx.x == 0) a.dloss[s] = ok ? dL : __builtin_nan("");
+    }
     if (a.mode != DIFF_GRAD || !ok) return;
     for (int i = threadIdx.x; i < sy.n; i += BS) {
       const double4 z = make_double4(0., 0., 0., 0.);
@@ -694,7 +833,7 @@ __global__ void __launch_bounds__(BS) k_diff(DiffArgs a) {
     const int s = a.order[item];
     if (threadIdx.x == 0) ssy = a.sys[s];
     __syncthreads();
-    DiffRunner<BS> r{a, ssy, s, smem, red, &sd, 0., mk(0., 0., 0.)};
+    DiffRunner<BS> r{a, ssy, s, smem, red, &sd, 0., 0., mk(0., 0., 0.)};
     r.run();
     __syncthreads();
   }
@@ -979,7 +1118,8 @@ static int diff_errors(gdem_diff* d) {
   return GDEM_OK;
 }
 
-static int diff_launch(gdem_diff* d, int mode) {
+static int diff_launch(gdem_diff* d, int mode, const double* dtheta = nullptr,
+                       double* dloss = nullptr) {
   DHIP(hipSetDevice(d->dev));
   const size_t N = (size_t)d->ntot;
   const size_t nck = (size_t)((d->nsteps + d->K - 1) / d->K);
@@ -1023,6 +1163,8 @@ static int diff_launch(gdem_diff* d, int mode) {
   a.gv0 = g + 10 * N;
   a.loss = d->d_loss;
   a.gtheta = d->d_gtheta;
+  a.dtheta = dtheta;
+  a.dloss = dloss;
   a.err = d->d_err;
   int wg = d->workgroups;
   if (wg <= 0) {
@@ -1082,6 +1224,24 @@ extern "C" int gdem_diff_gradient(gdem_diff* d, double* loss, double* grad_theta
       grad_v0[3 * i + 2] = h[i].z;
     }
   }
+  return rc;
+}
+
+extern "C" int gdem_diff_directional(gdem_diff* d, const double* dtheta, const double* dv0,
+                                     double* loss, double* dloss) {
+  if (!d || !d->committed || !dtheta || !dv0 || !dloss) return GDEM_ERR_ARG;
+  DHIP(hipSetDevice(d->dev));
+  const size_t N = (size_t)d->ntot, ns = d->sys.size();
+  std::vector<double4> h(N);
+  for (size_t i = 0; i < N; ++i) h[i] = make_double4(dv0[3 * i], dv0[3 * i + 1], dv0[3 * i + 2], 0.);
+  DHIP(hipMemcpy(d->d_adj + 10 * N, h.data(), sizeof(double4) * N, hipMemcpyHostToDevice));
+  double* d_dir = nullptr;
+  DHIP(hipMalloc(&d_dir, sizeof(double) * ns * 4));
+  DHIP(hipMemcpy(d_dir, dtheta, sizeof(double) * ns * 3, hipMemcpyHostToDevice));
+  const int rc = diff_launch(d, DIFF_TANGENT, d_dir, d_dir + 3 * ns);
+  if (loss) DHIP(hipMemcpy(loss, d->d_loss, sizeof(double) * ns, hipMemcpyDeviceToHost));
+  DHIP(hipMemcpy(dloss, d_dir + 3 * ns, sizeof(double) * ns, hipMemcpyDeviceToHost));
+  DHIP(hipFree(d_dir));
   return rc;
 }
 

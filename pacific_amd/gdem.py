@@ -130,6 +130,7 @@ def load_library(path: str = LIB_PATH):
         "gdem_diff_record": (C.c_int, [vp, vp]),
         "gdem_diff_loss": (C.c_int, [vp, vp]),
         "gdem_diff_gradient": (C.c_int, [vp, vp, vp, vp]),
+        "gdem_diff_directional": (C.c_int, [vp, vp, vp, vp, vp]),
         "gdem_diff_last_error": (C.c_int, [vp, P(i32), P(i32), P(i32), P(i32)]),
         "gdem_diff_info": (C.c_int, [vp, P(i32), P(i64)]),
     }
@@ -154,7 +155,8 @@ EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall"
             "gdem_batch_set_kernel_timing", "gdem_batch_get_counters", "gdem_batch_info",
             "gdem_diff_create", "gdem_diff_destroy", "gdem_diff_add_system", "gdem_diff_commit",
             "gdem_diff_set_launch", "gdem_diff_set_parameters", "gdem_diff_set_targets",
-            "gdem_diff_record", "gdem_diff_loss", "gdem_diff_gradient", "gdem_diff_last_error",
+            "gdem_diff_record", "gdem_diff_loss", "gdem_diff_gradient", "gdem_diff_directional",
+            "gdem_diff_last_error",
             "gdem_diff_info"]
 
 
@@ -561,6 +563,16 @@ class GdemDiff:
         out = np.empty(self.nsys)
         self._check(self.lib.gdem_diff_loss(self.h, _ptr(out)), "gdem_diff_loss")
         return out
+
+    def directional(self, dtheta, dv0):
+        """Forward-mode derivative of each system's loss along (dtheta, dv0)."""
+        dt = _f64(dtheta, (self.nsys, 3))
+        dv = _f64(dv0, (self.ntot, 3))
+        loss = np.empty(self.nsys)
+        dl = np.empty(self.nsys)
+        self._check(self.lib.gdem_diff_directional(self.h, _ptr(dt), _ptr(dv), _ptr(loss), _ptr(dl)),
+                    "gdem_diff_directional")
+        return loss, dl
 
     def gradient(self):
         loss = np.empty(self.nsys)

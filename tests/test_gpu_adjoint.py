@@ -7,9 +7,10 @@ relative.  The loss is piecewise smooth: a contact starting or ending, or a
 contact switching between sticking and Coulomb sliding, puts a kink in it, and
 a difference that straddles one is not a derivative of either piece.  The
 adjoint is the derivative with the contact set and branches of the forward
-held (the design), so each difference is first checked against the one at
-h/2; where the two disagree (an event within +-h) h is divided by 10, at most
-twice, and the comparison uses the converged difference."""
+held (the design), so each difference is checked against the one at 2h; where
+they disagree (an event within the step) h is divided by 10.  Independently
+of differences, the reverse sweep is checked against a forward-mode (dual
+tangent) derivative along random directions, to rounding."""
 import numpy as np
 import pytest
 
@@ -18,7 +19,10 @@ from pacific_amd import workloads
 
 pytestmark = pytest.mark.gpu
 
-NSTEPS, K, OBS, DT = 120, 25, (40, 80, 120), 1.0e-5
+# A short horizon: over longer ones the colliding packing amplifies rounding
+# and the loss becomes too rough for central differences to resolve 1e-5
+# (the adjoint itself is exact there too).
+NSTEPS, K, OBS, DT = 48, 10, (16, 32, 48), 1.0e-5
 GUESS = np.array([1.2, 0.9, 1.1])
 
 
@@ -55,16 +59,51 @@ def test_forward_follows_oracle(oracle_mod):
     eng.close()
 
 
+@pytest.mark.parametrize("nsteps", [48, 200])
+def test_adjoint_equals_forward_mode_directional_derivatives(nsteps):
+    """<gradient, direction> from the reverse sweep equals the forward-mode
+    (dual tangent) derivative along random directions: both are exact
+    derivatives of the same discrete map, so they agree to rounding, also over
+    horizons where central differences no longer resolve the derivative."""
+    print(f"[case] adjoint vs forward mode, {nsteps} steps", flush=True)
+    from pacific_amd import gdem
+    systems, hidden = workloads.calibration_systems(3, n=300, seed=21)
+    obs = tuple(int(v) for v in np.linspace(nsteps / 3, nsteps, 3).round())
+    eng = gdem.GdemDiff(systems, nsteps, DT, 10, obs)
+    v0 = np.concatenate([w["particles"]["v"] for w in systems])
+    eng.set_parameters(hidden, v0)
+    eng.set_targets(eng.record())
+    theta = hidden * GUESS[None, :]
+    eng.set_parameters(theta, v0)
+    loss, gth, gv0 = eng.gradient()
+    rng = np.random.default_rng(3)
+    off = eng.offsets
+    for trial in range(3):
+        dth = rng.normal(size=theta.shape) * theta
+        dv = rng.normal(size=v0.shape) * (0.0 if trial == 0 else 1.0)
+        if trial == 2:
+            dth[:] = 0.0
+        loss2, dl = eng.directional(dth, dv)
+        np.testing.assert_array_equal(loss2, loss)
+        for s in range(len(systems)):
+            want = float((gth[s] * dth[s]).sum() + (gv0[off[s]:off[s + 1]] * dv[off[s]:off[s + 1]]).sum())
+            scale = float(np.abs(gth[s] * dth[s]).sum() + np.abs(gv0[off[s]:off[s + 1]] * dv[off[s]:off[s + 1]]).sum())
+            assert abs(dl[s] - want) <= 1e-10 * scale, (trial, s, dl[s], want)
+    eng.close()
+
+
 def converged_difference(f, h0, tries=3):
-    """Central difference of f at 0 with step h0, h0/10, ...: the first whose
-    value agrees with the difference at half the step (relative 1e-6), i.e.
-    no kink of f within +-h; returns (difference, step)."""
+    """Central difference of f at 0 with step h0.  If the difference at 2 h0
+    differs by more than 1e-3 relative, a kink of f (a contact event) lies
+    within the step and h0 is divided by 10 (at most twice).  Smaller steps
+    are not tried otherwise: the loss of a colliding packing carries rounding
+    noise that a smaller step amplifies.  Returns (difference, step)."""
     h = h0
     d = None
     for _ in range(tries):
         d = (f(h) - f(-h)) / (2 * h)
-        d2 = (f(h / 2) - f(-h / 2)) / h
-        if abs(d - d2) <= 1e-6 * max(abs(d), abs(d2)):
+        d2 = (f(2 * h) - f(-2 * h)) / (4 * h)
+        if abs(d - d2) <= 1e-3 * max(abs(d), abs(d2)):
             return d, h
         h /= 10
     return d, h * 10
