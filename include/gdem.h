@@ -180,6 +180,20 @@ int gdem_step(gdem_ctx* ctx, int32_t nsteps, double dt);
    host code, Grains.cpp:286-452). */
 int gdem_set_graph(gdem_ctx* ctx, int32_t on);
 
+/* Skin of the sphere candidate lists (default: automatic, 0.2 r_max; 0:
+   lists rebuilt at every force evaluation).  Binning and the half-stencil
+   sweep run only when a particle has moved skin/2 since the last rebuild;
+   every evaluation still tests each candidate for contact exactly, so the
+   pair set of each step is the reference's (LinkedCell::ComputeForces,
+   Base/src/LinkedCell.cpp:1019-1103).  The skin is limited so that
+   2 r_max + skin fits in a cell edge, and the lists are off when that
+   leaves less than 0.05 r_max.  Negative: automatic. */
+int gdem_set_skin(gdem_ctx* ctx, double skin);
+/* Effective skin (0: lists off) and the rebuilds of the candidate lists since
+   the particles or the skin were last set.  No counterpart in the reference,
+   whose lists are rebuilt at every step. */
+int gdem_get_skin(gdem_ctx* ctx, double* skin, int64_t* rebuilds);
+
 /* Waits for the stream; returns the first deferred device error. */
 int gdem_synchronize(gdem_ctx* ctx);
 

@@ -199,12 +199,15 @@ struct BatchRunner {
       d.row_rank = a.row_rank + sy.row_off;
       d.grid = sy.grid;
       d.n = sy.n;
+      d.lstride = sy.n;
       d.out_list = a.out_list + (size_t)sy.off * kListCap;
       d.out_count = a.out_count + sy.off;
       d.in_list = a.in_list + (size_t)sy.off * kListCap;
       d.in_count = a.in_count + sy.off;
       d.err = err();
       d.nblk8 = 0;
+      d.skin = 0.;
+      d.gate = nullptr;
     }
     __syncthreads();
     for (int t = threadIdx.x; t < sy.n; t += BS) detect_particle(*sd, t);
@@ -228,6 +231,7 @@ struct BatchRunner {
       f.gravity[0] = sy.gravity[0]; f.gravity[1] = sy.gravity[1]; f.gravity[2] = sy.gravity[2];
       f.dt_vel = dtv;
       f.dt_disp = dtd;
+      f.lstride = sy.n;
       f.out_list = a.out_list + (size_t)sy.off * kListCap;
       f.out_count = a.out_count + sy.off;
       f.in_list = a.in_list + (size_t)sy.off * kListCap;
@@ -241,6 +245,10 @@ struct BatchRunner {
       f.rec_count = nullptr;
       f.rec_cap = 0;
       f.nblk8 = 0;
+      f.skin_did = nullptr;
+      f.skin_need = nullptr;
+      f.xref = nullptr;
+      f.half_skin2 = 0.;
     }
     __syncthreads();
     for (int t = threadIdx.x; t < sy.n; t += BS) ncont += (unsigned long long)force_particle<MODE, false>(*sf, t);

@@ -78,6 +78,11 @@ struct gdem_ctx {
   double4 h_wall_sc[kMaxWalls]{};
   CShape* d_wall_shapes = nullptr;
   double4* d_wall_sc = nullptr;
+  // skin lists (spheres): requested skin (< 0: automatic), effective skin,
+  // device words and reference positions
+  double skin_req = -1., skin = 0., rmax = 0.;
+  int32_t* d_skin = nullptr;
+  double4* d_xref = nullptr;
   // captured fused step (one graph per state-buffer parity)
   bool use_graph = true;
   int64_t graph_gen = 0;
@@ -157,7 +162,7 @@ extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
       hipMalloc(&c->d_row_jk, sizeof(int32_t) * nrows) != hipSuccess ||
       hipMalloc(&c->bins.cnt, sizeof(int32_t) * ncell) != hipSuccess ||
       hipMalloc(&c->bins.local, sizeof(int32_t) * ncell) != hipSuccess ||
-      hipMalloc(&c->bins.bsum, sizeof(int32_t) * nb) != hipSuccess ||
+      hipMalloc(&c->bins.bsum, sizeof(int32_t) * (nb + 1)) != hipSuccess ||
       hipMalloc(&c->bins.cell_start, sizeof(int32_t) * (ncell + 1)) != hipSuccess ||
       hipMalloc(&c->d_models, sizeof(c->h_models)) != hipSuccess ||
       hipMalloc(&c->d_walls, sizeof(c->h_walls)) != hipSuccess ||
@@ -172,6 +177,8 @@ extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
   (void)hipMemcpy(c->d_row_rank, rank.data(), sizeof(int32_t) * nrows, hipMemcpyHostToDevice);
   (void)hipMemcpy(c->d_row_jk, row_jk.data(), sizeof(int32_t) * nrows, hipMemcpyHostToDevice);
   (void)hipMemset(c->bins.cnt, 0, sizeof(int32_t) * ncell);
+  c->bins.ticket = c->bins.bsum + nb;
+  (void)hipMemset(c->bins.bsum, 0, sizeof(int32_t) * (nb + 1));
   (void)hipMemset(c->d_err, 0xff, sizeof(unsigned long long) * kErrWords);
   (void)hipMemset(c->d_counters, 0, sizeof(unsigned long long) * 2);
   (void)hipMemcpy(c->d_models, c->h_models, sizeof(c->h_models), hipMemcpyHostToDevice);
@@ -195,7 +202,8 @@ extern "C" int gdem_destroy(gdem_ctx* c) {
   free_cstate(c->cbuf[0]);
   free_cstate(c->cbuf[1]);
   (void)hipFree(c->d_cand); (void)hipFree(c->d_crec); (void)hipFree(c->d_mask); (void)hipFree(c->d_work);
-  (void)hipFree(c->d_nwork); (void)hipFree(c->d_shapes); (void)hipFree(c->d_wall_shapes); (void)hipFree(c->d_wall_sc);
+  (void)hipFree(c->d_nwork); (void)hipFree(c->d_shapes);
+  (void)hipFree(c->d_skin); (void)hipFree(c->d_xref); (void)hipFree(c->d_wall_shapes); (void)hipFree(c->d_wall_sc);
   for (auto& e : c->ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
   for (auto& g : c->graph_exec)
     if (g) (void)hipGraphExecDestroy(g);
@@ -219,6 +227,10 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
   (void)hipFree(c->d_rec); (void)hipFree(c->d_rec_count);
   (void)hipFree(c->d_cand); (void)hipFree(c->d_crec); (void)hipFree(c->d_mask); (void)hipFree(c->d_work);
   (void)hipFree(c->d_nwork);
+  (void)hipFree(c->d_skin); (void)hipFree(c->d_xref);
+  c->d_skin = nullptr;
+  c->d_xref = nullptr;
+  c->skin = 0.;
   c->d_F = c->d_T = nullptr;
   c->bins.key = c->bins.slot = c->bins.perm = nullptr;
   c->bins.pos = nullptr;
@@ -233,6 +245,7 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
   c->d_nwork = nullptr;
   c->n = n;
   c->convex = convex;
+  c->rmax = 0.;
   c->graph_gen++;
   const size_t nn = (size_t)std::max<int64_t>(n, 1);
   if (alloc_state(c->buf[0], n) || alloc_state(c->buf[1], n)) return GDEM_ERR_HIP;
@@ -246,6 +259,10 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
     HIP_TRY(hipMemset(c->d_nwork, 0, sizeof(int32_t)));
   } else {
     HIP_TRY(hipMalloc(&c->d_out_list, sizeof(int2) * nn * kListCap));
+    HIP_TRY(hipMalloc(&c->d_skin, sizeof(int32_t) * kSkinWords));
+    HIP_TRY(hipMalloc(&c->d_xref, sizeof(double4) * nn));
+    const int32_t words[kSkinWords] = {1, 1, 0};  // the first evaluation rebuilds
+    HIP_TRY(hipMemcpy(c->d_skin, words, sizeof(words), hipMemcpyHostToDevice));
   }
   const size_t in_cap = convex ? (size_t)kCandCap : (size_t)kListCap;
   HIP_TRY(hipMalloc(&c->d_F, sizeof(double4) * nn));
@@ -270,6 +287,51 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
   return GDEM_OK;
 }
 
+// Effective skin of the candidate lists: the requested one (automatic: 0.2
+// r_max), limited so that r_i + r_j + skin stays within one cell edge (the
+// half stencil then still finds every candidate); below 0.05 r_max the lists
+// are off (rebuilt at every evaluation).
+static void update_skin(gdem_ctx* c) {
+  if (c->convex || c->d_skin == nullptr) {
+    c->skin = 0.;
+    return;
+  }
+  double edge = c->grid.edge[0];
+  for (int d = 1; d < 3; ++d) edge = std::min(edge, c->grid.edge[d]);
+  const double room = edge - 2. * c->rmax;
+  const double want = c->skin_req < 0. ? 0.2 * c->rmax : c->skin_req;
+  const double eff = std::min(want, room);
+  // a skin much thinner than the particles would rebuild at almost every step
+  c->skin = (want > 0. && eff >= 0.05 * c->rmax) ? eff : 0.;
+  c->graph_gen++;
+}
+
+extern "C" int gdem_set_skin(gdem_ctx* c, double skin) {
+  if (!c || !(skin == skin)) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->skin_req = skin;
+  update_skin(c);
+  if (c->d_skin) {
+    const int32_t words[kSkinWords] = {1, 1, 0};  // rebuild at the next evaluation
+    HIP_TRY(hipMemcpy(c->d_skin, words, sizeof(words), hipMemcpyHostToDevice));
+  }
+  return GDEM_OK;
+}
+
+extern "C" int gdem_get_skin(gdem_ctx* c, double* skin, int64_t* rebuilds) {
+  if (!c) return GDEM_ERR_ARG;
+  int32_t words[kSkinWords] = {0, 0, 0};
+  if (c->d_skin) {
+    HIP_TRY(hipSetDevice(c->dev));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(words, c->d_skin, sizeof(words), hipMemcpyDeviceToHost));
+  }
+  if (skin) *skin = c->skin;
+  if (rebuilds) *rebuilds = words[kSkinRebuilds];
+  return GDEM_OK;
+}
+
 extern "C" int gdem_set_particles(gdem_ctx* c, int64_t n, const double* x, const double* v,
                                   const double* w, const double* q, const double* radius,
                                   const double* density, const double* crust,
@@ -288,6 +350,7 @@ extern "C" int gdem_set_particles(gdem_ctx* c, int64_t n, const double* x, const
     if (mat < 0 || mat >= kMaxMaterials) return GDEM_ERR_ARG;
     sphere_particle(x + 3 * i, v + 3 * i, w + 3 * i, q + 4 * i, rr, rho, crust[i], mat,
                     (int32_t)i, pr[i], vm[i], wr[i], wb[i], qq[i], ax[i]);
+    c->rmax = std::max(c->rmax, rr);
   }
   State& s = c->buf[0];
   c->cur = 0;
@@ -300,6 +363,7 @@ extern "C" int gdem_set_particles(gdem_ctx* c, int64_t n, const double* x, const
     HIP_TRY(hipMemcpy(s.quat, qq.data(), b4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s.aux, ax.data(), sizeof(Aux) * (size_t)n, hipMemcpyHostToDevice));
   }
+  update_skin(c);
   launch_keys(s, (int32_t)n, c->grid, c->d_row_rank, c->bins.key, c->d_err, c->stream);
   HIP_TRY(hipGetLastError());
   return GDEM_OK;
@@ -559,6 +623,9 @@ extern "C" int gdem_set_contact_model(gdem_ctx* c, int32_t a, int32_t b, int32_t
 }
 
 static int do_binning(gdem_ctx* c) {
+  // with skin lists the binning is part of the next force evaluation, which
+  // runs it when a rebuild is due
+  if (c->skin > 0.) return GDEM_OK;
   launch_binning(c->bins, c->buf[c->cur], (int32_t)c->n, c->grid.ncell, c->stream);
   HIP_TRY(hipGetLastError());
   c->binned = true;
@@ -651,7 +718,17 @@ static void launch_convex_step(gdem_ctx* c, int mode, double dtv, double dtd, bo
 // positions changed); no host bookkeeping, so the sequence can be captured.
 static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft,
                           std::pair<hipEvent_t, hipEvent_t>** evp) {
-  if (!c->binned) {
+  const bool skin = !c->convex && c->skin > 0.;
+  // need word read by this evaluation and the one of the next evaluation
+  int32_t* need = skin ? c->d_skin + kSkinNeed0 + c->cur : nullptr;
+  int32_t* need_next = skin ? c->d_skin + kSkinNeed0 + (1 - c->cur) : nullptr;
+  const int32_t* gate = need;
+  if (skin) {
+    // rebuild (binning + candidate sweep) only when a particle moved skin/2
+    launch_binning(c->bins, c->buf[c->cur], (int32_t)c->n, c->grid.ncell, c->stream, gate,
+                   c->d_in_count, need_next, c->d_skin + kSkinRebuilds);
+    HIP_TRY(hipGetLastError());
+  } else if (!c->binned) {
     launch_binning(c->bins, c->buf[c->cur], (int32_t)c->n, c->grid.ncell, c->stream);
     HIP_TRY(hipGetLastError());
   }
@@ -676,12 +753,15 @@ static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wr
     d.row_rank = c->d_row_rank;
     d.grid = c->grid;
     d.n = (int32_t)c->n;
+    d.lstride = (int32_t)c->n;
     d.out_list = c->d_out_list;
     d.out_count = c->d_out_count;
     d.in_list = c->d_in_list;
     d.in_count = c->d_in_count;
     d.err = c->d_err;
     d.nblk8 = nblk8;
+    d.skin = skin ? c->skin : 0.;
+    d.gate = gate;
     ForceArgs a{};
     a.in = c->buf[c->cur];
     a.out = c->buf[1 - c->cur];
@@ -696,6 +776,7 @@ static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wr
     for (int dd = 0; dd < 3; ++dd) a.gravity[dd] = c->cfg.gravity[dd];
     a.dt_vel = dtv;
     a.dt_disp = dtd;
+    a.lstride = (int32_t)c->n;
     a.out_list = c->d_out_list;
     a.out_count = c->d_out_count;
     a.in_list = c->d_in_list;
@@ -709,6 +790,12 @@ static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wr
     a.rec_count = c->d_rec_count;
     a.rec_cap = c->rec_cap;
     a.nblk8 = nblk8;
+    if (skin) {
+      a.skin_did = need;
+      a.skin_need = need_next;
+      a.xref = c->d_xref;
+      a.half_skin2 = 0.25 * c->skin * c->skin;
+    }
     launch_detect(d, c->stream);
     if (c->timing) {
       ev = next_event_pair(c);
@@ -790,6 +877,10 @@ extern "C" int gdem_move(gdem_ctx* c, double dtv, double dtd) {
   if (c->convex)
     launch_cmove(c->buf[c->cur], c->cbuf[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid,
                  c->d_row_rank, dtv, dtd, c->bins.key, c->d_err, c->stream);
+  else if (c->skin > 0.)
+    launch_move(c->buf[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid, c->d_row_rank, dtv, dtd,
+                c->bins.key, c->d_err, c->stream, c->d_skin + kSkinNeed0 + c->cur, c->d_xref,
+                0.25 * c->skin * c->skin);
   else
     launch_move(c->buf[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid, c->d_row_rank, dtv, dtd,
                 c->bins.key, c->d_err, c->stream);

@@ -313,12 +313,15 @@ struct DiffRunner {
       d.row_rank = rr;
       d.grid = sy.grid;
       d.n = n;
+      d.lstride = n;
       d.out_list = a.out_list + (size_t)sy.off * kListCap;
       d.out_count = a.out_count + sy.off;
       d.in_list = a.in_list + (size_t)sy.off * kListCap;
       d.in_count = a.in_count + sy.off;
       d.err = err();
       d.nblk8 = 0;
+      d.skin = 0.;
+      d.gate = nullptr;
     }
     __syncthreads();
     for (int t = threadIdx.x; t < n; t += BS) detect_particle(*sd, t);
@@ -388,17 +391,17 @@ struct DiffRunner {
         }
       };
       const int nout = min(a.out_count[sy.off + t], kListCap);
-      const int2* ol = a.out_list + (size_t)(sy.off + t) * kListCap;
-      for (int e = 0; e < nout; ++e) one(ol[e].x, ol[e].y);
+      const int2* ol = a.out_list + (size_t)sy.off * kListCap + t;
+      for (int e = 0; e < nout; ++e) one(ol[(size_t)e * sy.n].x, ol[(size_t)e * sy.n].y);
       const int nin = min(a.in_count[sy.off + t], kListCap);
       a.in_count[sy.off + t] = 0;
-      const int2* il = a.in_list + (size_t)(sy.off + t) * kListCap;
+      const int2* il = a.in_list + (size_t)sy.off * kListCap + t;
       long long last = -1;
       for (int e = 0; e < nin; ++e) {
         long long k;
-        const int f = next_incoming(il, nin, last, k);
+        const int f = next_incoming(il, nin, sy.n, last, k);
         if (f < 0) break;
-        one(il[f].x, il[f].y);
+        one(il[(size_t)f * sy.n].x, il[(size_t)f * sy.n].y);
         last = k;
       }
       const v3 x = ld3(X()[p]), vv = ld3(V()[p]), ww = ld3(W()[p]);
@@ -518,11 +521,11 @@ struct DiffRunner {
         }
       };
       const int nout = min(a.out_count[sy.off + t], kListCap);
-      const int2* ol = a.out_list + (size_t)(sy.off + t) * kListCap;
-      for (int e = 0; e < nout; ++e) one(ol[e].x, ol[e].y);
+      const int2* ol = a.out_list + (size_t)sy.off * kListCap + t;
+      for (int e = 0; e < nout; ++e) one(ol[(size_t)e * sy.n].x, ol[(size_t)e * sy.n].y);
       const int nin = min(a.in_count[sy.off + t], kListCap);
-      const int2* il = a.in_list + (size_t)(sy.off + t) * kListCap;
-      for (int e = 0; e < nin; ++e) one(il[e].x, il[e].y);
+      const int2* il = a.in_list + (size_t)sy.off * kListCap + t;
+      for (int e = 0; e < nin; ++e) one(il[(size_t)e * sy.n].x, il[(size_t)e * sy.n].y);
       const double4 pp = P()[p];
       const v3 x = ld3(X()[p]), vv = ld3(V()[p]), ww = ld3(W()[p]);
       const v3 dx = ld3(tx[p]), dvv = ld3(tv[p]), dww = ld3(tw[p]);
@@ -649,17 +652,17 @@ struct DiffRunner {
         }
       };
       const int nout = min(a.out_count[sy.off + t], kListCap);
-      const int2* ol = a.out_list + (size_t)(sy.off + t) * kListCap;
-      for (int e = 0; e < nout; ++e) one(ol[e].x, ol[e].y, true);
+      const int2* ol = a.out_list + (size_t)sy.off * kListCap + t;
+      for (int e = 0; e < nout; ++e) one(ol[(size_t)e * sy.n].x, ol[(size_t)e * sy.n].y, true);
       const int nin = min(a.in_count[sy.off + t], kListCap);
       a.in_count[sy.off + t] = 0;
-      const int2* il = a.in_list + (size_t)(sy.off + t) * kListCap;
+      const int2* il = a.in_list + (size_t)sy.off * kListCap + t;
       long long last = -1;
       for (int e = 0; e < nin; ++e) {
         long long k;
-        const int f = next_incoming(il, nin, last, k);
+        const int f = next_incoming(il, nin, sy.n, last, k);
         if (f < 0) break;
-        one(il[f].x, il[f].y, false);
+        one(il[(size_t)f * sy.n].x, il[(size_t)f * sy.n].y, false);
         last = k;
       }
       const double4 pp = P()[p];

@@ -45,6 +45,7 @@ struct ContactRec {
 // Maximum contacts one particle can detect (outgoing) or receive (incoming)
 // per step; exceeding it raises GDEM_ERR_CAPACITY.
 constexpr int kListCap = 24;
+static_assert(kListCap < 32, "list entries are tracked in 32-bit masks");
 
 // Device error words, min-reduced with integer atomics so the reported
 // offender does not depend on scheduling.
@@ -64,13 +65,27 @@ struct DetectArgs {
   const int32_t* row_rank;  // (j * nk + k) -> rank
   Grid grid;
   int32_t n;
-  int2* out_list;         // [n][kListCap]
+  // lists [kListCap][lstride]: entry k of slot t at k * lstride + t, so the
+  // k-th entries of neighbouring slots are adjacent
+  int32_t lstride;
+  int2* out_list;
   int32_t* out_count;     // [n]
-  int2* in_list;          // [n][kListCap]
+  int2* in_list;
   int32_t* in_count;      // [n], zero on entry
   unsigned long long* err;
   int32_t nblk8;  // blocks padded to a multiple of 8 (XCD-aware mapping)
+  // skin lists: skin > 0 lists candidates (gap < skin) instead of contacts;
+  // gate (nullable) makes the sweep run only when *gate != 0
+  double skin;
+  const int32_t* gate;
 };
+
+// Device words of the skin lists.  need[p] != 0: the force evaluation that
+// reads state copy p rebuilds the lists first (some particle moved more than
+// skin/2 since the last rebuild).  That evaluation reads need[p], clears
+// need[1 - p] in its first kernel and its integrator tail sets need[1 - p]
+// for the next one, so no kernel of an evaluation writes a word it reads.
+enum { kSkinNeed0 = 0, kSkinNeed1 = 1, kSkinRebuilds = 2, kSkinWords = 3 };
 
 struct ForceArgs {
   State in, out;
@@ -84,6 +99,7 @@ struct ForceArgs {
   int32_t n;
   double gravity[3];
   double dt_vel, dt_disp;
+  int32_t lstride;  // list layout as in DetectArgs
   const int2* out_list;
   const int32_t* out_count;
   const int2* in_list;
@@ -97,6 +113,12 @@ struct ForceArgs {
   int32_t* rec_count;
   int32_t rec_cap;
   int32_t nblk8;  // blocks padded to a multiple of 8 (XCD-aware mapping)
+  // skin lists (skin_did == nullptr: lists hold this step's contacts, state
+  // is reordered through perm every evaluation)
+  const int32_t* skin_did;  // need word of this evaluation: lists rebuilt
+  int32_t* skin_need;       // need word of the next evaluation
+  double4* xref;        // positions at the last rebuild, slot order
+  double half_skin2;    // (skin / 2)^2
 };
 
 enum { MODE_FORCE = 0, MODE_KICK = 1, MODE_MOVE = 2 };
@@ -212,18 +234,25 @@ struct Bins {
   int32_t* perm;        // sorted slot -> storage index
   int32_t* cell_start;  // ncell + 1
   float4* pos;          // sorted single-precision copy of pos_r
+  int32_t* ticket;      // bsum[nbsum]: blocks of the scan pass finished
   int32_t nbsum;
 };
 
 // Launchers (all asynchronous on the given stream).
 void launch_keys(const State& s, int32_t n, const Grid& g, const int32_t* row_rank, int32_t* key,
                  unsigned long long* err, hipStream_t st);
-void launch_binning(Bins& b, const State& s, int32_t n, int32_t ncell, hipStream_t st);
+// With skin lists: gate = need word of this evaluation (binning runs only
+// when it is set), in_count is cleared for the sweep, clear_need = need word of
+// the next evaluation (cleared), rebuilds counts the rebuilds.
+void launch_binning(Bins& b, const State& s, int32_t n, int32_t ncell, hipStream_t st,
+                    const int32_t* gate = nullptr, int32_t* in_count = nullptr,
+                    int32_t* clear_need = nullptr, int32_t* rebuilds = nullptr);
 void launch_detect(const DetectArgs& a, hipStream_t st);
 void launch_force(int mode, bool record, const ForceArgs& a, hipStream_t st);
 void launch_move(const State& s, const double4* F, const double4* T, int32_t n, const Grid& g,
                  const int32_t* row_rank, double dt_vel, double dt_disp, int32_t* key,
-                 unsigned long long* err, hipStream_t st);
+                 unsigned long long* err, hipStream_t st, int32_t* skin = nullptr,
+                 const double4* xref = nullptr, double half_skin2 = 0.);
 void launch_advance(const State& s, const double4* F, const double4* T, int32_t n,
                     double dt_vel, hipStream_t st);
 void launch_unpermute(const State& s, const double4* F, const double4* T, int32_t n,

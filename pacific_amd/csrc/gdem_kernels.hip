@@ -47,7 +47,14 @@ __global__ void __launch_bounds__(kBlock) k_keys(State s, int n, Grid g, const i
 // storage order is the previous step's cell order), so equal keys of
 // neighbouring lanes are merged into one atomic per run.
 __global__ void __launch_bounds__(kBlock) k_count(const int32_t* key, int32_t* slot, int32_t* cnt,
-                                                  int n) {
+                                                  int n, const int32_t* gate, int32_t* in_count,
+                                                  int32_t* clear_need, int32_t* rebuilds) {
+  // skin lists: this evaluation's first kernel clears the next one's need word
+  if (clear_need && blockIdx.x == 0 && threadIdx.x == 0) {
+    *clear_need = 0;
+    if (*gate != 0) *rebuilds += 1;
+  }
+  if (gate && *gate == 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const int k = i < n ? key[i] : -1;
@@ -62,6 +69,8 @@ __global__ void __launch_bounds__(kBlock) k_count(const int32_t* key, int32_t* s
   if (head && k >= 0) base = atomicAdd(&cnt[k], next_head - lane);
   base = __shfl(base, head_lane, 64);
   if (i < n) slot[i] = base + (lane - head_lane);
+  // skin lists: the rebuild's sweep appends to cleared incoming counts
+  if (in_count && i < n) in_count[i] = 0;
 }
 
 template <int BS>
@@ -90,9 +99,16 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* smem, int& total
   return prefix + x - v;
 }
 
-__global__ void __launch_bounds__(kScanBlock) k_scan_local(int32_t* cnt, int32_t* local,
-                                                           int32_t* bsum, int ncell) {
+// Exclusive scan of the cell counts in one launch: each block scans its
+// kScanBlock cells and publishes its total; the last block to finish (ticket)
+// scans the totals in place (nb <= 8 * kScanBlock) and resets the ticket.
+// No block waits for another.
+__global__ void __launch_bounds__(kScanBlock) k_scan(int32_t* cnt, int32_t* local, int32_t* bsum,
+                                                     int32_t* ticket, int ncell, int nb,
+                                                     const int32_t* gate) {
+  if (gate && *gate == 0) return;
   __shared__ int smem[kScanBlock / 64];
+  __shared__ int last;
   int c = blockIdx.x * kScanBlock + threadIdx.x;
   int v = c < ncell ? cnt[c] : 0;
   int total;
@@ -101,12 +117,14 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_local(int32_t* cnt, int32_t
     local[c] = ex;
     cnt[c] = 0;  // ready for the next count pass (no memset node in graphs)
   }
-  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
-}
-
-// Exclusive scan of the block sums by one workgroup (nb <= 8 * 1024).
-__global__ void __launch_bounds__(kScanBlock) k_scan_bsum(int32_t* bsum, int nb) {
-  __shared__ int smem[kScanBlock / 64];
+  if (threadIdx.x == 0) {
+    bsum[blockIdx.x] = total;
+    __threadfence();  // the total is visible device-wide before the ticket
+    last = atomicAdd(ticket, 1) == nb - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();  // every block's total is visible to this block
   const int per = (nb + kScanBlock - 1) / kScanBlock;
   const int beg = threadIdx.x * per;
   int loc[8];
@@ -116,18 +134,19 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_bsum(int32_t* bsum, int nb)
     loc[k] = idx < nb ? bsum[idx] : 0;
     s += loc[k];
   }
-  int total;
-  int ex = block_exclusive_scan<kScanBlock>(s, smem, total);
+  int bex = block_exclusive_scan<kScanBlock>(s, smem, total);
   for (int k = 0; k < per; ++k) {
     int idx = beg + k;
-    if (idx < nb) bsum[idx] = ex;
-    ex += loc[k];
+    if (idx < nb) bsum[idx] = bex;
+    bex += loc[k];
   }
+  if (threadIdx.x == 0) *ticket = 0;
 }
 
 __global__ void __launch_bounds__(kBlock) k_scatter(const int32_t* key, const int32_t* slot,
                                                     const int32_t* local, const int32_t* bsum,
-                                                    int32_t* perm, int n) {
+                                                    int32_t* perm, int n, const int32_t* gate) {
+  if (gate && *gate == 0) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int k = key[i];
@@ -135,10 +154,15 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const int32_t* key, const in
 }
 
 // Orders each cell by previous slot, which makes the permutation independent of
-// the order in which the count pass's atomics were served.
+// the order in which the count pass's atomics were served, then writes the
+// cell's entries of the sorted single-precision copy of position + radius
+// (read by the detection sweep's conservative pre-filter only: every contact
+// decision is made in FP64 on the full-precision state).
 __global__ void __launch_bounds__(kBlock) k_cellsort(const int32_t* local, const int32_t* bsum,
                                                      int32_t* perm, int32_t* cell_start,
-                                                     int ncell, int n) {
+                                                     const double4* pos_r, float4* pos_sorted,
+                                                     int ncell, int n, const int32_t* gate) {
+  if (gate && *gate == 0) return;
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   int beg = local[c] + bsum[c / kScanBlock];
@@ -154,21 +178,15 @@ __global__ void __launch_bounds__(kBlock) k_cellsort(const int32_t* local, const
   }
   cell_start[c] = beg;
   if (c == ncell - 1) cell_start[ncell] = n;
-}
-
-// Sorted single-precision copy of position + radius, read by the detection
-// sweep's conservative pre-filter only (every contact decision is made in
-// FP64 on the full-precision state).
-__global__ void __launch_bounds__(kBlock) k_gather_pos(const int32_t* perm, const double4* pos_r,
-                                                       float4* pos_sorted, int n) {
-  int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= n) return;
-  const double4 p = pos_r[perm[u]];
-  pos_sorted[u] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+  for (int u = beg; u < end; ++u) {
+    const double4 p = pos_r[perm[u]];
+    pos_sorted[u] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+  }
 }
 
 // ---------------------------------------------------------------- detect
 __global__ void __launch_bounds__(kBlock) k_detect(DetectArgs a) {
+  if (a.gate && *a.gate == 0) return;
   const int t = logical_index(a.nblk8);
   if (t >= a.n) return;
   detect_particle(a, t);
@@ -197,10 +215,11 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a) {
 __global__ void __launch_bounds__(kBlock) k_move(State s, const double4* Fa, const double4* Ta,
                                                  int n, Grid g, const int32_t* row_rank,
                                                  double dtv, double dtd, int32_t* key,
-                                                 unsigned long long* err) {
+                                                 unsigned long long* err, int32_t* skin,
+                                                 const double4* xref, double half_skin2) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  move_particle(s, Fa, Ta, i, g, row_rank, dtv, dtd, key, err);
+  move_particle(s, Fa, Ta, i, g, row_rank, dtv, dtd, key, err, skin, xref, half_skin2);
 }
 
 __global__ void __launch_bounds__(kBlock) k_advance(State s, const double4* Fa, const double4* Ta,
@@ -251,20 +270,20 @@ void launch_keys(const State& s, int32_t n, const Grid& g, const int32_t* row_ra
                      err);
 }
 
-void launch_binning(Bins& b, const State& s, int32_t n, int32_t ncell, hipStream_t st) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_count, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, b.key, b.slot, b.cnt, n);
+void launch_binning(Bins& b, const State& s, int32_t n, int32_t ncell, hipStream_t st,
+                    const int32_t* gate, int32_t* in_count, int32_t* clear_need,
+                    int32_t* rebuilds) {
+  // one block at least: with skin lists the count pass also clears a word
+  hipLaunchKernelGGL(k_count, dim3(nblocks(n > 0 ? n : 1, kBlock)), dim3(kBlock), 0, st, b.key,
+                     b.slot, b.cnt, n, gate, in_count, clear_need, rebuilds);
   const int nb = nblocks(ncell, kScanBlock);
-  hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kScanBlock), 0, st, b.cnt, b.local, b.bsum, ncell);
-  hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(kScanBlock), 0, st, b.bsum, nb);
+  hipLaunchKernelGGL(k_scan, dim3(nb), dim3(kScanBlock), 0, st, b.cnt, b.local, b.bsum, b.ticket,
+                     ncell, nb, gate);
   if (n > 0)
     hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, b.key, b.slot,
-                       b.local, b.bsum, b.perm, n);
+                       b.local, b.bsum, b.perm, n, gate);
   hipLaunchKernelGGL(k_cellsort, dim3(nblocks(ncell, kBlock)), dim3(kBlock), 0, st, b.local,
-                     b.bsum, b.perm, b.cell_start, ncell, n);
-  if (n > 0)
-    hipLaunchKernelGGL(k_gather_pos, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, b.perm,
-                       (const double4*)s.pos_r, b.pos, n);
+                     b.bsum, b.perm, b.cell_start, (const double4*)s.pos_r, b.pos, ncell, n, gate);
 }
 
 void launch_detect(const DetectArgs& a, hipStream_t st) {
@@ -290,10 +309,11 @@ void launch_force(int mode, bool record, const ForceArgs& a, hipStream_t st) {
 
 void launch_move(const State& s, const double4* F, const double4* T, int32_t n, const Grid& g,
                  const int32_t* row_rank, double dt_vel, double dt_disp, int32_t* key,
-                 unsigned long long* err, hipStream_t st) {
+                 unsigned long long* err, hipStream_t st, int32_t* skin, const double4* xref,
+                 double half_skin2) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_move, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, s, F, T, n, g,
-                     row_rank, dt_vel, dt_disp, key, err);
+                     row_rank, dt_vel, dt_disp, key, err, skin, xref, half_skin2);
 }
 
 void launch_advance(const State& s, const double4* F, const double4* T, int32_t n,

@@ -35,7 +35,7 @@ struct Detector {
     const float dxf = (p2f.x + sx * a.sizef[0]) - mef.x;
     const float dyf = (p2f.y + sy * a.sizef[1]) - mef.y;
     const float dzf = (p2f.z + sz * a.sizef[2]) - mef.z;
-    const float rsf = mef.w + p2f.w + a.margin;
+    const float rsf = mef.w + p2f.w + (float)a.skin + a.margin;
     if (dxf * dxf + dyf * dyf + dzf * dzf > rsf * rsf) return;
     if (!have_me) {
       const int s = a.perm[t];
@@ -60,22 +60,30 @@ struct Detector {
       B = mk(X.x + (-shx), X.y + (-shy), X.z + (-shz));
       rA = p2.w; rB = r; crA = ax2.crust; crB = ax.crust;
     }
-    const int rc = sphere_sphere(A, rA, crA, B, rB, crB, c, ov, dist);
-    if (rc < 0) {
-      atomicMin(&a.err[kErrContact], first ? pack_ids(ax.id, ax2.id) : pack_ids(ax2.id, ax.id));
-      return;
+    if (a.skin > 0.) {
+      // candidate: gap below the skin (the contact test of every step is
+      // made on the candidates by the force kernel, with the same body order)
+      const v3 AB = sub(B, A);
+      const double gap = norm(AB) - (rA + rB);
+      if (!(gap < a.skin)) return;
+    } else {
+      const int rc = sphere_sphere(A, rA, crA, B, rB, crB, c, ov, dist);
+      if (rc < 0) {
+        atomicMin(&a.err[kErrContact], first ? pack_ids(ax.id, ax2.id) : pack_ids(ax2.id, ax.id));
+        return;
+      }
+      if (rc == 0 || !(dist < 0.)) return;
     }
-    if (rc == 0 || !(dist < 0.)) return;
     const int code = pack_code(sx, sy, sz, first);
     if (nout < kListCap) {
-      a.out_list[(size_t)t * kListCap + nout] = make_int2(u, code);
+      a.out_list[(size_t)nout * a.lstride + t] = make_int2(u, code);
     } else {
       atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax.id);
     }
     ++nout;
     const int kin = atomicAdd(&a.in_count[u], 1);
     if (kin < kListCap) {
-      a.in_list[(size_t)u * kListCap + kin] = make_int2(t, mirror_code(code));
+      a.in_list[(size_t)kin * a.lstride + u] = make_int2(t, mirror_code(code));
     } else {
       atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax2.id);
     }
@@ -105,6 +113,24 @@ __device__ __forceinline__ void detect_particle(const DetectArgs& a, int t) {
 }
 
 // ---------------------------------------------------------------- force
+// Image shift (-1, 0, +1 periods) that brings a partner at offset delta from
+// a particle nearest to it along axis d.  Pairs of a candidate list are closer
+// than one cell edge, at most a third of the period, so the image is unique and
+// equals the neighbour-cell shift a fresh sweep assigns.
+__device__ __forceinline__ int image_shift(const Grid& g, int d, double delta) {
+  if (!g.periodic[d]) return 0;
+  const double h = 0.5 * g.size[d];
+  return delta > h ? -1 : (delta < -h ? 1 : 0);
+}
+
+// Squared displacement from xr to x, periodic axes taken by minimum image (a
+// particle that crossed a periodic face has moved by its step, not a period).
+__device__ __forceinline__ double displacement2(const Grid& g, v3 x, v3 xr) {
+  double e[3] = {x.x - xr.x, x.y - xr.y, x.z - xr.z};
+  for (int d = 0; d < 3; ++d) e[d] += image_shift(g, d, e[d]) * g.size[d];
+  return e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+}
+
 template <bool RECORD>
 struct Reducer {
   const ForceArgs& a;
@@ -114,17 +140,29 @@ struct Reducer {
   Aux ax;
   v3 F, T;
   int ncont, nrec;
+  bool cand;      // list entries are skin candidates: test each for contact
+  bool via_perm;  // the input copy is in the previous order (read through perm)
 
   // One contact of this particle's segment: partner slot p, code (shift to the
   // partner's image, first-body flag).  The pair is evaluated with A = first
   // body and B = second body moved to the first body's image, exactly as the
   // partner's thread does.
   __device__ __forceinline__ void contact(int p, int code) {
-    const int sx = (code & 3) - 1, sy = ((code >> 2) & 3) - 1, sz = ((code >> 4) & 3) - 1;
     const bool first = (code & 64) != 0;
-    const double shx = sx * a.grid.size[0], shy = sy * a.grid.size[1], shz = sz * a.grid.size[2];
-    const int s2 = a.perm[p];
+    const int s2 = via_perm ? a.perm[p] : p;
     const double4 p2 = a.in.pos_r[s2];
+    int sx, sy, sz;
+    if (cand) {
+      // either body may have crossed a periodic face since the list was
+      // built, so the stored shift is stale: take the nearest image
+      sx = image_shift(a.grid, 0, p2.x - X.x);
+      sy = image_shift(a.grid, 1, p2.y - X.y);
+      sz = image_shift(a.grid, 2, p2.z - X.z);
+    } else {
+      sx = (code & 3) - 1; sy = ((code >> 2) & 3) - 1; sz = ((code >> 4) & 3) - 1;
+    }
+    const double shx = sx * a.grid.size[0], shy = sy * a.grid.size[1], shz = sz * a.grid.size[2];
+    if (cand && apart(p2)) return;
     const double4 vm2 = a.in.vel_m[s2];
     const double4 wr2 = a.in.om_req[s2];
     const Aux ax2 = a.in.aux[s2];
@@ -139,7 +177,14 @@ struct Reducer {
       B = mk(X.x + (-shx), X.y + (-shy), X.z + (-shz));
       rA = p2.w; rB = r; crA = ax2.crust; crB = ax.crust;
     }
-    sphere_sphere(A, rA, crA, B, rB, crB, c, ov, dist);
+    const int rc = sphere_sphere(A, rA, crA, B, rB, crB, c, ov, dist);
+    if (cand) {
+      if (rc < 0) {
+        atomicMin(&a.err[kErrContact], first ? pack_ids(ax.id, ax2.id) : pack_ids(ax2.id, ax.id));
+        return;
+      }
+      if (rc == 0 || !(dist < 0.)) return;
+    }
     const Model md = a.models[(first ? ax.mat : ax2.mat) * a.model_stride + (first ? ax2.mat : ax.mat)];
     v3 f;
     if (first)
@@ -156,6 +201,40 @@ struct Reducer {
       F = add(F, fn);
       T = add(T, cross(sub(c, B), fn));
     }
+  }
+
+  // Candidate clearly beyond contact: the margin dwarfs the rounding of the
+  // exact test (sphere_sphere), which decides every other candidate.
+  __device__ __forceinline__ bool apart(const double4& p2) const {
+    const int sx = image_shift(a.grid, 0, p2.x - X.x);
+    const int sy = image_shift(a.grid, 1, p2.y - X.y);
+    const int sz = image_shift(a.grid, 2, p2.z - X.z);
+    const double ex = (p2.x + sx * a.grid.size[0]) - X.x;
+    const double ey = (p2.y + sy * a.grid.size[1]) - X.y;
+    const double ez = (p2.z + sz * a.grid.size[2]) - X.z;
+    const double rs = r + p2.w;
+    return ex * ex + ey * ey + ez * ez > rs * rs * (1. + 1e-10);
+  }
+
+  // Entries of a candidate list (cnt entries at l[e * stride]) that are not
+  // clearly apart, as a bit mask.  The partner positions are loaded four at a
+  // time with independent loads: most candidates are apart, and testing them
+  // one after another would chain two dependent loads per candidate.
+  __device__ __forceinline__ uint32_t near_mask(const int2* l, int cnt, int stride) const {
+    uint32_t mask = 0;
+    for (int e0 = 0; e0 < cnt; e0 += 4) {
+      int p[4];
+      double4 q[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) p[k] = e0 + k < cnt ? l[(size_t)(e0 + k) * stride].x : -1;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (p[k] >= 0) q[k] = a.in.pos_r[via_perm ? a.perm[p[k]] : p[k]];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (p[k] >= 0 && !apart(q[k])) mask |= 1u << (e0 + k);
+    }
+    return mask;
   }
 
   __device__ __forceinline__ void record(int id0, int id1, v3 c, v3 ov, double dist, v3 f) {
@@ -177,7 +256,11 @@ struct Reducer {
 // sorted order).  Returns the number of contacts it is the first body of.
 template <int MODE, bool RECORD>
 __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
-  const int s = a.perm[t];
+  // with skin lists the state moves through perm only after a rebuild;
+  // otherwise it stays in its slot
+  const bool skin = a.skin_did != nullptr;
+  const bool rebuilt = skin && *a.skin_did != 0;
+  const int s = (!skin || rebuilt) ? a.perm[t] : t;
   const double4 pr = a.in.pos_r[s];
   const double4 vm = a.in.vel_m[s];
   const double4 wr = a.in.om_req[s];
@@ -197,23 +280,33 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
   R.T = mk(0., 0., 0.);
   R.ncont = 0;
   R.nrec = 0;
+  R.cand = skin;
+  R.via_perm = !skin || rebuilt;
   // contact segment: detected contacts in sweep order, then the contacts
   // other particles detected, by increasing partner slot
   const int nout = min(a.out_count[t], kListCap);
   const int nin = min(a.in_count[t], kListCap);
-  a.in_count[t] = 0;
-  const int2* ol = a.out_list + (size_t)t * kListCap;
-  for (int e = 0; e < nout; ++e) {
-    const int2 en = ol[e];
+  if (!skin) a.in_count[t] = 0;  // skin lists keep their entries until the next rebuild
+  const int2* ol = a.out_list + t;
+  const int2* il = a.in_list + t;
+  // candidate lists: only the entries that may touch take part in the walk
+  // (the others add nothing), in the same order
+  uint32_t omask = (1u << nout) - 1u, imask = (1u << nin) - 1u;
+  if (skin) {
+    omask = R.near_mask(ol, nout, a.lstride);
+    imask = R.near_mask(il, nin, a.lstride);
+  }
+  for (uint32_t m = omask; m != 0u; m &= m - 1u) {
+    const int2 en = ol[(size_t)__builtin_ctz(m) * a.lstride];
     R.contact(en.x, en.y);
   }
-  const int2* il = a.in_list + (size_t)t * kListCap;
   long long last = -1;
-  for (int e = 0; e < nin; ++e) {
+  const int nlive = __builtin_popcount(imask);
+  for (int e = 0; e < nlive; ++e) {
     long long key;
-    const int f = next_incoming(il, nin, last, key);
+    const int f = next_incoming(il, nin, a.lstride, last, key, imask);
     if (f < 0) break;
-    const int2 en = il[f];
+    const int2 en = il[(size_t)f * a.lstride];
     R.contact(en.x, en.y);
     last = key;
   }
@@ -305,6 +398,19 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
       }
     }
   }
+  if (skin) {
+    // the reference position of the candidate lists: the positions of the
+    // rebuild's sweep; a rebuild is due once a particle has moved skin/2
+    v3 xr;
+    if (rebuilt) {
+      xr = X;
+      a.xref[t] = make_double4(X.x, X.y, X.z, 0.);
+    } else {
+      const double4 x4 = a.xref[t];
+      xr = mk(x4.x, x4.y, x4.z);
+    }
+    if (displacement2(a.grid, Xo, xr) > a.half_skin2) atomicOr(a.skin_need, 1);
+  }
   a.out.pos_r[t] = make_double4(Xo.x, Xo.y, Xo.z, R.r);
   a.out.vel_m[t] = make_double4(Vo.x, Vo.y, Vo.z, m);
   a.out.om_req[t] = make_double4(Wo.x, Wo.y, Wo.z, R.req);
@@ -325,7 +431,9 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
 __device__ __forceinline__ void move_particle(const State& s, const double4* Fa, const double4* Ta,
                                               int i, const Grid& g, const int32_t* row_rank,
                                               double dtv, double dtd, int32_t* key,
-                                              unsigned long long* err) {
+                                              unsigned long long* err, int32_t* skin = nullptr,
+                                              const double4* xref = nullptr,
+                                              double half_skin2 = 0.) {
   const double4 pr = s.pos_r[i], vm = s.vel_m[i], wr = s.om_req[i], wb = s.wbf_I[i],
                 qq = s.quat[i];
   const double4 f4 = Fa[i], t4 = Ta[i];
@@ -366,6 +474,10 @@ __device__ __forceinline__ void move_particle(const State& s, const double4* Fa,
   } else {
     key[i] = 0;
     atomicMin(&err[kErrSim], (unsigned long long)(uint32_t)ax.id);
+  }
+  if (skin) {
+    const double4 x4 = xref[i];
+    if (displacement2(g, X, mk(x4.x, x4.y, x4.z)) > half_skin2) atomicOr(skin, 1);
   }
   s.pos_r[i] = make_double4(X.x, X.y, X.z, pr.w);
   s.vel_m[i] = make_double4(V.x, V.y, V.z, vm.w);

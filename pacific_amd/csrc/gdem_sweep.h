@@ -101,18 +101,21 @@ __device__ __forceinline__ void half_stencil_sweep(const Grid& g, const int32_t*
   }
 }
 
-// Next entry of an incoming contact list in the order (partner slot, code,
-// list position), after the entry of key `last` (-1 before the first): returns
-// its position and key, or -1 when every entry has been taken.  The position
+// Next entry of an incoming contact list (entry f at il[f * stride], taken
+// only where bit f of mask is set) in the order (partner slot, code, list
+// position), after the entry of key `last` (-1 before the first): returns its
+// position and key, or -1 when every entry has been taken.  The position
 // breaks ties, so an entry that appears twice (a particle swept from a cell it
 // is not binned in, after it left the grid) is still visited once per copy and
 // the walk never runs past the list.
-__host__ __device__ __forceinline__ int next_incoming(const int2* il, int nin, long long last,
-                                                      long long& key) {
+__host__ __device__ __forceinline__ int next_incoming(const int2* il, int nin, int stride,
+                                                      long long last, long long& key,
+                                                      uint32_t mask = 0xffffffffu) {
   long long best = 0x7fffffffffffffffll;
   int bi = -1;
   for (int f = 0; f < nin; ++f) {
-    const int2 en = il[f];
+    if (!((mask >> f) & 1u)) continue;
+    const int2 en = il[(size_t)f * stride];
     const long long k = ((long long)en.x << 12) | ((long long)(en.y & 127) << 5) | (long long)f;
     if (k > last && k < best) {
       best = k;

@@ -93,6 +93,8 @@ def load_library(path: str = LIB_PATH):
         "gdem_prime": (C.c_int, [vp]),
         "gdem_step": (C.c_int, [vp, i32, d]),
         "gdem_set_graph": (C.c_int, [vp, i32]),
+        "gdem_set_skin": (C.c_int, [vp, d]),
+        "gdem_get_skin": (C.c_int, [vp, P(d), P(i64)]),
         "gdem_synchronize": (C.c_int, [vp]),
         "gdem_get_state": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
         "gdem_set_contact_recording": (C.c_int, [vp, i32]),
@@ -145,6 +147,7 @@ def load_library(path: str = LIB_PATH):
 EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall",
             "gdem_set_contact_model", "gdem_link_update", "gdem_compute_forces",
             "gdem_move", "gdem_advance_velocity", "gdem_prime", "gdem_step", "gdem_set_graph",
+            "gdem_set_skin", "gdem_get_skin",
             "gdem_synchronize", "gdem_get_state", "gdem_set_contact_recording",
             "gdem_get_contacts", "gdem_last_error", "gdem_set_kernel_timing",
             "gdem_get_counters", "gdem_reset_counters", "gdem_get_grid", "gdem_version",
@@ -288,6 +291,16 @@ class GdemSystem:
 
     def synchronize(self):
         self._check(self.lib.gdem_synchronize(self.h), "gdem_synchronize")
+
+    def set_skin(self, skin):
+        """Skin of the candidate lists (m); negative = automatic, 0 = off."""
+        self._check(self.lib.gdem_set_skin(self.h, float(skin)), "gdem_set_skin")
+
+    def get_skin(self):
+        """(effective skin in m, list rebuilds since the particles or skin were set)."""
+        s, k = C.c_double(), C.c_int64()
+        self._check(self.lib.gdem_get_skin(self.h, C.byref(s), C.byref(k)), "gdem_get_skin")
+        return s.value, k.value
 
     def set_graph(self, on):
         self._check(self.lib.gdem_set_graph(self.h, int(bool(on))), "gdem_set_graph")

@@ -241,3 +241,93 @@ def test_graph_replay_equals_eager_steps(name):
         e.compute_forces()
     g.close()
     e.close()
+
+
+@pytest.mark.parametrize("name", ["periodic_hertz", "periodic_hooke"])
+def test_skin_lists_follow_rebuild_every_step(name):
+    """Candidate lists with a skin (rebuilt only after a skin/2 displacement)
+    give the contact set of a rebuild at every step, step after step, and the
+    same trajectory up to the rounding of the sums (the body order of a pair
+    is the one of its last rebuild)."""
+    print(f"[case] {name} skin lists vs per-step rebuild", flush=True)
+    wl = case(name)
+    a = gdem.build_system(wl)          # automatic skin
+    b = gdem.build_system(wl)
+    b.set_skin(0.0)
+    a.set_contact_recording(32)
+    b.set_contact_recording(32)
+    for calls in range(6):
+        a.step(15, wl["dt"])
+        b.step(15, wl["dt"])
+        assert pair_set(a.get_contacts()) == pair_set(b.get_contacts()), calls
+    skin, rebuilds = a.get_skin()
+    print(f"[case] skin {skin:.3e} m, {rebuilds} rebuilds in 90 steps", flush=True)
+    assert skin > 0 and 1 <= rebuilds < 45
+    assert b.get_skin() == (0.0, 0)
+    sa, sb = a.get_state(), b.get_state()
+    x0 = wl["particles"]["x"]
+    L = np.array(wl["size"])
+    da, db = sa["x"] - x0, sb["x"] - x0
+    da -= L * np.round(da / L)
+    db -= L * np.round(db / L)
+    assert rel_err(da, db) <= 1e-11
+    assert rel_err(sa["v"], sb["v"]) <= 1e-11
+    assert rel_err(sa["w"], sb["w"]) <= 1e-11
+    a.close()
+    b.close()
+
+
+def face_chains(L=0.024, r=1.0e-3, gap=1.998e-3, k=6, v=0.5):
+    """Two chains of touching spheres (overlap 2 um) drifting through
+    periodic faces: one through the low x face (-x), one through the high z
+    face (+z).  The leading spheres cross within 10 steps of 5e-6 s while
+    every sphere stays within 0.08 mm of its start over 30 steps, so the
+    candidate lists (skin 0.2 mm) are built once and their pairs change image
+    while in contact."""
+    xs, vs = [], []
+    for j in range(k):
+        xs.append(((0.02e-3 - j * gap) % L, 0.006, 0.006))
+        vs.append((-v, 0.0, 0.0))
+    for j in range(k):
+        xs.append((0.012, 0.018, (L - 0.02e-3 + j * gap) % L))
+        vs.append((0.0, 0.0, v))
+    x, vv = np.array(xs), np.array(vs)
+    hertz = (1.0e7, 0.85, 1.0e7 / 4.0, 0.4, 0.0)
+    return {"name": "periodic_face_chains", "origin": (0.0, 0.0, 0.0), "size": (L, L, L),
+            "periodic": (1, 1, 1), "cell_size": 2.4 * r, "gravity": (0.0, 0.0, 0.0),
+            "dt": 5.0e-6, "steps": 30, "models": [(0, 0, workloads.MODEL_HERTZ, hertz)],
+            "walls": [], "particles": workloads._particles(x, vv, np.full(len(x), r))}
+
+
+def test_skin_lists_across_periodic_faces(oracle_mod):
+    """Pairs whose bodies cross a periodic face between rebuilds keep their
+    contact: the list is built once (no particle moves skin/2), yet every
+    5 steps the contact set equals the oracle's and the state follows it."""
+    print("[case] skin lists, contacts crossing periodic faces", flush=True)
+    wl = face_chains()
+    dt = wl["dt"]
+    orc = oracle_mod.Oracle(wl, log_capacity=64)
+    orc.prime()
+    sysg = gdem.build_system(wl)
+    sysg.set_contact_recording(8)
+    x0 = wl["particles"]["x"]
+    L = np.array(wl["size"])
+    wrapped = np.zeros(len(x0), bool)
+    for calls in range(6):
+        orc.step(5, dt)
+        sysg.step(5, dt)
+        oc = orc.contacts()
+        assert len(oc["id0"]) == 10
+        assert pair_set(sysg.get_contacts()) == pair_set(oc), calls
+        so, sg = orc.state(), sysg.get_state()
+        np.testing.assert_array_equal(np.floor((sg["x"] - x0) / L), np.floor((so["x"] - x0) / L))
+        wrapped |= np.abs(sg["x"] - x0).max(axis=1) > 0.5 * L.min()
+        do, dg = so["x"] - x0, sg["x"] - x0
+        do -= L * np.round(do / L)
+        dg -= L * np.round(dg / L)
+        assert rel_err(dg, do) <= 1e-12, calls
+        assert rel_err(sg["v"], so["v"]) <= 1e-12, calls
+    assert wrapped[0] and wrapped[6]
+    skin, rebuilds = sysg.get_skin()
+    assert skin > 0 and rebuilds == 1, (skin, rebuilds)
+    sysg.close()

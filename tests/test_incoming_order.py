@@ -1,7 +1,8 @@
 """The incoming-contact walk of the force kernels (next_incoming in
 pacific_amd/csrc/gdem_sweep.h) visits every list entry exactly once, in
 (partner slot, code, position) order, also when an entry appears twice, and
-then reports the end: it never yields an index outside the list.  Built for the
+then reports the end: it never yields an index outside the list; with a mask
+it visits the masked entries only, in the same order.  Built for the
 host with hipcc (no GPU needed)."""
 import os
 import subprocess
@@ -22,9 +23,22 @@ int main() {
   long long last = -1, key = 0;
   int m = 0;
   for (int e = 0; e < n + 2; ++e) {
-    const int f = gdem::next_incoming(il, n, last, key);
+    const int f = gdem::next_incoming(il, n, 1, last, key);
     if (f < 0) break;
     if (f >= n) { printf("out of range %d\n", f); return 1; }
+    printf("%d ", f);
+    last = key;
+    ++m;
+  }
+  printf("| %d ", m);
+  // only the entries of a mask (here 0, 3, 5, 6), in the same order
+  const unsigned mask = (1u << 0) | (1u << 3) | (1u << 5) | (1u << 6);
+  last = -1;
+  m = 0;
+  for (int e = 0; e < n + 2; ++e) {
+    const int f = gdem::next_incoming(il, n, 1, last, key, mask);
+    if (f < 0) break;
+    if (f >= n || !((mask >> f) & 1u)) { printf("bad %d\n", f); return 1; }
     printf("%d ", f);
     last = key;
     ++m;
@@ -44,4 +58,5 @@ def test_incoming_walk_visits_each_entry_once(tmp_path):
                            "-o", str(exe), str(src)])
     out = subprocess.check_output([str(exe)]).decode().split()
     # order: (0,127)#6, (3,1)#1, (3,9)#3, (7,2)#5, (7,5)#0, (7,5)#2, (12,0)#4
-    assert out == ["6", "1", "3", "5", "0", "2", "4", "|", "7"]
+    assert out == ["6", "1", "3", "5", "0", "2", "4", "|", "7",
+                   "6", "3", "5", "0", "|", "4"]
