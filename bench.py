@@ -254,7 +254,7 @@ def run_calibration(args, nsys=None):
     short.close()
     log(f"adjoint vs central differences (48 steps): max relative error {worst:.2e}")
     # the single-system step looped over systems, gradient by central differences
-    nloop = 4
+    nloop = 0 if getattr(args, "no_looped", False) else 4
     t1 = time.perf_counter()
     for s in range(nloop):
         wl = systems[s]
@@ -272,10 +272,12 @@ def run_calibration(args, nsys=None):
                     last = step
                     one.get_state()
                 one.close()
-    tl = (time.perf_counter() - t1) / nloop
-    loop_rate = 1.0 / tl
-    log(f"looped single-system step with central differences: {tl:.3f} s per gradient "
-        f"({loop_rate:.2f} gradients/s)")
+    loop_rate = None
+    if nloop:
+        tl = (time.perf_counter() - t1) / nloop
+        loop_rate = 1.0 / tl
+        log(f"looped single-system step with central differences: {tl:.3f} s per gradient "
+            f"({loop_rate:.2f} gradients/s)")
     return {
         "metric": "batched calibration gradients/s (secondary line)", "value": rate,
         "unit": "gradients/s", "n_gpus": 1, "steps": nsteps, "warmup": 1,
@@ -290,7 +292,7 @@ def run_calibration(args, nsys=None):
                      "central_differences": "h = 1e-6 relative, 48-step horizon, 2 systems"},
         "looped_single_system_fd": {"value": loop_rate, "unit": "gradients/s",
                                     "sample": f"{nloop} systems, 6 runs of {nsteps} steps each"},
-        "speedup_vs_looped": rate / loop_rate,
+        "speedup_vs_looped": rate / loop_rate if loop_rate else None,
     }
 
 
@@ -311,6 +313,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-calibration", action="store_true",
                     help="skip the batched calibration-gradient part of the default line")
+    ap.add_argument("--no-looped", action="store_true",
+                    help="calibration: skip the looped single-system baseline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the DEM step is not sharded "

@@ -248,6 +248,7 @@ struct BatchRunner {
       f.skin_did = nullptr;
       f.skin_need = nullptr;
       f.xref = nullptr;
+      f.cmask = nullptr;
       f.half_skin2 = 0.;
     }
     __syncthreads();

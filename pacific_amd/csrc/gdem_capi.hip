@@ -83,6 +83,7 @@ struct gdem_ctx {
   double skin_req = -1., skin = 0., rmax = 0.;
   int32_t* d_skin = nullptr;
   double4* d_xref = nullptr;
+  unsigned long long* d_cmask = nullptr;
   // captured fused step (one graph per state-buffer parity)
   bool use_graph = true;
   int64_t graph_gen = 0;
@@ -203,7 +204,8 @@ extern "C" int gdem_destroy(gdem_ctx* c) {
   free_cstate(c->cbuf[1]);
   (void)hipFree(c->d_cand); (void)hipFree(c->d_crec); (void)hipFree(c->d_mask); (void)hipFree(c->d_work);
   (void)hipFree(c->d_nwork); (void)hipFree(c->d_shapes);
-  (void)hipFree(c->d_skin); (void)hipFree(c->d_xref); (void)hipFree(c->d_wall_shapes); (void)hipFree(c->d_wall_sc);
+  (void)hipFree(c->d_skin); (void)hipFree(c->d_xref); (void)hipFree(c->d_cmask);
+  (void)hipFree(c->d_wall_shapes); (void)hipFree(c->d_wall_sc);
   for (auto& e : c->ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
   for (auto& g : c->graph_exec)
     if (g) (void)hipGraphExecDestroy(g);
@@ -227,9 +229,10 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
   (void)hipFree(c->d_rec); (void)hipFree(c->d_rec_count);
   (void)hipFree(c->d_cand); (void)hipFree(c->d_crec); (void)hipFree(c->d_mask); (void)hipFree(c->d_work);
   (void)hipFree(c->d_nwork);
-  (void)hipFree(c->d_skin); (void)hipFree(c->d_xref);
+  (void)hipFree(c->d_skin); (void)hipFree(c->d_xref); (void)hipFree(c->d_cmask);
   c->d_skin = nullptr;
   c->d_xref = nullptr;
+  c->d_cmask = nullptr;
   c->skin = 0.;
   c->d_F = c->d_T = nullptr;
   c->bins.key = c->bins.slot = c->bins.perm = nullptr;
@@ -261,6 +264,7 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
     HIP_TRY(hipMalloc(&c->d_out_list, sizeof(int2) * nn * kListCap));
     HIP_TRY(hipMalloc(&c->d_skin, sizeof(int32_t) * kSkinWords));
     HIP_TRY(hipMalloc(&c->d_xref, sizeof(double4) * nn));
+    HIP_TRY(hipMalloc(&c->d_cmask, sizeof(unsigned long long) * nn));
     const int32_t words[kSkinWords] = {1, 1, 0};  // the first evaluation rebuilds
     HIP_TRY(hipMemcpy(c->d_skin, words, sizeof(words), hipMemcpyHostToDevice));
   }
@@ -794,14 +798,20 @@ static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wr
       a.skin_did = need;
       a.skin_need = need_next;
       a.xref = c->d_xref;
+      a.cmask = c->d_cmask;
       a.half_skin2 = 0.25 * c->skin * c->skin;
     }
-    launch_detect(d, c->stream);
-    if (c->timing) {
-      ev = next_event_pair(c);
-      if (ev) HIP_TRY(hipEventRecord(ev->first, c->stream));
-    }
-    launch_force(mode, record, a, c->stream);
+    if (skin)
+      launch_detect_screen(d, a, c->stream);
+    else
+      launch_detect(d, c->stream);
+    // timing: the events take the force kernel's own start and end times
+    // (an event recorded between two launches would add the dispatch gap)
+    if (c->timing) ev = next_event_pair(c);
+    launch_force(mode, record, a, c->stream, ev ? ev->first : nullptr, ev ? ev->second : nullptr);
+    HIP_TRY(hipGetLastError());
+    *evp = ev;
+    return GDEM_OK;
   }
   HIP_TRY(hipGetLastError());
   if (ev) HIP_TRY(hipEventRecord(ev->second, c->stream));

@@ -118,6 +118,7 @@ struct ForceArgs {
   const int32_t* skin_did;  // need word of this evaluation: lists rebuilt
   int32_t* skin_need;       // need word of the next evaluation
   double4* xref;        // positions at the last rebuild, slot order
+  unsigned long long* cmask;  // screening masks (k_screen), slot order
   double half_skin2;    // (skin / 2)^2
 };
 
@@ -248,7 +249,12 @@ void launch_binning(Bins& b, const State& s, int32_t n, int32_t ncell, hipStream
                     const int32_t* gate = nullptr, int32_t* in_count = nullptr,
                     int32_t* clear_need = nullptr, int32_t* rebuilds = nullptr);
 void launch_detect(const DetectArgs& a, hipStream_t st);
-void launch_force(int mode, bool record, const ForceArgs& a, hipStream_t st);
+// With events, they record the kernel's own start and end (timing).
+void launch_force(int mode, bool record, const ForceArgs& a, hipStream_t st,
+                  hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+// Skin lists: candidate sweep (d.gate set) or screening of the lists (writes
+// f.cmask), in one launch.
+void launch_detect_screen(const DetectArgs& d, const ForceArgs& f, hipStream_t st);
 void launch_move(const State& s, const double4* F, const double4* T, int32_t n, const Grid& g,
                  const int32_t* row_rank, double dt_vel, double dt_disp, int32_t* key,
                  unsigned long long* err, hipStream_t st, int32_t* skin = nullptr,

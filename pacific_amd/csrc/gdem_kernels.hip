@@ -14,6 +14,7 @@
 //             values, no floating-point atomics -- then its tail applies the
 //             leap-frog kicks and drift (SecondOrderLeapFrog.cpp:44-94)
 //   move / advance : stand-alone integrator phases for the per-phase ABI.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "gdem_internal.h"
@@ -192,6 +193,18 @@ __global__ void __launch_bounds__(kBlock) k_detect(DetectArgs a) {
   detect_particle(a, t);
 }
 
+// Skin lists: the candidate sweep when the lists are rebuilt (need word set),
+// otherwise the screening of the existing lists for this evaluation (the force
+// kernel walks every candidate after a rebuild).
+__global__ void __launch_bounds__(kBlock) k_detect_screen(DetectArgs d, ForceArgs f) {
+  const int t = logical_index(d.nblk8);
+  if (t >= d.n) return;
+  if (*d.gate != 0)
+    detect_particle(d, t);
+  else
+    screen_particle(f, t);
+}
+
 // ---------------------------------------------------------------- force
 template <int MODE, bool RECORD>
 __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a) {
@@ -291,10 +304,22 @@ void launch_detect(const DetectArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(k_detect, dim3(a.nblk8), dim3(kBlock), 0, st, a);
 }
 
-void launch_force(int mode, bool record, const ForceArgs& a, hipStream_t st) {
+void launch_detect_screen(const DetectArgs& d, const ForceArgs& f, hipStream_t st) {
+  if (d.n <= 0) return;
+  hipLaunchKernelGGL(k_detect_screen, dim3(d.nblk8), dim3(kBlock), 0, st, d, f);
+}
+
+void launch_force(int mode, bool record, const ForceArgs& a, hipStream_t st, hipEvent_t start,
+                  hipEvent_t stop) {
   if (a.n <= 0) return;
   dim3 grid(a.nblk8), block(kBlock);
-#define GDEM_LAUNCH_FORCE(M, R) hipLaunchKernelGGL((k_force<M, R>), grid, block, 0, st, a)
+#define GDEM_LAUNCH_FORCE(M, R)                                                               \
+  do {                                                                                        \
+    if (start)                                                                                \
+      hipExtLaunchKernelGGL((k_force<M, R>), grid, block, 0, st, start, stop, 0, a);          \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_force<M, R>), grid, block, 0, st, a);                             \
+  } while (0)
   if (record) {
     if (mode == MODE_FORCE) GDEM_LAUNCH_FORCE(MODE_FORCE, true);
     else if (mode == MODE_KICK) GDEM_LAUNCH_FORCE(MODE_KICK, true);

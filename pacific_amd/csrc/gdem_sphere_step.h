@@ -131,6 +131,58 @@ __device__ __forceinline__ double displacement2(const Grid& g, v3 x, v3 xr) {
   return e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
 }
 
+// Candidate at p2 clearly beyond contact with the sphere (X, r): the margin
+// dwarfs the rounding of the exact test (sphere_sphere), which decides every
+// other candidate.
+__device__ __forceinline__ bool apart(const Grid& g, v3 X, double r, const double4& p2) {
+  const int sx = image_shift(g, 0, p2.x - X.x);
+  const int sy = image_shift(g, 1, p2.y - X.y);
+  const int sz = image_shift(g, 2, p2.z - X.z);
+  const double ex = (p2.x + sx * g.size[0]) - X.x;
+  const double ey = (p2.y + sy * g.size[1]) - X.y;
+  const double ez = (p2.z + sz * g.size[2]) - X.z;
+  const double rs = r + p2.w;
+  return ex * ex + ey * ey + ez * ez > rs * rs * (1. + 1e-10);
+}
+
+// Entries of a candidate list (cnt entries at l[e * a.lstride], partners in
+// slot order) that are not clearly apart from the sphere (X, r), as a bit
+// mask.  The partner positions are loaded four at a time with independent
+// loads.
+__device__ __forceinline__ uint32_t near_mask(const ForceArgs& a, v3 X, double r, const int2* l,
+                                              int cnt) {
+  uint32_t mask = 0;
+  for (int e0 = 0; e0 < cnt; e0 += 4) {
+    int p[4];
+    double4 q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) p[k] = e0 + k < cnt ? l[(size_t)(e0 + k) * a.lstride].x : -1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (p[k] >= 0) q[k] = a.in.pos_r[p[k]];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (p[k] >= 0 && !apart(a.grid, X, r, q[k])) mask |= 1u << (e0 + k);
+  }
+  return mask;
+}
+
+// Candidate screening of one particle between rebuilds (skin lists, storage
+// slot t; the state stays in slot order): bit e of the low word marks out-list
+// entry e, bit e of the high word in-list entry e, when the pair may touch.
+// The force kernel walks the marked entries only: most candidates are apart,
+// and testing them there would chain two dependent loads per candidate into a
+// kernel that holds many registers per lane.
+__device__ __forceinline__ void screen_particle(const ForceArgs& a, int t) {
+  const double4 pr = a.in.pos_r[t];
+  const v3 X = mk(pr.x, pr.y, pr.z);
+  const int nout = min(a.out_count[t], kListCap);
+  const int nin = min(a.in_count[t], kListCap);
+  const uint32_t om = near_mask(a, X, pr.w, a.out_list + t, nout);
+  const uint32_t im = near_mask(a, X, pr.w, a.in_list + t, nin);
+  a.cmask[t] = (unsigned long long)om | ((unsigned long long)im << 32);
+}
+
 template <bool RECORD>
 struct Reducer {
   const ForceArgs& a;
@@ -162,7 +214,7 @@ struct Reducer {
       sx = (code & 3) - 1; sy = ((code >> 2) & 3) - 1; sz = ((code >> 4) & 3) - 1;
     }
     const double shx = sx * a.grid.size[0], shy = sy * a.grid.size[1], shz = sz * a.grid.size[2];
-    if (cand && apart(p2)) return;
+    if (cand && apart(a.grid, X, r, p2)) return;
     const double4 vm2 = a.in.vel_m[s2];
     const double4 wr2 = a.in.om_req[s2];
     const Aux ax2 = a.in.aux[s2];
@@ -203,40 +255,6 @@ struct Reducer {
     }
   }
 
-  // Candidate clearly beyond contact: the margin dwarfs the rounding of the
-  // exact test (sphere_sphere), which decides every other candidate.
-  __device__ __forceinline__ bool apart(const double4& p2) const {
-    const int sx = image_shift(a.grid, 0, p2.x - X.x);
-    const int sy = image_shift(a.grid, 1, p2.y - X.y);
-    const int sz = image_shift(a.grid, 2, p2.z - X.z);
-    const double ex = (p2.x + sx * a.grid.size[0]) - X.x;
-    const double ey = (p2.y + sy * a.grid.size[1]) - X.y;
-    const double ez = (p2.z + sz * a.grid.size[2]) - X.z;
-    const double rs = r + p2.w;
-    return ex * ex + ey * ey + ez * ez > rs * rs * (1. + 1e-10);
-  }
-
-  // Entries of a candidate list (cnt entries at l[e * stride]) that are not
-  // clearly apart, as a bit mask.  The partner positions are loaded four at a
-  // time with independent loads: most candidates are apart, and testing them
-  // one after another would chain two dependent loads per candidate.
-  __device__ __forceinline__ uint32_t near_mask(const int2* l, int cnt, int stride) const {
-    uint32_t mask = 0;
-    for (int e0 = 0; e0 < cnt; e0 += 4) {
-      int p[4];
-      double4 q[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) p[k] = e0 + k < cnt ? l[(size_t)(e0 + k) * stride].x : -1;
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-        if (p[k] >= 0) q[k] = a.in.pos_r[via_perm ? a.perm[p[k]] : p[k]];
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-        if (p[k] >= 0 && !apart(q[k])) mask |= 1u << (e0 + k);
-    }
-    return mask;
-  }
-
   __device__ __forceinline__ void record(int id0, int id1, v3 c, v3 ov, double dist, v3 f) {
     if (nrec < a.rec_cap) {
       ContactRec& cr = a.rec[(size_t)t * a.rec_cap + nrec];
@@ -266,6 +284,9 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
   const double4 wr = a.in.om_req[s];
   const double4 wb = a.in.wbf_I[s];
   const double4 qq = a.in.quat[s];
+  // reference position of the skin lists, loaded with the state
+  double4 xr4 = make_double4(0., 0., 0., 0.);
+  if (skin && !rebuilt) xr4 = a.xref[t];
   Reducer<RECORD> R{a, t};
   R.ax = a.in.aux[s];
   R.X = mk(pr.x, pr.y, pr.z);
@@ -292,9 +313,10 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
   // candidate lists: only the entries that may touch take part in the walk
   // (the others add nothing), in the same order
   uint32_t omask = (1u << nout) - 1u, imask = (1u << nin) - 1u;
-  if (skin) {
-    omask = R.near_mask(ol, nout, a.lstride);
-    imask = R.near_mask(il, nin, a.lstride);
+  if (skin && !rebuilt) {
+    const unsigned long long cm = a.cmask[t];
+    omask = (uint32_t)cm;
+    imask = (uint32_t)(cm >> 32);
   }
   for (uint32_t m = omask; m != 0u; m &= m - 1u) {
     const int2 en = ol[(size_t)__builtin_ctz(m) * a.lstride];
@@ -406,8 +428,7 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
       xr = X;
       a.xref[t] = make_double4(X.x, X.y, X.z, 0.);
     } else {
-      const double4 x4 = a.xref[t];
-      xr = mk(x4.x, x4.y, x4.z);
+      xr = mk(xr4.x, xr4.y, xr4.z);
     }
     if (displacement2(a.grid, Xo, xr) > a.half_skin2) atomicOr(a.skin_need, 1);
   }
