@@ -68,6 +68,8 @@ extern "C" {
 
 #define GDEM_MODEL_HOOKE 1
 #define GDEM_MODEL_HERTZ 2
+#define GDEM_MODEL_HOOKE_MEMORY 3
+#define GDEM_MODEL_HERTZ_MEMORY 4
 
 #define GDEM_MAX_MATERIALS 32
 #define GDEM_MAX_WALLS 64
@@ -139,10 +141,29 @@ int gdem_add_wall(gdem_ctx* ctx, const double* basis, const double* centre,
                   const double* half_extent, double crust, int32_t material);
 
 /* HookeContactForceModel(map) (HookeContactForceModel.cpp:11-20):
-     kind GDEM_MODEL_HOOKE, p = {kn, en, etat, muc, kr}
+     kind GDEM_MODEL_HOOKE, p = {kn, en, etat, muc, kr}            (5 values)
    HertzContactForceModel(map) (HertzContactForceModel.cpp:12-22):
-     kind GDEM_MODEL_HERTZ, p = {Es, en, Gs, muc, kr}
-   kr must be 0 (no rolling resistance). */
+     kind GDEM_MODEL_HERTZ, p = {Es, en, Gs, muc, kr}              (5 values)
+   kr must be 0 (no rolling resistance) for these two.
+   HookeMemoryContactForceModel(map) (HookeMemoryContactForceModel.cpp:11-29):
+     kind GDEM_MODEL_HOOKE_MEMORY,
+     p = {kn, en, etat, muc, kt, mur, etarpf, epst}                 (8 values)
+   HertzMemoryContactForceModel(map) (HertzMemoryContactForceModel.cpp:12-31):
+     kind GDEM_MODEL_HERTZ_MEMORY,
+     p = {Es, en, Gs, muc, mur, etarpf, epst}                       (7 values)
+   The memory models keep a tangential spring, the previous contact normal and
+   a rolling-friction spring torque per contact while it lasts (the contact
+   maps of Component.cpp:567-689).  Their defaults as the reference's XML
+   reader leaves them: etat -1, Gs Es/4, mur 0, etarpf 0, and epst 0 (the
+   reader stores its 1e-10 default under another key,
+   HookeMemoryContactForceModel.cpp:300-306).  Sphere contexts only. */
+/* Time step of the force evaluations for the memory models (the dt that
+   Grains::computeParticlesForceAndTorque passes to App::ComputeForces,
+   Grains.cpp:459-482).  gdem_step(dt) and gdem_move(., dt) set it; a force
+   evaluation with a memory model and no time step set fails with
+   GDEM_ERR_ARG. */
+int gdem_set_time_step(gdem_ctx* ctx, double dt);
+
 int gdem_set_contact_model(gdem_ctx* ctx, int32_t mat_a, int32_t mat_b, int32_t kind,
                            const double* p);
 

@@ -19,10 +19,18 @@
 
 typedef struct {
   int kind;
-  double p[5];
+  double p[8];
   double beta;      /* HookeContactForceModel.cpp:19 / HertzContactForceModel.cpp:20 */
   double m2sqrt56;  /* HertzContactForceModel.cpp:21 */
 } model_t;
+
+/* One entry of a component's contact map (Component.hh:505-508): key = id of
+   the other component (the elementary ids are 1 for spheres), active flag,
+   cumulative tangential spring, previous normal, rolling spring torque. */
+typedef struct {
+  int key, active;
+  double kd[3], nrm[3], spr[3];
+} mem_t;
 
 typedef struct {
   double basis[9], origin[3], ext[3];
@@ -52,6 +60,11 @@ struct orc_sys {
   int32_t *log_id0, *log_id1;
   double *log_pt, *log_ov, *log_d, *log_f;
   int64_t ncand;
+  /* contact maps of the memory models (one per particle) */
+  int has_memory;
+  double dt_mem;
+  int* mem_n;
+  mem_t* mem;     /* [n][ORC_MEM_CAP] */
   /* error */
   int err_code;
   int32_t err_id0, err_id1;
@@ -106,7 +119,9 @@ static void free_particles(orc_sys* s) {
   free(s->x); free(s->v); free(s->w); free(s->wbf); free(s->q); free(s->basis);
   free(s->F); free(s->T); free(s->r); free(s->crust); free(s->mass);
   free(s->inertia); free(s->req); free(s->mat); free(s->next); free(s->prev);
-  free(s->cell);
+  free(s->cell); free(s->mem_n); free(s->mem);
+  s->mem_n = NULL;
+  s->mem = NULL;
 }
 
 void orc_destroy(orc_sys* s) {
@@ -146,6 +161,8 @@ int orc_set_particles(orc_sys* s, int n, const double* x, const double* v,
   s->next = (int*)malloc(sizeof(int) * (size_t)n);
   s->prev = (int*)malloc(sizeof(int) * (size_t)n);
   s->cell = (int*)malloc(sizeof(int) * (size_t)n);
+  s->mem_n = (int*)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
+  s->mem = (mem_t*)calloc((size_t)(n > 0 ? n : 1) * ORC_MEM_CAP, sizeof(mem_t));
   for (int i = 0; i < n; ++i) {
     double rr = radius[i], rho = density[i];
     double vol = 4.0 * ORC_PI * pow(rr, 3.) / 3.0;
@@ -231,16 +248,26 @@ int orc_add_wall(orc_sys* s, const double* basis, const double* centre,
    keys a model by the OR of two material bits, i.e. an unordered pair. */
 int orc_set_contact_model(orc_sys* s, int a, int b, int kind, const double* p) {
   if (a < 0 || b < 0 || a >= ORC_MAX_MATERIALS || b >= ORC_MAX_MATERIALS) return ORC_ERR_ARG;
-  if (kind != ORC_MODEL_HOOKE && kind != ORC_MODEL_HERTZ) return ORC_ERR_ARG;
+  int np = kind == ORC_MODEL_HOOKE || kind == ORC_MODEL_HERTZ ? 5
+         : kind == ORC_MODEL_HOOKE_MEMORY ? 8 : kind == ORC_MODEL_HERTZ_MEMORY ? 7 : 0;
+  if (np == 0) return ORC_ERR_ARG;
   model_t m;
   memset(&m, 0, sizeof(m));
   m.kind = kind;
-  memcpy(m.p, p, sizeof(double) * 5);
+  memcpy(m.p, p, sizeof(double) * (size_t)np);
   double en = p[1];
+  /* HookeMemoryContactForceModel.cpp:28, HertzMemoryContactForceModel.cpp:29-30 */
   m.beta = log(en) / sqrt(ORC_PI * ORC_PI + log(en) * log(en));
   m.m2sqrt56 = -2. * sqrt(5. / 6.);
   s->models[a][b] = m;
   s->models[b][a] = m;
+  if (kind >= ORC_MODEL_HOOKE_MEMORY) s->has_memory = 1;
+  return ORC_OK;
+}
+
+int orc_set_time_step(orc_sys* s, double dt) {
+  if (!(dt > 0.)) return ORC_ERR_ARG;
+  s->dt_mem = dt;
   return ORC_OK;
 }
 
@@ -528,6 +555,187 @@ void orc_contact_force(int kind, const double* p, const double* point,
   st(f, r);
 }
 
+/* ---------------------------------------------------------------------- */
+/* Memory models */
+
+/* Quaternion::setRotFromTwoVectors (Geometry/src/Quaternion.cpp:263-286):
+   (vect, real_part) scaled by 1 / Norm(qq) (Quaternion.cpp:420-424, 558-562) */
+static void rot_two(v3 u, v3 v, v3* qv, double* qw) {
+  double norm_u_norm_v = sqrt(dot(u, u) * dot(v, v));
+  double real_part = norm_u_norm_v + dot(u, v);
+  v3 vect;
+  if (real_part < 1.e-6 * norm_u_norm_v) {
+    real_part = 0.;
+    vect = fabs(u.x) > fabs(u.z) ? mk(-u.y, u.x, 0.) : mk(0., -u.z, u.y);
+  } else {
+    vect = cross(u, v);
+  }
+  double nq = sqrt(vect.x * vect.x + vect.y * vect.y + vect.z * vect.z + real_part * real_part);
+  double d = 1. / nq;
+  *qv = scl(vect, d);
+  *qw = d * real_part;
+}
+
+/* Quaternion::rotateVector (Quaternion.cpp:293-298) */
+static v3 rotate_vec(v3 qv, double qw, v3 v) {
+  double a = qw * qw - norm(qv) * norm(qv);
+  double b = 2. * dot(v, qv);
+  double c = 2. * qw;
+  return add(add(scl(v, a), scl(qv, b)), scl(cross(qv, v), c));
+}
+
+void orc_rotate_two_vectors(const double* u, const double* v, const double* x, double* out) {
+  v3 qv;
+  double qw;
+  rot_two(ld(u), ld(v), &qv, &qw);
+  st(out, rotate_vec(qv, qw, ld(x)));
+}
+
+/* computeTangentialVector (HookeMemoryContactForceModel.cpp:54-65) */
+static v3 tangential_vector(double n_t, v3 ut, v3 kdelta, double epst) {
+  if (norm(ut) > 0.001) return dvd(neg(ut), norm(ut));
+  if ((norm(kdelta) > epst || norm(scl(ut, n_t)) > epst) && norm(add(kdelta, scl(ut, n_t))) > epst)
+    return dvd(neg(add(kdelta, scl(ut, n_t))), norm(add(kdelta, scl(ut, n_t))));
+  return mk(0., 0., 0.);
+}
+
+/* HookeMemoryContactForceModel::performForcesCalculus
+   (HookeMemoryContactForceModel.cpp:72-193) and the Hertz one
+   (HertzMemoryContactForceModel.cpp:73-195).  kd, nrm, spr are the first
+   body's map entry (created as zeros when the contact did not exist); returns
+   delFN + delFT, the rolling moment in delM. */
+static v3 memory_force(const model_t* m, v3 pt, v3 pen, double odist, v3 x0, v3 v0, v3 w0,
+                       double m0, double req0, v3 x1, v3 v1, v3 w1, double m1, double req1,
+                       double dt, int existed, double* kd, double* nrm, double* spr, v3* delM) {
+  int hooke = m->kind == ORC_MODEL_HOOKE_MEMORY;
+  v3 normal = dvd(pen, norm(pen));
+  normal = round3(normal);
+  v3 tmpV = sub(vel_at(x0, v0, w0, pt), vel_at(x1, v1, w1, pt));
+  v3 v_n = scl(normal, dot(tmpV, normal));
+  v3 v_t = sub(tmpV, v_n);
+  double muc = m->p[3], mur, etarpf, epst, kn, kt, gamman, gammat, normFN;
+  v3 delFN;
+  if (hooke) {
+    kn = m->p[0];
+    kt = m->p[4];
+    mur = m->p[5];
+    etarpf = m->p[6];
+    epst = m->p[7];
+    delFN = scl(pen, kn);
+    double avmass = 1. / (1. / m0 + 1. / m1);
+    gamman = -2. * m->beta * sqrt(avmass * kn);
+    delFN = sub(delFN, scl(v_n, gamman));
+    normFN = norm(delFN);
+    double etat = m->p[2] == -1. ? -m->beta * sqrt(kn / avmass) : m->p[2];
+    gammat = 2. * etat * avmass;
+  } else {
+    double Es = m->p[0], Gs = m->p[2];
+    mur = m->p[4];
+    etarpf = m->p[5];
+    epst = m->p[6];
+    double deltan = -odist;
+    double Req = 1. / (1. / req0 + 1. / req1);
+    double avmass = 1. / (1. / m0 + 1. / m1);
+    double sq = sqrt(Req * deltan);
+    double Sn = 2. * Es * sq;
+    double St = 8. * Gs * sq;
+    (void)St;
+    kn = (4. / 3.) * Es * sq;
+    delFN = scl(pen, kn);
+    gamman = m->m2sqrt56 * m->beta * sqrt(avmass * Sn);
+    delFN = sub(delFN, scl(v_n, gamman));
+    normFN = norm(delFN);
+    gammat = m->m2sqrt56 * m->beta * sqrt(avmass * St);
+    kt = 8 * Gs * sq;
+  }
+  v3 viscousFT = scl(v_t, -gammat);
+  v3 K = ld(kd), qv = mk(0., 0., 0.);
+  double qw = 0.;
+  if (existed) {
+    rot_two(ld(nrm), normal, &qv, &qw);
+    K = rotate_vec(qv, qw, K);
+  }
+  K = add(K, scl(v_t, dt * kt));
+  st(nrm, normal);
+  v3 tij = tangential_vector(gammat, v_t, K, epst);
+  double normFT = norm(add(neg(K), viscousFT));
+  v3 delFT;
+  if (normFT <= muc * normFN) {
+    delFT = scl(tij, norm(add(neg(K), viscousFT)));
+  } else {
+    if (!hooke || kt != 0.) K = dvd(add(scl(tij, -muc * normFN), viscousFT), kt);
+    delFT = scl(tij, muc * normFN);
+  }
+  st(kd, K);
+  if (mur != 0.) {
+    double Req = 1. / (1. / req0 + 1. / req1);
+    double kr = 3. * kn * mur * mur * Req * Req;
+    double etar = 3. * gamman * mur * mur * Req * Req;
+    double max_normMk = mur * Req * normFN;
+    v3 wrel = sub(w0, w1);
+    v3 S = ld(spr);
+    if (existed) S = rotate_vec(qv, qw, S);
+    S = sub(S, scl(wrel, kr * dt));
+    double normMk = norm(S);
+    if (normMk > max_normMk) S = scl(S, max_normMk / normMk);
+    st(spr, S);
+    *delM = sub(S, scl(wrel, etarpf * etar));
+  } else {
+    *delM = mk(0., 0., 0.);
+  }
+  return add(delFN, delFT);
+}
+
+static int rolling_model(const model_t* m) {
+  return m->p[m->kind == ORC_MODEL_HOOKE_MEMORY ? 5 : 4] != 0.;
+}
+
+/* Component::getContactMemory (Component/src/Component.cpp:634-674) with
+   createContact = true: the entry of key in particle p's map (activated),
+   created as zeros when missing; *existed tells which. */
+static mem_t* get_memory(orc_sys* s, int p, int key, int* existed) {
+  mem_t* tab = s->mem + (size_t)p * ORC_MEM_CAP;
+  for (int k = 0; k < s->mem_n[p]; ++k)
+    if (tab[k].key == key) {
+      tab[k].active = 1;
+      *existed = 1;
+      return tab + k;
+    }
+  *existed = 0;
+  if (s->mem_n[p] >= ORC_MEM_CAP) return NULL;
+  mem_t* e = tab + s->mem_n[p]++;
+  memset(e, 0, sizeof(*e));
+  e->key = key;
+  e->active = 1;
+  return e;
+}
+
+/* Component::addDeplContactInMap (Component.cpp:680-689): sets (creating if
+   needed) particle p's entry of key, active. */
+static int set_memory(orc_sys* s, int p, int key, v3 kd, v3 nrm, v3 spr) {
+  int existed;
+  mem_t* e = get_memory(s, p, key, &existed);
+  if (!e) return -1;
+  st(e->kd, kd);
+  st(e->nrm, nrm);
+  st(e->spr, spr);
+  return 0;
+}
+
+int orc_get_memory(const orc_sys* s, int p, int cap, int32_t* key, double* kd, double* nrm,
+                   double* spr) {
+  if (p < 0 || p >= s->n) return -1;
+  int n = s->mem_n[p];
+  for (int k = 0; k < n && k < cap; ++k) {
+    const mem_t* e = s->mem + (size_t)p * ORC_MEM_CAP + k;
+    key[k] = e->key;
+    memcpy(kd + 3 * k, e->kd, sizeof(e->kd));
+    memcpy(nrm + 3 * k, e->nrm, sizeof(e->nrm));
+    memcpy(spr + 3 * k, e->spr, sizeof(e->spr));
+  }
+  return n;
+}
+
 static void log_contact(orc_sys* s, int id0, int id1, v3 pt, v3 ov, double d, v3 f) {
   if (s->log_n < s->log_cap) {
     int k = s->log_n;
@@ -560,17 +768,35 @@ static int interact(orc_sys* s, int p0, int p1, v3 shift) {
   const model_t* m = &s->models[s->mat[p0]][s->mat[p1]];
   if (m->kind == ORC_MODEL_NONE) return set_error(s, ORC_ERR_ARG, p0, p1, 0., 0.);
   v3 P = ld(pt);
-  v3 f = contact_force(m, P, ld(ov), dist, A, ld(s->v + 3 * p0), ld(s->w + 3 * p0),
-                       s->mass[p0], s->req[p0], B, ld(s->v + 3 * p1), ld(s->w + 3 * p1),
-                       s->mass[p1], s->req[p1]);
+  v3 f, delM = mk(0., 0., 0.);
+  int rolling = 0;
+  if (m->kind >= ORC_MODEL_HOOKE_MEMORY) {
+    /* HookeMemoryContactForceModel::computeForces (:200-249): body 0's map
+       entry (created if missing), then body 1's gets the negation */
+    int existed;
+    mem_t* e = get_memory(s, p0, p1, &existed);
+    if (!e) return set_error(s, ORC_ERR_ARG, p0, p1, 0., 0.);
+    f = memory_force(m, P, ld(ov), dist, A, ld(s->v + 3 * p0), ld(s->w + 3 * p0), s->mass[p0],
+                     s->req[p0], B, ld(s->v + 3 * p1), ld(s->w + 3 * p1), s->mass[p1],
+                     s->req[p1], s->dt_mem, existed, e->kd, e->nrm, e->spr, &delM);
+    if (set_memory(s, p1, p0, neg(ld(e->kd)), neg(ld(e->nrm)), neg(ld(e->spr))))
+      return set_error(s, ORC_ERR_ARG, p1, p0, 0., 0.);
+    rolling = rolling_model(m);
+  } else {
+    f = contact_force(m, P, ld(ov), dist, A, ld(s->v + 3 * p0), ld(s->w + 3 * p0),
+                      s->mass[p0], s->req[p0], B, ld(s->v + 3 * p1), ld(s->w + 3 * p1),
+                      s->mass[p1], s->req[p1]);
+  }
   /* coef = 1: body 0 gets coef*(delFN+delFT); body 1 gets coef*(-delFN-delFT),
      which is -(delFN+delFT) bit for bit */
   v3 f0 = scl(f, 1.);
   v3 f1 = neg(f);
   st(s->F + 3 * p0, add(ld(s->F + 3 * p0), f0));
   st(s->T + 3 * p0, add(ld(s->T + 3 * p0), cross(sub(P, A), f0)));
+  if (rolling) st(s->T + 3 * p0, add(ld(s->T + 3 * p0), scl(delM, 1.)));
   st(s->F + 3 * p1, add(ld(s->F + 3 * p1), f1));
   st(s->T + 3 * p1, add(ld(s->T + 3 * p1), cross(sub(P, B), f1)));
+  if (rolling) st(s->T + 3 * p1, add(ld(s->T + 3 * p1), scl(neg(delM), 1.)));
   log_contact(s, p0, p1, P, ld(ov), dist, f0);
   return ORC_OK;
 }
@@ -595,15 +821,60 @@ static int interact_wall(orc_sys* s, int p, int wi) {
   const model_t* m = &s->models[s->mat[p]][wl->material];
   if (m->kind == ORC_MODEL_NONE) return set_error(s, ORC_ERR_ARG, p, -1 - wi, 0., 0.);
   v3 zero = mk(0., 0., 0.);
-  v3 f = contact_force(m, pt, ov, dist, A, ld(s->v + 3 * p), ld(s->w + 3 * p), s->mass[p],
-                       s->req[p], ld(wl->origin), zero, zero, 1.e20, wl->req);
+  v3 f, delM = mk(0., 0., 0.);
+  int rolling = 0;
+  if (m->kind >= ORC_MODEL_HOOKE_MEMORY) {
+    /* the particle is body 0 (SimpleObstacle.cpp:262-264); its map keys the
+       wall by -1 - index (the obstacle's own copy is never read back) */
+    int existed;
+    mem_t* e = get_memory(s, p, -1 - wi, &existed);
+    if (!e) return set_error(s, ORC_ERR_ARG, p, -1 - wi, 0., 0.);
+    f = memory_force(m, pt, ov, dist, A, ld(s->v + 3 * p), ld(s->w + 3 * p), s->mass[p],
+                     s->req[p], ld(wl->origin), zero, zero, 1.e20, wl->req, s->dt_mem, existed,
+                     e->kd, e->nrm, e->spr, &delM);
+    rolling = rolling_model(m);
+  } else {
+    f = contact_force(m, pt, ov, dist, A, ld(s->v + 3 * p), ld(s->w + 3 * p), s->mass[p],
+                      s->req[p], ld(wl->origin), zero, zero, 1.e20, wl->req);
+  }
   st(s->F + 3 * p, add(ld(s->F + 3 * p), f));
   st(s->T + 3 * p, add(ld(s->T + 3 * p), cross(sub(pt, A), f)));
+  if (rolling) st(s->T + 3 * p, add(ld(s->T + 3 * p), scl(delM, 1.)));
   log_contact(s, p, -1 - wi, pt, ov, dist, f);
   return ORC_OK;
 }
 
-/* Grains::computeParticlesForceAndTorque (Base/src/Grains.cpp:459-482):
+/* Component::setContactMapToFalse / updateContactMap (Component.cpp:567-596),
+   over all particles (AllComponents.cpp:1777-1800) */
+static void memory_maps_inactive(orc_sys* s) {
+  for (int p = 0; p < s->n; ++p)
+    for (int k = 0; k < s->mem_n[p]; ++k) s->mem[(size_t)p * ORC_MEM_CAP + k].active = 0;
+}
+
+static void memory_maps_prune(orc_sys* s) {
+  for (int p = 0; p < s->n; ++p) {
+    mem_t* tab = s->mem + (size_t)p * ORC_MEM_CAP;
+    int w = 0;
+    for (int k = 0; k < s->mem_n[p]; ++k)
+      if (tab[k].active) tab[w++] = tab[k];
+    s->mem_n[p] = w;
+  }
+}
+
+static int compute_forces_body(orc_sys* s);
+
+/* Grains::computeParticlesForceAndTorque (Base/src/Grains.cpp:459-482): the
+   contact maps are marked inactive, forces computed, inactive entries erased. */
+int orc_compute_forces(orc_sys* s) {
+  if (s->has_memory && !(s->dt_mem > 0.)) return set_error(s, ORC_ERR_ARG, -1, -1, 0., 0.);
+  memory_maps_inactive(s);
+  int rc = compute_forces_body(s);
+  if (rc) return rc;
+  memory_maps_prune(s);
+  return ORC_OK;
+}
+
+/* Inside Grains::computeParticlesForceAndTorque (Base/src/Grains.cpp:459-482):
    AllComponents::InitializeForces (AllComponents.cpp:259-270) sets each torsor
    to the weight m (1 - rho_f / rho) g (Particle.cpp:414-419, rho_f = 0) at the
    centre; then LinkedCell::ComputeForces (LinkedCell.cpp:1019-1103): for each
@@ -611,7 +882,7 @@ static int interact_wall(orc_sys* s, int p, int wi) {
    its contact-neighbour cells in list order (the 13 cells were added with
    push_front, Cell.cpp:100-103, so they are visited in reverse stencil order),
    then the obstacles. */
-int orc_compute_forces(orc_sys* s) {
+static int compute_forces_body(orc_sys* s) {
   const double* g = s->cfg.gravity;
   s->log_n = 0;
   s->ncand = 0;
@@ -756,6 +1027,7 @@ int orc_prime(orc_sys* s) {
 
 /* Grains::Simulation kick-drift-kick body (Base/src/Grains.cpp:344-364) */
 int orc_step(orc_sys* s, int nsteps, double dt) {
+  s->dt_mem = dt;
   for (int k = 0; k < nsteps; ++k) {
     int rc = orc_move(s, 0.5 * dt, dt);
     if (rc) return rc;

@@ -41,6 +41,11 @@ extern "C" {
 #define ORC_MODEL_NONE 0
 #define ORC_MODEL_HOOKE 1
 #define ORC_MODEL_HERTZ 2
+#define ORC_MODEL_HOOKE_MEMORY 3
+#define ORC_MODEL_HERTZ_MEMORY 4
+
+/* Contacts a particle can remember at once (memory models). */
+#define ORC_MEM_CAP 32
 
 typedef struct orc_sys orc_sys;
 
@@ -65,9 +70,20 @@ int orc_set_particles(orc_sys* s, int n, const double* x, const double* v,
 int orc_add_wall(orc_sys* s, const double* basis, const double* centre,
                  const double* half_extent, double crust, int material);
 
-/* kind HOOKE: p = {kn, en, etat, muc, kr}; kind HERTZ: p = {Es, en, Gs, muc, kr} */
+/* kind HOOKE: p = {kn, en, etat, muc, kr}; kind HERTZ: p = {Es, en, Gs, muc, kr};
+   kind HOOKE_MEMORY: p = {kn, en, etat, muc, kt, mur, etarpf, epst};
+   kind HERTZ_MEMORY: p = {Es, en, Gs, muc, mur, etarpf, epst} */
 int orc_set_contact_model(orc_sys* s, int mat_a, int mat_b, int kind,
                           const double* p);
+/* dt of the force evaluations (memory models); orc_step sets it too. */
+int orc_set_time_step(orc_sys* s, double dt);
+/* Contact memory of particle p (its own map): up to cap entries of key
+   (partner id, or -1 - wall), kdelta, previous normal, spring torque. */
+int orc_get_memory(const orc_sys* s, int p, int cap, int32_t* key, double* kd,
+                   double* nrm, double* spr);
+/* Quaternion::setRotFromTwoVectors + rotateVector, for the pins. */
+void orc_rotate_two_vectors(const double* u, const double* v, const double* x,
+                            double* out);
 
 /* Phases of one leap-frog step, as in Grains::Simulation. */
 int orc_link(orc_sys* s);                    /* initial LinkedCell::Link */

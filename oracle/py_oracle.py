@@ -51,6 +51,9 @@ def load():
             "orc_set_particles": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
             "orc_add_wall": (i32, [vp, vp, vp, vp, d, i32]),
             "orc_set_contact_model": (i32, [vp, i32, i32, i32, vp]),
+            "orc_set_time_step": (i32, [vp, d]),
+            "orc_get_memory": (i32, [vp, i32, i32, vp, vp, vp, vp]),
+            "orc_rotate_two_vectors": (None, [vp, vp, vp, vp]),
             "orc_link": (i32, [vp]),
             "orc_link_update": (i32, [vp]),
             "orc_compute_forces": (i32, [vp]),
@@ -103,7 +106,10 @@ class Oracle:
             raise OracleError(rc, "orc_create", -1, -1)
         self.h = h
         for (a, b, kind, p) in wl["models"]:
-            self._chk(self.lib.orc_set_contact_model(h, a, b, kind, _p(_f(p, (5,)))), "model")
+            npar = {3: 8, 4: 7}.get(int(kind), 5)
+            self._chk(self.lib.orc_set_contact_model(h, a, b, kind, _p(_f(p, (npar,)))), "model")
+        if "dt" in wl:
+            self._chk(self.lib.orc_set_time_step(h, float(wl["dt"])), "time step")
         for (basis, centre, half, crust, mat) in wl["walls"]:
             self._chk(self.lib.orc_add_wall(h, _p(_f(basis, (9,))), _p(_f(centre, (3,))),
                                             _p(_f(half, (3,))), crust, mat), "wall")
@@ -150,6 +156,16 @@ class Oracle:
     def advance_velocity(self, dtv):
         self._chk(self.lib.orc_advance_velocity(self.h, dtv), "advance_velocity")
 
+    def set_time_step(self, dt):
+        self._chk(self.lib.orc_set_time_step(self.h, float(dt)), "time step")
+
+    def memory(self, p, cap=64):
+        """Contact map of particle p: {key: (kdelta, normal, spring torque)}."""
+        key = np.empty(cap, np.int32)
+        kd, nrm, spr = (np.empty((cap, 3)) for _ in range(3))
+        n = self.lib.orc_get_memory(self.h, int(p), cap, _p(key), _p(kd), _p(nrm), _p(spr))
+        return {int(key[k]): (kd[k].copy(), nrm[k].copy(), spr[k].copy()) for k in range(min(n, cap))}
+
     def state(self):
         n = self.n
         out = {k: np.empty((n, 3)) for k in ("x", "v", "w", "F", "T")}
@@ -185,6 +201,13 @@ class Oracle:
 
     def candidates(self):
         return self.lib.orc_candidates(self.h)
+
+
+def rotate_two_vectors(u, v, x):
+    """Quaternion::setRotFromTwoVectors(u, v) then rotateVector(x)."""
+    out = np.empty(3)
+    load().orc_rotate_two_vectors(_p(_f(u, (3,))), _p(_f(v, (3,))), _p(_f(x, (3,))), _p(out))
+    return out
 
 
 def sphere_sphere(a, ra, cra, b, rb, crb):

@@ -439,6 +439,7 @@ extern "C" int gdem_batch_add_system(gdem_batch* b, const gdem_batch_system* in,
     return GDEM_ERR_ARG;
   if (in->nwalls < 0 || in->nwalls > kMaxWalls) return GDEM_ERR_ARG;
   if (in->nwalls > 0 && (!in->wall_basis || !in->wall_centre || !in->wall_half)) return GDEM_ERR_ARG;
+  if (in->kind != GDEM_MODEL_HOOKE && in->kind != GDEM_MODEL_HERTZ) return GDEM_ERR_ARG;
   if (!model_params_ok(in->kind, in->pp) || !model_params_ok(in->kind, in->pw)) return GDEM_ERR_ARG;
   BatchSys sy{};
   if (!make_grid(in->cfg, sy.grid)) return GDEM_ERR_ARG;

@@ -84,6 +84,11 @@ struct gdem_ctx {
   int32_t* d_skin = nullptr;
   double4* d_xref = nullptr;
   unsigned long long* d_cmask = nullptr;
+  // history (memory) contact models: per-particle memory tables, one per
+  // state copy; allocated once a memory model is set and particles exist
+  bool has_memory = false;
+  History hist[2]{};
+  double dt_mem = 0.;
   // captured fused step (one graph per state-buffer parity)
   bool use_graph = true;
   int64_t graph_gen = 0;
@@ -188,10 +193,13 @@ extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
   return GDEM_OK;
 }
 
+static void free_history(gdem_ctx* c);
+
 extern "C" int gdem_destroy(gdem_ctx* c) {
   if (!c) return GDEM_OK;
   (void)hipSetDevice(c->dev);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_history(c);
   free_state(c->buf[0]);
   free_state(c->buf[1]);
   (void)hipFree(c->d_F); (void)hipFree(c->d_T);
@@ -214,6 +222,31 @@ extern "C" int gdem_destroy(gdem_ctx* c) {
   return GDEM_OK;
 }
 
+static void free_history(gdem_ctx* c) {
+  for (History& h : c->hist) {
+    (void)hipFree(h.count); (void)hipFree(h.key); (void)hipFree(h.kd); (void)hipFree(h.nrm);
+    (void)hipFree(h.spr);
+    h = History{};
+  }
+}
+
+// Memory tables of the history models (both copies), empty: no contact has
+// a memory yet.
+static int ensure_history(gdem_ctx* c) {
+  if (!c->has_memory || c->convex || c->n <= 0 || c->hist[0].count) return GDEM_OK;
+  const size_t nn = (size_t)c->n, ne = nn * kHistCap;
+  for (History& h : c->hist) {
+    HIP_TRY(hipMalloc(&h.count, sizeof(int32_t) * nn));
+    HIP_TRY(hipMalloc(&h.key, sizeof(int32_t) * ne));
+    HIP_TRY(hipMalloc(&h.kd, sizeof(double4) * ne));
+    HIP_TRY(hipMalloc(&h.nrm, sizeof(double4) * ne));
+    HIP_TRY(hipMalloc(&h.spr, sizeof(double4) * ne));
+    HIP_TRY(hipMemset(h.count, 0, sizeof(int32_t) * nn));
+  }
+  c->graph_gen++;
+  return GDEM_OK;
+}
+
 // Frees the previous particle set and allocates the buffers of n particles
 // (both state copies, torsors, binning, contact lists, recording).
 static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
@@ -230,6 +263,7 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
   (void)hipFree(c->d_cand); (void)hipFree(c->d_crec); (void)hipFree(c->d_mask); (void)hipFree(c->d_work);
   (void)hipFree(c->d_nwork);
   (void)hipFree(c->d_skin); (void)hipFree(c->d_xref); (void)hipFree(c->d_cmask);
+  free_history(c);
   c->d_skin = nullptr;
   c->d_xref = nullptr;
   c->d_cmask = nullptr;
@@ -288,7 +322,7 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
   c->primed = false;
   c->binned = false;
   c->rec_valid = false;
-  return GDEM_OK;
+  return ensure_history(c);
 }
 
 // Effective skin of the candidate lists: the requested one (automatic: 0.2
@@ -506,6 +540,7 @@ extern "C" int gdem_set_convex_particles(gdem_ctx* c, int64_t n, const double* x
   if (!c || n < 0 || n > 0x7fffffff) return GDEM_ERR_ARG;
   if (n > 0 && (!x || !v || !w || !q || !basis || !shape || !density || !crust))
     return GDEM_ERR_ARG;
+  if (c->has_memory) return GDEM_ERR_ARG;  // memory models: spheres only
   for (int64_t i = 0; i < n; ++i) {
     if (shape[i] < 0 || shape[i] >= c->nshapes) return GDEM_ERR_ARG;
     if (!(density[i] > 0.) || !(crust[i] > 0.)) return GDEM_ERR_ARG;
@@ -616,13 +651,25 @@ extern "C" int gdem_set_contact_model(gdem_ctx* c, int32_t a, int32_t b, int32_t
                                       const double* p) {
   if (!c || !p || a < 0 || b < 0 || a >= kMaxMaterials || b >= kMaxMaterials) return GDEM_ERR_ARG;
   if (!model_params_ok(kind, p)) return GDEM_ERR_ARG;
+  if (model_has_memory(kind) && c->convex) return GDEM_ERR_ARG;  // spheres only
   const Model m = make_model(kind, p);
   c->h_models[a * kMaxMaterials + b] = m;
   c->h_models[b * kMaxMaterials + a] = m;
+  bool mem = false;
+  for (const Model& md : c->h_models) mem = mem || model_has_memory(md.kind);
+  if (mem != c->has_memory) c->graph_gen++;  // the force kernel variant changes
+  c->has_memory = mem;
   HIP_TRY(hipSetDevice(c->dev));
   HIP_TRY(hipMemcpyAsync(c->d_models, c->h_models, sizeof(c->h_models), hipMemcpyHostToDevice,
                          c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return ensure_history(c);
+}
+
+extern "C" int gdem_set_time_step(gdem_ctx* c, double dt) {
+  if (!c || !(dt > 0.)) return GDEM_ERR_ARG;
+  if (dt != c->dt_mem) c->graph_gen++;
+  c->dt_mem = dt;
   return GDEM_OK;
 }
 
@@ -801,6 +848,12 @@ static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wr
       a.cmask = c->d_cmask;
       a.half_skin2 = 0.25 * c->skin * c->skin;
     }
+    if (c->has_memory) {
+      if (!(c->dt_mem > 0.) || !c->hist[0].count) return GDEM_ERR_ARG;
+      a.hin = c->hist[c->cur];
+      a.hout = c->hist[1 - c->cur];
+      a.dt_mem = c->dt_mem;
+    }
     if (skin)
       launch_detect_screen(d, a, c->stream);
     else
@@ -808,7 +861,8 @@ static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wr
     // timing: the events take the force kernel's own start and end times
     // (an event recorded between two launches would add the dispatch gap)
     if (c->timing) ev = next_event_pair(c);
-    launch_force(mode, record, a, c->stream, ev ? ev->first : nullptr, ev ? ev->second : nullptr);
+    launch_force(mode, record, a, c->stream, ev ? ev->first : nullptr, ev ? ev->second : nullptr,
+                 c->has_memory);
     HIP_TRY(hipGetLastError());
     *evp = ev;
     return GDEM_OK;
@@ -884,6 +938,10 @@ extern "C" int gdem_compute_forces(gdem_ctx* c) {
 extern "C" int gdem_move(gdem_ctx* c, double dtv, double dtd) {
   if (!c) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
+  if (dtd > 0. && dtd != c->dt_mem) {
+    c->dt_mem = dtd;  // the step of the next force evaluation
+    c->graph_gen++;
+  }
   if (c->convex)
     launch_cmove(c->buf[c->cur], c->cbuf[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid,
                  c->d_row_rank, dtv, dtd, c->bins.key, c->d_err, c->stream);
@@ -926,7 +984,8 @@ extern "C" int gdem_step(gdem_ctx* c, int32_t nsteps, double dt) {
   if (!c || nsteps < 0 || !(dt > 0.)) return GDEM_ERR_ARG;
   if (nsteps == 0) return GDEM_OK;
   HIP_TRY(hipSetDevice(c->dev));
-  int rc;
+  int rc = gdem_set_time_step(c, dt);
+  if (rc) return rc;
   if (!c->primed) {
     rc = gdem_prime(c);
     if (rc) return rc;

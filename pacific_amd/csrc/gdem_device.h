@@ -78,12 +78,20 @@ __host__ __device__ __forceinline__ quat qmul(quat p, quat r) {
 
 // Contact model parameters (one entry per unordered material pair).
 struct Model {
-  int32_t kind;  // 0 none, 1 Hooke, 2 Hertz
+  int32_t kind;  // 0 none, 1 Hooke, 2 Hertz, 3 HookeMemory, 4 HertzMemory
   int32_t pad;
-  double p[5];     // Hooke {kn, en, etat, muc, kr}; Hertz {Es, en, Gs, muc, kr}
+  // Hooke {kn, en, etat, muc, kr}; Hertz {Es, en, Gs, muc, kr};
+  // HookeMemory {kn, en, etat, muc, kt, mur, etarpf, epst};
+  // HertzMemory {Es, en, Gs, muc, mur, etarpf, epst}
+  double p[8];
   double beta;     // HookeContactForceModel.cpp:19
   double m2sqrt56; // HertzContactForceModel.cpp:21
 };
+
+// Number of parameters of a contact model kind (0 if unknown).
+__host__ __device__ __forceinline__ int model_nparams(int32_t kind) {
+  return kind == 1 || kind == 2 ? 5 : kind == 3 ? 8 : kind == 4 ? 7 : 0;
+}
 
 // Static box obstacle.
 struct Wall {
@@ -146,6 +154,139 @@ __host__ __device__ __forceinline__ v3 contact_force(const Model& m, v3 pt, v3 p
     v3 tangent = mk(0., 0., 0.);
     if (nvt > kEps) tangent = dvd(neg(v_t), nvt);
     delFT = scl(tangent, fn);
+  }
+  return add(delFN, delFT);
+}
+
+// ---------------------------------------------------------------- memory models
+// Quaternion::setRotFromTwoVectors (Geometry/src/Quaternion.cpp:263-286): the
+// unit quaternion (qv, qw) turning u onto v; a half turn about an axis normal
+// to u when they are opposite.
+__host__ __device__ __forceinline__ void rot_two_vectors(v3 u, v3 v, v3& qv, double& qw) {
+  const double nn = sqrt(dot(u, u) * dot(v, v));
+  double real = nn + dot(u, v);
+  v3 vect;
+  if (real < 1.e-6 * nn) {
+    real = 0.;
+    vect = fabs(u.x) > fabs(u.z) ? mk(-u.y, u.x, 0.) : mk(0., -u.z, u.y);
+  } else {
+    vect = cross(u, v);
+  }
+  // qq * (1 / Norm(qq)), Norm as Quaternion.cpp:558-562
+  const double d = 1. / sqrt(vect.x * vect.x + vect.y * vect.y + vect.z * vect.z + real * real);
+  qv = scl(vect, d);
+  qw = d * real;
+}
+
+// Quaternion::rotateVector (Quaternion.cpp:293-298)
+__host__ __device__ __forceinline__ v3 rotate_by(v3 qv, double qw, v3 v) {
+  const double nq = norm(qv);
+  const double a = qw * qw - nq * nq;
+  const double b = 2. * dot(v, qv);
+  const double c = 2. * qw;
+  return add(add(scl(v, a), scl(qv, b)), scl(cross(qv, v), c));
+}
+
+// computeTangentialVector of the memory models (HookeMemoryContactForceModel.cpp:54-65,
+// HertzMemoryContactForceModel.cpp:55-66)
+__host__ __device__ __forceinline__ v3 tangent_vector(double n_t, v3 ut, v3 kd, double epst) {
+  if (norm(ut) > 0.001) return dvd(neg(ut), norm(ut));
+  const v3 nut = scl(ut, n_t);
+  const v3 s = add(kd, nut);
+  if ((norm(kd) > epst || norm(nut) > epst) && norm(s) > epst) return dvd(neg(s), norm(s));
+  return mk(0., 0., 0.);
+}
+
+// Memory of one contact seen from its first body: cumulative tangential
+// spring (kdelta), contact normal of the previous evaluation, rolling-friction
+// spring torque (Component.hh:505-508).  The second body keeps the negation.
+struct ContactMemory {
+  v3 kd, nrm, spr;
+};
+
+// performForcesCalculus of HookeMemoryContactForceModel (HookeMemoryContactForceModel.cpp:72-193)
+// and HertzMemoryContactForceModel (HertzMemoryContactForceModel.cpp:73-195): the
+// force on body 0 (delFN + delFT) is returned, the rolling torque on body 0 in
+// delM; mem (body 0's view, found = the contact existed) is updated in place.
+__host__ __device__ __forceinline__ v3 memory_contact_force(const Model& m, v3 pt, v3 pen,
+                                                   double odist, v3 x0, v3 v0, v3 w0, double m0,
+                                                   double req0, v3 x1, v3 v1, v3 w1, double m1,
+                                                   double req1, double dt, bool found,
+                                                   ContactMemory& mem, v3& delM) {
+  const bool hooke = m.kind == 3;
+  v3 normal = dvd(pen, norm(pen));
+  normal = round3(normal);
+  const v3 tmpV = sub(vel_at(x0, v0, w0, pt), vel_at(x1, v1, w1, pt));
+  const v3 v_n = scl(normal, dot(tmpV, normal));
+  const v3 v_t = sub(tmpV, v_n);
+  const double muc = m.p[3];
+  v3 delFN;
+  double normFN, gamman, gammat, kn, kt, mur, etarpf, epst;
+  if (hooke) {
+    kn = m.p[0];
+    kt = m.p[4];
+    mur = m.p[5];
+    etarpf = m.p[6];
+    epst = m.p[7];
+    delFN = scl(pen, kn);
+    const double avmass = 1. / (1. / m0 + 1. / m1);
+    gamman = -2. * m.beta * sqrt(avmass * kn);
+    delFN = sub(delFN, scl(v_n, gamman));
+    normFN = norm(delFN);
+    const double etat = m.p[2] == -1. ? -m.beta * sqrt(kn / avmass) : m.p[2];
+    gammat = 2. * etat * avmass;
+  } else {
+    const double Es = m.p[0], Gs = m.p[2];
+    mur = m.p[4];
+    etarpf = m.p[5];
+    epst = m.p[6];
+    const double deltan = -odist;
+    const double Req = 1. / (1. / req0 + 1. / req1);
+    const double avmass = 1. / (1. / m0 + 1. / m1);
+    const double sq = sqrt(Req * deltan);
+    const double Sn = 2. * Es * sq;
+    const double St = 8. * Gs * sq;
+    kn = (4. / 3.) * Es * sq;
+    delFN = scl(pen, kn);
+    gamman = m.m2sqrt56 * m.beta * sqrt(avmass * Sn);
+    delFN = sub(delFN, scl(v_n, gamman));
+    normFN = norm(delFN);
+    gammat = m.m2sqrt56 * m.beta * sqrt(avmass * St);
+    kt = 8. * Gs * sq;
+  }
+  const v3 viscousFT = scl(v_t, -gammat);
+  // rotate the stored spring into the current contact plane
+  v3 qv = mk(0., 0., 0.);
+  double qw = 0.;
+  if (found) {
+    rot_two_vectors(mem.nrm, normal, qv, qw);
+    mem.kd = rotate_by(qv, qw, mem.kd);
+  }
+  mem.kd = add(mem.kd, scl(v_t, dt * kt));
+  mem.nrm = normal;
+  const v3 tij = tangent_vector(gammat, v_t, mem.kd, epst);
+  const double normFT = norm(add(neg(mem.kd), viscousFT));
+  v3 delFT;
+  if (normFT <= muc * normFN) {
+    delFT = scl(tij, norm(add(neg(mem.kd), viscousFT)));
+  } else {
+    // Coulomb limit: the spring is reset to what the limit force needs
+    if (!hooke || kt != 0.) mem.kd = dvd(add(scl(tij, -muc * normFN), viscousFT), kt);
+    delFT = scl(tij, muc * normFN);
+  }
+  if (mur != 0.) {
+    const double Req = 1. / (1. / req0 + 1. / req1);
+    const double kr = 3. * kn * mur * mur * Req * Req;
+    const double etar = 3. * gamman * mur * mur * Req * Req;
+    const double max_normMk = mur * Req * normFN;
+    const v3 wrel = sub(w0, w1);
+    if (found) mem.spr = rotate_by(qv, qw, mem.spr);
+    mem.spr = sub(mem.spr, scl(wrel, kr * dt));
+    const double normMk = norm(mem.spr);
+    if (normMk > max_normMk) mem.spr = scl(mem.spr, max_normMk / normMk);
+    delM = sub(mem.spr, scl(wrel, etarpf * etar));
+  } else {
+    delM = mk(0., 0., 0.);
   }
   return add(delFN, delFT);
 }

@@ -119,16 +119,26 @@ inline Wall make_wall(const double* basis, const double* centre, const double* h
 // Hooke / Hertz parameters the step supports: en in (0, 1], no rolling
 // resistance (kr = 0).
 inline bool model_params_ok(int32_t kind, const double* p) {
-  if (kind != GDEM_MODEL_HOOKE && kind != GDEM_MODEL_HERTZ) return false;
-  if (p[4] != 0.) return false;
-  return p[1] > 0. && p[1] <= 1.;
+  if (kind == GDEM_MODEL_HOOKE || kind == GDEM_MODEL_HERTZ) {
+    if (p[4] != 0.) return false;
+    return p[1] > 0. && p[1] <= 1.;
+  }
+  if (kind == GDEM_MODEL_HOOKE_MEMORY) return p[1] > 0. && p[1] <= 1. && p[4] >= 0. && p[5] >= 0.;
+  if (kind == GDEM_MODEL_HERTZ_MEMORY) return p[1] > 0. && p[1] <= 1. && p[0] > 0. && p[4] >= 0.;
+  return false;
 }
 
-// Model constants (HookeContactForceModel.cpp:19, HertzContactForceModel.cpp:20-21).
+inline bool model_has_memory(int32_t kind) {
+  return kind == GDEM_MODEL_HOOKE_MEMORY || kind == GDEM_MODEL_HERTZ_MEMORY;
+}
+
+// Model constants (HookeContactForceModel.cpp:19, HertzContactForceModel.cpp:20-21,
+// HookeMemoryContactForceModel.cpp:28, HertzMemoryContactForceModel.cpp:29-30).
+// p holds model_nparams(kind) values.
 inline Model make_model(int32_t kind, const double* p) {
   Model m{};
   m.kind = kind;
-  memcpy(m.p, p, sizeof(m.p));
+  memcpy(m.p, p, sizeof(double) * model_nparams(kind));
   m.beta = log(p[1]) / sqrt(kPi * kPi + log(p[1]) * log(p[1]));
   m.m2sqrt56 = -2. * sqrt(5. / 6.);
   return m;

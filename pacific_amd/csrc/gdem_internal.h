@@ -87,6 +87,19 @@ struct DetectArgs {
 // for the next one, so no kernel of an evaluation writes a word it reads.
 enum { kSkinNeed0 = 0, kSkinNeed1 = 1, kSkinRebuilds = 2, kSkinWords = 3 };
 
+// Contact memory of the history models (HookeMemory / HertzMemory): for each
+// particle (slot order, travelling with the state copies) up to kHistCap
+// entries, entry-major like the lists: key (partner id, or -1 - wall) and the
+// particle's own view of the contact's memory.
+constexpr int kHistCap = 16;
+struct History {
+  int32_t* count;  // [n]
+  int32_t* key;    // [kHistCap][lstride]
+  double4* kd;     // tangential spring (x, y, z, -)
+  double4* nrm;    // previous contact normal
+  double4* spr;    // rolling-friction spring torque
+};
+
 struct ForceArgs {
   State in, out;
   const int32_t* perm;
@@ -118,7 +131,10 @@ struct ForceArgs {
   const int32_t* skin_did;  // need word of this evaluation: lists rebuilt
   int32_t* skin_need;       // need word of the next evaluation
   double4* xref;        // positions at the last rebuild, slot order
-  unsigned long long* cmask;  // screening masks (k_screen), slot order
+  unsigned long long* cmask;  // screening masks (k_detect_screen), slot order
+  // history models: memory read at the input slot, written at the output slot
+  History hin, hout;
+  double dt_mem;        // time step of the memory update
   double half_skin2;    // (skin / 2)^2
 };
 
@@ -249,9 +265,10 @@ void launch_binning(Bins& b, const State& s, int32_t n, int32_t ncell, hipStream
                     const int32_t* gate = nullptr, int32_t* in_count = nullptr,
                     int32_t* clear_need = nullptr, int32_t* rebuilds = nullptr);
 void launch_detect(const DetectArgs& a, hipStream_t st);
-// With events, they record the kernel's own start and end (timing).
+// With events, they record the kernel's own start and end (timing).  memory:
+// the history-model variant (a.hin / a.hout used).
 void launch_force(int mode, bool record, const ForceArgs& a, hipStream_t st,
-                  hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+                  hipEvent_t start = nullptr, hipEvent_t stop = nullptr, bool memory = false);
 // Skin lists: candidate sweep (d.gate set) or screening of the lists (writes
 // f.cmask), in one launch.
 void launch_detect_screen(const DetectArgs& d, const ForceArgs& f, hipStream_t st);

@@ -206,12 +206,12 @@ __global__ void __launch_bounds__(kBlock) k_detect_screen(DetectArgs d, ForceArg
 }
 
 // ---------------------------------------------------------------- force
-template <int MODE, bool RECORD>
+template <int MODE, bool RECORD, bool MEM>
 __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a) {
   __shared__ unsigned long long sh_cnt[kBlock / 64];
   const int t = logical_index(a.nblk8);
   int ncont = 0;
-  if (t < a.n) ncont = force_particle<MODE, RECORD>(a, t);
+  if (t < a.n) ncont = force_particle<MODE, RECORD, MEM>(a, t);
   // contact counter: wave sum, then one integer atomic per workgroup
   unsigned long long c = (unsigned long long)ncont;
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
@@ -309,27 +309,33 @@ void launch_detect_screen(const DetectArgs& d, const ForceArgs& f, hipStream_t s
   hipLaunchKernelGGL(k_detect_screen, dim3(d.nblk8), dim3(kBlock), 0, st, d, f);
 }
 
+template <int M, bool R, bool MEM>
+static void force_launch(const ForceArgs& a, hipStream_t st, hipEvent_t start, hipEvent_t stop) {
+  const dim3 grid(a.nblk8), block(kBlock);
+  if (start)
+    hipExtLaunchKernelGGL((k_force<M, R, MEM>), grid, block, 0, st, start, stop, 0, a);
+  else
+    hipLaunchKernelGGL((k_force<M, R, MEM>), grid, block, 0, st, a);
+}
+
+template <bool R, bool MEM>
+static void force_launch_mode(int mode, const ForceArgs& a, hipStream_t st, hipEvent_t start,
+                              hipEvent_t stop) {
+  if (mode == MODE_FORCE) force_launch<MODE_FORCE, R, MEM>(a, st, start, stop);
+  else if (mode == MODE_KICK) force_launch<MODE_KICK, R, MEM>(a, st, start, stop);
+  else force_launch<MODE_MOVE, R, MEM>(a, st, start, stop);
+}
+
 void launch_force(int mode, bool record, const ForceArgs& a, hipStream_t st, hipEvent_t start,
-                  hipEvent_t stop) {
+                  hipEvent_t stop, bool memory) {
   if (a.n <= 0) return;
-  dim3 grid(a.nblk8), block(kBlock);
-#define GDEM_LAUNCH_FORCE(M, R)                                                               \
-  do {                                                                                        \
-    if (start)                                                                                \
-      hipExtLaunchKernelGGL((k_force<M, R>), grid, block, 0, st, start, stop, 0, a);          \
-    else                                                                                      \
-      hipLaunchKernelGGL((k_force<M, R>), grid, block, 0, st, a);                             \
-  } while (0)
-  if (record) {
-    if (mode == MODE_FORCE) GDEM_LAUNCH_FORCE(MODE_FORCE, true);
-    else if (mode == MODE_KICK) GDEM_LAUNCH_FORCE(MODE_KICK, true);
-    else GDEM_LAUNCH_FORCE(MODE_MOVE, true);
+  if (memory) {
+    if (record) force_launch_mode<true, true>(mode, a, st, start, stop);
+    else force_launch_mode<false, true>(mode, a, st, start, stop);
   } else {
-    if (mode == MODE_FORCE) GDEM_LAUNCH_FORCE(MODE_FORCE, false);
-    else if (mode == MODE_KICK) GDEM_LAUNCH_FORCE(MODE_KICK, false);
-    else GDEM_LAUNCH_FORCE(MODE_MOVE, false);
+    if (record) force_launch_mode<true, false>(mode, a, st, start, stop);
+    else force_launch_mode<false, false>(mode, a, st, start, stop);
   }
-#undef GDEM_LAUNCH_FORCE
 }
 
 void launch_move(const State& s, const double4* F, const double4* T, int32_t n, const Grid& g,

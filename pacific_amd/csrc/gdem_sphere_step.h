@@ -183,7 +183,7 @@ __device__ __forceinline__ void screen_particle(const ForceArgs& a, int t) {
   a.cmask[t] = (unsigned long long)om | ((unsigned long long)im << 32);
 }
 
-template <bool RECORD>
+template <bool RECORD, bool MEM = false>
 struct Reducer {
   const ForceArgs& a;
   int t;
@@ -194,6 +194,62 @@ struct Reducer {
   int ncont, nrec;
   bool cand;      // list entries are skin candidates: test each for contact
   bool via_perm;  // the input copy is in the previous order (read through perm)
+  int s_in;       // input slot of this particle (its memory is read there)
+  int nhist;      // memory entries written for this evaluation
+
+  // The memory of the contact with key in this particle's input table
+  // (Component::getContactMemory, Component.cpp:634-674, without creating).
+  __device__ __forceinline__ bool find_memory(int key, ContactMemory& mem) const {
+    const int nh = min(a.hin.count[s_in], kHistCap);
+    for (int k = 0; k < nh; ++k) {
+      const size_t at = (size_t)k * a.lstride + s_in;
+      if (a.hin.key[at] == key) {
+        const double4 kd = a.hin.kd[at], nr = a.hin.nrm[at], sp = a.hin.spr[at];
+        mem.kd = mk(kd.x, kd.y, kd.z);
+        mem.nrm = mk(nr.x, nr.y, nr.z);
+        mem.spr = mk(sp.x, sp.y, sp.z);
+        return true;
+      }
+    }
+    return false;
+  }
+
+  // Records the contact's memory for the next evaluation in this particle's
+  // output table: only contacts of this evaluation are kept, as the reference
+  // erases the entries no contact touched (Component::updateContactMap,
+  // Component.cpp:583-596).
+  __device__ __forceinline__ void store_memory(int key, const ContactMemory& mem) {
+    if (nhist < kHistCap) {
+      const size_t at = (size_t)nhist * a.lstride + t;
+      a.hout.key[at] = key;
+      a.hout.kd[at] = make_double4(mem.kd.x, mem.kd.y, mem.kd.z, 0.);
+      a.hout.nrm[at] = make_double4(mem.nrm.x, mem.nrm.y, mem.nrm.z, 0.);
+      a.hout.spr[at] = make_double4(mem.spr.x, mem.spr.y, mem.spr.z, 0.);
+    } else {
+      atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax.id);
+    }
+    ++nhist;
+  }
+
+  // Memory-model force of a pair or wall contact: both partners evaluate it
+  // from the first body's view (the second body holds the negated memory,
+  // Component::addDeplContactInMap, Component.cpp:680-689), so both get the
+  // same force, rolling torque and memory bit for bit.
+  __device__ __forceinline__ v3 memory_force(const Model& md, int key, bool first, v3 c, v3 ov,
+                                             double dist, v3 A, v3 VA, v3 WA, double mA,
+                                             double reqA, v3 B, v3 VB, v3 WB, double mB,
+                                             double reqB, v3& delM) {
+    ContactMemory mem{mk(0., 0., 0.), mk(0., 0., 0.), mk(0., 0., 0.)};
+    const bool found = find_memory(key, mem);
+    if (found && !first) mem = ContactMemory{neg(mem.kd), neg(mem.nrm), neg(mem.spr)};
+    const v3 f = memory_contact_force(md, c, ov, dist, A, VA, WA, mA, reqA, B, VB, WB, mB, reqB,
+                                      a.dt_mem, found, mem, delM);
+    if (first)
+      store_memory(key, mem);
+    else
+      store_memory(key, ContactMemory{neg(mem.kd), neg(mem.nrm), neg(mem.spr)});
+    return f;
+  }
 
   // One contact of this particle's segment: partner slot p, code (shift to the
   // partner's image, first-body flag).  The pair is evaluated with A = first
@@ -239,19 +295,33 @@ struct Reducer {
     }
     const Model md = a.models[(first ? ax.mat : ax2.mat) * a.model_stride + (first ? ax2.mat : ax.mat)];
     v3 f;
-    if (first)
+    v3 delM = mk(0., 0., 0.);
+    bool rolling = false;
+    if (MEM && md.kind >= 3) {
+      // memory key: the partner's id (Component.cpp:634, idmap (1, id, 1))
+      if (first)
+        f = memory_force(md, ax2.id, true, c, ov, dist, A, V, W, m, req, B, V2, W2, vm2.w, wr2.w,
+                         delM);
+      else
+        f = memory_force(md, ax2.id, false, c, ov, dist, A, V2, W2, vm2.w, wr2.w, B, V, W, m,
+                         req, delM);
+      rolling = md.p[md.kind == 3 ? 5 : 4] != 0.;
+    } else if (first) {
       f = contact_force(md, c, ov, dist, A, V, W, m, req, B, V2, W2, vm2.w, wr2.w);
-    else
+    } else {
       f = contact_force(md, c, ov, dist, A, V2, W2, vm2.w, wr2.w, B, V, W, m, req);
+    }
     if (first) {
       F = add(F, f);
       T = add(T, cross(sub(c, A), f));
+      if (rolling) T = add(T, delM);  // addTorque(delM) (HookeMemoryContactForceModel.cpp:236)
       ++ncont;
       if (RECORD) record(ax.id, ax2.id, c, ov, dist, f);
     } else {
       const v3 fn = neg(f);
       F = add(F, fn);
       T = add(T, cross(sub(c, B), fn));
+      if (rolling) T = add(T, neg(delM));  // addTorque(-delM) (:241)
     }
   }
 
@@ -272,7 +342,7 @@ struct Reducer {
 
 // Force, torque and integrator tail of one particle (storage slot t of the
 // sorted order).  Returns the number of contacts it is the first body of.
-template <int MODE, bool RECORD>
+template <int MODE, bool RECORD, bool MEM = false>
 __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
   // with skin lists the state moves through perm only after a rebuild;
   // otherwise it stays in its slot
@@ -287,7 +357,9 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
   // reference position of the skin lists, loaded with the state
   double4 xr4 = make_double4(0., 0., 0., 0.);
   if (skin && !rebuilt) xr4 = a.xref[t];
-  Reducer<RECORD> R{a, t};
+  Reducer<RECORD, MEM> R{a, t};
+  R.s_in = s;
+  R.nhist = 0;
   R.ax = a.in.aux[s];
   R.X = mk(pr.x, pr.y, pr.z);
   R.V = mk(vm.x, vm.y, vm.z);
@@ -349,15 +421,27 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
     if (rc == 0 || !(dist < 0.)) continue;
     const Model md = a.models[R.ax.mat * a.model_stride + wl.material];
     const v3 zero = mk(0., 0., 0.);
-    const v3 f = contact_force(md, c, ov, dist, R.X, R.V, R.W, R.m, R.req,
-                               mk(wl.origin[0], wl.origin[1], wl.origin[2]), zero, zero, 1.e20,
-                               wl.req);
-    R.F = add(R.F, f);
-    R.T = add(R.T, cross(sub(c, R.X), f));
+    const v3 wo = mk(wl.origin[0], wl.origin[1], wl.origin[2]);
+    v3 f;
+    if (MEM && md.kind >= 3) {
+      // the particle is the first body (SimpleObstacle.cpp:262-264); the
+      // wall's own copy of the memory is never read back
+      v3 delM;
+      f = R.memory_force(md, -1 - wi, true, c, ov, dist, R.X, R.V, R.W, R.m, R.req, wo, zero,
+                         zero, 1.e20, wl.req, delM);
+      R.F = add(R.F, f);
+      R.T = add(R.T, cross(sub(c, R.X), f));
+      if (md.p[md.kind == 3 ? 5 : 4] != 0.) R.T = add(R.T, delM);
+    } else {
+      f = contact_force(md, c, ov, dist, R.X, R.V, R.W, R.m, R.req, wo, zero, zero, 1.e20, wl.req);
+      R.F = add(R.F, f);
+      R.T = add(R.T, cross(sub(c, R.X), f));
+    }
     ++R.ncont;
     if (RECORD) R.record(R.ax.id, -1 - wi, c, ov, dist, f);
   }
   if (RECORD) a.rec_count[t] = R.nrec;
+  if (MEM) a.hout.count[t] = min(R.nhist, kHistCap);
   const v3 F = R.F, T = R.T, X = R.X, V = R.V, W = R.W;
   const double m = R.m;
 

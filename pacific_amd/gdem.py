@@ -21,6 +21,9 @@ LIB_PATH = os.path.join(_HERE, "lib", "libgdem.so")
 
 OK, ERR_CONTACT, ERR_MOTION, ERR_SIMULATION, ERR_HIP, ERR_ARG, ERR_CAPACITY = range(7)
 MODEL_HOOKE, MODEL_HERTZ = 1, 2
+MODEL_HOOKE_MEMORY, MODEL_HERTZ_MEMORY = 3, 4
+# parameters per model kind (include/gdem.h)
+MODEL_NPARAMS = {MODEL_HOOKE: 5, MODEL_HERTZ: 5, MODEL_HOOKE_MEMORY: 8, MODEL_HERTZ_MEMORY: 7}
 
 
 class GdemError(RuntimeError):
@@ -86,6 +89,7 @@ def load_library(path: str = LIB_PATH):
         "gdem_set_particles": (C.c_int, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]),
         "gdem_add_wall": (C.c_int, [vp, vp, vp, vp, d, i32]),
         "gdem_set_contact_model": (C.c_int, [vp, i32, i32, i32, vp]),
+        "gdem_set_time_step": (C.c_int, [vp, d]),
         "gdem_link_update": (C.c_int, [vp]),
         "gdem_compute_forces": (C.c_int, [vp]),
         "gdem_move": (C.c_int, [vp, d, d]),
@@ -145,7 +149,7 @@ def load_library(path: str = LIB_PATH):
 
 
 EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall",
-            "gdem_set_contact_model", "gdem_link_update", "gdem_compute_forces",
+            "gdem_set_contact_model", "gdem_set_time_step", "gdem_link_update", "gdem_compute_forces",
             "gdem_move", "gdem_advance_velocity", "gdem_prime", "gdem_step", "gdem_set_graph",
             "gdem_set_skin", "gdem_get_skin",
             "gdem_synchronize", "gdem_get_state", "gdem_set_contact_recording",
@@ -266,9 +270,14 @@ class GdemSystem:
                     "gdem_add_wall")
 
     def set_contact_model(self, mat_a, mat_b, kind, params):
-        p = _f64(params, (5,))
+        p = _f64(params, (MODEL_NPARAMS.get(int(kind), 5),))
         self._check(self.lib.gdem_set_contact_model(self.h, int(mat_a), int(mat_b), int(kind), _ptr(p)),
                     "gdem_set_contact_model")
+
+    def set_time_step(self, dt):
+        """dt of the force evaluations for the memory models (gdem_step and
+        move set it too)."""
+        self._check(self.lib.gdem_set_time_step(self.h, float(dt)), "gdem_set_time_step")
 
     # -- phases
     def link_update(self):
@@ -375,6 +384,8 @@ def build_system(wl, device=0):
     else:
         s.set_particles(p["x"], p["v"], p["w"], p["q"], p["radius"], p["density"], p["crust"],
                         p["material"])
+    if "dt" in wl:
+        s.set_time_step(wl["dt"])
     return s
 
 

@@ -17,7 +17,7 @@ import os
 
 import numpy as np
 
-from .gdem import MODEL_HERTZ, MODEL_HOOKE
+from .gdem import MODEL_HERTZ, MODEL_HERTZ_MEMORY, MODEL_HOOKE, MODEL_HOOKE_MEMORY
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _WL_PATH = os.path.join(_HERE, "lib", "libgdem_workloads.so")
@@ -121,6 +121,49 @@ def periodic_hertz(n=1_000_000, seed=2, steps=500, phi=0.30, cell_factor=1.2):
             "gravity": (0.0, 0.0, 0.0), "dt": 5.0e-6, "steps": steps,
             "models": [(0, 0, MODEL_HERTZ, hertz)], "walls": [],
             "particles": _particles(x, v, r)}
+
+
+# ------------------------------------------------------------------ memory models
+def sphere_on_wall(model, v0=(0.0, 0.0, 0.0), w0=(0.0, 0.0, 0.0), r=1.0e-3, dt=1.0e-6,
+                   length=0.1):
+    """One sphere (density 2500) resting on the top face (z = 0) of a box
+    wall, at the static overlap m g / kn of a Hooke model (kn = model[1][0]),
+    with velocity v0 and angular velocity w0; the domain runs length along x.
+    model = (kind, params) for the sphere-wall pair."""
+    kind, prm = model
+    m = DENSITY * 4.0 * np.pi * r ** 3 / 3.0
+    delta = m * 9.81 / prm[0] if kind in (MODEL_HOOKE, MODEL_HOOKE_MEMORY) else 0.0
+    x = np.array([[0.1 * length, 0.01, r - delta]])
+    lo, size = (0.0, 0.0, -2.0e-3), (length, 0.02, 0.01)
+    wall = (IDENTITY, (0.5 * length, 0.01, -1.0e-3), (length, 0.02, 1.0e-3), CRUST, 1)
+    return {"name": "sphere_on_wall", "origin": lo, "size": size, "periodic": (0, 0, 0),
+            "cell_size": 2.0 * r, "gravity": (0.0, 0.0, -9.81), "dt": dt, "steps": 1000,
+            "models": [(0, 0, kind, prm), (0, 1, kind, prm)], "walls": [wall],
+            "particles": _particles(x, np.array([v0], dtype=float), np.array([r]))
+            | {"w": np.array([w0], dtype=float)}}
+
+
+# HookeMemory {kn, en, etat, muc, kt, mur, etarpf, epst} with rolling friction
+HOOKE_MEMORY = (1.0e5, 0.9, -1.0, 0.5, 2.0e4, 0.02, 0.5, 0.0)
+# HertzMemory {Es, en, Gs, muc, mur, etarpf, epst}
+HERTZ_MEMORY = (1.0e7, 0.85, 1.0e7 / 4.0, 0.4, 0.02, 0.5, 0.0)
+
+
+def memory_box(n=1000, seed=1):
+    """plain_box with the HookeMemory model (tangential spring kt 2e4,
+    rolling friction mur 0.02, etarpf 0.5) between particles and walls."""
+    wl = plain_box(n, seed)
+    wl["name"] = f"memory_box_{n}"
+    wl["models"] = [(0, 0, MODEL_HOOKE_MEMORY, HOOKE_MEMORY), (0, 1, MODEL_HOOKE_MEMORY, HOOKE_MEMORY)]
+    return wl
+
+
+def memory_periodic(n=20000, seed=2):
+    """periodic_hertz with the HertzMemory model (rolling friction mur 0.02)."""
+    wl = periodic_hertz(n, seed, steps=100)
+    wl["name"] = f"memory_periodic_{n}"
+    wl["models"] = [(0, 0, MODEL_HERTZ_MEMORY, HERTZ_MEMORY)]
+    return wl
 
 
 def periodic_hooke(n=2000, seed=12, phi=0.30):
