@@ -73,7 +73,7 @@ class ContactForceModel {
   const double* parameters() const { return m_p; }
 
  protected:
-  double m_p[5];
+  double m_p[8] = {0., 0., 0., 0., 0., 0., 0., 0.};  // as in include/gdem.h, per kind
 };
 
 // Component/src/HookeContactForceModel.cpp:11-20; keys kn, en, etat, muc, kr
@@ -100,6 +100,34 @@ class HertzContactForceModel : public ContactForceModel {
                                 double req1, double v0, double dt,
                                 std::ostream& OUT) const override;
   int32_t kind() const override { return GDEM_MODEL_HERTZ; }
+};
+
+// Component/src/HookeMemoryContactForceModel.cpp:11-29; keys kn, en, muc, kt;
+// etat defaults to -1, mur and etarpf to 0, and epst to 0: defineParameters
+// (:256-307) stores its 1e-10 default under the key "eps", so the constructor
+// reads 0.  Tangential spring, contact-plane rotation and rolling friction
+// with memory of each contact (gdem_set_contact_model, kind
+// GDEM_MODEL_HOOKE_MEMORY); spheres only.
+class HookeMemoryContactForceModel : public ContactForceModel {
+ public:
+  explicit HookeMemoryContactForceModel(std::map<std::string, double>& parameters);
+  std::string name() const override { return "HookeMemoryContactForceModel"; }
+  void computeAndWriteEstimates(double m0, double m1, double crust0, double crust1, double req0,
+                                double req1, double v0, double dt,
+                                std::ostream& OUT) const override;
+  int32_t kind() const override { return GDEM_MODEL_HOOKE_MEMORY; }
+};
+
+// Component/src/HertzMemoryContactForceModel.cpp:12-31; keys Es, en, muc; Gs
+// defaults to Es / 4, mur and etarpf to 0, epst to 0 (as for HookeMemory).
+class HertzMemoryContactForceModel : public ContactForceModel {
+ public:
+  explicit HertzMemoryContactForceModel(std::map<std::string, double>& parameters);
+  std::string name() const override { return "HertzMemoryContactForceModel"; }
+  void computeAndWriteEstimates(double m0, double m1, double crust0, double crust1, double req0,
+                                double req1, double v0, double dt,
+                                std::ostream& OUT) const override;
+  int32_t kind() const override { return GDEM_MODEL_HERTZ_MEMORY; }
 };
 
 // Component/src/ContactBuilderFactory.cpp:22-192: materials and the model of
@@ -211,11 +239,15 @@ class Grains {
   void getState(std::vector<double>& x, std::vector<double>& v, std::vector<double>& w,
                 std::vector<double>& q, std::vector<double>& F, std::vector<double>& T) const;
   double time() const { return m_time; }
+  // Grains::m_dt (read from the input's time step in the reference); the
+  // memory contact models use it in every force evaluation
+  void setTimeStep(double dt) { m_dt = dt; }
 
  private:
   void check(int rc) const;
   LinkedCell& m_lc;
   double m_time = 0.;
+  double m_dt = 0.;
   bool m_primed = false;
 };
 

@@ -4,9 +4,10 @@
 //
 //   grains_run <input.bin> <output.bin>
 //
-// Input (little endian): "GDEMIN01", int32 n, nwalls, nmodels, nsteps,
-// double dt, origin[3], size[3], int32 periodic[3], double cell_size,
-// gravity[3]; per model int32 kind, mat_a, mat_b, double p[5]; per wall
+// Input (little endian): "GDEMIN01" or "GDEMIN02", int32 n, nwalls, nmodels,
+// nsteps, double dt, origin[3], size[3], int32 periodic[3], double cell_size,
+// gravity[3]; per model int32 kind, mat_a, mat_b, double p[5] (GDEMIN01) or
+// p[8] (GDEMIN02: the memory models' parameters, include/gdem.h); per wall
 // double basis[9], centre[3], half_extent[3], crust, int32 material; then
 // x[3n], v[3n], w[3n], q[4n], radius[n], density[n], crust[n], int32 material[n].
 // Output: "GDEMOUT1", int32 n, status (0 or the GDEM_ERR_* the step raised),
@@ -35,7 +36,10 @@ int main(int argc, char** argv) {
   char magic[8];
   int32_t n, nwalls, nmodels, nsteps, periodic[3];
   double dt, origin[3], size[3], cell, grav[3];
-  if (!rd(f, magic, 8) || memcmp(magic, "GDEMIN01", 8) || !rd(f, &n, 1) || !rd(f, &nwalls, 1) ||
+  if (!rd(f, magic, 8) || (memcmp(magic, "GDEMIN01", 8) && memcmp(magic, "GDEMIN02", 8)))
+    return 2;
+  const int np = memcmp(magic, "GDEMIN02", 8) == 0 ? 8 : 5;
+  if (!rd(f, &n, 1) || !rd(f, &nwalls, 1) ||
       !rd(f, &nmodels, 1) || !rd(f, &nsteps, 1) || !rd(f, &dt, 1) || !rd(f, origin, 3) ||
       !rd(f, size, 3) || !rd(f, periodic, 3) || !rd(f, &cell, 1) || !rd(f, grav, 3))
     return 2;
@@ -43,16 +47,26 @@ int main(int argc, char** argv) {
   ContactBuilderFactory::reset();
   for (int m = 0; m < nmodels; ++m) {
     int32_t kind, ma, mb;
-    double p[5];
-    if (!rd(f, &kind, 1) || !rd(f, &ma, 1) || !rd(f, &mb, 1) || !rd(f, p, 5)) return 2;
+    double p[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+    if (!rd(f, &kind, 1) || !rd(f, &ma, 1) || !rd(f, &mb, 1) || !rd(f, p, (size_t)np)) return 2;
     std::map<std::string, double> par;
     std::unique_ptr<ContactForceModel> model;
     if (kind == GDEM_MODEL_HOOKE) {
       par = {{"kn", p[0]}, {"en", p[1]}, {"etat", p[2]}, {"muc", p[3]}, {"kr", p[4]}};
       model.reset(new HookeContactForceModel(par));
-    } else {
+    } else if (kind == GDEM_MODEL_HERTZ) {
       par = {{"Es", p[0]}, {"en", p[1]}, {"Gs", p[2]}, {"muc", p[3]}, {"kr", p[4]}};
       model.reset(new HertzContactForceModel(par));
+    } else if (kind == GDEM_MODEL_HOOKE_MEMORY) {
+      par = {{"kn", p[0]}, {"en", p[1]}, {"etat", p[2]}, {"muc", p[3]}, {"kt", p[4]},
+             {"mur", p[5]}, {"etarpf", p[6]}, {"epst", p[7]}};
+      model.reset(new HookeMemoryContactForceModel(par));
+    } else if (kind == GDEM_MODEL_HERTZ_MEMORY) {
+      par = {{"Es", p[0]}, {"en", p[1]}, {"Gs", p[2]}, {"muc", p[3]}, {"mur", p[4]},
+             {"etarpf", p[5]}, {"epst", p[6]}};
+      model.reset(new HertzMemoryContactForceModel(par));
+    } else {
+      return 2;
     }
     // materials are named by index so that indices match the input file
     for (int q = 0; q <= std::max(ma, mb); ++q)
@@ -104,6 +118,7 @@ int main(int argc, char** argv) {
   Grains app(lc, *ti);
   int32_t status = 0, id0 = -1, id1 = -1;
   try {
+    app.setTimeStep(dt);  // the input's time step, before a0 (memory models use it)
     app.do_before_time_stepping();
     app.Simulation(nsteps * dt, dt);
   } catch (ContactError const& e) {

@@ -107,6 +107,51 @@ void HookeContactForceModel::computeAndWriteEstimates(double m0, double m1, doub
   if (dmax > crust0 + crust1) OUT << "  WARNING: delta_max > maximum overlap allowed by crusts\n";
 }
 
+HookeMemoryContactForceModel::HookeMemoryContactForceModel(std::map<std::string, double>& p)
+    : ContactForceModel(p) {
+  m_p[0] = need(p, "kn");
+  m_p[1] = need(p, "en");
+  m_p[2] = p.count("etat") ? p["etat"] : -1.;
+  m_p[3] = need(p, "muc");
+  m_p[4] = need(p, "kt");
+  m_p[5] = p.count("mur") ? p["mur"] : 0.;
+  m_p[6] = p.count("etarpf") ? p["etarpf"] : 0.;
+  m_p[7] = p.count("epst") ? p["epst"] : 0.;
+}
+
+// HookeMemoryContactForceModel::computeAndWriteEstimates (:319-353) uses the
+// normal Hooke law of HookeContactForceModel.
+void HookeMemoryContactForceModel::computeAndWriteEstimates(double m0, double m1, double crust0,
+                                                            double crust1, double req0,
+                                                            double req1, double v0, double dt,
+                                                            std::ostream& OUT) const {
+  std::map<std::string, double> q{{"kn", m_p[0]}, {"en", m_p[1]}, {"muc", m_p[3]}};
+  HookeContactForceModel(q).computeAndWriteEstimates(m0, m1, crust0, crust1, req0, req1, v0, dt,
+                                                     OUT);
+}
+
+HertzMemoryContactForceModel::HertzMemoryContactForceModel(std::map<std::string, double>& p)
+    : ContactForceModel(p) {
+  m_p[0] = need(p, "Es");
+  m_p[1] = need(p, "en");
+  m_p[2] = p.count("Gs") ? p["Gs"] : m_p[0] / 4.;
+  m_p[3] = need(p, "muc");
+  m_p[4] = p.count("mur") ? p["mur"] : 0.;
+  m_p[5] = p.count("etarpf") ? p["etarpf"] : 0.;
+  m_p[6] = p.count("epst") ? p["epst"] : 0.;
+}
+
+// HertzMemoryContactForceModel::computeAndWriteEstimates (:316-360) uses the
+// normal Hertz law of HertzContactForceModel.
+void HertzMemoryContactForceModel::computeAndWriteEstimates(double m0, double m1, double crust0,
+                                                            double crust1, double req0,
+                                                            double req1, double v0, double dt,
+                                                            std::ostream& OUT) const {
+  std::map<std::string, double> q{{"Es", m_p[0]}, {"en", m_p[1]}, {"Gs", m_p[2]}, {"muc", m_p[3]}};
+  HertzContactForceModel(q).computeAndWriteEstimates(m0, m1, crust0, crust1, req0, req1, v0, dt,
+                                                     OUT);
+}
+
 HertzContactForceModel::HertzContactForceModel(std::map<std::string, double>& p)
     : ContactForceModel(p) {
   m_p[0] = need(p, "Es");
@@ -281,7 +326,9 @@ void LinkedCell::LinkUpdate(double, double) {
   if (gdem_link_update(m_ctx)) throw std::runtime_error("gdem_link_update failed");
 }
 
-void LinkedCell::ComputeForces(double, double) {
+void LinkedCell::ComputeForces(double, double dt) {
+  if (dt > 0. && gdem_set_time_step(m_ctx, dt))
+    throw std::runtime_error("gdem_set_time_step failed");
   if (gdem_compute_forces(m_ctx)) throw std::runtime_error("gdem_compute_forces failed");
 }
 
@@ -306,12 +353,14 @@ void Grains::check(int rc) const {
 }
 
 void Grains::do_before_time_stepping() {
+  if (m_dt > 0.) check(gdem_set_time_step(m_lc.context(), m_dt));
   check(gdem_prime(m_lc.context()));
   check(gdem_synchronize(m_lc.context()));
   m_primed = true;
 }
 
 void Grains::Simulation(double time_interval, double dt) {
+  m_dt = dt;
   if (!m_primed) do_before_time_stepping();
   const int32_t nsteps = (int32_t)llround(time_interval / dt);
   check(gdem_step(m_lc.context(), nsteps, dt));
@@ -324,7 +373,7 @@ void Grains::moveParticlesAndObstacles(double dtv, double dtd) {
   m_lc.LinkUpdate(m_time, dtd);
 }
 
-void Grains::computeParticlesForceAndTorque() { m_lc.ComputeForces(m_time, 0.); }
+void Grains::computeParticlesForceAndTorque() { m_lc.ComputeForces(m_time, m_dt); }
 
 void Grains::advanceParticlesVelocity(double dtv) {
   check(gdem_advance_velocity(m_lc.context(), dtv));

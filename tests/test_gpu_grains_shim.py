@@ -21,8 +21,9 @@ RUN = os.path.join(ROOT, "pacific_amd", "lib", "grains_run")
 def write_input(path, wl, nsteps):
     p = wl["particles"]
     n = len(p["radius"])
+    memory = any(kind >= 3 for (_, _, kind, _) in wl["models"])
     with open(path, "wb") as f:
-        f.write(b"GDEMIN01")
+        f.write(b"GDEMIN02" if memory else b"GDEMIN01")
         f.write(struct.pack("<4i", n, len(wl["walls"]), len(wl["models"]), nsteps))
         f.write(struct.pack("<d", wl["dt"]))
         f.write(struct.pack("<3d", *wl["origin"]))
@@ -32,7 +33,10 @@ def write_input(path, wl, nsteps):
         f.write(struct.pack("<3d", *wl["gravity"]))
         for (a, b, kind, par) in wl["models"]:
             f.write(struct.pack("<3i", kind, a, b))
-            f.write(struct.pack("<5d", *par))
+            if memory:
+                f.write(struct.pack("<8d", *(list(par) + [0.0] * (8 - len(par)))))
+            else:
+                f.write(struct.pack("<5d", *par))
         for (basis, centre, half, crust, mat) in wl["walls"]:
             f.write(struct.pack("<9d", *basis))
             f.write(struct.pack("<3d", *centre))
@@ -72,6 +76,25 @@ def test_driver_matches_oracle(oracle_mod):
     orc = oracle_mod.Oracle(wl)
     orc.prime()
     orc.step(100, wl["dt"])
+    o = orc.state()
+    for k in ("x", "v", "w", "q"):
+        assert rel_err(g[k], o[k]) <= 1e-9, k
+
+
+def test_driver_memory_model_matches_oracle(oracle_mod):
+    """HookeMemoryContactForceModel through the host classes: the time step
+    reaches the memory update of every force evaluation, the priming one
+    included."""
+    print("[case] grains_run memory box 40 steps", flush=True)
+    wl = workloads.memory_box(1000, seed=1)
+    wl["particles"]["radius"] = wl["particles"]["radius"] * 1.08
+    wl["cell_size"] = 2.0 * wl["particles"]["radius"].max()
+    status, _, _, g = run(wl, 40)
+    assert status == 0
+    # 40 steps: the oracle's one-ulp self-divergence is ~1e-12 here
+    orc = oracle_mod.Oracle(wl)
+    orc.prime()
+    orc.step(40, wl["dt"])
     o = orc.state()
     for k in ("x", "v", "w", "q"):
         assert rel_err(g[k], o[k]) <= 1e-9, k
