@@ -223,6 +223,59 @@ int gdem_synchronize(gdem_ctx* ctx);
 int gdem_get_state(gdem_ctx* ctx, double* x, double* v, double* w, double* q,
                    double* F, double* T);
 
+/* Restores particle state in id order (any pointer may be NULL: kept), as
+   the reference's reload does (Particle::read2014 / read2014_binary,
+   Component/src/Particle.cpp:973-1081; ParticleKinematics::
+   readParticleKinematics2014, ParticleKinematics.cpp:378-406: the body-frame
+   angular velocity follows from w and q).  F and T go together: given, they
+   become the leap-frog's a0 and the next gdem_step starts from them without a
+   force evaluation (SecondOrderLeapFrog::readParticleKinematics2014,
+   SecondOrderLeapFrog.cpp:129-150).  Sphere contexts. */
+int gdem_set_state(gdem_ctx* ctx, const double* x, const double* v, const double* w,
+                   const double* q, const double* F, const double* T);
+
+/* Contact memory of the history models (Component::m_contactMap,
+   Component.hh:505-508): every entry, sorted by (owner id, key); key is the
+   partner's id, or -1 - k for wall k; kd, nrm, spr are the owner's view of the
+   tangential spring, previous normal and rolling spring torque (3 doubles
+   each).  Writes min(count, capacity) entries; *n_out = count. */
+int gdem_get_memory(gdem_ctx* ctx, int64_t capacity, int64_t* n_out, int32_t* owner,
+                    int32_t* key, double* kd, double* nrm, double* spr);
+/* Replaces the contact memory by count entries as above
+   (Component::copyContactInMap via readContactMap2014, Component.cpp:613-624,
+   858-916).  Needs a memory model to be set. */
+int gdem_set_memory(gdem_ctx* ctx, int64_t count, const int32_t* owner, const int32_t* key,
+                    const double* kd, const double* nrm, const double* spr);
+
+/* Particle count, walls, convex flag, memory models in use. */
+int gdem_info(gdem_ctx* ctx, int64_t* n, int32_t* nwalls, int32_t* convex,
+              int32_t* memory_models);
+
+/* Particle records of a reload file in the reference's 2014 format
+   (AllComponents::write, Component/src/AllComponents.cpp:1148-1181;
+   Particle::write2014 / write2014_binary, Particle.cpp:1168-1199): one record
+   per particle in id order with geometric type geom_type[id] (NULL: 0), id
+   = id + id_offset, tag 0, activity 1, transform (origin = x, basis = R(q)
+   for spheres or the context's basis for convex bodies, type), kinematics
+   (v, q as w then x, y, z, angular velocity), the leap-frog's F and T of
+   the last force evaluation and the contact memory (keys (1, partner id +
+   id_offset, 1), walls (1, wall_id_offset + k, 1)).  binary != 0: the
+   hybrid "_Particles.result.bin" file (each record preceded by its byte count
+   as a size_t); else the text "_Particles.result" file. */
+int gdem_write_particles_2014(gdem_ctx* ctx, const char* path, int32_t binary,
+                              int32_t id_offset, int32_t wall_id_offset,
+                              const int32_t* geom_type);
+/* Reads such a file into the context (its particles already set, in the same
+   number, with their radius, density, crust and material, e.g. from the
+   reload file's reference particles): record ids map to particle id - id_offset,
+   memory keys to particles or to walls (wall_id_offset + k); positions,
+   orientations, velocities, F, T (the context is then primed) and contact
+   memory are restored.  geom_type_out[id] receives the records' types (may be
+   NULL).  Sphere contexts. */
+int gdem_read_particles_2014(gdem_ctx* ctx, const char* path, int32_t binary,
+                             int32_t id_offset, int32_t wall_id_offset,
+                             int32_t* geom_type_out);
+
 /* Contact list of the next force evaluations, kept for the last one.
    capacity_per_particle bounds the contacts a particle can be the first body
    of; 0 disables recording.  Records are in canonical order: sorted by (id0,

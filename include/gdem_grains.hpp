@@ -242,6 +242,14 @@ class Grains {
   // Grains::m_dt (read from the input's time step in the reference); the
   // memory contact models use it in every force evaluation
   void setTimeStep(double dt) { m_dt = dt; }
+  // The particle records of a reload file (the "_Particles.result[.bin]" part
+  // of Grains::saveReload, Base/src/Grains.cpp:2543-2581, and of the reload
+  // read, AllComponents::read): write the current state, or restore it so that
+  // Simulation continues from the stored F, T without recomputing a0.
+  void saveParticles2014(std::string const& path, bool binary, int id_offset = 0,
+                         int wall_id_offset = 0);
+  void readParticles2014(std::string const& path, bool binary, int id_offset = 0,
+                         int wall_id_offset = 0);
 
  private:
   void check(int rc) const;

@@ -1068,6 +1068,156 @@ extern "C" int gdem_get_state(gdem_ctx* c, double* x, double* v, double* w, doub
   return collect_errors(c);
 }
 
+static int ensure_stage(gdem_ctx* c, size_t need) {
+  if (c->stage_len < need) {
+    (void)hipFree(c->d_stage);
+    c->d_stage = nullptr;
+    HIP_TRY(hipMalloc(&c->d_stage, sizeof(double) * std::max<size_t>(need, 1)));
+    c->stage_len = need;
+  }
+  return GDEM_OK;
+}
+
+extern "C" int gdem_set_state(gdem_ctx* c, const double* x, const double* v, const double* w,
+                              const double* q, const double* F, const double* T) {
+  if (!c || c->convex || (!F) != (!T)) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const size_t n = (size_t)c->n;
+  if (ensure_stage(c, n * 19)) return GDEM_ERR_HIP;
+  double* dx = c->d_stage;
+  double* dv = dx + 3 * n;
+  double* dw = dv + 3 * n;
+  double* dq = dw + 3 * n;
+  double* dF = dq + 4 * n;
+  double* dT = dF + 3 * n;
+  if (n > 0) {
+    if (x) HIP_TRY(hipMemcpy(dx, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    if (v) HIP_TRY(hipMemcpy(dv, v, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    if (w) HIP_TRY(hipMemcpy(dw, w, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    if (q) HIP_TRY(hipMemcpy(dq, q, sizeof(double) * 4 * n, hipMemcpyHostToDevice));
+    if (F) HIP_TRY(hipMemcpy(dF, F, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    if (T) HIP_TRY(hipMemcpy(dT, T, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  }
+  launch_set_state(c->buf[c->cur], c->d_F, c->d_T, (int32_t)n, x ? dx : nullptr,
+                   v ? dv : nullptr, w ? dw : nullptr, q ? dq : nullptr, F ? dF : nullptr,
+                   T ? dT : nullptr, c->stream);
+  HIP_TRY(hipGetLastError());
+  if (x) {
+    launch_keys(c->buf[c->cur], (int32_t)n, c->grid, c->d_row_rank, c->bins.key, c->d_err,
+                c->stream);
+    HIP_TRY(hipGetLastError());
+    c->binned = false;
+    if (c->d_skin) {
+      const int32_t words[kSkinWords] = {1, 1, 0};  // the lists are rebuilt
+      HIP_TRY(hipMemcpy(c->d_skin, words, sizeof(words), hipMemcpyHostToDevice));
+    }
+  }
+  // given F and T are the leap-frog's a0: the next step starts from them
+  if (F) c->primed = true;
+  else if (x || v || w || q) c->primed = false;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return collect_errors(c);
+}
+
+// Downloads the ids of the storage order of the current copy.
+static int slot_ids(gdem_ctx* c, std::vector<int32_t>& id) {
+  const size_t n = (size_t)c->n;
+  std::vector<Aux> ax(n);
+  if (n) HIP_TRY(hipMemcpy(ax.data(), c->buf[c->cur].aux, sizeof(Aux) * n, hipMemcpyDeviceToHost));
+  id.resize(n);
+  for (size_t s = 0; s < n; ++s) id[s] = ax[s].id;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_get_memory(gdem_ctx* c, int64_t capacity, int64_t* n_out, int32_t* owner,
+                               int32_t* key, double* kd, double* nrm, double* spr) {
+  if (!c || capacity < 0 || !n_out) return GDEM_ERR_ARG;
+  *n_out = 0;
+  if (!c->has_memory || !c->hist[0].count) return GDEM_OK;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const size_t n = (size_t)c->n, ne = n * kHistCap;
+  std::vector<int32_t> id, cnt(n), hk(ne);
+  std::vector<double4> hkd(ne), hnr(ne), hsp(ne);
+  if (slot_ids(c, id)) return GDEM_ERR_HIP;
+  const History& h = c->hist[c->cur];
+  if (n) {
+    HIP_TRY(hipMemcpy(cnt.data(), h.count, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hk.data(), h.key, sizeof(int32_t) * ne, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hkd.data(), h.kd, sizeof(double4) * ne, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hnr.data(), h.nrm, sizeof(double4) * ne, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hsp.data(), h.spr, sizeof(double4) * ne, hipMemcpyDeviceToHost));
+  }
+  // entries by (owner id, key)
+  std::vector<std::pair<std::pair<int32_t, int32_t>, size_t>> e;
+  for (size_t s = 0; s < n; ++s)
+    for (int k = 0; k < std::min(cnt[s], kHistCap); ++k) {
+      const size_t at = (size_t)k * n + s;
+      e.push_back({{id[s], hk[at]}, at});
+    }
+  std::sort(e.begin(), e.end());
+  *n_out = (int64_t)e.size();
+  for (size_t j = 0; j < e.size() && (int64_t)j < capacity; ++j) {
+    const size_t at = e[j].second;
+    if (owner) owner[j] = e[j].first.first;
+    if (key) key[j] = e[j].first.second;
+    const double4 a = hkd[at], b = hnr[at], d = hsp[at];
+    if (kd) { kd[3 * j] = a.x; kd[3 * j + 1] = a.y; kd[3 * j + 2] = a.z; }
+    if (nrm) { nrm[3 * j] = b.x; nrm[3 * j + 1] = b.y; nrm[3 * j + 2] = b.z; }
+    if (spr) { spr[3 * j] = d.x; spr[3 * j + 1] = d.y; spr[3 * j + 2] = d.z; }
+  }
+  return GDEM_OK;
+}
+
+extern "C" int gdem_set_memory(gdem_ctx* c, int64_t count, const int32_t* owner,
+                               const int32_t* key, const double* kd, const double* nrm,
+                               const double* spr) {
+  if (!c || count < 0 || (count > 0 && (!owner || !key || !kd || !nrm || !spr)))
+    return GDEM_ERR_ARG;
+  if (!c->has_memory || !c->hist[0].count) return count == 0 ? GDEM_OK : GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const size_t n = (size_t)c->n, ne = n * kHistCap;
+  std::vector<int32_t> id;
+  if (slot_ids(c, id)) return GDEM_ERR_HIP;
+  std::vector<int32_t> slot(n, -1), cnt(n, 0), hk(ne, 0);
+  std::vector<double4> hkd(ne), hnr(ne), hsp(ne);
+  for (size_t s = 0; s < n; ++s)
+    if (id[s] >= 0 && (size_t)id[s] < n) slot[id[s]] = (int32_t)s;
+  for (int64_t j = 0; j < count; ++j) {
+    if (owner[j] < 0 || (size_t)owner[j] >= n || slot[owner[j]] < 0) return GDEM_ERR_ARG;
+    if (key[j] >= 0 && (size_t)key[j] >= n) return GDEM_ERR_ARG;
+    if (key[j] < 0 && -1 - key[j] >= c->nwalls) return GDEM_ERR_ARG;
+    const size_t s = (size_t)slot[owner[j]];
+    if (cnt[s] >= kHistCap) return GDEM_ERR_CAPACITY;
+    const size_t at = (size_t)cnt[s]++ * n + s;
+    hk[at] = key[j];
+    hkd[at] = make_double4(kd[3 * j], kd[3 * j + 1], kd[3 * j + 2], 0.);
+    hnr[at] = make_double4(nrm[3 * j], nrm[3 * j + 1], nrm[3 * j + 2], 0.);
+    hsp[at] = make_double4(spr[3 * j], spr[3 * j + 1], spr[3 * j + 2], 0.);
+  }
+  const History& h = c->hist[c->cur];
+  if (n) {
+    HIP_TRY(hipMemcpy(h.count, cnt.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h.key, hk.data(), sizeof(int32_t) * ne, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h.kd, hkd.data(), sizeof(double4) * ne, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h.nrm, hnr.data(), sizeof(double4) * ne, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h.spr, hsp.data(), sizeof(double4) * ne, hipMemcpyHostToDevice));
+  }
+  return GDEM_OK;
+}
+
+extern "C" int gdem_info(gdem_ctx* c, int64_t* n, int32_t* nwalls, int32_t* convex,
+                         int32_t* memory_models) {
+  if (!c) return GDEM_ERR_ARG;
+  if (n) *n = c->n;
+  if (nwalls) *nwalls = c->nwalls;
+  if (convex) *convex = c->convex ? 1 : 0;
+  if (memory_models) *memory_models = c->has_memory ? 1 : 0;
+  return GDEM_OK;
+}
+
 extern "C" int gdem_set_contact_recording(gdem_ctx* c, int32_t cap) {
   if (!c || cap < 0 || cap > 256) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));

@@ -281,5 +281,9 @@ void launch_advance(const State& s, const double4* F, const double4* T, int32_t 
 void launch_unpermute(const State& s, const double4* F, const double4* T, int32_t n,
                       double* x, double* v, double* w, double* q, double* f, double* t,
                       hipStream_t st);
+// id-ordered device arrays (NULL: kept) into the storage order of s
+void launch_set_state(const State& s, double4* F, double4* T, int32_t n, const double* x,
+                      const double* v, const double* w, const double* q, const double* f,
+                      const double* t, hipStream_t st);
 
 }  // namespace gdem

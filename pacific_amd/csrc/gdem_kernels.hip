@@ -273,6 +273,40 @@ __global__ void __launch_bounds__(kBlock) k_unpermute(State s, const double4* Fa
   if (t) { double4 a = Ta[i]; t[3 * id] = a.x; t[3 * id + 1] = a.y; t[3 * id + 2] = a.z; }
 }
 
+// Dynamic state given in id order (NULL: kept) written into the storage order
+// of the copy s; the body-frame angular velocity follows from the new w and q
+// as the reference's reload does (ParticleKinematics::readParticleKinematics2014,
+// Component/src/ParticleKinematics.cpp:378-388).
+__global__ void __launch_bounds__(kBlock) k_set_state(State s, double4* Fa, double4* Ta, int n,
+                                                      const double* x, const double* v,
+                                                      const double* w, const double* q,
+                                                      const double* f, const double* t) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t id = (size_t)s.aux[i].id;
+  if (x) {
+    const double4 a = s.pos_r[i];
+    s.pos_r[i] = make_double4(x[3 * id], x[3 * id + 1], x[3 * id + 2], a.w);
+  }
+  if (v) {
+    const double4 a = s.vel_m[i];
+    s.vel_m[i] = make_double4(v[3 * id], v[3 * id + 1], v[3 * id + 2], a.w);
+  }
+  if (q) s.quat[i] = make_double4(q[4 * id], q[4 * id + 1], q[4 * id + 2], q[4 * id + 3]);
+  if (w) {
+    const double4 a = s.om_req[i];
+    s.om_req[i] = make_double4(w[3 * id], w[3 * id + 1], w[3 * id + 2], a.w);
+  }
+  if (w || q) {
+    const double4 qq = s.quat[i], wr = s.om_req[i], wb = s.wbf_I[i];
+    const quat qi = qmk(mk(qq.x, qq.y, qq.z), qq.w);
+    const v3 wbf = qmulv(qconj(qi), qleftvec(qi, mk(wr.x, wr.y, wr.z)));
+    s.wbf_I[i] = make_double4(wbf.x, wbf.y, wbf.z, wb.w);
+  }
+  if (f) Fa[i] = make_double4(f[3 * id], f[3 * id + 1], f[3 * id + 2], 0.);
+  if (t) Ta[i] = make_double4(t[3 * id], t[3 * id + 1], t[3 * id + 2], 0.);
+}
+
 // ---------------------------------------------------------------- launchers
 static inline int nblocks(int n, int bs) { return (n + bs - 1) / bs; }
 
@@ -357,6 +391,14 @@ void launch_unpermute(const State& s, const double4* F, const double4* T, int32_
                       double* v, double* w, double* q, double* f, double* t, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_unpermute, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, s, F, T, n, x,
+                     v, w, q, f, t);
+}
+
+void launch_set_state(const State& s, double4* F, double4* T, int32_t n, const double* x,
+                      const double* v, const double* w, const double* q, const double* f,
+                      const double* t, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_set_state, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, s, F, T, n, x,
                      v, w, q, f, t);
 }
 

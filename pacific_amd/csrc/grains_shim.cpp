@@ -368,6 +368,19 @@ void Grains::Simulation(double time_interval, double dt) {
   check(gdem_synchronize(m_lc.context()));
 }
 
+void Grains::saveParticles2014(std::string const& path, bool binary, int id_offset,
+                               int wall_id_offset) {
+  check(gdem_write_particles_2014(m_lc.context(), path.c_str(), binary ? 1 : 0, id_offset,
+                                  wall_id_offset, nullptr));
+}
+
+void Grains::readParticles2014(std::string const& path, bool binary, int id_offset,
+                               int wall_id_offset) {
+  check(gdem_read_particles_2014(m_lc.context(), path.c_str(), binary ? 1 : 0, id_offset,
+                                 wall_id_offset, nullptr));
+  m_primed = true;  // the stored F, T are a0
+}
+
 void Grains::moveParticlesAndObstacles(double dtv, double dtd) {
   check(gdem_move(m_lc.context(), dtv, dtd));
   m_lc.LinkUpdate(m_time, dtd);

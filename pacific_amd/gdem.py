@@ -101,6 +101,12 @@ def load_library(path: str = LIB_PATH):
         "gdem_get_skin": (C.c_int, [vp, P(d), P(i64)]),
         "gdem_synchronize": (C.c_int, [vp]),
         "gdem_get_state": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+        "gdem_set_state": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+        "gdem_get_memory": (C.c_int, [vp, i64, P(i64), vp, vp, vp, vp, vp]),
+        "gdem_set_memory": (C.c_int, [vp, i64, vp, vp, vp, vp, vp]),
+        "gdem_info": (C.c_int, [vp, P(i64), P(i32), P(i32), P(i32)]),
+        "gdem_write_particles_2014": (C.c_int, [vp, C.c_char_p, i32, i32, i32, vp]),
+        "gdem_read_particles_2014": (C.c_int, [vp, C.c_char_p, i32, i32, i32, vp]),
         "gdem_set_contact_recording": (C.c_int, [vp, i32]),
         "gdem_get_contacts": (C.c_int, [vp, i64, P(i64), vp, vp, vp, vp, vp, vp]),
         "gdem_last_error": (C.c_int, [vp, P(i32), P(i32), P(i32)]),
@@ -152,7 +158,9 @@ EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall"
             "gdem_set_contact_model", "gdem_set_time_step", "gdem_link_update", "gdem_compute_forces",
             "gdem_move", "gdem_advance_velocity", "gdem_prime", "gdem_step", "gdem_set_graph",
             "gdem_set_skin", "gdem_get_skin",
-            "gdem_synchronize", "gdem_get_state", "gdem_set_contact_recording",
+            "gdem_synchronize", "gdem_get_state", "gdem_set_state", "gdem_get_memory",
+            "gdem_set_memory", "gdem_info", "gdem_write_particles_2014",
+            "gdem_read_particles_2014", "gdem_set_contact_recording",
             "gdem_get_contacts", "gdem_last_error", "gdem_set_kernel_timing",
             "gdem_get_counters", "gdem_reset_counters", "gdem_get_grid", "gdem_version",
             "gdem_add_box_shape", "gdem_add_polyhedron_shape", "gdem_set_convex_particles",
@@ -313,6 +321,51 @@ class GdemSystem:
 
     def set_graph(self, on):
         self._check(self.lib.gdem_set_graph(self.h, int(bool(on))), "gdem_set_graph")
+
+    # -- state transfer and reload files
+    def set_state(self, x=None, v=None, w=None, q=None, F=None, T=None):
+        """Restore particle state in id order (None: kept); F and T together
+        become the leap-frog's a0 (no force evaluation before the next step)."""
+        n = self.n
+        arrs = [None if a is None else _f64(a, (n, k)) for a, k in
+                ((x, 3), (v, 3), (w, 3), (q, 4), (F, 3), (T, 3))]
+        self._check(self.lib.gdem_set_state(self.h, *[_ptr(a) for a in arrs]), "gdem_set_state")
+
+    def get_memory(self):
+        """Contact memory: dict of arrays owner, key, kd, nrm, spr (by owner, key)."""
+        cnt = C.c_int64()
+        self._check(self.lib.gdem_get_memory(self.h, 0, C.byref(cnt), None, None, None, None, None),
+                    "gdem_get_memory")
+        m = cnt.value
+        out = {"owner": np.empty(m, np.int32), "key": np.empty(m, np.int32),
+               "kd": np.empty((m, 3)), "nrm": np.empty((m, 3)), "spr": np.empty((m, 3))}
+        self._check(self.lib.gdem_get_memory(self.h, m, C.byref(cnt), _ptr(out["owner"]),
+                                             _ptr(out["key"]), _ptr(out["kd"]), _ptr(out["nrm"]),
+                                             _ptr(out["spr"])), "gdem_get_memory")
+        return out
+
+    def set_memory(self, mem):
+        m = len(mem["owner"])
+        own = np.ascontiguousarray(mem["owner"], dtype=np.int32)
+        key = np.ascontiguousarray(mem["key"], dtype=np.int32)
+        arrs = [_f64(mem[k], (m, 3)) for k in ("kd", "nrm", "spr")]
+        self._check(self.lib.gdem_set_memory(self.h, m, _ptr(own), _ptr(key), *[_ptr(a) for a in arrs]),
+                    "gdem_set_memory")
+
+    def write_particles_2014(self, path, binary=True, id_offset=0, wall_id_offset=0, geom_type=None):
+        gt = None if geom_type is None else np.ascontiguousarray(geom_type, dtype=np.int32)
+        self._check(self.lib.gdem_write_particles_2014(self.h, str(path).encode(), int(bool(binary)),
+                                                       int(id_offset), int(wall_id_offset), _ptr(gt)),
+                    "gdem_write_particles_2014")
+
+    def read_particles_2014(self, path, binary=True, id_offset=0, wall_id_offset=0):
+        """Restore state and memory from a particle reload file; returns the
+        records' geometric types in id order."""
+        gt = np.zeros(self.n, np.int32)
+        self._check(self.lib.gdem_read_particles_2014(self.h, str(path).encode(), int(bool(binary)),
+                                                      int(id_offset), int(wall_id_offset), _ptr(gt)),
+                    "gdem_read_particles_2014")
+        return gt
 
     # -- results
     def get_state(self):
