@@ -71,6 +71,13 @@ extern "C" {
 #define GDEM_MODEL_HOOKE_MEMORY 3
 #define GDEM_MODEL_HERTZ_MEMORY 4
 
+/* Imposed wall motions (ObstacleImposedVelocity types) */
+#define GDEM_MOTION_CONSTANT_TRANSLATION 1
+#define GDEM_MOTION_SIN_TRANSLATION 2
+#define GDEM_MOTION_CRENEL_TRANSLATION 3
+#define GDEM_MOTION_MULTISIN_TRANSLATION 4
+#define GDEM_MOTION_CONSTANT_ROTATION 5
+
 #define GDEM_MAX_MATERIALS 32
 #define GDEM_MAX_WALLS 64
 #define GDEM_MAX_SHAPES 16
@@ -157,6 +164,37 @@ int gdem_add_wall(gdem_ctx* ctx, const double* basis, const double* centre,
    reader leaves them: etat -1, Gs Es/4, mur 0, etarpf 0, and epst 0 (the
    reader stores its 1e-10 default under another key,
    HookeMemoryContactForceModel.cpp:300-306).  Sphere contexts only. */
+/* Imposed velocity of wall k over [t_start, t_end]
+   (ObstacleImposedVelocity, Component/src/ObstacleImposedVelocity.cpp:54-257,
+   moved as AllComponents::Move / SimpleObstacle::Move do,
+   AllComponents.cpp:151-220, SimpleObstacle.cpp:119-175).  Parameters:
+     CONSTANT_TRANSLATION  p = {vx, vy, vz}
+     SIN_TRANSLATION       p = {dx, dy, dz, amplitude, period, phase_degrees}
+     CRENEL_TRANSLATION    p = {dx, dy, dz, amplitude, period}
+     MULTISIN_TRANSLATION  p = {dx, dy, dz, ax, ay, az, px, py, pz, phix, phiy, phiz}
+     CONSTANT_ROTATION     p = {wx, wy, wz, cx, cy, cz, about_centre}
+   (direction d normalised; rotation about the wall's centre when
+   about_centre != 0, else about (cx, cy, cz)).  Each step moves the walls
+   over [t - dt, t] and gives them the summed velocities, which the contact
+   law sees (Obstacle::getVelocityAtPoint, Obstacle.cpp:128-133).  Sphere
+   contexts; graph replay is off while walls move. */
+int gdem_add_wall_motion(gdem_ctx* ctx, int32_t wall, int32_t type, double t_start,
+                         double t_end, const double* p);
+/* Simulation time (Grains::m_time, Base/src/Grains.cpp:114, :290); gdem_step
+   advances it by dt per step. */
+int gdem_set_time(gdem_ctx* ctx, double time);
+int gdem_get_time(gdem_ctx* ctx, double* time);
+/* The obstacle part of AllComponents::Move (Component/src/AllComponents.cpp:
+   151-220): sets the time and moves the walls over [time - dt, time], for
+   callers that drive the phases themselves (gdem_move, then this, then
+   gdem_compute_forces).  gdem_prime sets the initial wall velocities
+   (AllComponents::setKinematicsObstacleWithoutMoving, :235-251) once the
+   time step is known. */
+int gdem_move_walls(gdem_ctx* ctx, double time, double dt);
+/* Current transform and velocity of wall k (any pointer may be NULL). */
+int gdem_get_wall(gdem_ctx* ctx, int32_t wall, double* centre, double* basis, double* vel,
+                  double* omega);
+
 /* Time step of the force evaluations for the memory models (the dt that
    Grains::computeParticlesForceAndTorque passes to App::ComputeForces,
    Grains.cpp:459-482).  gdem_step(dt) and gdem_move(., dt) set it; a force

@@ -188,6 +188,23 @@ struct BoxObstacle {
   std::string material;
 };
 
+// ObstacleImposedVelocity (Component/include/ObstacleImposedVelocity.hh): an
+// imposed motion of a named obstacle over [tstart, tend].  type and the
+// parameter names follow the reference's XML node (ObstacleImposedVelocity.cpp:
+// 54-257):
+//   "ConstantTranslation"  VX VY VZ
+//   "SinTranslation"       VX VY VZ (direction) Amplitude Period PhaseShift (degrees)
+//   "CrenelTranslation"    VX VY VZ (direction) Amplitude Period
+//   "MultiSinTranslation"  VX VY VZ (direction) AX AY AZ PX PY PZ PhiX PhiY PhiZ
+//   "ConstantRotation"     WX WY WZ, and CX CY CZ for a rotation about that
+//                          point instead of the obstacle's centre
+struct ObstacleImposedVelocity {
+  std::string obstacle;
+  std::string type;
+  double tstart = 0., tend = 0.;
+  std::map<std::string, double> parameters;
+};
+
 // Base/include/LinkedCell.hh (an AppCollision application): the grid, the
 // particle/obstacle links, LinkUpdate and ComputeForces.
 class LinkedCell {
@@ -211,9 +228,13 @@ class LinkedCell {
   void getCells(int32_t nc[3], double edge[3]) const;
   size_t numberOfParticles() const { return m_n; }
   gdem_ctx* context() const { return m_ctx; }
+  // index of the linked obstacle of that name (-1 if none)
+  int obstacleIndex(std::string const& name) const;
 
  private:
   void push_models();
+  std::map<std::string, int> m_obstacles;
+  int m_nobstacles = 0;
   double m_origin[3], m_size[3];
   bool m_periodic[3];
   int m_device;
@@ -231,8 +252,14 @@ class Grains {
   void do_before_time_stepping();
   // Kick-drift-kick loop from the current time over time_interval
   void Simulation(double time_interval, double dt);
-  // Phases (Grains.cpp:459-527), for callers that interleave their own work
-  void moveParticlesAndObstacles(double dt_particle_vel, double dt_particle_disp);
+  // AllComponents::LinkImposedMotion (Component/src/AllComponents.cpp:130-139)
+  void LinkImposedMotion(ObstacleImposedVelocity const& motion);
+  // Grains::m_time (the Simulation loop adds dt before each step's move)
+  void setTime(double time);
+  // Phases (Grains.cpp:459-527), for callers that interleave their own work;
+  // obstacles move over [time - dt_obstacle, time] when dt_obstacle > 0
+  void moveParticlesAndObstacles(double dt_particle_vel, double dt_particle_disp,
+                                 double dt_obstacle = 0.);
   void computeParticlesForceAndTorque();
   void advanceParticlesVelocity(double dt_particle_vel);
   // Particle state in insertion order (6 vectors of 3 per particle + quaternion)

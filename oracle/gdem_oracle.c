@@ -37,8 +37,19 @@ typedef struct {
   double inv_basis[9], inv_origin[3];
   double bb_centre[3], bb_ext[3];
   double crust, req, corners[8][3];
+  double vel[3], omega[3]; /* Obstacle::m_translationalVelocity, m_angularVelocity */
   int material;
 } wall_t;
+
+/* ObstacleImposedVelocity (Component/src/ObstacleImposedVelocity.cpp:54-257):
+   one imposed motion of a wall over [t0, t1] */
+typedef struct {
+  int wall, type, live, about_centre;
+  double t0, t1;
+  double vel[3], omega[3], unit[3], centre[3];
+  double amp, period, phase;
+  double ms_amp[3], ms_period[3], ms_phase[3];
+} motion_t;
 
 struct orc_sys {
   orc_config cfg;
@@ -51,6 +62,11 @@ struct orc_sys {
   int32_t* mat;
   int nwalls;
   wall_t walls[ORC_MAX_WALLS];
+  /* simulation time and imposed wall motions, each wall's in the order of
+     ObstacleKinematicsVelocity::append; live = not yet completed */
+  double time;
+  int nmot, active_prev;
+  motion_t mot[ORC_MAX_MOTIONS];
   model_t models[ORC_MAX_MATERIALS][ORC_MAX_MATERIALS];
   /* cell lists: doubly linked, head insertion (Cell.cpp:90-93, push_front) */
   int *head, *next, *prev, *cell;
@@ -197,11 +213,8 @@ int orc_set_particles(orc_sys* s, int n, const double* x, const double* v,
    rotation basis; Convex::bbox (Convex.cpp:29-41) + BBox::setValue
    (BBox.cpp:235-239) for the wall bounding box; Box::setCornersFaces
    (Box.cpp:92-111); Box::getVolume (Box.cpp:273-279). */
-int orc_add_wall(orc_sys* s, const double* basis, const double* centre,
-                 const double* half_extent, double crust, int material) {
-  if (s->nwalls >= ORC_MAX_WALLS || material < 0 || material >= ORC_MAX_MATERIALS)
-    return ORC_ERR_ARG;
-  wall_t* wl = &s->walls[s->nwalls++];
+static void build_wall(wall_t* wl, const double* basis, const double* centre,
+                       const double* half_extent, double crust, int material) {
   memset(wl, 0, sizeof(*wl));
   memcpy(wl->basis, basis, sizeof(double) * 9);
   memcpy(wl->origin, centre, sizeof(double) * 3);
@@ -239,6 +252,260 @@ int orc_add_wall(orc_sys* s, const double* basis, const double* centre,
     for (int d = 0; d < 3; ++d) wl->corners[k][d] = sg[k][d] < 0 ? -half_extent[d] : half_extent[d];
   double vol = 8.0 * e0 * e1 * e2;
   wl->req = pow((0.75 / ORC_PI) * vol, 1. / 3.);
+}
+
+int orc_add_wall(orc_sys* s, const double* basis, const double* centre,
+                 const double* half_extent, double crust, int material) {
+  if (s->nwalls >= ORC_MAX_WALLS || material < 0 || material >= ORC_MAX_MATERIALS)
+    return ORC_ERR_ARG;
+  build_wall(&s->walls[s->nwalls++], basis, centre, half_extent, crust, material);
+  return ORC_OK;
+}
+
+/* ---- imposed wall motions ---------------------------------------------- */
+
+/* Vector3::normalize (Geometry/src/Vector3.cpp:67-70, Group3.cpp:261-267) */
+static v3 normalized(double x, double y, double z) {
+  v3 u = mk(x, y, z);
+  return dvd(u, norm(u));
+}
+
+/* constructor (ObstacleImposedVelocity.cpp:54-257) and
+   ObstacleKinematicsVelocity::append (ObstacleKinematicsVelocity.cpp:30-41) */
+int orc_add_wall_motion(orc_sys* s, int wall, int type, double t0, double t1,
+                        const double* p) {
+  if (wall < 0 || wall >= s->nwalls || s->nmot >= ORC_MAX_MOTIONS || !(t1 >= t0))
+    return ORC_ERR_ARG;
+  motion_t m;
+  memset(&m, 0, sizeof(m));
+  m.wall = wall;
+  m.type = type;
+  m.t0 = t0;
+  m.t1 = t1;
+  m.live = 1;
+  m.about_centre = 1;
+  v3 u;
+  switch (type) {
+    case ORC_MOTION_CONSTANT_TRANSLATION:
+      m.vel[0] = p[0]; m.vel[1] = p[1]; m.vel[2] = p[2];
+      break;
+    case ORC_MOTION_SIN_TRANSLATION:
+      u = normalized(p[0], p[1], p[2]);
+      st(m.unit, u);
+      m.amp = p[3];
+      m.period = p[4];
+      m.phase = p[5] * ORC_PI / 180.;
+      break;
+    case ORC_MOTION_CRENEL_TRANSLATION:
+      u = normalized(p[0], p[1], p[2]);
+      st(m.unit, u);
+      m.amp = p[3];
+      m.period = p[4];
+      break;
+    case ORC_MOTION_MULTISIN_TRANSLATION:
+      u = normalized(p[0], p[1], p[2]);
+      st(m.unit, u);
+      for (int i = 0; i < 3; ++i) {
+        m.ms_period[i] = p[6 + i];
+        m.ms_phase[i] = p[9 + i] * ORC_PI / 180.;
+        m.ms_amp[i] = p[3 + i] * m.unit[i]; /* stored times the direction (:249-254) */
+      }
+      break;
+    case ORC_MOTION_CONSTANT_ROTATION:
+      m.omega[0] = p[0]; m.omega[1] = p[1]; m.omega[2] = p[2];
+      m.centre[0] = p[3]; m.centre[1] = p[4]; m.centre[2] = p[5];
+      m.about_centre = p[6] != 0.;
+      break;
+    default:
+      return ORC_ERR_ARG;
+  }
+  int pos = s->nmot;
+  for (int i = 0; i < s->nmot; ++i)
+    if (s->mot[i].wall == wall && !(s->mot[i].t0 < t0)) { pos = i; break; }
+  memmove(&s->mot[pos + 1], &s->mot[pos], sizeof(motion_t) * (size_t)(s->nmot - pos));
+  s->mot[pos] = m;
+  s->nmot++;
+  return ORC_OK;
+}
+
+/* isActif / isCompleted (ObstacleImposedVelocity.cpp:305-325) */
+static int mot_active(const motion_t* m, double td, double te, double dt, double* subint) {
+  *subint = te - td;
+  if (td < m->t0) *subint -= (m->t0 - td);
+  if (m->t1 < te) *subint -= (te - m->t1);
+  return *subint > dt * 1.e-5;
+}
+static int mot_completed(const motion_t* m, double t, double dt) {
+  return t > m->t1 + dt * 1.e-5;
+}
+
+/* translationalVelocity (ObstacleImposedVelocity.cpp:330-370); updates the
+   stored velocity as the reference does */
+static v3 mot_velocity(motion_t* m, double time, double dt, v3 cg) {
+  if (mot_completed(m, time, dt)) {
+    m->vel[0] = m->vel[1] = m->vel[2] = 0.;
+  } else if (m->type == ORC_MOTION_SIN_TRANSLATION) {
+    double a = m->amp * sin(2. * ORC_PI * (time - m->t0) / m->period + m->phase);
+    for (int i = 0; i < 3; ++i) m->vel[i] = a * m->unit[i];
+  } else if (m->type == ORC_MOTION_CRENEL_TRANSLATION) {
+    double a = m->amp * ((int)((time - m->t0) / m->period) % 2 == 0 ? 1 : -1);
+    for (int i = 0; i < 3; ++i) m->vel[i] = a * m->unit[i];
+  } else if (m->type == ORC_MOTION_MULTISIN_TRANSLATION) {
+    for (int i = 0; i < 3; ++i)
+      m->vel[i] = m->ms_amp[i] *
+                  sin(2. * ORC_PI * (time - m->t0) / m->ms_period[i] + m->ms_phase[i]) *
+                  m->unit[i];
+  } else if (m->type == ORC_MOTION_CONSTANT_ROTATION && !m->about_centre) {
+    st(m->vel, cross(ld(m->omega), sub(cg, ld(m->centre))));
+  }
+  return ld(m->vel);
+}
+
+/* angularVelocity (:375-381) and angularMotion (:449-455) */
+static v3 mot_omega(motion_t* m, double time, double dt) {
+  if (mot_completed(m, time, dt)) m->omega[0] = m->omega[1] = m->omega[2] = 0.;
+  return ld(m->omega);
+}
+static v3 mot_rotation(const motion_t* m, double subint) { return scl(ld(m->omega), subint); }
+
+/* rotation vector -> quaternion (sin(d/2)/d rota, cos(d/2)), identity for 0
+   (ObstacleKinematicsVelocity.cpp:104-110) */
+static quat rot_quat(v3 rota) {
+  double d = norm(rota);
+  if (d != 0.) return qmk(scl(rota, sin(d / 2.) / d), cos(d / 2.));
+  return qmk(mk(0., 0., 0.), 1.);
+}
+
+/* translationalMotion (ObstacleImposedVelocity.cpp:387-443) */
+static v3 mot_translation(const motion_t* m, double time, double dt, double subint, v3 cg) {
+  double ti = time - dt < m->t0 ? m->t0 : time - dt;
+  double te = time > m->t1 ? m->t1 : time;
+  v3 tr = mk(0., 0., 0.);
+  if (m->type == ORC_MOTION_CONSTANT_TRANSLATION) {
+    tr = scl(ld(m->vel), subint);
+  } else if (m->type == ORC_MOTION_CRENEL_TRANSLATION) {
+    double a = m->amp * ((int)((0.5 * (ti + te) - m->t0) / m->period) % 2 == 0 ? 1 : -1) * subint;
+    tr = scl(ld(m->unit), a);
+  } else if (m->type == ORC_MOTION_SIN_TRANSLATION) {
+    double a = (m->amp * m->period / (2. * ORC_PI)) *
+               (cos(2. * ORC_PI * (ti - m->t0) / m->period + m->phase) -
+                cos(2. * ORC_PI * (te - m->t0) / m->period + m->phase));
+    tr = scl(ld(m->unit), a);
+  } else if (m->type == ORC_MOTION_MULTISIN_TRANSLATION) {
+    double c[3];
+    for (int i = 0; i < 3; ++i)
+      c[i] = (m->ms_amp[i] * m->ms_period[i] / (2. * ORC_PI)) *
+             (cos(2. * ORC_PI * (ti - m->t0) / m->ms_period[i] + m->ms_phase[i]) -
+              cos(2. * ORC_PI * (te - m->t0) / m->ms_period[i] + m->ms_phase[i])) *
+             m->unit[i];
+    tr = ld(c);
+  } else if (m->type == ORC_MOTION_CONSTANT_ROTATION && !m->about_centre) {
+    quat q = rot_quat(mot_rotation(m, subint));
+    v3 lever = sub(cg, ld(m->centre));
+    /* Quaternion::rotateVector (Geometry/src/Quaternion.cpp:293-298) */
+    double nq = norm(q.v);
+    v3 r = add(add(scl(lever, q.w * q.w - nq * nq), scl(q.v, 2. * dot(lever, q.v))),
+               scl(cross(q.v, lever), 2. * q.w));
+    tr = sub(r, lever);
+  }
+  return tr;
+}
+
+/* ObstacleKinematicsVelocity::ImposedMotion (ObstacleKinematicsVelocity.cpp:
+   86-122) and SimpleObstacle::Move (Component/src/SimpleObstacle.cpp:116-175)
+   of wall w: translation, rotation of the basis about the centre, new
+   bounding box, velocities of the kinematics when it moved. */
+static void move_wall(orc_sys* s, int w, double time, double dt, int displace) {
+  wall_t* wl = &s->walls[w];
+  v3 cg = ld(wl->origin), depl = mk(0., 0., 0.), rota = mk(0., 0., 0.);
+  v3 vt = mk(0., 0., 0.), vr = mk(0., 0., 0.);
+  double subint = 0.;
+  for (int i = 0; i < s->nmot; ++i) {
+    motion_t* m = &s->mot[i];
+    if (m->wall != w || !mot_active(m, time - dt, time, dt, &subint)) continue;
+    depl = add(depl, mot_translation(m, time, dt, subint, cg));
+    rota = add(rota, mot_rotation(m, subint));
+    vt = add(vt, mot_velocity(m, time, dt, cg));
+    vr = add(vr, mot_omega(m, time, dt));
+  }
+  quat qrot = rot_quat(rota);
+  if (!(norm(depl) != 0. || norm(rota) != 0.)) return;
+  if (displace) {
+    /* composeLeftByTranslation (Transform.cpp:300-304), then
+       RigidBody::Rotate with qrot * identity (RigidBody.cpp:270-273) */
+    quat q = qmul(qrot, qmk(mk(0., 0., 0.), 1.));
+    double basis[9], origin[3], ext[3], crust = wl->crust;
+    int material = wl->material;
+    memcpy(basis, wl->basis, sizeof(basis));
+    memcpy(ext, wl->ext, sizeof(ext));
+    origin[0] = wl->origin[0] + depl.x;
+    origin[1] = wl->origin[1] + depl.y;
+    origin[2] = wl->origin[2] + depl.z;
+    compose_left_rotation(basis, q);
+    build_wall(wl, basis, origin, ext, crust, material);
+  }
+  /* Obstacle::setVelocity (Obstacle.cpp:635-640) */
+  st(wl->vel, vt);
+  st(wl->omega, vr);
+}
+
+/* Obstacle::resetKinematics (Obstacle.cpp:160-167) of every wall */
+static void reset_walls(orc_sys* s) {
+  for (int w = 0; w < s->nwalls; ++w)
+    for (int d = 0; d < 3; ++d) s->walls[w].vel[d] = s->walls[w].omega[d] = 0.;
+}
+
+static int any_live(const orc_sys* s) {
+  for (int i = 0; i < s->nmot; ++i)
+    if (s->mot[i].live) return 1;
+  return 0;
+}
+
+static void drop_completed(orc_sys* s, double time, double dt) {
+  for (int i = 0; i < s->nmot; ++i)
+    if (s->mot[i].live && mot_completed(&s->mot[i], time, dt)) s->mot[i].live = 0;
+}
+
+/* obstacle part of AllComponents::Move (Component/src/AllComponents.cpp:
+   151-220) at `time`, over [time - dt, time] */
+int orc_move_walls(orc_sys* s, double time, double dt) {
+  s->time = time;
+  if (!any_live(s)) return ORC_OK;
+  int anyactive = 0;
+  double subint;
+  for (int i = 0; i < s->nmot; ++i)
+    if (s->mot[i].live && mot_active(&s->mot[i], time - dt, time, dt, &subint)) anyactive = 1;
+  if (anyactive || s->active_prev) reset_walls(s);
+  s->active_prev = anyactive;
+  if (anyactive)
+    for (int w = 0; w < s->nwalls; ++w) move_wall(s, w, time, dt, 1);
+  drop_completed(s, time, dt);
+  return ORC_OK;
+}
+
+/* AllComponents::setKinematicsObstacleWithoutMoving (AllComponents.cpp:235-251) */
+static void init_wall_kinematics(orc_sys* s, double time, double dt) {
+  if (!any_live(s)) return;
+  reset_walls(s);
+  for (int w = 0; w < s->nwalls; ++w) move_wall(s, w, time, dt, 0);
+  drop_completed(s, time, dt);
+}
+
+int orc_set_time(orc_sys* s, double time) {
+  s->time = time;
+  return ORC_OK;
+}
+
+double orc_get_time(const orc_sys* s) { return s->time; }
+
+int orc_get_wall(const orc_sys* s, int w, double* centre, double* basis, double* vel,
+                 double* omega) {
+  if (w < 0 || w >= s->nwalls) return ORC_ERR_ARG;
+  const wall_t* wl = &s->walls[w];
+  if (centre) memcpy(centre, wl->origin, sizeof(wl->origin));
+  if (basis) memcpy(basis, wl->basis, sizeof(wl->basis));
+  if (vel) memcpy(vel, wl->vel, sizeof(wl->vel));
+  if (omega) memcpy(omega, wl->omega, sizeof(wl->omega));
   return ORC_OK;
 }
 
@@ -803,8 +1070,9 @@ static int interact(orc_sys* s, int p0, int p1, v3 shift) {
 
 /* SimpleObstacle::InterAction (Component/src/SimpleObstacle.cpp:222-271):
    BBox::InZone pre-test (BBox.cpp:166-171), sphere-box closest point, force
-   with the particle as first body and a fixed obstacle (mass 1e20,
-   Component.cpp:16, zero velocity, Obstacle.cpp:128-133) as second body. */
+   with the particle as first body and the obstacle (mass 1e20,
+   Component.cpp:16, velocity at a point vel + omega x (point - centre),
+   Obstacle.cpp:128-133) as second body. */
 static int interact_wall(orc_sys* s, int p, int wi) {
   const wall_t* wl = &s->walls[wi];
   const double* xp = s->x + 3 * p;
@@ -820,7 +1088,7 @@ static int interact_wall(orc_sys* s, int p, int wi) {
   if (rc == 0 || !(dist < 0.)) return ORC_OK;
   const model_t* m = &s->models[s->mat[p]][wl->material];
   if (m->kind == ORC_MODEL_NONE) return set_error(s, ORC_ERR_ARG, p, -1 - wi, 0., 0.);
-  v3 zero = mk(0., 0., 0.);
+  v3 wv = ld(wl->vel), ww = ld(wl->omega);
   v3 f, delM = mk(0., 0., 0.);
   int rolling = 0;
   if (m->kind >= ORC_MODEL_HOOKE_MEMORY) {
@@ -830,12 +1098,12 @@ static int interact_wall(orc_sys* s, int p, int wi) {
     mem_t* e = get_memory(s, p, -1 - wi, &existed);
     if (!e) return set_error(s, ORC_ERR_ARG, p, -1 - wi, 0., 0.);
     f = memory_force(m, pt, ov, dist, A, ld(s->v + 3 * p), ld(s->w + 3 * p), s->mass[p],
-                     s->req[p], ld(wl->origin), zero, zero, 1.e20, wl->req, s->dt_mem, existed,
+                     s->req[p], ld(wl->origin), wv, ww, 1.e20, wl->req, s->dt_mem, existed,
                      e->kd, e->nrm, e->spr, &delM);
     rolling = rolling_model(m);
   } else {
     f = contact_force(m, pt, ov, dist, A, ld(s->v + 3 * p), ld(s->w + 3 * p), s->mass[p],
-                      s->req[p], ld(wl->origin), zero, zero, 1.e20, wl->req);
+                      s->req[p], ld(wl->origin), wv, ww, 1.e20, wl->req);
   }
   st(s->F + 3 * p, add(ld(s->F + 3 * p), f));
   st(s->T + 3 * p, add(ld(s->T + 3 * p), cross(sub(pt, A), f)));
@@ -1020,16 +1288,22 @@ int orc_advance_velocity(orc_sys* s, double dtv) {
    linked at insertion, then a0 is computed for the leap-frog (:169-174). */
 int orc_prime(orc_sys* s) {
   s->err_code = ORC_OK;
+  /* initial obstacle kinematics (Grains.cpp:123-124) */
+  if (s->dt_mem > 0.) init_wall_kinematics(s, s->time, s->dt_mem);
   int rc = orc_link(s);
   if (rc) return rc;
   return orc_compute_forces(s);
 }
 
-/* Grains::Simulation kick-drift-kick body (Base/src/Grains.cpp:344-364) */
+/* Grains::Simulation kick-drift-kick body (Base/src/Grains.cpp:290, 344-364):
+   time += dt, particles then walls move (AllComponents::Move), forces, kick */
 int orc_step(orc_sys* s, int nsteps, double dt) {
   s->dt_mem = dt;
   for (int k = 0; k < nsteps; ++k) {
+    s->time += dt;
     int rc = orc_move(s, 0.5 * dt, dt);
+    if (rc) return rc;
+    rc = orc_move_walls(s, s->time, dt);
     if (rc) return rc;
     rc = orc_link_update(s);
     if (rc) return rc;

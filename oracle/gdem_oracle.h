@@ -75,6 +75,24 @@ int orc_add_wall(orc_sys* s, const double* basis, const double* centre,
    kind HERTZ_MEMORY: p = {Es, en, Gs, muc, mur, etarpf, epst} */
 int orc_set_contact_model(orc_sys* s, int mat_a, int mat_b, int kind,
                           const double* p);
+/* Imposed wall motions (ObstacleImposedVelocity); p as for the library:
+   1 constant {vx,vy,vz}; 2 sin {dx,dy,dz,amp,period,phase_deg};
+   3 crenel {dx,dy,dz,amp,period}; 4 multisin {dx,dy,dz,ax,ay,az,px,py,pz,
+   phix,phiy,phiz}; 5 constant rotation {wx,wy,wz,cx,cy,cz,about_centre}. */
+#define ORC_MOTION_CONSTANT_TRANSLATION 1
+#define ORC_MOTION_SIN_TRANSLATION 2
+#define ORC_MOTION_CRENEL_TRANSLATION 3
+#define ORC_MOTION_MULTISIN_TRANSLATION 4
+#define ORC_MOTION_CONSTANT_ROTATION 5
+#define ORC_MAX_MOTIONS 64
+int orc_add_wall_motion(orc_sys* s, int wall, int type, double t_start, double t_end,
+                        const double* p);
+/* time (orc_step advances it); walls moved over [time - dt, time] */
+int orc_set_time(orc_sys* s, double time);
+double orc_get_time(const orc_sys* s);
+int orc_move_walls(orc_sys* s, double time, double dt);
+int orc_get_wall(const orc_sys* s, int wall, double* centre, double* basis, double* vel,
+                 double* omega);
 /* dt of the force evaluations (memory models); orc_step sets it too. */
 int orc_set_time_step(orc_sys* s, double dt);
 /* Contact memory of particle p (its own map): up to cap entries of key

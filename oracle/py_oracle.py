@@ -52,6 +52,11 @@ def load():
             "orc_add_wall": (i32, [vp, vp, vp, vp, d, i32]),
             "orc_set_contact_model": (i32, [vp, i32, i32, i32, vp]),
             "orc_set_time_step": (i32, [vp, d]),
+            "orc_add_wall_motion": (i32, [vp, i32, i32, d, d, vp]),
+            "orc_set_time": (i32, [vp, d]),
+            "orc_get_time": (d, [vp]),
+            "orc_move_walls": (i32, [vp, d, d]),
+            "orc_get_wall": (i32, [vp, i32, vp, vp, vp, vp]),
             "orc_get_memory": (i32, [vp, i32, i32, vp, vp, vp, vp]),
             "orc_rotate_two_vectors": (None, [vp, vp, vp, vp]),
             "orc_link": (i32, [vp]),
@@ -120,6 +125,10 @@ class Oracle:
                 _f(p["radius"], (n,)), _f(p["density"], (n,)), _f(p["crust"], (n,))]
         mat = np.ascontiguousarray(p["material"], dtype=np.int32)
         self._chk(self.lib.orc_set_particles(h, n, *[_p(a) for a in keep], _p(mat)), "particles")
+        if "time" in wl:
+            self.lib.orc_set_time(h, float(wl["time"]))
+        for (wall, kind, t0, t1, prm) in wl.get("motions", []):
+            self.add_wall_motion(wall, kind, t0, t1, prm)
         if log_capacity:
             self.lib.orc_enable_contact_log(h, int(log_capacity))
 
@@ -155,6 +164,22 @@ class Oracle:
 
     def advance_velocity(self, dtv):
         self._chk(self.lib.orc_advance_velocity(self.h, dtv), "advance_velocity")
+
+    def add_wall_motion(self, wall, kind, t0, t1, prm):
+        npar = {1: 3, 2: 6, 3: 5, 4: 12, 5: 7}[int(kind)]
+        self._chk(self.lib.orc_add_wall_motion(self.h, int(wall), int(kind), float(t0), float(t1),
+                                               _p(_f(prm, (npar,)))), "wall motion")
+
+    def move_walls(self, time, dt):
+        self._chk(self.lib.orc_move_walls(self.h, float(time), float(dt)), "move walls")
+
+    def time(self):
+        return self.lib.orc_get_time(self.h)
+
+    def wall(self, k):
+        c, b, v, w = np.empty(3), np.empty(9), np.empty(3), np.empty(3)
+        self._chk(self.lib.orc_get_wall(self.h, int(k), _p(c), _p(b), _p(v), _p(w)), "wall")
+        return c, b, v, w
 
     def set_time_step(self, dt):
         self._chk(self.lib.orc_set_time_step(self.h, float(dt)), "time step")

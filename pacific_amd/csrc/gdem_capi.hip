@@ -17,6 +17,7 @@
 #include "../../include/gdem.h"
 #include "gdem_host.h"
 #include "gdem_internal.h"
+#include "gdem_wall_motion.h"
 
 #pragma clang fp contract(off)
 
@@ -95,6 +96,15 @@ struct gdem_ctx {
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   double graph_dt[2] = {0., 0.};
   int64_t graph_made[2] = {-1, -1};
+  // simulation time and imposed wall motions; while motions are pending,
+  // gdem_step computes the walls of each of its steps on the host
+  // (h_traj) and each force evaluation reads its own copy (step_walls)
+  double time = 0.;
+  WallKinematics kin;
+  std::vector<Wall> h_traj;
+  Wall* d_traj = nullptr;
+  size_t traj_cap = 0;
+  const Wall* step_walls = nullptr;
 };
 
 #define HIP_TRY(expr)                                                        \
@@ -213,7 +223,7 @@ extern "C" int gdem_destroy(gdem_ctx* c) {
   (void)hipFree(c->d_cand); (void)hipFree(c->d_crec); (void)hipFree(c->d_mask); (void)hipFree(c->d_work);
   (void)hipFree(c->d_nwork); (void)hipFree(c->d_shapes);
   (void)hipFree(c->d_skin); (void)hipFree(c->d_xref); (void)hipFree(c->d_cmask);
-  (void)hipFree(c->d_wall_shapes); (void)hipFree(c->d_wall_sc);
+  (void)hipFree(c->d_wall_shapes); (void)hipFree(c->d_wall_sc); (void)hipFree(c->d_traj);
   for (auto& e : c->ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
   for (auto& g : c->graph_exec)
     if (g) (void)hipGraphExecDestroy(g);
@@ -821,7 +831,7 @@ static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wr
     a.grid = c->grid;
     a.models = c->d_models;
     a.model_stride = kMaxMaterials;
-    a.walls = c->d_walls;
+    a.walls = c->step_walls ? c->step_walls : c->d_walls;
     a.nwalls = c->nwalls;
     a.n = (int32_t)c->n;
     for (int dd = 0; dd < 3; ++dd) a.gravity[dd] = c->cfg.gravity[dd];
@@ -969,15 +979,72 @@ extern "C" int gdem_advance_velocity(gdem_ctx* c, double dtv) {
   return GDEM_OK;
 }
 
+static int upload_walls(gdem_ctx* c) {
+  HIP_TRY(hipMemcpyAsync(c->d_walls, c->h_walls, sizeof(Wall) * c->nwalls, hipMemcpyHostToDevice,
+                         c->stream));
+  // h_walls changes on the next move: the copy has to be done by then
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GDEM_OK;
+}
+
 extern "C" int gdem_prime(gdem_ctx* c) {
   if (!c) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
+  // initial wall velocities (Grains::do_before_time_stepping,
+  // Base/src/Grains.cpp:123-124)
+  if (c->dt_mem > 0. && c->kin.init(c->h_walls, c->nwalls, c->time, c->dt_mem)) {
+    const int ru = upload_walls(c);
+    if (ru) return ru;
+  }
   int rc = do_binning(c);
   if (rc) return rc;
   rc = do_force(c, MODE_FORCE, 0., 0., true);
   if (rc) return rc;
   c->primed = true;
   return GDEM_OK;
+}
+
+// gdem_step while wall motions are pending (the fused step continues after
+// the first kick + drift): the walls of steps k = 1..nsteps, moved over
+// [t_k - dt, t_k] as AllComponents::Move does after the particles
+// (Component/src/AllComponents.cpp:151-220), are computed on the host in
+// chunks and uploaded; force evaluation k reads walls at t_k.  No graph
+// replay (each evaluation reads its own wall array).
+static int moving_wall_steps(gdem_ctx* c, int32_t nsteps, double dt) {
+  const int nw = std::max(c->nwalls, 1);
+  const int32_t chunk = (int32_t)std::max<size_t>(1, (8u << 20) / (sizeof(Wall) * (size_t)nw));
+  for (int32_t k0 = 0; k0 < nsteps; k0 += chunk) {
+    const int32_t m = std::min(chunk, nsteps - k0);
+    const size_t need = (size_t)m * (size_t)nw;
+    // the previous chunk's kernels read d_traj and its copy reads h_traj
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->traj_cap < need) {
+      (void)hipFree(c->d_traj);
+      c->d_traj = nullptr;
+      c->traj_cap = 0;
+      HIP_TRY(hipMalloc(&c->d_traj, sizeof(Wall) * need));
+      c->traj_cap = need;
+    }
+    c->h_traj.resize(need);
+    double t = c->time;
+    for (int32_t k = 0; k < m; ++k) {
+      t += dt;
+      c->kin.move(c->h_walls, c->nwalls, t, dt);
+      std::copy(c->h_walls, c->h_walls + c->nwalls, c->h_traj.begin() + (size_t)k * nw);
+    }
+    if (c->nwalls > 0)
+      HIP_TRY(hipMemcpyAsync(c->d_traj, c->h_traj.data(), sizeof(Wall) * need,
+                             hipMemcpyHostToDevice, c->stream));
+    for (int32_t k = 0; k < m; ++k) {
+      c->time += dt;
+      c->step_walls = c->nwalls > 0 ? c->d_traj + (size_t)k * nw : nullptr;
+      const bool last = (k0 + k == nsteps - 1);
+      const int rc = do_force(c, last ? MODE_KICK : MODE_MOVE, 0.5 * dt, dt, last);
+      c->step_walls = nullptr;
+      if (rc) return rc;
+    }
+  }
+  return upload_walls(c);
 }
 
 extern "C" int gdem_step(gdem_ctx* c, int32_t nsteps, double dt) {
@@ -994,8 +1061,10 @@ extern "C" int gdem_step(gdem_ctx* c, int32_t nsteps, double dt) {
   // opens the next one
   rc = gdem_move(c, 0.5 * dt, dt);
   if (rc) return rc;
+  if (c->kin.any_live()) return moving_wall_steps(c, nsteps, dt);
   const bool graph = c->use_graph && !c->timing && c->rec_cap == 0;
   for (int32_t k = 0; k < nsteps; ++k) {
+    c->time += dt;
     const bool last = (k == nsteps - 1);
     if (!last && graph) rc = graph_force(c, dt);
     else rc = do_force(c, last ? MODE_KICK : MODE_MOVE, 0.5 * dt, dt, last);
@@ -1215,6 +1284,55 @@ extern "C" int gdem_info(gdem_ctx* c, int64_t* n, int32_t* nwalls, int32_t* conv
   if (nwalls) *nwalls = c->nwalls;
   if (convex) *convex = c->convex ? 1 : 0;
   if (memory_models) *memory_models = c->has_memory ? 1 : 0;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_add_wall_motion(gdem_ctx* c, int32_t wall, int32_t type, double t_start,
+                                    double t_end, const double* p) {
+  if (!c || !p || wall < 0 || wall >= c->nwalls || c->convex) return GDEM_ERR_ARG;
+  if (!(t_end >= t_start)) return GDEM_ERR_ARG;
+  WallMotion m;
+  if (!make_wall_motion(wall, type, t_start, t_end, p, m)) return GDEM_ERR_ARG;
+  if ((type == GDEM_MOTION_SIN_TRANSLATION || type == GDEM_MOTION_CRENEL_TRANSLATION) &&
+      !(p[4] > 0.))
+    return GDEM_ERR_ARG;
+  if (type == GDEM_MOTION_MULTISIN_TRANSLATION && !(p[6] > 0. && p[7] > 0. && p[8] > 0.))
+    return GDEM_ERR_ARG;
+  if (type >= GDEM_MOTION_SIN_TRANSLATION && type <= GDEM_MOTION_MULTISIN_TRANSLATION &&
+      !(norm(mk(p[0], p[1], p[2])) > 0.))
+    return GDEM_ERR_ARG;
+  c->kin.add(m);
+  return GDEM_OK;
+}
+
+extern "C" int gdem_set_time(gdem_ctx* c, double time) {
+  if (!c) return GDEM_ERR_ARG;
+  c->time = time;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_get_time(gdem_ctx* c, double* time) {
+  if (!c || !time) return GDEM_ERR_ARG;
+  *time = c->time;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_move_walls(gdem_ctx* c, double time, double dt) {
+  if (!c || !(dt > 0.)) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  c->time = time;
+  if (c->kin.move(c->h_walls, c->nwalls, time, dt)) return upload_walls(c);
+  return GDEM_OK;
+}
+
+extern "C" int gdem_get_wall(gdem_ctx* c, int32_t wall, double* centre, double* basis,
+                             double* vel, double* omega) {
+  if (!c || wall < 0 || wall >= c->nwalls) return GDEM_ERR_ARG;
+  const Wall& wl = c->h_walls[wall];
+  if (centre) memcpy(centre, wl.origin, sizeof(wl.origin));
+  if (basis) memcpy(basis, wl.basis, sizeof(wl.basis));
+  if (vel) memcpy(vel, wl.vel, sizeof(wl.vel));
+  if (omega) memcpy(omega, wl.omega, sizeof(wl.omega));
   return GDEM_OK;
 }
 

@@ -99,6 +99,9 @@ struct Wall {
   double inv_basis[9], inv_origin[3];
   double bb_centre[3], bb_ext[3];
   double crust, req;
+  // velocity of an imposed motion (Obstacle::getVelocityAtPoint,
+  // Component/src/Obstacle.cpp:128-133: vel + omega x (p - origin))
+  double vel[3], omega[3];
   int32_t material, pad;
 };
 

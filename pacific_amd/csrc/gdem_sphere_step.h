@@ -420,20 +420,23 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
     }
     if (rc == 0 || !(dist < 0.)) continue;
     const Model md = a.models[R.ax.mat * a.model_stride + wl.material];
-    const v3 zero = mk(0., 0., 0.);
+    // the wall's velocity at the contact point is vel + omega x (c - origin)
+    // (Obstacle::getVelocityAtPoint, Component/src/Obstacle.cpp:128-133)
     const v3 wo = mk(wl.origin[0], wl.origin[1], wl.origin[2]);
+    const v3 wv = mk(wl.vel[0], wl.vel[1], wl.vel[2]);
+    const v3 ww = mk(wl.omega[0], wl.omega[1], wl.omega[2]);
     v3 f;
     if (MEM && md.kind >= 3) {
       // the particle is the first body (SimpleObstacle.cpp:262-264); the
       // wall's own copy of the memory is never read back
       v3 delM;
-      f = R.memory_force(md, -1 - wi, true, c, ov, dist, R.X, R.V, R.W, R.m, R.req, wo, zero,
-                         zero, 1.e20, wl.req, delM);
+      f = R.memory_force(md, -1 - wi, true, c, ov, dist, R.X, R.V, R.W, R.m, R.req, wo, wv, ww,
+                         1.e20, wl.req, delM);
       R.F = add(R.F, f);
       R.T = add(R.T, cross(sub(c, R.X), f));
       if (md.p[md.kind == 3 ? 5 : 4] != 0.) R.T = add(R.T, delM);
     } else {
-      f = contact_force(md, c, ov, dist, R.X, R.V, R.W, R.m, R.req, wo, zero, zero, 1.e20, wl.req);
+      f = contact_force(md, c, ov, dist, R.X, R.V, R.W, R.m, R.req, wo, wv, ww, 1.e20, wl.req);
       R.F = add(R.F, f);
       R.T = add(R.T, cross(sub(c, R.X), f));
     }

@@ -4,12 +4,15 @@
 //
 //   grains_run <input.bin> <output.bin>
 //
-// Input (little endian): "GDEMIN01" or "GDEMIN02", int32 n, nwalls, nmodels,
-// nsteps, double dt, origin[3], size[3], int32 periodic[3], double cell_size,
-// gravity[3]; per model int32 kind, mat_a, mat_b, double p[5] (GDEMIN01) or
-// p[8] (GDEMIN02: the memory models' parameters, include/gdem.h); per wall
-// double basis[9], centre[3], half_extent[3], crust, int32 material; then
-// x[3n], v[3n], w[3n], q[4n], radius[n], density[n], crust[n], int32 material[n].
+// Input (little endian): "GDEMIN01", "GDEMIN02" or "GDEMIN03", int32 n, nwalls,
+// nmodels, nsteps, double dt, origin[3], size[3], int32 periodic[3], double
+// cell_size, gravity[3]; per model int32 kind, mat_a, mat_b, double p[5]
+// (GDEMIN01) or p[8] (GDEMIN02/03: the memory models' parameters,
+// include/gdem.h); per wall double basis[9], centre[3], half_extent[3], crust,
+// int32 material; GDEMIN03 then: int32 nmotions, per motion int32 wall, type
+// (GDEM_MOTION_*), double t_start, t_end, p[12] (gdem_add_wall_motion's
+// parameters); then x[3n], v[3n], w[3n], q[4n], radius[n], density[n], crust[n],
+// int32 material[n].
 // Output: "GDEMOUT1", int32 n, status (0 or the GDEM_ERR_* the step raised),
 // id0, id1, then x, v, w, q, F, T of the final state.
 #include <cstdio>
@@ -36,9 +39,11 @@ int main(int argc, char** argv) {
   char magic[8];
   int32_t n, nwalls, nmodels, nsteps, periodic[3];
   double dt, origin[3], size[3], cell, grav[3];
-  if (!rd(f, magic, 8) || (memcmp(magic, "GDEMIN01", 8) && memcmp(magic, "GDEMIN02", 8)))
+  if (!rd(f, magic, 8) || (memcmp(magic, "GDEMIN01", 8) && memcmp(magic, "GDEMIN02", 8) &&
+                            memcmp(magic, "GDEMIN03", 8)))
     return 2;
-  const int np = memcmp(magic, "GDEMIN02", 8) == 0 ? 8 : 5;
+  const bool with_motions = memcmp(magic, "GDEMIN03", 8) == 0;
+  const int np = memcmp(magic, "GDEMIN01", 8) == 0 ? 5 : 8;
   if (!rd(f, &n, 1) || !rd(f, &nwalls, 1) ||
       !rd(f, &nmodels, 1) || !rd(f, &nsteps, 1) || !rd(f, &dt, 1) || !rd(f, origin, 3) ||
       !rd(f, size, 3) || !rd(f, periodic, 3) || !rd(f, &cell, 1) || !rd(f, grav, 3))
@@ -83,7 +88,37 @@ int main(int argc, char** argv) {
       return 2;
     for (int d = 0; d < 3; ++d) w.edges[d] = 2. * half[d];
     w.material = "mat" + std::to_string(mat);
-    w.name = "wall";
+    w.name = "wall" + std::to_string(&w - walls.data());
+  }
+  // imposed velocities as the reference's XML nodes name their parameters
+  std::vector<ObstacleImposedVelocity> motions;
+  int32_t nmotions = 0;
+  if (with_motions && !rd(f, &nmotions, 1)) return 2;
+  static const char* const kNames[6][12] = {
+      {},
+      {"VX", "VY", "VZ"},
+      {"VX", "VY", "VZ", "Amplitude", "Period", "PhaseShift"},
+      {"VX", "VY", "VZ", "Amplitude", "Period"},
+      {"VX", "VY", "VZ", "AX", "AY", "AZ", "PX", "PY", "PZ", "PhiX", "PhiY", "PhiZ"},
+      {"WX", "WY", "WZ", "CX", "CY", "CZ"}};
+  static const char* const kTypes[6] = {"", "ConstantTranslation", "SinTranslation",
+                                        "CrenelTranslation", "MultiSinTranslation",
+                                        "ConstantRotation"};
+  for (int32_t k = 0; k < nmotions; ++k) {
+    int32_t wall, type;
+    double t[2], p[12];
+    if (!rd(f, &wall, 1) || !rd(f, &type, 1) || !rd(f, t, 2) || !rd(f, p, 12)) return 2;
+    if (type < 1 || type > 5) return 2;
+    ObstacleImposedVelocity mo;
+    mo.obstacle = "wall" + std::to_string(wall);
+    mo.type = kTypes[type];
+    mo.tstart = t[0];
+    mo.tend = t[1];
+    // a rotation about the obstacle's centre has no CX, CY, CZ (p[6] != 0)
+    static const int kCount[5] = {0, 3, 6, 5, 12};
+    const int np_k = type == 5 ? (p[6] != 0. ? 3 : 6) : kCount[type];
+    for (int i = 0; i < np_k; ++i) mo.parameters[kNames[type][i]] = p[i];
+    motions.push_back(mo);
   }
   std::vector<double> x(3 * n), v(3 * n), w(3 * n), q(4 * n), r(n), rho(n), cr(n);
   std::vector<int32_t> mat(n);
@@ -117,6 +152,7 @@ int main(int argc, char** argv) {
   std::unique_ptr<TimeIntegrator> ti(TimeIntegratorBuilderFactory::create());
   Grains app(lc, *ti);
   int32_t status = 0, id0 = -1, id1 = -1;
+  for (auto const& mo : motions) app.LinkImposedMotion(mo);
   try {
     app.setTimeStep(dt);  // the input's time step, before a0 (memory models use it)
     app.do_before_time_stepping();

@@ -319,7 +319,13 @@ void LinkedCell::Link(BoxObstacle const& ob) {
   if (gdem_add_wall(m_ctx, ob.basis, ob.centre, half, ob.crust,
                     ContactBuilderFactory::materialIndex(ob.material)))
     throw std::invalid_argument("obstacle " + ob.name + " rejected");
+  m_obstacles.emplace(ob.name, m_nobstacles++);
   push_models();
+}
+
+int LinkedCell::obstacleIndex(std::string const& name) const {
+  auto it = m_obstacles.find(name);
+  return it == m_obstacles.end() ? -1 : it->second;
 }
 
 void LinkedCell::LinkUpdate(double, double) {
@@ -364,8 +370,51 @@ void Grains::Simulation(double time_interval, double dt) {
   if (!m_primed) do_before_time_stepping();
   const int32_t nsteps = (int32_t)llround(time_interval / dt);
   check(gdem_step(m_lc.context(), nsteps, dt));
-  m_time += nsteps * dt;
   check(gdem_synchronize(m_lc.context()));
+  check(gdem_get_time(m_lc.context(), &m_time));  // m_time += dt at every step
+}
+
+void Grains::setTime(double time) {
+  m_time = time;
+  check(gdem_set_time(m_lc.context(), time));
+}
+
+// The parameters of the reference's XML node, in the order gdem_add_wall_motion
+// takes them; a rotation about a given point when CX is present.
+void Grains::LinkImposedMotion(ObstacleImposedVelocity const& mo) {
+  const int wall = m_lc.obstacleIndex(mo.obstacle);
+  if (wall < 0) throw std::invalid_argument("no obstacle named " + mo.obstacle);
+  auto get = [&](const char* k) {
+    auto it = mo.parameters.find(k);
+    if (it == mo.parameters.end())
+      throw std::invalid_argument(mo.type + " of " + mo.obstacle + ": no " + k);
+    return it->second;
+  };
+  std::vector<double> p;
+  int32_t type = 0;
+  if (mo.type == "ConstantTranslation") {
+    type = GDEM_MOTION_CONSTANT_TRANSLATION;
+    p = {get("VX"), get("VY"), get("VZ")};
+  } else if (mo.type == "SinTranslation") {
+    type = GDEM_MOTION_SIN_TRANSLATION;
+    p = {get("VX"), get("VY"), get("VZ"), get("Amplitude"), get("Period"), get("PhaseShift")};
+  } else if (mo.type == "CrenelTranslation") {
+    type = GDEM_MOTION_CRENEL_TRANSLATION;
+    p = {get("VX"), get("VY"), get("VZ"), get("Amplitude"), get("Period")};
+  } else if (mo.type == "MultiSinTranslation") {
+    type = GDEM_MOTION_MULTISIN_TRANSLATION;
+    p = {get("VX"), get("VY"), get("VZ"), get("AX"), get("AY"), get("AZ"),
+         get("PX"), get("PY"), get("PZ"), get("PhiX"), get("PhiY"), get("PhiZ")};
+  } else if (mo.type == "ConstantRotation") {
+    type = GDEM_MOTION_CONSTANT_ROTATION;
+    const bool centre = mo.parameters.count("CX") != 0;
+    p = {get("WX"), get("WY"), get("WZ"), centre ? get("CX") : 0., centre ? get("CY") : 0.,
+         centre ? get("CZ") : 0., centre ? 0. : 1.};
+  } else {
+    throw std::invalid_argument("imposed velocity type " + mo.type + " is not supported");
+  }
+  if (gdem_add_wall_motion(m_lc.context(), wall, type, mo.tstart, mo.tend, p.data()))
+    throw std::invalid_argument(mo.type + " of " + mo.obstacle + " rejected");
 }
 
 void Grains::saveParticles2014(std::string const& path, bool binary, int id_offset,
@@ -381,8 +430,9 @@ void Grains::readParticles2014(std::string const& path, bool binary, int id_offs
   m_primed = true;  // the stored F, T are a0
 }
 
-void Grains::moveParticlesAndObstacles(double dtv, double dtd) {
+void Grains::moveParticlesAndObstacles(double dtv, double dtd, double dto) {
   check(gdem_move(m_lc.context(), dtv, dtd));
+  if (dto > 0.) check(gdem_move_walls(m_lc.context(), m_time, dto));
   m_lc.LinkUpdate(m_time, dtd);
 }
 

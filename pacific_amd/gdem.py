@@ -24,6 +24,10 @@ MODEL_HOOKE, MODEL_HERTZ = 1, 2
 MODEL_HOOKE_MEMORY, MODEL_HERTZ_MEMORY = 3, 4
 # parameters per model kind (include/gdem.h)
 MODEL_NPARAMS = {MODEL_HOOKE: 5, MODEL_HERTZ: 5, MODEL_HOOKE_MEMORY: 8, MODEL_HERTZ_MEMORY: 7}
+# imposed wall motions (ObstacleImposedVelocity types) and their parameter counts
+MOTION_CONSTANT_TRANSLATION, MOTION_SIN_TRANSLATION, MOTION_CRENEL_TRANSLATION = 1, 2, 3
+MOTION_MULTISIN_TRANSLATION, MOTION_CONSTANT_ROTATION = 4, 5
+MOTION_NPARAMS = {1: 3, 2: 6, 3: 5, 4: 12, 5: 7}
 
 
 class GdemError(RuntimeError):
@@ -105,6 +109,11 @@ def load_library(path: str = LIB_PATH):
         "gdem_get_memory": (C.c_int, [vp, i64, P(i64), vp, vp, vp, vp, vp]),
         "gdem_set_memory": (C.c_int, [vp, i64, vp, vp, vp, vp, vp]),
         "gdem_info": (C.c_int, [vp, P(i64), P(i32), P(i32), P(i32)]),
+        "gdem_add_wall_motion": (C.c_int, [vp, i32, i32, d, d, vp]),
+        "gdem_set_time": (C.c_int, [vp, d]),
+        "gdem_get_time": (C.c_int, [vp, P(d)]),
+        "gdem_move_walls": (C.c_int, [vp, d, d]),
+        "gdem_get_wall": (C.c_int, [vp, i32, vp, vp, vp, vp]),
         "gdem_write_particles_2014": (C.c_int, [vp, C.c_char_p, i32, i32, i32, vp]),
         "gdem_read_particles_2014": (C.c_int, [vp, C.c_char_p, i32, i32, i32, vp]),
         "gdem_set_contact_recording": (C.c_int, [vp, i32]),
@@ -159,7 +168,8 @@ EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall"
             "gdem_move", "gdem_advance_velocity", "gdem_prime", "gdem_step", "gdem_set_graph",
             "gdem_set_skin", "gdem_get_skin",
             "gdem_synchronize", "gdem_get_state", "gdem_set_state", "gdem_get_memory",
-            "gdem_set_memory", "gdem_info", "gdem_write_particles_2014",
+            "gdem_set_memory", "gdem_info", "gdem_add_wall_motion", "gdem_set_time",
+            "gdem_get_time", "gdem_move_walls", "gdem_get_wall", "gdem_write_particles_2014",
             "gdem_read_particles_2014", "gdem_set_contact_recording",
             "gdem_get_contacts", "gdem_last_error", "gdem_set_kernel_timing",
             "gdem_get_counters", "gdem_reset_counters", "gdem_get_grid", "gdem_version",
@@ -276,6 +286,32 @@ class GdemSystem:
         self._check(self.lib.gdem_add_wall(self.h, _ptr(_f64(basis, (9,))), _ptr(_f64(centre, (3,))),
                                            _ptr(_f64(half_extent, (3,))), float(crust), int(material)),
                     "gdem_add_wall")
+
+    def add_wall_motion(self, wall, kind, t_start, t_end, params):
+        """Imposed velocity of a wall over [t_start, t_end] (MOTION_* kinds,
+        parameters as MOTION_NPARAMS counts them)."""
+        p = _f64(params, (MOTION_NPARAMS[int(kind)],))
+        self._check(self.lib.gdem_add_wall_motion(self.h, int(wall), int(kind), float(t_start),
+                                                  float(t_end), _ptr(p)), "gdem_add_wall_motion")
+
+    def set_time(self, t):
+        self._check(self.lib.gdem_set_time(self.h, float(t)), "gdem_set_time")
+
+    def get_time(self):
+        t = C.c_double()
+        self._check(self.lib.gdem_get_time(self.h, C.byref(t)), "gdem_get_time")
+        return t.value
+
+    def move_walls(self, time, dt):
+        """Moves the walls over [time - dt, time] (obstacle part of a step)."""
+        self._check(self.lib.gdem_move_walls(self.h, float(time), float(dt)), "gdem_move_walls")
+
+    def get_wall(self, k):
+        """(centre, row-major basis (9,), velocity, angular velocity) of wall k."""
+        c, b, v, w = np.empty(3), np.empty(9), np.empty(3), np.empty(3)
+        self._check(self.lib.gdem_get_wall(self.h, int(k), _ptr(c), _ptr(b), _ptr(v), _ptr(w)),
+                    "gdem_get_wall")
+        return c, b, v, w
 
     def set_contact_model(self, mat_a, mat_b, kind, params):
         p = _f64(params, (MODEL_NPARAMS.get(int(kind), 5),))
@@ -439,6 +475,10 @@ def build_system(wl, device=0):
                         p["material"])
     if "dt" in wl:
         s.set_time_step(wl["dt"])
+    if "time" in wl:
+        s.set_time(wl["time"])
+    for (wall, kind, t0, t1, prm) in wl.get("motions", []):
+        s.add_wall_motion(wall, kind, t0, t1, prm)
     return s
 
 

@@ -143,6 +143,48 @@ def sphere_on_wall(model, v0=(0.0, 0.0, 0.0), w0=(0.0, 0.0, 0.0), r=1.0e-3, dt=1
             | {"w": np.array([w0], dtype=float)}}
 
 
+def sphere_on_plate(model, half=(0.05, 0.05), x0=(0.0, 0.0, None), r=1.0e-3, dt=5.0e-6,
+                    gravity=-9.81, motions=()):
+    """One sphere (density 2500) on a square plate (top face z = 0, centred
+    on the z axis, half extents half x 1 mm), at the static overlap of a
+    Hooke model when x0[2] is None; motions = imposed plate motions
+    (wall 0) as (kind, t_start, t_end, params)."""
+    kind, prm = model
+    m = DENSITY * 4.0 * np.pi * r ** 3 / 3.0
+    delta = m * -gravity / prm[0] if kind in (MODEL_HOOKE, MODEL_HOOKE_MEMORY) else 0.0
+    z = r - delta if x0[2] is None else x0[2]
+    x = np.array([[x0[0], x0[1], z]])
+    hx, hy = half
+    wall = (IDENTITY, (0.0, 0.0, -1.0e-3), (hx, hy, 1.0e-3), CRUST, 1)
+    return {"name": "sphere_on_plate", "origin": (-hx, -hy, -2.0e-3), "size": (2 * hx, 2 * hy, 0.01),
+            "periodic": (0, 0, 0), "cell_size": 2.0 * r, "gravity": (0.0, 0.0, gravity), "dt": dt,
+            "models": [(0, 0, kind, prm), (0, 1, kind, prm)], "walls": [wall], "time": 0.0,
+            "motions": [(0,) + tuple(mo) for mo in motions],
+            "particles": _particles(x, np.zeros((1, 3)), np.array([r]))}
+
+
+def shaken_box(n=1000, seed=1, memory=False):
+    """plain_box (HookeMemory between particles and walls when memory) with an
+    imposed motion on five walls (wall k of box_walls: 2 d + (side > 0)):
+    floor sine in z (0.1 m/s, 5 ms), x-min crenel along +x (0.1 m/s, 2 ms),
+    x-max multi-sine inwards (0.2 m/s, 6 ms, phase 180 degrees: the
+    direction enters squared, so its sign alone would not turn it), y-min constant 0.05 m/s
+    along +y over [0.5, 2.5] ms, y-max rotating at 10 rad/s about x through
+    its centre.  The walls reach the lattice (50 um from the floor and x-min,
+    250 um from the others) within about a millisecond."""
+    wl = memory_box(n, seed) if memory else plain_box(n, seed)
+    wl["name"] = f"shaken_box_{n}" + ("_memory" if memory else "")
+    wl["time"] = 0.0
+    wl["motions"] = [
+        (4, 2, 0.0, 1.0, (0.0, 0.0, 1.0, 0.1, 5.0e-3, 0.0)),
+        (0, 3, 0.0, 1.0, (1.0, 0.0, 0.0, 0.1, 2.0e-3)),
+        (1, 4, 0.0, 1.0, (-1.0, 0.0, 0.0, 0.2, 0.0, 0.0, 6.0e-3, 1.0, 1.0, 180.0, 0.0, 0.0)),
+        (2, 1, 5.0e-4, 2.5e-3, (0.0, 0.05, 0.0)),
+        (3, 5, 0.0, 1.0, (10.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0)),
+    ]
+    return wl
+
+
 # HookeMemory {kn, en, etat, muc, kt, mur, etarpf, epst} with rolling friction
 HOOKE_MEMORY = (1.0e5, 0.9, -1.0, 0.5, 2.0e4, 0.02, 0.5, 0.0)
 # HertzMemory {Es, en, Gs, muc, mur, etarpf, epst}

@@ -22,8 +22,9 @@ def write_input(path, wl, nsteps):
     p = wl["particles"]
     n = len(p["radius"])
     memory = any(kind >= 3 for (_, _, kind, _) in wl["models"])
+    motions = wl.get("motions", [])
     with open(path, "wb") as f:
-        f.write(b"GDEMIN02" if memory else b"GDEMIN01")
+        f.write(b"GDEMIN03" if motions else b"GDEMIN02" if memory else b"GDEMIN01")
         f.write(struct.pack("<4i", n, len(wl["walls"]), len(wl["models"]), nsteps))
         f.write(struct.pack("<d", wl["dt"]))
         f.write(struct.pack("<3d", *wl["origin"]))
@@ -33,7 +34,7 @@ def write_input(path, wl, nsteps):
         f.write(struct.pack("<3d", *wl["gravity"]))
         for (a, b, kind, par) in wl["models"]:
             f.write(struct.pack("<3i", kind, a, b))
-            if memory:
+            if memory or motions:
                 f.write(struct.pack("<8d", *(list(par) + [0.0] * (8 - len(par)))))
             else:
                 f.write(struct.pack("<5d", *par))
@@ -43,6 +44,11 @@ def write_input(path, wl, nsteps):
             f.write(struct.pack("<3d", *half))
             f.write(struct.pack("<d", crust))
             f.write(struct.pack("<i", mat))
+        if motions:
+            f.write(struct.pack("<i", len(motions)))
+            for (wall, kind, t0, t1, prm) in motions:
+                f.write(struct.pack("<2i2d", wall, kind, t0, t1))
+                f.write(struct.pack("<12d", *(list(prm) + [0.0] * (12 - len(prm)))))
         for k in ("x", "v", "w", "q", "radius", "density", "crust"):
             f.write(np.ascontiguousarray(p[k], dtype="<f8").tobytes())
         f.write(np.ascontiguousarray(p["material"], dtype="<i4").tobytes())
@@ -98,6 +104,35 @@ def test_driver_memory_model_matches_oracle(oracle_mod):
     o = orc.state()
     for k in ("x", "v", "w", "q"):
         assert rel_err(g[k], o[k]) <= 1e-9, k
+
+
+def test_driver_wall_motions_match_oracle(oracle_mod):
+    """ObstacleImposedVelocity records through Grains::LinkImposedMotion (the
+    reference's XML parameter names), the shaken box with HookeMemory."""
+    print("[case] grains_run shaken box 150 steps", flush=True)
+    wl = workloads.shaken_box(1000, seed=1, memory=True)
+    wl["particles"]["radius"] = wl["particles"]["radius"] * 1.04
+    wl["cell_size"] = 2.0 * wl["particles"]["radius"].max()
+    status, _, _, g = run(wl, 150)
+    assert status == 0
+    orc = oracle_mod.Oracle(wl)
+    orc.prime()
+    orc.step(150, wl["dt"])
+    o = orc.state()
+    # the oracle's own one-ulp divergence sets the bar (branch thresholds of
+    # the memory model amplify rounding differences)
+    wl_u = workloads.shaken_box(1000, seed=1, memory=True)
+    wl_u["particles"]["radius"] = wl["particles"]["radius"]
+    wl_u["cell_size"] = wl["cell_size"]
+    v = wl_u["particles"]["v"]
+    up = np.random.default_rng(7).random(v.shape) < 0.5
+    wl_u["particles"]["v"] = np.nextafter(v, np.where(up, np.inf, -np.inf))
+    orc_u = oracle_mod.Oracle(wl_u)
+    orc_u.prime()
+    orc_u.step(150, wl["dt"])
+    ou = orc_u.state()
+    for k in ("x", "v", "w", "q"):
+        assert rel_err(g[k], o[k]) <= max(1e-9, 10.0 * rel_err(ou[k], o[k])), k
 
 
 def test_driver_reports_contact_error():
