@@ -180,6 +180,38 @@ int gdem_add_wall(gdem_ctx* ctx, const double* basis, const double* centre,
    contexts; graph replay is off while walls move. */
 int gdem_add_wall_motion(gdem_ctx* ctx, int32_t wall, int32_t type, double t_start,
                          double t_end, const double* p);
+/* Composite obstacles (CompositeObstacle, Component/src/CompositeObstacle.cpp):
+   a group of walls (and composites) moving as one.  gdem_add_composite
+   creates an empty composite under parent (-1: the root) and returns its
+   index; gdem_attach_wall makes wall k a member (once).  A composite's centre
+   is the volume-weighted centre of its members (computeVolumeCenterOfMass,
+   :228-255).  Its imposed motions (gdem_add_composite_motion, same types and
+   parameters as gdem_add_wall_motion) move its centre and are composed onto
+   every member at its lever from the centre (CompositeObstacle::Move,
+   :135-220; ObstacleKinematicsVelocity::Compose, :62-77), on top of the
+   members' own motions.  Sphere contexts; at most 64 composites. */
+int gdem_add_composite(gdem_ctx* ctx, int32_t parent, int32_t* composite);
+int gdem_attach_wall(gdem_ctx* ctx, int32_t wall, int32_t composite);
+int gdem_add_composite_motion(gdem_ctx* ctx, int32_t composite, int32_t type, double t_start,
+                              double t_end, const double* p);
+/* Centre and velocities of a composite (any pointer may be NULL). */
+int gdem_get_composite(gdem_ctx* ctx, int32_t composite, double* centre, double* vel,
+                       double* omega);
+/* CylindricalShell with box elements (Component/src/CylindricalShell.cpp:13-93):
+   a composite of n walls, box i of edges (width, ly, height) centred at
+   radius + width/2 from the shell axis at angle (i + 1/2) 2 pi / n, with
+   ly = 2 (radius + width)(sin d - (1 - cos d)/tan d), d = 2 pi / n, placed by
+   the shell's transform (row-major basis, centre: its axis is the basis' z).
+   The walls are appended in order i; *composite receives the shell's index. */
+int gdem_add_cylindrical_shell(gdem_ctx* ctx, const double* basis, const double* centre,
+                               double height, double radius, double width, int32_t n,
+                               double crust, int32_t material, int32_t parent,
+                               int32_t* composite);
+/* The shell's boxes alone (no context): basis (9n), centre (3n), half edges (3n). */
+int gdem_cylindrical_shell_boxes(const double* basis, const double* centre, double height,
+                                 double radius, double width, int32_t n, double* wall_basis,
+                                 double* wall_centre, double* wall_half);
+
 /* Simulation time (Grains::m_time, Base/src/Grains.cpp:114, :290); gdem_step
    advances it by dt per step. */
 int gdem_set_time(gdem_ctx* ctx, double time);

@@ -188,6 +188,19 @@ struct BoxObstacle {
   std::string material;
 };
 
+// CylindricalShell (Component/include/CylindricalShell.hh) with Box elements:
+// n boxes (named name0 .. name{n-1}) around the axis of the shell's transform
+// (centre, row-major basis; the axis is the basis' z), a composite obstacle.
+struct CylindricalShell {
+  std::string name;
+  double centre[3];
+  double basis[9];
+  double height, radius, width;
+  int n;
+  double crust;
+  std::string material;
+};
+
 // ObstacleImposedVelocity (Component/include/ObstacleImposedVelocity.hh): an
 // imposed motion of a named obstacle over [tstart, tend].  type and the
 // parameter names follow the reference's XML node (ObstacleImposedVelocity.cpp:
@@ -228,12 +241,21 @@ class LinkedCell {
   void getCells(int32_t nc[3], double edge[3]) const;
   size_t numberOfParticles() const { return m_n; }
   gdem_ctx* context() const { return m_ctx; }
-  // index of the linked obstacle of that name (-1 if none)
+  // CompositeObstacle (Component/src/CompositeObstacle.cpp): an empty
+  // composite, member of `parent` ("" for the root)
+  void LinkComposite(std::string const& name, std::string const& parent = "");
+  // CompositeObstacle::append (CompositeObstacle.cpp:77-80) of a linked box
+  void Append(std::string const& obstacle, std::string const& composite);
+  // CylindricalShell: a composite of its boxes, member of `parent`
+  void Link(CylindricalShell const& shell, std::string const& parent = "");
+  // index of the linked box obstacle / composite of that name (-1 if none)
   int obstacleIndex(std::string const& name) const;
+  int compositeIndex(std::string const& name) const;
 
  private:
   void push_models();
   std::map<std::string, int> m_obstacles;
+  std::map<std::string, int> m_composites;
   int m_nobstacles = 0;
   double m_origin[3], m_size[3];
   bool m_periodic[3];

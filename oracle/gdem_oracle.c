@@ -51,6 +51,23 @@ typedef struct {
   double ms_amp[3], ms_period[3], ms_phase[3];
 } motion_t;
 
+/* per-step state of ObstacleKinematicsVelocity (ObstacleKinematicsVelocity.cpp:
+   60-177): translation and rotation over the step, rotation quaternion over
+   the step, velocities */
+typedef struct {
+  v3 trans, rot, vt, vr;
+  quat q;
+} kin_t;
+
+/* CompositeObstacle (Component/src/CompositeObstacle.cpp): members in attach
+   order (wall k, or composite c as -1 - c) */
+typedef struct {
+  int parent, nmem;
+  int mem[ORC_MAX_WALLS + ORC_MAX_COMPOSITES];
+  double centre[3], basis[9], vel[3], omega[3];
+  kin_t kin;
+} comp_t;
+
 struct orc_sys {
   orc_config cfg;
   int nc[3];
@@ -67,6 +84,10 @@ struct orc_sys {
   double time;
   int nmot, active_prev;
   motion_t mot[ORC_MAX_MOTIONS];
+  kin_t wall_kin[ORC_MAX_WALLS];
+  int wall_parent[ORC_MAX_WALLS];
+  int ncomp;
+  comp_t comp[ORC_MAX_COMPOSITES];
   model_t models[ORC_MAX_MATERIALS][ORC_MAX_MATERIALS];
   /* cell lists: doubly linked, head insertion (Cell.cpp:90-93, push_front) */
   int *head, *next, *prev, *cell;
@@ -258,6 +279,8 @@ int orc_add_wall(orc_sys* s, const double* basis, const double* centre,
                  const double* half_extent, double crust, int material) {
   if (s->nwalls >= ORC_MAX_WALLS || material < 0 || material >= ORC_MAX_MATERIALS)
     return ORC_ERR_ARG;
+  s->wall_parent[s->nwalls] = -1;
+  s->wall_kin[s->nwalls].q.w = 1.;
   build_wall(&s->walls[s->nwalls++], basis, centre, half_extent, crust, material);
   return ORC_OK;
 }
@@ -272,10 +295,8 @@ static v3 normalized(double x, double y, double z) {
 
 /* constructor (ObstacleImposedVelocity.cpp:54-257) and
    ObstacleKinematicsVelocity::append (ObstacleKinematicsVelocity.cpp:30-41) */
-int orc_add_wall_motion(orc_sys* s, int wall, int type, double t0, double t1,
-                        const double* p) {
-  if (wall < 0 || wall >= s->nwalls || s->nmot >= ORC_MAX_MOTIONS || !(t1 >= t0))
-    return ORC_ERR_ARG;
+static int add_motion(orc_sys* s, int wall, int type, double t0, double t1, const double* p) {
+  if (s->nmot >= ORC_MAX_MOTIONS || !(t1 >= t0)) return ORC_ERR_ARG;
   motion_t m;
   memset(&m, 0, sizeof(m));
   m.wall = wall;
@@ -326,6 +347,11 @@ int orc_add_wall_motion(orc_sys* s, int wall, int type, double t0, double t1,
   s->mot[pos] = m;
   s->nmot++;
   return ORC_OK;
+}
+
+int orc_add_wall_motion(orc_sys* s, int wall, int type, double t0, double t1, const double* p) {
+  if (wall < 0 || wall >= s->nwalls) return ORC_ERR_ARG;
+  return add_motion(s, wall, type, t0, t1, p);
 }
 
 /* isActif / isCompleted (ObstacleImposedVelocity.cpp:305-325) */
@@ -411,48 +437,124 @@ static v3 mot_translation(const motion_t* m, double time, double dt, double subi
   return tr;
 }
 
+/* Quaternion::rotateVector (Geometry/src/Quaternion.cpp:293-298) */
+static v3 q_rotate(quat q, v3 v) {
+  double nq = norm(q.v);
+  return add(add(scl(v, q.w * q.w - nq * nq), scl(q.v, 2. * dot(v, q.v))),
+             scl(cross(q.v, v), 2. * q.w));
+}
+
+static kin_t kin_zero(void) {
+  kin_t k;
+  k.trans = k.rot = k.vt = k.vr = mk(0., 0., 0.);
+  k.q = qmk(mk(0., 0., 0.), 1.);
+  return k;
+}
+
+/* ObstacleKinematicsVelocity::Compose (ObstacleKinematicsVelocity.cpp:62-77) */
+static void kin_compose(kin_t* k, const kin_t* o, v3 lever) {
+  k->rot = add(k->rot, o->rot);
+  k->q = qmul(o->q, k->q);
+  k->trans = add(o->trans, sub(q_rotate(o->q, lever), lever));
+  k->vr = add(k->vr, o->vr);
+  k->vt = add(k->vt, add(o->vt, cross(o->vr, lever)));
+}
+
 /* ObstacleKinematicsVelocity::ImposedMotion (ObstacleKinematicsVelocity.cpp:
-   86-122) and SimpleObstacle::Move (Component/src/SimpleObstacle.cpp:116-175)
-   of wall w: translation, rotation of the basis about the centre, new
-   bounding box, velocities of the kinematics when it moved. */
-static void move_wall(orc_sys* s, int w, double time, double dt, int displace) {
-  wall_t* wl = &s->walls[w];
-  v3 cg = ld(wl->origin), depl = mk(0., 0., 0.), rota = mk(0., 0., 0.);
-  v3 vt = mk(0., 0., 0.), vr = mk(0., 0., 0.);
+   86-122) of obstacle `target` (wall k, or composite c as -1 - c): its active
+   motions added to its kinematics; returns whether they moved it */
+static int kin_imposed(orc_sys* s, kin_t* k, int target, double time, double dt, v3 cg) {
+  v3 depl = mk(0., 0., 0.), rota = mk(0., 0., 0.), vt = mk(0., 0., 0.), vr = mk(0., 0., 0.);
   double subint = 0.;
   for (int i = 0; i < s->nmot; ++i) {
     motion_t* m = &s->mot[i];
-    if (m->wall != w || !mot_active(m, time - dt, time, dt, &subint)) continue;
+    if (m->wall != target || !mot_active(m, time - dt, time, dt, &subint)) continue;
     depl = add(depl, mot_translation(m, time, dt, subint, cg));
     rota = add(rota, mot_rotation(m, subint));
     vt = add(vt, mot_velocity(m, time, dt, cg));
     vr = add(vr, mot_omega(m, time, dt));
   }
   quat qrot = rot_quat(rota);
-  if (!(norm(depl) != 0. || norm(rota) != 0.)) return;
+  k->trans = add(k->trans, depl);
+  k->rot = add(k->rot, rota);
+  k->q = qmul(qrot, k->q);
+  k->vr = add(k->vr, vr);
+  k->vt = add(k->vt, vt);
+  return norm(depl) != 0. || norm(rota) != 0.;
+}
+
+/* SimpleObstacle::Move (Component/src/SimpleObstacle.cpp:116-175) of wall w:
+   translation (Transform::composeLeftByTranslation, Transform.cpp:300-304),
+   rotation of the basis (RigidBody::Rotate, RigidBody.cpp:270-273), new
+   bounding box, velocities of the kinematics when it moved */
+static void move_wall(orc_sys* s, int w, double time, double dt, int mother, int displace) {
+  wall_t* wl = &s->walls[w];
+  kin_t* k = &s->wall_kin[w];
+  int moving = kin_imposed(s, k, w, time, dt, ld(wl->origin)) || mother;
+  if (!moving) return;
   if (displace) {
-    /* composeLeftByTranslation (Transform.cpp:300-304), then
-       RigidBody::Rotate with qrot * identity (RigidBody.cpp:270-273) */
-    quat q = qmul(qrot, qmk(mk(0., 0., 0.), 1.));
     double basis[9], origin[3], ext[3], crust = wl->crust;
     int material = wl->material;
     memcpy(basis, wl->basis, sizeof(basis));
     memcpy(ext, wl->ext, sizeof(ext));
-    origin[0] = wl->origin[0] + depl.x;
-    origin[1] = wl->origin[1] + depl.y;
-    origin[2] = wl->origin[2] + depl.z;
-    compose_left_rotation(basis, q);
+    origin[0] = wl->origin[0] + k->trans.x;
+    origin[1] = wl->origin[1] + k->trans.y;
+    origin[2] = wl->origin[2] + k->trans.z;
+    compose_left_rotation(basis, k->q);
     build_wall(wl, basis, origin, ext, crust, material);
   }
   /* Obstacle::setVelocity (Obstacle.cpp:635-640) */
-  st(wl->vel, vt);
-  st(wl->omega, vr);
+  st(wl->vel, k->vt);
+  st(wl->omega, k->vr);
 }
 
-/* Obstacle::resetKinematics (Obstacle.cpp:160-167) of every wall */
-static void reset_walls(orc_sys* s) {
+static void move_members(orc_sys* s, int parent, double time, double dt, int mother,
+                         int displace);
+
+/* CompositeObstacle::Move (Component/src/CompositeObstacle.cpp:135-220) */
+static void move_composite(orc_sys* s, int c, double time, double dt, int mother, int displace) {
+  comp_t* C = &s->comp[c];
+  int moving = kin_imposed(s, &C->kin, -1 - c, time, dt, ld(C->centre)) || mother;
+  if (moving && displace) {
+    st(C->centre, add(ld(C->centre), C->kin.trans));
+    compose_left_rotation(C->basis, C->kin.q);
+  }
+  if (moving)
+    for (int i = 0; i < C->nmem; ++i) {
+      int mbr = C->mem[i];
+      if (mbr < 0) {
+        comp_t* M = &s->comp[-1 - mbr];
+        kin_compose(&M->kin, &C->kin, sub(ld(M->centre), ld(C->centre)));
+      } else {
+        kin_compose(&s->wall_kin[mbr], &C->kin, sub(ld(s->walls[mbr].origin), ld(C->centre)));
+      }
+    }
+  move_members(s, c, time, dt, moving, displace);
+  if (moving) {
+    st(C->vel, C->kin.vt);
+    st(C->omega, C->kin.vr);
+  }
+}
+
+static void move_members(orc_sys* s, int parent, double time, double dt, int mother,
+                         int displace) {
+  for (int k = 0; k < s->ncomp; ++k)
+    if (s->comp[k].parent == parent) move_composite(s, k, time, dt, mother, displace);
   for (int w = 0; w < s->nwalls; ++w)
+    if (s->wall_parent[w] == parent) move_wall(s, w, time, dt, mother, displace);
+}
+
+/* Obstacle::resetKinematics (Obstacle.cpp:160-167), from the root down
+   (CompositeObstacle.cpp:449-460) */
+static void reset_walls(orc_sys* s) {
+  for (int w = 0; w < s->nwalls; ++w) {
     for (int d = 0; d < 3; ++d) s->walls[w].vel[d] = s->walls[w].omega[d] = 0.;
+    s->wall_kin[w] = kin_zero();
+  }
+  for (int c = 0; c < s->ncomp; ++c) {
+    s->comp[c].kin = kin_zero();
+    for (int d = 0; d < 3; ++d) s->comp[c].vel[d] = s->comp[c].omega[d] = 0.;
+  }
 }
 
 static int any_live(const orc_sys* s) {
@@ -477,8 +579,7 @@ int orc_move_walls(orc_sys* s, double time, double dt) {
     if (s->mot[i].live && mot_active(&s->mot[i], time - dt, time, dt, &subint)) anyactive = 1;
   if (anyactive || s->active_prev) reset_walls(s);
   s->active_prev = anyactive;
-  if (anyactive)
-    for (int w = 0; w < s->nwalls; ++w) move_wall(s, w, time, dt, 1);
+  if (anyactive) move_members(s, -1, time, dt, 0, 1);
   drop_completed(s, time, dt);
   return ORC_OK;
 }
@@ -487,8 +588,123 @@ int orc_move_walls(orc_sys* s, double time, double dt) {
 static void init_wall_kinematics(orc_sys* s, double time, double dt) {
   if (!any_live(s)) return;
   reset_walls(s);
-  for (int w = 0; w < s->nwalls; ++w) move_wall(s, w, time, dt, 0);
+  move_members(s, -1, time, dt, 0, 0);
   drop_completed(s, time, dt);
+}
+
+/* CompositeObstacle::computeVolumeCenterOfMass (CompositeObstacle.cpp:228-255),
+   SimpleObstacle's (SimpleObstacle.cpp:732-735) with Box::getVolume (Box.cpp:
+   273-279): volume-weighted centre of the members, in attach order */
+static double volume_centre(orc_sys* s, int c, v3* out) {
+  comp_t* C = &s->comp[c];
+  v3 centre = mk(0., 0., 0.);
+  double vol = 0.;
+  for (int i = 0; i < C->nmem; ++i) {
+    int mbr = C->mem[i];
+    v3 pc;
+    double pv;
+    if (mbr < 0) {
+      pv = volume_centre(s, -1 - mbr, &pc);
+    } else {
+      const wall_t* wl = &s->walls[mbr];
+      pc = ld(wl->origin);
+      pv = 8.0 * wl->ext[0] * wl->ext[1] * wl->ext[2];
+    }
+    centre = add(centre, scl(pc, pv));
+    vol += pv;
+  }
+  if (vol > 0.) centre = dvd(centre, vol);
+  st(C->centre, centre);
+  *out = centre;
+  return vol;
+}
+
+static void composite_centres(orc_sys* s) {
+  v3 c;
+  for (int k = 0; k < s->ncomp; ++k)
+    if (s->comp[k].parent < 0) volume_centre(s, k, &c);
+}
+
+int orc_add_composite(orc_sys* s, int parent, int* id) {
+  if (parent < -1 || parent >= s->ncomp || s->ncomp >= ORC_MAX_COMPOSITES) return ORC_ERR_ARG;
+  comp_t* C = &s->comp[s->ncomp];
+  memset(C, 0, sizeof(*C));
+  C->parent = parent;
+  C->basis[0] = C->basis[4] = C->basis[8] = 1.;
+  C->kin = kin_zero();
+  if (parent >= 0) s->comp[parent].mem[s->comp[parent].nmem++] = -1 - s->ncomp;
+  *id = s->ncomp++;
+  composite_centres(s);
+  return ORC_OK;
+}
+
+int orc_attach_wall(orc_sys* s, int wall, int c) {
+  if (wall < 0 || wall >= s->nwalls || c < 0 || c >= s->ncomp || s->wall_parent[wall] != -1)
+    return ORC_ERR_ARG;
+  s->wall_parent[wall] = c;
+  s->comp[c].mem[s->comp[c].nmem++] = wall;
+  composite_centres(s);
+  return ORC_OK;
+}
+
+int orc_add_composite_motion(orc_sys* s, int c, int type, double t0, double t1, const double* p) {
+  if (c < 0 || c >= s->ncomp) return ORC_ERR_ARG;
+  return add_motion(s, -1 - c, type, t0, t1, p);
+}
+
+int orc_get_composite(const orc_sys* s, int c, double* centre, double* vel, double* omega) {
+  if (c < 0 || c >= s->ncomp) return ORC_ERR_ARG;
+  if (centre) memcpy(centre, s->comp[c].centre, sizeof(double) * 3);
+  if (vel) memcpy(vel, s->comp[c].vel, sizeof(double) * 3);
+  if (omega) memcpy(omega, s->comp[c].omega, sizeof(double) * 3);
+  return ORC_OK;
+}
+
+/* CylindricalShell with box elements (Component/src/CylindricalShell.cpp:
+   13-93): box i of edges (width, ly, height) at radius + width/2, angle
+   (i + 1/2) 2 pi / n, rotated by that angle about z, placed by the shell's
+   transform (Transform::composeLeftByTransform, Transform.cpp:355-360;
+   Matrix products Matrix.cpp:517-549), then translated by zero */
+void orc_cylindrical_shell_boxes(const double* sb, const double* sc, double height, double radius,
+                                 double width, int n, double* basis, double* centre,
+                                 double* half) {
+  double ext_rad = radius + width / 2.;
+  double delta = 2. * ORC_PI / (double)n;
+  double ly = 2. * (radius + width) * (sin(delta) - (1. - cos(delta)) / tan(delta));
+  for (int i = 0; i < n; ++i) {
+    double angle = (0.5 + (double)i) * delta;
+    double ca = cos(angle), sa = sin(angle);
+    double cg[3] = {ext_rad * ca, ext_rad * sa, 0.};
+    double m[9] = {ca, -sa, 0., sa, ca, 0., 0., 0., 1.};
+    for (int r = 0; r < 3; ++r) {
+      centre[3 * i + r] = sc[r] + (sb[3 * r] * cg[0] + sb[3 * r + 1] * cg[1] + sb[3 * r + 2] * cg[2]);
+      for (int col = 0; col < 3; ++col)
+        basis[9 * i + 3 * r + col] =
+            sb[3 * r] * m[col] + sb[3 * r + 1] * m[3 + col] + sb[3 * r + 2] * m[6 + col];
+      centre[3 * i + r] = centre[3 * i + r] + 0.;
+    }
+    /* Box(width, ly, height), Box.cpp:27-33 */
+    half[3 * i] = fabs(width) / 2.;
+    half[3 * i + 1] = fabs(ly) / 2.;
+    half[3 * i + 2] = fabs(height) / 2.;
+  }
+}
+
+int orc_add_cylindrical_shell(orc_sys* s, const double* basis, const double* centre,
+                              double height, double radius, double width, int n, double crust,
+                              int material, int parent, int* id) {
+  if (n < 3 || s->nwalls + n > ORC_MAX_WALLS) return ORC_ERR_ARG;
+  double* wb = (double*)malloc(sizeof(double) * 15 * (size_t)n);
+  if (!wb) return ORC_ERR_ARG;
+  orc_cylindrical_shell_boxes(basis, centre, height, radius, width, n, wb, wb + 9 * n,
+                              wb + 12 * n);
+  int rc = orc_add_composite(s, parent, id);
+  for (int i = 0; i < n && !rc; ++i) {
+    rc = orc_add_wall(s, wb + 9 * i, wb + 9 * n + 3 * i, wb + 12 * n + 3 * i, crust, material);
+    if (!rc) rc = orc_attach_wall(s, s->nwalls - 1, *id);
+  }
+  free(wb);
+  return rc;
 }
 
 int orc_set_time(orc_sys* s, double time) {

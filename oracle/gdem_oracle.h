@@ -87,6 +87,20 @@ int orc_set_contact_model(orc_sys* s, int mat_a, int mat_b, int kind,
 #define ORC_MAX_MOTIONS 64
 int orc_add_wall_motion(orc_sys* s, int wall, int type, double t_start, double t_end,
                         const double* p);
+/* composite obstacles and box-element cylindrical shells (as gdem.h) */
+#define ORC_MAX_COMPOSITES 64
+int orc_add_composite(orc_sys* s, int parent, int* id);
+int orc_attach_wall(orc_sys* s, int wall, int composite);
+int orc_add_composite_motion(orc_sys* s, int composite, int type, double t_start,
+                             double t_end, const double* p);
+int orc_get_composite(const orc_sys* s, int composite, double* centre, double* vel,
+                      double* omega);
+void orc_cylindrical_shell_boxes(const double* basis, const double* centre, double height,
+                                 double radius, double width, int n, double* wall_basis,
+                                 double* wall_centre, double* wall_half);
+int orc_add_cylindrical_shell(orc_sys* s, const double* basis, const double* centre,
+                              double height, double radius, double width, int n, double crust,
+                              int material, int parent, int* id);
 /* time (orc_step advances it); walls moved over [time - dt, time] */
 int orc_set_time(orc_sys* s, double time);
 double orc_get_time(const orc_sys* s);

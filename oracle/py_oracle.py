@@ -53,6 +53,12 @@ def load():
             "orc_set_contact_model": (i32, [vp, i32, i32, i32, vp]),
             "orc_set_time_step": (i32, [vp, d]),
             "orc_add_wall_motion": (i32, [vp, i32, i32, d, d, vp]),
+            "orc_add_composite": (i32, [vp, i32, P(i32)]),
+            "orc_attach_wall": (i32, [vp, i32, i32]),
+            "orc_add_composite_motion": (i32, [vp, i32, i32, d, d, vp]),
+            "orc_get_composite": (i32, [vp, i32, vp, vp, vp]),
+            "orc_cylindrical_shell_boxes": (None, [vp, vp, d, d, d, i32, vp, vp, vp]),
+            "orc_add_cylindrical_shell": (i32, [vp, vp, vp, d, d, d, i32, d, i32, i32, P(i32)]),
             "orc_set_time": (i32, [vp, d]),
             "orc_get_time": (d, [vp]),
             "orc_move_walls": (i32, [vp, d, d]),
@@ -127,8 +133,24 @@ class Oracle:
         self._chk(self.lib.orc_set_particles(h, n, *[_p(a) for a in keep], _p(mat)), "particles")
         if "time" in wl:
             self.lib.orc_set_time(h, float(wl["time"]))
+        for (parent, members) in wl.get("composites", []):
+            k = C.c_int()
+            self._chk(self.lib.orc_add_composite(h, int(parent), C.byref(k)), "composite")
+            for w in members:
+                self._chk(self.lib.orc_attach_wall(h, int(w), k.value), "attach")
+        for (basis, centre, height, radius, width, n, crust, mat, parent) in wl.get("shells", []):
+            k = C.c_int()
+            self._chk(self.lib.orc_add_cylindrical_shell(
+                h, _p(_f(basis, (9,))), _p(_f(centre, (3,))), float(height), float(radius),
+                float(width), int(n), float(crust), int(mat), int(parent), C.byref(k)), "shell")
+        for (wall, comp) in wl.get("attach", []):
+            self._chk(self.lib.orc_attach_wall(h, int(wall), int(comp)), "attach")
         for (wall, kind, t0, t1, prm) in wl.get("motions", []):
             self.add_wall_motion(wall, kind, t0, t1, prm)
+        for (comp, kind, t0, t1, prm) in wl.get("composite_motions", []):
+            npar = {1: 3, 2: 6, 3: 5, 4: 12, 5: 7}[int(kind)]
+            self._chk(self.lib.orc_add_composite_motion(h, int(comp), int(kind), float(t0), float(t1),
+                                                        _p(_f(prm, (npar,)))), "composite motion")
         if log_capacity:
             self.lib.orc_enable_contact_log(h, int(log_capacity))
 
@@ -172,6 +194,11 @@ class Oracle:
 
     def move_walls(self, time, dt):
         self._chk(self.lib.orc_move_walls(self.h, float(time), float(dt)), "move walls")
+
+    def composite(self, k):
+        c, v, w = np.empty(3), np.empty(3), np.empty(3)
+        self._chk(self.lib.orc_get_composite(self.h, int(k), _p(c), _p(v), _p(w)), "composite")
+        return c, v, w
 
     def time(self):
         return self.lib.orc_get_time(self.h)
@@ -226,6 +253,14 @@ class Oracle:
 
     def candidates(self):
         return self.lib.orc_candidates(self.h)
+
+
+def cylindrical_shell_boxes(basis, centre, height, radius, width, n):
+    lib = load()
+    b, c, h = np.empty((n, 9)), np.empty((n, 3)), np.empty((n, 3))
+    lib.orc_cylindrical_shell_boxes(_p(_f(basis, (9,))), _p(_f(centre, (3,))), float(height),
+                                    float(radius), float(width), int(n), _p(b), _p(c), _p(h))
+    return b, c, h
 
 
 def rotate_two_vectors(u, v, x):

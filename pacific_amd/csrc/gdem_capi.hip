@@ -1287,12 +1287,12 @@ extern "C" int gdem_info(gdem_ctx* c, int64_t* n, int32_t* nwalls, int32_t* conv
   return GDEM_OK;
 }
 
-extern "C" int gdem_add_wall_motion(gdem_ctx* c, int32_t wall, int32_t type, double t_start,
-                                    double t_end, const double* p) {
-  if (!c || !p || wall < 0 || wall >= c->nwalls || c->convex) return GDEM_ERR_ARG;
-  if (!(t_end >= t_start)) return GDEM_ERR_ARG;
+// target: wall k, or composite c as -1 - c
+static int add_motion(gdem_ctx* c, int32_t target, int32_t type, double t_start, double t_end,
+                      const double* p) {
+  if (!p || c->convex || !(t_end >= t_start)) return GDEM_ERR_ARG;
   WallMotion m;
-  if (!make_wall_motion(wall, type, t_start, t_end, p, m)) return GDEM_ERR_ARG;
+  if (!make_wall_motion(target, type, t_start, t_end, p, m)) return GDEM_ERR_ARG;
   if ((type == GDEM_MOTION_SIN_TRANSLATION || type == GDEM_MOTION_CRENEL_TRANSLATION) &&
       !(p[4] > 0.))
     return GDEM_ERR_ARG;
@@ -1301,7 +1301,97 @@ extern "C" int gdem_add_wall_motion(gdem_ctx* c, int32_t wall, int32_t type, dou
   if (type >= GDEM_MOTION_SIN_TRANSLATION && type <= GDEM_MOTION_MULTISIN_TRANSLATION &&
       !(norm(mk(p[0], p[1], p[2])) > 0.))
     return GDEM_ERR_ARG;
-  c->kin.add(m);
+  c->kin.insert_motion(m);
+  return GDEM_OK;
+}
+
+extern "C" int gdem_add_wall_motion(gdem_ctx* c, int32_t wall, int32_t type, double t_start,
+                                    double t_end, const double* p) {
+  if (!c || wall < 0 || wall >= c->nwalls) return GDEM_ERR_ARG;
+  return add_motion(c, wall, type, t_start, t_end, p);
+}
+
+extern "C" int gdem_add_composite_motion(gdem_ctx* c, int32_t composite, int32_t type,
+                                         double t_start, double t_end, const double* p) {
+  if (!c || composite < 0 || composite >= (int32_t)c->kin.comps.size()) return GDEM_ERR_ARG;
+  return add_motion(c, -1 - composite, type, t_start, t_end, p);
+}
+
+// centres of all composites from their members (each top-level composite
+// recomputes its subtree)
+static void composite_centres(gdem_ctx* c) {
+  for (int32_t k = 0; k < (int32_t)c->kin.comps.size(); ++k)
+    if (c->kin.comps[k].parent < 0) c->kin.volume_centre(k, c->h_walls);
+}
+
+extern "C" int gdem_add_composite(gdem_ctx* c, int32_t parent, int32_t* id) {
+  if (!c || !id || c->convex || parent < -1 || parent >= (int32_t)c->kin.comps.size() ||
+      c->kin.comps.size() >= (size_t)kMaxWalls)
+    return GDEM_ERR_ARG;
+  Composite cp;
+  cp.parent = parent;
+  const int32_t k = (int32_t)c->kin.comps.size();
+  c->kin.comps.push_back(cp);
+  if (parent >= 0) c->kin.comps[parent].members.push_back(-1 - k);
+  *id = k;
+  composite_centres(c);
+  return GDEM_OK;
+}
+
+extern "C" int gdem_attach_wall(gdem_ctx* c, int32_t wall, int32_t composite) {
+  if (!c || wall < 0 || wall >= c->nwalls || composite < 0 ||
+      composite >= (int32_t)c->kin.comps.size())
+    return GDEM_ERR_ARG;
+  c->kin.fit(c->nwalls);
+  if (c->kin.wall_parent[wall] != -1) return GDEM_ERR_ARG;  // already a member
+  c->kin.wall_parent[wall] = composite;
+  c->kin.comps[composite].members.push_back(wall);
+  composite_centres(c);
+  return GDEM_OK;
+}
+
+extern "C" int gdem_add_cylindrical_shell(gdem_ctx* c, const double* basis, const double* centre,
+                                          double height, double radius, double width, int32_t n,
+                                          double crust, int32_t material, int32_t parent,
+                                          int32_t* composite) {
+  if (!c || !basis || !centre || !composite || n < 3 || !(height > 0.) || !(radius > 0.) ||
+      !(width > 0.) || c->nwalls + n > kMaxWalls)
+    return GDEM_ERR_ARG;
+  std::vector<double> wb(9 * (size_t)n), wc(3 * (size_t)n), wh(3 * (size_t)n);
+  cylindrical_shell_boxes(basis, centre, height, radius, width, n, wb.data(), wc.data(), wh.data());
+  int32_t id = -1;
+  int rc = gdem_add_composite(c, parent, &id);
+  if (rc) return rc;
+  for (int32_t i = 0; i < n; ++i) {
+    rc = gdem_add_wall(c, wb.data() + 9 * i, wc.data() + 3 * i, wh.data() + 3 * i, crust, material);
+    if (rc) return rc;
+    rc = gdem_attach_wall(c, c->nwalls - 1, id);
+    if (rc) return rc;
+  }
+  *composite = id;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_cylindrical_shell_boxes(const double* basis, const double* centre,
+                                            double height, double radius, double width, int32_t n,
+                                            double* wall_basis, double* wall_centre,
+                                            double* wall_half) {
+  if (!basis || !centre || !wall_basis || !wall_centre || !wall_half || n < 3) return GDEM_ERR_ARG;
+  cylindrical_shell_boxes(basis, centre, height, radius, width, n, wall_basis, wall_centre,
+                          wall_half);
+  return GDEM_OK;
+}
+
+extern "C" int gdem_get_composite(gdem_ctx* c, int32_t composite, double* centre, double* vel,
+                                  double* omega) {
+  if (!c || composite < 0 || composite >= (int32_t)c->kin.comps.size()) return GDEM_ERR_ARG;
+  const Composite& cp = c->kin.comps[composite];
+  const double ce[3] = {cp.centre.x, cp.centre.y, cp.centre.z};
+  const double ve[3] = {cp.vel.x, cp.vel.y, cp.vel.z};
+  const double om[3] = {cp.omega.x, cp.omega.y, cp.omega.z};
+  if (centre) memcpy(centre, ce, sizeof(ce));
+  if (vel) memcpy(vel, ve, sizeof(ve));
+  if (omega) memcpy(omega, om, sizeof(om));
   return GDEM_OK;
 }
 

@@ -5,6 +5,7 @@
 #pragma once
 #include <cmath>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "../../include/gdem.h"
@@ -125,6 +126,13 @@ inline quat rotation_quat(v3 rota) {
   return qmk(mk(0., 0., 0.), 1.);
 }
 
+// Quaternion::rotateVector (Geometry/src/Quaternion.cpp:293-298)
+inline v3 rotate_vector(quat q, v3 v) {
+  const double nq = norm(q.v);
+  return add(add(scl(v, q.w * q.w - nq * nq), scl(q.v, 2. * dot(v, q.v))),
+             scl(cross(q.v, v), 2. * q.w));
+}
+
 // translationalMotion (ObstacleImposedVelocity.cpp:370-427)
 inline v3 motion_translation(WallMotion& m, double time, double dt, double subint, v3 cg) {
   const double ti = time - dt < m.t0 ? m.t0 : time - dt;
@@ -154,11 +162,7 @@ inline v3 motion_translation(WallMotion& m, double time, double dt, double subin
       if (!m.about_centre) {
         const quat q = rotation_quat(motion_rotation(m, subint));
         const v3 lever = sub(cg, m.centre);
-        // Quaternion::rotateVector (Geometry/src/Quaternion.cpp:293-298)
-        const double nq = norm(q.v);
-        const v3 r = add(add(scl(lever, q.w * q.w - nq * nq), scl(q.v, 2. * dot(lever, q.v))),
-                         scl(cross(q.v, lever), 2. * q.w));
-        return sub(r, lever);
+        return sub(rotate_vector(q, lever), lever);
       }
       return mk(0., 0., 0.);
     default:
@@ -166,25 +170,51 @@ inline v3 motion_translation(WallMotion& m, double time, double dt, double subin
   }
 }
 
-// One wall's kinematics over [time - dt, time]
-// (ObstacleKinematicsVelocity::ImposedMotion, Component/src/ObstacleKinematicsVelocity.cpp:72-103):
-// the summed displacement, rotation quaternion and velocities of its active
-// motions; returns whether it moved.
-inline bool wall_imposed_motion(std::vector<WallMotion>& motions, int32_t wall, double time,
-                                double dt, v3 cg, v3& depl, quat& qrot, v3& vt, v3& vr) {
-  depl = mk(0., 0., 0.);
-  v3 rota = mk(0., 0., 0.);
-  vt = mk(0., 0., 0.);
-  vr = mk(0., 0., 0.);
+// Per-step state of an obstacle's ObstacleKinematicsVelocity
+// (ObstacleKinematicsVelocity.cpp:60-177): translation and rotation over the
+// step, rotation quaternion over the step, velocities.
+struct Kin {
+  v3 trans, rot, vt, vr;
+  quat q;
+};
+inline Kin kin_zero() {
+  Kin k;
+  k.trans = k.rot = k.vt = k.vr = mk(0., 0., 0.);
+  k.q = qmk(mk(0., 0., 0.), 1.);
+  return k;
+}
+
+// Compose (ObstacleKinematicsVelocity.cpp:62-77): the motion of the composite
+// an obstacle belongs to, with the obstacle at lever from its centre
+inline void kin_compose(Kin& k, const Kin& o, v3 lever) {
+  k.rot = add(k.rot, o.rot);
+  k.q = qmul(o.q, k.q);
+  k.trans = add(o.trans, sub(rotate_vector(o.q, lever), lever));
+  k.vr = add(k.vr, o.vr);
+  k.vt = add(k.vt, add(o.vt, cross(o.vr, lever)));
+}
+
+// ImposedMotion (ObstacleKinematicsVelocity.cpp:86-122) of obstacle `target`
+// (wall k >= 0, composite c as -1 - c) over [time - dt, time]: its active
+// motions' displacement, rotation and velocities added to its kinematics;
+// returns whether its own motions moved it.
+inline bool kin_imposed(Kin& k, std::vector<WallMotion>& motions, int32_t target, double time,
+                        double dt, v3 cg) {
+  v3 depl = mk(0., 0., 0.), rota = mk(0., 0., 0.), vt = mk(0., 0., 0.), vr = mk(0., 0., 0.);
   double subint = 0.;
   for (WallMotion& m : motions) {
-    if (m.wall != wall || !motion_active(m, time - dt, time, dt, subint)) continue;
+    if (m.wall != target || !motion_active(m, time - dt, time, dt, subint)) continue;
     depl = add(depl, motion_translation(m, time, dt, subint, cg));
     rota = add(rota, motion_rotation(m, subint));
     vt = add(vt, motion_velocity(m, time, dt, cg));
     vr = add(vr, motion_omega(m, time, dt));
   }
-  qrot = rotation_quat(rota);
+  const quat qrot = rotation_quat(rota);
+  k.trans = add(k.trans, depl);
+  k.rot = add(k.rot, rota);
+  k.q = qmul(qrot, k.q);
+  k.vr = add(k.vr, vr);
+  k.vt = add(k.vt, vt);
   return norm(depl) != 0. || norm(rota) != 0.;
 }
 
@@ -204,18 +234,37 @@ inline void compose_left_rotation(quat q, double* b) {
   }
 }
 
-// The imposed motions of all walls and the step-to-step state of the
-// obstacle part of AllComponents::Move (Component/src/AllComponents.cpp:151-220).
+// A CompositeObstacle (Component/src/CompositeObstacle.cpp): a group of walls
+// and composites that moves as one; its position is the volume-weighted centre
+// of its members (computeVolumeCenterOfMass, :228-255).
+struct Composite {
+  int32_t parent = -1;  // -1: the root obstacle
+  // members in the order they were attached: wall k, or composite c as -1 - c
+  std::vector<int32_t> members;
+  v3 centre = {0., 0., 0.};
+  double basis[9] = {1., 0., 0., 0., 1., 0., 0., 0., 1.};
+  double volume = 0.;
+  Kin kin = kin_zero();
+  v3 vel = {0., 0., 0.}, omega = {0., 0., 0.};
+};
+
+// The imposed motions of all obstacles, the composite tree and the
+// step-to-step state of the obstacle part of AllComponents::Move
+// (Component/src/AllComponents.cpp:151-220).
 struct WallKinematics {
-  // every motion, each wall's in the order of ObstacleKinematicsVelocity::append
+  // every motion, each obstacle's in the order of ObstacleKinematicsVelocity::append
   // (ObstacleKinematicsVelocity.cpp:30-41: before the first one that does
-  // not start earlier, operator< of ObstacleImposedVelocity.cpp:474-478)
+  // not start earlier, operator< of ObstacleImposedVelocity.cpp:474-478);
+  // WallMotion::wall is the obstacle (wall k, or composite c as -1 - c)
   std::vector<WallMotion> motions;
   // still in the list of motions not completed (m_AllImposedVelocitiesOnObstacles)
   std::vector<char> live;
   bool active_prev = false;  // anyactive_previousdt
+  std::vector<Composite> comps;
+  std::vector<int32_t> wall_parent;  // composite of each wall, -1 for the root
+  std::vector<Kin> wall_kin;
 
-  void add(const WallMotion& m) {
+  void insert_motion(const WallMotion& m) {
     size_t pos = motions.size();
     for (size_t i = 0; i < motions.size(); ++i)
       if (motions[i].wall == m.wall && !(motions[i].t0 < m.t0)) {
@@ -234,44 +283,111 @@ struct WallKinematics {
     for (size_t i = 0; i < motions.size(); ++i)
       if (live[i] && motion_completed(motions[i], time, dt)) live[i] = 0;
   }
-  static void reset(Wall* walls, int nwalls) {
-    // Obstacle::resetKinematics (Component/src/Obstacle.cpp:160-167)
-    for (int w = 0; w < nwalls; ++w)
+  void fit(int nwalls) {
+    if ((int)wall_parent.size() < nwalls) {
+      wall_parent.resize((size_t)nwalls, -1);
+      wall_kin.resize((size_t)nwalls, kin_zero());
+    }
+  }
+  // computeVolumeCenterOfMass of composite c (CompositeObstacle.cpp:228-255,
+  // SimpleObstacle.cpp:732-735 with Box::getVolume, Box.cpp:273-279):
+  // members in the order they were attached (walls by index here)
+  std::pair<v3, double> volume_centre(int32_t c, const Wall* walls) {
+    v3 centre = mk(0., 0., 0.);
+    double vol = 0.;
+    for (int32_t mbr : comps[c].members) {
+      std::pair<v3, double> pp;
+      if (mbr < 0) {
+        pp = volume_centre(-1 - mbr, walls);
+      } else {
+        const Wall& wl = walls[mbr];
+        pp = {mk(wl.origin[0], wl.origin[1], wl.origin[2]), 8.0 * wl.ext[0] * wl.ext[1] * wl.ext[2]};
+      }
+      centre = add(centre, scl(pp.first, pp.second));
+      vol += pp.second;
+    }
+    if (vol > 0.) centre = dvd(centre, vol);
+    comps[c].centre = centre;
+    comps[c].volume = vol;
+    return {centre, vol};
+  }
+  void reset(Wall* walls, int nwalls) {
+    // Obstacle::resetKinematics (Component/src/Obstacle.cpp:160-167),
+    // recursively from the root (CompositeObstacle.cpp:449-460)
+    for (int w = 0; w < nwalls; ++w) {
       for (int d = 0; d < 3; ++d) walls[w].vel[d] = walls[w].omega[d] = 0.;
+      wall_kin[w] = kin_zero();
+    }
+    for (Composite& c : comps) {
+      c.kin = kin_zero();
+      c.vel = c.omega = mk(0., 0., 0.);
+    }
   }
   // SimpleObstacle::Move (Component/src/SimpleObstacle.cpp:116-175) of wall w:
   // translate, rotate about its centre, new bounding box; velocities of its
   // kinematics when it moved
-  void move_wall(Wall* walls, int w, double time, double dt, bool displace) {
+  void move_wall(Wall* walls, int w, double time, double dt, bool mother, bool displace) {
     Wall& wl = walls[w];
-    v3 depl, vt, vr;
-    quat qrot;
+    Kin& k = wall_kin[w];
     const v3 cg = mk(wl.origin[0], wl.origin[1], wl.origin[2]);
-    if (!wall_imposed_motion(motions, w, time, dt, cg, depl, qrot, vt, vr)) return;
+    const bool moving = kin_imposed(k, motions, w, time, dt, cg) || mother;
+    if (!moving) return;
     if (displace) {
-      // m_QuaternionRotationOverDt = qrot * identity after the reset
-      // (ObstacleKinematicsVelocity.cpp:115-117, :168-177)
-      const quat q = qmul(qrot, qmk(mk(0., 0., 0.), 1.));
-      const double origin[3] = {wl.origin[0] + depl.x, wl.origin[1] + depl.y,
-                                wl.origin[2] + depl.z};
+      const double origin[3] = {wl.origin[0] + k.trans.x, wl.origin[1] + k.trans.y,
+                                wl.origin[2] + k.trans.z};
       double basis[9], ext[3];
       memcpy(basis, wl.basis, sizeof(basis));
       memcpy(ext, wl.ext, sizeof(ext));
-      compose_left_rotation(q, basis);
+      compose_left_rotation(k.q, basis);
       wl = make_wall(basis, origin, ext, wl.crust, wl.material);
     }
     // Obstacle::setVelocity (Obstacle.cpp:635-640)
-    wl.vel[0] = vt.x;
-    wl.vel[1] = vt.y;
-    wl.vel[2] = vt.z;
-    wl.omega[0] = vr.x;
-    wl.omega[1] = vr.y;
-    wl.omega[2] = vr.z;
+    wl.vel[0] = k.vt.x;
+    wl.vel[1] = k.vt.y;
+    wl.vel[2] = k.vt.z;
+    wl.omega[0] = k.vr.x;
+    wl.omega[1] = k.vr.y;
+    wl.omega[2] = k.vr.z;
+  }
+  // CompositeObstacle::Move (Component/src/CompositeObstacle.cpp:135-220):
+  // the composite's own motion, composed onto each member at its lever from
+  // the (moved) centre, then the members move
+  void move_composite(Wall* walls, int nwalls, int32_t c, double time, double dt, bool mother,
+                      bool displace) {
+    Composite& C = comps[c];
+    const bool moving = kin_imposed(C.kin, motions, -1 - c, time, dt, C.centre) || mother;
+    if (moving && displace) {
+      C.centre = add(C.centre, C.kin.trans);
+      compose_left_rotation(C.kin.q, C.basis);
+    }
+    if (moving)
+      for (int32_t mbr : C.members) {
+        if (mbr < 0) {
+          Composite& M = comps[-1 - mbr];
+          kin_compose(M.kin, C.kin, sub(M.centre, C.centre));
+        } else {
+          const v3 pos = mk(walls[mbr].origin[0], walls[mbr].origin[1], walls[mbr].origin[2]);
+          kin_compose(wall_kin[mbr], C.kin, sub(pos, C.centre));
+        }
+      }
+    move_members(walls, nwalls, c, time, dt, moving, displace);
+    if (moving) {
+      C.vel = C.kin.vt;
+      C.omega = C.kin.vr;
+    }
+  }
+  void move_members(Wall* walls, int nwalls, int32_t parent, double time, double dt, bool mother,
+                    bool displace) {
+    for (int32_t k = 0; k < (int32_t)comps.size(); ++k)
+      if (comps[k].parent == parent) move_composite(walls, nwalls, k, time, dt, mother, displace);
+    for (int w = 0; w < nwalls; ++w)
+      if (wall_parent[w] == parent) move_wall(walls, w, time, dt, mother, displace);
   }
   // Obstacle part of AllComponents::Move at `time` (over [time - dt, time]);
   // returns whether any wall changed.
   bool move(Wall* walls, int nwalls, double time, double dt) {
     if (!any_live()) return false;
+    fit(nwalls);
     bool anyactive = false;
     double subint = 0.;
     for (size_t i = 0; i < motions.size(); ++i)
@@ -279,20 +395,53 @@ struct WallKinematics {
     const bool changed = anyactive || active_prev;
     if (changed) reset(walls, nwalls);
     active_prev = anyactive;
-    if (anyactive)
-      for (int w = 0; w < nwalls; ++w) move_wall(walls, w, time, dt, true);
+    if (anyactive) move_members(walls, nwalls, -1, time, dt, false, true);
     drop_completed(time, dt);
     return changed;
   }
   // AllComponents::setKinematicsObstacleWithoutMoving (AllComponents.cpp:235-251):
-  // the velocities at the initial time, walls left in place.
+  // the velocities at the initial time, obstacles left in place.
   bool init(Wall* walls, int nwalls, double time, double dt) {
     if (!any_live()) return false;
+    fit(nwalls);
     reset(walls, nwalls);
-    for (int w = 0; w < nwalls; ++w) move_wall(walls, w, time, dt, false);
+    move_members(walls, nwalls, -1, time, dt, false, false);
     drop_completed(time, dt);
     return true;
   }
 };
+
+// CylindricalShell with box elements (Component/src/CylindricalShell.cpp:13-93):
+// n boxes of edges (width, ly, height) on a circle of radius radius + width/2
+// about the shell's z axis, box i at angle (i + 1/2) 2 pi / n and rotated by
+// that angle, then placed by the shell's transform (basis, centre) with
+// Transform::composeLeftByTransform (Geometry/src/Transform.cpp:355-360) and a
+// zero translation.  Writes box i's basis (row-major), centre and half edges.
+inline void cylindrical_shell_boxes(const double* sb, const double* sc, double height,
+                                    double radius, double width, int n, double* basis,
+                                    double* centre, double* half) {
+  const double ext_rad = radius + width / 2.;
+  const double delta = 2. * kPi / double(n);
+  const double ly = 2. * (radius + width) * (sin(delta) - (1. - cos(delta)) / tan(delta));
+  for (int i = 0; i < n; ++i) {
+    const double angle = (0.5 + double(i)) * delta;
+    const double ca = cos(angle), sa = sin(angle);
+    const double cg[3] = {ext_rad * ca, ext_rad * sa, 0.};
+    const double m[9] = {ca, -sa, 0., sa, ca, 0., 0., 0., 1.};
+    double* B = basis + 9 * i;
+    double* C = centre + 3 * i;
+    for (int r = 0; r < 3; ++r) {
+      // Matrix * Vector3 (Matrix.cpp:517-520) and Matrix * Matrix (:537-549)
+      C[r] = sc[r] + (sb[3 * r] * cg[0] + sb[3 * r + 1] * cg[1] + sb[3 * r + 2] * cg[2]);
+      for (int col = 0; col < 3; ++col)
+        B[3 * r + col] = sb[3 * r] * m[col] + sb[3 * r + 1] * m[3 + col] + sb[3 * r + 2] * m[6 + col];
+      C[r] = C[r] + 0.;  // SimpleObstacle::Translate by the zero vector
+    }
+    // Box(width, ly, height) (Geometry/src/Box.cpp:27-33)
+    half[3 * i] = fabs(width) / 2.;
+    half[3 * i + 1] = fabs(ly) / 2.;
+    half[3 * i + 2] = fabs(height) / 2.;
+  }
+}
 
 }  // namespace gdem

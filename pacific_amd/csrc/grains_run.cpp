@@ -11,8 +11,13 @@
 // include/gdem.h); per wall double basis[9], centre[3], half_extent[3], crust,
 // int32 material; GDEMIN03 then: int32 nmotions, per motion int32 wall, type
 // (GDEM_MOTION_*), double t_start, t_end, p[12] (gdem_add_wall_motion's
-// parameters); then x[3n], v[3n], w[3n], q[4n], radius[n], density[n], crust[n],
-// int32 material[n].
+// parameters); GDEMIN04 then also: int32 ncomposites, per composite int32
+// parent (-1 root), nmembers, member walls[nmembers]; int32 nshells, per shell
+// double basis[9], centre[3], height, radius, width, int32 n, double crust,
+// int32 material, parent; int32 ncomposite_motions, per motion int32
+// composite, type, double t_start, t_end, p[12] (composites numbered in
+// creation order, then the shells; a shell's boxes follow the walls); then
+// x[3n], v[3n], w[3n], q[4n], radius[n], density[n], crust[n], int32 material[n].
 // Output: "GDEMOUT1", int32 n, status (0 or the GDEM_ERR_* the step raised),
 // id0, id1, then x, v, w, q, F, T of the final state.
 #include <cstdio>
@@ -40,9 +45,10 @@ int main(int argc, char** argv) {
   int32_t n, nwalls, nmodels, nsteps, periodic[3];
   double dt, origin[3], size[3], cell, grav[3];
   if (!rd(f, magic, 8) || (memcmp(magic, "GDEMIN01", 8) && memcmp(magic, "GDEMIN02", 8) &&
-                            memcmp(magic, "GDEMIN03", 8)))
+                            memcmp(magic, "GDEMIN03", 8) && memcmp(magic, "GDEMIN04", 8)))
     return 2;
-  const bool with_motions = memcmp(magic, "GDEMIN03", 8) == 0;
+  const bool with_tree = memcmp(magic, "GDEMIN04", 8) == 0;
+  const bool with_motions = with_tree || memcmp(magic, "GDEMIN03", 8) == 0;
   const int np = memcmp(magic, "GDEMIN01", 8) == 0 ? 5 : 8;
   if (!rd(f, &n, 1) || !rd(f, &nwalls, 1) ||
       !rd(f, &nmodels, 1) || !rd(f, &nsteps, 1) || !rd(f, &dt, 1) || !rd(f, origin, 3) ||
@@ -91,7 +97,9 @@ int main(int argc, char** argv) {
     w.name = "wall" + std::to_string(&w - walls.data());
   }
   // imposed velocities as the reference's XML nodes name their parameters
-  std::vector<ObstacleImposedVelocity> motions;
+  // imposed velocities, read as (target, record); target = wall index, or
+  // -1 - composite index for a composite's motion
+  std::vector<std::pair<int32_t, ObstacleImposedVelocity>> motions;
   int32_t nmotions = 0;
   if (with_motions && !rd(f, &nmotions, 1)) return 2;
   static const char* const kNames[6][12] = {
@@ -104,13 +112,12 @@ int main(int argc, char** argv) {
   static const char* const kTypes[6] = {"", "ConstantTranslation", "SinTranslation",
                                         "CrenelTranslation", "MultiSinTranslation",
                                         "ConstantRotation"};
-  for (int32_t k = 0; k < nmotions; ++k) {
-    int32_t wall, type;
+  auto read_motion = [&](bool composite) -> bool {
+    int32_t target, type;
     double t[2], p[12];
-    if (!rd(f, &wall, 1) || !rd(f, &type, 1) || !rd(f, t, 2) || !rd(f, p, 12)) return 2;
-    if (type < 1 || type > 5) return 2;
+    if (!rd(f, &target, 1) || !rd(f, &type, 1) || !rd(f, t, 2) || !rd(f, p, 12)) return false;
+    if (type < 1 || type > 5) return false;
     ObstacleImposedVelocity mo;
-    mo.obstacle = "wall" + std::to_string(wall);
     mo.type = kTypes[type];
     mo.tstart = t[0];
     mo.tend = t[1];
@@ -118,7 +125,40 @@ int main(int argc, char** argv) {
     static const int kCount[5] = {0, 3, 6, 5, 12};
     const int np_k = type == 5 ? (p[6] != 0. ? 3 : 6) : kCount[type];
     for (int i = 0; i < np_k; ++i) mo.parameters[kNames[type][i]] = p[i];
-    motions.push_back(mo);
+    motions.emplace_back(composite ? -1 - target : target, mo);
+    return true;
+  };
+  for (int32_t k = 0; k < nmotions; ++k)
+    if (!read_motion(false)) return 2;
+  // composites, box-element cylindrical shells and their motions
+  std::vector<std::pair<int32_t, std::vector<int32_t>>> composites;
+  std::vector<std::pair<CylindricalShell, int32_t>> shells;
+  if (with_tree) {
+    int32_t nc = 0, ns = 0, ncm = 0;
+    if (!rd(f, &nc, 1)) return 2;
+    for (int32_t k = 0; k < nc; ++k) {
+      int32_t parent, nm;
+      if (!rd(f, &parent, 1) || !rd(f, &nm, 1) || nm < 0) return 2;
+      std::vector<int32_t> mem((size_t)nm);
+      if (nm && !rd(f, mem.data(), (size_t)nm)) return 2;
+      composites.emplace_back(parent, mem);
+    }
+    if (!rd(f, &ns, 1)) return 2;
+    for (int32_t k = 0; k < ns; ++k) {
+      CylindricalShell sh;
+      int32_t nbox, mat, parent;
+      if (!rd(f, sh.basis, 9) || !rd(f, sh.centre, 3) || !rd(f, &sh.height, 1) ||
+          !rd(f, &sh.radius, 1) || !rd(f, &sh.width, 1) || !rd(f, &nbox, 1) || !rd(f, &sh.crust, 1) ||
+          !rd(f, &mat, 1) || !rd(f, &parent, 1))
+        return 2;
+      sh.n = nbox;
+      sh.material = "mat" + std::to_string(mat);
+      sh.name = "shell" + std::to_string(k) + "_";
+      shells.emplace_back(sh, parent);
+    }
+    if (!rd(f, &ncm, 1)) return 2;
+    for (int32_t k = 0; k < ncm; ++k)
+      if (!read_motion(true)) return 2;
   }
   std::vector<double> x(3 * n), v(3 * n), w(3 * n), q(4 * n), r(n), rho(n), cr(n);
   std::vector<int32_t> mat(n);
@@ -148,11 +188,29 @@ int main(int argc, char** argv) {
     return 3;
   }
   for (auto const& wl : walls) lc.Link(wl);
+  // names of composites in creation order, and of every wall by index
+  std::vector<std::string> comp_names, wall_names;
+  for (auto const& wl : walls) wall_names.push_back(wl.name);
+  for (auto const& c : composites) {
+    const std::string name = "comp" + std::to_string(comp_names.size());
+    lc.LinkComposite(name, c.first < 0 ? "" : comp_names.at((size_t)c.first));
+    comp_names.push_back(name);
+    for (int32_t w : c.second) lc.Append(wall_names.at((size_t)w), name);
+  }
+  for (auto const& sh : shells) {
+    lc.Link(sh.first, sh.second < 0 ? "" : comp_names.at((size_t)sh.second));
+    comp_names.push_back(sh.first.name);
+    for (int i = 0; i < sh.first.n; ++i) wall_names.push_back(sh.first.name + std::to_string(i));
+  }
   lc.Link(ps);
   std::unique_ptr<TimeIntegrator> ti(TimeIntegratorBuilderFactory::create());
   Grains app(lc, *ti);
   int32_t status = 0, id0 = -1, id1 = -1;
-  for (auto const& mo : motions) app.LinkImposedMotion(mo);
+  for (auto& tm : motions) {
+    tm.second.obstacle = tm.first >= 0 ? wall_names.at((size_t)tm.first)
+                                       : comp_names.at((size_t)(-1 - tm.first));
+    app.LinkImposedMotion(tm.second);
+  }
   try {
     app.setTimeStep(dt);  // the input's time step, before a0 (memory models use it)
     app.do_before_time_stepping();

@@ -328,6 +328,41 @@ int LinkedCell::obstacleIndex(std::string const& name) const {
   return it == m_obstacles.end() ? -1 : it->second;
 }
 
+int LinkedCell::compositeIndex(std::string const& name) const {
+  auto it = m_composites.find(name);
+  return it == m_composites.end() ? -1 : it->second;
+}
+
+void LinkedCell::LinkComposite(std::string const& name, std::string const& parent) {
+  if (!m_ctx) throw std::logic_error("LinkedCell::set must be called first");
+  const int p = parent.empty() ? -1 : compositeIndex(parent);
+  if (!parent.empty() && p < 0) throw std::invalid_argument("no composite named " + parent);
+  int32_t id = -1;
+  if (gdem_add_composite(m_ctx, p, &id)) throw std::invalid_argument("composite " + name + " rejected");
+  m_composites[name] = id;
+}
+
+void LinkedCell::Append(std::string const& obstacle, std::string const& composite) {
+  const int w = obstacleIndex(obstacle), c = compositeIndex(composite);
+  if (w < 0 || c < 0 || gdem_attach_wall(m_ctx, w, c))
+    throw std::invalid_argument("cannot append " + obstacle + " to " + composite);
+}
+
+void LinkedCell::Link(CylindricalShell const& sh, std::string const& parent) {
+  if (!m_ctx) throw std::logic_error("LinkedCell::set must be called first");
+  ContactBuilderFactory::defineMaterial(sh.material, true);
+  const int p = parent.empty() ? -1 : compositeIndex(parent);
+  if (!parent.empty() && p < 0) throw std::invalid_argument("no composite named " + parent);
+  int32_t id = -1;
+  if (gdem_add_cylindrical_shell(m_ctx, sh.basis, sh.centre, sh.height, sh.radius, sh.width, sh.n,
+                                 sh.crust, ContactBuilderFactory::materialIndex(sh.material), p,
+                                 &id))
+    throw std::invalid_argument("cylindrical shell " + sh.name + " rejected");
+  m_composites[sh.name] = id;
+  for (int i = 0; i < sh.n; ++i) m_obstacles.emplace(sh.name + std::to_string(i), m_nobstacles++);
+  push_models();
+}
+
 void LinkedCell::LinkUpdate(double, double) {
   if (gdem_link_update(m_ctx)) throw std::runtime_error("gdem_link_update failed");
 }
@@ -383,7 +418,8 @@ void Grains::setTime(double time) {
 // takes them; a rotation about a given point when CX is present.
 void Grains::LinkImposedMotion(ObstacleImposedVelocity const& mo) {
   const int wall = m_lc.obstacleIndex(mo.obstacle);
-  if (wall < 0) throw std::invalid_argument("no obstacle named " + mo.obstacle);
+  const int comp = m_lc.compositeIndex(mo.obstacle);
+  if (wall < 0 && comp < 0) throw std::invalid_argument("no obstacle named " + mo.obstacle);
   auto get = [&](const char* k) {
     auto it = mo.parameters.find(k);
     if (it == mo.parameters.end())
@@ -413,8 +449,10 @@ void Grains::LinkImposedMotion(ObstacleImposedVelocity const& mo) {
   } else {
     throw std::invalid_argument("imposed velocity type " + mo.type + " is not supported");
   }
-  if (gdem_add_wall_motion(m_lc.context(), wall, type, mo.tstart, mo.tend, p.data()))
-    throw std::invalid_argument(mo.type + " of " + mo.obstacle + " rejected");
+  const int rc =
+      comp >= 0 ? gdem_add_composite_motion(m_lc.context(), comp, type, mo.tstart, mo.tend, p.data())
+                : gdem_add_wall_motion(m_lc.context(), wall, type, mo.tstart, mo.tend, p.data());
+  if (rc) throw std::invalid_argument(mo.type + " of " + mo.obstacle + " rejected");
 }
 
 void Grains::saveParticles2014(std::string const& path, bool binary, int id_offset,

@@ -110,6 +110,12 @@ def load_library(path: str = LIB_PATH):
         "gdem_set_memory": (C.c_int, [vp, i64, vp, vp, vp, vp, vp]),
         "gdem_info": (C.c_int, [vp, P(i64), P(i32), P(i32), P(i32)]),
         "gdem_add_wall_motion": (C.c_int, [vp, i32, i32, d, d, vp]),
+        "gdem_add_composite": (C.c_int, [vp, i32, P(i32)]),
+        "gdem_attach_wall": (C.c_int, [vp, i32, i32]),
+        "gdem_add_composite_motion": (C.c_int, [vp, i32, i32, d, d, vp]),
+        "gdem_get_composite": (C.c_int, [vp, i32, vp, vp, vp]),
+        "gdem_add_cylindrical_shell": (C.c_int, [vp, vp, vp, d, d, d, i32, d, i32, i32, P(i32)]),
+        "gdem_cylindrical_shell_boxes": (C.c_int, [vp, vp, d, d, d, i32, vp, vp, vp]),
         "gdem_set_time": (C.c_int, [vp, d]),
         "gdem_get_time": (C.c_int, [vp, P(d)]),
         "gdem_move_walls": (C.c_int, [vp, d, d]),
@@ -168,7 +174,9 @@ EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall"
             "gdem_move", "gdem_advance_velocity", "gdem_prime", "gdem_step", "gdem_set_graph",
             "gdem_set_skin", "gdem_get_skin",
             "gdem_synchronize", "gdem_get_state", "gdem_set_state", "gdem_get_memory",
-            "gdem_set_memory", "gdem_info", "gdem_add_wall_motion", "gdem_set_time",
+            "gdem_set_memory", "gdem_info", "gdem_add_wall_motion", "gdem_add_composite",
+            "gdem_attach_wall", "gdem_add_composite_motion", "gdem_get_composite",
+            "gdem_add_cylindrical_shell", "gdem_cylindrical_shell_boxes", "gdem_set_time",
             "gdem_get_time", "gdem_move_walls", "gdem_get_wall", "gdem_write_particles_2014",
             "gdem_read_particles_2014", "gdem_set_contact_recording",
             "gdem_get_contacts", "gdem_last_error", "gdem_set_kernel_timing",
@@ -293,6 +301,38 @@ class GdemSystem:
         p = _f64(params, (MOTION_NPARAMS[int(kind)],))
         self._check(self.lib.gdem_add_wall_motion(self.h, int(wall), int(kind), float(t_start),
                                                   float(t_end), _ptr(p)), "gdem_add_wall_motion")
+
+    def add_composite(self, parent=-1):
+        """A composite obstacle under parent (-1: the root); returns its index."""
+        k = C.c_int32()
+        self._check(self.lib.gdem_add_composite(self.h, int(parent), C.byref(k)), "gdem_add_composite")
+        return k.value
+
+    def attach_wall(self, wall, composite):
+        self._check(self.lib.gdem_attach_wall(self.h, int(wall), int(composite)), "gdem_attach_wall")
+
+    def add_composite_motion(self, composite, kind, t_start, t_end, params):
+        p = _f64(params, (MOTION_NPARAMS[int(kind)],))
+        self._check(self.lib.gdem_add_composite_motion(self.h, int(composite), int(kind),
+                                                       float(t_start), float(t_end), _ptr(p)),
+                    "gdem_add_composite_motion")
+
+    def get_composite(self, k):
+        """(centre, velocity, angular velocity) of composite k."""
+        c, v, w = np.empty(3), np.empty(3), np.empty(3)
+        self._check(self.lib.gdem_get_composite(self.h, int(k), _ptr(c), _ptr(v), _ptr(w)),
+                    "gdem_get_composite")
+        return c, v, w
+
+    def add_cylindrical_shell(self, basis, centre, height, radius, width, n, crust, material,
+                              parent=-1):
+        """Box-element CylindricalShell as a composite of n walls; returns its index."""
+        k = C.c_int32()
+        self._check(self.lib.gdem_add_cylindrical_shell(
+            self.h, _ptr(_f64(basis, (9,))), _ptr(_f64(centre, (3,))), float(height), float(radius),
+            float(width), int(n), float(crust), int(material), int(parent), C.byref(k)),
+            "gdem_add_cylindrical_shell")
+        return k.value
 
     def set_time(self, t):
         self._check(self.lib.gdem_set_time(self.h, float(t)), "gdem_set_time")
@@ -477,9 +517,41 @@ def build_system(wl, device=0):
         s.set_time_step(wl["dt"])
     if "time" in wl:
         s.set_time(wl["time"])
+    add_obstacle_tree(s, wl)
+    return s
+
+
+def add_obstacle_tree(s, wl):
+    """Composites, shells and imposed motions of a workload dict, in the
+    order both the library and the oracle create them: composites
+    (parent, member walls), then shells (basis, centre, height, radius, width,
+    n, crust, material, parent), then further members (wall, composite), then
+    wall and composite motions."""
+    for (parent, members) in wl.get("composites", []):
+        k = s.add_composite(parent)
+        for w in members:
+            s.attach_wall(w, k)
+    for sh in wl.get("shells", []):
+        s.add_cylindrical_shell(*sh)
+    for (wall, comp) in wl.get("attach", []):
+        s.attach_wall(wall, comp)
     for (wall, kind, t0, t1, prm) in wl.get("motions", []):
         s.add_wall_motion(wall, kind, t0, t1, prm)
-    return s
+    for (comp, kind, t0, t1, prm) in wl.get("composite_motions", []):
+        s.add_composite_motion(comp, kind, t0, t1, prm)
+
+
+def cylindrical_shell_boxes(basis, centre, height, radius, width, n):
+    """The box walls of a box-element CylindricalShell (no context needed):
+    (basis (n, 9), centre (n, 3), half edges (n, 3))."""
+    lib = load_library()
+    b, c, h = np.empty((n, 9)), np.empty((n, 3)), np.empty((n, 3))
+    rc = lib.gdem_cylindrical_shell_boxes(_ptr(_f64(basis, (9,))), _ptr(_f64(centre, (3,))),
+                                          float(height), float(radius), float(width), int(n),
+                                          _ptr(b), _ptr(c), _ptr(h))
+    if rc:
+        raise GdemError(rc, "gdem_cylindrical_shell_boxes")
+    return b, c, h
 
 
 def _batch_system(wl):

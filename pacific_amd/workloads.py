@@ -185,6 +185,63 @@ def shaken_box(n=1000, seed=1, memory=False):
     return wl
 
 
+# axis of the drums along y: the shell's local z is the world's y
+DRUM_BASIS = (1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, -1.0, 0.0)
+
+
+def rotating_drum(seed=6, radius=0.01, length=0.04, nbox=24, omega=5.0, fill=0.55, memory=False,
+                  r=1.0e-3, gap=2.0e-5):
+    """Spheres (r 1 mm, density 2500) in a horizontal drum: a box-element
+    CylindricalShell (inner radius 1 cm, wall 1 mm, 24 boxes, axis along y)
+    and two end caps in one composite (the drum: caps, then the shell's own
+    composite) rotating at omega about the drum axis (y); in each 2.2 mm slice along the axis, a ring of sites gap from the
+    shell's faces (2.2 mm apart) and a jittered 2.2 mm lattice inside it, in
+    the lower part of the drum (z below (2 fill - 1) radius), v ~ N(0,
+    0.05^2); Hooke kn 1e5, en 0.9, mu 0.5 (HookeMemory when memory); dt 1e-5."""
+    rng = np.random.default_rng(seed)
+    spacing = 2.2e-3
+    ring = radius - r - gap        # a ring of sites gap from the shell's faces
+    rin = ring - spacing           # lattice sites inside it
+    zmax = (2.0 * fill - 1.0) * radius
+    dth = 2.0 * np.pi / np.floor(2.0 * np.pi * ring / spacing)   # chord >= spacing - tiny
+    pts = []
+    for y in np.arange(-length / 2 + r + gap + 1e-4, length / 2 - r - gap, spacing):
+        for th in np.arange(0.0, 2.0 * np.pi - dth / 2, dth) + 0.5 * dth:
+            if ring * np.sin(th) <= zmax:
+                pts.append((ring * np.cos(th), y, ring * np.sin(th)))
+        for xx in np.arange(-rin, rin + 1e-12, spacing):
+            for zz in np.arange(-rin, rin + 1e-12, spacing):
+                if xx * xx + zz * zz <= rin * rin and zz <= zmax:
+                    pts.append((xx, y, zz))
+    pts = np.array(pts)
+    jit = rng.uniform(-1.0e-5, 1.0e-5, size=pts.shape)
+    # ring sites move only along the shell (no jitter towards it)
+    rad = np.hypot(pts[:, 0], pts[:, 2])
+    on_ring = rad > rin + 1e-4
+    jit[on_ring, 0] = 0.0
+    jit[on_ring, 2] = 0.0
+    x = pts + jit
+    n = len(x)
+    v = rng.normal(0.0, 0.05, size=(n, 3))
+    width = 1.0e-3
+    cap = radius + width + 1.0e-3
+    walls = [(IDENTITY, (0.0, -length / 2 - 2.5e-4, 0.0), (cap, 2.5e-4, cap), CRUST, 1),
+             (IDENTITY, (0.0, length / 2 + 2.5e-4, 0.0), (cap, 2.5e-4, cap), CRUST, 1)]
+    hooke = (1.0e5, 0.9, -1.0, 0.5, 0.0)
+    models = ([(0, 0, MODEL_HOOKE_MEMORY, HOOKE_MEMORY), (0, 1, MODEL_HOOKE_MEMORY, HOOKE_MEMORY)]
+              if memory else [(0, 0, MODEL_HOOKE, hooke), (0, 1, MODEL_HOOKE, hooke)])
+    half = (radius + width + 2 * r, length / 2 + r, radius + width + 2 * r)
+    return {"name": f"rotating_drum_{n}" + ("_memory" if memory else ""),
+            "origin": tuple(-h for h in half), "size": tuple(2 * h for h in half),
+            "periodic": (0, 0, 0), "cell_size": 2.0 * r, "gravity": (0.0, 0.0, -9.81), "dt": 1.0e-5,
+            "models": models, "walls": walls, "time": 0.0,
+            # composite 0 (the drum) holds the caps and the shell (composite 1)
+            "composites": [(-1, [0, 1])],
+            "shells": [(DRUM_BASIS, (0.0, 0.0, 0.0), length, radius, width, nbox, CRUST, 1, 0)],
+            "composite_motions": [(0, 5, 0.0, 10.0, (0.0, omega, 0.0, 0.0, 0.0, 0.0, 1.0))],
+            "particles": _particles(x, v, np.full(n, r))}
+
+
 # HookeMemory {kn, en, etat, muc, kt, mur, etarpf, epst} with rolling friction
 HOOKE_MEMORY = (1.0e5, 0.9, -1.0, 0.5, 2.0e4, 0.02, 0.5, 0.0)
 # HertzMemory {Es, en, Gs, muc, mur, etarpf, epst}

@@ -23,8 +23,11 @@ def write_input(path, wl, nsteps):
     n = len(p["radius"])
     memory = any(kind >= 3 for (_, _, kind, _) in wl["models"])
     motions = wl.get("motions", [])
+    tree = bool(wl.get("composites") or wl.get("shells") or wl.get("composite_motions"))
+    assert not wl.get("attach"), "members are given with their composite"
     with open(path, "wb") as f:
-        f.write(b"GDEMIN03" if motions else b"GDEMIN02" if memory else b"GDEMIN01")
+        f.write(b"GDEMIN04" if tree else b"GDEMIN03" if motions else
+                b"GDEMIN02" if memory else b"GDEMIN01")
         f.write(struct.pack("<4i", n, len(wl["walls"]), len(wl["models"]), nsteps))
         f.write(struct.pack("<d", wl["dt"]))
         f.write(struct.pack("<3d", *wl["origin"]))
@@ -34,7 +37,7 @@ def write_input(path, wl, nsteps):
         f.write(struct.pack("<3d", *wl["gravity"]))
         for (a, b, kind, par) in wl["models"]:
             f.write(struct.pack("<3i", kind, a, b))
-            if memory or motions:
+            if memory or motions or tree:
                 f.write(struct.pack("<8d", *(list(par) + [0.0] * (8 - len(par)))))
             else:
                 f.write(struct.pack("<5d", *par))
@@ -44,10 +47,25 @@ def write_input(path, wl, nsteps):
             f.write(struct.pack("<3d", *half))
             f.write(struct.pack("<d", crust))
             f.write(struct.pack("<i", mat))
-        if motions:
+        if motions or tree:
             f.write(struct.pack("<i", len(motions)))
             for (wall, kind, t0, t1, prm) in motions:
                 f.write(struct.pack("<2i2d", wall, kind, t0, t1))
+                f.write(struct.pack("<12d", *(list(prm) + [0.0] * (12 - len(prm)))))
+        if tree:
+            comps = wl.get("composites", [])
+            f.write(struct.pack("<i", len(comps)))
+            for (parent, members) in comps:
+                f.write(struct.pack(f"<2i{len(members)}i", parent, len(members), *members))
+            shells = wl.get("shells", [])
+            f.write(struct.pack("<i", len(shells)))
+            for (basis, centre, height, radius, width, n, crust, mat, parent) in shells:
+                f.write(struct.pack("<9d3d3d", *basis, *centre, height, radius, width))
+                f.write(struct.pack("<id2i", n, crust, mat, parent))
+            cm = wl.get("composite_motions", [])
+            f.write(struct.pack("<i", len(cm)))
+            for (comp, kind, t0, t1, prm) in cm:
+                f.write(struct.pack("<2i2d", comp, kind, t0, t1))
                 f.write(struct.pack("<12d", *(list(prm) + [0.0] * (12 - len(prm)))))
         for k in ("x", "v", "w", "q", "radius", "density", "crust"):
             f.write(np.ascontiguousarray(p[k], dtype="<f8").tobytes())
@@ -133,6 +151,22 @@ def test_driver_wall_motions_match_oracle(oracle_mod):
     ou = orc_u.state()
     for k in ("x", "v", "w", "q"):
         assert rel_err(g[k], o[k]) <= max(1e-9, 10.0 * rel_err(ou[k], o[k])), k
+
+
+def test_driver_rotating_drum_matches_oracle(oracle_mod):
+    """CylindricalShell and CompositeObstacle through the host classes
+    (LinkComposite, Append, Link(CylindricalShell)) with a ConstantRotation
+    on the drum composite."""
+    print("[case] grains_run rotating drum 150 steps", flush=True)
+    wl = workloads.rotating_drum()
+    status, _, _, g = run(wl, 150)
+    assert status == 0
+    orc = oracle_mod.Oracle(wl)
+    orc.prime()
+    orc.step(150, wl["dt"])
+    o = orc.state()
+    for k in ("x", "v", "w", "q"):
+        assert rel_err(g[k], o[k]) <= 1e-9, k
 
 
 def test_driver_reports_contact_error():

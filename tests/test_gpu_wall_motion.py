@@ -163,3 +163,70 @@ def test_wall_motion_arguments():
     with pytest.raises(gdem.GdemError):
         conv.add_wall_motion(0, gdem.MOTION_CONSTANT_TRANSLATION, 0.0, 1.0, (1.0, 0.0, 0.0))
     conv.close()
+
+
+def drum_case(memory):
+    return workloads.rotating_drum(memory=memory)
+
+
+@pytest.mark.parametrize("skin", ["auto", "off"])
+@pytest.mark.parametrize("memory", [False, True])
+def test_rotating_drum_matches_oracle(oracle_mod, memory, skin):
+    """A box-element cylindrical shell and its end caps turning as one
+    composite (CylindricalShell + ConstantRotation on the composite)."""
+    print(f"[case] rotating drum, memory {memory}, skin {skin}, 300 steps", flush=True)
+    wl = drum_case(memory)
+    n = len(wl["particles"]["radius"])
+    dt = wl["dt"]
+    orc = oracle_mod.Oracle(wl, log_capacity=10 * n)
+    orc.prime()
+    wl_u = drum_case(memory)
+    v = wl_u["particles"]["v"]
+    up = np.random.default_rng(7).random(v.shape) < 0.5
+    wl_u["particles"]["v"] = np.nextafter(v, np.where(up, np.inf, -np.inf))
+    orc_u = oracle_mod.Oracle(wl_u)
+    orc_u.prime()
+    s = gdem.build_system(wl)
+    if skin == "off":
+        s.set_skin(0.0)
+    s.set_contact_recording(32)
+    x0 = wl["particles"]["x"]
+    nwalls = len(wl["walls"]) + wl["shells"][0][5]
+    shell_contacts = 0
+    for calls in range(6):
+        orc.step(50, dt)
+        orc_u.step(50, dt)
+        s.step(50, dt)
+        _walls_equal(s, orc, nwalls, f"after {50 * (calls + 1)} steps")
+        for a, b in zip(s.get_composite(0), orc.composite(0)):
+            np.testing.assert_array_equal(a, b)
+        pairs = pair_set(s.get_contacts())
+        assert pairs == pair_set(orc.contacts()), calls
+        shell_contacts += sum(1 for (_, b) in pairs if b <= -3)
+        o = orc.state()
+        self_div = _errors(orc_u.state(), o, x0)
+        bars = [max(1e-9, 10.0 * e) for e in self_div]
+        errs = _errors(s.get_state(), o, x0)
+        print(f"[case] steps {50 * (calls + 1)}: x/v/w error {errs}, oracle one-ulp "
+              f"divergence {self_div}", flush=True)
+        assert all(e <= b for e, b in zip(errs, bars)), (calls, errs, bars)
+    assert shell_contacts >= 10, shell_contacts
+    s.close()
+
+
+def test_sphere_in_spinning_drum_on_gpu():
+    print("[case] sphere in a spinning drum", flush=True)
+    omega = 5.0
+    hooke = (1.0e5, 0.5, -1.0, 0.3, 0.0)
+    wl = workloads.rotating_drum(nbox=26, omega=omega)
+    wl["composite_motions"] = [(0, gdem.MOTION_CONSTANT_ROTATION, -1.0, 10.0,
+                                (0.0, omega, 0.0, 0.0, 0.0, 0.0, 1.0))]
+    m = workloads.DENSITY * 4.0 * math.pi * 1e-9 / 3.0
+    wl["models"] = [(0, 0, gdem.MODEL_HOOKE, hooke), (0, 1, gdem.MODEL_HOOKE, hooke)]
+    wl["particles"] = workloads._particles(np.array([[0.0, 0.0, -0.01 + 1.0e-3 - m * 9.81 / 1.0e5]]),
+                                           np.zeros((1, 3)), np.array([1.0e-3]))
+    s = gdem.build_system(wl)
+    s.prime()
+    F = s.get_state()["F"][0]
+    assert F[0] == pytest.approx(-0.3 * m * 9.81, rel=1e-12)
+    s.close()
