@@ -317,8 +317,8 @@ def main():
                     help="calibration: skip the looped single-system baseline")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: the DEM step is not sharded "
-                                   "across GPUs (see DESIGN.md, multi-GPU design)"}))
+        print(json.dumps({"error": "the DEM step runs on one GPU: it is not sharded across "
+                                   "GPUs (the multi-GPU decomposition is a design only)"}))
         sys.exit(2)
 
     from pacific_amd import gdem

@@ -1,7 +1,7 @@
 """Pins of the CPU oracle against closed forms, brute force and invariants.
 
 The reference's C++ cannot be built here (every geometry header includes
-WriterXML.hh, which needs Xerces-C), and its own tests carry no
+WriterXML.hh, which needs Xerces-C, absent from this image), and its own tests carry no
 numeric fixture for the Hooke/Hertz/leap-frog path, so each oracle function is
 checked against mathematics that a plausible slip (swapped radii, wrong sign,
 dropped term, wrong index) would break.
