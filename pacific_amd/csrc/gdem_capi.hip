@@ -170,7 +170,7 @@ extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return GDEM_ERR_HIP; }
   std::vector<int32_t> rank = row_morton_rank(c->grid.nc);
   const int64_t nrows = (int64_t)c->grid.nc[1] * c->grid.nc[2];
-  const int nb = (int)((ncell + 1023) / 1024);
+  const int nb = (int)((ncell + kScanCells - 1) / kScanCells);
   c->bins.nbsum = nb;
   std::vector<int32_t> row_jk((size_t)nrows);
   for (int64_t q = 0; q < nrows; ++q) row_jk[(size_t)rank[(size_t)q]] = (int32_t)q;

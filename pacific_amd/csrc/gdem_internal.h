@@ -45,6 +45,11 @@ struct ContactRec {
 // Maximum contacts one particle can detect (outgoing) or receive (incoming)
 // per step; exceeding it raises GDEM_ERR_CAPACITY.
 constexpr int kListCap = 24;
+
+// Cells per block of the binning scan (1,024 threads x 4 cells); the
+// block totals bsum[] are indexed by cell / kScanCells.
+constexpr int kScanPer = 4;
+constexpr int kScanCells = 1024 * kScanPer;
 static_assert(kListCap < 32, "list entries are tracked in 32-bit masks");
 
 // Device error words, min-reduced with integer atomics so the reported

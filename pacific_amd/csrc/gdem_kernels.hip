@@ -27,6 +27,11 @@ namespace gdem {
 
 constexpr int kBlock = 256;
 constexpr int kScanBlock = 1024;
+static_assert(kScanCells == kScanBlock * kScanPer, "scan block geometry");
+// Workgroups of the binning kernels when a gate word may skip them: a
+// grid-stride grid this size exits in about a microsecond when no rebuild is
+// due, where a full-size grid costs several.
+constexpr int kGateGrid = 1024;
 
 // ---------------------------------------------------------------- keys
 __global__ void __launch_bounds__(kBlock) k_keys(State s, int n, Grid g, const int32_t* row_rank,
@@ -56,22 +61,26 @@ __global__ void __launch_bounds__(kBlock) k_count(const int32_t* key, int32_t* s
     if (*gate != 0) *rebuilds += 1;
   }
   if (gate && *gate == 0) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  const int k = i < n ? key[i] : -1;
-  const int kprev = __shfl_up(k, 1, 64);
-  const bool head = (lane == 0) || (k != kprev);
-  const unsigned long long heads = __ballot(head);
-  const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-  const int head_lane = 63 - __clzll(heads & upto);
-  const unsigned long long after = heads & ~upto;
-  const int next_head = after ? __ffsll((long long)after) - 1 : 64;
-  int base = 0;
-  if (head && k >= 0) base = atomicAdd(&cnt[k], next_head - lane);
-  base = __shfl(base, head_lane, 64);
-  if (i < n) slot[i] = base + (lane - head_lane);
-  // skin lists: the rebuild's sweep appends to cleared incoming counts
-  if (in_count && i < n) in_count[i] = 0;
+  // grid-stride over blocks of particles: the loop bound is uniform in the
+  // block, so every lane of a wave takes part in the shuffles and ballots
+  for (int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
+    const int i = b0 + threadIdx.x;
+    const int k = i < n ? key[i] : -1;
+    const int kprev = __shfl_up(k, 1, 64);
+    const bool head = (lane == 0) || (k != kprev);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const int head_lane = 63 - __clzll(heads & upto);
+    const unsigned long long after = heads & ~upto;
+    const int next_head = after ? __ffsll((long long)after) - 1 : 64;
+    int base = 0;
+    if (head && k >= 0) base = atomicAdd(&cnt[k], next_head - lane);
+    base = __shfl(base, head_lane, 64);
+    if (i < n) slot[i] = base + (lane - head_lane);
+    // skin lists: the rebuild's sweep appends to cleared incoming counts
+    if (in_count && i < n) in_count[i] = 0;
+  }
 }
 
 template <int BS>
@@ -101,22 +110,32 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* smem, int& total
 }
 
 // Exclusive scan of the cell counts in one launch: each block scans its
-// kScanBlock cells and publishes its total; the last block to finish (ticket)
-// scans the totals in place (nb <= 8 * kScanBlock) and resets the ticket.
-// No block waits for another.
+// kScanCells cells (kScanPer consecutive cells per thread) and publishes its
+// total; the last block to finish (ticket) scans the totals in place
+// (nb <= 8 * kScanBlock) and resets the ticket.  No block waits for another.
 __global__ void __launch_bounds__(kScanBlock) k_scan(int32_t* cnt, int32_t* local, int32_t* bsum,
                                                      int32_t* ticket, int ncell, int nb,
                                                      const int32_t* gate) {
   if (gate && *gate == 0) return;
   __shared__ int smem[kScanBlock / 64];
   __shared__ int last;
-  int c = blockIdx.x * kScanBlock + threadIdx.x;
-  int v = c < ncell ? cnt[c] : 0;
+  const int c0 = blockIdx.x * kScanCells + threadIdx.x * kScanPer;
+  int v[kScanPer];
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    v[k] = c0 + k < ncell ? cnt[c0 + k] : 0;
+    sum += v[k];
+  }
   int total;
-  int ex = block_exclusive_scan<kScanBlock>(v, smem, total);
-  if (c < ncell) {
-    local[c] = ex;
-    cnt[c] = 0;  // ready for the next count pass (no memset node in graphs)
+  int ex = block_exclusive_scan<kScanBlock>(sum, smem, total);
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    if (c0 + k < ncell) {
+      local[c0 + k] = ex;
+      cnt[c0 + k] = 0;  // ready for the next count pass (no memset node in graphs)
+    }
+    ex += v[k];
   }
   if (threadIdx.x == 0) {
     bsum[blockIdx.x] = total;
@@ -148,26 +167,18 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const int32_t* key, const in
                                                     const int32_t* local, const int32_t* bsum,
                                                     int32_t* perm, int n, const int32_t* gate) {
   if (gate && *gate == 0) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int k = key[i];
-  perm[local[k] + bsum[k / kScanBlock] + slot[i]] = i;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int k = key[i];
+    perm[local[k] + bsum[k / kScanCells] + slot[i]] = i;
+  }
 }
 
-// Orders each cell by previous slot, which makes the permutation independent of
-// the order in which the count pass's atomics were served, then writes the
-// cell's entries of the sorted single-precision copy of position + radius
-// (read by the detection sweep's conservative pre-filter only: every contact
-// decision is made in FP64 on the full-precision state).
-__global__ void __launch_bounds__(kBlock) k_cellsort(const int32_t* local, const int32_t* bsum,
-                                                     int32_t* perm, int32_t* cell_start,
-                                                     const double4* pos_r, float4* pos_sorted,
-                                                     int ncell, int n, const int32_t* gate) {
-  if (gate && *gate == 0) return;
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  int beg = local[c] + bsum[c / kScanBlock];
-  int end = (c + 1 < ncell) ? local[c + 1] + bsum[(c + 1) / kScanBlock] : n;
+// One cell of k_cellsort.
+__device__ __forceinline__ void sort_cell(const int32_t* local, const int32_t* bsum, int32_t* perm,
+                                          int32_t* cell_start, const double4* pos_r,
+                                          float4* pos_sorted, int c, int ncell, int n) {
+  int beg = local[c] + bsum[c / kScanCells];
+  int end = (c + 1 < ncell) ? local[c + 1] + bsum[(c + 1) / kScanCells] : n;
   for (int a = beg + 1; a < end; ++a) {
     int v = perm[a];
     int b = a - 1;
@@ -183,6 +194,20 @@ __global__ void __launch_bounds__(kBlock) k_cellsort(const int32_t* local, const
     const double4 p = pos_r[perm[u]];
     pos_sorted[u] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
   }
+}
+
+// Orders each cell by previous slot, which makes the permutation independent of
+// the order in which the count pass's atomics were served, then writes the
+// cell's entries of the sorted single-precision copy of position + radius
+// (read by the detection sweep's conservative pre-filter only: every contact
+// decision is made in FP64 on the full-precision state).
+__global__ void __launch_bounds__(kBlock) k_cellsort(const int32_t* local, const int32_t* bsum,
+                                                     int32_t* perm, int32_t* cell_start,
+                                                     const double4* pos_r, float4* pos_sorted,
+                                                     int ncell, int n, const int32_t* gate) {
+  if (gate && *gate == 0) return;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x)
+    sort_cell(local, bsum, perm, cell_start, pos_r, pos_sorted, c, ncell, n);
 }
 
 // ---------------------------------------------------------------- detect
@@ -320,17 +345,20 @@ void launch_keys(const State& s, int32_t n, const Grid& g, const int32_t* row_ra
 void launch_binning(Bins& b, const State& s, int32_t n, int32_t ncell, hipStream_t st,
                     const int32_t* gate, int32_t* in_count, int32_t* clear_need,
                     int32_t* rebuilds) {
-  // one block at least: with skin lists the count pass also clears a word
-  hipLaunchKernelGGL(k_count, dim3(nblocks(n > 0 ? n : 1, kBlock)), dim3(kBlock), 0, st, b.key,
-                     b.slot, b.cnt, n, gate, in_count, clear_need, rebuilds);
-  const int nb = nblocks(ncell, kScanBlock);
+  // with a gate word (skin lists) the grids are capped and the kernels
+  // stride; one block at least: the count pass also clears a word
+  auto capped = [gate](int blocks) { return gate ? std::min(blocks, kGateGrid) : blocks; };
+  hipLaunchKernelGGL(k_count, dim3(capped(nblocks(n > 0 ? n : 1, kBlock))), dim3(kBlock), 0, st,
+                     b.key, b.slot, b.cnt, n, gate, in_count, clear_need, rebuilds);
+  const int nb = nblocks(ncell, kScanCells);
   hipLaunchKernelGGL(k_scan, dim3(nb), dim3(kScanBlock), 0, st, b.cnt, b.local, b.bsum, b.ticket,
                      ncell, nb, gate);
   if (n > 0)
-    hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, b.key, b.slot,
-                       b.local, b.bsum, b.perm, n, gate);
-  hipLaunchKernelGGL(k_cellsort, dim3(nblocks(ncell, kBlock)), dim3(kBlock), 0, st, b.local,
-                     b.bsum, b.perm, b.cell_start, (const double4*)s.pos_r, b.pos, ncell, n, gate);
+    hipLaunchKernelGGL(k_scatter, dim3(capped(nblocks(n, kBlock))), dim3(kBlock), 0, st, b.key,
+                       b.slot, b.local, b.bsum, b.perm, n, gate);
+  hipLaunchKernelGGL(k_cellsort, dim3(capped(nblocks(ncell, kBlock))), dim3(kBlock), 0, st,
+                     b.local, b.bsum, b.perm, b.cell_start, (const double4*)s.pos_r, b.pos, ncell,
+                     n, gate);
 }
 
 void launch_detect(const DetectArgs& a, hipStream_t st) {
