@@ -212,6 +212,16 @@ int gdem_cylindrical_shell_boxes(const double* basis, const double* centre, doub
                                  double radius, double width, int32_t n, double* wall_basis,
                                  double* wall_centre, double* wall_half);
 
+/* Loads on the walls (AllComponents::computeObstaclesLoad,
+   Component/src/AllComponents.cpp:1685-1693): with tracking on, every force
+   evaluation also sums, per wall, the reactions of its contacts (force, and
+   torque about the wall's centre, rolling torques included; Obstacle::addForce /
+   addTorque, Component/src/Obstacle.cpp:575-597), in a fixed order without
+   atomics.  gdem_get_wall_loads returns those of the last evaluation,
+   3 * nwalls values each (either pointer may be NULL).  Sphere contexts. */
+int gdem_set_wall_loads(gdem_ctx* ctx, int32_t on);
+int gdem_get_wall_loads(gdem_ctx* ctx, double* force, double* torque);
+
 /* Simulation time (Grains::m_time, Base/src/Grains.cpp:114, :290); gdem_step
    advances it by dt per step. */
 int gdem_set_time(gdem_ctx* ctx, double time);

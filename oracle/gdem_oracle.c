@@ -38,6 +38,7 @@ typedef struct {
   double bb_centre[3], bb_ext[3];
   double crust, req, corners[8][3];
   double vel[3], omega[3]; /* Obstacle::m_translationalVelocity, m_angularVelocity */
+  double load_f[3], load_t[3]; /* torsor of the last force evaluation, about the centre */
   int material;
 } wall_t;
 
@@ -714,6 +715,14 @@ int orc_set_time(orc_sys* s, double time) {
 
 double orc_get_time(const orc_sys* s) { return s->time; }
 
+int orc_get_wall_loads(const orc_sys* s, double* force, double* torque) {
+  for (int w = 0; w < s->nwalls; ++w) {
+    if (force) memcpy(force + 3 * w, s->walls[w].load_f, sizeof(double) * 3);
+    if (torque) memcpy(torque + 3 * w, s->walls[w].load_t, sizeof(double) * 3);
+  }
+  return ORC_OK;
+}
+
 int orc_get_wall(const orc_sys* s, int w, double* centre, double* basis, double* vel,
                  double* omega) {
   if (w < 0 || w >= s->nwalls) return ORC_ERR_ARG;
@@ -1324,6 +1333,16 @@ static int interact_wall(orc_sys* s, int p, int wi) {
   st(s->F + 3 * p, add(ld(s->F + 3 * p), f));
   st(s->T + 3 * p, add(ld(s->T + 3 * p), cross(sub(pt, A), f)));
   if (rolling) st(s->T + 3 * p, add(ld(s->T + 3 * p), scl(delM, 1.)));
+  /* the reaction on the wall, the second body (HookeContactForceModel.cpp:
+     150-153, Obstacle::addForce / addTorque, Obstacle.cpp:575-597, Torsor::
+     addForce, Torsor.cpp:184-189) */
+  {
+    wall_t* ww_ = &s->walls[wi];
+    v3 fw = neg(f);
+    st(ww_->load_f, add(ld(ww_->load_f), fw));
+    st(ww_->load_t, add(ld(ww_->load_t), cross(sub(pt, ld(ww_->origin)), fw)));
+    if (rolling) st(ww_->load_t, add(ld(ww_->load_t), neg(delM)));
+  }
   log_contact(s, p, -1 - wi, pt, ov, dist, f);
   return ORC_OK;
 }
@@ -1370,6 +1389,10 @@ static int compute_forces_body(orc_sys* s) {
   const double* g = s->cfg.gravity;
   s->log_n = 0;
   s->ncand = 0;
+  /* obstacle torsors at their centres, zero (Component::InitializeForce,
+     Component.cpp:460-463, from AllComponents::InitializeForces :268-269) */
+  for (int w = 0; w < s->nwalls; ++w)
+    for (int d = 0; d < 3; ++d) s->walls[w].load_f[d] = s->walls[w].load_t[d] = 0.;
   for (int p = 0; p < s->n; ++p) {
     double m = s->mass[p] * (1. - 0. / 1.);
     s->F[3 * p] = g[0] * m;

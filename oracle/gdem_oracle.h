@@ -107,6 +107,8 @@ double orc_get_time(const orc_sys* s);
 int orc_move_walls(orc_sys* s, double time, double dt);
 int orc_get_wall(const orc_sys* s, int wall, double* centre, double* basis, double* vel,
                  double* omega);
+/* wall torsors of the last force evaluation (3 * nwalls each) */
+int orc_get_wall_loads(const orc_sys* s, double* force, double* torque);
 /* dt of the force evaluations (memory models); orc_step sets it too. */
 int orc_set_time_step(orc_sys* s, double dt);
 /* Contact memory of particle p (its own map): up to cap entries of key

@@ -53,6 +53,7 @@ def load():
             "orc_set_contact_model": (i32, [vp, i32, i32, i32, vp]),
             "orc_set_time_step": (i32, [vp, d]),
             "orc_add_wall_motion": (i32, [vp, i32, i32, d, d, vp]),
+            "orc_get_wall_loads": (i32, [vp, vp, vp]),
             "orc_add_composite": (i32, [vp, i32, P(i32)]),
             "orc_attach_wall": (i32, [vp, i32, i32]),
             "orc_add_composite_motion": (i32, [vp, i32, i32, d, d, vp]),
@@ -121,6 +122,7 @@ class Oracle:
             self._chk(self.lib.orc_set_contact_model(h, a, b, kind, _p(_f(p, (npar,)))), "model")
         if "dt" in wl:
             self._chk(self.lib.orc_set_time_step(h, float(wl["dt"])), "time step")
+        self.nwalls = len(wl["walls"]) + sum(int(sh[5]) for sh in wl.get("shells", []))
         for (basis, centre, half, crust, mat) in wl["walls"]:
             self._chk(self.lib.orc_add_wall(h, _p(_f(basis, (9,))), _p(_f(centre, (3,))),
                                             _p(_f(half, (3,))), crust, mat), "wall")
@@ -194,6 +196,12 @@ class Oracle:
 
     def move_walls(self, time, dt):
         self._chk(self.lib.orc_move_walls(self.h, float(time), float(dt)), "move walls")
+
+    def wall_loads(self):
+        nw = self.nwalls
+        f, t = np.zeros((max(nw, 1), 3)), np.zeros((max(nw, 1), 3))
+        self.lib.orc_get_wall_loads(self.h, _p(f), _p(t))
+        return f[:nw], t[:nw]
 
     def composite(self, k):
         c, v, w = np.empty(3), np.empty(3), np.empty(3)

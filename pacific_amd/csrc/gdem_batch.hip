@@ -190,6 +190,7 @@ struct BatchRunner {
   __device__ __forceinline__ void detect(const State& cur) {
     if (threadIdx.x == 0) {
       DetectArgs& d = *sd;
+      d = DetectArgs{};  // fields not set below are null / zero
       d.pos = a.pos + sy.off;
       d.sizef[0] = sy.sizef[0]; d.sizef[1] = sy.sizef[1]; d.sizef[2] = sy.sizef[2];
       d.margin = sy.margin;
@@ -218,6 +219,7 @@ struct BatchRunner {
   __device__ __forceinline__ void force(const State& cur, const State& nxt, double dtv, double dtd, bool write_ft) {
     if (threadIdx.x == 0) {
       ForceArgs& f = *sf;
+      f = ForceArgs{};  // every field not set below is null / zero (no wall loads, no history)
       f.in = cur;
       f.out = nxt;
       f.perm = a.perm + sy.off;

@@ -105,6 +105,15 @@ struct gdem_ctx {
   Wall* d_traj = nullptr;
   size_t traj_cap = 0;
   const Wall* step_walls = nullptr;
+  // wall loads of the last force evaluation (when tracked): per-particle wall
+  // contacts, block partials and totals
+  bool track_loads = false;
+  int32_t* d_wl_count = nullptr;
+  int32_t* d_wl_wall = nullptr;
+  double4* d_wl_f = nullptr;
+  double4* d_wl_t = nullptr;
+  double4* d_wl_part = nullptr;
+  double4* d_wall_load = nullptr;
 };
 
 #define HIP_TRY(expr)                                                        \
@@ -205,6 +214,35 @@ extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
 
 static void free_history(gdem_ctx* c);
 
+static void free_wall_loads(gdem_ctx* c) {
+  (void)hipFree(c->d_wl_count); (void)hipFree(c->d_wl_wall); (void)hipFree(c->d_wl_f);
+  (void)hipFree(c->d_wl_t); (void)hipFree(c->d_wl_part); (void)hipFree(c->d_wall_load);
+  c->d_wl_count = c->d_wl_wall = nullptr;
+  c->d_wl_f = c->d_wl_t = c->d_wl_part = c->d_wall_load = nullptr;
+}
+
+// Buffers of the wall-load reduction for the current particle count.
+static int ensure_wall_loads(gdem_ctx* c) {
+  free_wall_loads(c);
+  c->graph_gen++;
+  if (!c->track_loads) return GDEM_OK;
+  const size_t n = (size_t)std::max<int64_t>(c->n, 1);
+  const size_t nb = (n + 255) / 256;
+  if (hipMalloc(&c->d_wl_count, sizeof(int32_t) * n) != hipSuccess ||
+      hipMalloc(&c->d_wl_wall, sizeof(int32_t) * n * kWallLoadCap) != hipSuccess ||
+      hipMalloc(&c->d_wl_f, sizeof(double4) * n * kWallLoadCap) != hipSuccess ||
+      hipMalloc(&c->d_wl_t, sizeof(double4) * n * kWallLoadCap) != hipSuccess ||
+      hipMalloc(&c->d_wl_part, sizeof(double4) * nb * kMaxWalls * 2) != hipSuccess ||
+      hipMalloc(&c->d_wall_load, sizeof(double4) * kMaxWalls * 2) != hipSuccess) {
+    free_wall_loads(c);
+    return GDEM_ERR_HIP;
+  }
+  if (hipMemset(c->d_wl_count, 0, sizeof(int32_t) * n) != hipSuccess ||
+      hipMemset(c->d_wall_load, 0, sizeof(double4) * kMaxWalls * 2) != hipSuccess)
+    return GDEM_ERR_HIP;
+  return GDEM_OK;
+}
+
 extern "C" int gdem_destroy(gdem_ctx* c) {
   if (!c) return GDEM_OK;
   (void)hipSetDevice(c->dev);
@@ -224,6 +262,7 @@ extern "C" int gdem_destroy(gdem_ctx* c) {
   (void)hipFree(c->d_nwork); (void)hipFree(c->d_shapes);
   (void)hipFree(c->d_skin); (void)hipFree(c->d_xref); (void)hipFree(c->d_cmask);
   (void)hipFree(c->d_wall_shapes); (void)hipFree(c->d_wall_sc); (void)hipFree(c->d_traj);
+  free_wall_loads(c);
   for (auto& e : c->ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
   for (auto& g : c->graph_exec)
     if (g) (void)hipGraphExecDestroy(g);
@@ -332,6 +371,10 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
   c->primed = false;
   c->binned = false;
   c->rec_valid = false;
+  if (c->track_loads && !convex) {
+    const int rl = ensure_wall_loads(c);
+    if (rl) return rl;
+  }
   return ensure_history(c);
 }
 
@@ -864,6 +907,13 @@ static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wr
       a.hout = c->hist[1 - c->cur];
       a.dt_mem = c->dt_mem;
     }
+    if (c->track_loads) {
+      if (!c->d_wl_count) return GDEM_ERR_ARG;
+      a.wl_count = c->d_wl_count;
+      a.wl_wall = c->d_wl_wall;
+      a.wl_f = c->d_wl_f;
+      a.wl_t = c->d_wl_t;
+    }
     if (skin)
       launch_detect_screen(d, a, c->stream);
     else
@@ -874,6 +924,10 @@ static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wr
     launch_force(mode, record, a, c->stream, ev ? ev->first : nullptr, ev ? ev->second : nullptr,
                  c->has_memory);
     HIP_TRY(hipGetLastError());
+    if (c->track_loads) {
+      launch_wall_loads(a, c->d_wl_part, c->d_wall_load, c->stream);
+      HIP_TRY(hipGetLastError());
+    }
     *evp = ev;
     return GDEM_OK;
   }
@@ -1393,6 +1447,30 @@ extern "C" int gdem_get_composite(gdem_ctx* c, int32_t composite, double* centre
   if (vel) memcpy(vel, ve, sizeof(ve));
   if (omega) memcpy(omega, om, sizeof(om));
   return GDEM_OK;
+}
+
+extern "C" int gdem_set_wall_loads(gdem_ctx* c, int32_t on) {
+  if (!c || c->convex) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->track_loads = on != 0;
+  return ensure_wall_loads(c);
+}
+
+extern "C" int gdem_get_wall_loads(gdem_ctx* c, double* force, double* torque) {
+  if (!c || !c->track_loads || !c->d_wall_load) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  double4 h[kMaxWalls * 2];
+  HIP_TRY(hipMemcpyAsync(h, c->d_wall_load, sizeof(double4) * 2 * (size_t)std::max(c->nwalls, 1),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int w = 0; w < c->nwalls; ++w) {
+    if (force) { force[3 * w] = h[2 * w].x; force[3 * w + 1] = h[2 * w].y; force[3 * w + 2] = h[2 * w].z; }
+    if (torque) {
+      torque[3 * w] = h[2 * w + 1].x; torque[3 * w + 1] = h[2 * w + 1].y; torque[3 * w + 2] = h[2 * w + 1].z;
+    }
+  }
+  return collect_errors(c);
 }
 
 extern "C" int gdem_set_time(gdem_ctx* c, double time) {

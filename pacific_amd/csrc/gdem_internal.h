@@ -141,7 +141,16 @@ struct ForceArgs {
   History hin, hout;
   double dt_mem;        // time step of the memory update
   double half_skin2;    // (skin / 2)^2
+  // wall loads (when tracked): per particle up to kWallLoadCap wall contacts
+  // [k * lstride + slot]: the wall, the force on it and its torque about the
+  // wall's centre (Obstacle::addForce / addTorque); null when not tracked
+  int32_t* wl_count;
+  int32_t* wl_wall;
+  double4* wl_f;
+  double4* wl_t;
 };
+
+constexpr int kWallLoadCap = 4;
 
 enum { MODE_FORCE = 0, MODE_KICK = 1, MODE_MOVE = 2 };
 
@@ -274,6 +283,10 @@ void launch_detect(const DetectArgs& a, hipStream_t st);
 // the history-model variant (a.hin / a.hout used).
 void launch_force(int mode, bool record, const ForceArgs& a, hipStream_t st,
                   hipEvent_t start = nullptr, hipEvent_t stop = nullptr, bool memory = false);
+// Loads of a.nwalls walls from the wall contacts the force kernel stored
+// (a.wl_*): block partials part[nblocks(n)][nwalls][2], totals out[nwalls][2]
+// (force, torque about the wall's centre).
+void launch_wall_loads(const ForceArgs& a, double4* part, double4* out, hipStream_t st);
 // Skin lists: candidate sweep (d.gate set) or screening of the lists (writes
 // f.cmask), in one launch.
 void launch_detect_screen(const DetectArgs& d, const ForceArgs& f, hipStream_t st);

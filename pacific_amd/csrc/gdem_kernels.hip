@@ -249,6 +249,66 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a) {
   }
 }
 
+// ---------------------------------------------------------------- wall loads
+// Load of every wall from one force evaluation (AllComponents::computeObstaclesLoad,
+// Component/src/AllComponents.cpp:1685-1693, of the walls' torsors): each
+// block sums the wall contacts of its kBlock particles in slot order, thread w
+// for wall w; one block per wall then adds the block partials with a fixed
+// tree.  Deterministic, no atomics.
+__global__ void __launch_bounds__(kBlock) k_wall_load_partial(const int32_t* count,
+                                                              const int32_t* wall,
+                                                              const double4* f, const double4* tq,
+                                                              int n, int lstride, int nwalls,
+                                                              double4* part) {
+  __shared__ int32_t s_cnt[kBlock];
+  const int b0 = blockIdx.x * kBlock;
+  const int i = b0 + threadIdx.x;
+  s_cnt[threadIdx.x] = i < n ? count[i] : 0;
+  __syncthreads();
+  const int w = threadIdx.x;
+  if (w >= nwalls) return;
+  v3 F = mk(0., 0., 0.), T = mk(0., 0., 0.);
+  for (int j = 0; j < kBlock; ++j) {
+    const int nc = s_cnt[j];
+    for (int k = 0; k < nc; ++k) {
+      const size_t at = (size_t)k * lstride + b0 + j;
+      if (wall[at] != w) continue;
+      const double4 fa = f[at], ta = tq[at];
+      F = add(F, mk(fa.x, fa.y, fa.z));
+      T = add(T, mk(ta.x, ta.y, ta.z));
+    }
+  }
+  const size_t o = ((size_t)blockIdx.x * nwalls + w) * 2;
+  part[o] = make_double4(F.x, F.y, F.z, 0.);
+  part[o + 1] = make_double4(T.x, T.y, T.z, 0.);
+}
+
+__global__ void __launch_bounds__(256) k_wall_load_final(const double4* part, int nblk, int nwalls,
+                                                         double4* out) {
+  __shared__ double s[6][256];
+  const int w = blockIdx.x, tid = threadIdx.x;
+  double acc[6] = {0., 0., 0., 0., 0., 0.};
+  for (int b = tid; b < nblk; b += 256) {
+    const size_t o = ((size_t)b * nwalls + w) * 2;
+    const double4 fa = part[o], ta = part[o + 1];
+    acc[0] += fa.x; acc[1] += fa.y; acc[2] += fa.z;
+    acc[3] += ta.x; acc[4] += ta.y; acc[5] += ta.z;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) s[k][tid] = acc[k];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) s[k][tid] += s[k][tid + h];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out[2 * w] = make_double4(s[0][0], s[1][0], s[2][0], 0.);
+    out[2 * w + 1] = make_double4(s[3][0], s[4][0], s[5][0], 0.);
+  }
+}
+
 // ---------------------------------------------------------------- integrator phases
 __global__ void __launch_bounds__(kBlock) k_move(State s, const double4* Fa, const double4* Ta,
                                                  int n, Grid g, const int32_t* row_rank,
@@ -359,6 +419,17 @@ void launch_binning(Bins& b, const State& s, int32_t n, int32_t ncell, hipStream
   hipLaunchKernelGGL(k_cellsort, dim3(capped(nblocks(ncell, kBlock))), dim3(kBlock), 0, st,
                      b.local, b.bsum, b.perm, b.cell_start, (const double4*)s.pos_r, b.pos, ncell,
                      n, gate);
+}
+
+void launch_wall_loads(const ForceArgs& a, double4* part, double4* out, hipStream_t st) {
+  if (a.nwalls <= 0) return;
+  const int nb = a.n > 0 ? nblocks(a.n, kBlock) : 0;
+  if (nb > 0)
+    hipLaunchKernelGGL(k_wall_load_partial, dim3(nb), dim3(kBlock), 0, st, a.wl_count, a.wl_wall,
+                       (const double4*)a.wl_f, (const double4*)a.wl_t, a.n, a.lstride, a.nwalls,
+                       part);
+  hipLaunchKernelGGL(k_wall_load_final, dim3(a.nwalls), dim3(256), 0, st, (const double4*)part, nb,
+                     a.nwalls, out);
 }
 
 void launch_detect(const DetectArgs& a, hipStream_t st) {

@@ -340,6 +340,25 @@ struct Reducer {
   }
 };
 
+// The reaction of one particle-wall contact on the wall (the wall is the
+// second body: Obstacle::addForce(point, -f) about its centre, then
+// addTorque(-delM); Component/src/Obstacle.cpp:575-597, Torsor.cpp:184-189),
+// stored as the particle's k-th wall contact for the load reduction.
+__device__ __forceinline__ void wall_load(const ForceArgs& a, int t, int32_t id, int& k, int wi,
+                                          v3 c, v3 wo, v3 f, v3 mwall) {
+  if (k < kWallLoadCap) {
+    const size_t at = (size_t)k * a.lstride + t;
+    const v3 fw = neg(f);
+    const v3 tw = add(cross(sub(c, wo), fw), mwall);
+    a.wl_wall[at] = wi;
+    a.wl_f[at] = make_double4(fw.x, fw.y, fw.z, 0.);
+    a.wl_t[at] = make_double4(tw.x, tw.y, tw.z, 0.);
+  } else {
+    atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)id);
+  }
+  ++k;
+}
+
 // Force, torque and integrator tail of one particle (storage slot t of the
 // sorted order).  Returns the number of contacts it is the first body of.
 template <int MODE, bool RECORD, bool MEM = false>
@@ -405,6 +424,7 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
     last = key;
   }
   // SimpleObstacle::InterAction (Component/src/SimpleObstacle.cpp:222-271)
+  int nwl = 0;
   for (int wi = 0; wi < a.nwalls; ++wi) {
     const Wall& wl = a.walls[wi];
     if (!(fabs(wl.bb_centre[0] - R.X.x) <= wl.bb_ext[0] + R.r &&
@@ -434,16 +454,21 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
                          1.e20, wl.req, delM);
       R.F = add(R.F, f);
       R.T = add(R.T, cross(sub(c, R.X), f));
-      if (md.p[md.kind == 3 ? 5 : 4] != 0.) R.T = add(R.T, delM);
+      const bool rolling = md.p[md.kind == 3 ? 5 : 4] != 0.;
+      if (rolling) R.T = add(R.T, delM);
+      if (a.wl_count)
+        wall_load(a, t, R.ax.id, nwl, wi, c, wo, f, rolling ? neg(delM) : mk(0., 0., 0.));
     } else {
       f = contact_force(md, c, ov, dist, R.X, R.V, R.W, R.m, R.req, wo, wv, ww, 1.e20, wl.req);
       R.F = add(R.F, f);
       R.T = add(R.T, cross(sub(c, R.X), f));
+      if (a.wl_count) wall_load(a, t, R.ax.id, nwl, wi, c, wo, f, mk(0., 0., 0.));
     }
     ++R.ncont;
     if (RECORD) R.record(R.ax.id, -1 - wi, c, ov, dist, f);
   }
   if (RECORD) a.rec_count[t] = R.nrec;
+  if (a.wl_count) a.wl_count[t] = min(nwl, kWallLoadCap);
   if (MEM) a.hout.count[t] = min(R.nhist, kHistCap);
   const v3 F = R.F, T = R.T, X = R.X, V = R.V, W = R.W;
   const double m = R.m;

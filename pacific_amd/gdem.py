@@ -110,6 +110,8 @@ def load_library(path: str = LIB_PATH):
         "gdem_set_memory": (C.c_int, [vp, i64, vp, vp, vp, vp, vp]),
         "gdem_info": (C.c_int, [vp, P(i64), P(i32), P(i32), P(i32)]),
         "gdem_add_wall_motion": (C.c_int, [vp, i32, i32, d, d, vp]),
+        "gdem_set_wall_loads": (C.c_int, [vp, i32]),
+        "gdem_get_wall_loads": (C.c_int, [vp, vp, vp]),
         "gdem_add_composite": (C.c_int, [vp, i32, P(i32)]),
         "gdem_attach_wall": (C.c_int, [vp, i32, i32]),
         "gdem_add_composite_motion": (C.c_int, [vp, i32, i32, d, d, vp]),
@@ -174,7 +176,8 @@ EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall"
             "gdem_move", "gdem_advance_velocity", "gdem_prime", "gdem_step", "gdem_set_graph",
             "gdem_set_skin", "gdem_get_skin",
             "gdem_synchronize", "gdem_get_state", "gdem_set_state", "gdem_get_memory",
-            "gdem_set_memory", "gdem_info", "gdem_add_wall_motion", "gdem_add_composite",
+            "gdem_set_memory", "gdem_info", "gdem_add_wall_motion", "gdem_set_wall_loads",
+            "gdem_get_wall_loads", "gdem_add_composite",
             "gdem_attach_wall", "gdem_add_composite_motion", "gdem_get_composite",
             "gdem_add_cylindrical_shell", "gdem_cylindrical_shell_boxes", "gdem_set_time",
             "gdem_get_time", "gdem_move_walls", "gdem_get_wall", "gdem_write_particles_2014",
@@ -301,6 +304,19 @@ class GdemSystem:
         p = _f64(params, (MOTION_NPARAMS[int(kind)],))
         self._check(self.lib.gdem_add_wall_motion(self.h, int(wall), int(kind), float(t_start),
                                                   float(t_end), _ptr(p)), "gdem_add_wall_motion")
+
+    def set_wall_loads(self, on=True):
+        """Track the load on every wall in each force evaluation."""
+        self._check(self.lib.gdem_set_wall_loads(self.h, int(bool(on))), "gdem_set_wall_loads")
+
+    def get_wall_loads(self):
+        """(force (nwalls, 3), torque about each wall's centre (nwalls, 3)) of
+        the last force evaluation."""
+        nw = C.c_int32()
+        self._check(self.lib.gdem_info(self.h, None, C.byref(nw), None, None), "gdem_info")
+        f, t = np.zeros((max(nw.value, 1), 3)), np.zeros((max(nw.value, 1), 3))
+        self._check(self.lib.gdem_get_wall_loads(self.h, _ptr(f), _ptr(t)), "gdem_get_wall_loads")
+        return f[:nw.value], t[:nw.value]
 
     def add_composite(self, parent=-1):
         """A composite obstacle under parent (-1: the root); returns its index."""
