@@ -222,6 +222,19 @@ int gdem_cylindrical_shell_boxes(const double* basis, const double* centre, doub
 int gdem_set_wall_loads(gdem_ctx* ctx, int32_t on);
 int gdem_get_wall_loads(gdem_ctx* ctx, double* force, double* torque);
 
+/* Imposed force on wall k over [t_start, t_end] (ObstacleImposedForce,
+   ConstantTranslation, Component/src/ObstacleImposedForce.cpp:30-90, 227-296):
+   p = {ax, ay, az, mass, vmax}.  Each step the wall moves so that its load
+   (the previous force evaluation's) tends to (ax, ay, az): a PID law on the
+   load error along that direction with gains from mass (mass <= 0: chosen from
+   dt, |a| and the wall's crust, as the reference's automatic mass); until
+   the first nonzero load it moves at vmax against that direction.  As in the
+   reference, the controller's memory is shared by all imposed forces.  Turns
+   wall-load tracking on; gdem_step then advances one step per host round
+   trip while the force is pending.  Sphere contexts. */
+int gdem_add_wall_force(gdem_ctx* ctx, int32_t wall, double t_start, double t_end,
+                        const double* p);
+
 /* Simulation time (Grains::m_time, Base/src/Grains.cpp:114, :290); gdem_step
    advances it by dt per step. */
 int gdem_set_time(gdem_ctx* ctx, double time);

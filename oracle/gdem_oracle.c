@@ -52,6 +52,12 @@ typedef struct {
   double ms_amp[3], ms_period[3], ms_phase[3];
 } motion_t;
 
+/* ObstacleImposedForce, ConstantTranslation (ObstacleImposedForce.cpp:30-90) */
+typedef struct {
+  int wall, live, automass;
+  double t0, t1, amp[3], dir[3], mass, vmax, vt[3];
+} force_t;
+
 /* per-step state of ObstacleKinematicsVelocity (ObstacleKinematicsVelocity.cpp:
    60-177): translation and rotation over the step, rotation quaternion over
    the step, velocities */
@@ -89,6 +95,14 @@ struct orc_sys {
   int wall_parent[ORC_MAX_WALLS];
   int ncomp;
   comp_t comp[ORC_MAX_COMPOSITES];
+  /* imposed forces, the controller memory (function statics of
+     ObstacleImposedForce::translationalVelocity, :252-257, shared by all of
+     them) and each wall's ObstacleKinematicsForce state */
+  int nforce;
+  force_t force[ORC_MAX_MOTIONS];
+  double pid_dforce_nm1[3], pid_dforce_nm2[3], pid_depl_nm1[3];
+  int pid_nonzero;
+  double conf_vt[ORC_MAX_WALLS][3], conf_trans[ORC_MAX_WALLS][3];
   model_t models[ORC_MAX_MATERIALS][ORC_MAX_MATERIALS];
   /* cell lists: doubly linked, head insertion (Cell.cpp:90-93, push_front) */
   int *head, *next, *prev, *cell;
@@ -488,12 +502,47 @@ static int kin_imposed(orc_sys* s, kin_t* k, int target, double time, double dt,
    translation (Transform::composeLeftByTranslation, Transform.cpp:300-304),
    rotation of the basis (RigidBody::Rotate, RigidBody.cpp:270-273), new
    bounding box, velocities of the kinematics when it moved */
+/* ObstacleImposedForce::translationalVelocity (ObstacleImposedForce.cpp:
+   227-296), ConstantTranslation */
+static void force_velocity(orc_sys* s, force_t* m, double time, double dt, double subint,
+                           v3 force, double crust, double* vt, double* trans) {
+  if (m->automass) m->mass = dt * dt * norm(ld(m->amp)) * 1.e7 / (2. * crust);
+  double Kp = 0.5 * dt * dt / m->mass;
+  double Ki = 2. * Kp / dt;
+  double Kd = 3. * Kp * dt / 24.;
+  if (!s->pid_nonzero) s->pid_nonzero = norm(force) > ORC_EPSILON;
+  double dforce[3];
+  for (int i = 0; i < 3; ++i) {
+    dforce[i] = (i == 0 ? force.x : i == 1 ? force.y : force.z) - m->amp[i];
+    dforce[i] *= m->dir[i];
+  }
+  if (s->pid_nonzero) {
+    double A = Kp + Ki * dt + Kd / dt, B = Kp + 2. * Kd / dt, Cc = Kd / dt;
+    double depl[3];
+    for (int i = 0; i < 3; ++i) {
+      depl[i] = s->pid_depl_nm1[i] + A * dforce[i] - B * s->pid_dforce_nm1[i] +
+                Cc * s->pid_dforce_nm2[i];
+      m->vt[i] = depl[i] / dt;
+      trans[i] = (subint / dt) * depl[i];
+    }
+    memcpy(s->pid_dforce_nm2, s->pid_dforce_nm1, sizeof(dforce));
+    memcpy(s->pid_dforce_nm1, dforce, sizeof(dforce));
+    memcpy(s->pid_depl_nm1, depl, sizeof(depl));
+  } else {
+    for (int i = 0; i < 3; ++i) {
+      m->vt[i] = -m->vmax * m->dir[i];
+      trans[i] = subint * m->vt[i];
+    }
+  }
+  if (time > m->t1 + dt * 1.e-5) m->vt[0] = m->vt[1] = m->vt[2] = 0.;
+  memcpy(vt, m->vt, sizeof(double) * 3);
+}
+
 static void move_wall(orc_sys* s, int w, double time, double dt, int mother, int displace) {
   wall_t* wl = &s->walls[w];
   kin_t* k = &s->wall_kin[w];
   int moving = kin_imposed(s, k, w, time, dt, ld(wl->origin)) || mother;
-  if (!moving) return;
-  if (displace) {
+  if (moving && displace) {
     double basis[9], origin[3], ext[3], crust = wl->crust;
     int material = wl->material;
     memcpy(basis, wl->basis, sizeof(basis));
@@ -502,10 +551,45 @@ static void move_wall(orc_sys* s, int w, double time, double dt, int mother, int
     origin[1] = wl->origin[1] + k->trans.y;
     origin[2] = wl->origin[2] + k->trans.z;
     compose_left_rotation(basis, k->q);
+    double lf[3], lt[3];
+    memcpy(lf, wl->load_f, sizeof(lf));
+    memcpy(lt, wl->load_t, sizeof(lt));
     build_wall(wl, basis, origin, ext, crust, material);
+    memcpy(wl->load_f, lf, sizeof(lf));
+    memcpy(wl->load_t, lt, sizeof(lt));
   }
-  /* Obstacle::setVelocity (Obstacle.cpp:635-640) */
-  st(wl->vel, k->vt);
+  /* imposed forces (ObstacleKinematicsForce::ImposedMotion,
+     ObstacleKinematicsForce.cpp:40-63: each active one in turn), then the
+     translation (SimpleObstacle.cpp:142-152) */
+  double* cvt = s->conf_vt[w];
+  double* ctr = s->conf_trans[w];
+  for (int i = 0; i < s->nforce; ++i) {
+    force_t* m = &s->force[i];
+    double sub_dt = 0.;
+    if (m->wall != w) continue;
+    sub_dt = time - (time - dt);
+    if (time - dt < m->t0) sub_dt -= (m->t0 - (time - dt));
+    if (m->t1 < time) sub_dt -= (time - m->t1);
+    if (!(sub_dt > dt * 1.e-5)) continue;
+    force_velocity(s, m, time, dt, sub_dt, ld(wl->load_f), wl->crust, cvt, ctr);
+  }
+  int move_force = norm(ld(cvt)) != 0.;
+  if (move_force && displace) {
+    double basis[9], origin[3], ext[3], crust = wl->crust;
+    int material = wl->material;
+    double lf[3], lt[3];
+    memcpy(basis, wl->basis, sizeof(basis));
+    memcpy(ext, wl->ext, sizeof(ext));
+    memcpy(lf, wl->load_f, sizeof(lf));
+    memcpy(lt, wl->load_t, sizeof(lt));
+    for (int d = 0; d < 3; ++d) origin[d] = wl->origin[d] + ctr[d];
+    build_wall(wl, basis, origin, ext, crust, material);
+    memcpy(wl->load_f, lf, sizeof(lf));
+    memcpy(wl->load_t, lt, sizeof(lt));
+  }
+  if (!moving && !move_force) return;
+  /* Obstacle::setVelocity (Obstacle.cpp:635-640): both kinematics */
+  st(wl->vel, add(k->vt, ld(cvt)));
   st(wl->omega, k->vr);
 }
 
@@ -551,6 +635,8 @@ static void reset_walls(orc_sys* s) {
   for (int w = 0; w < s->nwalls; ++w) {
     for (int d = 0; d < 3; ++d) s->walls[w].vel[d] = s->walls[w].omega[d] = 0.;
     s->wall_kin[w] = kin_zero();
+    /* ObstacleKinematicsForce::reset (ObstacleKinematicsForce.cpp:118-124) */
+    for (int d = 0; d < 3; ++d) s->conf_vt[w][d] = s->conf_trans[w][d] = 0.;
   }
   for (int c = 0; c < s->ncomp; ++c) {
     s->comp[c].kin = kin_zero();
@@ -561,12 +647,35 @@ static void reset_walls(orc_sys* s) {
 static int any_live(const orc_sys* s) {
   for (int i = 0; i < s->nmot; ++i)
     if (s->mot[i].live) return 1;
+  for (int i = 0; i < s->nforce; ++i)
+    if (s->force[i].live) return 1;
   return 0;
 }
 
 static void drop_completed(orc_sys* s, double time, double dt) {
   for (int i = 0; i < s->nmot; ++i)
     if (s->mot[i].live && mot_completed(&s->mot[i], time, dt)) s->mot[i].live = 0;
+  for (int i = 0; i < s->nforce; ++i)
+    if (s->force[i].live && time > s->force[i].t1 + dt * 1.e-5) s->force[i].live = 0;
+}
+
+/* ObstacleImposedForce constructor, ConstantTranslation (:52-72) and
+   ObstacleKinematicsForce::append (ObstacleKinematicsForce.cpp:28-31) */
+int orc_add_wall_force(orc_sys* s, int wall, double t0, double t1, const double* p) {
+  if (wall < 0 || wall >= s->nwalls || s->nforce >= ORC_MAX_MOTIONS || !(t1 >= t0))
+    return ORC_ERR_ARG;
+  force_t* f = &s->force[s->nforce++];
+  memset(f, 0, sizeof(*f));
+  f->wall = wall;
+  f->live = 1;
+  f->t0 = t0;
+  f->t1 = t1;
+  st(f->amp, ld(p));
+  st(f->dir, dvd(ld(p), norm(ld(p))));
+  f->automass = !(p[3] > 0.);
+  f->mass = f->automass ? 0. : p[3];
+  f->vmax = p[4];
+  return ORC_OK;
 }
 
 /* obstacle part of AllComponents::Move (Component/src/AllComponents.cpp:
@@ -578,6 +687,14 @@ int orc_move_walls(orc_sys* s, double time, double dt) {
   double subint;
   for (int i = 0; i < s->nmot; ++i)
     if (s->mot[i].live && mot_active(&s->mot[i], time - dt, time, dt, &subint)) anyactive = 1;
+  for (int i = 0; i < s->nforce; ++i) {
+    const force_t* f = &s->force[i];
+    if (!f->live) continue;
+    subint = time - (time - dt);   /* ObstacleImposedForce::isActif (:160-167) */
+    if (time - dt < f->t0) subint -= (f->t0 - (time - dt));
+    if (f->t1 < time) subint -= (time - f->t1);
+    if (subint > dt * 1.e-5) anyactive = 1;
+  }
   if (anyactive || s->active_prev) reset_walls(s);
   s->active_prev = anyactive;
   if (anyactive) move_members(s, -1, time, dt, 0, 1);

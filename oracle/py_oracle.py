@@ -54,6 +54,7 @@ def load():
             "orc_set_time_step": (i32, [vp, d]),
             "orc_add_wall_motion": (i32, [vp, i32, i32, d, d, vp]),
             "orc_get_wall_loads": (i32, [vp, vp, vp]),
+            "orc_add_wall_force": (i32, [vp, i32, d, d, vp]),
             "orc_add_composite": (i32, [vp, i32, P(i32)]),
             "orc_attach_wall": (i32, [vp, i32, i32]),
             "orc_add_composite_motion": (i32, [vp, i32, i32, d, d, vp]),
@@ -153,6 +154,9 @@ class Oracle:
             npar = {1: 3, 2: 6, 3: 5, 4: 12, 5: 7}[int(kind)]
             self._chk(self.lib.orc_add_composite_motion(h, int(comp), int(kind), float(t0), float(t1),
                                                         _p(_f(prm, (npar,)))), "composite motion")
+        for (wall, t0, t1, prm) in wl.get("wall_forces", []):
+            self._chk(self.lib.orc_add_wall_force(h, int(wall), float(t0), float(t1),
+                                                  _p(_f(prm, (5,)))), "wall force")
         if log_capacity:
             self.lib.orc_enable_contact_log(h, int(log_capacity))
 

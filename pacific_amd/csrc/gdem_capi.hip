@@ -114,6 +114,7 @@ struct gdem_ctx {
   double4* d_wl_t = nullptr;
   double4* d_wl_part = nullptr;
   double4* d_wall_load = nullptr;
+  std::vector<double> h_loads;  // host copy read by the imposed-force controller
 };
 
 #define HIP_TRY(expr)                                                        \
@@ -1064,9 +1065,30 @@ extern "C" int gdem_prime(gdem_ctx* c) {
 // (Component/src/AllComponents.cpp:151-220), are computed on the host in
 // chunks and uploaded; force evaluation k reads walls at t_k.  No graph
 // replay (each evaluation reads its own wall array).
+// The loads of the last force evaluation into h_loads for the controller of
+// the imposed forces (Obstacle::getForce in ObstacleImposedForce.cpp:232).
+static int fetch_loads(gdem_ctx* c) {
+  double4 h[kMaxWalls * 2];
+  HIP_TRY(hipMemcpyAsync(h, c->d_wall_load, sizeof(double4) * 2 * (size_t)std::max(c->nwalls, 1),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->h_loads.assign(3 * (size_t)std::max(c->nwalls, 1), 0.);
+  for (int w = 0; w < c->nwalls; ++w) {
+    c->h_loads[3 * w] = h[2 * w].x;
+    c->h_loads[3 * w + 1] = h[2 * w].y;
+    c->h_loads[3 * w + 2] = h[2 * w].z;
+  }
+  c->kin.loads = c->h_loads.data();
+  return GDEM_OK;
+}
+
 static int moving_wall_steps(gdem_ctx* c, int32_t nsteps, double dt) {
   const int nw = std::max(c->nwalls, 1);
-  const int32_t chunk = (int32_t)std::max<size_t>(1, (8u << 20) / (sizeof(Wall) * (size_t)nw));
+  // an imposed force needs the loads of the previous evaluation: one step
+  // at a time
+  const bool controlled = c->kin.any_force_live();
+  const int32_t chunk =
+      controlled ? 1 : (int32_t)std::max<size_t>(1, (8u << 20) / (sizeof(Wall) * (size_t)nw));
   for (int32_t k0 = 0; k0 < nsteps; k0 += chunk) {
     const int32_t m = std::min(chunk, nsteps - k0);
     const size_t need = (size_t)m * (size_t)nw;
@@ -1080,6 +1102,10 @@ static int moving_wall_steps(gdem_ctx* c, int32_t nsteps, double dt) {
       c->traj_cap = need;
     }
     c->h_traj.resize(need);
+    if (controlled) {
+      const int rl = fetch_loads(c);
+      if (rl) return rl;
+    }
     double t = c->time;
     for (int32_t k = 0; k < m; ++k) {
       t += dt;
@@ -1365,6 +1391,30 @@ extern "C" int gdem_add_wall_motion(gdem_ctx* c, int32_t wall, int32_t type, dou
   return add_motion(c, wall, type, t_start, t_end, p);
 }
 
+extern "C" int gdem_add_wall_force(gdem_ctx* c, int32_t wall, double t_start, double t_end,
+                                   const double* p) {
+  if (!c || !p || c->convex || wall < 0 || wall >= c->nwalls || !(t_end >= t_start))
+    return GDEM_ERR_ARG;
+  WallForce f;
+  f.wall = wall;
+  f.t0 = t_start;
+  f.t1 = t_end;
+  f.amp = mk(p[0], p[1], p[2]);
+  const double na = norm(f.amp);
+  if (!(na > 0.) || !(p[4] >= 0.)) return GDEM_ERR_ARG;
+  f.dir = dvd(f.amp, na);  // ObstacleImposedForce.cpp:74
+  f.automass = !(p[3] > 0.);
+  f.mass = f.automass ? 0. : p[3];
+  f.vmax = p[4];
+  if (!c->track_loads) {
+    const int rc = gdem_set_wall_loads(c, 1);
+    if (rc) return rc;
+  }
+  c->kin.fit(c->nwalls);
+  c->kin.insert_force(f);
+  return GDEM_OK;
+}
+
 extern "C" int gdem_add_composite_motion(gdem_ctx* c, int32_t composite, int32_t type,
                                          double t_start, double t_end, const double* p) {
   if (!c || composite < 0 || composite >= (int32_t)c->kin.comps.size()) return GDEM_ERR_ARG;
@@ -1451,6 +1501,7 @@ extern "C" int gdem_get_composite(gdem_ctx* c, int32_t composite, double* centre
 
 extern "C" int gdem_set_wall_loads(gdem_ctx* c, int32_t on) {
   if (!c || c->convex) return GDEM_ERR_ARG;
+  if (!on && !c->kin.forces.empty()) return GDEM_ERR_ARG;  // the controllers read the loads
   HIP_TRY(hipSetDevice(c->dev));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->track_loads = on != 0;
@@ -1489,6 +1540,10 @@ extern "C" int gdem_move_walls(gdem_ctx* c, double time, double dt) {
   if (!c || !(dt > 0.)) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
   c->time = time;
+  if (c->kin.any_force_live()) {
+    const int rl = fetch_loads(c);
+    if (rl) return rl;
+  }
   if (c->kin.move(c->h_walls, c->nwalls, time, dt)) return upload_walls(c);
   return GDEM_OK;
 }

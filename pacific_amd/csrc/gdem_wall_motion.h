@@ -83,7 +83,16 @@ inline bool motion_active(const WallMotion& m, double td, double te, double dt, 
   if (m.t1 < te) subint -= (te - m.t1);
   return subint > dt * 1.e-5;
 }
-inline bool motion_completed(const WallMotion& m, double t, double dt) { return t > m.t1 + dt * 1.e-5; }
+inline bool motion_completed_at(double t1, double t, double dt) { return t > t1 + dt * 1.e-5; }
+inline bool motion_completed(const WallMotion& m, double t, double dt) {
+  return motion_completed_at(m.t1, t, dt);
+}
+inline bool interval_active(double t0, double t1, double td, double te, double dt, double& subint) {
+  subint = te - td;
+  if (td < t0) subint -= (t0 - td);
+  if (t1 < te) subint -= (te - t1);
+  return subint > dt * 1.e-5;
+}
 
 inline double comp(v3 a, int i) { return i == 0 ? a.x : i == 1 ? a.y : a.z; }
 inline void set_comp(v3& a, int i, double v) { (i == 0 ? a.x : i == 1 ? a.y : a.z) = v; }
@@ -218,6 +227,63 @@ inline bool kin_imposed(Kin& k, std::vector<WallMotion>& motions, int32_t target
   return norm(depl) != 0. || norm(rota) != 0.;
 }
 
+// ObstacleImposedForce, type ConstantTranslation (Component/src/
+// ObstacleImposedForce.cpp:30-90): a wall driven over [t0, t1] so that its
+// contact load tends to amp (a proportional-integral-derivative law on the
+// load error along amp's direction), or moving at vmax against that direction
+// while it has never felt a load.  mass <= 0: chosen from the time step, the
+// amplitude and the crust (m_automass).
+struct WallForce {
+  int32_t wall = 0;
+  double t0 = 0., t1 = 0.;
+  v3 amp = {0., 0., 0.}, dir = {0., 0., 0.};
+  double mass = 0., vmax = 0.;
+  bool automass = false;
+  v3 vt = {0., 0., 0.};  // m_translationalVelocity
+};
+
+// The controller's memory: function-level statics of
+// ObstacleImposedForce::translationalVelocity (:252-257), so shared by every
+// imposed force of the run.
+struct ForcePid {
+  v3 dforce_nm1 = {0., 0., 0.}, dforce_nm2 = {0., 0., 0.}, depl_nm1 = {0., 0., 0.};
+  bool nonzeroforce = false;
+};
+
+// Per-step state of a wall's ObstacleKinematicsForce (ObstacleKinematicsForce.cpp)
+struct Conf {
+  v3 vt = {0., 0., 0.}, trans = {0., 0., 0.};
+};
+
+// ObstacleImposedForce::translationalVelocity (ObstacleImposedForce.cpp:227-296)
+// for ConstantTranslation: the wall's load from the last force evaluation in,
+// the velocity and the translation over the active subinterval out.
+inline void force_velocity(WallForce& m, ForcePid& pid, double time, double dt, double subint,
+                           v3 force, double crust, v3& vt, v3& trans) {
+  if (m.automass) m.mass = dt * dt * norm(m.amp) * 1.e7 / (2. * crust);
+  const double Kp = 0.5 * dt * dt / m.mass;
+  const double Ki = 2. * Kp / dt;
+  const double Kd = 3. * Kp * dt / 24.;
+  if (!pid.nonzeroforce) pid.nonzeroforce = norm(force) > kEps;
+  const v3 d0 = sub(force, m.amp);
+  const v3 dforce = mk(d0.x * m.dir.x, d0.y * m.dir.y, d0.z * m.dir.z);
+  if (pid.nonzeroforce) {
+    const v3 depl = add(sub(add(pid.depl_nm1, scl(dforce, Kp + Ki * dt + Kd / dt)),
+                            scl(pid.dforce_nm1, Kp + 2. * Kd / dt)),
+                        scl(pid.dforce_nm2, Kd / dt));
+    m.vt = dvd(depl, dt);
+    trans = scl(depl, subint / dt);
+    pid.dforce_nm2 = pid.dforce_nm1;
+    pid.dforce_nm1 = dforce;
+    pid.depl_nm1 = depl;
+  } else {
+    m.vt = scl(m.dir, -m.vmax);
+    trans = scl(m.vt, subint);
+  }
+  if (motion_completed_at(m.t1, time, dt)) m.vt = mk(0., 0., 0.);
+  vt = m.vt;
+}
+
 // Transform::composeLeftByRotation (Geometry/src/Transform.cpp:140-159):
 // basis <- R(q) basis, one column at a time
 inline void compose_left_rotation(quat q, double* b) {
@@ -263,6 +329,14 @@ struct WallKinematics {
   std::vector<Composite> comps;
   std::vector<int32_t> wall_parent;  // composite of each wall, -1 for the root
   std::vector<Kin> wall_kin;
+  // imposed forces (ObstacleKinematicsForce::append pushes back, :28-31), the
+  // shared controller memory, each wall's confinement state, and the wall
+  // loads of the last force evaluation (3 per wall) the controller reads
+  std::vector<WallForce> forces;
+  std::vector<char> force_live;
+  ForcePid pid;
+  std::vector<Conf> wall_conf;
+  const double* loads = nullptr;
 
   void insert_motion(const WallMotion& m) {
     size_t pos = motions.size();
@@ -274,19 +348,31 @@ struct WallKinematics {
     motions.insert(motions.begin() + (long)pos, m);
     live.insert(live.begin() + (long)pos, 1);
   }
+  void insert_force(const WallForce& f) {
+    forces.push_back(f);
+    force_live.push_back(1);
+  }
+  bool any_force_live() const {
+    for (char l : force_live)
+      if (l) return true;
+    return false;
+  }
   bool any_live() const {
     for (char l : live)
       if (l) return true;
-    return false;
+    return any_force_live();
   }
   void drop_completed(double time, double dt) {
     for (size_t i = 0; i < motions.size(); ++i)
       if (live[i] && motion_completed(motions[i], time, dt)) live[i] = 0;
+    for (size_t i = 0; i < forces.size(); ++i)
+      if (force_live[i] && motion_completed_at(forces[i].t1, time, dt)) force_live[i] = 0;
   }
   void fit(int nwalls) {
     if ((int)wall_parent.size() < nwalls) {
       wall_parent.resize((size_t)nwalls, -1);
       wall_kin.resize((size_t)nwalls, kin_zero());
+      wall_conf.resize((size_t)nwalls, Conf{});
     }
   }
   // computeVolumeCenterOfMass of composite c (CompositeObstacle.cpp:228-255,
@@ -317,6 +403,7 @@ struct WallKinematics {
     for (int w = 0; w < nwalls; ++w) {
       for (int d = 0; d < 3; ++d) walls[w].vel[d] = walls[w].omega[d] = 0.;
       wall_kin[w] = kin_zero();
+      wall_conf[w] = Conf{};  // ObstacleKinematicsForce::reset (:118-124)
     }
     for (Composite& c : comps) {
       c.kin = kin_zero();
@@ -329,10 +416,10 @@ struct WallKinematics {
   void move_wall(Wall* walls, int w, double time, double dt, bool mother, bool displace) {
     Wall& wl = walls[w];
     Kin& k = wall_kin[w];
+    Conf& cf = wall_conf[w];
     const v3 cg = mk(wl.origin[0], wl.origin[1], wl.origin[2]);
     const bool moving = kin_imposed(k, motions, w, time, dt, cg) || mother;
-    if (!moving) return;
-    if (displace) {
+    if (moving && displace) {
       const double origin[3] = {wl.origin[0] + k.trans.x, wl.origin[1] + k.trans.y,
                                 wl.origin[2] + k.trans.z};
       double basis[9], ext[3];
@@ -341,10 +428,31 @@ struct WallKinematics {
       compose_left_rotation(k.q, basis);
       wl = make_wall(basis, origin, ext, wl.crust, wl.material);
     }
-    // Obstacle::setVelocity (Obstacle.cpp:635-640)
-    wl.vel[0] = k.vt.x;
-    wl.vel[1] = k.vt.y;
-    wl.vel[2] = k.vt.z;
+    // the imposed forces (ObstacleKinematicsForce::ImposedMotion,
+    // ObstacleKinematicsForce.cpp:40-63: every active one in turn, the last
+    // one's velocity and translation kept), then the translation
+    // (SimpleObstacle.cpp:142-152)
+    const v3 load = loads ? mk(loads[3 * w], loads[3 * w + 1], loads[3 * w + 2]) : mk(0., 0., 0.);
+    for (WallForce& m : forces) {
+      double sub_dt = 0.;
+      if (m.wall != w || !interval_active(m.t0, m.t1, time - dt, time, dt, sub_dt)) continue;
+      force_velocity(m, pid, time, dt, sub_dt, load, wl.crust, cf.vt, cf.trans);
+    }
+    const bool move_force = norm(cf.vt) != 0.;
+    if (move_force && displace) {
+      const double origin[3] = {wl.origin[0] + cf.trans.x, wl.origin[1] + cf.trans.y,
+                                wl.origin[2] + cf.trans.z};
+      double basis[9], ext[3];
+      memcpy(basis, wl.basis, sizeof(basis));
+      memcpy(ext, wl.ext, sizeof(ext));
+      wl = make_wall(basis, origin, ext, wl.crust, wl.material);
+    }
+    if (!moving && !move_force) return;
+    // Obstacle::setVelocity (Obstacle.cpp:635-640): both kinematics
+    const v3 vel = add(k.vt, cf.vt);
+    wl.vel[0] = vel.x;
+    wl.vel[1] = vel.y;
+    wl.vel[2] = vel.z;
     wl.omega[0] = k.vr.x;
     wl.omega[1] = k.vr.y;
     wl.omega[2] = k.vr.z;
@@ -392,6 +500,10 @@ struct WallKinematics {
     double subint = 0.;
     for (size_t i = 0; i < motions.size(); ++i)
       if (live[i] && motion_active(motions[i], time - dt, time, dt, subint)) anyactive = true;
+    for (size_t i = 0; i < forces.size(); ++i)
+      if (force_live[i] &&
+          interval_active(forces[i].t0, forces[i].t1, time - dt, time, dt, subint))
+        anyactive = true;
     const bool changed = anyactive || active_prev;
     if (changed) reset(walls, nwalls);
     active_prev = anyactive;

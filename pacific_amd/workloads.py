@@ -185,6 +185,23 @@ def shaken_box(n=1000, seed=1, memory=False):
     return wl
 
 
+def piston_box(n=1000, seed=1, memory=False, force=0.05, vmax=0.2, mass=2.5e-5):
+    """plain_box (HookeMemory when memory) whose lid (wall 5) starts 50 um
+    above the highest sphere and is driven by an imposed force of `force` N
+    (ObstacleImposedForce, ConstantTranslation, vmax m/s, controller mass
+    `mass` kg): it closes in at vmax, then presses the packing until its load
+    is `force`."""
+    wl = memory_box(n, seed) if memory else plain_box(n, seed)
+    wl["name"] = f"piston_box_{n}" + ("_memory" if memory else "")
+    x, r = wl["particles"]["x"], wl["particles"]["radius"]
+    top = float((x[:, 2] + r).max())
+    basis, centre, half, crust, mat = wl["walls"][5]
+    wl["walls"][5] = (basis, (centre[0], centre[1], top + 5.0e-5 + half[2]), half, crust, mat)
+    wl["time"] = 0.0
+    wl["wall_forces"] = [(5, 0.0, 1.0, (0.0, 0.0, force, mass, vmax))]
+    return wl
+
+
 # axis of the drums along y: the shell's local z is the world's y
 DRUM_BASIS = (1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, -1.0, 0.0)
 

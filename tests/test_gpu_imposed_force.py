@@ -1,0 +1,104 @@
+"""Imposed-force walls on the GPU path (gdem_add_wall_force) against the
+oracle, through the C ABI.
+
+* A plate closing in on a resting sphere: while it feels no load its path is
+  the oracle's bit for bit (vmax against the force); then the controller,
+  fed each step with the GPU's own wall loads, settles the plate's load on
+  the imposed force and the floor's on that force plus the sphere's weight.
+* A lid driven onto a box of colliding spheres (Hooke and HookeMemory): pair
+  sets equal at every checkpoint, lid within 1e-9 of the oracle's, particle
+  state within the parity bar (max(1e-9, 10 x the oracle's own one-ulp
+  divergence); the loads the controller reads are sums in another order, so
+  the lid path agrees to rounding, not bit for bit).
+* The arguments are checked, and load tracking cannot be switched off while
+  an imposed force needs it."""
+import math
+
+import numpy as np
+import pytest
+
+from gdem_testutil import pair_set, rel_err
+from pacific_amd import gdem, workloads
+from test_oracle_imposed_force_pins import DT, piston, weight
+
+pytestmark = pytest.mark.gpu
+
+
+def test_piston_on_sphere_gpu(oracle_mod):
+    print("[case] piston on a sphere", flush=True)
+    mg = weight()
+    A = 10.0 * mg
+    wl = piston(A, 2.5e-5)
+    o = oracle_mod.Oracle(wl)
+    o.prime()
+    s = gdem.build_system(wl)
+    for _ in range(3):          # 300 steps of approach: no load yet
+        o.step(100, DT)
+        s.step(100, DT)
+        for a, b in zip(s.get_wall(1), o.wall(1)):
+            np.testing.assert_array_equal(a, b)
+    s.step(2100, DT)
+    f, _ = s.get_wall_loads()
+    print(f"[case] plate load {f[1][2] / A:.12f} A, floor {f[0][2] / (A + mg):.12f} (A + mg)",
+          flush=True)
+    assert f[1][2] == pytest.approx(A, rel=1e-8)
+    assert f[0][2] == pytest.approx(-(A + mg), rel=1e-8)
+    s.close()
+
+
+def _errors(got, o, x0):
+    return (rel_err(got["x"] - x0, o["x"] - x0), rel_err(got["v"], o["v"]),
+            rel_err(got["w"], o["w"]))
+
+
+@pytest.mark.parametrize("memory", [False, True])
+def test_piston_box_matches_oracle(oracle_mod, memory):
+    print(f"[case] piston box, memory {memory}, 300 steps", flush=True)
+    wl = workloads.piston_box(1000, seed=1, memory=memory)
+    n = len(wl["particles"]["radius"])
+    dt = wl["dt"]
+    orc = oracle_mod.Oracle(wl, log_capacity=10 * n)
+    orc.prime()
+    wl_u = workloads.piston_box(1000, seed=1, memory=memory)
+    v = wl_u["particles"]["v"]
+    up = np.random.default_rng(7).random(v.shape) < 0.5
+    wl_u["particles"]["v"] = np.nextafter(v, np.where(up, np.inf, -np.inf))
+    orc_u = oracle_mod.Oracle(wl_u)
+    orc_u.prime()
+    s = gdem.build_system(wl)
+    s.set_contact_recording(32)
+    x0 = wl["particles"]["x"]
+    lid_contacts = 0
+    for calls in range(6):
+        orc.step(50, dt)
+        orc_u.step(50, dt)
+        s.step(50, dt)
+        pairs = pair_set(s.get_contacts())
+        assert pairs == pair_set(orc.contacts()), calls
+        lid_contacts += sum(1 for (_, b) in pairs if b == -6)
+        np.testing.assert_allclose(s.get_wall(5)[0], orc.wall(5)[0], rtol=1e-9)
+        o = orc.state()
+        self_div = _errors(orc_u.state(), o, x0)
+        bars = [max(1e-9, 10.0 * e) for e in self_div]
+        errs = _errors(s.get_state(), o, x0)
+        print(f"[case] steps {50 * (calls + 1)}: lid z {s.get_wall(5)[0][2]:.9f}, x/v/w error "
+              f"{errs}, oracle one-ulp divergence {self_div}", flush=True)
+        assert all(e <= b for e, b in zip(errs, bars)), (calls, errs, bars)
+    assert lid_contacts >= 1
+    s.close()
+
+
+def test_imposed_force_arguments():
+    print("[case] imposed force arguments", flush=True)
+    wl = workloads.plain_box(100, seed=3)
+    s = gdem.build_system(wl)
+    with pytest.raises(gdem.GdemError):
+        s.add_wall_force(6, 0.0, 1.0, (0.0, 0.0, 1.0, 0.0, 0.1))      # no such wall
+    with pytest.raises(gdem.GdemError):
+        s.add_wall_force(5, 0.0, 1.0, (0.0, 0.0, 0.0, 0.0, 0.1))      # no direction
+    with pytest.raises(gdem.GdemError):
+        s.add_wall_force(5, 1.0, 0.0, (0.0, 0.0, 1.0, 0.0, 0.1))      # empty interval
+    s.add_wall_force(5, 0.0, 1.0, (0.0, 0.0, 1.0, 0.0, 0.1))
+    with pytest.raises(gdem.GdemError):
+        s.set_wall_loads(False)
+    s.close()
