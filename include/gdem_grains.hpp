@@ -218,6 +218,16 @@ struct ObstacleImposedVelocity {
   std::map<std::string, double> parameters;
 };
 
+// ObstacleImposedForce (Component/include/ObstacleImposedForce.hh), type
+// "ConstantTranslation": parameters AX AY AZ (the force amplitude), Vmax, and
+// Mass (absent: chosen automatically, as the reference does without it).
+struct ObstacleImposedForce {
+  std::string obstacle;
+  std::string type;
+  double tstart = 0., tend = 0.;
+  std::map<std::string, double> parameters;
+};
+
 // Base/include/LinkedCell.hh (an AppCollision application): the grid, the
 // particle/obstacle links, LinkUpdate and ComputeForces.
 class LinkedCell {
@@ -274,8 +284,14 @@ class Grains {
   void do_before_time_stepping();
   // Kick-drift-kick loop from the current time over time_interval
   void Simulation(double time_interval, double dt);
-  // AllComponents::LinkImposedMotion (Component/src/AllComponents.cpp:130-139)
+  // AllComponents::LinkImposedMotion (Component/src/AllComponents.cpp:130-146)
   void LinkImposedMotion(ObstacleImposedVelocity const& motion);
+  void LinkImposedMotion(ObstacleImposedForce const& force);
+  // AllComponents::computeObstaclesLoad (AllComponents.cpp:1685-1693): track
+  // the obstacles' loads, and read a box obstacle's torsor (force, torque
+  // about its centre) of the last force evaluation
+  void trackObstaclesLoad(bool on);
+  void getObstacleLoad(std::string const& name, double force[3], double torque[3]) const;
   // Grains::m_time (the Simulation loop adds dt before each step's move)
   void setTime(double time);
   // Phases (Grains.cpp:459-527), for callers that interleave their own work;

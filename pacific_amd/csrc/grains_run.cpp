@@ -16,8 +16,10 @@
 // double basis[9], centre[3], height, radius, width, int32 n, double crust,
 // int32 material, parent; int32 ncomposite_motions, per motion int32
 // composite, type, double t_start, t_end, p[12] (composites numbered in
-// creation order, then the shells; a shell's boxes follow the walls); then
-// x[3n], v[3n], w[3n], q[4n], radius[n], density[n], crust[n], int32 material[n].
+// creation order, then the shells; a shell's boxes follow the walls); int32
+// nforces, per imposed force int32 wall, double t_start, t_end, ax, ay, az,
+// mass (<= 0: automatic), vmax; then x[3n], v[3n], w[3n], q[4n], radius[n],
+// density[n], crust[n], int32 material[n].
 // Output: "GDEMOUT1", int32 n, status (0 or the GDEM_ERR_* the step raised),
 // id0, id1, then x, v, w, q, F, T of the final state.
 #include <cstdio>
@@ -133,6 +135,7 @@ int main(int argc, char** argv) {
   // composites, box-element cylindrical shells and their motions
   std::vector<std::pair<int32_t, std::vector<int32_t>>> composites;
   std::vector<std::pair<CylindricalShell, int32_t>> shells;
+  std::vector<std::pair<int32_t, ObstacleImposedForce>> forces;
   if (with_tree) {
     int32_t nc = 0, ns = 0, ncm = 0;
     if (!rd(f, &nc, 1)) return 2;
@@ -159,6 +162,20 @@ int main(int argc, char** argv) {
     if (!rd(f, &ncm, 1)) return 2;
     for (int32_t k = 0; k < ncm; ++k)
       if (!read_motion(true)) return 2;
+    int32_t nf = 0;
+    if (!rd(f, &nf, 1)) return 2;
+    for (int32_t k = 0; k < nf; ++k) {
+      int32_t wall;
+      double v[7];
+      if (!rd(f, &wall, 1) || !rd(f, v, 7)) return 2;
+      ObstacleImposedForce fo;
+      fo.type = "ConstantTranslation";
+      fo.tstart = v[0];
+      fo.tend = v[1];
+      fo.parameters = {{"AX", v[2]}, {"AY", v[3]}, {"AZ", v[4]}, {"Vmax", v[6]}};
+      if (v[5] > 0.) fo.parameters["Mass"] = v[5];
+      forces.emplace_back(wall, fo);
+    }
   }
   std::vector<double> x(3 * n), v(3 * n), w(3 * n), q(4 * n), r(n), rho(n), cr(n);
   std::vector<int32_t> mat(n);
@@ -210,6 +227,10 @@ int main(int argc, char** argv) {
     tm.second.obstacle = tm.first >= 0 ? wall_names.at((size_t)tm.first)
                                        : comp_names.at((size_t)(-1 - tm.first));
     app.LinkImposedMotion(tm.second);
+  }
+  for (auto& fo : forces) {
+    fo.second.obstacle = wall_names.at((size_t)fo.first);
+    app.LinkImposedMotion(fo.second);
   }
   try {
     app.setTimeStep(dt);  // the input's time step, before a0 (memory models use it)

@@ -468,6 +468,41 @@ void Grains::readParticles2014(std::string const& path, bool binary, int id_offs
   m_primed = true;  // the stored F, T are a0
 }
 
+void Grains::LinkImposedMotion(ObstacleImposedForce const& mo) {
+  const int wall = m_lc.obstacleIndex(mo.obstacle);
+  if (wall < 0) throw std::invalid_argument("no box obstacle named " + mo.obstacle);
+  if (mo.type != "ConstantTranslation")
+    throw std::invalid_argument("imposed force type " + mo.type + " is not supported");
+  auto get = [&](const char* k, double dflt, bool required) {
+    auto it = mo.parameters.find(k);
+    if (it == mo.parameters.end()) {
+      if (required) throw std::invalid_argument("imposed force on " + mo.obstacle + ": no " + k);
+      return dflt;
+    }
+    return it->second;
+  };
+  const double p[5] = {get("AX", 0., true), get("AY", 0., true), get("AZ", 0., true),
+                       get("Mass", 0., false), get("Vmax", 0., true)};
+  if (gdem_add_wall_force(m_lc.context(), wall, mo.tstart, mo.tend, p))
+    throw std::invalid_argument("imposed force on " + mo.obstacle + " rejected");
+}
+
+void Grains::trackObstaclesLoad(bool on) { check(gdem_set_wall_loads(m_lc.context(), on ? 1 : 0)); }
+
+void Grains::getObstacleLoad(std::string const& name, double force[3], double torque[3]) const {
+  const int wall = m_lc.obstacleIndex(name);
+  if (wall < 0) throw std::invalid_argument("no box obstacle named " + name);
+  int64_t n = 0;
+  int32_t nw = 0;
+  check(gdem_info(m_lc.context(), &n, &nw, nullptr, nullptr));
+  std::vector<double> f(3 * (size_t)std::max(nw, 1)), t(3 * (size_t)std::max(nw, 1));
+  check(gdem_get_wall_loads(m_lc.context(), f.data(), t.data()));
+  for (int d = 0; d < 3; ++d) {
+    force[d] = f[3 * wall + d];
+    torque[d] = t[3 * wall + d];
+  }
+}
+
 void Grains::moveParticlesAndObstacles(double dtv, double dtd, double dto) {
   check(gdem_move(m_lc.context(), dtv, dtd));
   if (dto > 0.) check(gdem_move_walls(m_lc.context(), m_time, dto));

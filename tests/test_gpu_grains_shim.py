@@ -23,7 +23,8 @@ def write_input(path, wl, nsteps):
     n = len(p["radius"])
     memory = any(kind >= 3 for (_, _, kind, _) in wl["models"])
     motions = wl.get("motions", [])
-    tree = bool(wl.get("composites") or wl.get("shells") or wl.get("composite_motions"))
+    tree = bool(wl.get("composites") or wl.get("shells") or wl.get("composite_motions")
+                or wl.get("wall_forces"))
     assert not wl.get("attach"), "members are given with their composite"
     with open(path, "wb") as f:
         f.write(b"GDEMIN04" if tree else b"GDEMIN03" if motions else
@@ -67,6 +68,10 @@ def write_input(path, wl, nsteps):
             for (comp, kind, t0, t1, prm) in cm:
                 f.write(struct.pack("<2i2d", comp, kind, t0, t1))
                 f.write(struct.pack("<12d", *(list(prm) + [0.0] * (12 - len(prm)))))
+            wf = wl.get("wall_forces", [])
+            f.write(struct.pack("<i", len(wf)))
+            for (wall, t0, t1, prm) in wf:
+                f.write(struct.pack("<i7d", wall, t0, t1, *prm))
         for k in ("x", "v", "w", "q", "radius", "density", "crust"):
             f.write(np.ascontiguousarray(p[k], dtype="<f8").tobytes())
         f.write(np.ascontiguousarray(p["material"], dtype="<i4").tobytes())
@@ -159,6 +164,21 @@ def test_driver_rotating_drum_matches_oracle(oracle_mod):
     on the drum composite."""
     print("[case] grains_run rotating drum 150 steps", flush=True)
     wl = workloads.rotating_drum()
+    status, _, _, g = run(wl, 150)
+    assert status == 0
+    orc = oracle_mod.Oracle(wl)
+    orc.prime()
+    orc.step(150, wl["dt"])
+    o = orc.state()
+    for k in ("x", "v", "w", "q"):
+        assert rel_err(g[k], o[k]) <= 1e-9, k
+
+
+def test_driver_piston_box_matches_oracle(oracle_mod):
+    """ObstacleImposedForce through Grains::LinkImposedMotion: a lid pressing
+    a box of spheres."""
+    print("[case] grains_run piston box 150 steps", flush=True)
+    wl = workloads.piston_box(1000, seed=1)
     status, _, _, g = run(wl, 150)
     assert status == 0
     orc = oracle_mod.Oracle(wl)
