@@ -194,6 +194,12 @@ int gdem_add_composite(gdem_ctx* ctx, int32_t parent, int32_t* composite);
 int gdem_attach_wall(gdem_ctx* ctx, int32_t wall, int32_t composite);
 int gdem_add_composite_motion(gdem_ctx* ctx, int32_t composite, int32_t type, double t_start,
                               double t_end, const double* p);
+/* Obstacle::setRestrictedGeomDirMotion (Component/src/Obstacle.cpp:613-617):
+   the geometric translation of wall k (composite < 0) or of a composite
+   (wall < 0) is zeroed along the directions of mask (bit 0 x, 1 y, 2 z); its
+   velocity is kept (SimpleObstacle.cpp:131-136, 146-149;
+   CompositeObstacle.cpp:153-160). */
+int gdem_set_motion_restriction(gdem_ctx* ctx, int32_t wall, int32_t composite, int32_t mask);
 /* Centre and velocities of a composite (any pointer may be NULL). */
 int gdem_get_composite(gdem_ctx* ctx, int32_t composite, double* centre, double* vel,
                        double* omega);

@@ -10,6 +10,7 @@
 #pragma once
 
 #include <array>
+#include <list>
 #include <map>
 #include <memory>
 #include <ostream>
@@ -258,6 +259,10 @@ class LinkedCell {
   void Append(std::string const& obstacle, std::string const& composite);
   // CylindricalShell: a composite of its boxes, member of `parent`
   void Link(CylindricalShell const& shell, std::string const& parent = "");
+  // Obstacle::setRestrictedGeomDirMotion (Component/src/Obstacle.cpp:613-617)
+  // of a linked box obstacle or composite: directions 0 (x), 1 (y), 2 (z)
+  // whose geometric translation is suppressed
+  void setRestrictedGeomDirMotion(std::string const& obstacle, std::list<size_t> const& dirs);
   // index of the linked box obstacle / composite of that name (-1 if none)
   int obstacleIndex(std::string const& name) const;
   int compositeIndex(std::string const& name) const;

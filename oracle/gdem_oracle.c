@@ -69,7 +69,7 @@ typedef struct {
 /* CompositeObstacle (Component/src/CompositeObstacle.cpp): members in attach
    order (wall k, or composite c as -1 - c) */
 typedef struct {
-  int parent, nmem;
+  int parent, nmem, restrict_mask;
   int mem[ORC_MAX_WALLS + ORC_MAX_COMPOSITES];
   double centre[3], basis[9], vel[3], omega[3];
   kin_t kin;
@@ -103,6 +103,7 @@ struct orc_sys {
   double pid_dforce_nm1[3], pid_dforce_nm2[3], pid_depl_nm1[3];
   int pid_nonzero;
   double conf_vt[ORC_MAX_WALLS][3], conf_trans[ORC_MAX_WALLS][3];
+  int wall_restrict[ORC_MAX_WALLS];   /* Obstacle::setRestrictedGeomDirMotion */
   model_t models[ORC_MAX_MATERIALS][ORC_MAX_MATERIALS];
   /* cell lists: doubly linked, head insertion (Cell.cpp:90-93, push_front) */
   int *head, *next, *prev, *cell;
@@ -547,9 +548,10 @@ static void move_wall(orc_sys* s, int w, double time, double dt, int mother, int
     int material = wl->material;
     memcpy(basis, wl->basis, sizeof(basis));
     memcpy(ext, wl->ext, sizeof(ext));
-    origin[0] = wl->origin[0] + k->trans.x;
-    origin[1] = wl->origin[1] + k->trans.y;
-    origin[2] = wl->origin[2] + k->trans.z;
+    /* SimpleObstacle.cpp:131-136: restricted directions do not move */
+    origin[0] = wl->origin[0] + ((s->wall_restrict[w] & 1) ? 0. : k->trans.x);
+    origin[1] = wl->origin[1] + ((s->wall_restrict[w] & 2) ? 0. : k->trans.y);
+    origin[2] = wl->origin[2] + ((s->wall_restrict[w] & 4) ? 0. : k->trans.z);
     compose_left_rotation(basis, k->q);
     double lf[3], lt[3];
     memcpy(lf, wl->load_f, sizeof(lf));
@@ -582,7 +584,8 @@ static void move_wall(orc_sys* s, int w, double time, double dt, int mother, int
     memcpy(ext, wl->ext, sizeof(ext));
     memcpy(lf, wl->load_f, sizeof(lf));
     memcpy(lt, wl->load_t, sizeof(lt));
-    for (int d = 0; d < 3; ++d) origin[d] = wl->origin[d] + ctr[d];
+    for (int d = 0; d < 3; ++d)
+      origin[d] = wl->origin[d] + ((s->wall_restrict[w] >> d) & 1 ? 0. : ctr[d]);
     build_wall(wl, basis, origin, ext, crust, material);
     memcpy(wl->load_f, lf, sizeof(lf));
     memcpy(wl->load_t, lt, sizeof(lt));
@@ -601,7 +604,12 @@ static void move_composite(orc_sys* s, int c, double time, double dt, int mother
   comp_t* C = &s->comp[c];
   int moving = kin_imposed(s, &C->kin, -1 - c, time, dt, ld(C->centre)) || mother;
   if (moving && displace) {
-    st(C->centre, add(ld(C->centre), C->kin.trans));
+    /* CompositeObstacle.cpp:153-160: the composite's own translation only */
+    v3 tr = C->kin.trans;
+    if (C->restrict_mask & 1) tr.x = 0.;
+    if (C->restrict_mask & 2) tr.y = 0.;
+    if (C->restrict_mask & 4) tr.z = 0.;
+    st(C->centre, add(ld(C->centre), tr));
     compose_left_rotation(C->basis, C->kin.q);
   }
   if (moving)
@@ -768,6 +776,18 @@ int orc_attach_wall(orc_sys* s, int wall, int c) {
 int orc_add_composite_motion(orc_sys* s, int c, int type, double t0, double t1, const double* p) {
   if (c < 0 || c >= s->ncomp) return ORC_ERR_ARG;
   return add_motion(s, -1 - c, type, t0, t1, p);
+}
+
+int orc_set_motion_restriction(orc_sys* s, int wall, int c, int mask) {
+  if (mask < 0 || mask > 7) return ORC_ERR_ARG;
+  if (wall >= 0) {
+    if (wall >= s->nwalls || c >= 0) return ORC_ERR_ARG;
+    s->wall_restrict[wall] = mask;
+    return ORC_OK;
+  }
+  if (c < 0 || c >= s->ncomp) return ORC_ERR_ARG;
+  s->comp[c].restrict_mask = mask;
+  return ORC_OK;
 }
 
 int orc_get_composite(const orc_sys* s, int c, double* centre, double* vel, double* omega) {

@@ -95,6 +95,7 @@ int orc_add_composite_motion(orc_sys* s, int composite, int type, double t_start
                              double t_end, const double* p);
 int orc_get_composite(const orc_sys* s, int composite, double* centre, double* vel,
                       double* omega);
+int orc_set_motion_restriction(orc_sys* s, int wall, int composite, int mask);
 void orc_cylindrical_shell_boxes(const double* basis, const double* centre, double height,
                                  double radius, double width, int n, double* wall_basis,
                                  double* wall_centre, double* wall_half);

@@ -55,6 +55,7 @@ def load():
             "orc_add_wall_motion": (i32, [vp, i32, i32, d, d, vp]),
             "orc_get_wall_loads": (i32, [vp, vp, vp]),
             "orc_add_wall_force": (i32, [vp, i32, d, d, vp]),
+            "orc_set_motion_restriction": (i32, [vp, i32, i32, i32]),
             "orc_add_composite": (i32, [vp, i32, P(i32)]),
             "orc_attach_wall": (i32, [vp, i32, i32]),
             "orc_add_composite_motion": (i32, [vp, i32, i32, d, d, vp]),
@@ -148,6 +149,9 @@ class Oracle:
                 float(width), int(n), float(crust), int(mat), int(parent), C.byref(k)), "shell")
         for (wall, comp) in wl.get("attach", []):
             self._chk(self.lib.orc_attach_wall(h, int(wall), int(comp)), "attach")
+        for (wall, comp, mask) in wl.get("restrictions", []):
+            self._chk(self.lib.orc_set_motion_restriction(h, int(wall), int(comp), int(mask)),
+                      "restriction")
         for (wall, kind, t0, t1, prm) in wl.get("motions", []):
             self.add_wall_motion(wall, kind, t0, t1, prm)
         for (comp, kind, t0, t1, prm) in wl.get("composite_motions", []):

@@ -1486,6 +1486,20 @@ extern "C" int gdem_cylindrical_shell_boxes(const double* basis, const double* c
   return GDEM_OK;
 }
 
+extern "C" int gdem_set_motion_restriction(gdem_ctx* c, int32_t wall, int32_t composite,
+                                           int32_t mask) {
+  if (!c || mask < 0 || mask > 7) return GDEM_ERR_ARG;
+  if (wall >= 0) {
+    if (wall >= c->nwalls || composite >= 0) return GDEM_ERR_ARG;
+    c->kin.fit(c->nwalls);
+    c->kin.wall_restrict[wall] = (uint32_t)mask;
+    return GDEM_OK;
+  }
+  if (composite < 0 || composite >= (int32_t)c->kin.comps.size()) return GDEM_ERR_ARG;
+  c->kin.comps[composite].restrict_mask = (uint32_t)mask;
+  return GDEM_OK;
+}
+
 extern "C" int gdem_get_composite(gdem_ctx* c, int32_t composite, double* centre, double* vel,
                                   double* omega) {
   if (!c || composite < 0 || composite >= (int32_t)c->kin.comps.size()) return GDEM_ERR_ARG;

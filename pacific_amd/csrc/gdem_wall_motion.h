@@ -305,6 +305,7 @@ inline void compose_left_rotation(quat q, double* b) {
 // of its members (computeVolumeCenterOfMass, :228-255).
 struct Composite {
   int32_t parent = -1;  // -1: the root obstacle
+  uint32_t restrict_mask = 0;  // directions whose geometric translation is zeroed
   // members in the order they were attached: wall k, or composite c as -1 - c
   std::vector<int32_t> members;
   v3 centre = {0., 0., 0.};
@@ -337,6 +338,9 @@ struct WallKinematics {
   ForcePid pid;
   std::vector<Conf> wall_conf;
   const double* loads = nullptr;
+  // Obstacle::setRestrictedGeomDirMotion (Component/src/Obstacle.cpp:613-617):
+  // bit d set = no geometric translation along d (the velocity is kept)
+  std::vector<uint32_t> wall_restrict;
 
   void insert_motion(const WallMotion& m) {
     size_t pos = motions.size();
@@ -373,7 +377,14 @@ struct WallKinematics {
       wall_parent.resize((size_t)nwalls, -1);
       wall_kin.resize((size_t)nwalls, kin_zero());
       wall_conf.resize((size_t)nwalls, Conf{});
+      wall_restrict.resize((size_t)nwalls, 0u);
     }
+  }
+  static v3 restricted(v3 t, uint32_t mask) {
+    if (mask & 1u) t.x = 0.;
+    if (mask & 2u) t.y = 0.;
+    if (mask & 4u) t.z = 0.;
+    return t;
   }
   // computeVolumeCenterOfMass of composite c (CompositeObstacle.cpp:228-255,
   // SimpleObstacle.cpp:732-735 with Box::getVolume, Box.cpp:273-279):
@@ -420,8 +431,9 @@ struct WallKinematics {
     const v3 cg = mk(wl.origin[0], wl.origin[1], wl.origin[2]);
     const bool moving = kin_imposed(k, motions, w, time, dt, cg) || mother;
     if (moving && displace) {
-      const double origin[3] = {wl.origin[0] + k.trans.x, wl.origin[1] + k.trans.y,
-                                wl.origin[2] + k.trans.z};
+      // SimpleObstacle.cpp:131-136: restricted directions do not move
+      const v3 tr = restricted(k.trans, wall_restrict[w]);
+      const double origin[3] = {wl.origin[0] + tr.x, wl.origin[1] + tr.y, wl.origin[2] + tr.z};
       double basis[9], ext[3];
       memcpy(basis, wl.basis, sizeof(basis));
       memcpy(ext, wl.ext, sizeof(ext));
@@ -440,8 +452,8 @@ struct WallKinematics {
     }
     const bool move_force = norm(cf.vt) != 0.;
     if (move_force && displace) {
-      const double origin[3] = {wl.origin[0] + cf.trans.x, wl.origin[1] + cf.trans.y,
-                                wl.origin[2] + cf.trans.z};
+      const v3 tr = restricted(cf.trans, wall_restrict[w]);  // SimpleObstacle.cpp:146-149
+      const double origin[3] = {wl.origin[0] + tr.x, wl.origin[1] + tr.y, wl.origin[2] + tr.z};
       double basis[9], ext[3];
       memcpy(basis, wl.basis, sizeof(basis));
       memcpy(ext, wl.ext, sizeof(ext));
@@ -465,7 +477,8 @@ struct WallKinematics {
     Composite& C = comps[c];
     const bool moving = kin_imposed(C.kin, motions, -1 - c, time, dt, C.centre) || mother;
     if (moving && displace) {
-      C.centre = add(C.centre, C.kin.trans);
+      // CompositeObstacle.cpp:153-160: the composite's own translation only
+      C.centre = add(C.centre, restricted(C.kin.trans, C.restrict_mask));
       compose_left_rotation(C.kin.q, C.basis);
     }
     if (moving)

@@ -328,6 +328,19 @@ int LinkedCell::obstacleIndex(std::string const& name) const {
   return it == m_obstacles.end() ? -1 : it->second;
 }
 
+void LinkedCell::setRestrictedGeomDirMotion(std::string const& obstacle,
+                                            std::list<size_t> const& dirs) {
+  int32_t mask = 0;
+  for (size_t d : dirs) {
+    if (d > 2) throw std::invalid_argument("direction out of range");
+    mask |= 1 << d;
+  }
+  const int w = obstacleIndex(obstacle), c = compositeIndex(obstacle);
+  if (w < 0 && c < 0) throw std::invalid_argument("no obstacle named " + obstacle);
+  if (gdem_set_motion_restriction(m_ctx, w >= 0 ? w : -1, w >= 0 ? -1 : c, mask))
+    throw std::invalid_argument("restriction of " + obstacle + " rejected");
+}
+
 int LinkedCell::compositeIndex(std::string const& name) const {
   auto it = m_composites.find(name);
   return it == m_composites.end() ? -1 : it->second;

@@ -111,6 +111,7 @@ def load_library(path: str = LIB_PATH):
         "gdem_info": (C.c_int, [vp, P(i64), P(i32), P(i32), P(i32)]),
         "gdem_add_wall_motion": (C.c_int, [vp, i32, i32, d, d, vp]),
         "gdem_set_wall_loads": (C.c_int, [vp, i32]),
+        "gdem_set_motion_restriction": (C.c_int, [vp, i32, i32, i32]),
         "gdem_add_wall_force": (C.c_int, [vp, i32, d, d, vp]),
         "gdem_get_wall_loads": (C.c_int, [vp, vp, vp]),
         "gdem_add_composite": (C.c_int, [vp, i32, P(i32)]),
@@ -178,7 +179,8 @@ EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall"
             "gdem_set_skin", "gdem_get_skin",
             "gdem_synchronize", "gdem_get_state", "gdem_set_state", "gdem_get_memory",
             "gdem_set_memory", "gdem_info", "gdem_add_wall_motion", "gdem_set_wall_loads",
-            "gdem_get_wall_loads", "gdem_add_wall_force", "gdem_add_composite",
+            "gdem_get_wall_loads", "gdem_add_wall_force", "gdem_set_motion_restriction",
+            "gdem_add_composite",
             "gdem_attach_wall", "gdem_add_composite_motion", "gdem_get_composite",
             "gdem_add_cylindrical_shell", "gdem_cylindrical_shell_boxes", "gdem_set_time",
             "gdem_get_time", "gdem_move_walls", "gdem_get_wall", "gdem_write_particles_2014",
@@ -323,6 +325,12 @@ class GdemSystem:
         """Imposed force on a wall: params = (ax, ay, az, mass (<= 0: automatic), vmax)."""
         self._check(self.lib.gdem_add_wall_force(self.h, int(wall), float(t_start), float(t_end),
                                                  _ptr(_f64(params, (5,)))), "gdem_add_wall_force")
+
+    def set_motion_restriction(self, mask, wall=-1, composite=-1):
+        """No geometric translation along the directions of mask (bit 0 x, 1 y,
+        2 z) for a wall or a composite; its velocity is kept."""
+        self._check(self.lib.gdem_set_motion_restriction(self.h, int(wall), int(composite), int(mask)),
+                    "gdem_set_motion_restriction")
 
     def add_composite(self, parent=-1):
         """A composite obstacle under parent (-1: the root); returns its index."""
@@ -557,6 +565,8 @@ def add_obstacle_tree(s, wl):
         s.add_cylindrical_shell(*sh)
     for (wall, comp) in wl.get("attach", []):
         s.attach_wall(wall, comp)
+    for (wall, comp, mask) in wl.get("restrictions", []):
+        s.set_motion_restriction(mask, wall, comp)
     for (wall, kind, t0, t1, prm) in wl.get("motions", []):
         s.add_wall_motion(wall, kind, t0, t1, prm)
     for (comp, kind, t0, t1, prm) in wl.get("composite_motions", []):

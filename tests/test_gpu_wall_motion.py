@@ -230,3 +230,28 @@ def test_sphere_in_spinning_drum_on_gpu():
     F = s.get_state()["F"][0]
     assert F[0] == pytest.approx(-0.3 * m * 9.81, rel=1e-12)
     s.close()
+
+
+def test_restricted_walls_match_oracle(oracle_mod):
+    """Obstacle::setRestrictedGeomDirMotion on the shaken box: the crenel wall
+    (restricted along x) and the floor (restricted along z) keep their place
+    but carry their velocities into the contacts."""
+    print("[case] shaken box with restricted walls, 150 steps", flush=True)
+    wl = case(False)
+    wl["restrictions"] = [(0, -1, 1), (4, -1, 4)]
+    x0 = wl["particles"]["x"]
+    orc = oracle_mod.Oracle(wl, log_capacity=10000)
+    orc.prime()
+    s = gdem.build_system(wl)
+    s.set_contact_recording(32)
+    c0 = [s.get_wall(k)[0].copy() for k in (0, 4)]
+    for calls in range(3):
+        orc.step(50, wl["dt"])
+        s.step(50, wl["dt"])
+        _walls_equal(s, orc, len(wl["walls"]), f"after {50 * (calls + 1)} steps")
+        assert pair_set(s.get_contacts()) == pair_set(orc.contacts())
+        errs = _errors(s.get_state(), orc.state(), x0)
+        assert all(e <= 1e-9 for e in errs), errs
+    assert s.get_wall(0)[0][0] == c0[0][0] and s.get_wall(4)[0][2] == c0[1][2]
+    assert s.get_wall(4)[2][2] != 0.0      # the floor still carries its velocity
+    s.close()

@@ -179,6 +179,35 @@ def test_motions_add_in_reference_order(oracle_mod):
     assert v3[-1, 0] == 1.0000000000000002
 
 
+def test_restricted_directions(oracle_mod):
+    """Obstacle::setRestrictedGeomDirMotion: a wall translating at V with x
+    restricted keeps its x while y and z move, and keeps the full velocity;
+    a composite restricted along z keeps its centre's z, but its members get
+    the full composed translation (the reference restricts only the
+    composite's own centre, CompositeObstacle.cpp:153-160)."""
+    V = np.array([0.3, -0.2, 0.1])
+    wl = workloads.sphere_on_plate((MODEL_HOOKE, HOOKE), x0=(0.0, 0.0, 5.0e-3), gravity=0.0, dt=DT,
+                                   motions=[(MOTION_CONSTANT_TRANSLATION, 0.0, 1.0, tuple(V))])
+    wl["restrictions"] = [(0, -1, 1)]
+    o = oracle_mod.Oracle(wl)
+    o.prime()
+    o.step(20, DT)
+    c, _, v, _ = o.wall(0)
+    np.testing.assert_allclose(c, C0 + np.array([0.0, 1.0, 1.0]) * V * 20 * DT, rtol=0, atol=1e-17)
+    np.testing.assert_array_equal(v, V)
+    wl = workloads.sphere_on_plate((MODEL_HOOKE, HOOKE), x0=(0.0, 0.0, 5.0e-3), gravity=0.0, dt=DT)
+    wl["composites"] = [(-1, [0])]
+    wl["composite_motions"] = [(0, MOTION_CONSTANT_TRANSLATION, 0.0, 1.0, tuple(V))]
+    wl["restrictions"] = [(-1, 0, 4)]
+    o = oracle_mod.Oracle(wl)
+    o.prime()
+    c0 = o.composite(0)[0]
+    o.step(20, DT)
+    np.testing.assert_allclose(o.composite(0)[0], c0 + np.array([1.0, 1.0, 0.0]) * V * 20 * DT,
+                               rtol=0, atol=1e-17)
+    np.testing.assert_allclose(o.wall(0)[0], C0 + V * 20 * DT, rtol=0, atol=1e-17)
+
+
 def _mass_inertia(r=1.0e-3):
     m = workloads.DENSITY * 4.0 * math.pi * r ** 3 / 3.0
     return m, 0.4 * m * r * r
