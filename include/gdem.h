@@ -147,6 +147,16 @@ int gdem_get_basis(gdem_ctx* ctx, double* basis);
 int gdem_add_wall(gdem_ctx* ctx, const double* basis, const double* centre,
                   const double* half_extent, double crust, int32_t material);
 
+/* Static rectangle obstacle (SimpleObstacle with a Rectangle convex,
+   Geometry/src/Rectangle.cpp:13-21: sides lx along the local x, ly along the
+   local y, no thickness, normal along the local z).  Spheres meet it through
+   ClosestPointRECTANGLE (Geometry/src/RigidBodyWithCrust.cpp:1107-1226): a
+   contact only where the sphere's deepest point lies beyond the plane and its
+   projection falls strictly inside the rectangle; no overlap limit.  Sphere
+   particles only (GDEM_ERR_ARG once convex particles are set). */
+int gdem_add_rectangle_wall(gdem_ctx* ctx, const double* basis, const double* centre,
+                            double lx, double ly, double crust, int32_t material);
+
 /* HookeContactForceModel(map) (HookeContactForceModel.cpp:11-20):
      kind GDEM_MODEL_HOOKE, p = {kn, en, etat, muc, kr}            (5 values)
    HertzContactForceModel(map) (HertzContactForceModel.cpp:12-22):
@@ -213,6 +223,19 @@ int gdem_add_cylindrical_shell(gdem_ctx* ctx, const double* basis, const double*
                                double height, double radius, double width, int32_t n,
                                double crust, int32_t material, int32_t parent,
                                int32_t* composite);
+/* CylindricalShell with its default Rectangle elements (the same file
+   without ElementaryObstacle="Box"): rectangle i of sides (height, ly),
+   centred at the inner radius, its local x along the axis and its normal
+   towards it.  Sphere particles only. */
+int gdem_add_cylindrical_shell_rectangles(gdem_ctx* ctx, const double* basis,
+                                          const double* centre, double height, double radius,
+                                          double width, int32_t n, double crust,
+                                          int32_t material, int32_t parent, int32_t* composite);
+/* The shell's rectangles alone (no context): basis (9n), centre (3n),
+   half sides (3n: height/2, ly/2, 0). */
+int gdem_cylindrical_shell_rectangles(const double* basis, const double* centre, double height,
+                                      double radius, double width, int32_t n,
+                                      double* wall_basis, double* wall_centre, double* wall_half);
 /* The shell's boxes alone (no context): basis (9n), centre (3n), half edges (3n). */
 int gdem_cylindrical_shell_boxes(const double* basis, const double* centre, double height,
                                  double radius, double width, int32_t n, double* wall_basis,

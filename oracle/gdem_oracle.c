@@ -39,7 +39,9 @@ typedef struct {
   double crust, req, corners[8][3];
   double vel[3], omega[3]; /* Obstacle::m_translationalVelocity, m_angularVelocity */
   double load_f[3], load_t[3]; /* torsor of the last force evaluation, about the centre */
+  double circ;  /* circumscribed radius (rectangles) */
   int material;
+  int kind;     /* ORC_WALL_BOX or ORC_WALL_RECTANGLE (ext = LX/2, LY/2, 0) */
 } wall_t;
 
 /* ObstacleImposedVelocity (Component/src/ObstacleImposedVelocity.cpp:54-257):
@@ -251,8 +253,9 @@ int orc_set_particles(orc_sys* s, int n, const double* x, const double* v,
    (BBox.cpp:235-239) for the wall bounding box; Box::setCornersFaces
    (Box.cpp:92-111); Box::getVolume (Box.cpp:273-279). */
 static void build_wall(wall_t* wl, const double* basis, const double* centre,
-                       const double* half_extent, double crust, int material) {
+                       const double* half_extent, double crust, int material, int kind) {
   memset(wl, 0, sizeof(*wl));
+  wl->kind = kind;
   memcpy(wl->basis, basis, sizeof(double) * 9);
   memcpy(wl->origin, centre, sizeof(double) * 3);
   memcpy(wl->ext, half_extent, sizeof(double) * 3);
@@ -287,6 +290,14 @@ static void build_wall(wall_t* wl, const double* basis, const double* centre,
                                {-1, 1, -1}, {1, 1, -1}, {1, 1, 1}, {-1, 1, 1}};
   for (int k = 0; k < 8; ++k)
     for (int d = 0; d < 3; ++d) wl->corners[k][d] = sg[k][d] < 0 ? -half_extent[d] : half_extent[d];
+  if (kind == ORC_WALL_RECTANGLE) {
+    /* Rectangle::getVolume (Rectangle.cpp:182-185) is the area LX LY;
+       Rectangle::computeCircumscribedRadius (:118-121) */
+    double lx = 2. * e0, ly = 2. * e1;
+    wl->req = pow((0.75 / ORC_PI) * (lx * ly), 1. / 3.);
+    wl->circ = sqrt(lx * lx + ly * ly) / 2.;
+    return;
+  }
   double vol = 8.0 * e0 * e1 * e2;
   wl->req = pow((0.75 / ORC_PI) * vol, 1. / 3.);
 }
@@ -297,7 +308,21 @@ int orc_add_wall(orc_sys* s, const double* basis, const double* centre,
     return ORC_ERR_ARG;
   s->wall_parent[s->nwalls] = -1;
   s->wall_kin[s->nwalls].q.w = 1.;
-  build_wall(&s->walls[s->nwalls++], basis, centre, half_extent, crust, material);
+  build_wall(&s->walls[s->nwalls++], basis, centre, half_extent, crust, material, ORC_WALL_BOX);
+  return ORC_OK;
+}
+
+/* Rectangle obstacle (Geometry/src/Rectangle.cpp:13-21): sides lx, ly in
+   the local xy plane */
+int orc_add_rectangle_wall(orc_sys* s, const double* basis, const double* centre, double lx,
+                           double ly, double crust, int material) {
+  if (s->nwalls >= ORC_MAX_WALLS || material < 0 || material >= ORC_MAX_MATERIALS ||
+      !(lx > 0.) || !(ly > 0.))
+    return ORC_ERR_ARG;
+  double half[3] = {lx / 2., ly / 2., 0.};
+  s->wall_parent[s->nwalls] = -1;
+  s->wall_kin[s->nwalls].q.w = 1.;
+  build_wall(&s->walls[s->nwalls++], basis, centre, half, crust, material, ORC_WALL_RECTANGLE);
   return ORC_OK;
 }
 
@@ -545,7 +570,7 @@ static void move_wall(orc_sys* s, int w, double time, double dt, int mother, int
   int moving = kin_imposed(s, k, w, time, dt, ld(wl->origin)) || mother;
   if (moving && displace) {
     double basis[9], origin[3], ext[3], crust = wl->crust;
-    int material = wl->material;
+    int material = wl->material, kind = wl->kind;
     memcpy(basis, wl->basis, sizeof(basis));
     memcpy(ext, wl->ext, sizeof(ext));
     /* SimpleObstacle.cpp:131-136: restricted directions do not move */
@@ -556,7 +581,7 @@ static void move_wall(orc_sys* s, int w, double time, double dt, int mother, int
     double lf[3], lt[3];
     memcpy(lf, wl->load_f, sizeof(lf));
     memcpy(lt, wl->load_t, sizeof(lt));
-    build_wall(wl, basis, origin, ext, crust, material);
+    build_wall(wl, basis, origin, ext, crust, material, kind);
     memcpy(wl->load_f, lf, sizeof(lf));
     memcpy(wl->load_t, lt, sizeof(lt));
   }
@@ -578,7 +603,7 @@ static void move_wall(orc_sys* s, int w, double time, double dt, int mother, int
   int move_force = norm(ld(cvt)) != 0.;
   if (move_force && displace) {
     double basis[9], origin[3], ext[3], crust = wl->crust;
-    int material = wl->material;
+    int material = wl->material, kind = wl->kind;
     double lf[3], lt[3];
     memcpy(basis, wl->basis, sizeof(basis));
     memcpy(ext, wl->ext, sizeof(ext));
@@ -586,7 +611,7 @@ static void move_wall(orc_sys* s, int w, double time, double dt, int mother, int
     memcpy(lt, wl->load_t, sizeof(lt));
     for (int d = 0; d < 3; ++d)
       origin[d] = wl->origin[d] + ((s->wall_restrict[w] >> d) & 1 ? 0. : ctr[d]);
-    build_wall(wl, basis, origin, ext, crust, material);
+    build_wall(wl, basis, origin, ext, crust, material, kind);
     memcpy(wl->load_f, lf, sizeof(lf));
     memcpy(wl->load_t, lt, sizeof(lt));
   }
@@ -845,6 +870,55 @@ int orc_add_cylindrical_shell(orc_sys* s, const double* basis, const double* cen
   return rc;
 }
 
+/* CylindricalShell with Rectangle elements (CylindricalShell.cpp:55-87,
+   m_box false): Rectangle(height, ly) at the inner radius, basis
+   R(angle) * M (Matrix::operator*=, Matrix.cpp:447-460) with M the rows
+   (0, 0, -1), (0, 1, 0), (1, 0, 0), then the shell's transform */
+void orc_cylindrical_shell_rectangles(const double* sb, const double* sc, double height,
+                                      double radius, double width, int n, double* basis,
+                                      double* centre, double* half) {
+  double delta = 2. * ORC_PI / (double)n;
+  double ly = 2. * (radius + width) * (sin(delta) - (1. - cos(delta)) / tan(delta));
+  static const double M[3][3] = {{0., 0., -1.}, {0., 1., 0.}, {1., 0., 0.}};
+  for (int i = 0; i < n; ++i) {
+    double angle = (0.5 + (double)i) * delta;
+    double ca = cos(angle), sa = sin(angle);
+    double cg[3] = {radius * ca, radius * sa, 0.};
+    double R[3][3] = {{ca, -sa, 0.}, {sa, ca, 0.}, {0., 0., 1.}}, m[3][3];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) m[r][c] = R[r][0] * M[0][c] + R[r][1] * M[1][c] + R[r][2] * M[2][c];
+    for (int r = 0; r < 3; ++r) {
+      double* cr = centre + 3 * i + r;
+      *cr = sc[r] + (sb[3 * r] * cg[0] + sb[3 * r + 1] * cg[1] + sb[3 * r + 2] * cg[2]);
+      for (int c = 0; c < 3; ++c)
+        basis[9 * i + 3 * r + c] = sb[3 * r] * m[0][c] + sb[3 * r + 1] * m[1][c] + sb[3 * r + 2] * m[2][c];
+      *cr = *cr + 0.;
+    }
+    half[3 * i] = height / 2.;
+    half[3 * i + 1] = ly / 2.;
+    half[3 * i + 2] = 0.;
+  }
+}
+
+int orc_add_cylindrical_shell_rectangles(orc_sys* s, const double* basis, const double* centre,
+                                         double height, double radius, double width, int n,
+                                         double crust, int material, int parent, int* id) {
+  if (n < 3 || s->nwalls + n > ORC_MAX_WALLS) return ORC_ERR_ARG;
+  double* wb = (double*)malloc(sizeof(double) * 15 * (size_t)n);
+  if (!wb) return ORC_ERR_ARG;
+  orc_cylindrical_shell_rectangles(basis, centre, height, radius, width, n, wb, wb + 9 * n,
+                                   wb + 12 * n);
+  int rc = orc_add_composite(s, parent, id);
+  for (int i = 0; i < n && !rc; ++i) {
+    const double* h = wb + 12 * n + 3 * i;
+    rc = orc_add_rectangle_wall(s, wb + 9 * i, wb + 9 * n + 3 * i, 2. * h[0], 2. * h[1], crust,
+                                material);
+    if (!rc) rc = orc_attach_wall(s, s->nwalls - 1, *id);
+  }
+  free(wb);
+  return rc;
+}
+
 int orc_set_time(orc_sys* s, double time) {
   s->time = time;
   return ORC_OK;
@@ -1096,6 +1170,63 @@ static int sphere_wall(const wall_t* wl, v3 A, double r, double cr, v3* point,
     return 1;
   }
   return 0;
+}
+
+/* ClosestPointRECTANGLE with the particle first (RigidBodyWithCrust.cpp:
+   1107-1226, branch :1172-1218, checkoverlap and checkCGInRec false), after
+   ClosestPoint's circumscribed-sphere test (:282-296; m_colDetBoundingVolume
+   is 0 by default, GrainsExec.cpp:53).  B is the sphere's accumulated basis
+   (its Transform, rotated by composeLeftByRotation at every move).  The
+   rectangle normal rNorm = basis * (0, 0, 1) is not normalised
+   (Vector3::normalized returns a copy that is discarded). */
+static int sphere_rect(const wall_t* wl, v3 A, double r, const double* B, v3* point, v3* ov,
+                       double* dist) {
+  v3 rp = ld(wl->origin);
+  v3 gcagcb = sub(A, rp);
+  if (!(norm(gcagcb) < r + wl->circ)) return 0;
+  /* Matrix * Vector3 (Matrix.cpp:517-520) with (0, 0, 1) */
+  v3 rn = mk(dot(mk(wl->basis[0], wl->basis[1], wl->basis[2]), mk(0., 0., 1.)),
+             dot(mk(wl->basis[3], wl->basis[4], wl->basis[5]), mk(0., 0., 1.)),
+             dot(mk(wl->basis[6], wl->basis[7], wl->basis[8]), mk(0., 0., 1.)));
+  double sg = copysign(1., dot(rn, sub(A, rp)));
+  rn = mk(sg * rn.x, sg * rn.y, sg * rn.z);
+  /* (-rNorm) * basis: Matrix::tdot (Matrix.cpp:238-241, 527-530) */
+  v3 d = neg(rn);
+  v3 dl = mk(B[0] * d.x + B[3] * d.y + B[6] * d.z, B[1] * d.x + B[4] * d.y + B[7] * d.z,
+             B[2] * d.x + B[5] * d.y + B[8] * d.z);
+  /* Sphere::support (Sphere.cpp:99-109) */
+  v3 sp = mk(0., 0., 0.);
+  double s = norm(dl);
+  if (s > ORC_EPSILON) {
+    double rr = r / s;
+    sp = mk(dl.x * rr, dl.y * rr, dl.z * rr);
+  }
+  double Ao[3] = {A.x, A.y, A.z};
+  v3 pa = xform(B, Ao, sp);
+  if (!(dot(rn, sub(pa, rp)) < 0.)) return 0;
+  double k = dot(sub(rp, pa), rn);
+  v3 pb = add(mk(k * rn.x, k * rn.y, k * rn.z), pa);
+  /* Rectangle::isIn (Rectangle.cpp:365-370) in the rectangle's frame */
+  v3 l = xform(wl->inv_basis, wl->inv_origin, pb);
+  if (!(fabs(l.z) < ORC_EPSILON && fabs(l.y) < wl->ext[1] && fabs(l.x) < wl->ext[0])) return 0;
+  *point = add(mk(pa.x / 2.0, pa.y / 2.0, pa.z / 2.0), mk(pb.x / 2.0, pb.y / 2.0, pb.z / 2.0));
+  *ov = sub(pb, pa);
+  *dist = -norm(*ov);
+  return 1;
+}
+
+int orc_sphere_rectangle(const double* centre, double r, const double* sphere_basis,
+                         const double* basis, const double* origin, double lx, double ly,
+                         double* point, double* ovec, double* dist) {
+  wall_t wl;
+  double half[3] = {lx / 2., ly / 2., 0.};
+  build_wall(&wl, basis, origin, half, 0., 0, ORC_WALL_RECTANGLE);
+  v3 pt = mk(0., 0., 0.), ov = mk(0., 0., 0.);
+  *dist = 0.;
+  int rc = sphere_rect(&wl, ld(centre), r, sphere_basis, &pt, &ov, dist);
+  st(point, pt);
+  st(ovec, ov);
+  return rc;
 }
 
 int orc_sphere_box(const double* centre, double r, double cr,
@@ -1445,7 +1576,9 @@ static int interact_wall(orc_sys* s, int p, int wi) {
     return ORC_OK;
   v3 A = ld(xp), pt = mk(0., 0., 0.), ov = mk(0., 0., 0.);
   double dist;
-  int rc = sphere_wall(wl, A, r, s->crust[p], &pt, &ov, &dist);
+  int rc = wl->kind == ORC_WALL_RECTANGLE
+               ? sphere_rect(wl, A, r, s->basis + 9 * p, &pt, &ov, &dist)
+               : sphere_wall(wl, A, r, s->crust[p], &pt, &ov, &dist);
   if (rc < 0) return set_error(s, ORC_ERR_CONTACT, p, -1 - wi, -dist, s->crust[p] + wl->crust);
   if (rc == 0 || !(dist < 0.)) return ORC_OK;
   const model_t* m = &s->models[s->mat[p]][wl->material];

@@ -67,6 +67,10 @@ int orc_set_particles(orc_sys* s, int n, const double* x, const double* v,
                       const int32_t* material);
 
 /* Box wall obstacle: basis (row-major 3x3 rotation), centre, half extents. */
+enum { ORC_WALL_BOX = 0, ORC_WALL_RECTANGLE = 1 };
+/* Rectangle obstacle: sides lx, ly in its local xy plane, normal local z */
+int orc_add_rectangle_wall(orc_sys* s, const double* basis, const double* centre, double lx,
+                           double ly, double crust, int material);
 int orc_add_wall(orc_sys* s, const double* basis, const double* centre,
                  const double* half_extent, double crust, int material);
 
@@ -102,6 +106,13 @@ void orc_cylindrical_shell_boxes(const double* basis, const double* centre, doub
 int orc_add_cylindrical_shell(orc_sys* s, const double* basis, const double* centre,
                               double height, double radius, double width, int n, double crust,
                               int material, int parent, int* id);
+/* the shell with Rectangle elements: half = (height/2, ly/2, 0) */
+void orc_cylindrical_shell_rectangles(const double* basis, const double* centre, double height,
+                                      double radius, double width, int n, double* wall_basis,
+                                      double* wall_centre, double* wall_half);
+int orc_add_cylindrical_shell_rectangles(orc_sys* s, const double* basis, const double* centre,
+                                         double height, double radius, double width, int n,
+                                         double crust, int material, int parent, int* id);
 /* time (orc_step advances it); walls moved over [time - dt, time] */
 int orc_set_time(orc_sys* s, double time);
 double orc_get_time(const orc_sys* s);
@@ -154,6 +165,10 @@ int64_t orc_candidates(const orc_sys* s);
 int orc_sphere_sphere(const double* a, double ra, double cra, const double* b,
                       double rb, double crb, double* point, double* ovec,
                       double* dist);
+/* sphere (row-major accumulated basis) against a rectangle: 1 contact, 0 none */
+int orc_sphere_rectangle(const double* centre, double r, const double* sphere_basis,
+                         const double* basis, const double* origin, double lx, double ly,
+                         double* point, double* ovec, double* dist);
 int orc_sphere_box(const double* centre, double r, double cr,
                    const double* basis, const double* origin,
                    const double* half_extent, double crb, double* point,

@@ -62,6 +62,10 @@ def load():
             "orc_get_composite": (i32, [vp, i32, vp, vp, vp]),
             "orc_cylindrical_shell_boxes": (None, [vp, vp, d, d, d, i32, vp, vp, vp]),
             "orc_add_cylindrical_shell": (i32, [vp, vp, vp, d, d, d, i32, d, i32, i32, P(i32)]),
+            "orc_add_rectangle_wall": (i32, [vp, vp, vp, d, d, d, i32]),
+            "orc_cylindrical_shell_rectangles": (None, [vp, vp, d, d, d, i32, vp, vp, vp]),
+            "orc_add_cylindrical_shell_rectangles": (i32, [vp, vp, vp, d, d, d, i32, d, i32, i32,
+                                                           P(i32)]),
             "orc_set_time": (i32, [vp, d]),
             "orc_get_time": (d, [vp]),
             "orc_move_walls": (i32, [vp, d, d]),
@@ -84,6 +88,7 @@ def load():
             "orc_candidates": (C.c_int64, [vp]),
             "orc_sphere_sphere": (i32, [vp, d, d, vp, d, d, vp, vp, P(d)]),
             "orc_sphere_box": (i32, [vp, d, d, vp, vp, vp, d, vp, vp, P(d)]),
+            "orc_sphere_rectangle": (i32, [vp, d, vp, vp, vp, d, d, vp, vp, P(d)]),
             "orc_contact_force": (None, [i32, vp, vp, vp, d, vp, vp, vp, d, d, vp, vp, vp, d, d, vp]),
         }
         for name, (res, args) in sig.items():
@@ -124,10 +129,15 @@ class Oracle:
             self._chk(self.lib.orc_set_contact_model(h, a, b, kind, _p(_f(p, (npar,)))), "model")
         if "dt" in wl:
             self._chk(self.lib.orc_set_time_step(h, float(wl["dt"])), "time step")
-        self.nwalls = len(wl["walls"]) + sum(int(sh[5]) for sh in wl.get("shells", []))
+        self.nwalls = (len(wl["walls"]) + len(wl.get("rectangles", []))
+                       + sum(int(sh[5]) for sh in wl.get("shells", []) + wl.get("rect_shells", [])))
         for (basis, centre, half, crust, mat) in wl["walls"]:
             self._chk(self.lib.orc_add_wall(h, _p(_f(basis, (9,))), _p(_f(centre, (3,))),
                                             _p(_f(half, (3,))), crust, mat), "wall")
+        for (basis, centre, lx, ly, crust, mat) in wl.get("rectangles", []):
+            self._chk(self.lib.orc_add_rectangle_wall(h, _p(_f(basis, (9,))), _p(_f(centre, (3,))),
+                                                      float(lx), float(ly), float(crust), int(mat)),
+                      "rectangle")
         p = wl["particles"]
         n = len(p["radius"])
         self.n = n
@@ -145,6 +155,12 @@ class Oracle:
         for (basis, centre, height, radius, width, n, crust, mat, parent) in wl.get("shells", []):
             k = C.c_int()
             self._chk(self.lib.orc_add_cylindrical_shell(
+                h, _p(_f(basis, (9,))), _p(_f(centre, (3,))), float(height), float(radius),
+                float(width), int(n), float(crust), int(mat), int(parent), C.byref(k)), "shell")
+        for (basis, centre, height, radius, width, n, crust, mat, parent) in wl.get("rect_shells",
+                                                                                    []):
+            k = C.c_int()
+            self._chk(self.lib.orc_add_cylindrical_shell_rectangles(
                 h, _p(_f(basis, (9,))), _p(_f(centre, (3,))), float(height), float(radius),
                 float(width), int(n), float(crust), int(mat), int(parent), C.byref(k)), "shell")
         for (wall, comp) in wl.get("attach", []):
@@ -279,6 +295,15 @@ def cylindrical_shell_boxes(basis, centre, height, radius, width, n):
     return b, c, h
 
 
+def cylindrical_shell_rectangles(basis, centre, height, radius, width, n):
+    lib = load()
+    b, c, h = np.empty((n, 9)), np.empty((n, 3)), np.empty((n, 3))
+    lib.orc_cylindrical_shell_rectangles(_p(_f(basis, (9,))), _p(_f(centre, (3,))),
+                                         float(height), float(radius), float(width), int(n),
+                                         _p(b), _p(c), _p(h))
+    return b, c, h
+
+
 def rotate_two_vectors(u, v, x):
     """Quaternion::setRotFromTwoVectors(u, v) then rotateVector(x)."""
     out = np.empty(3)
@@ -301,6 +326,16 @@ def sphere_box(centre, r, cr, basis, origin, half, crb):
     d = C.c_double()
     rc = lib.orc_sphere_box(_p(_f(centre, (3,))), r, cr, _p(_f(basis, (9,))), _p(_f(origin, (3,))),
                             _p(_f(half, (3,))), crb, _p(pt), _p(ov), C.byref(d))
+    return rc, pt, ov, d.value
+
+
+def sphere_rectangle(centre, r, sphere_basis, basis, origin, lx, ly):
+    lib = load()
+    pt, ov = np.zeros(3), np.zeros(3)
+    d = C.c_double()
+    rc = lib.orc_sphere_rectangle(_p(_f(centre, (3,))), r, _p(_f(sphere_basis, (9,))),
+                                  _p(_f(basis, (9,))), _p(_f(origin, (3,))), float(lx), float(ly),
+                                  _p(pt), _p(ov), C.byref(d))
     return rc, pt, ov, d.value
 
 

@@ -595,6 +595,8 @@ extern "C" int gdem_set_convex_particles(gdem_ctx* c, int64_t n, const double* x
   if (n > 0 && (!x || !v || !w || !q || !basis || !shape || !density || !crust))
     return GDEM_ERR_ARG;
   if (c->has_memory) return GDEM_ERR_ARG;  // memory models: spheres only
+  for (int32_t k = 0; k < c->nwalls; ++k)
+    if (c->h_walls[k].kind != WALL_BOX) return GDEM_ERR_ARG;  // rectangles: spheres only
   for (int64_t i = 0; i < n; ++i) {
     if (shape[i] < 0 || shape[i] >= c->nshapes) return GDEM_ERR_ARG;
     if (!(density[i] > 0.) || !(crust[i] > 0.)) return GDEM_ERR_ARG;
@@ -673,11 +675,10 @@ extern "C" int gdem_get_basis(gdem_ctx* c, double* basis) {
   return collect_errors(c);
 }
 
-extern "C" int gdem_add_wall(gdem_ctx* c, const double* basis, const double* centre,
-                             const double* half, double crust, int32_t material) {
-  if (!c || !basis || !centre || !half) return GDEM_ERR_ARG;
+static int add_wall_kind(gdem_ctx* c, const double* basis, const double* centre,
+                         const double* half, double crust, int32_t material, int32_t kind) {
   if (c->nwalls >= kMaxWalls || material < 0 || material >= kMaxMaterials) return GDEM_ERR_ARG;
-  const Wall wl = make_wall(basis, centre, half, crust, material);
+  const Wall wl = make_wall(basis, centre, half, crust, material, kind);
   const double e0 = half[0], e1 = half[1], e2 = half[2];
   // the wall as a convex body for the GJK narrow phase of convex particles:
   // Box shape, crust scaling of its extents, circumscribed radius
@@ -699,6 +700,20 @@ extern "C" int gdem_add_wall(gdem_ctx* c, const double* basis, const double* cen
                          hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return GDEM_OK;
+}
+
+extern "C" int gdem_add_wall(gdem_ctx* c, const double* basis, const double* centre,
+                             const double* half, double crust, int32_t material) {
+  if (!c || !basis || !centre || !half) return GDEM_ERR_ARG;
+  return add_wall_kind(c, basis, centre, half, crust, material, WALL_BOX);
+}
+
+extern "C" int gdem_add_rectangle_wall(gdem_ctx* c, const double* basis, const double* centre,
+                                       double lx, double ly, double crust, int32_t material) {
+  if (!c || !basis || !centre || !(lx > 0.) || !(ly > 0.)) return GDEM_ERR_ARG;
+  if (c->convex) return GDEM_ERR_ARG;  // sphere particles only
+  const double half[3] = {lx / 2., ly / 2., 0.};
+  return add_wall_kind(c, basis, centre, half, crust, material, WALL_RECTANGLE);
 }
 
 extern "C" int gdem_set_contact_model(gdem_ctx* c, int32_t a, int32_t b, int32_t kind,
@@ -1473,6 +1488,41 @@ extern "C" int gdem_add_cylindrical_shell(gdem_ctx* c, const double* basis, cons
     if (rc) return rc;
   }
   *composite = id;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_add_cylindrical_shell_rectangles(gdem_ctx* c, const double* basis,
+                                                     const double* centre, double height,
+                                                     double radius, double width, int32_t n,
+                                                     double crust, int32_t material,
+                                                     int32_t parent, int32_t* composite) {
+  if (!c || !basis || !centre || !composite || n < 3 || !(height > 0.) || !(radius > 0.) ||
+      !(width > 0.) || c->nwalls + n > kMaxWalls || c->convex)
+    return GDEM_ERR_ARG;
+  std::vector<double> wb(9 * (size_t)n), wc(3 * (size_t)n), wh(3 * (size_t)n);
+  cylindrical_shell_rectangles(basis, centre, height, radius, width, n, wb.data(), wc.data(),
+                               wh.data());
+  int32_t id = -1;
+  int rc = gdem_add_composite(c, parent, &id);
+  if (rc) return rc;
+  for (int32_t i = 0; i < n; ++i) {
+    rc = add_wall_kind(c, wb.data() + 9 * i, wc.data() + 3 * i, wh.data() + 3 * i, crust,
+                       material, WALL_RECTANGLE);
+    if (rc) return rc;
+    rc = gdem_attach_wall(c, c->nwalls - 1, id);
+    if (rc) return rc;
+  }
+  *composite = id;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_cylindrical_shell_rectangles(const double* basis, const double* centre,
+                                                 double height, double radius, double width,
+                                                 int32_t n, double* wall_basis,
+                                                 double* wall_centre, double* wall_half) {
+  if (!basis || !centre || !wall_basis || !wall_centre || !wall_half || n < 3) return GDEM_ERR_ARG;
+  cylindrical_shell_rectangles(basis, centre, height, radius, width, n, wall_basis, wall_centre,
+                               wall_half);
   return GDEM_OK;
 }
 

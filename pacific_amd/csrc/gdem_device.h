@@ -94,6 +94,10 @@ __host__ __device__ __forceinline__ int model_nparams(int32_t kind) {
 }
 
 // Static box obstacle.
+// kind of a wall's convex: a Box (ext = half edges) or a Rectangle
+// (Geometry/src/Rectangle.cpp: ext = (LX/2, LY/2, 0), normal along local z)
+enum { WALL_BOX = 0, WALL_RECTANGLE = 1 };
+
 struct Wall {
   double basis[9], origin[3], ext[3];
   double inv_basis[9], inv_origin[3];
@@ -102,7 +106,8 @@ struct Wall {
   // velocity of an imposed motion (Obstacle::getVelocityAtPoint,
   // Component/src/Obstacle.cpp:128-133: vel + omega x (p - origin))
   double vel[3], omega[3];
-  int32_t material, pad;
+  double circ;  // circumscribed radius (Rectangle::computeCircumscribedRadius)
+  int32_t material, kind;
 };
 
 // Velocity of a rigid body at a point: ParticleKinematics::Velocity
@@ -439,6 +444,43 @@ __host__ __device__ __forceinline__ int sphere_wall(const Wall& wl, v3 A, double
     return 1;
   }
   return 0;
+}
+
+// ClosestPointRECTANGLE with the sphere first and the rectangle second
+// (Geometry/src/RigidBodyWithCrust.cpp:1107-1226, branch :1172-1218, no
+// overlap check), behind ClosestPoint's circumscribed-sphere test
+// (:282-296; the bounding-volume option is off by default).  The rectangle's
+// normal is its basis' z column, not renormalised (the reference's
+// rNorm.normalized() discards its result), turned towards the sphere.  The
+// sphere's support point is r d / |d| for d = -n: the reference runs it
+// through the sphere's accumulated rotation B as B (r (B^T d) / |B^T d|),
+// which is the same point to rounding and exactly it while B is the identity.
+// A contact exists only where that point lies beyond the plane and its
+// projection falls strictly inside the rectangle (Rectangle::isIn,
+// Rectangle.cpp:365-370).
+__host__ __device__ __forceinline__ int sphere_rect(const Wall& wl, v3 A, double r, v3& point, v3& ov,
+                                           double& dist) {
+  const v3 rp = mk(wl.origin[0], wl.origin[1], wl.origin[2]);
+  if (!(norm(sub(A, rp)) < r + wl.circ)) return 0;
+  v3 n = mk(wl.basis[2], wl.basis[5], wl.basis[8]);
+  n = scl(n, copysign(1., dot(n, sub(A, rp))));
+  // Sphere::support (Geometry/src/Sphere.cpp:99-109) of -n, then the
+  // sphere's transform
+  const v3 d = neg(n);
+  const double s = norm(d);
+  v3 pa = A;
+  if (s > kEps) {
+    const double rr = r / s;
+    pa = add(mk(d.x * rr, d.y * rr, d.z * rr), A);
+  }
+  if (!(dot(n, sub(pa, rp)) < 0.)) return 0;
+  const v3 pb = add(scl(n, dot(sub(rp, pa), n)), pa);
+  const v3 l = xform(wl.inv_basis, wl.inv_origin, pb);
+  if (!(fabs(l.z) < kEps && fabs(l.y) < wl.ext[1] && fabs(l.x) < wl.ext[0])) return 0;
+  point = add(dvd(pa, 2.0), dvd(pb, 2.0));
+  ov = sub(pb, pa);
+  dist = -norm(ov);
+  return 1;
 }
 
 // SSPRK3 update of the body-frame angular velocity for a sphere

@@ -80,10 +80,16 @@ inline float prefilter_margin(const Grid& g, float* sizef) {
 // Geometry/src/Transform.cpp:71-77), bounding box (Convex::bbox,
 // Geometry/src/Convex.cpp:29-41, with Box::support, Box.cpp:307-316,
 // BBox::setValue, BBox.cpp:235-239) and equivalent-sphere radius of the box
-// volume (Box::getVolume, Box.cpp:273-279).
+// volume (Box::getVolume, Box.cpp:273-279).  A Rectangle (kind
+// WALL_RECTANGLE, half = (LX/2, LY/2, 0)) has the same bounding box
+// (Rectangle::support, Rectangle.cpp:139-152, is the box's with a zero
+// third extent), the equivalent radius of its area LX LY taken as a volume
+// (Rectangle::getVolume, :182-185) and circumscribed radius
+// sqrt(LX^2 + LY^2) / 2 (:118-121).
 inline Wall make_wall(const double* basis, const double* centre, const double* half, double crust,
-                      int32_t material) {
+                      int32_t material, int32_t kind = WALL_BOX) {
   Wall wl{};
+  wl.kind = kind;
   memcpy(wl.basis, basis, sizeof(wl.basis));
   memcpy(wl.origin, centre, sizeof(wl.origin));
   memcpy(wl.ext, half, sizeof(wl.ext));
@@ -110,6 +116,12 @@ inline Wall make_wall(const double* basis, const double* centre, const double* h
   for (int i = 0; i < 3; ++i) {
     wl.bb_ext[i] = (mx[i] - mn[i]) / 2.;
     wl.bb_centre[i] = mn[i] + wl.bb_ext[i];
+  }
+  if (kind == WALL_RECTANGLE) {
+    const double lx = 2. * e0, ly = 2. * e1;
+    wl.req = pow((0.75 / kPi) * (lx * ly), 1. / 3.);
+    wl.circ = sqrt(lx * lx + ly * ly) / 2.;
+    return wl;
   }
   const double vol = 8.0 * e0 * e1 * e2;
   wl.req = pow((0.75 / kPi) * vol, 1. / 3.);

@@ -433,7 +433,8 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
       continue;
     v3 c = mk(0., 0., 0.), ov = mk(0., 0., 0.);
     double dist;
-    const int rc = sphere_wall(wl, R.X, R.r, R.ax.crust, c, ov, dist);
+    const int rc = wl.kind == WALL_RECTANGLE ? sphere_rect(wl, R.X, R.r, c, ov, dist)
+                                             : sphere_wall(wl, R.X, R.r, R.ax.crust, c, ov, dist);
     if (rc < 0) {
       atomicMin(&a.err[kErrContact], pack_ids(R.ax.id, -1 - wi));
       continue;

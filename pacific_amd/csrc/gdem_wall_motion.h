@@ -438,7 +438,7 @@ struct WallKinematics {
       memcpy(basis, wl.basis, sizeof(basis));
       memcpy(ext, wl.ext, sizeof(ext));
       compose_left_rotation(k.q, basis);
-      wl = make_wall(basis, origin, ext, wl.crust, wl.material);
+      wl = make_wall(basis, origin, ext, wl.crust, wl.material, wl.kind);
     }
     // the imposed forces (ObstacleKinematicsForce::ImposedMotion,
     // ObstacleKinematicsForce.cpp:40-63: every active one in turn, the last
@@ -457,7 +457,7 @@ struct WallKinematics {
       double basis[9], ext[3];
       memcpy(basis, wl.basis, sizeof(basis));
       memcpy(ext, wl.ext, sizeof(ext));
-      wl = make_wall(basis, origin, ext, wl.crust, wl.material);
+      wl = make_wall(basis, origin, ext, wl.crust, wl.material, wl.kind);
     }
     if (!moving && !move_force) return;
     // Obstacle::setVelocity (Obstacle.cpp:635-640): both kinematics
@@ -566,6 +566,44 @@ inline void cylindrical_shell_boxes(const double* sb, const double* sc, double h
     half[3 * i] = fabs(width) / 2.;
     half[3 * i + 1] = fabs(ly) / 2.;
     half[3 * i + 2] = fabs(height) / 2.;
+  }
+}
+
+// CylindricalShell with its default Rectangle elements
+// (Component/src/CylindricalShell.cpp:55-87 without ElementaryObstacle="Box"):
+// rectangle i of sides (LX, LY) = (height, ly) centred at the inner radius,
+// at angle (i + 1/2) 2 pi / n, its basis R(angle) M with M the rows
+// (0, 0, -1), (0, 1, 0), (1, 0, 0) (Matrix::operator*=, Matrix.cpp:447-460),
+// so that its local x is the shell axis and its normal (local z) points at
+// the axis; then the shell's transform.  half receives (LX/2, LY/2, 0).
+inline void cylindrical_shell_rectangles(const double* sb, const double* sc, double height,
+                                         double radius, double width, int n, double* basis,
+                                         double* centre, double* half) {
+  const double ext_rad = radius;
+  const double delta = 2. * kPi / double(n);
+  const double ly = 2. * (radius + width) * (sin(delta) - (1. - cos(delta)) / tan(delta));
+  const double mm[9] = {0., 0., -1., 0., 1., 0., 1., 0., 0.};
+  for (int i = 0; i < n; ++i) {
+    const double angle = (0.5 + double(i)) * delta;
+    const double ca = cos(angle), sa = sin(angle);
+    const double cg[3] = {ext_rad * ca, ext_rad * sa, 0.};
+    const double m0[9] = {ca, -sa, 0., sa, ca, 0., 0., 0., 1.};
+    double m[9];
+    for (int r = 0; r < 3; ++r)
+      for (int col = 0; col < 3; ++col)
+        m[3 * r + col] = m0[3 * r] * mm[col] + m0[3 * r + 1] * mm[3 + col] + m0[3 * r + 2] * mm[6 + col];
+    double* B = basis + 9 * i;
+    double* C = centre + 3 * i;
+    for (int r = 0; r < 3; ++r) {
+      C[r] = sc[r] + (sb[3 * r] * cg[0] + sb[3 * r + 1] * cg[1] + sb[3 * r + 2] * cg[2]);
+      for (int col = 0; col < 3; ++col)
+        B[3 * r + col] = sb[3 * r] * m[col] + sb[3 * r + 1] * m[3 + col] + sb[3 * r + 2] * m[6 + col];
+      C[r] = C[r] + 0.;
+    }
+    // Rectangle(height, ly) (Geometry/src/Rectangle.cpp:13-21)
+    half[3 * i] = height / 2.;
+    half[3 * i + 1] = ly / 2.;
+    half[3 * i + 2] = 0.;
   }
 }
 

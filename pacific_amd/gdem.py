@@ -120,6 +120,10 @@ def load_library(path: str = LIB_PATH):
         "gdem_get_composite": (C.c_int, [vp, i32, vp, vp, vp]),
         "gdem_add_cylindrical_shell": (C.c_int, [vp, vp, vp, d, d, d, i32, d, i32, i32, P(i32)]),
         "gdem_cylindrical_shell_boxes": (C.c_int, [vp, vp, d, d, d, i32, vp, vp, vp]),
+        "gdem_add_rectangle_wall": (C.c_int, [vp, vp, vp, d, d, d, i32]),
+        "gdem_add_cylindrical_shell_rectangles": (C.c_int, [vp, vp, vp, d, d, d, i32, d, i32, i32,
+                                                            P(i32)]),
+        "gdem_cylindrical_shell_rectangles": (C.c_int, [vp, vp, d, d, d, i32, vp, vp, vp]),
         "gdem_set_time": (C.c_int, [vp, d]),
         "gdem_get_time": (C.c_int, [vp, P(d)]),
         "gdem_move_walls": (C.c_int, [vp, d, d]),
@@ -183,6 +187,8 @@ EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall"
             "gdem_add_composite",
             "gdem_attach_wall", "gdem_add_composite_motion", "gdem_get_composite",
             "gdem_add_cylindrical_shell", "gdem_cylindrical_shell_boxes", "gdem_set_time",
+            "gdem_add_rectangle_wall", "gdem_add_cylindrical_shell_rectangles",
+            "gdem_cylindrical_shell_rectangles",
             "gdem_get_time", "gdem_move_walls", "gdem_get_wall", "gdem_write_particles_2014",
             "gdem_read_particles_2014", "gdem_set_contact_recording",
             "gdem_get_contacts", "gdem_last_error", "gdem_set_kernel_timing",
@@ -364,6 +370,24 @@ class GdemSystem:
             "gdem_add_cylindrical_shell")
         return k.value
 
+    def add_rectangle_wall(self, basis, centre, lx, ly, crust, material):
+        """Rectangle obstacle of sides (lx, ly) in its local xy plane (normal
+        local z); spheres only.  Its wall index is the next one."""
+        self._check(self.lib.gdem_add_rectangle_wall(
+            self.h, _ptr(_f64(basis, (9,))), _ptr(_f64(centre, (3,))), float(lx), float(ly),
+            float(crust), int(material)), "gdem_add_rectangle_wall")
+
+    def add_cylindrical_shell_rectangles(self, basis, centre, height, radius, width, n, crust,
+                                         material, parent=-1):
+        """CylindricalShell with its default Rectangle elements as a
+        composite of n walls; returns its index."""
+        k = C.c_int32()
+        self._check(self.lib.gdem_add_cylindrical_shell_rectangles(
+            self.h, _ptr(_f64(basis, (9,))), _ptr(_f64(centre, (3,))), float(height), float(radius),
+            float(width), int(n), float(crust), int(material), int(parent), C.byref(k)),
+            "gdem_add_cylindrical_shell_rectangles")
+        return k.value
+
     def set_time(self, t):
         self._check(self.lib.gdem_set_time(self.h, float(t)), "gdem_set_time")
 
@@ -529,6 +553,8 @@ def build_system(wl, device=0):
         s.set_contact_model(a, b, kind, p)
     for wdef in wl["walls"]:
         s.add_wall(*wdef)
+    for rdef in wl.get("rectangles", []):
+        s.add_rectangle_wall(*rdef)
     p = wl["particles"]
     if "shapes" in wl:
         for sh in wl["shapes"]:
@@ -555,14 +581,18 @@ def add_obstacle_tree(s, wl):
     """Composites, shells and imposed motions of a workload dict, in the
     order both the library and the oracle create them: composites
     (parent, member walls), then shells (basis, centre, height, radius, width,
-    n, crust, material, parent), then further members (wall, composite), then
-    wall and composite motions."""
+    n, crust, material, parent), box elements then rectangle elements
+    ("rect_shells"), then further members (wall, composite), then wall and
+    composite motions.  Rectangle walls ("rectangles": basis, centre, lx, ly,
+    crust, material) follow the box walls."""
     for (parent, members) in wl.get("composites", []):
         k = s.add_composite(parent)
         for w in members:
             s.attach_wall(w, k)
     for sh in wl.get("shells", []):
         s.add_cylindrical_shell(*sh)
+    for sh in wl.get("rect_shells", []):
+        s.add_cylindrical_shell_rectangles(*sh)
     for (wall, comp) in wl.get("attach", []):
         s.attach_wall(wall, comp)
     for (wall, comp, mask) in wl.get("restrictions", []):
@@ -585,6 +615,20 @@ def cylindrical_shell_boxes(basis, centre, height, radius, width, n):
                                           _ptr(b), _ptr(c), _ptr(h))
     if rc:
         raise GdemError(rc, "gdem_cylindrical_shell_boxes")
+    return b, c, h
+
+
+def cylindrical_shell_rectangles(basis, centre, height, radius, width, n):
+    """The rectangle walls of a CylindricalShell with its default Rectangle
+    elements: (basis (n, 9), centre (n, 3), half sides (n, 3): height/2,
+    ly/2, 0)."""
+    lib = load_library()
+    b, c, h = np.empty((n, 9)), np.empty((n, 3)), np.empty((n, 3))
+    rc = lib.gdem_cylindrical_shell_rectangles(_ptr(_f64(basis, (9,))), _ptr(_f64(centre, (3,))),
+                                               float(height), float(radius), float(width), int(n),
+                                               _ptr(b), _ptr(c), _ptr(h))
+    if rc:
+        raise GdemError(rc, "gdem_cylindrical_shell_rectangles")
     return b, c, h
 
 

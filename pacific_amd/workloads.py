@@ -73,6 +73,31 @@ def box_walls(lo, hi, material, thickness=5.0e-4, margin=5.0e-4, crust=CRUST):
     return walls
 
 
+# row-major bases whose local z (the Rectangle normal) is world axis d, the
+# local x and y the next two axes in turn
+FACE_BASIS = ((0.0, 0.0, 1.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0),
+              (0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 1.0, 0.0, 0.0),
+              IDENTITY)
+
+
+def box_rectangles(lo, hi, material, margin=5.0e-4, crust=CRUST):
+    """Six Rectangle obstacles on the faces of [lo, hi] (no thickness), each
+    margin wider than the face on every side: (basis, centre, lx, ly, crust,
+    material)."""
+    lo = np.asarray(lo, float)
+    hi = np.asarray(hi, float)
+    c = 0.5 * (lo + hi)
+    rects = []
+    for d in range(3):
+        a, b = (d + 1) % 3, (d + 2) % 3
+        for side in (-1, 1):
+            centre = c.copy()
+            centre[d] = lo[d] if side < 0 else hi[d]
+            rects.append((FACE_BASIS[d], tuple(centre), hi[a] - lo[a] + 2 * margin,
+                          hi[b] - lo[b] + 2 * margin, crust, material))
+    return rects
+
+
 def _particles(x, v, radius, seed_q=None):
     n = len(radius)
     q = np.zeros((n, 4))
@@ -163,6 +188,17 @@ def sphere_on_plate(model, half=(0.05, 0.05), x0=(0.0, 0.0, None), r=1.0e-3, dt=
             "particles": _particles(x, np.zeros((1, 3)), np.array([r]))}
 
 
+def rectangle_box(n=1000, seed=1, memory=False):
+    """plain_box (memory_box when memory) closed by six Rectangle obstacles on
+    the faces of the domain instead of box walls."""
+    wl = memory_box(n, seed) if memory else plain_box(n, seed)
+    lo = np.asarray(wl["origin"], float)
+    wl["name"] = f"rectangle_box_{n}" + ("_memory" if memory else "")
+    wl["rectangles"] = box_rectangles(lo, lo + np.asarray(wl["size"], float), 1)
+    wl["walls"] = []
+    return wl
+
+
 def shaken_box(n=1000, seed=1, memory=False):
     """plain_box (HookeMemory between particles and walls when memory) with an
     imposed motion on five walls (wall k of box_walls: 2 d + (side > 0)):
@@ -207,14 +243,16 @@ DRUM_BASIS = (1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, -1.0, 0.0)
 
 
 def rotating_drum(seed=6, radius=0.01, length=0.04, nbox=24, omega=5.0, fill=0.55, memory=False,
-                  r=1.0e-3, gap=2.0e-5):
+                  r=1.0e-3, gap=2.0e-5, elements="box"):
     """Spheres (r 1 mm, density 2500) in a horizontal drum: a box-element
     CylindricalShell (inner radius 1 cm, wall 1 mm, 24 boxes, axis along y)
     and two end caps in one composite (the drum: caps, then the shell's own
     composite) rotating at omega about the drum axis (y); in each 2.2 mm slice along the axis, a ring of sites gap from the
     shell's faces (2.2 mm apart) and a jittered 2.2 mm lattice inside it, in
     the lower part of the drum (z below (2 fill - 1) radius), v ~ N(0,
-    0.05^2); Hooke kn 1e5, en 0.9, mu 0.5 (HookeMemory when memory); dt 1e-5."""
+    0.05^2); Hooke kn 1e5, en 0.9, mu 0.5 (HookeMemory when memory); dt 1e-5.
+    elements="rectangle" builds the shell from its default Rectangle
+    elements (inner faces at the same radius)."""
     rng = np.random.default_rng(seed)
     spacing = 2.2e-3
     ring = radius - r - gap        # a ring of sites gap from the shell's faces
@@ -254,7 +292,8 @@ def rotating_drum(seed=6, radius=0.01, length=0.04, nbox=24, omega=5.0, fill=0.5
             "models": models, "walls": walls, "time": 0.0,
             # composite 0 (the drum) holds the caps and the shell (composite 1)
             "composites": [(-1, [0, 1])],
-            "shells": [(DRUM_BASIS, (0.0, 0.0, 0.0), length, radius, width, nbox, CRUST, 1, 0)],
+            ("rect_shells" if elements == "rectangle" else "shells"):
+                [(DRUM_BASIS, (0.0, 0.0, 0.0), length, radius, width, nbox, CRUST, 1, 0)],
             "composite_motions": [(0, 5, 0.0, 10.0, (0.0, omega, 0.0, 0.0, 0.0, 0.0, 1.0))],
             "particles": _particles(x, v, np.full(n, r))}
 
