@@ -189,9 +189,24 @@ struct BoxObstacle {
   std::string material;
 };
 
-// CylindricalShell (Component/include/CylindricalShell.hh) with Box elements:
-// n boxes (named name0 .. name{n-1}) around the axis of the shell's transform
+// A static SimpleObstacle with a Rectangle convex (Geometry/include/
+// Rectangle.hh: sides LX along the local x, LY along the local y, no
+// thickness, normal along the local z).  Sphere particles only.
+struct RectangleObstacle {
+  std::string name;
+  double centre[3];
+  double LX, LY;
+  double basis[9];
+  double crust;
+  std::string material;
+};
+
+// CylindricalShell (Component/include/CylindricalShell.hh): n elements
+// (named name0 .. name{n-1}) around the axis of the shell's transform
 // (centre, row-major basis; the axis is the basis' z), a composite obstacle.
+// As the XML attribute ElementaryObstacle: "Box" gives box elements, any
+// other value (the default, empty) the reference's default Rectangle
+// elements (CylindricalShell.cpp:31-35).
 struct CylindricalShell {
   std::string name;
   double centre[3];
@@ -200,6 +215,7 @@ struct CylindricalShell {
   int n;
   double crust;
   std::string material;
+  std::string elementary_obstacle;
 };
 
 // ObstacleImposedVelocity (Component/include/ObstacleImposedVelocity.hh): an
@@ -244,6 +260,7 @@ class LinkedCell {
   void Link(std::vector<SphereParticle> const& particles);
   // LinkedCell::Link(Obstacle*) (LinkedCell.cpp:1371-1445)
   void Link(BoxObstacle const& obstacle);
+  void Link(RectangleObstacle const& obstacle);
   // AppCollision::LinkUpdate (LinkedCell.cpp:1452-1547)
   void LinkUpdate(double time, double dt);
   // App::ComputeForces (Base/include/App.hh:38-39)

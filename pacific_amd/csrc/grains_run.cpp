@@ -18,8 +18,11 @@
 // composite, type, double t_start, t_end, p[12] (composites numbered in
 // creation order, then the shells; a shell's boxes follow the walls); int32
 // nforces, per imposed force int32 wall, double t_start, t_end, ax, ay, az,
-// mass (<= 0: automatic), vmax; then x[3n], v[3n], w[3n], q[4n], radius[n],
-// density[n], crust[n], int32 material[n].
+// mass (<= 0: automatic), vmax.  GDEMIN05 is GDEMIN04 with, after the walls,
+// int32 nrectangles, per rectangle double basis[9], centre[3], lx, ly,
+// crust, int32 material (wall indices after the boxes), and each shell
+// record ending with int32 elements (1 Box, 0 Rectangle); then x[3n], v[3n],
+// w[3n], q[4n], radius[n], density[n], crust[n], int32 material[n].
 // Output: "GDEMOUT1", int32 n, status (0 or the GDEM_ERR_* the step raised),
 // id0, id1, then x, v, w, q, F, T of the final state.
 #include <cstdio>
@@ -47,9 +50,11 @@ int main(int argc, char** argv) {
   int32_t n, nwalls, nmodels, nsteps, periodic[3];
   double dt, origin[3], size[3], cell, grav[3];
   if (!rd(f, magic, 8) || (memcmp(magic, "GDEMIN01", 8) && memcmp(magic, "GDEMIN02", 8) &&
-                            memcmp(magic, "GDEMIN03", 8) && memcmp(magic, "GDEMIN04", 8)))
+                            memcmp(magic, "GDEMIN03", 8) && memcmp(magic, "GDEMIN04", 8) &&
+                            memcmp(magic, "GDEMIN05", 8)))
     return 2;
-  const bool with_tree = memcmp(magic, "GDEMIN04", 8) == 0;
+  const bool with_rect = memcmp(magic, "GDEMIN05", 8) == 0;
+  const bool with_tree = with_rect || memcmp(magic, "GDEMIN04", 8) == 0;
   const bool with_motions = with_tree || memcmp(magic, "GDEMIN03", 8) == 0;
   const int np = memcmp(magic, "GDEMIN01", 8) == 0 ? 5 : 8;
   if (!rd(f, &n, 1) || !rd(f, &nwalls, 1) ||
@@ -98,6 +103,20 @@ int main(int argc, char** argv) {
     w.material = "mat" + std::to_string(mat);
     w.name = "wall" + std::to_string(&w - walls.data());
   }
+  std::vector<RectangleObstacle> rects;
+  if (with_rect) {
+    int32_t nr = 0;
+    if (!rd(f, &nr, 1) || nr < 0) return 2;
+    rects.resize((size_t)nr);
+    for (auto& rc : rects) {
+      int32_t mat;
+      if (!rd(f, rc.basis, 9) || !rd(f, rc.centre, 3) || !rd(f, &rc.LX, 1) || !rd(f, &rc.LY, 1) ||
+          !rd(f, &rc.crust, 1) || !rd(f, &mat, 1))
+        return 2;
+      rc.material = "mat" + std::to_string(mat);
+      rc.name = "wall" + std::to_string(walls.size() + (size_t)(&rc - rects.data()));
+    }
+  }
   // imposed velocities as the reference's XML nodes name their parameters
   // imposed velocities, read as (target, record); target = wall index, or
   // -1 - composite index for a composite's motion
@@ -132,7 +151,7 @@ int main(int argc, char** argv) {
   };
   for (int32_t k = 0; k < nmotions; ++k)
     if (!read_motion(false)) return 2;
-  // composites, box-element cylindrical shells and their motions
+  // composites, cylindrical shells and their motions
   std::vector<std::pair<int32_t, std::vector<int32_t>>> composites;
   std::vector<std::pair<CylindricalShell, int32_t>> shells;
   std::vector<std::pair<int32_t, ObstacleImposedForce>> forces;
@@ -154,6 +173,9 @@ int main(int argc, char** argv) {
           !rd(f, &sh.radius, 1) || !rd(f, &sh.width, 1) || !rd(f, &nbox, 1) || !rd(f, &sh.crust, 1) ||
           !rd(f, &mat, 1) || !rd(f, &parent, 1))
         return 2;
+      int32_t box = 1;
+      if (with_rect && !rd(f, &box, 1)) return 2;
+      sh.elementary_obstacle = box ? "Box" : "";
       sh.n = nbox;
       sh.material = "mat" + std::to_string(mat);
       sh.name = "shell" + std::to_string(k) + "_";
@@ -205,9 +227,11 @@ int main(int argc, char** argv) {
     return 3;
   }
   for (auto const& wl : walls) lc.Link(wl);
+  for (auto const& rc : rects) lc.Link(rc);
   // names of composites in creation order, and of every wall by index
   std::vector<std::string> comp_names, wall_names;
   for (auto const& wl : walls) wall_names.push_back(wl.name);
+  for (auto const& rc : rects) wall_names.push_back(rc.name);
   for (auto const& c : composites) {
     const std::string name = "comp" + std::to_string(comp_names.size());
     lc.LinkComposite(name, c.first < 0 ? "" : comp_names.at((size_t)c.first));

@@ -323,6 +323,16 @@ void LinkedCell::Link(BoxObstacle const& ob) {
   push_models();
 }
 
+void LinkedCell::Link(RectangleObstacle const& ob) {
+  if (!m_ctx) throw std::logic_error("LinkedCell::set must be called first");
+  ContactBuilderFactory::defineMaterial(ob.material, true);
+  if (gdem_add_rectangle_wall(m_ctx, ob.basis, ob.centre, ob.LX, ob.LY, ob.crust,
+                              ContactBuilderFactory::materialIndex(ob.material)))
+    throw std::invalid_argument("obstacle " + ob.name + " rejected");
+  m_obstacles.emplace(ob.name, m_nobstacles++);
+  push_models();
+}
+
 int LinkedCell::obstacleIndex(std::string const& name) const {
   auto it = m_obstacles.find(name);
   return it == m_obstacles.end() ? -1 : it->second;
@@ -367,10 +377,14 @@ void LinkedCell::Link(CylindricalShell const& sh, std::string const& parent) {
   const int p = parent.empty() ? -1 : compositeIndex(parent);
   if (!parent.empty() && p < 0) throw std::invalid_argument("no composite named " + parent);
   int32_t id = -1;
-  if (gdem_add_cylindrical_shell(m_ctx, sh.basis, sh.centre, sh.height, sh.radius, sh.width, sh.n,
-                                 sh.crust, ContactBuilderFactory::materialIndex(sh.material), p,
-                                 &id))
-    throw std::invalid_argument("cylindrical shell " + sh.name + " rejected");
+  const int32_t mat = ContactBuilderFactory::materialIndex(sh.material);
+  const int rc = sh.elementary_obstacle == "Box"
+                     ? gdem_add_cylindrical_shell(m_ctx, sh.basis, sh.centre, sh.height, sh.radius,
+                                                  sh.width, sh.n, sh.crust, mat, p, &id)
+                     : gdem_add_cylindrical_shell_rectangles(m_ctx, sh.basis, sh.centre, sh.height,
+                                                             sh.radius, sh.width, sh.n, sh.crust,
+                                                             mat, p, &id);
+  if (rc) throw std::invalid_argument("cylindrical shell " + sh.name + " rejected");
   m_composites[sh.name] = id;
   for (int i = 0; i < sh.n; ++i) m_obstacles.emplace(sh.name + std::to_string(i), m_nobstacles++);
   push_models();
