@@ -263,6 +263,15 @@ int gdem_get_wall_loads(gdem_ctx* ctx, double* force, double* torque);
    trip while the force is pending.  Sphere contexts. */
 int gdem_add_wall_force(gdem_ctx* ctx, int32_t wall, double t_start, double t_end,
                         const double* p);
+/* Imposed force on a composite (CompositeObstacle::LinkImposedMotion(
+   ObstacleImposedForce*) and Move, Component/src/CompositeObstacle.cpp:
+   112-120, 182-203): the same law on the composite's load, the sum of its
+   members' (CompositeObstacle::getTorsor, :650-660, as left after the
+   previous step: zero at the first step); its translation moves the
+   composite's centre and is added to every member's.  mass must be > 0 (the
+   automatic mass would divide by the composite's zero crust). */
+int gdem_add_composite_force(gdem_ctx* ctx, int32_t composite, double t_start, double t_end,
+                             const double* p);
 
 /* Simulation time (Grains::m_time, Base/src/Grains.cpp:114, :290); gdem_step
    advances it by dt per step. */

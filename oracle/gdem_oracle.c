@@ -75,6 +75,7 @@ typedef struct {
   int mem[ORC_MAX_WALLS + ORC_MAX_COMPOSITES];
   double centre[3], basis[9], vel[3], omega[3];
   kin_t kin;
+  double conf_vt[3], conf_trans[3];  /* its ObstacleKinematicsForce */
 } comp_t;
 
 struct orc_sys {
@@ -105,6 +106,8 @@ struct orc_sys {
   double pid_dforce_nm1[3], pid_dforce_nm2[3], pid_depl_nm1[3];
   int pid_nonzero;
   double conf_vt[ORC_MAX_WALLS][3], conf_trans[ORC_MAX_WALLS][3];
+  /* composites' torsors exist once a step has ended (Grains.cpp:381) */
+  int comp_loads;
   int wall_restrict[ORC_MAX_WALLS];   /* Obstacle::setRestrictedGeomDirMotion */
   model_t models[ORC_MAX_MATERIALS][ORC_MAX_MATERIALS];
   /* cell lists: doubly linked, head insertion (Cell.cpp:90-93, push_front) */
@@ -564,7 +567,8 @@ static void force_velocity(orc_sys* s, force_t* m, double time, double dt, doubl
   memcpy(vt, m->vt, sizeof(double) * 3);
 }
 
-static void move_wall(orc_sys* s, int w, double time, double dt, int mother, int displace) {
+static void move_wall(orc_sys* s, int w, double time, double dt, int mother, int mother_force,
+                      int displace) {
   wall_t* wl = &s->walls[w];
   kin_t* k = &s->wall_kin[w];
   int moving = kin_imposed(s, k, w, time, dt, ld(wl->origin)) || mother;
@@ -600,7 +604,7 @@ static void move_wall(orc_sys* s, int w, double time, double dt, int mother, int
     if (!(sub_dt > dt * 1.e-5)) continue;
     force_velocity(s, m, time, dt, sub_dt, ld(wl->load_f), wl->crust, cvt, ctr);
   }
-  int move_force = norm(ld(cvt)) != 0.;
+  int move_force = norm(ld(cvt)) != 0. || mother_force;
   if (move_force && displace) {
     double basis[9], origin[3], ext[3], crust = wl->crust;
     int material = wl->material, kind = wl->kind;
@@ -622,10 +626,23 @@ static void move_wall(orc_sys* s, int w, double time, double dt, int mother, int
 }
 
 static void move_members(orc_sys* s, int parent, double time, double dt, int mother,
-                         int displace);
+                         int mother_force, int displace);
+
+/* CompositeObstacle::getTorsor (CompositeObstacle.cpp:650-660), its force:
+   zero, then += each member's in attach order */
+static v3 composite_load(const orc_sys* s, int c) {
+  const comp_t* C = &s->comp[c];
+  v3 f = mk(0., 0., 0.);
+  for (int i = 0; i < C->nmem; ++i) {
+    int mbr = C->mem[i];
+    f = add(f, mbr < 0 ? composite_load(s, -1 - mbr) : ld(s->walls[mbr].load_f));
+  }
+  return f;
+}
 
 /* CompositeObstacle::Move (Component/src/CompositeObstacle.cpp:135-220) */
-static void move_composite(orc_sys* s, int c, double time, double dt, int mother, int displace) {
+static void move_composite(orc_sys* s, int c, double time, double dt, int mother,
+                           int mother_force, int displace) {
   comp_t* C = &s->comp[c];
   int moving = kin_imposed(s, &C->kin, -1 - c, time, dt, ld(C->centre)) || mother;
   if (moving && displace) {
@@ -647,19 +664,44 @@ static void move_composite(orc_sys* s, int c, double time, double dt, int mother
         kin_compose(&s->wall_kin[mbr], &C->kin, sub(ld(s->walls[mbr].origin), ld(C->centre)));
       }
     }
-  move_members(s, c, time, dt, moving, displace);
-  if (moving) {
-    st(C->vel, C->kin.vt);
+  /* imposed forces on the composite's load (CompositeObstacle.cpp:182-203) */
+  v3 load = s->comp_loads ? composite_load(s, c) : mk(0., 0., 0.);
+  for (int i = 0; i < s->nforce; ++i) {
+    force_t* m = &s->force[i];
+    double sub_dt;
+    if (m->wall != -1 - c) continue;
+    sub_dt = time - (time - dt);
+    if (time - dt < m->t0) sub_dt -= (m->t0 - (time - dt));
+    if (m->t1 < time) sub_dt -= (time - m->t1);
+    if (!(sub_dt > dt * 1.e-5)) continue;
+    force_velocity(s, m, time, dt, sub_dt, load, 0., C->conf_vt, C->conf_trans);
+  }
+  int move_force = norm(ld(C->conf_vt)) != 0. || mother_force;
+  if (move_force && displace)
+    for (int d = 0; d < 3; ++d)
+      C->centre[d] = C->centre[d] + ((C->restrict_mask >> d) & 1 ? 0. : C->conf_trans[d]);
+  /* ObstacleKinematicsForce::Compose (ObstacleKinematicsForce.cpp:51-55) */
+  if (move_force)
+    for (int i = 0; i < C->nmem; ++i) {
+      int mbr = C->mem[i];
+      double* vt = mbr < 0 ? s->comp[-1 - mbr].conf_vt : s->conf_vt[mbr];
+      double* tr = mbr < 0 ? s->comp[-1 - mbr].conf_trans : s->conf_trans[mbr];
+      st(vt, add(ld(vt), ld(C->conf_vt)));
+      st(tr, add(ld(tr), ld(C->conf_trans)));
+    }
+  move_members(s, c, time, dt, moving, move_force, displace);
+  if (moving || move_force) {
+    st(C->vel, add(C->kin.vt, ld(C->conf_vt)));
     st(C->omega, C->kin.vr);
   }
 }
 
 static void move_members(orc_sys* s, int parent, double time, double dt, int mother,
-                         int displace) {
+                         int mother_force, int displace) {
   for (int k = 0; k < s->ncomp; ++k)
-    if (s->comp[k].parent == parent) move_composite(s, k, time, dt, mother, displace);
+    if (s->comp[k].parent == parent) move_composite(s, k, time, dt, mother, mother_force, displace);
   for (int w = 0; w < s->nwalls; ++w)
-    if (s->wall_parent[w] == parent) move_wall(s, w, time, dt, mother, displace);
+    if (s->wall_parent[w] == parent) move_wall(s, w, time, dt, mother, mother_force, displace);
 }
 
 /* Obstacle::resetKinematics (Obstacle.cpp:160-167), from the root down
@@ -673,7 +715,8 @@ static void reset_walls(orc_sys* s) {
   }
   for (int c = 0; c < s->ncomp; ++c) {
     s->comp[c].kin = kin_zero();
-    for (int d = 0; d < 3; ++d) s->comp[c].vel[d] = s->comp[c].omega[d] = 0.;
+    for (int d = 0; d < 3; ++d)
+      s->comp[c].vel[d] = s->comp[c].omega[d] = s->comp[c].conf_vt[d] = s->comp[c].conf_trans[d] = 0.;
   }
 }
 
@@ -711,6 +754,25 @@ int orc_add_wall_force(orc_sys* s, int wall, double t0, double t1, const double*
   return ORC_OK;
 }
 
+/* ObstacleImposedForce on a composite (CompositeObstacle::LinkImposedMotion,
+   CompositeObstacle.cpp:112-120) */
+int orc_add_composite_force(orc_sys* s, int comp, double t0, double t1, const double* p) {
+  if (comp < 0 || comp >= s->ncomp || s->nforce >= ORC_MAX_MOTIONS || !(t1 >= t0) ||
+      !(p[3] > 0.))
+    return ORC_ERR_ARG;
+  force_t* f = &s->force[s->nforce++];
+  memset(f, 0, sizeof(*f));
+  f->wall = -1 - comp;
+  f->live = 1;
+  f->t0 = t0;
+  f->t1 = t1;
+  st(f->amp, ld(p));
+  st(f->dir, dvd(ld(p), norm(ld(p))));
+  f->mass = p[3];
+  f->vmax = p[4];
+  return ORC_OK;
+}
+
 /* obstacle part of AllComponents::Move (Component/src/AllComponents.cpp:
    151-220) at `time`, over [time - dt, time] */
 int orc_move_walls(orc_sys* s, double time, double dt) {
@@ -730,8 +792,9 @@ int orc_move_walls(orc_sys* s, double time, double dt) {
   }
   if (anyactive || s->active_prev) reset_walls(s);
   s->active_prev = anyactive;
-  if (anyactive) move_members(s, -1, time, dt, 0, 1);
+  if (anyactive) move_members(s, -1, time, dt, 0, 0, 1);
   drop_completed(s, time, dt);
+  s->comp_loads = 1;
   return ORC_OK;
 }
 
@@ -739,7 +802,7 @@ int orc_move_walls(orc_sys* s, double time, double dt) {
 static void init_wall_kinematics(orc_sys* s, double time, double dt) {
   if (!any_live(s)) return;
   reset_walls(s);
-  move_members(s, -1, time, dt, 0, 0);
+  move_members(s, -1, time, dt, 0, 0, 0);
   drop_completed(s, time, dt);
 }
 

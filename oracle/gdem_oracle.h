@@ -121,6 +121,8 @@ int orc_get_wall(const orc_sys* s, int wall, double* centre, double* basis, doub
                  double* omega);
 /* imposed force on a wall, p = {ax, ay, az, mass (<= 0: automatic), vmax} */
 int orc_add_wall_force(orc_sys* s, int wall, double t_start, double t_end, const double* p);
+/* the same on composite comp's load (mass must be > 0) */
+int orc_add_composite_force(orc_sys* s, int comp, double t_start, double t_end, const double* p);
 /* wall torsors of the last force evaluation (3 * nwalls each) */
 int orc_get_wall_loads(const orc_sys* s, double* force, double* torque);
 /* dt of the force evaluations (memory models); orc_step sets it too. */

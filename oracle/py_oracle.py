@@ -55,6 +55,7 @@ def load():
             "orc_add_wall_motion": (i32, [vp, i32, i32, d, d, vp]),
             "orc_get_wall_loads": (i32, [vp, vp, vp]),
             "orc_add_wall_force": (i32, [vp, i32, d, d, vp]),
+            "orc_add_composite_force": (i32, [vp, i32, d, d, vp]),
             "orc_set_motion_restriction": (i32, [vp, i32, i32, i32]),
             "orc_add_composite": (i32, [vp, i32, P(i32)]),
             "orc_attach_wall": (i32, [vp, i32, i32]),
@@ -177,6 +178,9 @@ class Oracle:
         for (wall, t0, t1, prm) in wl.get("wall_forces", []):
             self._chk(self.lib.orc_add_wall_force(h, int(wall), float(t0), float(t1),
                                                   _p(_f(prm, (5,)))), "wall force")
+        for (comp, t0, t1, prm) in wl.get("composite_forces", []):
+            self._chk(self.lib.orc_add_composite_force(h, int(comp), float(t0), float(t1),
+                                                       _p(_f(prm, (5,)))), "composite force")
         if log_capacity:
             self.lib.orc_enable_contact_log(h, int(log_capacity))
 

@@ -1406,12 +1406,11 @@ extern "C" int gdem_add_wall_motion(gdem_ctx* c, int32_t wall, int32_t type, dou
   return add_motion(c, wall, type, t_start, t_end, p);
 }
 
-extern "C" int gdem_add_wall_force(gdem_ctx* c, int32_t wall, double t_start, double t_end,
-                                   const double* p) {
-  if (!c || !p || c->convex || wall < 0 || wall >= c->nwalls || !(t_end >= t_start))
-    return GDEM_ERR_ARG;
+static int add_force(gdem_ctx* c, int32_t target, double t_start, double t_end,
+                     const double* p) {
+  if (!(t_end >= t_start)) return GDEM_ERR_ARG;
   WallForce f;
-  f.wall = wall;
+  f.wall = target;
   f.t0 = t_start;
   f.t1 = t_end;
   f.amp = mk(p[0], p[1], p[2]);
@@ -1428,6 +1427,21 @@ extern "C" int gdem_add_wall_force(gdem_ctx* c, int32_t wall, double t_start, do
   c->kin.fit(c->nwalls);
   c->kin.insert_force(f);
   return GDEM_OK;
+}
+
+extern "C" int gdem_add_wall_force(gdem_ctx* c, int32_t wall, double t_start, double t_end,
+                                   const double* p) {
+  if (!c || !p || c->convex || wall < 0 || wall >= c->nwalls) return GDEM_ERR_ARG;
+  return add_force(c, wall, t_start, t_end, p);
+}
+
+extern "C" int gdem_add_composite_force(gdem_ctx* c, int32_t composite, double t_start,
+                                        double t_end, const double* p) {
+  if (!c || !p || c->convex || composite < 0 || composite >= (int32_t)c->kin.comps.size())
+    return GDEM_ERR_ARG;
+  // the automatic mass divides by the composite's crust, which is zero
+  if (!(p[3] > 0.)) return GDEM_ERR_ARG;
+  return add_force(c, -1 - composite, t_start, t_end, p);
 }
 
 extern "C" int gdem_add_composite_motion(gdem_ctx* c, int32_t composite, int32_t type,

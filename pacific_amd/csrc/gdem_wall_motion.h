@@ -234,7 +234,7 @@ inline bool kin_imposed(Kin& k, std::vector<WallMotion>& motions, int32_t target
 // while it has never felt a load.  mass <= 0: chosen from the time step, the
 // amplitude and the crust (m_automass).
 struct WallForce {
-  int32_t wall = 0;
+  int32_t wall = 0;  // wall k, or composite c as -1 - c
   double t0 = 0., t1 = 0.;
   v3 amp = {0., 0., 0.}, dir = {0., 0., 0.};
   double mass = 0., vmax = 0.;
@@ -312,6 +312,7 @@ struct Composite {
   double basis[9] = {1., 0., 0., 0., 1., 0., 0., 0., 1.};
   double volume = 0.;
   Kin kin = kin_zero();
+  Conf conf;  // its ObstacleKinematicsForce
   v3 vel = {0., 0., 0.}, omega = {0., 0., 0.};
 };
 
@@ -338,6 +339,9 @@ struct WallKinematics {
   ForcePid pid;
   std::vector<Conf> wall_conf;
   const double* loads = nullptr;
+  // a composite's load is the sum its getTorsor leaves after each step
+  // (Grains.cpp:381, CompositeObstacle.cpp:650-660): none before the first
+  bool comp_loads = false;
   // Obstacle::setRestrictedGeomDirMotion (Component/src/Obstacle.cpp:613-617):
   // bit d set = no geometric translation along d (the velocity is kept)
   std::vector<uint32_t> wall_restrict;
@@ -418,13 +422,26 @@ struct WallKinematics {
     }
     for (Composite& c : comps) {
       c.kin = kin_zero();
+      c.conf = Conf{};
       c.vel = c.omega = mk(0., 0., 0.);
     }
+  }
+  // Force of composite c's torsor (CompositeObstacle::getTorsor,
+  // Component/src/CompositeObstacle.cpp:650-660): zero, then += each member's
+  // in the order they were attached, a member composite's own sum in its place
+  v3 composite_load(int32_t c) const {
+    v3 f = mk(0., 0., 0.);
+    for (int32_t mbr : comps[c].members)
+      f = add(f, mbr < 0 ? composite_load(-1 - mbr)
+                         : (loads ? mk(loads[3 * mbr], loads[3 * mbr + 1], loads[3 * mbr + 2])
+                                  : mk(0., 0., 0.)));
+    return f;
   }
   // SimpleObstacle::Move (Component/src/SimpleObstacle.cpp:116-175) of wall w:
   // translate, rotate about its centre, new bounding box; velocities of its
   // kinematics when it moved
-  void move_wall(Wall* walls, int w, double time, double dt, bool mother, bool displace) {
+  void move_wall(Wall* walls, int w, double time, double dt, bool mother, bool mother_force,
+                 bool displace) {
     Wall& wl = walls[w];
     Kin& k = wall_kin[w];
     Conf& cf = wall_conf[w];
@@ -450,7 +467,7 @@ struct WallKinematics {
       if (m.wall != w || !interval_active(m.t0, m.t1, time - dt, time, dt, sub_dt)) continue;
       force_velocity(m, pid, time, dt, sub_dt, load, wl.crust, cf.vt, cf.trans);
     }
-    const bool move_force = norm(cf.vt) != 0.;
+    const bool move_force = norm(cf.vt) != 0. || mother_force;
     if (move_force && displace) {
       const v3 tr = restricted(cf.trans, wall_restrict[w]);  // SimpleObstacle.cpp:146-149
       const double origin[3] = {wl.origin[0] + tr.x, wl.origin[1] + tr.y, wl.origin[2] + tr.z};
@@ -473,7 +490,7 @@ struct WallKinematics {
   // the composite's own motion, composed onto each member at its lever from
   // the (moved) centre, then the members move
   void move_composite(Wall* walls, int nwalls, int32_t c, double time, double dt, bool mother,
-                      bool displace) {
+                      bool mother_force, bool displace) {
     Composite& C = comps[c];
     const bool moving = kin_imposed(C.kin, motions, -1 - c, time, dt, C.centre) || mother;
     if (moving && displace) {
@@ -491,18 +508,37 @@ struct WallKinematics {
           kin_compose(wall_kin[mbr], C.kin, sub(pos, C.centre));
         }
       }
-    move_members(walls, nwalls, c, time, dt, moving, displace);
-    if (moving) {
-      C.vel = C.kin.vt;
+    // the composite's imposed forces on its torsor (CompositeObstacle.cpp:
+    // 182-203): its translation, then composed onto every member
+    // (ObstacleKinematicsForce::Compose, ObstacleKinematicsForce.cpp:51-55)
+    const v3 load = comp_loads ? composite_load(c) : mk(0., 0., 0.);
+    for (WallForce& m : forces) {
+      double sub_dt = 0.;
+      if (m.wall != -1 - c || !interval_active(m.t0, m.t1, time - dt, time, dt, sub_dt)) continue;
+      force_velocity(m, pid, time, dt, sub_dt, load, 0., C.conf.vt, C.conf.trans);
+    }
+    const bool move_force = norm(C.conf.vt) != 0. || mother_force;
+    if (move_force && displace) C.centre = add(C.centre, restricted(C.conf.trans, C.restrict_mask));
+    if (move_force)
+      for (int32_t mbr : C.members) {
+        Conf& m = mbr < 0 ? comps[-1 - mbr].conf : wall_conf[mbr];
+        m.vt = add(m.vt, C.conf.vt);
+        m.trans = add(m.trans, C.conf.trans);
+      }
+    move_members(walls, nwalls, c, time, dt, moving, move_force, displace);
+    // Obstacle::setVelocity (Obstacle.cpp:635-640): both kinematics
+    if (moving || move_force) {
+      C.vel = add(C.kin.vt, C.conf.vt);
       C.omega = C.kin.vr;
     }
   }
   void move_members(Wall* walls, int nwalls, int32_t parent, double time, double dt, bool mother,
-                    bool displace) {
+                    bool mother_force, bool displace) {
     for (int32_t k = 0; k < (int32_t)comps.size(); ++k)
-      if (comps[k].parent == parent) move_composite(walls, nwalls, k, time, dt, mother, displace);
+      if (comps[k].parent == parent)
+        move_composite(walls, nwalls, k, time, dt, mother, mother_force, displace);
     for (int w = 0; w < nwalls; ++w)
-      if (wall_parent[w] == parent) move_wall(walls, w, time, dt, mother, displace);
+      if (wall_parent[w] == parent) move_wall(walls, w, time, dt, mother, mother_force, displace);
   }
   // Obstacle part of AllComponents::Move at `time` (over [time - dt, time]);
   // returns whether any wall changed.
@@ -520,8 +556,9 @@ struct WallKinematics {
     const bool changed = anyactive || active_prev;
     if (changed) reset(walls, nwalls);
     active_prev = anyactive;
-    if (anyactive) move_members(walls, nwalls, -1, time, dt, false, true);
+    if (anyactive) move_members(walls, nwalls, -1, time, dt, false, false, true);
     drop_completed(time, dt);
+    comp_loads = true;
     return changed;
   }
   // AllComponents::setKinematicsObstacleWithoutMoving (AllComponents.cpp:235-251):
@@ -530,7 +567,7 @@ struct WallKinematics {
     if (!any_live()) return false;
     fit(nwalls);
     reset(walls, nwalls);
-    move_members(walls, nwalls, -1, time, dt, false, false);
+    move_members(walls, nwalls, -1, time, dt, false, false, false);
     drop_completed(time, dt);
     return true;
   }

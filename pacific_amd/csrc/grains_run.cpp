@@ -17,8 +17,8 @@
 // int32 material, parent; int32 ncomposite_motions, per motion int32
 // composite, type, double t_start, t_end, p[12] (composites numbered in
 // creation order, then the shells; a shell's boxes follow the walls); int32
-// nforces, per imposed force int32 wall, double t_start, t_end, ax, ay, az,
-// mass (<= 0: automatic), vmax.  GDEMIN05 is GDEMIN04 with, after the walls,
+// nforces, per imposed force int32 wall (-1 - c for composite c), double
+// t_start, t_end, ax, ay, az, mass (<= 0: automatic), vmax.  GDEMIN05 is GDEMIN04 with, after the walls,
 // int32 nrectangles, per rectangle double basis[9], centre[3], lx, ly,
 // crust, int32 material (wall indices after the boxes), and each shell
 // record ending with int32 elements (1 Box, 0 Rectangle); then x[3n], v[3n],
@@ -253,7 +253,8 @@ int main(int argc, char** argv) {
     app.LinkImposedMotion(tm.second);
   }
   for (auto& fo : forces) {
-    fo.second.obstacle = wall_names.at((size_t)fo.first);
+    fo.second.obstacle = fo.first >= 0 ? wall_names.at((size_t)fo.first)
+                                       : comp_names.at((size_t)(-1 - fo.first));
     app.LinkImposedMotion(fo.second);
   }
   try {

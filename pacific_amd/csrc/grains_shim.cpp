@@ -496,8 +496,10 @@ void Grains::readParticles2014(std::string const& path, bool binary, int id_offs
 }
 
 void Grains::LinkImposedMotion(ObstacleImposedForce const& mo) {
+  // a simple obstacle, or a composite (CompositeObstacle::LinkImposedMotion)
   const int wall = m_lc.obstacleIndex(mo.obstacle);
-  if (wall < 0) throw std::invalid_argument("no box obstacle named " + mo.obstacle);
+  const int comp = wall < 0 ? m_lc.compositeIndex(mo.obstacle) : -1;
+  if (wall < 0 && comp < 0) throw std::invalid_argument("no obstacle named " + mo.obstacle);
   if (mo.type != "ConstantTranslation")
     throw std::invalid_argument("imposed force type " + mo.type + " is not supported");
   auto get = [&](const char* k, double dflt, bool required) {
@@ -510,8 +512,9 @@ void Grains::LinkImposedMotion(ObstacleImposedForce const& mo) {
   };
   const double p[5] = {get("AX", 0., true), get("AY", 0., true), get("AZ", 0., true),
                        get("Mass", 0., false), get("Vmax", 0., true)};
-  if (gdem_add_wall_force(m_lc.context(), wall, mo.tstart, mo.tend, p))
-    throw std::invalid_argument("imposed force on " + mo.obstacle + " rejected");
+  const int rc = wall >= 0 ? gdem_add_wall_force(m_lc.context(), wall, mo.tstart, mo.tend, p)
+                           : gdem_add_composite_force(m_lc.context(), comp, mo.tstart, mo.tend, p);
+  if (rc) throw std::invalid_argument("imposed force on " + mo.obstacle + " rejected");
 }
 
 void Grains::trackObstaclesLoad(bool on) { check(gdem_set_wall_loads(m_lc.context(), on ? 1 : 0)); }

@@ -113,6 +113,7 @@ def load_library(path: str = LIB_PATH):
         "gdem_set_wall_loads": (C.c_int, [vp, i32]),
         "gdem_set_motion_restriction": (C.c_int, [vp, i32, i32, i32]),
         "gdem_add_wall_force": (C.c_int, [vp, i32, d, d, vp]),
+        "gdem_add_composite_force": (C.c_int, [vp, i32, d, d, vp]),
         "gdem_get_wall_loads": (C.c_int, [vp, vp, vp]),
         "gdem_add_composite": (C.c_int, [vp, i32, P(i32)]),
         "gdem_attach_wall": (C.c_int, [vp, i32, i32]),
@@ -188,6 +189,7 @@ EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall"
             "gdem_attach_wall", "gdem_add_composite_motion", "gdem_get_composite",
             "gdem_add_cylindrical_shell", "gdem_cylindrical_shell_boxes", "gdem_set_time",
             "gdem_add_rectangle_wall", "gdem_add_cylindrical_shell_rectangles",
+            "gdem_add_composite_force",
             "gdem_cylindrical_shell_rectangles",
             "gdem_get_time", "gdem_move_walls", "gdem_get_wall", "gdem_write_particles_2014",
             "gdem_read_particles_2014", "gdem_set_contact_recording",
@@ -331,6 +333,12 @@ class GdemSystem:
         """Imposed force on a wall: params = (ax, ay, az, mass (<= 0: automatic), vmax)."""
         self._check(self.lib.gdem_add_wall_force(self.h, int(wall), float(t_start), float(t_end),
                                                  _ptr(_f64(params, (5,)))), "gdem_add_wall_force")
+
+    def add_composite_force(self, composite, t_start, t_end, params):
+        """Imposed force on a composite's load: params = (ax, ay, az, mass > 0, vmax)."""
+        self._check(self.lib.gdem_add_composite_force(self.h, int(composite), float(t_start),
+                                                      float(t_end), _ptr(_f64(params, (5,)))),
+                    "gdem_add_composite_force")
 
     def set_motion_restriction(self, mask, wall=-1, composite=-1):
         """No geometric translation along the directions of mask (bit 0 x, 1 y,
@@ -603,6 +611,8 @@ def add_obstacle_tree(s, wl):
         s.add_composite_motion(comp, kind, t0, t1, prm)
     for (wall, t0, t1, prm) in wl.get("wall_forces", []):
         s.add_wall_force(wall, t0, t1, prm)
+    for (comp, t0, t1, prm) in wl.get("composite_forces", []):
+        s.add_composite_force(comp, t0, t1, prm)
 
 
 def cylindrical_shell_boxes(basis, centre, height, radius, width, n):

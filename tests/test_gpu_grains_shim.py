@@ -25,7 +25,7 @@ def write_input(path, wl, nsteps):
     motions = wl.get("motions", [])
     rect = bool(wl.get("rectangles") or wl.get("rect_shells"))
     tree = rect or bool(wl.get("composites") or wl.get("shells") or wl.get("composite_motions")
-                        or wl.get("wall_forces"))
+                        or wl.get("wall_forces") or wl.get("composite_forces"))
     assert not wl.get("attach"), "members are given with their composite"
     with open(path, "wb") as f:
         f.write(b"GDEMIN05" if rect else b"GDEMIN04" if tree else b"GDEMIN03" if motions else
@@ -77,7 +77,8 @@ def write_input(path, wl, nsteps):
             for (comp, kind, t0, t1, prm) in cm:
                 f.write(struct.pack("<2i2d", comp, kind, t0, t1))
                 f.write(struct.pack("<12d", *(list(prm) + [0.0] * (12 - len(prm)))))
-            wf = wl.get("wall_forces", [])
+            wf = wl.get("wall_forces", []) + \
+                [(-1 - c, t0, t1, prm) for (c, t0, t1, prm) in wl.get("composite_forces", [])]
             f.write(struct.pack("<i", len(wf)))
             for (wall, t0, t1, prm) in wf:
                 f.write(struct.pack("<i7d", wall, t0, t1, *prm))
@@ -220,11 +221,16 @@ def test_driver_rectangles_match_oracle(oracle_mod, name):
         assert rel_err(g[k], o[k]) <= bar, k
 
 
-def test_driver_piston_box_matches_oracle(oracle_mod):
+@pytest.mark.parametrize("lid", ["wall", "composite"])
+def test_driver_piston_box_matches_oracle(oracle_mod, lid):
     """ObstacleImposedForce through Grains::LinkImposedMotion: a lid pressing
-    a box of spheres."""
-    print("[case] grains_run piston box 150 steps", flush=True)
-    wl = workloads.piston_box(1000, seed=1)
+    a box of spheres (one wall, or two half lids in a composite)."""
+    print(f"[case] grains_run piston box ({lid} lid) 150 steps", flush=True)
+    if lid == "wall":
+        wl = workloads.piston_box(1000, seed=1)
+    else:
+        from test_gpu_imposed_force import split_lid_box
+        wl = split_lid_box(False)
     status, _, _, g = run(wl, 150)
     assert status == 0
     orc = oracle_mod.Oracle(wl)

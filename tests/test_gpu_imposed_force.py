@@ -10,6 +10,11 @@ oracle, through the C ABI.
   state within the parity bar (max(1e-9, 10 x the oracle's own one-ulp
   divergence); the loads the controller reads are sums in another order, so
   the lid path agrees to rounding, not bit for bit).
+* The same with the force on a composite lid (two half plates, or the box
+  lid split in two): the composite reads the sum of its plates' loads as
+  the step before left it, its translation moves both plates; walls equal to
+  the oracle's bit for bit during the approach and within 1e-9 after, the
+  composite's load settled on the imposed force.
 * The arguments are checked, and load tracking cannot be switched off while
   an imposed force needs it."""
 import math
@@ -19,6 +24,7 @@ import pytest
 
 from gdem_testutil import pair_set, rel_err
 from pacific_amd import gdem, workloads
+from test_oracle_composite_force_pins import composite_piston
 from test_oracle_imposed_force_pins import DT, piston, weight
 
 pytestmark = pytest.mark.gpu
@@ -88,6 +94,84 @@ def test_piston_box_matches_oracle(oracle_mod, memory):
     s.close()
 
 
+def test_composite_piston_on_sphere_gpu(oracle_mod):
+    print("[case] composite piston on a sphere", flush=True)
+    mg = weight()
+    A = 10.0 * mg
+    wl = composite_piston(A, 2.5e-5)
+    o = oracle_mod.Oracle(wl)
+    o.prime()
+    s = gdem.build_system(wl)
+    for _ in range(3):          # 300 steps of approach: no load yet
+        o.step(100, DT)
+        s.step(100, DT)
+        for w in (1, 2):
+            for a, b in zip(s.get_wall(w), o.wall(w)):
+                np.testing.assert_array_equal(a, b)
+        for a, b in zip(s.get_composite(0), o.composite(0)):
+            np.testing.assert_array_equal(a, b)
+    s.step(2100, DT)
+    f, _ = s.get_wall_loads()
+    print(f"[case] lid load {(f[1][2] + f[2][2]) / A:.12f} A", flush=True)
+    assert f[1][2] + f[2][2] == pytest.approx(A, rel=1e-8)
+    assert f[0][2] == pytest.approx(-(A + mg), rel=1e-8)
+    assert s.get_wall(1)[0][2] == s.get_wall(2)[0][2]
+    s.close()
+
+
+def split_lid_box(memory):
+    wl = workloads.piston_box(1000, seed=1, memory=memory)
+    basis, centre, half, crust, mat = wl["walls"][5]
+    hx = half[0] / 2.0
+    wl["walls"][5] = (basis, (centre[0] - hx, centre[1], centre[2]), (hx, half[1], half[2]), crust,
+                      mat)
+    wl["walls"].append((basis, (centre[0] + hx, centre[1], centre[2]), (hx, half[1], half[2]),
+                        crust, mat))
+    wl["wall_forces"] = []
+    wl["composites"] = [(-1, [5, 6])]
+    wl["composite_forces"] = [(0, 0.0, 1.0, (0.0, 0.0, 0.05, 2.5e-5, 0.2))]
+    return wl
+
+
+@pytest.mark.parametrize("memory", [False, True])
+def test_composite_piston_box_matches_oracle(oracle_mod, memory):
+    print(f"[case] split-lid piston box, memory {memory}, 300 steps", flush=True)
+    wl = split_lid_box(memory)
+    n = len(wl["particles"]["radius"])
+    dt = wl["dt"]
+    orc = oracle_mod.Oracle(wl, log_capacity=10 * n)
+    orc.prime()
+    wl_u = split_lid_box(memory)
+    v = wl_u["particles"]["v"]
+    up = np.random.default_rng(7).random(v.shape) < 0.5
+    wl_u["particles"]["v"] = np.nextafter(v, np.where(up, np.inf, -np.inf))
+    orc_u = oracle_mod.Oracle(wl_u)
+    orc_u.prime()
+    s = gdem.build_system(wl)
+    s.set_contact_recording(32)
+    x0 = wl["particles"]["x"]
+    lid_contacts = 0
+    for calls in range(6):
+        orc.step(50, dt)
+        orc_u.step(50, dt)
+        s.step(50, dt)
+        pairs = pair_set(s.get_contacts())
+        assert pairs == pair_set(orc.contacts()), calls
+        lid_contacts += sum(1 for (_, b) in pairs if b in (-6, -7))
+        for w in (5, 6):
+            np.testing.assert_allclose(s.get_wall(w)[0], orc.wall(w)[0], rtol=1e-9)
+        assert s.get_wall(5)[0][2] == s.get_wall(6)[0][2]
+        o = orc.state()
+        self_div = _errors(orc_u.state(), o, x0)
+        bars = [max(1e-9, 10.0 * e) for e in self_div]
+        errs = _errors(s.get_state(), o, x0)
+        print(f"[case] steps {50 * (calls + 1)}: lid z {s.get_wall(5)[0][2]:.9f}, x/v/w error "
+              f"{errs}", flush=True)
+        assert all(e <= b for e, b in zip(errs, bars)), (calls, errs, bars)
+    assert lid_contacts >= 1
+    s.close()
+
+
 def test_imposed_force_arguments():
     print("[case] imposed force arguments", flush=True)
     wl = workloads.plain_box(100, seed=3)
@@ -101,4 +185,11 @@ def test_imposed_force_arguments():
     s.add_wall_force(5, 0.0, 1.0, (0.0, 0.0, 1.0, 0.0, 0.1))
     with pytest.raises(gdem.GdemError):
         s.set_wall_loads(False)
+    with pytest.raises(gdem.GdemError):
+        s.add_composite_force(0, 0.0, 1.0, (0.0, 0.0, 1.0, 1.0e-5, 0.1))   # no composite yet
+    k = s.add_composite()
+    s.attach_wall(4, k)
+    with pytest.raises(gdem.GdemError):
+        s.add_composite_force(k, 0.0, 1.0, (0.0, 0.0, 1.0, 0.0, 0.1))      # automatic mass
+    s.add_composite_force(k, 0.0, 1.0, (0.0, 0.0, 1.0, 1.0e-5, 0.1))
     s.close()
