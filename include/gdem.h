@@ -152,8 +152,9 @@ int gdem_add_wall(gdem_ctx* ctx, const double* basis, const double* centre,
    local y, no thickness, normal along the local z).  Spheres meet it through
    ClosestPointRECTANGLE (Geometry/src/RigidBodyWithCrust.cpp:1107-1226): a
    contact only where the sphere's deepest point lies beyond the plane and its
-   projection falls strictly inside the rectangle; no overlap limit.  Sphere
-   particles only (GDEM_ERR_ARG once convex particles are set). */
+   projection falls strictly inside the rectangle; no overlap limit.  Convex
+   particles meet it the same way through their full shape's support point
+   (the reference's branch for any convex first body). */
 int gdem_add_rectangle_wall(gdem_ctx* ctx, const double* basis, const double* centre,
                             double lx, double ly, double crust, int32_t material);
 
@@ -226,7 +227,7 @@ int gdem_add_cylindrical_shell(gdem_ctx* ctx, const double* basis, const double*
 /* CylindricalShell with its default Rectangle elements (the same file
    without ElementaryObstacle="Box"): rectangle i of sides (height, ly),
    centred at the inner radius, its local x along the axis and its normal
-   towards it.  Sphere particles only. */
+   towards it. */
 int gdem_add_cylindrical_shell_rectangles(gdem_ctx* ctx, const double* basis,
                                           const double* centre, double height, double radius,
                                           double width, int32_t n, double crust,

@@ -203,6 +203,8 @@ int orc_c_set_particles(orc_csys* s, int n, const double* x, const double* v,
                         const double* w, const double* q, const double* basis,
                         const int32_t* shape, const double* density,
                         const double* crust, const int32_t* material);
+int orc_c_add_rectangle_wall(orc_csys* s, const double* basis, const double* centre, double lx,
+                             double ly, double crust, int material);
 int orc_c_add_wall(orc_csys* s, const double* basis, const double* centre,
                    const double* half_extent, double crust, int material);
 int orc_c_set_contact_model(orc_csys* s, int mat_a, int mat_b, int kind,

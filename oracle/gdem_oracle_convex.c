@@ -48,6 +48,7 @@ typedef struct {
   double bb_centre[3], bb_ext[3];
   double crust, req, circ, scale[3];
   int material;
+  int rect;  /* a Rectangle (ext = LX/2, LY/2, 0) */
 } cwall_t;
 
 struct orc_csys {
@@ -580,6 +581,23 @@ int orc_c_add_wall(orc_csys* s, const double* basis, const double* centre,
   return ORC_OK;
 }
 
+/* Rectangle obstacle (Rectangle.cpp:13-21): the box's bounding box with a
+   zero third extent; equivalent radius of the area LX LY (getVolume,
+   :182-185), circumscribed radius sqrt(LX^2 + LY^2) / 2 (:118-121) */
+int orc_c_add_rectangle_wall(orc_csys* s, const double* basis, const double* centre, double lx,
+                             double ly, double crust, int material) {
+  if (!(lx > 0.) || !(ly > 0.)) return ORC_ERR_ARG;
+  double half[3] = {lx / 2., ly / 2., 0.};
+  int rc = orc_c_add_wall(s, basis, centre, half, crust, material);
+  if (rc) return rc;
+  cwall_t* wl = &s->walls[s->nwalls - 1];
+  wl->rect = 1;
+  double a = 2. * half[0], b = 2. * half[1];
+  wl->req = pow((0.75 / ORC_PI) * (a * b), 1. / 3.);
+  wl->circ = sqrt(a * a + b * b) / 2.;
+  return ORC_OK;
+}
+
 int orc_c_set_contact_model(orc_csys* s, int a, int b, int kind, const double* p) {
   if (a < 0 || b < 0 || a >= ORC_MAX_MATERIALS || b >= ORC_MAX_MATERIALS) return ORC_ERR_ARG;
   cmodel_t m;
@@ -720,6 +738,45 @@ static body_t particle_body(const orc_csys* s, int p, v3 shift) {
   return b;
 }
 
+/* ClosestPointRECTANGLE, particle first (RigidBodyWithCrust.cpp:1107-1226,
+   branch :1172-1218), behind ClosestPoint's circumscribed test (:282-296):
+   the particle's full shape through its transform (basis, centre) */
+static int crect(const body_t* A, const cwall_t* wl, v3* point, v3* ov, double* dist) {
+  v3 rp = ld(wl->origin), xa = ld(A->origin);
+  if (!(norm(sub(xa, rp)) < A->circ + wl->circ)) return 0;
+  v3 rn = mk(dot(mk(wl->basis[0], wl->basis[1], wl->basis[2]), mk(0., 0., 1.)),
+             dot(mk(wl->basis[3], wl->basis[4], wl->basis[5]), mk(0., 0., 1.)),
+             dot(mk(wl->basis[6], wl->basis[7], wl->basis[8]), mk(0., 0., 1.)));
+  double sg = copysign(1., dot(rn, sub(xa, rp)));
+  rn = mk(sg * rn.x, sg * rn.y, sg * rn.z);
+  v3 d = neg(rn);
+  const double* B = A->basis;
+  v3 dl = mk(B[0] * d.x + B[3] * d.y + B[6] * d.z, B[1] * d.x + B[4] * d.y + B[7] * d.z,
+             B[2] * d.x + B[5] * d.y + B[8] * d.z);
+  v3 sp = support(A->sh, dl);
+  /* Transform::operator() (Transform.cpp:312-317) */
+  v3 pa = mk(dot(mk(B[0], B[1], B[2]), sp) + A->origin[0], dot(mk(B[3], B[4], B[5]), sp) + A->origin[1],
+             dot(mk(B[6], B[7], B[8]), sp) + A->origin[2]);
+  if (!(dot(rn, sub(pa, rp)) < 0.)) return 0;
+  double k = dot(sub(rp, pa), rn);
+  v3 pb = add(mk(k * rn.x, k * rn.y, k * rn.z), pa);
+  /* Rectangle::isIn (Rectangle.cpp:365-370) after the inverse transform
+     (Transform::setToInverseTransform, Transform.cpp:71-77) */
+  double io[3];
+  for (int i = 0; i < 3; ++i) {
+    v3 col = mk(wl->basis[i], wl->basis[3 + i], wl->basis[6 + i]);
+    io[i] = dot(neg(col), rp);
+  }
+  v3 l = mk(dot(mk(wl->basis[0], wl->basis[3], wl->basis[6]), pb) + io[0],
+            dot(mk(wl->basis[1], wl->basis[4], wl->basis[7]), pb) + io[1],
+            dot(mk(wl->basis[2], wl->basis[5], wl->basis[8]), pb) + io[2]);
+  if (!(fabs(l.z) < ORC_EPSILON && fabs(l.y) < wl->ext[1] && fabs(l.x) < wl->ext[0])) return 0;
+  *point = add(mk(pa.x / 2.0, pa.y / 2.0, pa.z / 2.0), mk(pb.x / 2.0, pb.y / 2.0, pb.z / 2.0));
+  *ov = sub(pb, pa);
+  *dist = -norm(*ov);
+  return 1;
+}
+
 /* Particle::InterAction (Particle.cpp:478-546) for two convex particles */
 static int cinteract(orc_csys* s, int p0, int p1, v3 shift) {
   body_t A = particle_body(s, p0, mk(0., 0., 0.)), B = particle_body(s, p1, shift);
@@ -789,7 +846,7 @@ static int cinteract_wall(orc_csys* s, int p, int wi) {
   B.circ = wl->circ;
   v3 c, ov;
   double dist;
-  int rc = closest_point(&A, &B, &c, &ov, &dist, &s->ngjk);
+  int rc = wl->rect ? crect(&A, wl, &c, &ov, &dist) : closest_point(&A, &B, &c, &ov, &dist, &s->ngjk);
   if (rc < 0) return cset_error(s, ORC_ERR_CONTACT, p, -1 - wi);
   if (rc == 0 || !(dist < 0.)) return ORC_OK;
   const cmodel_t* m = &s->models[s->mat[p]][wl->material];
@@ -823,7 +880,7 @@ int orc_c_wall_pair(orc_csys* s, int p, int wi, double* point, double* ovec, dou
   v3 c, ov;
   double dist;
   int64_t ng = 0;
-  int rc = closest_point(&A, &B, &c, &ov, &dist, &ng);
+  int rc = wl->rect ? crect(&A, wl, &c, &ov, &dist) : closest_point(&A, &B, &c, &ov, &dist, &ng);
   if (rc <= 0) return rc;
   st(point, c);
   st(ovec, ov);

@@ -369,6 +369,7 @@ def _load_convex(lib):
         "orc_c_shape_info": (i32, [vp, i32, vp, vp, vp, vp]),
         "orc_c_set_particles": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "orc_c_add_wall": (i32, [vp, vp, vp, vp, d, i32]),
+        "orc_c_add_rectangle_wall": (i32, [vp, vp, vp, d, d, d, i32]),
         "orc_c_set_contact_model": (i32, [vp, i32, i32, i32, vp]),
         "orc_c_prime": (i32, [vp]),
         "orc_c_step": (i32, [vp, i32, d]),
@@ -414,6 +415,10 @@ class ConvexOracle:
         for (basis, centre, half, crust, mat) in wl["walls"]:
             self._chk(self.lib.orc_c_add_wall(h, _p(_f(basis, (9,))), _p(_f(centre, (3,))),
                                               _p(_f(half, (3,))), crust, mat), "wall")
+        for (basis, centre, lx, ly, crust, mat) in wl.get("rectangles", []):
+            self._chk(self.lib.orc_c_add_rectangle_wall(h, _p(_f(basis, (9,))), _p(_f(centre, (3,))),
+                                                        float(lx), float(ly), float(crust), int(mat)),
+                      "rectangle")
         p = wl["particles"]
         n = len(p["shape"])
         self.n = n

@@ -595,8 +595,6 @@ extern "C" int gdem_set_convex_particles(gdem_ctx* c, int64_t n, const double* x
   if (n > 0 && (!x || !v || !w || !q || !basis || !shape || !density || !crust))
     return GDEM_ERR_ARG;
   if (c->has_memory) return GDEM_ERR_ARG;  // memory models: spheres only
-  for (int32_t k = 0; k < c->nwalls; ++k)
-    if (c->h_walls[k].kind != WALL_BOX) return GDEM_ERR_ARG;  // rectangles: spheres only
   for (int64_t i = 0; i < n; ++i) {
     if (shape[i] < 0 || shape[i] >= c->nshapes) return GDEM_ERR_ARG;
     if (!(density[i] > 0.) || !(crust[i] > 0.)) return GDEM_ERR_ARG;
@@ -711,7 +709,6 @@ extern "C" int gdem_add_wall(gdem_ctx* c, const double* basis, const double* cen
 extern "C" int gdem_add_rectangle_wall(gdem_ctx* c, const double* basis, const double* centre,
                                        double lx, double ly, double crust, int32_t material) {
   if (!c || !basis || !centre || !(lx > 0.) || !(ly > 0.)) return GDEM_ERR_ARG;
-  if (c->convex) return GDEM_ERR_ARG;  // sphere particles only
   const double half[3] = {lx / 2., ly / 2., 0.};
   return add_wall_kind(c, basis, centre, half, crust, material, WALL_RECTANGLE);
 }
@@ -1511,7 +1508,7 @@ extern "C" int gdem_add_cylindrical_shell_rectangles(gdem_ctx* c, const double* 
                                                      double crust, int32_t material,
                                                      int32_t parent, int32_t* composite) {
   if (!c || !basis || !centre || !composite || n < 3 || !(height > 0.) || !(radius > 0.) ||
-      !(width > 0.) || c->nwalls + n > kMaxWalls || c->convex)
+      !(width > 0.) || c->nwalls + n > kMaxWalls)
     return GDEM_ERR_ARG;
   std::vector<double> wb(9 * (size_t)n), wc(3 * (size_t)n), wh(3 * (size_t)n);
   cylindrical_shell_rectangles(basis, centre, height, radius, width, n, wb.data(), wc.data(),

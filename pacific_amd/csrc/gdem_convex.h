@@ -319,6 +319,34 @@ __host__ __device__ inline int closest_point(const CBody& A, const CBody& B, v3&
   return 1;
 }
 
+// ClosestPointRECTANGLE with a convex particle first (RigidBodyWithCrust.cpp:
+// 1107-1226, branch :1172-1218), after ClosestPoint's circumscribed test
+// (:282-296): the particle's full shape (not crust-shrunk) through its own
+// transform, support of (-n) * basis (Matrix::tdot, Matrix.cpp:238-241),
+// then as for a sphere: beyond the plane, projection strictly inside.
+__host__ __device__ inline int convex_rect(const CBody& A, const Wall& wl, v3& contact, v3& ov,
+                                           double& dist) {
+  const v3 rp = mk(wl.origin[0], wl.origin[1], wl.origin[2]);
+  if (!(norm(sub(A.origin, rp)) < A.circ + wl.circ)) return 0;
+  v3 n = mk(wl.basis[2], wl.basis[5], wl.basis[8]);
+  n = scl(n, copysign(1., dot(n, sub(A.origin, rp))));
+  const v3 d = neg(n);
+  const double* b = A.basis;
+  const v3 dl = mk(b[0] * d.x + b[3] * d.y + b[6] * d.z, b[1] * d.x + b[4] * d.y + b[7] * d.z,
+                   b[2] * d.x + b[5] * d.y + b[8] * d.z);
+  const v3 sp = support(*A.sh, dl);
+  const double ao[3] = {A.origin.x, A.origin.y, A.origin.z};
+  const v3 pa = xform(A.basis, ao, sp);
+  if (!(dot(n, sub(pa, rp)) < 0.)) return 0;
+  const v3 pb = add(scl(n, dot(sub(rp, pa), n)), pa);
+  const v3 l = xform(wl.inv_basis, wl.inv_origin, pb);
+  if (!(fabs(l.z) < kEps && fabs(l.y) < wl.ext[1] && fabs(l.x) < wl.ext[0])) return 0;
+  contact = add(dvd(pa, 2.0), dvd(pb, 2.0));
+  ov = sub(pb, pa);
+  dist = -norm(ov);
+  return 1;
+}
+
 // ParticleKinematics3D::computeAngularAccelerationBodyFixed
 // (Component/src/ParticleKinematics3D.cpp:56-77), predictor mode
 __host__ __device__ inline v3 ang_acc3(v3 tq, v3 om, double I0, double I1, double I2) {

@@ -175,7 +175,8 @@ __global__ void __launch_bounds__(kCBlock) k_cgjk(CGjkArgs a) {
                     wscale, wl.crust, wsc.w};
       v3 c = mk(0., 0., 0.), ov = mk(0., 0., 0.);
       double dist = 0.;
-      const int rc = closest_point(A, B, c, ov, dist);
+      const int rc = wl.kind == WALL_RECTANGLE ? convex_rect(A, wl, c, ov, dist)
+                                               : closest_point(A, B, c, ov, dist);
       if (rc < 0) {
         atomicMin(&a.err[kErrContact], pack_ids(ax1.id, -1 - u));
         continue;

@@ -47,7 +47,25 @@ def push_walls_in(wl, depth=5e-5):
     return wl
 
 
+def walls_to_rectangles(wl):
+    """Each (pushed-in) box wall replaced by a Rectangle on its inner face,
+    of the same extent; the particles meet them through the reference's
+    ClosestPointRECTANGLE (a particle's deepest support point, projected)."""
+    rects = []
+    for k, (basis, centre, half, crust, mat) in enumerate(wl["walls"]):
+        d, side = k // 2, (-1 if k % 2 == 0 else 1)
+        a, b = (d + 1) % 3, (d + 2) % 3
+        c = list(centre)
+        c[d] += half[d] if side < 0 else -half[d]
+        rects.append((workloads.FACE_BASIS[d], tuple(c), 2.0 * half[a], 2.0 * half[b], crust, mat))
+    wl["walls"] = []
+    wl["rectangles"] = rects
+    return wl
+
+
 def case(name):
+    if name == "convex_rectangles":
+        return walls_to_rectangles(case("convex_walls"))
     if name == "convex_walls":
         return push_walls_in(workloads.convex_box(600, seed=5, phi=0.15, height=0.015,
                                                   overlap=4e-5, sigma_v=0.2))
@@ -57,7 +75,7 @@ def case(name):
     raise KeyError(name)
 
 
-CASES = ["convex_walls", "convex_dense"]
+CASES = ["convex_walls", "convex_dense", "convex_rectangles"]
 
 
 def _masses(orc, wl):

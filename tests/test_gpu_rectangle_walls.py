@@ -18,8 +18,8 @@ gdem_add_cylindrical_shell_rectangles) against the oracle, through the C ABI.
   agrees to rounding.
 * Wall loads of the rectangle box agree with the oracle's to 1e-11 of the
   wall-contact force scale.
-* Rectangles are refused with convex particles, and sides must be
-  positive."""
+* Sides must be positive.  (Convex particles against rectangles:
+  tests/test_gpu_convex_parity.py.)"""
 import math
 
 import numpy as np
@@ -226,12 +226,3 @@ def test_rectangle_arguments():
         s.add_rectangle_wall(workloads.IDENTITY, (0.0, 0.0, 0.0), 0.01, -1.0, workloads.CRUST, 1)
     s.add_rectangle_wall(workloads.IDENTITY, (0.0, 0.0, 0.0), 0.01, 0.01, workloads.CRUST, 1)
     s.close()
-    # convex particles: no rectangles, before or after
-    cw = workloads.convex_box(600, seed=5, phi=0.15, height=0.015, steps=10)
-    s = gdem.build_system(cw)
-    with pytest.raises(gdem.GdemError):
-        s.add_rectangle_wall(workloads.IDENTITY, (0.0, 0.0, 0.0), 0.01, 0.01, workloads.CRUST, 1)
-    s.close()
-    cw["rectangles"] = [(workloads.IDENTITY, (0.0, 0.0, 0.0), 0.01, 0.01, workloads.CRUST, 1)]
-    with pytest.raises(gdem.GdemError):
-        gdem.build_system(cw)

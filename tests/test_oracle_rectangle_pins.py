@@ -160,3 +160,35 @@ def test_momentum_balance_with_rectangles(oracle_mod):
         assert np.abs(lin).max() <= 1e-13 * sf
         assert np.abs(ang).max() <= 1e-13 * stq
     assert len(centres) == 6
+
+
+def test_cube_on_rectangle_touches_at_one_corner(oracle_mod):
+    """A convex particle meets a rectangle at its support point alone: a cube
+    lying flat with its lower face 2e-6 below the plane has one contact, at
+    the corner the box support picks for (0, 0, -1) (Box::support,
+    Box.cpp:307-316: +e for a zero component), midway between that corner
+    and its projection."""
+    e = workloads.CUBE_HALF[0]
+    depth = 2.0e-6
+    x = np.array([[0.001, -0.002, e - depth]])
+    parts = {"x": x, "v": np.zeros((1, 3)), "w": np.zeros((1, 3)),
+             "q": np.array([[0.0, 0.0, 0.0, 1.0]]), "basis": np.array([workloads.IDENTITY]),
+             "shape": np.zeros(1, np.int32), "density": np.full(1, workloads.DENSITY),
+             "crust": np.full(1, workloads.CRUST), "material": np.zeros(1, np.int32)}
+    wl = {"origin": (-0.02, -0.02, -0.005), "size": (0.04, 0.04, 0.02), "periodic": (0, 0, 0),
+          "cell_size": 4.0 * e, "gravity": (0.0, 0.0, 0.0), "dt": 1.0e-6,
+          "models": [(0, 0, MODEL_HOOKE, HOOKE), (0, 1, MODEL_HOOKE, HOOKE)],
+          "walls": [], "shapes": [("box", workloads.CUBE_HALF)], "particles": parts,
+          "rectangles": [(workloads.IDENTITY, (0.0, 0.0, 0.0), 0.02, 0.02, workloads.CRUST, 1)]}
+    o = oracle_mod.ConvexOracle(wl, log_capacity=16)
+    o.prime()
+    c = o.contacts()
+    assert len(c["id0"]) == 1 and c["id1"][0] == -1
+    corner = x[0] + np.array([e, e, -e])
+    np.testing.assert_allclose(c["point"][0], [corner[0], corner[1], -depth / 2], rtol=0,
+                               atol=1e-18)
+    np.testing.assert_allclose(c["ovec"][0], [0.0, 0.0, depth], rtol=1e-9, atol=1e-21)
+    # the single off-centre contact turns the cube: torque (corner - x) x F
+    F, T = o.state()["F"][0], o.state()["T"][0]
+    np.testing.assert_allclose(T, np.cross(c["point"][0] - x[0], F), rtol=1e-12, atol=1e-20)
+    assert np.abs(T[:2]).max() > 0.0
