@@ -248,7 +248,11 @@ int gdem_cylindrical_shell_boxes(const double* basis, const double* centre, doub
    torque about the wall's centre, rolling torques included; Obstacle::addForce /
    addTorque, Component/src/Obstacle.cpp:575-597), in a fixed order without
    atomics.  gdem_get_wall_loads returns those of the last evaluation,
-   3 * nwalls values each (either pointer may be NULL).  Sphere contexts. */
+   3 * nwalls values each (either pointer may be NULL).  Sphere contexts.
+   Switching on what is already on keeps the loads.  Switched on after
+   gdem_prime and before the first step, the priming evaluation is run again
+   to form its loads; later, with a memory contact model, or after state was
+   set with F and T, that is refused (GDEM_ERR_ARG): switch loads on first. */
 int gdem_set_wall_loads(gdem_ctx* ctx, int32_t on);
 int gdem_get_wall_loads(gdem_ctx* ctx, double* force, double* torque);
 

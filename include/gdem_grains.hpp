@@ -191,7 +191,7 @@ struct BoxObstacle {
 
 // A static SimpleObstacle with a Rectangle convex (Geometry/include/
 // Rectangle.hh: sides LX along the local x, LY along the local y, no
-// thickness, normal along the local z).  Sphere particles only.
+// thickness, normal along the local z).  Sphere and convex particles.
 struct RectangleObstacle {
   std::string name;
   double centre[3];

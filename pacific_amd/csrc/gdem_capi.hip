@@ -57,6 +57,7 @@ struct gdem_ctx {
   size_t stage_len = 0;
   // counters / timing
   int64_t force_evals = 0;
+  int64_t prime_evals = -1;  // force_evals right after the priming evaluation
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
   size_t ev_used = 0;
@@ -1068,6 +1069,7 @@ extern "C" int gdem_prime(gdem_ctx* c) {
   rc = do_force(c, MODE_FORCE, 0., 0., true);
   if (rc) return rc;
   c->primed = true;
+  c->prime_evals = c->force_evals;
   return GDEM_OK;
 }
 
@@ -1277,6 +1279,7 @@ extern "C" int gdem_set_state(gdem_ctx* c, const double* x, const double* v, con
   // given F and T are the leap-frog's a0: the next step starts from them
   if (F) c->primed = true;
   else if (x || v || w || q) c->primed = false;
+  c->prime_evals = -1;  // given F, T: no priming evaluation to re-run for loads
   HIP_TRY(hipStreamSynchronize(c->stream));
   return collect_errors(c);
 }
@@ -1577,10 +1580,23 @@ extern "C" int gdem_get_composite(gdem_ctx* c, int32_t composite, double* centre
 extern "C" int gdem_set_wall_loads(gdem_ctx* c, int32_t on) {
   if (!c || c->convex) return GDEM_ERR_ARG;
   if (!on && !c->kin.forces.empty()) return GDEM_ERR_ARG;  // the controllers read the loads
+  if ((on != 0) == c->track_loads) return GDEM_OK;  // already in that state: loads kept
+  // Loads switched on after the priming evaluation: that evaluation's loads
+  // (the ones the reference holds at this point) are formed again by
+  // re-running it on the unchanged state.  This is exact only before the
+  // first step and without contact memory (an evaluation advances the
+  // tangential springs); otherwise the caller must switch loads on first.
+  const bool redo = on && c->primed;
+  if (redo && (c->has_memory || c->force_evals != c->prime_evals)) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->track_loads = on != 0;
-  return ensure_wall_loads(c);
+  const int rc = ensure_wall_loads(c);
+  if (rc || !redo) return rc;
+  const int rf = do_force(c, MODE_FORCE, 0., 0., true);
+  if (rf) return rf;
+  c->prime_evals = c->force_evals;
+  return GDEM_OK;
 }
 
 extern "C" int gdem_get_wall_loads(gdem_ctx* c, double* force, double* torque) {

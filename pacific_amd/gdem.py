@@ -380,7 +380,7 @@ class GdemSystem:
 
     def add_rectangle_wall(self, basis, centre, lx, ly, crust, material):
         """Rectangle obstacle of sides (lx, ly) in its local xy plane (normal
-        local z); spheres only.  Its wall index is the next one."""
+        local z); sphere and convex particles.  Its wall index is the next one."""
         self._check(self.lib.gdem_add_rectangle_wall(
             self.h, _ptr(_f64(basis, (9,))), _ptr(_f64(centre, (3,))), float(lx), float(ly),
             float(crust), int(material)), "gdem_add_rectangle_wall")
