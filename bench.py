@@ -99,17 +99,38 @@ def read_traffic():
     return None
 
 
-def cpu_baseline(wl, steps, warm=2):
-    """The oracle (restatement of the reference CPU step), one thread."""
+def cpu_baseline(wl, steps, warm=2, threads=1):
+    """The oracle (restatement of the reference CPU step): one thread, or the
+    all-core variant (slabs of cell rows in two colours, OpenMP)."""
     from oracle import py_oracle
     o = py_oracle.ConvexOracle(wl) if "shapes" in wl else py_oracle.Oracle(wl)
     o.prime()
-    o.step(warm, wl["dt"])
+    step = o.step if threads == 1 else (lambda k, dt: o.step_parallel(k, dt, threads))
+    step(warm, wl["dt"])
     t = time.perf_counter()
-    o.step(steps, wl["dt"])
+    step(steps, wl["dt"])
     el = time.perf_counter() - t
     n = len(wl["particles"]["x"])
     return n * steps / el, el
+
+
+def host_cores():
+    """Threads this process may use (the box's CPU share), and the CPU model."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        n = min(n, int(os.environ["OMP_NUM_THREADS"]))
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return max(n, 1), model
 
 
 def run_batch(args):
@@ -352,32 +373,38 @@ def main():
 
     # timed region: K steps, no instrumentation
     sysg.reset_counters()
+    rebuilds0 = sysg.get_skin()[1] if not convex else 0
     sysg.synchronize()
     t0 = time.perf_counter()
     sysg.step(args.steps, dt)
     sysg.synchronize()
     el = time.perf_counter() - t0
     cnt = sysg.counters()
+    rebuilds = (sysg.get_skin()[1] - rebuilds0) if not convex else None
     mean_contacts = cnt["contacts"] / max(cnt["force_evals"], 1)
     value = n * args.steps / el
     log(f"timed {args.steps} steps in {el:.4f} s: {value:.4e} particle-steps/s, "
-        f"mean contacts {mean_contacts:.0f}")
+        f"mean contacts {mean_contacts:.0f}, list rebuilds {rebuilds}")
 
-    # force-kernel duration with HIP events on the context's stream
+    # kernel durations with HIP events on the context's stream (the extended
+    # launch records each kernel's own start and end)
     ksteps = min(args.steps, 200)
     sysg.reset_counters()
     sysg.set_kernel_timing(True)
     sysg.step(ksteps, dt)
     sysg.synchronize()
     kc = sysg.counters()
+    kt = sysg.kernel_times()
     sysg.set_kernel_timing(False)
     k_ms = kc["force_kernel_ms"] / max(kc["timed_launches"], 1)
+    pair_ms = kt["ms"][1] / kt["launches"][1] if kt["launches"][1] else None
     k_contacts = kc["contacts"] / max(kc["force_evals"], 1)
     alg_bytes = BYTES_PER_PARTICLE_STEP * n + BYTES_PER_CONTACT * k_contacts
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
     step_bytes = BYTES_PER_PARTICLE_STEP * n + BYTES_PER_CONTACT * mean_contacts
     step_achieved = step_bytes * args.steps / el / 1e9
-    log(f"force kernel {k_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s algorithmic")
+    log(f"force/integrator kernel {k_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s algorithmic; "
+        f"pair kernel {pair_ms * 1e3 if pair_ms else 0:.1f} us/launch")
     sysg.close()
 
     cpu = None
@@ -385,9 +412,22 @@ def main():
         log(f"CPU baseline: oracle, 1 thread, {args.cpu_steps} steps of the same workload")
         cv, cel = cpu_baseline(wl, args.cpu_steps)
         log(f"CPU {cv:.4e} particle-steps/s ({cel:.1f} s)")
-        cpu = {"value": cv, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+        one = {"value": cv, "unit": "particle-steps/s", "cores": 1, "kind": "port",
                "sample": f"{n} particles of the same workload, {args.cpu_steps} steps after 2 "
                          f"warm-up steps, single thread ({cel:.1f} s)"}
+        cpu = one
+        cores, model = host_cores()
+        if not convex and cores > 1:
+            asteps = 4 * args.cpu_steps
+            log(f"CPU baseline: oracle, {cores} threads, {asteps} steps")
+            av, ael = cpu_baseline(wl, asteps, threads=cores)
+            log(f"CPU all cores {av:.4e} particle-steps/s ({ael:.1f} s)")
+            cpu = {"value": av, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+                   "sample": f"{n} particles of the same workload, {asteps} steps after 2 "
+                             f"warm-up steps, {cores} OpenMP threads: cell-row slabs swept "
+                             f"in two colours ({ael:.1f} s)",
+                   "cpu_model": model, "one_core": one,
+                   "gpu_vs_all_cores": value / av, "gpu_vs_one_core": value / cv}
 
     if convex:
         out = {
@@ -421,9 +461,13 @@ def main():
                    "cell_factor": 1.2, "grid": nc, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_force<MODE_MOVE>", "kernel_us": k_ms * 1e3,
+                     "counter_frac": (traffic / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+                                      if traffic else None),
+                     "kernel": "k_integ<MODE_MOVE>", "kernel_us": k_ms * 1e3,
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "mean_contacts": k_contacts},
+        "pair_kernel_us": pair_ms * 1e3 if pair_ms else None,
+        "rebuilds_in_timed_window": rebuilds,
         "step_roofline": {"achieved": step_achieved, "unit": "GB/s",
                           "frac": step_achieved / HBM_PEAK_GBS},
         "cpu_baseline": cpu,

@@ -427,10 +427,19 @@ int gdem_last_error(gdem_ctx* ctx, int32_t* code, int32_t* id0, int32_t* id1);
 
 /* Counters since the last reset: force evaluations, contacts (pairs with
    overlap < 0, walls included) summed over them, and the event-timed total of
-   the force kernel in ms when timing is on (events on the context's stream). */
+   the force kernel in ms when timing is on (events on the context's stream).
+   In the packed layout of gdem_step (spheres with skin lists and memoryless
+   models) the force kernel is the integrator kernel, which sums each
+   particle's contact records and applies the leap-frog tail. */
 int gdem_set_kernel_timing(gdem_ctx* ctx, int32_t on);
 int gdem_get_counters(gdem_ctx* ctx, int64_t* force_evals, int64_t* contacts,
                       double* force_kernel_ms, int64_t* timed_launches);
+/* Event-timed totals (ms) and launch counts of two kernels since the last
+   reset: [0] the force kernel (as gdem_get_counters), [1] the pair kernel of
+   the packed layout (screening and evaluation of each candidate pair once,
+   with the candidate sweep on a rebuild).  No reference counterpart: the
+   reference's phase timers (Grains.cpp:246-258) time whole phases on the CPU. */
+int gdem_get_kernel_times(gdem_ctx* ctx, double* ms, int64_t* launches);
 int gdem_reset_counters(gdem_ctx* ctx);
 
 /* Grid actually used: cells per direction and edge lengths. */

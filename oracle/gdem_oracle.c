@@ -1887,6 +1887,188 @@ int orc_step(orc_sys* s, int nsteps, double dt) {
   return ORC_OK;
 }
 
+/* ---------------------------------------------------------------------- */
+/* All-core variant of orc_step: the timed CPU baseline on every host core.
+   It stands in for GrainsMPI's domain decomposition (Base/src/GrainsMPI.cpp)
+   on shared memory.  The cell rows along j are split into an even number of
+   slabs; slabs of one colour (even, then odd) are swept concurrently, one
+   thread each.  The half stencil of a row reaches only rows j and j+1
+   (LinkedCell.cpp:965-967), so a slab writes torsors of its own rows and of
+   the next slab's first row, which no slab of the same colour touches.  Each
+   pair is evaluated once, with the serial step's body order and arithmetic;
+   only the order in which a particle's contacts are summed differs.
+   Memoryless models, static walls, no contact log, no wall loads. */
+
+/* interact() without the candidate counter, contact log and memory maps */
+static int interact_plain(orc_sys* s, int p0, int p1, v3 shift) {
+  v3 A = ld(s->x + 3 * p0);
+  v3 B = add(ld(s->x + 3 * p1), shift);
+  double a[3], b[3], pt[3], ov[3], dist;
+  st(a, A);
+  st(b, B);
+  int rc = orc_sphere_sphere(a, s->r[p0], s->crust[p0], b, s->r[p1], s->crust[p1], pt,
+                             ov, &dist);
+  if (rc < 0) return ORC_ERR_CONTACT;
+  if (rc == 0 || !(dist < 0.)) return ORC_OK;
+  const model_t* m = &s->models[s->mat[p0]][s->mat[p1]];
+  if (m->kind == ORC_MODEL_NONE) return ORC_ERR_ARG;
+  v3 P = ld(pt);
+  v3 f = contact_force(m, P, ld(ov), dist, A, ld(s->v + 3 * p0), ld(s->w + 3 * p0),
+                       s->mass[p0], s->req[p0], B, ld(s->v + 3 * p1), ld(s->w + 3 * p1),
+                       s->mass[p1], s->req[p1]);
+  v3 f0 = scl(f, 1.);
+  v3 f1 = neg(f);
+  st(s->F + 3 * p0, add(ld(s->F + 3 * p0), f0));
+  st(s->T + 3 * p0, add(ld(s->T + 3 * p0), cross(sub(P, A), f0)));
+  st(s->F + 3 * p1, add(ld(s->F + 3 * p1), f1));
+  st(s->T + 3 * p1, add(ld(s->T + 3 * p1), cross(sub(P, B), f1)));
+  return ORC_OK;
+}
+
+/* interact_wall() without the wall's torsor */
+static int wall_plain(orc_sys* s, int p, int wi) {
+  const wall_t* wl = &s->walls[wi];
+  const double* xp = s->x + 3 * p;
+  double r = s->r[p];
+  if (!(fabs(wl->bb_centre[0] - xp[0]) <= wl->bb_ext[0] + r &&
+        fabs(wl->bb_centre[1] - xp[1]) <= wl->bb_ext[1] + r &&
+        fabs(wl->bb_centre[2] - xp[2]) <= wl->bb_ext[2] + r))
+    return ORC_OK;
+  v3 A = ld(xp), pt = mk(0., 0., 0.), ov = mk(0., 0., 0.);
+  double dist;
+  int rc = wl->kind == ORC_WALL_RECTANGLE
+               ? sphere_rect(wl, A, r, s->basis + 9 * p, &pt, &ov, &dist)
+               : sphere_wall(wl, A, r, s->crust[p], &pt, &ov, &dist);
+  if (rc < 0) return ORC_ERR_CONTACT;
+  if (rc == 0 || !(dist < 0.)) return ORC_OK;
+  const model_t* m = &s->models[s->mat[p]][wl->material];
+  if (m->kind == ORC_MODEL_NONE) return ORC_ERR_ARG;
+  v3 f = contact_force(m, pt, ov, dist, A, ld(s->v + 3 * p), ld(s->w + 3 * p), s->mass[p],
+                       s->req[p], ld(wl->origin), ld(wl->vel), ld(wl->omega), 1.e20, wl->req);
+  st(s->F + 3 * p, add(ld(s->F + 3 * p), f));
+  st(s->T + 3 * p, add(ld(s->T + 3 * p), cross(sub(pt, A), f)));
+  return ORC_OK;
+}
+
+/* The pairs particle p detects and its walls (compute_forces_body's body). */
+static int sweep_particle(orc_sys* s, int p) {
+  const int ni = s->nc[0];
+  int ijk[3];
+  if (get_cell(s, s->x + 3 * p, ijk) < 0) return ORC_ERR_SIMULATION;
+  v3 zero = mk(0., 0., 0.);
+  for (int q = s->next[p]; q >= 0; q = s->next[q]) {
+    int rc = interact_plain(s, p, q, zero);
+    if (rc) return rc;
+  }
+  for (int jj = 12; jj >= 0; --jj) {
+    int t[3];
+    double sh[3] = {0., 0., 0.};
+    int ok = 1;
+    for (int d = 0; d < 3; ++d) {
+      t[d] = ijk[d] + s->stencil[jj][d];
+      int n = s->nc[d];
+      if (t[d] < 0 || t[d] >= n) {
+        if (!s->cfg.periodic[d]) { ok = 0; break; }
+        if (t[d] < 0) { t[d] += n; sh[d] = -s->cfg.size[d]; }
+        else { t[d] -= n; sh[d] = s->cfg.size[d]; }
+      }
+    }
+    if (!ok) continue;
+    int cn = t[1] * s->nc[2] * ni + t[2] * ni + t[0];
+    for (int q = s->head[cn]; q >= 0; q = s->next[q]) {
+      int rc = interact_plain(s, p, q, mk(sh[0], sh[1], sh[2]));
+      if (rc) return rc;
+    }
+  }
+  for (int wi = 0; wi < s->nwalls; ++wi) {
+    int rc = wall_plain(s, p, wi);
+    if (rc) return rc;
+  }
+  return ORC_OK;
+}
+
+/* rows [j0, j1) of the cell grid: every particle of those cells, cell by cell */
+static int sweep_rows(orc_sys* s, int j0, int j1) {
+  const int ni = s->nc[0], nk = s->nc[2];
+  for (int j = j0; j < j1; ++j)
+    for (int k = 0; k < nk; ++k)
+      for (int i = 0; i < ni; ++i)
+        for (int p = s->head[j * nk * ni + k * ni + i]; p >= 0; p = s->next[p]) {
+          int rc = sweep_particle(s, p);
+          if (rc) return rc;
+        }
+  return ORC_OK;
+}
+
+int orc_step_parallel(orc_sys* s, int nsteps, double dt, int nthreads) {
+  if (s->has_memory || any_live(s) || nthreads < 1) return ORC_ERR_ARG;
+  const int nj = s->nc[1];
+  int nslab = 2 * nthreads;
+  if (nslab > nj) nslab = nj - (nj & 1);
+  if (nslab < 2) nslab = 0; /* one row: serial sweep */
+  s->dt_mem = dt;
+  for (int k = 0; k < nsteps; ++k) {
+    s->time += dt;
+    int err = ORC_OK, err_p = -1;
+    double err_v = 0.;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int p = 0; p < s->n; ++p) {
+      double depl = move_one(s, p, 0.5 * dt, dt);
+      if (depl > s->crust[p]) {
+#pragma omp critical(orc_err)
+        if (!err) { err = ORC_ERR_MOTION; err_p = p; err_v = depl; }
+      }
+      for (int d = 0; d < 3; ++d) {
+        if (!s->cfg.periodic[d]) continue;
+        double lo = s->cfg.origin[d], L = s->cfg.size[d];
+        double* xd = s->x + 3 * p + d;
+        if (*xd >= lo + L) *xd -= L;
+        else if (*xd < lo) *xd += L;
+      }
+    }
+    if (err) return set_error(s, err, err_p, -1, err_v, s->crust[err_p]);
+    int rc = orc_link_update(s);
+    if (rc) return rc;
+    const double* g = s->cfg.gravity;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int p = 0; p < s->n; ++p) {
+      double m = s->mass[p] * (1. - 0. / 1.);
+      s->F[3 * p] = g[0] * m;
+      s->F[3 * p + 1] = g[1] * m;
+      s->F[3 * p + 2] = g[2] * m;
+      s->T[3 * p] = s->T[3 * p + 1] = s->T[3 * p + 2] = 0.;
+    }
+    if (nslab == 0) {
+      err = sweep_rows(s, 0, nj);
+    } else {
+      for (int colour = 0; colour < 2; ++colour) {
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+        for (int h = colour; h < nslab; h += 2) {
+          int e = sweep_rows(s, (int)((long)h * nj / nslab), (int)((long)(h + 1) * nj / nslab));
+          if (e) {
+#pragma omp critical(orc_err)
+            if (!err) err = e;
+          }
+        }
+      }
+    }
+    if (err) return set_error(s, err, -1, -1, 0., 0.);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int p = 0; p < s->n; ++p) {
+      quat q = qmk(mk(s->q[4 * p], s->q[4 * p + 1], s->q[4 * p + 2]), s->q[4 * p + 3]);
+      quat pc = qconj(q);
+      v3 tq = qmulv(pc, qleftvec(q, ld(s->T + 3 * p)));
+      double mcf = s->mass[p] * 1.;
+      v3 v = add(ld(s->v + 3 * p), scl(dvd(ld(s->F + 3 * p), mcf), 0.5 * dt));
+      v3 wbf = ssprk3(tq, ld(s->wbf + 3 * p), s->inertia[p], 0.5 * dt);
+      st(s->v + 3 * p, v);
+      st(s->wbf + 3 * p, wbf);
+      st(s->w + 3 * p, qmulv(q, qleftvec(pc, wbf)));
+    }
+  }
+  return ORC_OK;
+}
+
 int orc_get_state(const orc_sys* s, double* x, double* v, double* w, double* q,
                   double* F, double* T) {
   size_t n3 = sizeof(double) * 3 * (size_t)s->n;

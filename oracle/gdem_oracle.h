@@ -144,6 +144,11 @@ int orc_advance_velocity(orc_sys* s, double dt_vel);
 /* Initial binning + a0 (Grains::do_before_time_stepping), then nsteps KDK. */
 int orc_prime(orc_sys* s);
 int orc_step(orc_sys* s, int nsteps, double dt);
+/* orc_step on nthreads threads (OpenMP): cell rows split into slabs swept
+   in two colours, each pair evaluated once as in orc_step (the summation
+   order of a particle's contacts differs).  Memoryless models and static
+   walls only; no contact log, no wall loads.  The all-core CPU baseline. */
+int orc_step_parallel(orc_sys* s, int nsteps, double dt, int nthreads);
 
 int orc_get_state(const orc_sys* s, double* x, double* v, double* w,
                   double* q, double* F, double* T);

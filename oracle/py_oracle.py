@@ -80,6 +80,7 @@ def load():
             "orc_advance_velocity": (i32, [vp, d]),
             "orc_prime": (i32, [vp]),
             "orc_step": (i32, [vp, i32, d]),
+            "orc_step_parallel": (i32, [vp, i32, d, i32]),
             "orc_get_state": (i32, [vp, vp, vp, vp, vp, vp, vp]),
             "orc_set_state": (i32, [vp, vp, vp, vp, vp, vp, vp]),
             "orc_get_grid": (i32, [vp, vp, vp, vp]),
@@ -201,6 +202,11 @@ class Oracle:
 
     def step(self, n, dt):
         self._chk(self.lib.orc_step(self.h, int(n), float(dt)), "step")
+
+    def step_parallel(self, n, dt, threads):
+        """orc_step on `threads` OpenMP threads (memoryless models, static walls)."""
+        self._chk(self.lib.orc_step_parallel(self.h, int(n), float(dt), int(threads)),
+                  "step_parallel")
 
     def link(self):
         self._chk(self.lib.orc_link(self.h), "link")

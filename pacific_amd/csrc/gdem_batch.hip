@@ -208,7 +208,7 @@ struct BatchRunner {
       d.err = err();
       d.nblk8 = 0;
       d.skin = 0.;
-      d.gate = nullptr;
+      d.gate = SkinGate{};
     }
     __syncthreads();
     for (int t = threadIdx.x; t < sy.n; t += BS) detect_particle(*sd, t);
@@ -247,14 +247,14 @@ struct BatchRunner {
       f.rec_count = nullptr;
       f.rec_cap = 0;
       f.nblk8 = 0;
-      f.skin_did = nullptr;
-      f.skin_need = nullptr;
-      f.xref = nullptr;
+      f.gate = SkinGate{};
+      f.words = nullptr;
       f.cmask = nullptr;
-      f.half_skin2 = 0.;
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < sy.n; t += BS) ncont += (unsigned long long)force_particle<MODE, false>(*sf, t);
+    double drift;
+    for (int t = threadIdx.x; t < sy.n; t += BS)
+      ncont += (unsigned long long)force_particle<MODE, false>(*sf, t, drift);
     __syncthreads();
   }
 

@@ -23,6 +23,12 @@
 
 using namespace gdem;
 
+struct TimedLaunch {
+  hipEvent_t start, stop;
+  int kind;
+};
+constexpr int kTimedKinds = 2;
+
 struct gdem_ctx {
   gdem_config cfg{};
   int dev = 0;
@@ -55,14 +61,15 @@ struct gdem_ctx {
   // staging for gdem_get_state
   double* d_stage = nullptr;
   size_t stage_len = 0;
-  // counters / timing
+  // counters / timing: events bracket kernels of kind 0 (the force kernel, or
+  // the integrator kernel of the packed layout) and kind 1 (the pair kernel)
   int64_t force_evals = 0;
   int64_t prime_evals = -1;  // force_evals right after the priming evaluation
   bool timing = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+  std::vector<TimedLaunch> ev_pool;
   size_t ev_used = 0;
-  double force_ms = 0.;
-  int64_t timed = 0;
+  double kernel_ms[kTimedKinds] = {0., 0.};
+  int64_t timed[kTimedKinds] = {0, 0};
   // last error
   int32_t err_code = 0, err_id0 = -1, err_id1 = -1;
   // convex particles: shapes, orientation state, stored contacts, wall shapes
@@ -82,10 +89,18 @@ struct gdem_ctx {
   double4* d_wall_sc = nullptr;
   // skin lists (spheres): requested skin (< 0: automatic), effective skin,
   // device words and reference positions
-  double skin_req = -1., skin = 0., rmax = 0.;
-  int32_t* d_skin = nullptr;
-  double4* d_xref = nullptr;
+  double skin_req = -1., skin = 0., rmax = 0., crust_min = 0.;
+  SkinWords* d_skin = nullptr;
   unsigned long long* d_cmask = nullptr;
+  // packed layout of the sphere step (gdem_pack.hip): used by gdem_step with
+  // skin lists and memoryless models; the canonical copies buf[] are brought
+  // up to date before any other call reads or writes the state
+  bool packed = false;
+  Pack pk[2]{};
+  PackConst pkc[2]{};
+  uint32_t* d_omask = nullptr;
+  uint32_t* d_imask = nullptr;
+  PairRec* d_prec = nullptr;
   // history (memory) contact models: per-particle memory tables, one per
   // state copy; allocated once a memory model is set and particles exist
   bool has_memory = false;
@@ -245,6 +260,80 @@ static int ensure_wall_loads(gdem_ctx* c) {
   return GDEM_OK;
 }
 
+static void free_pack(gdem_ctx* c) {
+  for (int k = 0; k < 2; ++k) {
+    (void)hipFree(c->pk[k].A); (void)hipFree(c->pk[k].B); (void)hipFree(c->pk[k].C);
+    (void)hipFree(c->pk[k].D);
+    (void)hipFree(c->pkc[k].rr); (void)hipFree(c->pkc[k].mi); (void)hipFree(c->pkc[k].aux);
+    c->pk[k] = Pack{};
+    c->pkc[k] = PackConst{};
+  }
+  (void)hipFree(c->d_omask); (void)hipFree(c->d_imask); (void)hipFree(c->d_prec);
+  c->d_omask = c->d_imask = nullptr;
+  c->d_prec = nullptr;
+  c->packed = false;
+}
+
+// Buffers of the packed layout for the current particle count (allocated on
+// first use: about 2 KB per particle, most of it the pair records).
+static int alloc_pack(gdem_ctx* c) {
+  if (c->pk[0].A) return GDEM_OK;
+  const size_t n = (size_t)std::max<int64_t>(c->n, 1);
+  for (int k = 0; k < 2; ++k) {
+    HIP_TRY(hipMalloc(&c->pk[k].A, sizeof(double4) * n));
+    HIP_TRY(hipMalloc(&c->pk[k].B, sizeof(double4) * n));
+    HIP_TRY(hipMalloc(&c->pk[k].C, sizeof(double4) * n));
+    HIP_TRY(hipMalloc(&c->pk[k].D, sizeof(double4) * n));
+    HIP_TRY(hipMalloc(&c->pkc[k].rr, sizeof(double2) * n));
+    HIP_TRY(hipMalloc(&c->pkc[k].mi, sizeof(double2) * n));
+    HIP_TRY(hipMalloc(&c->pkc[k].aux, sizeof(Aux) * n));
+  }
+  HIP_TRY(hipMalloc(&c->d_omask, sizeof(uint32_t) * n));
+  HIP_TRY(hipMalloc(&c->d_imask, sizeof(uint32_t) * n));
+  HIP_TRY(hipMalloc(&c->d_prec, sizeof(PairRec) * n * kListCap));
+  HIP_TRY(hipMemset(c->d_imask, 0, sizeof(uint32_t) * n));
+  return GDEM_OK;
+}
+
+// Brings the canonical state copy up to date from the packed layout (every
+// call other than gdem_step reads and writes the canonical copies).
+static int ensure_canon(gdem_ctx* c) {
+  if (!c->packed) return GDEM_OK;
+  launch_from_pack(c->pk[c->cur], c->pkc[c->cur], c->buf[c->cur], (int32_t)c->n, c->stream);
+  HIP_TRY(hipGetLastError());
+  c->packed = false;
+  c->graph_gen++;
+  return GDEM_OK;
+}
+
+// The packed layout from the canonical copy (both constant copies filled).
+static int ensure_packed(gdem_ctx* c) {
+  if (c->packed) return GDEM_OK;
+  const int ra = alloc_pack(c);
+  if (ra) return ra;
+  launch_to_pack(c->buf[c->cur], c->pk[c->cur], c->pkc[0], c->pkc[1], (int32_t)c->n, c->stream);
+  HIP_TRY(hipGetLastError());
+  c->packed = true;
+  c->graph_gen++;
+  return GDEM_OK;
+}
+
+// The packed layout serves gdem_step for spheres with skin lists, memoryless
+// contact models, static walls, and neither contact recording nor wall loads.
+static bool pack_eligible(const gdem_ctx* c) {
+  return !c->convex && c->n > 0 && c->skin > 0. && !c->has_memory && c->rec_cap == 0 &&
+         !c->track_loads;
+}
+
+// The next evaluation rebuilds the skin lists; the drift bound restarts.
+static int reset_skin_words(gdem_ctx* c) {
+  if (!c->d_skin) return GDEM_OK;
+  SkinWords w{};
+  w.forced[0] = w.forced[1] = 1;
+  HIP_TRY(hipMemcpy(c->d_skin, &w, sizeof(w), hipMemcpyHostToDevice));
+  return GDEM_OK;
+}
+
 extern "C" int gdem_destroy(gdem_ctx* c) {
   if (!c) return GDEM_OK;
   (void)hipSetDevice(c->dev);
@@ -262,10 +351,11 @@ extern "C" int gdem_destroy(gdem_ctx* c) {
   free_cstate(c->cbuf[1]);
   (void)hipFree(c->d_cand); (void)hipFree(c->d_crec); (void)hipFree(c->d_mask); (void)hipFree(c->d_work);
   (void)hipFree(c->d_nwork); (void)hipFree(c->d_shapes);
-  (void)hipFree(c->d_skin); (void)hipFree(c->d_xref); (void)hipFree(c->d_cmask);
+  (void)hipFree(c->d_skin); (void)hipFree(c->d_cmask);
   (void)hipFree(c->d_wall_shapes); (void)hipFree(c->d_wall_sc); (void)hipFree(c->d_traj);
   free_wall_loads(c);
-  for (auto& e : c->ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+  free_pack(c);
+  for (auto& e : c->ev_pool) { (void)hipEventDestroy(e.start); (void)hipEventDestroy(e.stop); }
   for (auto& g : c->graph_exec)
     if (g) (void)hipGraphExecDestroy(g);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -313,10 +403,10 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
   (void)hipFree(c->d_rec); (void)hipFree(c->d_rec_count);
   (void)hipFree(c->d_cand); (void)hipFree(c->d_crec); (void)hipFree(c->d_mask); (void)hipFree(c->d_work);
   (void)hipFree(c->d_nwork);
-  (void)hipFree(c->d_skin); (void)hipFree(c->d_xref); (void)hipFree(c->d_cmask);
+  (void)hipFree(c->d_skin); (void)hipFree(c->d_cmask);
   free_history(c);
+  free_pack(c);
   c->d_skin = nullptr;
-  c->d_xref = nullptr;
   c->d_cmask = nullptr;
   c->skin = 0.;
   c->d_F = c->d_T = nullptr;
@@ -347,11 +437,10 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
     HIP_TRY(hipMemset(c->d_nwork, 0, sizeof(int32_t)));
   } else {
     HIP_TRY(hipMalloc(&c->d_out_list, sizeof(int2) * nn * kListCap));
-    HIP_TRY(hipMalloc(&c->d_skin, sizeof(int32_t) * kSkinWords));
-    HIP_TRY(hipMalloc(&c->d_xref, sizeof(double4) * nn));
+    HIP_TRY(hipMalloc(&c->d_skin, sizeof(SkinWords)));
     HIP_TRY(hipMalloc(&c->d_cmask, sizeof(unsigned long long) * nn));
-    const int32_t words[kSkinWords] = {1, 1, 0};  // the first evaluation rebuilds
-    HIP_TRY(hipMemcpy(c->d_skin, words, sizeof(words), hipMemcpyHostToDevice));
+    const int rw = reset_skin_words(c);  // the first evaluation rebuilds
+    if (rw) return rw;
   }
   const size_t in_cap = convex ? (size_t)kCandCap : (size_t)kListCap;
   HIP_TRY(hipMalloc(&c->d_F, sizeof(double4) * nn));
@@ -403,25 +492,23 @@ extern "C" int gdem_set_skin(gdem_ctx* c, double skin) {
   if (!c || !(skin == skin)) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  const int rl = ensure_canon(c);
+  if (rl) return rl;
   c->skin_req = skin;
   update_skin(c);
-  if (c->d_skin) {
-    const int32_t words[kSkinWords] = {1, 1, 0};  // rebuild at the next evaluation
-    HIP_TRY(hipMemcpy(c->d_skin, words, sizeof(words), hipMemcpyHostToDevice));
-  }
-  return GDEM_OK;
+  return reset_skin_words(c);  // rebuild at the next evaluation
 }
 
 extern "C" int gdem_get_skin(gdem_ctx* c, double* skin, int64_t* rebuilds) {
   if (!c) return GDEM_ERR_ARG;
-  int32_t words[kSkinWords] = {0, 0, 0};
+  SkinWords w{};
   if (c->d_skin) {
     HIP_TRY(hipSetDevice(c->dev));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipMemcpy(words, c->d_skin, sizeof(words), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&w, c->d_skin, sizeof(w), hipMemcpyDeviceToHost));
   }
   if (skin) *skin = c->skin;
-  if (rebuilds) *rebuilds = words[kSkinRebuilds];
+  if (rebuilds) *rebuilds = w.rebuilds;
   return GDEM_OK;
 }
 
@@ -444,6 +531,7 @@ extern "C" int gdem_set_particles(gdem_ctx* c, int64_t n, const double* x, const
     sphere_particle(x + 3 * i, v + 3 * i, w + 3 * i, q + 4 * i, rr, rho, crust[i], mat,
                     (int32_t)i, pr[i], vm[i], wr[i], wb[i], qq[i], ax[i]);
     c->rmax = std::max(c->rmax, rr);
+    c->crust_min = i == 0 ? crust[i] : std::min(c->crust_min, crust[i]);
   }
   State& s = c->buf[0];
   c->cur = 0;
@@ -740,23 +828,32 @@ extern "C" int gdem_set_time_step(gdem_ctx* c, double dt) {
   return GDEM_OK;
 }
 
+static PosSrc canon_src(const State& s) { return PosSrc{s.pos_r, nullptr, s.aux}; }
+
+// Rebuild limit of the drift bound (SkinWords): skin / 2, less a relative
+// 1e-6 that dwarfs the rounding of the positions and of the bound.
+static double skin_limit(const gdem_ctx* c) { return 0.5 * c->skin * (1. - 1e-6); }
+
 static int do_binning(gdem_ctx* c) {
   // with skin lists the binning is part of the next force evaluation, which
   // runs it when a rebuild is due
   if (c->skin > 0.) return GDEM_OK;
-  launch_binning(c->bins, c->buf[c->cur], (int32_t)c->n, c->grid.ncell, c->stream);
+  launch_binning(c->bins, canon_src(c->buf[c->cur]), (int32_t)c->n, c->grid, c->d_row_rank,
+                 c->d_err, c->stream);
   HIP_TRY(hipGetLastError());
   c->binned = true;
   return GDEM_OK;
 }
 
-static std::pair<hipEvent_t, hipEvent_t>* next_event_pair(gdem_ctx* c) {
+static TimedLaunch* next_event_pair(gdem_ctx* c, int kind) {
   if (c->ev_used == c->ev_pool.size()) {
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return nullptr;
-    c->ev_pool.emplace_back(a, b);
+    c->ev_pool.push_back(TimedLaunch{a, b, 0});
   }
-  return &c->ev_pool[c->ev_used++];
+  TimedLaunch* e = &c->ev_pool[c->ev_used++];
+  e->kind = kind;
+  return e;
 }
 
 static void launch_convex_step(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft,
@@ -832,33 +929,109 @@ static void launch_convex_step(gdem_ctx* c, int mode, double dtv, double dtd, bo
   launch_cforce(mode, record, a, c->stream);
 }
 
+// Kernels of one evaluation in the packed layout: gated binning, pair kernel
+// (candidate sweep on a rebuild), integrator kernel.
+static int pack_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft) {
+  const int p = c->cur;
+  const SkinGate gate{c->d_skin, p, skin_limit(c)};
+  const PackConst& kin = c->pkc[p];
+  const PosSrc src{c->pk[p].A, kin.rr, kin.aux};
+  launch_binning(c->bins, src, (int32_t)c->n, c->grid, c->d_row_rank, c->d_err, c->stream, gate,
+                 c->d_skin, c->d_in_count);
+  HIP_TRY(hipGetLastError());
+  const int nblk = (int)((c->n + 255) / 256);
+  const int nblk8 = ((nblk + 7) / 8) * 8;
+  PairArgs pa{};
+  DetectArgs& d = pa.d;
+  d.pos = c->bins.pos;
+  d.margin = prefilter_margin(c->grid, d.sizef);
+  d.perm = c->bins.perm;
+  d.cell_start = c->bins.cell_start;
+  d.row_rank = c->d_row_rank;
+  d.grid = c->grid;
+  d.n = (int32_t)c->n;
+  d.lstride = (int32_t)c->n;
+  d.out_list = c->d_out_list;
+  d.out_count = c->d_out_count;
+  d.in_list = c->d_in_list;
+  d.in_count = c->d_in_count;
+  d.err = c->d_err;
+  d.nblk8 = nblk8;
+  d.skin = c->skin;
+  d.gate = gate;
+  pa.in = c->pk[p];
+  pa.kin = kin;
+  pa.omask = c->d_omask;
+  pa.imask = c->d_imask;
+  pa.rec = c->d_prec;
+  pa.models = c->d_models;
+  pa.model_stride = kMaxMaterials;
+  pa.contacts = c->d_counters;
+  TimedLaunch* ev = c->timing ? next_event_pair(c, 1) : nullptr;
+  launch_pair(pa, c->stream, ev ? ev->start : nullptr, ev ? ev->stop : nullptr);
+  HIP_TRY(hipGetLastError());
+  IntegArgs a{};
+  a.in = c->pk[p];
+  a.out = c->pk[1 - p];
+  a.kin = kin;
+  a.kout = c->pkc[1 - p];
+  a.perm = c->bins.perm;
+  a.grid = c->grid;
+  for (int dd = 0; dd < 3; ++dd) a.gmax[dd] = c->grid.nc[dd] * c->grid.edge[dd] * (1. - 1e-12);
+  a.models = c->d_models;
+  a.model_stride = kMaxMaterials;
+  a.walls = c->d_walls;
+  a.nwalls = c->nwalls;
+  a.n = (int32_t)c->n;
+  a.lstride = (int32_t)c->n;
+  for (int dd = 0; dd < 3; ++dd) a.gravity[dd] = c->cfg.gravity[dd];
+  a.dt_vel = dtv;
+  a.dt_disp = dtd;
+  a.crust_min = c->crust_min;
+  a.out_list = c->d_out_list;
+  a.in_list = c->d_in_list;
+  a.in_count = c->d_in_count;
+  a.omask = c->d_omask;
+  a.imask = c->d_imask;
+  a.rec = c->d_prec;
+  a.F = write_ft ? c->d_F : nullptr;
+  a.T = write_ft ? c->d_T : nullptr;
+  a.gate = gate;
+  a.words = c->d_skin;
+  a.err = c->d_err;
+  a.contacts = c->d_counters;
+  a.nblk8 = nblk8;
+  ev = c->timing ? next_event_pair(c, 0) : nullptr;
+  launch_integ(mode, a, c->stream, ev ? ev->start : nullptr, ev ? ev->stop : nullptr);
+  HIP_TRY(hipGetLastError());
+  return GDEM_OK;
+}
+
 // Kernels of one force evaluation on the current copy (binning first when
 // positions changed); no host bookkeeping, so the sequence can be captured.
-static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft,
-                          std::pair<hipEvent_t, hipEvent_t>** evp) {
+static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft) {
+  if (c->packed) return pack_launches(c, mode, dtv, dtd, write_ft);
   const bool skin = !c->convex && c->skin > 0.;
-  // need word read by this evaluation and the one of the next evaluation
-  int32_t* need = skin ? c->d_skin + kSkinNeed0 + c->cur : nullptr;
-  int32_t* need_next = skin ? c->d_skin + kSkinNeed0 + (1 - c->cur) : nullptr;
-  const int32_t* gate = need;
+  const SkinGate gate = skin ? SkinGate{c->d_skin, c->cur, skin_limit(c)} : SkinGate{};
   if (skin) {
-    // rebuild (binning + candidate sweep) only when a particle moved skin/2
-    launch_binning(c->bins, c->buf[c->cur], (int32_t)c->n, c->grid.ncell, c->stream, gate,
-                   c->d_in_count, need_next, c->d_skin + kSkinRebuilds);
+    // rebuild (binning + candidate sweep) only when the drift bound says so
+    launch_binning(c->bins, canon_src(c->buf[c->cur]), (int32_t)c->n, c->grid, c->d_row_rank,
+                   c->d_err, c->stream, gate, c->d_skin, c->d_in_count);
     HIP_TRY(hipGetLastError());
   } else if (!c->binned) {
-    launch_binning(c->bins, c->buf[c->cur], (int32_t)c->n, c->grid.ncell, c->stream);
+    launch_binning(c->bins, canon_src(c->buf[c->cur]), (int32_t)c->n, c->grid, c->d_row_rank,
+                   c->d_err, c->stream);
     HIP_TRY(hipGetLastError());
   }
   const int nblk = (int)((c->n + 255) / 256);
   const int nblk8 = ((nblk + 7) / 8) * 8;
   const bool record = c->rec_cap > 0 && c->d_rec != nullptr;
-  std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
+  TimedLaunch* ev = nullptr;
   if (c->convex) {
     // events bracket detection + force: the GJK narrow phase runs in detection
     if (c->timing) {
-      ev = next_event_pair(c);
-      if (ev) HIP_TRY(hipEventRecord(ev->first, c->stream));
+      ev = next_event_pair(c, 0);
+      if (ev) HIP_TRY(hipEventRecord(ev->start, c->stream));
     }
     launch_convex_step(c, mode, dtv, dtd, write_ft, record, nblk8);
   } else {
@@ -909,11 +1082,9 @@ static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wr
     a.rec_cap = c->rec_cap;
     a.nblk8 = nblk8;
     if (skin) {
-      a.skin_did = need;
-      a.skin_need = need_next;
-      a.xref = c->d_xref;
+      a.gate = gate;
+      a.words = c->d_skin;
       a.cmask = c->d_cmask;
-      a.half_skin2 = 0.25 * c->skin * c->skin;
     }
     if (c->has_memory) {
       if (!(c->dt_mem > 0.) || !c->hist[0].count) return GDEM_ERR_ARG;
@@ -934,20 +1105,18 @@ static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wr
       launch_detect(d, c->stream);
     // timing: the events take the force kernel's own start and end times
     // (an event recorded between two launches would add the dispatch gap)
-    if (c->timing) ev = next_event_pair(c);
-    launch_force(mode, record, a, c->stream, ev ? ev->first : nullptr, ev ? ev->second : nullptr,
+    if (c->timing) ev = next_event_pair(c, 0);
+    launch_force(mode, record, a, c->stream, ev ? ev->start : nullptr, ev ? ev->stop : nullptr,
                  c->has_memory);
     HIP_TRY(hipGetLastError());
     if (c->track_loads) {
       launch_wall_loads(a, c->d_wl_part, c->d_wall_load, c->stream);
       HIP_TRY(hipGetLastError());
     }
-    *evp = ev;
     return GDEM_OK;
   }
   HIP_TRY(hipGetLastError());
-  if (ev) HIP_TRY(hipEventRecord(ev->second, c->stream));
-  *evp = ev;
+  if (ev) HIP_TRY(hipEventRecord(ev->stop, c->stream));
   return GDEM_OK;
 }
 
@@ -959,8 +1128,7 @@ static void after_force(gdem_ctx* c) {
 }
 
 static int do_force(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft) {
-  std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
-  const int rc = force_launches(c, mode, dtv, dtd, write_ft, &ev);
+  const int rc = force_launches(c, mode, dtv, dtd, write_ft);
   if (rc) return rc;
   after_force(c);
   return GDEM_OK;
@@ -980,8 +1148,7 @@ static int graph_force(gdem_ctx* c, double dt) {
     const bool binned = c->binned;
     c->binned = false;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
-    const int rc = force_launches(c, MODE_MOVE, 0.5 * dt, dt, false, &ev);
+    const int rc = force_launches(c, MODE_MOVE, 0.5 * dt, dt, false);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &graph);
     c->binned = binned;
@@ -1004,29 +1171,38 @@ static int graph_force(gdem_ctx* c, double dt) {
 extern "C" int gdem_link_update(gdem_ctx* c) {
   if (!c) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
+  const int rl = ensure_canon(c);
+  if (rl) return rl;
   return do_binning(c);
 }
 
 extern "C" int gdem_compute_forces(gdem_ctx* c) {
   if (!c) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
+  const int rl = ensure_canon(c);
+  if (rl) return rl;
   return do_force(c, MODE_FORCE, 0., 0., true);
+}
+
+static void note_move_step(gdem_ctx* c, double dtd) {
+  if (dtd > 0. && dtd != c->dt_mem) {
+    c->dt_mem = dtd;  // the step of the next force evaluation
+    c->graph_gen++;
+  }
 }
 
 extern "C" int gdem_move(gdem_ctx* c, double dtv, double dtd) {
   if (!c) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
-  if (dtd > 0. && dtd != c->dt_mem) {
-    c->dt_mem = dtd;  // the step of the next force evaluation
-    c->graph_gen++;
-  }
+  const int rl = ensure_canon(c);
+  if (rl) return rl;
+  note_move_step(c, dtd);
   if (c->convex)
     launch_cmove(c->buf[c->cur], c->cbuf[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid,
                  c->d_row_rank, dtv, dtd, c->bins.key, c->d_err, c->stream);
   else if (c->skin > 0.)
     launch_move(c->buf[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid, c->d_row_rank, dtv, dtd,
-                c->bins.key, c->d_err, c->stream, c->d_skin + kSkinNeed0 + c->cur, c->d_xref,
-                0.25 * c->skin * c->skin);
+                c->bins.key, c->d_err, c->stream, c->d_skin, c->cur);
   else
     launch_move(c->buf[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid, c->d_row_rank, dtv, dtd,
                 c->bins.key, c->d_err, c->stream);
@@ -1038,6 +1214,8 @@ extern "C" int gdem_move(gdem_ctx* c, double dtv, double dtd) {
 extern "C" int gdem_advance_velocity(gdem_ctx* c, double dtv) {
   if (!c) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
+  const int rl = ensure_canon(c);
+  if (rl) return rl;
   if (c->convex)
     launch_cadvance(c->buf[c->cur], c->cbuf[c->cur], c->d_F, c->d_T, (int32_t)c->n, dtv,
                     c->stream);
@@ -1058,6 +1236,8 @@ static int upload_walls(gdem_ctx* c) {
 extern "C" int gdem_prime(gdem_ctx* c) {
   if (!c) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
+  const int rl = ensure_canon(c);
+  if (rl) return rl;
   // initial wall velocities (Grains::do_before_time_stepping,
   // Base/src/Grains.cpp:123-124)
   if (c->dt_mem > 0. && c->kin.init(c->h_walls, c->nwalls, c->time, c->dt_mem)) {
@@ -1153,8 +1333,17 @@ extern "C" int gdem_step(gdem_ctx* c, int32_t nsteps, double dt) {
   }
   // step 1's kick + drift; afterwards each force kernel closes its step and
   // opens the next one
-  rc = gdem_move(c, 0.5 * dt, dt);
-  if (rc) return rc;
+  if (pack_eligible(c) && !c->kin.any_live()) {
+    rc = ensure_packed(c);
+    if (rc) return rc;
+    note_move_step(c, dt);
+    launch_move_pack(c->pk[c->cur], c->pkc[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid,
+                     0.5 * dt, dt, c->d_err, c->d_skin, c->cur, c->stream);
+    HIP_TRY(hipGetLastError());
+  } else {
+    rc = gdem_move(c, 0.5 * dt, dt);
+    if (rc) return rc;
+  }
   if (c->kin.any_live()) return moving_wall_steps(c, nsteps, dt);
   const bool graph = c->use_graph && !c->timing && c->rec_cap == 0;
   for (int32_t k = 0; k < nsteps; ++k) {
@@ -1201,6 +1390,11 @@ extern "C" int gdem_get_state(gdem_ctx* c, double* x, double* v, double* w, doub
                               double* T) {
   if (!c) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
+  {
+    const int rl = ensure_canon(c);
+    if (rl) return rl;
+  }
+  HIP_TRY(hipSetDevice(c->dev));
   const size_t n = (size_t)c->n;
   const size_t need = n * 19;
   if (c->stage_len < need) {
@@ -1245,6 +1439,10 @@ extern "C" int gdem_set_state(gdem_ctx* c, const double* x, const double* v, con
                               const double* q, const double* F, const double* T) {
   if (!c || c->convex || (!F) != (!T)) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
+  {
+    const int rl = ensure_canon(c);
+    if (rl) return rl;
+  }
   HIP_TRY(hipStreamSynchronize(c->stream));
   const size_t n = (size_t)c->n;
   if (ensure_stage(c, n * 19)) return GDEM_ERR_HIP;
@@ -1271,10 +1469,8 @@ extern "C" int gdem_set_state(gdem_ctx* c, const double* x, const double* v, con
                 c->stream);
     HIP_TRY(hipGetLastError());
     c->binned = false;
-    if (c->d_skin) {
-      const int32_t words[kSkinWords] = {1, 1, 0};  // the lists are rebuilt
-      HIP_TRY(hipMemcpy(c->d_skin, words, sizeof(words), hipMemcpyHostToDevice));
-    }
+    const int rw = reset_skin_words(c);  // the lists are rebuilt
+    if (rw) return rw;
   }
   // given F and T are the leap-frog's a0: the next step starts from them
   if (F) c->primed = true;
@@ -1286,6 +1482,9 @@ extern "C" int gdem_set_state(gdem_ctx* c, const double* x, const double* v, con
 
 // Downloads the ids of the storage order of the current copy.
 static int slot_ids(gdem_ctx* c, std::vector<int32_t>& id) {
+  const int rl = ensure_canon(c);
+  if (rl) return rl;
+  HIP_TRY(hipStreamSynchronize(c->stream));
   const size_t n = (size_t)c->n;
   std::vector<Aux> ax(n);
   if (n) HIP_TRY(hipMemcpy(ax.data(), c->buf[c->cur].aux, sizeof(Aux) * n, hipMemcpyDeviceToHost));
@@ -1589,6 +1788,10 @@ extern "C" int gdem_set_wall_loads(gdem_ctx* c, int32_t on) {
   const bool redo = on && c->primed;
   if (redo && (c->has_memory || c->force_evals != c->prime_evals)) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
+  {
+    const int rl = ensure_canon(c);
+    if (rl) return rl;
+  }
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->track_loads = on != 0;
   const int rc = ensure_wall_loads(c);
@@ -1721,24 +1924,44 @@ extern "C" int gdem_set_kernel_timing(gdem_ctx* c, int32_t on) {
   return GDEM_OK;
 }
 
+// Adds the elapsed times of the recorded launches to the per-kind totals.
+static int fold_timings(gdem_ctx* c) {
+  for (size_t k = 0; k < c->ev_used; ++k) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev_pool[k].start, c->ev_pool[k].stop));
+    c->kernel_ms[c->ev_pool[k].kind] += ms;
+    c->timed[c->ev_pool[k].kind]++;
+  }
+  c->ev_used = 0;
+  return GDEM_OK;
+}
+
 extern "C" int gdem_get_counters(gdem_ctx* c, int64_t* force_evals, int64_t* contacts,
                                  double* force_kernel_ms, int64_t* timed_launches) {
   if (!c) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  for (size_t k = 0; k < c->ev_used; ++k) {
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, c->ev_pool[k].first, c->ev_pool[k].second));
-    c->force_ms += ms;
-    c->timed++;
-  }
-  c->ev_used = 0;
+  const int rt = fold_timings(c);
+  if (rt) return rt;
   unsigned long long h[2];
   HIP_TRY(hipMemcpy(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost));
   if (force_evals) *force_evals = c->force_evals;
   if (contacts) *contacts = (int64_t)h[0];
-  if (force_kernel_ms) *force_kernel_ms = c->force_ms;
-  if (timed_launches) *timed_launches = c->timed;
+  if (force_kernel_ms) *force_kernel_ms = c->kernel_ms[0];
+  if (timed_launches) *timed_launches = c->timed[0];
+  return GDEM_OK;
+}
+
+extern "C" int gdem_get_kernel_times(gdem_ctx* c, double* ms, int64_t* launches) {
+  if (!c) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const int rt = fold_timings(c);
+  if (rt) return rt;
+  for (int k = 0; k < kTimedKinds; ++k) {
+    if (ms) ms[k] = c->kernel_ms[k];
+    if (launches) launches[k] = c->timed[k];
+  }
   return GDEM_OK;
 }
 
@@ -1748,8 +1971,10 @@ extern "C" int gdem_reset_counters(gdem_ctx* c) {
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipMemset(c->d_counters, 0, sizeof(unsigned long long) * 2));
   c->force_evals = 0;
-  c->force_ms = 0.;
-  c->timed = 0;
+  for (int k = 0; k < kTimedKinds; ++k) {
+    c->kernel_ms[k] = 0.;
+    c->timed[k] = 0;
+  }
   c->ev_used = 0;
   return GDEM_OK;
 }

@@ -321,7 +321,7 @@ struct DiffRunner {
       d.err = err();
       d.nblk8 = 0;
       d.skin = 0.;
-      d.gate = nullptr;
+      d.gate = SkinGate{};
     }
     __syncthreads();
     for (int t = threadIdx.x; t < n; t += BS) detect_particle(*sd, t);

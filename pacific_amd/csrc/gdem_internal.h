@@ -57,9 +57,62 @@ static_assert(kListCap < 32, "list entries are tracked in 32-bit masks");
 enum { kErrContact = 0, kErrMotion = 1, kErrSim = 2, kErrCapacity = 3, kErrWords = 4 };
 constexpr unsigned long long kNoErr = ~0ull;
 
+// Device words of the skin candidate lists.  The lists built from the
+// positions of the last rebuild stay exact while every particle has moved
+// less than skin/2 since then.  Instead of keeping those positions, each
+// evaluation records the largest drift of its step, and the bound
+// dacc[p] + mstep[p] >= (displacement of any particle since the rebuild) is
+// kept for state copy p:
+//   * the evaluation reading copy p rebuilds when forced[p] is set or the
+//     bound of copy p exceeds the limit; its first kernel clears forced[1-p]
+//     and mstep[1-p], its integrator tail max-reduces the drift of each
+//     particle into mstep[1-p] (double bits: non-negative doubles order as
+//     unsigned integers) and one thread writes dacc[1-p] (0 after a rebuild,
+//     else dacc[p] + mstep[p]) and rebuilt[1-p];
+//   * a stand-alone move in place drifts copy p: it max-reduces into mtmp,
+//     then folds dacc[p] += mstep[p], mstep[p] = mtmp, mtmp = 0.
+// No kernel of an evaluation writes a word it reads.
+struct SkinWords {
+  int32_t forced[2];
+  int32_t rebuilt[2];  // rebuilt[p]: the evaluation that wrote copy p rebuilt
+  int32_t rebuilds;
+  int32_t pad_;
+  double dacc[2];
+  unsigned long long mstep[2];
+  unsigned long long mtmp;
+};
+
+// Rebuild decision of the evaluation reading copy p (the same in every
+// kernel of that evaluation); w == nullptr: no skin lists, the lists are
+// rebuilt at every evaluation.
+struct SkinGate {
+  const SkinWords* w;
+  int32_t p;
+  double limit;  // skin / 2, less a relative 1e-6 that dwarfs the rounding
+  __device__ __forceinline__ bool rebuild() const {
+    if (!w) return true;
+    const double bound = w->dacc[p] + __longlong_as_double((long long)w->mstep[p]);
+    return w->forced[p] != 0 || !(bound <= limit);
+  }
+};
+
+// Full-precision position source of the binning and sweep kernels: xyz (and
+// the radius in .w when rad is null), the radius in rad[i].x otherwise.
+struct PosSrc {
+  const double4* xyz;
+  const double2* rad;
+  const Aux* aux;
+  __device__ __forceinline__ double4 pos(int i) const {
+    const double4 a = xyz[i];
+    return rad ? make_double4(a.x, a.y, a.z, rad[i].x) : a;
+  }
+};
+
 // A contact-list entry: partner slot and a code packing the periodic shift to
-// apply to the partner ((s+1) in 2 bits per axis) and whether this particle is
-// the first body of the pair (bit 6).
+// apply to the partner ((s+1) in 2 bits per axis), whether this particle is
+// the first body of the pair (bit 6) and, in bits 7-11, the position of the
+// mirror entry in the partner's other list (an out entry holds its index in
+// the partner's in-list, an in entry its index in the partner's out-list).
 struct DetectArgs {
   const float4* pos;      // single-precision positions + radius, sorted order
   float sizef[3];         // box size in single precision (periodic shifts)
@@ -80,17 +133,10 @@ struct DetectArgs {
   unsigned long long* err;
   int32_t nblk8;  // blocks padded to a multiple of 8 (XCD-aware mapping)
   // skin lists: skin > 0 lists candidates (gap < skin) instead of contacts;
-  // gate (nullable) makes the sweep run only when *gate != 0
+  // the sweep runs only when the gate says a rebuild is due
   double skin;
-  const int32_t* gate;
+  SkinGate gate;
 };
-
-// Device words of the skin lists.  need[p] != 0: the force evaluation that
-// reads state copy p rebuilds the lists first (some particle moved more than
-// skin/2 since the last rebuild).  That evaluation reads need[p], clears
-// need[1 - p] in its first kernel and its integrator tail sets need[1 - p]
-// for the next one, so no kernel of an evaluation writes a word it reads.
-enum { kSkinNeed0 = 0, kSkinNeed1 = 1, kSkinRebuilds = 2, kSkinWords = 3 };
 
 // Contact memory of the history models (HookeMemory / HertzMemory): for each
 // particle (slot order, travelling with the state copies) up to kHistCap
@@ -131,16 +177,14 @@ struct ForceArgs {
   int32_t* rec_count;
   int32_t rec_cap;
   int32_t nblk8;  // blocks padded to a multiple of 8 (XCD-aware mapping)
-  // skin lists (skin_did == nullptr: lists hold this step's contacts, state
+  // skin lists (gate.w == nullptr: lists hold this step's contacts, state
   // is reordered through perm every evaluation)
-  const int32_t* skin_did;  // need word of this evaluation: lists rebuilt
-  int32_t* skin_need;       // need word of the next evaluation
-  double4* xref;        // positions at the last rebuild, slot order
+  SkinGate gate;
+  SkinWords* words;
   unsigned long long* cmask;  // screening masks (k_detect_screen), slot order
   // history models: memory read at the input slot, written at the output slot
   History hin, hout;
   double dt_mem;        // time step of the memory update
-  double half_skin2;    // (skin / 2)^2
   // wall loads (when tracked): per particle up to kWallLoadCap wall contacts
   // [k * lstride + slot]: the wall, the force on it and its torque about the
   // wall's centre (Obstacle::addForce / addTorque); null when not tracked
@@ -272,12 +316,14 @@ struct Bins {
 // Launchers (all asynchronous on the given stream).
 void launch_keys(const State& s, int32_t n, const Grid& g, const int32_t* row_rank, int32_t* key,
                  unsigned long long* err, hipStream_t st);
-// With skin lists: gate = need word of this evaluation (binning runs only
-// when it is set), in_count is cleared for the sweep, clear_need = need word of
-// the next evaluation (cleared), rebuilds counts the rebuilds.
-void launch_binning(Bins& b, const State& s, int32_t n, int32_t ncell, hipStream_t st,
-                    const int32_t* gate = nullptr, int32_t* in_count = nullptr,
-                    int32_t* clear_need = nullptr, int32_t* rebuilds = nullptr);
+// Counting sort of the positions of src by cell key.  Without skin lists
+// (gate.w null) the keys were written by the previous integrator tail.  With
+// them the binning runs only when the gate says a rebuild is due; it then
+// computes the keys from src itself and clears in_count for the sweep, and
+// its first kernel does the evaluation's skin-word duties (SkinWords).
+void launch_binning(Bins& b, const PosSrc& src, int32_t n, const Grid& g, const int32_t* row_rank,
+                    unsigned long long* err, hipStream_t st, SkinGate gate = SkinGate{},
+                    SkinWords* words = nullptr, int32_t* in_count = nullptr);
 void launch_detect(const DetectArgs& a, hipStream_t st);
 // With events, they record the kernel's own start and end (timing).  memory:
 // the history-model variant (a.hin / a.hout used).
@@ -290,10 +336,12 @@ void launch_wall_loads(const ForceArgs& a, double4* part, double4* out, hipStrea
 // Skin lists: candidate sweep (d.gate set) or screening of the lists (writes
 // f.cmask), in one launch.
 void launch_detect_screen(const DetectArgs& d, const ForceArgs& f, hipStream_t st);
+// words (skin lists): the drift bound of copy `parity` is updated (mtmp + fold).
 void launch_move(const State& s, const double4* F, const double4* T, int32_t n, const Grid& g,
                  const int32_t* row_rank, double dt_vel, double dt_disp, int32_t* key,
-                 unsigned long long* err, hipStream_t st, int32_t* skin = nullptr,
-                 const double4* xref = nullptr, double half_skin2 = 0.);
+                 unsigned long long* err, hipStream_t st, SkinWords* words = nullptr,
+                 int parity = 0);
+void launch_skin_fold(SkinWords* words, int parity, hipStream_t st);
 void launch_advance(const State& s, const double4* F, const double4* T, int32_t n,
                     double dt_vel, hipStream_t st);
 void launch_unpermute(const State& s, const double4* F, const double4* T, int32_t n,
@@ -303,5 +351,86 @@ void launch_unpermute(const State& s, const double4* F, const double4* T, int32_
 void launch_set_state(const State& s, double4* F, double4* T, int32_t n, const double* x,
                       const double* v, const double* w, const double* q, const double* f,
                       const double* t, hipStream_t st);
+
+// ---- sphere step in the packed layout (skin lists, memoryless models)
+// Per particle, in slot order.  The four arrays that change at every step
+// hold the 16 doubles the step updates (the quaternion's components ride in
+// the .w lanes), two copies (ping-pong: partners read the old copy while the
+// owner writes the new one).  The constants are read-only between rebuilds
+// and have two copies too: a rebuild writes them in the new order, and the
+// evaluation after it copies them into the other copy.
+struct Pack {
+  double4* A;  // x, y, z, q.x
+  double4* B;  // v, q.y
+  double4* C;  // world angular velocity, q.z
+  double4* D;  // body-frame angular velocity, q.w
+};
+struct PackConst {
+  double2* rr;  // radius, equivalent-sphere radius
+  double2* mi;  // mass, principal inertia
+  Aux* aux;
+};
+
+// The contact of one candidate pair, evaluated once with the reference's body
+// order (first body A, second body B at A's image): the force on A, the torque
+// on A about its centre and the torque on B about its image's centre.  Stored
+// at the detecting particle's out-list entry, [e * lstride + t].
+struct PairRec {
+  double f[3], ta[3], tb[3];
+};
+
+struct PairArgs {
+  DetectArgs d;        // the candidate sweep of a rebuild (d.in unused)
+  Pack in;
+  PackConst kin;
+  uint32_t* omask;     // [n] out entries in contact (written)
+  uint32_t* imask;     // [n] in entries in contact (atomic OR, zero on entry)
+  PairRec* rec;
+  const Model* models;
+  int32_t model_stride;
+  unsigned long long* contacts;
+};
+
+struct IntegArgs {
+  Pack in, out;
+  PackConst kin, kout;
+  const int32_t* perm;
+  Grid grid;
+  double gmax[3];      // below nc * edge by a relative 1e-12: surely inside the grid
+  const Model* models;
+  int32_t model_stride;
+  const Wall* walls;
+  int32_t nwalls;
+  int32_t n, lstride;
+  double gravity[3];
+  double dt_vel, dt_disp;
+  double crust_min;    // smallest crust: a drift below it raises no MotionError
+  const int2* out_list;
+  const int2* in_list;
+  const int32_t* in_count;
+  const uint32_t* omask;
+  uint32_t* imask;     // cleared after reading
+  const PairRec* rec;
+  double4* F;          // written when non-null
+  double4* T;
+  SkinGate gate;
+  SkinWords* words;
+  unsigned long long* err;
+  unsigned long long* contacts;
+  int32_t nblk8;
+};
+
+void launch_to_pack(const State& s, const Pack& pk, const PackConst& k0, const PackConst& k1,
+                    int32_t n, hipStream_t st);
+void launch_from_pack(const Pack& pk, const PackConst& k, const State& s, int32_t n,
+                      hipStream_t st);
+void launch_pair(const PairArgs& a, hipStream_t st, hipEvent_t start = nullptr,
+                 hipEvent_t stop = nullptr);
+void launch_integ(int mode, const IntegArgs& a, hipStream_t st, hipEvent_t start = nullptr,
+                  hipEvent_t stop = nullptr);
+// Kick + drift of the packed copy in place (the opening half of gdem_step).
+void launch_move_pack(const Pack& pk, const PackConst& k, const double4* F, const double4* T,
+                      int32_t n, const Grid& g, double dt_vel, double dt_disp,
+                      unsigned long long* err, SkinWords* words, int parity, hipStream_t st);
 
 }  // namespace gdem

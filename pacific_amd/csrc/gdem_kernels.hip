@@ -49,24 +49,48 @@ __global__ void __launch_bounds__(kBlock) k_keys(State s, int n, Grid g, const i
 }
 
 // ---------------------------------------------------------------- binning
+// Cell key of position p (as k_keys); -1 outside the grid.
+__device__ __forceinline__ int key_of(const Grid& g, const int32_t* row_rank, double4 p) {
+  int ijk[3];
+  if (!cell_of(g, mk(p.x, p.y, p.z), ijk)) return -1;
+  return row_base(g, row_rank, ijk[1], ijk[2]) + ijk[0];
+}
+
 // Arrival slot of each particle in its cell.  Keys arrive almost sorted (the
 // storage order is the previous step's cell order), so equal keys of
-// neighbouring lanes are merged into one atomic per run.
-__global__ void __launch_bounds__(kBlock) k_count(const int32_t* key, int32_t* slot, int32_t* cnt,
-                                                  int n, const int32_t* gate, int32_t* in_count,
-                                                  int32_t* clear_need, int32_t* rebuilds) {
-  // skin lists: this evaluation's first kernel clears the next one's need word
-  if (clear_need && blockIdx.x == 0 && threadIdx.x == 0) {
-    *clear_need = 0;
-    if (*gate != 0) *rebuilds += 1;
+// neighbouring lanes are merged into one atomic per run.  With skin lists
+// (words non-null) the keys are computed here from src, and block 0 does the
+// evaluation's first-kernel duties on the skin words.
+__global__ void __launch_bounds__(kBlock) k_count(int32_t* key, int32_t* slot, int32_t* cnt,
+                                                  int n, SkinGate gate, SkinWords* words,
+                                                  int32_t* in_count, PosSrc src, Grid g,
+                                                  const int32_t* row_rank,
+                                                  unsigned long long* err) {
+  const bool rebuild = gate.rebuild();
+  if (words && blockIdx.x == 0 && threadIdx.x == 0) {
+    words->forced[1 - gate.p] = 0;
+    words->mstep[1 - gate.p] = 0ull;
+    if (rebuild) words->rebuilds += 1;
   }
-  if (gate && *gate == 0) return;
+  if (!rebuild) return;
   const int lane = threadIdx.x & 63;
   // grid-stride over blocks of particles: the loop bound is uniform in the
   // block, so every lane of a wave takes part in the shuffles and ballots
   for (int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
     const int i = b0 + threadIdx.x;
-    const int k = i < n ? key[i] : -1;
+    int k = -1;
+    if (i < n) {
+      if (words) {
+        k = key_of(g, row_rank, src.pos(i));
+        if (k < 0) {
+          atomicMin(&err[kErrSim], (unsigned long long)(uint32_t)src.aux[i].id);
+          k = 0;
+        }
+        key[i] = k;
+      } else {
+        k = key[i];
+      }
+    }
     const int kprev = __shfl_up(k, 1, 64);
     const bool head = (lane == 0) || (k != kprev);
     const unsigned long long heads = __ballot(head);
@@ -115,8 +139,8 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* smem, int& total
 // (nb <= 8 * kScanBlock) and resets the ticket.  No block waits for another.
 __global__ void __launch_bounds__(kScanBlock) k_scan(int32_t* cnt, int32_t* local, int32_t* bsum,
                                                      int32_t* ticket, int ncell, int nb,
-                                                     const int32_t* gate) {
-  if (gate && *gate == 0) return;
+                                                     SkinGate gate) {
+  if (!gate.rebuild()) return;
   __shared__ int smem[kScanBlock / 64];
   __shared__ int last;
   const int c0 = blockIdx.x * kScanCells + threadIdx.x * kScanPer;
@@ -165,8 +189,8 @@ __global__ void __launch_bounds__(kScanBlock) k_scan(int32_t* cnt, int32_t* loca
 
 __global__ void __launch_bounds__(kBlock) k_scatter(const int32_t* key, const int32_t* slot,
                                                     const int32_t* local, const int32_t* bsum,
-                                                    int32_t* perm, int n, const int32_t* gate) {
-  if (gate && *gate == 0) return;
+                                                    int32_t* perm, int n, SkinGate gate) {
+  if (!gate.rebuild()) return;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int k = key[i];
     perm[local[k] + bsum[k / kScanCells] + slot[i]] = i;
@@ -175,7 +199,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const int32_t* key, const in
 
 // One cell of k_cellsort.
 __device__ __forceinline__ void sort_cell(const int32_t* local, const int32_t* bsum, int32_t* perm,
-                                          int32_t* cell_start, const double4* pos_r,
+                                          int32_t* cell_start, const PosSrc& src,
                                           float4* pos_sorted, int c, int ncell, int n) {
   int beg = local[c] + bsum[c / kScanCells];
   int end = (c + 1 < ncell) ? local[c + 1] + bsum[(c + 1) / kScanCells] : n;
@@ -191,7 +215,7 @@ __device__ __forceinline__ void sort_cell(const int32_t* local, const int32_t* b
   cell_start[c] = beg;
   if (c == ncell - 1) cell_start[ncell] = n;
   for (int u = beg; u < end; ++u) {
-    const double4 p = pos_r[perm[u]];
+    const double4 p = src.pos(perm[u]);
     pos_sorted[u] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
   }
 }
@@ -203,16 +227,16 @@ __device__ __forceinline__ void sort_cell(const int32_t* local, const int32_t* b
 // decision is made in FP64 on the full-precision state).
 __global__ void __launch_bounds__(kBlock) k_cellsort(const int32_t* local, const int32_t* bsum,
                                                      int32_t* perm, int32_t* cell_start,
-                                                     const double4* pos_r, float4* pos_sorted,
-                                                     int ncell, int n, const int32_t* gate) {
-  if (gate && *gate == 0) return;
+                                                     PosSrc src, float4* pos_sorted,
+                                                     int ncell, int n, SkinGate gate) {
+  if (!gate.rebuild()) return;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x)
-    sort_cell(local, bsum, perm, cell_start, pos_r, pos_sorted, c, ncell, n);
+    sort_cell(local, bsum, perm, cell_start, src, pos_sorted, c, ncell, n);
 }
 
 // ---------------------------------------------------------------- detect
 __global__ void __launch_bounds__(kBlock) k_detect(DetectArgs a) {
-  if (a.gate && *a.gate == 0) return;
+  if (!a.gate.rebuild()) return;
   const int t = logical_index(a.nblk8);
   if (t >= a.n) return;
   detect_particle(a, t);
@@ -224,7 +248,7 @@ __global__ void __launch_bounds__(kBlock) k_detect(DetectArgs a) {
 __global__ void __launch_bounds__(kBlock) k_detect_screen(DetectArgs d, ForceArgs f) {
   const int t = logical_index(d.nblk8);
   if (t >= d.n) return;
-  if (*d.gate != 0)
+  if (d.gate.rebuild())
     detect_particle(d, t);
   else
     screen_particle(f, t);
@@ -236,7 +260,20 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a) {
   __shared__ unsigned long long sh_cnt[kBlock / 64];
   const int t = logical_index(a.nblk8);
   int ncont = 0;
-  if (t < a.n) ncont = force_particle<MODE, RECORD, MEM>(a, t);
+  double drift = 0.;
+  if (t < a.n) ncont = force_particle<MODE, RECORD, MEM>(a, t, drift);
+  if (a.words) {
+    // skin words of the copy this evaluation writes (SkinWords)
+    const unsigned long long m = wave_max_bits(drift);
+    if ((threadIdx.x & 63) == 0 && m != 0ull) atomicMax(&a.words->mstep[1 - a.gate.p], m);
+    if (t == 0) {
+      const bool rebuilt = a.gate.rebuild();
+      const int p = a.gate.p;
+      a.words->dacc[1 - p] =
+          rebuilt ? 0. : a.words->dacc[p] + __longlong_as_double((long long)a.words->mstep[p]);
+      a.words->rebuilt[1 - p] = rebuilt ? 1 : 0;
+    }
+  }
   // contact counter: wave sum, then one integer atomic per workgroup
   unsigned long long c = (unsigned long long)ncont;
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
@@ -313,11 +350,22 @@ __global__ void __launch_bounds__(256) k_wall_load_final(const double4* part, in
 __global__ void __launch_bounds__(kBlock) k_move(State s, const double4* Fa, const double4* Ta,
                                                  int n, Grid g, const int32_t* row_rank,
                                                  double dtv, double dtd, int32_t* key,
-                                                 unsigned long long* err, int32_t* skin,
-                                                 const double4* xref, double half_skin2) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  move_particle(s, Fa, Ta, i, g, row_rank, dtv, dtd, key, err, skin, xref, half_skin2);
+                                                 unsigned long long* err, SkinWords* words) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double drift = 0.;
+  if (i < n) drift = move_particle(s, Fa, Ta, i, g, row_rank, dtv, dtd, key, err);
+  if (words) {
+    const unsigned long long m = wave_max_bits(drift);
+    if ((threadIdx.x & 63) == 0 && m != 0ull) atomicMax(&words->mtmp, m);
+  }
+}
+
+// After a move in place of copy p: its drift joins the bound (SkinWords).
+__global__ void k_skin_fold(SkinWords* w, int p) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  w->dacc[p] = w->dacc[p] + __longlong_as_double((long long)w->mstep[p]);
+  w->mstep[p] = w->mtmp;
+  w->mtmp = 0ull;
 }
 
 __global__ void __launch_bounds__(kBlock) k_advance(State s, const double4* Fa, const double4* Ta,
@@ -402,14 +450,16 @@ void launch_keys(const State& s, int32_t n, const Grid& g, const int32_t* row_ra
                      err);
 }
 
-void launch_binning(Bins& b, const State& s, int32_t n, int32_t ncell, hipStream_t st,
-                    const int32_t* gate, int32_t* in_count, int32_t* clear_need,
-                    int32_t* rebuilds) {
-  // with a gate word (skin lists) the grids are capped and the kernels
-  // stride; one block at least: the count pass also clears a word
-  auto capped = [gate](int blocks) { return gate ? std::min(blocks, kGateGrid) : blocks; };
+void launch_binning(Bins& b, const PosSrc& src, int32_t n, const Grid& g, const int32_t* row_rank,
+                    unsigned long long* err, hipStream_t st, SkinGate gate, SkinWords* words,
+                    int32_t* in_count) {
+  // with skin lists the grids are capped and the kernels stride; one block at
+  // least: the count pass also updates the skin words
+  const bool gated = gate.w != nullptr;
+  auto capped = [gated](int blocks) { return gated ? std::min(blocks, kGateGrid) : blocks; };
+  const int ncell = g.ncell;
   hipLaunchKernelGGL(k_count, dim3(capped(nblocks(n > 0 ? n : 1, kBlock))), dim3(kBlock), 0, st,
-                     b.key, b.slot, b.cnt, n, gate, in_count, clear_need, rebuilds);
+                     b.key, b.slot, b.cnt, n, gate, words, in_count, src, g, row_rank, err);
   const int nb = nblocks(ncell, kScanCells);
   hipLaunchKernelGGL(k_scan, dim3(nb), dim3(kScanBlock), 0, st, b.cnt, b.local, b.bsum, b.ticket,
                      ncell, nb, gate);
@@ -417,8 +467,7 @@ void launch_binning(Bins& b, const State& s, int32_t n, int32_t ncell, hipStream
     hipLaunchKernelGGL(k_scatter, dim3(capped(nblocks(n, kBlock))), dim3(kBlock), 0, st, b.key,
                        b.slot, b.local, b.bsum, b.perm, n, gate);
   hipLaunchKernelGGL(k_cellsort, dim3(capped(nblocks(ncell, kBlock))), dim3(kBlock), 0, st,
-                     b.local, b.bsum, b.perm, b.cell_start, (const double4*)s.pos_r, b.pos, ncell,
-                     n, gate);
+                     b.local, b.bsum, b.perm, b.cell_start, src, b.pos, ncell, n, gate);
 }
 
 void launch_wall_loads(const ForceArgs& a, double4* part, double4* out, hipStream_t st) {
@@ -473,11 +522,15 @@ void launch_force(int mode, bool record, const ForceArgs& a, hipStream_t st, hip
 
 void launch_move(const State& s, const double4* F, const double4* T, int32_t n, const Grid& g,
                  const int32_t* row_rank, double dt_vel, double dt_disp, int32_t* key,
-                 unsigned long long* err, hipStream_t st, int32_t* skin, const double4* xref,
-                 double half_skin2) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_move, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, s, F, T, n, g,
-                     row_rank, dt_vel, dt_disp, key, err, skin, xref, half_skin2);
+                 unsigned long long* err, hipStream_t st, SkinWords* words, int parity) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_move, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, s, F, T, n, g,
+                       row_rank, dt_vel, dt_disp, key, err, words);
+  if (words) launch_skin_fold(words, parity, st);
+}
+
+void launch_skin_fold(SkinWords* words, int parity, hipStream_t st) {
+  hipLaunchKernelGGL(k_skin_fold, dim3(1), dim3(64), 0, st, words, parity);
 }
 
 void launch_advance(const State& s, const double4* F, const double4* T, int32_t n,

@@ -15,6 +15,7 @@ namespace gdem {
 // ---------------------------------------------------------------- detect
 struct Detector {
   const DetectArgs& a;
+  PosSrc src;  // full-precision positions and records in storage order
   int t;
   float4 mef;
   bool have_me;
@@ -23,8 +24,8 @@ struct Detector {
   Aux ax;
   int nout;
 
-  __device__ __forceinline__ Detector(const DetectArgs& a_, int t_, float4 mef_)
-      : a(a_), t(t_), mef(mef_), have_me(false), X{}, r(0.), ax{}, nout(0) {}
+  __device__ __forceinline__ Detector(const DetectArgs& a_, const PosSrc& src_, int t_, float4 mef_)
+      : a(a_), src(src_), t(t_), mef(mef_), have_me(false), X{}, r(0.), ax{}, nout(0) {}
 
   // One candidate u with periodic shift s (applied to u).  Cross-cell
   // candidates of the half stencil have this particle as the first body; a
@@ -39,15 +40,15 @@ struct Detector {
     if (dxf * dxf + dyf * dyf + dzf * dzf > rsf * rsf) return;
     if (!have_me) {
       const int s = a.perm[t];
-      const double4 me = a.in.pos_r[s];
+      const double4 me = src.pos(s);
       X = mk(me.x, me.y, me.z);
       r = me.w;
-      ax = a.in.aux[s];
+      ax = src.aux[s];
       have_me = true;
     }
     const int s2 = a.perm[u];
-    const double4 p2 = a.in.pos_r[s2];
-    const Aux ax2 = a.in.aux[s2];
+    const double4 p2 = src.pos(s2);
+    const Aux ax2 = src.aux[s2];
     const double shx = sx * a.grid.size[0], shy = sy * a.grid.size[1], shz = sz * a.grid.size[2];
     const v3 Bn = mk(p2.x + shx, p2.y + shy, p2.z + shz);
     const bool first = same ? (ax.id < ax2.id) : true;
@@ -74,19 +75,21 @@ struct Detector {
       }
       if (rc == 0 || !(dist < 0.)) return;
     }
+    // each entry also records where its mirror sits in the partner's other
+    // list (bits 7-11; an index at or beyond the capacity is never used)
     const int code = pack_code(sx, sy, sz, first);
+    const int kin = atomicAdd(&a.in_count[u], 1);
     if (nout < kListCap) {
-      a.out_list[(size_t)nout * a.lstride + t] = make_int2(u, code);
+      a.out_list[(size_t)nout * a.lstride + t] = make_int2(u, code | (min(kin, 31) << 7));
     } else {
       atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax.id);
     }
-    ++nout;
-    const int kin = atomicAdd(&a.in_count[u], 1);
     if (kin < kListCap) {
-      a.in_list[(size_t)kin * a.lstride + u] = make_int2(t, mirror_code(code));
+      a.in_list[(size_t)kin * a.lstride + u] = make_int2(t, mirror_code(code) | (min(nout, 31) << 7));
     } else {
       atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax2.id);
     }
+    ++nout;
   }
 
   __device__ __forceinline__ void range(int beg, int end, int sx, int sy, int sz, bool same) {
@@ -101,15 +104,21 @@ struct Detector {
   }
 };
 
-// Detection sweep of one particle (storage slot t of the sorted order).
-__device__ __forceinline__ void detect_particle(const DetectArgs& a, int t) {
+// Detection sweep of one particle (storage slot t of the sorted order), with
+// positions from src.
+__device__ __forceinline__ int detect_particle(const DetectArgs& a, const PosSrc& src, int t) {
   // the cell index comes from the full-precision position, as in the binning
-  const double4 me = a.in.pos_r[a.perm[t]];
+  const double4 me = src.pos(a.perm[t]);
   int ijk[3];
   clamped_cell(a.grid, mk(me.x, me.y, me.z), ijk);
-  Detector det(a, t, a.pos[t]);
+  Detector det(a, src, t, a.pos[t]);
   half_stencil_sweep(a.grid, a.row_rank, a.cell_start, t, ijk, det);
   a.out_count[t] = det.nout;
+  return det.nout;
+}
+
+__device__ __forceinline__ void detect_particle(const DetectArgs& a, int t) {
+  detect_particle(a, PosSrc{a.in.pos_r, nullptr, a.in.aux}, t);
 }
 
 // ---------------------------------------------------------------- force
@@ -121,14 +130,6 @@ __device__ __forceinline__ int image_shift(const Grid& g, int d, double delta) {
   if (!g.periodic[d]) return 0;
   const double h = 0.5 * g.size[d];
   return delta > h ? -1 : (delta < -h ? 1 : 0);
-}
-
-// Squared displacement from xr to x, periodic axes taken by minimum image (a
-// particle that crossed a periodic face has moved by its step, not a period).
-__device__ __forceinline__ double displacement2(const Grid& g, v3 x, v3 xr) {
-  double e[3] = {x.x - xr.x, x.y - xr.y, x.z - xr.z};
-  for (int d = 0; d < 3; ++d) e[d] += image_shift(g, d, e[d]) * g.size[d];
-  return e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
 }
 
 // Candidate at p2 clearly beyond contact with the sphere (X, r): the margin
@@ -360,22 +361,21 @@ __device__ __forceinline__ void wall_load(const ForceArgs& a, int t, int32_t id,
 }
 
 // Force, torque and integrator tail of one particle (storage slot t of the
-// sorted order).  Returns the number of contacts it is the first body of.
+// sorted order).  Returns the number of contacts it is the first body of;
+// drift = length of its displacement (0 without one).
 template <int MODE, bool RECORD, bool MEM = false>
-__device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
+__device__ __forceinline__ int force_particle(const ForceArgs& a, int t, double& drift) {
   // with skin lists the state moves through perm only after a rebuild;
   // otherwise it stays in its slot
-  const bool skin = a.skin_did != nullptr;
-  const bool rebuilt = skin && *a.skin_did != 0;
+  const bool skin = a.gate.w != nullptr;
+  const bool rebuilt = skin && a.gate.rebuild();
   const int s = (!skin || rebuilt) ? a.perm[t] : t;
   const double4 pr = a.in.pos_r[s];
   const double4 vm = a.in.vel_m[s];
   const double4 wr = a.in.om_req[s];
   const double4 wb = a.in.wbf_I[s];
   const double4 qq = a.in.quat[s];
-  // reference position of the skin lists, loaded with the state
-  double4 xr4 = make_double4(0., 0., 0., 0.);
-  if (skin && !rebuilt) xr4 = a.xref[t];
+  drift = 0.;
   Reducer<RECORD, MEM> R{a, t};
   R.s_in = s;
   R.nhist = 0;
@@ -513,6 +513,7 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
       }
       q = qmul(dq, q);
       const double depl = norm(dx);
+      drift = depl;
       if (depl > R.ax.crust)
         atomicMin(&a.err[kErrMotion], (unsigned long long)(uint32_t)R.ax.id);
       double xo[3] = {Xo.x, Xo.y, Xo.z};
@@ -533,18 +534,6 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
       }
     }
   }
-  if (skin) {
-    // the reference position of the candidate lists: the positions of the
-    // rebuild's sweep; a rebuild is due once a particle has moved skin/2
-    v3 xr;
-    if (rebuilt) {
-      xr = X;
-      a.xref[t] = make_double4(X.x, X.y, X.z, 0.);
-    } else {
-      xr = mk(xr4.x, xr4.y, xr4.z);
-    }
-    if (displacement2(a.grid, Xo, xr) > a.half_skin2) atomicOr(a.skin_need, 1);
-  }
   a.out.pos_r[t] = make_double4(Xo.x, Xo.y, Xo.z, R.r);
   a.out.vel_m[t] = make_double4(Vo.x, Vo.y, Vo.z, m);
   a.out.om_req[t] = make_double4(Wo.x, Wo.y, Wo.z, R.req);
@@ -562,12 +551,11 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t) {
 // SecondOrderLeapFrog::Move (TimeIntegration/src/SecondOrderLeapFrog.cpp:44-67)
 // via ParticleKinematics::Move (Component/src/ParticleKinematics.cpp:62-104) of
 // the particle in storage slot i, in place; writes its new cell key.
-__device__ __forceinline__ void move_particle(const State& s, const double4* Fa, const double4* Ta,
-                                              int i, const Grid& g, const int32_t* row_rank,
-                                              double dtv, double dtd, int32_t* key,
-                                              unsigned long long* err, int32_t* skin = nullptr,
-                                              const double4* xref = nullptr,
-                                              double half_skin2 = 0.) {
+// Returns the length of the displacement.
+__device__ __forceinline__ double move_particle(const State& s, const double4* Fa, const double4* Ta,
+                                                int i, const Grid& g, const int32_t* row_rank,
+                                                double dtv, double dtd, int32_t* key,
+                                                unsigned long long* err) {
   const double4 pr = s.pos_r[i], vm = s.vel_m[i], wr = s.om_req[i], wb = s.wbf_I[i],
                 qq = s.quat[i];
   const double4 f4 = Fa[i], t4 = Ta[i];
@@ -593,7 +581,8 @@ __device__ __forceinline__ void move_particle(const State& s, const double4* Fa,
     dq = qmk(mk(0., 0., 0.), 1.);
   }
   q = qmul(dq, q);
-  if (norm(dx) > ax.crust) atomicMin(&err[kErrMotion], (unsigned long long)(uint32_t)ax.id);
+  const double depl = norm(dx);
+  if (depl > ax.crust) atomicMin(&err[kErrMotion], (unsigned long long)(uint32_t)ax.id);
   double xo[3] = {X.x, X.y, X.z};
   for (int d = 0; d < 3; ++d) {
     if (!g.periodic[d]) continue;
@@ -609,15 +598,22 @@ __device__ __forceinline__ void move_particle(const State& s, const double4* Fa,
     key[i] = 0;
     atomicMin(&err[kErrSim], (unsigned long long)(uint32_t)ax.id);
   }
-  if (skin) {
-    const double4 x4 = xref[i];
-    if (displacement2(g, X, mk(x4.x, x4.y, x4.z)) > half_skin2) atomicOr(skin, 1);
-  }
   s.pos_r[i] = make_double4(X.x, X.y, X.z, pr.w);
   s.vel_m[i] = make_double4(V.x, V.y, V.z, vm.w);
   s.om_req[i] = make_double4(W.x, W.y, W.z, wr.w);
   s.wbf_I[i] = make_double4(WBF.x, WBF.y, WBF.z, wb.w);
   s.quat[i] = make_double4(q.v.x, q.v.y, q.v.z, q.w);
+  return depl;
+}
+
+// Largest value of v over the wave (v >= 0), as unsigned-integer bits.
+__device__ __forceinline__ unsigned long long wave_max_bits(double v) {
+  unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long c = __shfl_xor(b, o, 64);
+    b = c > b ? c : b;
+  }
+  return b;
 }
 
 }  // namespace gdem

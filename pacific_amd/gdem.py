@@ -137,6 +137,7 @@ def load_library(path: str = LIB_PATH):
         "gdem_set_kernel_timing": (C.c_int, [vp, i32]),
         "gdem_get_counters": (C.c_int, [vp, P(i64), P(i64), P(d), P(i64)]),
         "gdem_reset_counters": (C.c_int, [vp]),
+        "gdem_get_kernel_times": (C.c_int, [vp, P(d), P(i64)]),
         "gdem_get_grid": (C.c_int, [vp, P(i32), P(d)]),
         "gdem_version": (C.c_char_p, []),
         "gdem_add_box_shape": (C.c_int, [vp, vp, P(i32)]),
@@ -194,7 +195,8 @@ EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall"
             "gdem_get_time", "gdem_move_walls", "gdem_get_wall", "gdem_write_particles_2014",
             "gdem_read_particles_2014", "gdem_set_contact_recording",
             "gdem_get_contacts", "gdem_last_error", "gdem_set_kernel_timing",
-            "gdem_get_counters", "gdem_reset_counters", "gdem_get_grid", "gdem_version",
+            "gdem_get_counters", "gdem_reset_counters", "gdem_get_kernel_times", "gdem_get_grid",
+            "gdem_version",
             "gdem_add_box_shape", "gdem_add_polyhedron_shape", "gdem_set_convex_particles",
             "gdem_get_basis", "gdem_batch_create", "gdem_batch_destroy", "gdem_batch_add_system",
             "gdem_batch_commit", "gdem_batch_set_launch", "gdem_batch_prime", "gdem_batch_step",
@@ -543,6 +545,14 @@ class GdemSystem:
                     "gdem_get_counters")
         return {"force_evals": fe.value, "contacts": ct.value, "force_kernel_ms": ms.value,
                 "timed_launches": tl.value}
+
+    def kernel_times(self):
+        """Event-timed totals (ms) and launch counts: [force / integrator kernel,
+        pair kernel of the packed layout]."""
+        ms = (C.c_double * 2)()
+        n = (C.c_int64 * 2)()
+        self._check(self.lib.gdem_get_kernel_times(self.h, ms, n), "gdem_get_kernel_times")
+        return {"ms": list(ms), "launches": list(n)}
 
     def reset_counters(self):
         self._check(self.lib.gdem_reset_counters(self.h), "gdem_reset_counters")
