@@ -35,7 +35,13 @@
  *
  * Conventions
  *   - Return 0 on success, else one of GDEM_ERR_*; the exception the reference
- *     would throw maps to CONTACT (ContactError, RigidBodyWithCrust.cpp:624-630),
+ *     would throw maps to CONTACT (ContactError, RigidBodyWithCrust.cpp:624-630,
+ *     raised as the reference raises it: a sphere pair or sphere-wall overlap
+ *     that reaches the crust sum is first retried as
+ *     ClosestPoint_ErreurHandling does, RigidBodyWithCrust.cpp:534-588, with an
+ *     imposed overlap equal to the crust sum, and only an overlap the bodies
+ *     shrunk by ten crusts still have is an error; the convex and the
+ *     differentiated steps raise it at the crust sum),
  *     MOTION (MotionError, Particle.cpp:447-457) or SIMULATION
  *     (SimulationError, LinkedCell.cpp:1506-1531).  gdem_last_error names the
  *     offending ids (smallest particle id among the offenders; a wall is -1-k).
@@ -434,6 +440,20 @@ int gdem_last_error(gdem_ctx* ctx, int32_t* code, int32_t* id0, int32_t* id1);
 int gdem_set_kernel_timing(gdem_ctx* ctx, int32_t on);
 int gdem_get_counters(gdem_ctx* ctx, int64_t* force_evals, int64_t* contacts,
                       double* force_kernel_ms, int64_t* timed_launches);
+/* Contact statistics of the run (AppCollision::addToContactsFeatures,
+   Base/src/AppCollision.cpp:160-190, reported at the end of Grains::Simulation,
+   Base/src/Grains.cpp:225-240): switched on (and reset) by
+   gdem_set_contact_stats; every force evaluation then adds its contacts'
+   overlaps (fixed-order workgroup reductions, reproducible) and the GJK
+   iterations the closest-point routines report (1 for a sphere contact, 2 for
+   a two-sphere contact-error retry, none for rectangles and sphere-wall
+   retries).  gdem_get_contact_stats fills out[6] = {largest overlap, mean
+   overlap, time of the largest overlap, mean GJK iterations, overlaps counted,
+   GJK reports counted}.  Sphere contexts; off by default (one extra kernel per
+   evaluation when on). */
+int gdem_set_contact_stats(gdem_ctx* ctx, int32_t on);
+int gdem_get_contact_stats(gdem_ctx* ctx, double* out);
+
 /* Event-timed totals (ms) and launch counts of two kernels since the last
    reset: [0] the force kernel (as gdem_get_counters), [1] the pair kernel of
    the packed layout (screening and evaluation of each candidate pair once,

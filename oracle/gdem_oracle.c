@@ -118,6 +118,9 @@ struct orc_sys {
   int32_t *log_id0, *log_id1;
   double *log_pt, *log_ov, *log_d, *log_f;
   int64_t ncand;
+  /* contact statistics of AppCollision (AppCollision.hh:213-219) and the
+     function-static counters of addToContactsFeatures (AppCollision.cpp:163,184) */
+  double ov_mean, ov_max, ov_time_max, gjk_mean, ov_count, gjk_count;
   /* contact maps of the memory models (one per particle) */
   int has_memory;
   double dt_mem;
@@ -1576,6 +1579,107 @@ static void log_contact(orc_sys* s, int id0, int id1, v3 pt, v3 ov, double d, v3
    HookeContactForceModel::computeForces (HookeContactForceModel.cpp:121-163)
    and Torsor::addForce(point, f) (Torsor.cpp:184-189) on both bodies.
    shift is the periodic vector applied to the second body (minimum image). */
+/* AppCollision::addToContactsFeatures (Base/src/AppCollision.cpp:160-190):
+   running mean and maximum of the overlaps, running mean of the reported GJK
+   iterations, over every closest-point result of the run. */
+static void add_contact_features(orc_sys* s, double overlap, int nbIter) {
+  if (overlap < 0.) {
+    overlap = fabs(overlap);
+    s->ov_mean = (s->ov_mean * s->ov_count + overlap) / (s->ov_count + 1.);
+    s->ov_count += 1.;
+    if (overlap > s->ov_max) {
+      s->ov_max = overlap;
+      s->ov_time_max = s->time;
+    }
+  }
+  if (nbIter) {
+    s->gjk_mean = (s->gjk_mean * s->gjk_count + (double)nbIter) / (s->gjk_count + 1.);
+    s->gjk_count += 1.;
+  }
+}
+
+/* RigidBodyWithCrust::ClosestPoint_ErreurHandling (Geometry/src/
+   RigidBodyWithCrust.cpp:534-588), the retry of a pair whose overlap reaches
+   the crust sum (Particle.cpp:508-525): both bodies shrunk by ten crusts
+   (getTransformWithCrust(10, 0.5), :770-795), closest points, contact point
+   midway, overlap imposed to the crust sum along B -> A; a second-level
+   ContactError when the shrunk bodies still touch.  The sphere's rotation is
+   not used (the GPU step keeps none; it cancels for a sphere up to
+   rounding). */
+static double retry_scale(double extent, double crust) {
+  double sc = extent < ORC_EPSILON ? 1. : (extent - 10. * crust) / extent;
+  return sc < 0.5 ? 0.5 : sc;
+}
+
+static int retry_contact(v3 pA, v3 pB, double gjk_dist, double crA, double crB, double* pt,
+                         double* ov, double* dist) {
+  if (gjk_dist < ORC_EPSILON) return -1;
+  v3 ba = sub(pA, pB);
+  ba = dvd(ba, norm(ba));
+  v3 c = add(dvd(pA, 2.0), dvd(pB, 2.0));
+  double imposed = 1. * (crA + crB);
+  st(pt, c);
+  st(ov, scl(ba, imposed));
+  *dist = -imposed;
+  return 1;
+}
+
+/* Two spheres: Johnson GJK (Convex.cpp:536-627) on the shrunk spheres starts
+   from v = A - B and takes the support points along -v and +v.  Apart, their
+   difference w points along v and GJK stops at its second iteration with that
+   one-point simplex (the new supports repeat the first ones to rounding): its
+   witness points are the first iteration's, computed here with the same
+   operations; two iterations.  Touching (w not along v), the second
+   iteration's supports lie on the far sides, the segment of the two simplex
+   points holds the origin and GJK ends at distance 0: the second-level
+   ContactError. */
+int orc_sphere_sphere_retry(const double* a, double ra, double cra, const double* b, double rb,
+                            double crb, double* pt, double* ov, double* dist, int* nbIter) {
+  double sa = retry_scale(ra, cra), sb = retry_scale(rb, crb);
+  v3 A = ld(a), B = ld(b);
+  v3 v = sub(A, B);  /* a2w(support(0)) - b2w(support(0)): the centres */
+  /* Sphere::support of -v seen through the scaled basis, back to world */
+  v3 ua = mk(sa * -v.x + 0. * -v.y + 0. * -v.z, 0. * -v.x + sa * -v.y + 0. * -v.z,
+             0. * -v.x + 0. * -v.y + sa * -v.z);
+  v3 ub = mk(sb * v.x + 0. * v.y + 0. * v.z, 0. * v.x + sb * v.y + 0. * v.z,
+             0. * v.x + 0. * v.y + sb * v.z);
+  double na = norm(ua), nb = norm(ub);
+  v3 qa = na > ORC_EPSILON ? scl(ua, ra / na) : mk(0., 0., 0.);
+  v3 qb = nb > ORC_EPSILON ? scl(ub, rb / nb) : mk(0., 0., 0.);
+  v3 pA = mk(sa * qa.x + 0. * qa.y + 0. * qa.z + A.x, 0. * qa.x + sa * qa.y + 0. * qa.z + A.y,
+             0. * qa.x + 0. * qa.y + sa * qa.z + A.z);
+  v3 pB = mk(sb * qb.x + 0. * qb.y + 0. * qb.z + B.x, 0. * qb.x + sb * qb.y + 0. * qb.z + B.y,
+             0. * qb.x + 0. * qb.y + sb * qb.z + B.z);
+  *nbIter = 2;
+  v3 w = sub(pA, pB);
+  if (!(dot(w, v) > 0.)) return -1;
+  return retry_contact(pA, pB, norm(w), cra, crb, pt, ov, dist);
+}
+
+/* A sphere against a box wall: the exact closest points of the shrunk box
+   (the point nearest to the centre) and of the shrunk sphere, the limit GJK
+   converges to (the reference stops at its 1e-10 optimality gap).  Reports
+   no GJK iterations. */
+static int sphere_wall_retry(const wall_t* wl, v3 A, double r, double cr, double* pt, double* ov,
+                             double* dist, int* nbIter) {
+  *nbIter = 0;
+  v3 l = xform(wl->inv_basis, wl->inv_origin, A);
+  double q[3] = {l.x, l.y, l.z};
+  int inside = 1;
+  for (int d = 0; d < 3; ++d) {
+    double e = wl->ext[d] * retry_scale(wl->ext[d], wl->crust);
+    if (q[d] > e) { q[d] = e; inside = 0; }
+    else if (q[d] < -e) { q[d] = -e; inside = 0; }
+  }
+  if (inside) return -1;
+  v3 pB = xform(wl->basis, wl->origin, mk(q[0], q[1], q[2]));
+  v3 u = sub(pB, A);
+  double nu = norm(u);
+  double ra = r * retry_scale(r, cr);
+  v3 pA = add(A, scl(dvd(u, nu), ra));
+  return retry_contact(pA, pB, nu - ra, cr, wl->crust, pt, ov, dist);
+}
+
 static int interact(orc_sys* s, int p0, int p1, v3 shift) {
   v3 A = ld(s->x + 3 * p0);
   v3 B = add(ld(s->x + 3 * p1), shift);
@@ -1585,9 +1689,16 @@ static int interact(orc_sys* s, int p0, int p1, v3 shift) {
   s->ncand++;
   int rc = orc_sphere_sphere(a, s->r[p0], s->crust[p0], b, s->r[p1], s->crust[p1], pt,
                              ov, &dist);
+  int nbIter = 1;  /* ClosestPointSPHERESPHERE reports one iteration */
+  double over = -dist;
   if (rc < 0)
-    return set_error(s, ORC_ERR_CONTACT, p0, p1, -dist, s->crust[p0] + s->crust[p1]);
-  if (rc == 0 || !(dist < 0.)) return ORC_OK;
+    rc = orc_sphere_sphere_retry(a, s->r[p0], s->crust[p0], b, s->r[p1], s->crust[p1], pt, ov,
+                                 &dist, &nbIter);
+  if (rc < 0)
+    return set_error(s, ORC_ERR_CONTACT, p0, p1, over, s->crust[p0] + s->crust[p1]);
+  if (rc == 0) return ORC_OK;
+  add_contact_features(s, dist, nbIter);
+  if (!(dist < 0.)) return ORC_OK;
   const model_t* m = &s->models[s->mat[p0]][s->mat[p1]];
   if (m->kind == ORC_MODEL_NONE) return set_error(s, ORC_ERR_ARG, p0, p1, 0., 0.);
   v3 P = ld(pt);
@@ -1642,8 +1753,17 @@ static int interact_wall(orc_sys* s, int p, int wi) {
   int rc = wl->kind == ORC_WALL_RECTANGLE
                ? sphere_rect(wl, A, r, s->basis + 9 * p, &pt, &ov, &dist)
                : sphere_wall(wl, A, r, s->crust[p], &pt, &ov, &dist);
-  if (rc < 0) return set_error(s, ORC_ERR_CONTACT, p, -1 - wi, -dist, s->crust[p] + wl->crust);
+  int nbIter = wl->kind == ORC_WALL_RECTANGLE ? 0 : 1;  /* ClosestPointRECTANGLE reports none */
+  double over = -dist;
+  if (rc < 0) {
+    double p3[3], o3[3];
+    rc = sphere_wall_retry(wl, A, r, s->crust[p], p3, o3, &dist, &nbIter);
+    pt = ld(p3);
+    ov = ld(o3);
+  }
+  if (rc < 0) return set_error(s, ORC_ERR_CONTACT, p, -1 - wi, over, s->crust[p] + wl->crust);
   if (rc == 0 || !(dist < 0.)) return ORC_OK;
+  add_contact_features(s, dist, nbIter);
   const model_t* m = &s->models[s->mat[p]][wl->material];
   if (m->kind == ORC_MODEL_NONE) return set_error(s, ORC_ERR_ARG, p, -1 - wi, 0., 0.);
   v3 wv = ld(wl->vel), ww = ld(wl->omega);
@@ -1908,6 +2028,10 @@ static int interact_plain(orc_sys* s, int p0, int p1, v3 shift) {
   st(b, B);
   int rc = orc_sphere_sphere(a, s->r[p0], s->crust[p0], b, s->r[p1], s->crust[p1], pt,
                              ov, &dist);
+  int nbIter;
+  if (rc < 0)
+    rc = orc_sphere_sphere_retry(a, s->r[p0], s->crust[p0], b, s->r[p1], s->crust[p1], pt, ov,
+                                 &dist, &nbIter);
   if (rc < 0) return ORC_ERR_CONTACT;
   if (rc == 0 || !(dist < 0.)) return ORC_OK;
   const model_t* m = &s->models[s->mat[p0]][s->mat[p1]];
@@ -1939,6 +2063,13 @@ static int wall_plain(orc_sys* s, int p, int wi) {
   int rc = wl->kind == ORC_WALL_RECTANGLE
                ? sphere_rect(wl, A, r, s->basis + 9 * p, &pt, &ov, &dist)
                : sphere_wall(wl, A, r, s->crust[p], &pt, &ov, &dist);
+  if (rc < 0) {
+    double p3[3], o3[3];
+    int nbIter;
+    rc = sphere_wall_retry(wl, A, r, s->crust[p], p3, o3, &dist, &nbIter);
+    pt = ld(p3);
+    ov = ld(o3);
+  }
   if (rc < 0) return ORC_ERR_CONTACT;
   if (rc == 0 || !(dist < 0.)) return ORC_OK;
   const model_t* m = &s->models[s->mat[p]][wl->material];
@@ -2148,3 +2279,15 @@ int orc_last_error(const orc_sys* s, int* code, int32_t* id0, int32_t* id1,
 }
 
 int64_t orc_candidates(const orc_sys* s) { return s->ncand; }
+
+/* AppCollision::getOverlapMax / getOverlapMean / getTimeOverlapMax /
+   getNbIterGJKMean (Base/src/AppCollision.cpp:205-240): out[0..3], and the
+   number of overlaps and of GJK reports counted, out[4..5]. */
+void orc_get_contact_stats(const orc_sys* s, double* out) {
+  out[0] = s->ov_max;
+  out[1] = s->ov_mean;
+  out[2] = s->ov_time_max;
+  out[3] = s->gjk_mean;
+  out[4] = s->ov_count;
+  out[5] = s->gjk_count;
+}

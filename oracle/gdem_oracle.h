@@ -166,6 +166,12 @@ int orc_last_error(const orc_sys* s, int* code, int32_t* id0, int32_t* id1,
                    double* value, double* limit);
 /* Candidate pairs tested in the last ComputeForces (statistics). */
 int64_t orc_candidates(const orc_sys* s);
+/* contact statistics of the run: max overlap, mean overlap, time of the max,
+   mean GJK iterations, overlaps counted, GJK reports counted */
+void orc_get_contact_stats(const orc_sys* s, double* out);
+/* ClosestPoint_ErreurHandling of two spheres (closed form of its GJK) */
+int orc_sphere_sphere_retry(const double* a, double ra, double cra, const double* b, double rb,
+                            double crb, double* pt, double* ov, double* dist, int* nbIter);
 
 /* ---- Stand-alone kernels of the restatement (used by the unit tests) ---- */
 /* returns 1 contact, 0 no contact, -1 ContactError */
@@ -199,6 +205,7 @@ int orc_c_create(const orc_config* cfg, orc_csys** out);
 void orc_c_destroy(orc_csys* s);
 /* returns the shape index, or -1 */
 int orc_c_add_box_shape(orc_csys* s, const double* half_extent);
+int orc_c_add_sphere_shape(orc_csys* s, double r);
 int orc_c_add_polyhedron_shape(orc_csys* s, int nv, const double* verts, int nf,
                                const int32_t* face_size, const int32_t* face_idx);
 int orc_c_shape_info(const orc_csys* s, int shape, double* vol, double* inertia,

@@ -407,6 +407,20 @@ int orc_c_add_box_shape(orc_csys* s, const double* half) {
   return s->nshape++;
 }
 
+/* Sphere(r) (Geometry/src/Sphere.cpp): radius in ext[0]; only for closest
+   points (the particles of the convex step are boxes and polyhedra). */
+int orc_c_add_sphere_shape(orc_csys* s, double r) {
+  if (s->nshape >= MAXSH || !(r > 0.)) return -1;
+  shape_t* sh = &s->shapes[s->nshape];
+  memset(sh, 0, sizeof(*sh));
+  sh->type = ORC_SHAPE_SPHERE;
+  sh->ext[0] = r;
+  sh->vol = 4.0 * ORC_PI * r * r * r / 3.0;
+  sh->circ = r;
+  finish_shape(sh);
+  return s->nshape++;
+}
+
 /* GrainsExec::computeVolumeInertiaContrib (Base/src/GrainsExec.cpp:748-800),
    tetrahedron (0, A2, A3, A4) after F. Tonon, J. Math. Stat. 1 (2004) */
 static void tet_contrib(v3 A2, v3 A3, v3 A4, double* vol, double* in) {

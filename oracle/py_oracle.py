@@ -88,6 +88,8 @@ def load():
             "orc_get_contacts": (i32, [vp, i32, P(i32), vp, vp, vp, vp, vp, vp]),
             "orc_last_error": (i32, [vp, P(i32), P(C.c_int32), P(C.c_int32), P(d), P(d)]),
             "orc_candidates": (C.c_int64, [vp]),
+            "orc_get_contact_stats": (None, [vp, vp]),
+            "orc_sphere_sphere_retry": (i32, [vp, d, d, vp, d, d, vp, vp, P(d), P(i32)]),
             "orc_sphere_sphere": (i32, [vp, d, d, vp, d, d, vp, vp, P(d)]),
             "orc_sphere_box": (i32, [vp, d, d, vp, vp, vp, d, vp, vp, P(d)]),
             "orc_sphere_rectangle": (i32, [vp, d, vp, vp, vp, d, d, vp, vp, P(d)]),
@@ -293,6 +295,15 @@ class Oracle:
                                   _p(res["force"]))
         return res
 
+    def contact_stats(self):
+        """AppCollision's contact features of the run: overlap max, mean, time of
+        the max, mean GJK iterations, and the numbers of overlaps and GJK
+        reports counted."""
+        out = np.zeros(6)
+        self.lib.orc_get_contact_stats(self.h, _p(out))
+        return {"overlap_max": out[0], "overlap_mean": out[1], "time_overlap_max": out[2],
+                "gjk_mean": out[3], "overlaps": int(out[4]), "gjk_reports": int(out[5])}
+
     def candidates(self):
         return self.lib.orc_candidates(self.h)
 
@@ -319,6 +330,17 @@ def rotate_two_vectors(u, v, x):
     out = np.empty(3)
     load().orc_rotate_two_vectors(_p(_f(u, (3,))), _p(_f(v, (3,))), _p(_f(x, (3,))), _p(out))
     return out
+
+
+def sphere_sphere_retry(a, ra, cra, b, rb, crb):
+    """ClosestPoint_ErreurHandling of two spheres: (rc, point, overlap vector,
+    overlap, GJK iterations)."""
+    lib = load()
+    pt, ov = np.zeros(3), np.zeros(3)
+    dist, it = C.c_double(), C.c_int()
+    rc = lib.orc_sphere_sphere_retry(_p(_f(a, (3,))), ra, cra, _p(_f(b, (3,))), rb, crb, _p(pt),
+                                     _p(ov), C.byref(dist), C.byref(it))
+    return rc, pt, ov, dist.value, it.value
 
 
 def sphere_sphere(a, ra, cra, b, rb, crb):
@@ -371,6 +393,7 @@ def _load_convex(lib):
         "orc_c_create": (i32, [P(_Config), P(vp)]),
         "orc_c_destroy": (None, [vp]),
         "orc_c_add_box_shape": (i32, [vp, vp]),
+        "orc_c_add_sphere_shape": (i32, [vp, d]),
         "orc_c_add_polyhedron_shape": (i32, [vp, i32, vp, i32, vp, vp]),
         "orc_c_shape_info": (i32, [vp, i32, vp, vp, vp, vp]),
         "orc_c_set_particles": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
@@ -441,6 +464,8 @@ class ConvexOracle:
     def add_shape(self, sh):
         if sh[0] == "box":
             return self.lib.orc_c_add_box_shape(self.h, _p(_f(sh[1], (3,))))
+        if sh[0] == "sphere":
+            return self.lib.orc_c_add_sphere_shape(self.h, float(sh[1]))
         verts = _f(sh[1], (-1, 3))
         faces = [list(f) for f in sh[2]]
         sizes = np.array([len(f) for f in faces], np.int32)

@@ -37,6 +37,7 @@ struct gdem_ctx {
   int32_t* d_row_rank = nullptr;
   int32_t* d_row_jk = nullptr;
   int64_t n = 0;
+  int32_t ncu = 0;  // workgroups of the one-launch rebuild (one per compute unit)
   State buf[2]{};
   int cur = 0;
   double4* d_F = nullptr;
@@ -52,7 +53,13 @@ struct gdem_ctx {
   Wall* d_walls = nullptr;
   int nwalls = 0;
   unsigned long long* d_err = nullptr;
-  unsigned long long* d_counters = nullptr;  // [0] contacts
+  unsigned long long* d_counters = nullptr;  // [0] contacts (convex kernels)
+  unsigned long long* d_blk_contacts = nullptr;  // contacts counted per workgroup (spheres)
+  int32_t nblk_slots = 0;
+  // contact statistics (AppCollision::addToContactsFeatures), when switched on
+  bool stats_on = false;
+  ContactStats* d_stats = nullptr;
+  StatPart* d_stat_part = nullptr;
   bool primed = false, binned = false;
   int32_t rec_cap = 0;
   ContactRec* d_rec = nullptr;
@@ -91,6 +98,7 @@ struct gdem_ctx {
   // device words and reference positions
   double skin_req = -1., skin = 0., rmax = 0., crust_min = 0.;
   SkinWords* d_skin = nullptr;
+  float* d_path = nullptr;  // path length of each particle since the last rebuild
   unsigned long long* d_cmask = nullptr;
   // packed layout of the sphere step (gdem_pack.hip): used by gdem_step with
   // skin lists and memoryless models; the canonical copies buf[] are brought
@@ -194,6 +202,11 @@ extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
   const int64_t ncell = c->grid.ncell;
   if (hipSetDevice(c->dev) != hipSuccess) { delete c; return GDEM_ERR_HIP; }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return GDEM_ERR_HIP; }
+  if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->dev) != hipSuccess ||
+      c->ncu < 1) {
+    gdem_destroy(c);
+    return GDEM_ERR_HIP;
+  }
   std::vector<int32_t> rank = row_morton_rank(c->grid.nc);
   const int64_t nrows = (int64_t)c->grid.nc[1] * c->grid.nc[2];
   const int nb = (int)((ncell + kScanCells - 1) / kScanCells);
@@ -205,6 +218,7 @@ extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
       hipMalloc(&c->bins.cnt, sizeof(int32_t) * ncell) != hipSuccess ||
       hipMalloc(&c->bins.local, sizeof(int32_t) * ncell) != hipSuccess ||
       hipMalloc(&c->bins.bsum, sizeof(int32_t) * (nb + 1)) != hipSuccess ||
+      hipMalloc(&c->bins.rsum, sizeof(int32_t) * c->ncu) != hipSuccess ||
       hipMalloc(&c->bins.cell_start, sizeof(int32_t) * (ncell + 1)) != hipSuccess ||
       hipMalloc(&c->d_models, sizeof(c->h_models)) != hipSuccess ||
       hipMalloc(&c->d_walls, sizeof(c->h_walls)) != hipSuccess ||
@@ -329,7 +343,7 @@ static bool pack_eligible(const gdem_ctx* c) {
 static int reset_skin_words(gdem_ctx* c) {
   if (!c->d_skin) return GDEM_OK;
   SkinWords w{};
-  w.forced[0] = w.forced[1] = 1;
+  w.need[0] = w.need[1] = 1;
   HIP_TRY(hipMemcpy(c->d_skin, &w, sizeof(w), hipMemcpyHostToDevice));
   return GDEM_OK;
 }
@@ -345,13 +359,15 @@ extern "C" int gdem_destroy(gdem_ctx* c) {
   (void)hipFree(c->bins.key); (void)hipFree(c->bins.slot); (void)hipFree(c->bins.perm); (void)hipFree(c->bins.pos);
   (void)hipFree(c->d_out_list); (void)hipFree(c->d_in_list); (void)hipFree(c->d_out_count); (void)hipFree(c->d_in_count);
   (void)hipFree(c->bins.cnt); (void)hipFree(c->bins.local); (void)hipFree(c->bins.bsum); (void)hipFree(c->bins.cell_start);
+  (void)hipFree(c->bins.rsum);
   (void)hipFree(c->d_row_rank); (void)hipFree(c->d_row_jk); (void)hipFree(c->d_models); (void)hipFree(c->d_walls); (void)hipFree(c->d_err);
   (void)hipFree(c->d_counters); (void)hipFree(c->d_rec); (void)hipFree(c->d_rec_count); (void)hipFree(c->d_stage);
   free_cstate(c->cbuf[0]);
   free_cstate(c->cbuf[1]);
   (void)hipFree(c->d_cand); (void)hipFree(c->d_crec); (void)hipFree(c->d_mask); (void)hipFree(c->d_work);
   (void)hipFree(c->d_nwork); (void)hipFree(c->d_shapes);
-  (void)hipFree(c->d_skin); (void)hipFree(c->d_cmask);
+  (void)hipFree(c->d_skin); (void)hipFree(c->d_cmask); (void)hipFree(c->d_path);
+  (void)hipFree(c->d_blk_contacts); (void)hipFree(c->d_stats); (void)hipFree(c->d_stat_part);
   (void)hipFree(c->d_wall_shapes); (void)hipFree(c->d_wall_sc); (void)hipFree(c->d_traj);
   free_wall_loads(c);
   free_pack(c);
@@ -403,11 +419,15 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
   (void)hipFree(c->d_rec); (void)hipFree(c->d_rec_count);
   (void)hipFree(c->d_cand); (void)hipFree(c->d_crec); (void)hipFree(c->d_mask); (void)hipFree(c->d_work);
   (void)hipFree(c->d_nwork);
-  (void)hipFree(c->d_skin); (void)hipFree(c->d_cmask);
+  (void)hipFree(c->d_skin); (void)hipFree(c->d_cmask); (void)hipFree(c->d_path);
+  (void)hipFree(c->d_blk_contacts); (void)hipFree(c->d_stat_part);
   free_history(c);
   free_pack(c);
   c->d_skin = nullptr;
   c->d_cmask = nullptr;
+  c->d_path = nullptr;
+  c->d_blk_contacts = nullptr;
+  c->d_stat_part = nullptr;
   c->skin = 0.;
   c->d_F = c->d_T = nullptr;
   c->bins.key = c->bins.slot = c->bins.perm = nullptr;
@@ -439,9 +459,17 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
     HIP_TRY(hipMalloc(&c->d_out_list, sizeof(int2) * nn * kListCap));
     HIP_TRY(hipMalloc(&c->d_skin, sizeof(SkinWords)));
     HIP_TRY(hipMalloc(&c->d_cmask, sizeof(unsigned long long) * nn));
+    HIP_TRY(hipMalloc(&c->d_path, sizeof(float) * nn));
+    HIP_TRY(hipMemset(c->d_path, 0, sizeof(float) * nn));
     const int rw = reset_skin_words(c);  // the first evaluation rebuilds
     if (rw) return rw;
   }
+  // contact counts per workgroup of the sphere kernels, statistics parts
+  // (two per workgroup: pairs, walls)
+  c->nblk_slots = (int32_t)((((nn + 255) / 256 + 7) / 8) * 8);
+  HIP_TRY(hipMalloc(&c->d_blk_contacts, sizeof(unsigned long long) * c->nblk_slots));
+  HIP_TRY(hipMemset(c->d_blk_contacts, 0, sizeof(unsigned long long) * c->nblk_slots));
+  HIP_TRY(hipMalloc(&c->d_stat_part, sizeof(StatPart) * 2 * (size_t)c->nblk_slots));
   const size_t in_cap = convex ? (size_t)kCandCap : (size_t)kListCap;
   HIP_TRY(hipMalloc(&c->d_F, sizeof(double4) * nn));
   HIP_TRY(hipMalloc(&c->d_T, sizeof(double4) * nn));
@@ -832,7 +860,12 @@ static PosSrc canon_src(const State& s) { return PosSrc{s.pos_r, nullptr, s.aux}
 
 // Rebuild limit of the drift bound (SkinWords): skin / 2, less a relative
 // 1e-6 that dwarfs the rounding of the positions and of the bound.
-static double skin_limit(const gdem_ctx* c) { return 0.5 * c->skin * (1. - 1e-6); }
+static float skin_limit(const gdem_ctx* c) {
+  const double x = 0.5 * c->skin * (1. - 1e-6);
+  float f = (float)x;
+  if ((double)f > x) f = std::nextafter(f, 0.f);  // rounded down
+  return f;
+}
 
 static int do_binning(gdem_ctx* c) {
   // with skin lists the binning is part of the next force evaluation, which
@@ -936,8 +969,8 @@ static int pack_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wri
   const SkinGate gate{c->d_skin, p, skin_limit(c)};
   const PackConst& kin = c->pkc[p];
   const PosSrc src{c->pk[p].A, kin.rr, kin.aux};
-  launch_binning(c->bins, src, (int32_t)c->n, c->grid, c->d_row_rank, c->d_err, c->stream, gate,
-                 c->d_skin, c->d_in_count);
+  launch_rebuild(c->bins, src, (int32_t)c->n, c->grid, c->d_row_rank, c->d_err, gate, c->d_skin,
+                 c->d_in_count, c->ncu, c->stream);
   HIP_TRY(hipGetLastError());
   const int nblk = (int)((c->n + 255) / 256);
   const int nblk8 = ((nblk + 7) / 8) * 8;
@@ -966,7 +999,8 @@ static int pack_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wri
   pa.rec = c->d_prec;
   pa.models = c->d_models;
   pa.model_stride = kMaxMaterials;
-  pa.contacts = c->d_counters;
+  pa.contacts = c->d_blk_contacts;
+  pa.stat = c->stats_on ? c->d_stat_part : nullptr;
   TimedLaunch* ev = c->timing ? next_event_pair(c, 1) : nullptr;
   launch_pair(pa, c->stream, ev ? ev->start : nullptr, ev ? ev->stop : nullptr);
   HIP_TRY(hipGetLastError());
@@ -998,12 +1032,19 @@ static int pack_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wri
   a.T = write_ft ? c->d_T : nullptr;
   a.gate = gate;
   a.words = c->d_skin;
+  a.path = c->d_path;
   a.err = c->d_err;
-  a.contacts = c->d_counters;
+  a.contacts = c->d_blk_contacts;
   a.nblk8 = nblk8;
+  a.stat = c->stats_on ? c->d_stat_part + nblk8 : nullptr;
   ev = c->timing ? next_event_pair(c, 0) : nullptr;
   launch_integ(mode, a, c->stream, ev ? ev->start : nullptr, ev ? ev->stop : nullptr);
   HIP_TRY(hipGetLastError());
+  if (c->stats_on) {
+    launch_stats_final(c->d_stat_part, c->nwalls > 0 ? 2 * nblk8 : nblk8, c->d_stats,
+                       mode == MODE_FORCE ? 0. : dtd, c->stream);
+    HIP_TRY(hipGetLastError());
+  }
   return GDEM_OK;
 }
 
@@ -1014,9 +1055,10 @@ static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wr
   const bool skin = !c->convex && c->skin > 0.;
   const SkinGate gate = skin ? SkinGate{c->d_skin, c->cur, skin_limit(c)} : SkinGate{};
   if (skin) {
-    // rebuild (binning + candidate sweep) only when the drift bound says so
-    launch_binning(c->bins, canon_src(c->buf[c->cur]), (int32_t)c->n, c->grid, c->d_row_rank,
-                   c->d_err, c->stream, gate, c->d_skin, c->d_in_count);
+    // rebuild (binning + candidate sweep) only when some particle's path
+    // since the last rebuild passed skin/2
+    launch_rebuild(c->bins, canon_src(c->buf[c->cur]), (int32_t)c->n, c->grid, c->d_row_rank,
+                   c->d_err, gate, c->d_skin, c->d_in_count, c->ncu, c->stream);
     HIP_TRY(hipGetLastError());
   } else if (!c->binned) {
     launch_binning(c->bins, canon_src(c->buf[c->cur]), (int32_t)c->n, c->grid, c->d_row_rank,
@@ -1076,7 +1118,8 @@ static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wr
     a.T = write_ft ? c->d_T : nullptr;
     a.key_out = c->bins.key;
     a.err = c->d_err;
-    a.contacts = c->d_counters;
+    a.contacts = c->d_blk_contacts;
+    a.stat = c->stats_on ? c->d_stat_part : nullptr;
     a.rec = c->d_rec;
     a.rec_count = c->d_rec_count;
     a.rec_cap = c->rec_cap;
@@ -1084,6 +1127,7 @@ static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wr
     if (skin) {
       a.gate = gate;
       a.words = c->d_skin;
+      a.path = c->d_path;
       a.cmask = c->d_cmask;
     }
     if (c->has_memory) {
@@ -1111,6 +1155,11 @@ static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wr
     HIP_TRY(hipGetLastError());
     if (c->track_loads) {
       launch_wall_loads(a, c->d_wl_part, c->d_wall_load, c->stream);
+      HIP_TRY(hipGetLastError());
+    }
+    if (c->stats_on) {
+      launch_stats_final(c->d_stat_part, nblk8, c->d_stats, mode == MODE_FORCE ? 0. : dtd,
+                         c->stream);
       HIP_TRY(hipGetLastError());
     }
     return GDEM_OK;
@@ -1202,7 +1251,8 @@ extern "C" int gdem_move(gdem_ctx* c, double dtv, double dtd) {
                  c->d_row_rank, dtv, dtd, c->bins.key, c->d_err, c->stream);
   else if (c->skin > 0.)
     launch_move(c->buf[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid, c->d_row_rank, dtv, dtd,
-                c->bins.key, c->d_err, c->stream, c->d_skin, c->cur);
+                c->bins.key, c->d_err, c->stream, SkinGate{c->d_skin, c->cur, skin_limit(c)},
+                c->d_path);
   else
     launch_move(c->buf[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid, c->d_row_rank, dtv, dtd,
                 c->bins.key, c->d_err, c->stream);
@@ -1338,7 +1388,8 @@ extern "C" int gdem_step(gdem_ctx* c, int32_t nsteps, double dt) {
     if (rc) return rc;
     note_move_step(c, dt);
     launch_move_pack(c->pk[c->cur], c->pkc[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid,
-                     0.5 * dt, dt, c->d_err, c->d_skin, c->cur, c->stream);
+                     0.5 * dt, dt, c->d_err, SkinGate{c->d_skin, c->cur, skin_limit(c)},
+                     c->d_path, c->stream);
     HIP_TRY(hipGetLastError());
   } else {
     rc = gdem_move(c, 0.5 * dt, dt);
@@ -1373,6 +1424,18 @@ static int collect_errors(gdem_ctx* c) {
                    (int32_t)(uint32_t)(h[kErrContact] & 0xffffffffull));
   else if (h[kErrCapacity] != kNoErr)
     code = set_err(c, GDEM_ERR_CAPACITY, (int32_t)h[kErrCapacity], -1);
+  if (code == GDEM_OK && c->d_skin) {
+    // a barrier of the one-launch rebuild timed out: the evaluations since
+    // then did nothing
+    SkinWords w{};
+    HIP_TRY(hipMemcpy(&w, c->d_skin, sizeof(w), hipMemcpyDeviceToHost));
+    if (w.failed) {
+      fprintf(stderr, "gdem: a workgroup barrier of the list rebuild did not complete\n");
+      code = set_err(c, GDEM_ERR_HIP, -1, -1);
+      const int rw = reset_skin_words(c);
+      if (rw) return rw;
+    }
+  }
   if (code != GDEM_OK) {
     HIP_TRY(hipMemset(c->d_err, 0xff, sizeof(h)));
   }
@@ -1821,6 +1884,12 @@ extern "C" int gdem_get_wall_loads(gdem_ctx* c, double* force, double* torque) {
 extern "C" int gdem_set_time(gdem_ctx* c, double time) {
   if (!c) return GDEM_ERR_ARG;
   c->time = time;
+  if (c->stats_on) {
+    // the statistics take the time of each evaluation from the device copy
+    HIP_TRY(hipSetDevice(c->dev));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(&c->d_stats->time, &time, sizeof(double), hipMemcpyHostToDevice));
+  }
   return GDEM_OK;
 }
 
@@ -1945,6 +2014,12 @@ extern "C" int gdem_get_counters(gdem_ctx* c, int64_t* force_evals, int64_t* con
   if (rt) return rt;
   unsigned long long h[2];
   HIP_TRY(hipMemcpy(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost));
+  if (c->d_blk_contacts) {
+    std::vector<unsigned long long> blk((size_t)c->nblk_slots);
+    HIP_TRY(hipMemcpy(blk.data(), c->d_blk_contacts, sizeof(unsigned long long) * blk.size(),
+                      hipMemcpyDeviceToHost));
+    for (unsigned long long b : blk) h[0] += b;
+  }
   if (force_evals) *force_evals = c->force_evals;
   if (contacts) *contacts = (int64_t)h[0];
   if (force_kernel_ms) *force_kernel_ms = c->kernel_ms[0];
@@ -1970,6 +2045,8 @@ extern "C" int gdem_reset_counters(gdem_ctx* c) {
   HIP_TRY(hipSetDevice(c->dev));
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipMemset(c->d_counters, 0, sizeof(unsigned long long) * 2));
+  if (c->d_blk_contacts)
+    HIP_TRY(hipMemset(c->d_blk_contacts, 0, sizeof(unsigned long long) * c->nblk_slots));
   c->force_evals = 0;
   for (int k = 0; k < kTimedKinds; ++k) {
     c->kernel_ms[k] = 0.;
@@ -1977,6 +2054,36 @@ extern "C" int gdem_reset_counters(gdem_ctx* c) {
   }
   c->ev_used = 0;
   return GDEM_OK;
+}
+
+extern "C" int gdem_set_contact_stats(gdem_ctx* c, int32_t on) {
+  if (!c || c->convex) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (on && !c->d_stats) HIP_TRY(hipMalloc(&c->d_stats, sizeof(ContactStats)));
+  if (on) {
+    ContactStats z{};
+    z.time = c->time;
+    HIP_TRY(hipMemcpy(c->d_stats, &z, sizeof(z), hipMemcpyHostToDevice));
+  }
+  c->stats_on = on != 0;
+  c->graph_gen++;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_get_contact_stats(gdem_ctx* c, double* out) {
+  if (!c || !out || !c->stats_on) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  ContactStats h{};
+  HIP_TRY(hipMemcpy(&h, c->d_stats, sizeof(h), hipMemcpyDeviceToHost));
+  out[0] = h.omax;
+  out[1] = h.on ? h.osum / (double)h.on : 0.;
+  out[2] = h.tmax;
+  out[3] = h.gn ? (double)h.gsum / (double)h.gn : 0.;
+  out[4] = (double)h.on;
+  out[5] = (double)h.gn;
+  return collect_errors(c);
 }
 
 extern "C" int gdem_get_grid(gdem_ctx* c, int32_t* nc, double* edge) {

@@ -285,6 +285,97 @@ __host__ __device__ inline double closest_points(const CShape& a, const CShape& 
   return dist;
 }
 
+// ------------------------------------------------------------ contact-error retry
+// RigidBodyWithCrust::ClosestPoint_ErreurHandling (Geometry/src/
+// RigidBodyWithCrust.cpp:534-588), reached when an overlap reaches the sum of
+// the crusts (Particle.cpp:508-525, SimpleObstacle.cpp:235-252): GJK between
+// the bodies shrunk by ten crusts (getTransformWithCrust(10, 0.5), :770-795),
+// contact point midway between the witness points, overlap imposed to the
+// crust sum along B -> A.  The sphere step keeps no sphere rotation, so the
+// sphere's transform is its scaled identity (the rotation cancels for a
+// sphere up to rounding).  Returns 1, or -1 when the shrunk bodies still
+// touch (the second-level ContactError).
+
+// Scale of one bounding-box axis, shrunk by ten crusts, capped at 0.5.
+__host__ __device__ inline double retry_scale(double extent, double crust) {
+  const double s = extent < kEps ? 1. : (extent - 10. * crust) / extent;
+  return s < 0.5 ? 0.5 : s;
+}
+
+__host__ __device__ inline int retry_contact(v3 pA, v3 pB, double gjk_dist, double crA, double crB,
+                                             v3& c, v3& ov, double& dist) {
+  if (gjk_dist < kEps) return -1;
+  v3 ba = sub(pA, pB);
+  ba = dvd(ba, norm(ba));  // Vector3::normalize
+  c = add(dvd(pA, 2.0), dvd(pB, 2.0));
+  const double imposed = 1. * (crA + crB);
+  ov = scl(ba, imposed);
+  dist = -imposed;
+  return 1;
+}
+
+__host__ __device__ inline Xf scaled_identity(v3 origin, double s) {
+  const double I[9] = {1., 0., 0., 0., 1., 0., 0., 0., 1.};
+  const double sc[3] = {s, s, s};
+  return crust_xf(I, origin, sc);
+}
+
+// Two spheres.  closest_points starts from v = A - B (the supports of the
+// zero direction are the centres) and takes the support points along -v and
+// +v.  When the shrunk spheres are apart, the difference w of those points
+// points along v, the second iteration's support points repeat the first
+// ones to rounding and the optimality gap ends it with the one-point simplex
+// of the first iteration: the witness points are those of the first
+// iteration, here computed with the same operations (two iterations
+// reported).  When they touch (w not along v), the second iteration's support
+// points lie on the far sides, the segment between the two simplex points
+// holds the origin and GJK ends at distance 0: the second-level error.
+__host__ __device__ inline int sphere_sphere_retry(v3 A, double rA, double crA, v3 B, double rB,
+                                                   double crB, v3& c, v3& ov, double& dist,
+                                                   int& nbIter) {
+  CShape sa{}, sb{};
+  sa.type = sb.type = SHAPE_SPHERE;
+  sa.ext[0] = rA;
+  sb.ext[0] = rB;
+  const Xf a2w = scaled_identity(A, retry_scale(rA, crA));
+  const Xf b2w = scaled_identity(B, retry_scale(rB, crB));
+  const v3 zero = mk(0., 0., 0.);
+  const v3 v = sub(xf_apply(a2w, support(sa, zero)), xf_apply(b2w, support(sb, zero)));
+  const v3 pA = xf_apply(a2w, support(sa, vec_mat(neg(v), a2w.b)));
+  const v3 pB = xf_apply(b2w, support(sb, vec_mat(v, b2w.b)));
+  nbIter = 2;
+  const v3 w = sub(pA, pB);
+  if (!(dot(w, v) > 0.)) return -1;
+  return retry_contact(pA, pB, norm(w), crA, crB, c, ov, dist);
+}
+
+// A sphere against a box wall.  GJK between the shrunk sphere and the shrunk
+// box converges to their exact closest points: the point of the shrunk box
+// nearest to the centre, and the shrunk sphere's surface point towards it.
+// Those points are taken here in closed form; the reference stops GJK at its
+// tolerance (relative optimality gap 1e-10), which leaves its witness points
+// within about 1e-7 m of these at an edge (1e-8 m on a face).  Reports no GJK
+// iterations (nbIter = 0): no GJK runs.
+__host__ __device__ inline int sphere_wall_retry(const Wall& wl, v3 A, double r, double cr, v3& c,
+                                                 v3& ov, double& dist, int& nbIter) {
+  nbIter = 0;
+  const v3 l = xform(wl.inv_basis, wl.inv_origin, A);
+  double q[3] = {l.x, l.y, l.z};
+  bool inside = true;
+  for (int d = 0; d < 3; ++d) {
+    const double e = wl.ext[d] * retry_scale(wl.ext[d], wl.crust);
+    if (q[d] > e) { q[d] = e; inside = false; }
+    else if (q[d] < -e) { q[d] = -e; inside = false; }
+  }
+  if (inside) return -1;  // the centre lies in the shrunk box
+  const v3 pB = xform(wl.basis, wl.origin, mk(q[0], q[1], q[2]));
+  const v3 u = sub(pB, A);
+  const double nu = norm(u);
+  const double ra = r * retry_scale(r, cr);
+  const v3 pA = add(A, scl(dvd(u, nu), ra));
+  return retry_contact(pA, pB, nu - ra, cr, wl.crust, c, ov, dist);
+}
+
 // A body as seen by the narrow phase
 struct CBody {
   const CShape* sh;

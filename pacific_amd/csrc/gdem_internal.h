@@ -59,27 +59,23 @@ constexpr unsigned long long kNoErr = ~0ull;
 
 // Device words of the skin candidate lists.  The lists built from the
 // positions of the last rebuild stay exact while every particle has moved
-// less than skin/2 since then.  Instead of keeping those positions, each
-// evaluation records the largest drift of its step, and the bound
-// dacc[p] + mstep[p] >= (displacement of any particle since the rebuild) is
-// kept for state copy p:
-//   * the evaluation reading copy p rebuilds when forced[p] is set or the
-//     bound of copy p exceeds the limit; its first kernel clears forced[1-p]
-//     and mstep[1-p], its integrator tail max-reduces the drift of each
-//     particle into mstep[1-p] (double bits: non-negative doubles order as
-//     unsigned integers) and one thread writes dacc[1-p] (0 after a rebuild,
-//     else dacc[p] + mstep[p]) and rebuilt[1-p];
-//   * a stand-alone move in place drifts copy p: it max-reduces into mtmp,
-//     then folds dacc[p] += mstep[p], mstep[p] = mtmp, mtmp = 0.
-// No kernel of an evaluation writes a word it reads.
+// less than skin/2 since then.  Each particle keeps the length of its path
+// since the last rebuild (single precision rounded up, so it bounds the
+// displacement); a step that takes it past the limit raises the need word of
+// the next evaluation:
+//   * the evaluation reading copy p rebuilds when need[p] is set; its first
+//     kernel clears need[1-p] and counts the rebuild; its integrator tail
+//     (path restarted on a rebuild) raises need[1-p] and one thread writes
+//     rebuilt[1-p];
+//   * a stand-alone move in place of copy p raises need[p].
+// No kernel of an evaluation writes a word it reads, and the need words are
+// written only by the few particles that cross the limit (integer OR).
 struct SkinWords {
-  int32_t forced[2];
+  int32_t need[2];
   int32_t rebuilt[2];  // rebuilt[p]: the evaluation that wrote copy p rebuilt
   int32_t rebuilds;
-  int32_t pad_;
-  double dacc[2];
-  unsigned long long mstep[2];
-  unsigned long long mtmp;
+  int32_t failed;      // a workgroup barrier of the rebuild did not complete
+  uint32_t bar[2];     // barrier of the one-launch rebuild: arrivals, generation
 };
 
 // Rebuild decision of the evaluation reading copy p (the same in every
@@ -88,13 +84,22 @@ struct SkinWords {
 struct SkinGate {
   const SkinWords* w;
   int32_t p;
-  double limit;  // skin / 2, less a relative 1e-6 that dwarfs the rounding
-  __device__ __forceinline__ bool rebuild() const {
-    if (!w) return true;
-    const double bound = w->dacc[p] + __longlong_as_double((long long)w->mstep[p]);
-    return w->forced[p] != 0 || !(bound <= limit);
-  }
+  float limit;  // skin / 2, rounded down
+  __device__ __forceinline__ bool rebuild() const { return !w || w->need[p] != 0; }
+  // the evaluation stops (state left as it was) once a rebuild failed
+  __device__ __forceinline__ bool failed() const { return w && w->failed != 0; }
 };
+
+// Path length of a particle after a drift of length depl, restarted when the
+// lists were rebuilt from the positions before the drift.  The sum is formed
+// in double precision and rounded up to single precision, so the stored
+// length never falls below the true one (the double sum's rounding is far
+// below a single-precision step).
+__device__ __forceinline__ float skin_path(float before, bool restart, double depl) {
+  const double len = restart ? depl : (double)before + depl;
+  const float f = __double2float_ru(len);
+  return restart ? f : nextafterf(f, __builtin_huge_valf());
+}
 
 // Full-precision position source of the binning and sweep kernels: xyz (and
 // the radius in .w when rad is null), the radius in rad[i].x otherwise.
@@ -138,6 +143,39 @@ struct DetectArgs {
   SkinGate gate;
 };
 
+// Contact statistics (AppCollision::addToContactsFeatures, Base/src/
+// AppCollision.cpp:160-190): the overlaps of the contacts (sum, count,
+// largest) and the GJK iterations reported by the closest-point routines
+// (sum, count of those that report some).  Each workgroup of an evaluation
+// writes its part (fixed-order reductions); one kernel adds the parts in
+// workgroup order to the running totals, so the figures are reproducible.
+struct StatPart {
+  double osum, omax;
+  unsigned long long on, gsum, gn;
+};
+struct ContactStats {
+  double osum, omax, tmax, time;  // time: of the current evaluation
+  unsigned long long on, gsum, gn;
+};
+
+// Per-thread accumulator of the statistics.
+struct StatAcc {
+  double osum = 0., omax = 0.;
+  uint32_t on = 0, gsum = 0, gn = 0;
+  __device__ __forceinline__ void add(double dist, int nbIter) {
+    if (nbIter) {
+      gsum += (uint32_t)nbIter;
+      ++gn;
+    }
+    if (dist < 0.) {
+      const double o = fabs(dist);
+      osum += o;
+      ++on;
+      omax = o > omax ? o : omax;
+    }
+  }
+};
+
 // Contact memory of the history models (HookeMemory / HertzMemory): for each
 // particle (slot order, travelling with the state copies) up to kHistCap
 // entries, entry-major like the lists: key (partner id, or -1 - wall) and the
@@ -172,7 +210,7 @@ struct ForceArgs {
   double4* T;
   int32_t* key_out;
   unsigned long long* err;
-  unsigned long long* contacts;  // contact counter
+  unsigned long long* contacts;  // contact counts, one per workgroup (added to)
   ContactRec* rec;               // recording (nullable)
   int32_t* rec_count;
   int32_t rec_cap;
@@ -181,6 +219,7 @@ struct ForceArgs {
   // is reordered through perm every evaluation)
   SkinGate gate;
   SkinWords* words;
+  float* path;                // path length since the last rebuild, slot order
   unsigned long long* cmask;  // screening masks (k_detect_screen), slot order
   // history models: memory read at the input slot, written at the output slot
   History hin, hout;
@@ -192,6 +231,7 @@ struct ForceArgs {
   int32_t* wl_wall;
   double4* wl_f;
   double4* wl_t;
+  StatPart* stat;  // contact statistics of this evaluation (nullable)
 };
 
 constexpr int kWallLoadCap = 4;
@@ -311,11 +351,18 @@ struct Bins {
   float4* pos;          // sorted single-precision copy of pos_r
   int32_t* ticket;      // bsum[nbsum]: blocks of the scan pass finished
   int32_t nbsum;
+  int32_t* rsum;        // cell-range totals of the one-launch rebuild, one per workgroup
 };
 
 // Launchers (all asynchronous on the given stream).
 void launch_keys(const State& s, int32_t n, const Grid& g, const int32_t* row_rank, int32_t* key,
                  unsigned long long* err, hipStream_t st);
+// With skin lists: the whole counting sort in one launch (grid of nblocks
+// workgroups of 1024 threads, all resident), exiting at once when no rebuild
+// is due.  Does the evaluation's first-kernel duties on the skin words.
+void launch_rebuild(Bins& b, const PosSrc& src, int32_t n, const Grid& g, const int32_t* row_rank,
+                    unsigned long long* err, SkinGate gate, SkinWords* words, int32_t* in_count,
+                    int32_t nblocks, hipStream_t st);
 // Counting sort of the positions of src by cell key.  Without skin lists
 // (gate.w null) the keys were written by the previous integrator tail.  With
 // them the binning runs only when the gate says a rebuild is due; it then
@@ -336,12 +383,16 @@ void launch_wall_loads(const ForceArgs& a, double4* part, double4* out, hipStrea
 // Skin lists: candidate sweep (d.gate set) or screening of the lists (writes
 // f.cmask), in one launch.
 void launch_detect_screen(const DetectArgs& d, const ForceArgs& f, hipStream_t st);
-// words (skin lists): the drift bound of copy `parity` is updated (mtmp + fold).
+// gate (skin lists): the path lengths grow and need[gate.p] is raised when
+// one passes the limit.
 void launch_move(const State& s, const double4* F, const double4* T, int32_t n, const Grid& g,
                  const int32_t* row_rank, double dt_vel, double dt_disp, int32_t* key,
-                 unsigned long long* err, hipStream_t st, SkinWords* words = nullptr,
-                 int parity = 0);
-void launch_skin_fold(SkinWords* words, int parity, hipStream_t st);
+                 unsigned long long* err, hipStream_t st, SkinGate gate = SkinGate{},
+                 float* path = nullptr);
+// Adds the nparts workgroup parts of one evaluation to the running totals,
+// the evaluation's time being st->time + dt_step.
+void launch_stats_final(const StatPart* part, int32_t nparts, ContactStats* st, double dt_step,
+                        hipStream_t stream);
 void launch_advance(const State& s, const double4* F, const double4* T, int32_t n,
                     double dt_vel, hipStream_t st);
 void launch_unpermute(const State& s, const double4* F, const double4* T, int32_t n,
@@ -388,7 +439,8 @@ struct PairArgs {
   PairRec* rec;
   const Model* models;
   int32_t model_stride;
-  unsigned long long* contacts;
+  unsigned long long* contacts;  // contact counts, one per workgroup (added to)
+  StatPart* stat;      // contact statistics, one part per workgroup (nullable)
 };
 
 struct IntegArgs {
@@ -415,9 +467,11 @@ struct IntegArgs {
   double4* T;
   SkinGate gate;
   SkinWords* words;
+  float* path;         // path length since the last rebuild, slot order
   unsigned long long* err;
-  unsigned long long* contacts;
+  unsigned long long* contacts;  // per workgroup (walls)
   int32_t nblk8;
+  StatPart* stat;      // wall contacts' statistics, one part per workgroup (nullable)
 };
 
 void launch_to_pack(const State& s, const Pack& pk, const PackConst& k0, const PackConst& k1,
@@ -431,6 +485,6 @@ void launch_integ(int mode, const IntegArgs& a, hipStream_t st, hipEvent_t start
 // Kick + drift of the packed copy in place (the opening half of gdem_step).
 void launch_move_pack(const Pack& pk, const PackConst& k, const double4* F, const double4* T,
                       int32_t n, const Grid& g, double dt_vel, double dt_disp,
-                      unsigned long long* err, SkinWords* words, int parity, hipStream_t st);
+                      unsigned long long* err, SkinGate gate, float* path, hipStream_t st);
 
 }  // namespace gdem

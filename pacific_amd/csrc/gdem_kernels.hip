@@ -68,8 +68,7 @@ __global__ void __launch_bounds__(kBlock) k_count(int32_t* key, int32_t* slot, i
                                                   unsigned long long* err) {
   const bool rebuild = gate.rebuild();
   if (words && blockIdx.x == 0 && threadIdx.x == 0) {
-    words->forced[1 - gate.p] = 0;
-    words->mstep[1 - gate.p] = 0ull;
+    words->need[1 - gate.p] = 0;
     if (rebuild) words->rebuilds += 1;
   }
   if (!rebuild) return;
@@ -234,6 +233,157 @@ __global__ void __launch_bounds__(kBlock) k_cellsort(const int32_t* local, const
     sort_cell(local, bsum, perm, cell_start, src, pos_sorted, c, ncell, n);
 }
 
+// ---------------------------------------------------------------- one-launch rebuild
+constexpr int kRebuildBlock = 1024;
+
+// Barrier of every workgroup of a launch whose workgroups are all resident
+// (no more of them than the device holds at once; the previous launch on the
+// stream has ended).  Generation counting: the last workgroup to arrive resets
+// the count and advances the generation.  The wait is bounded, so every wave
+// always finishes: a workgroup that waited too long marks the rebuild failed
+// and returns false (the evaluation's other kernels then do nothing).
+__device__ bool grid_barrier(SkinWords* w, uint32_t nblocks) {
+  __shared__ int ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t* count = &w->bar[0];
+    uint32_t* gen = &w->bar[1];
+    const uint32_t g0 = __hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();  // this workgroup's writes, visible before it arrives
+    ok = 1;
+    if (atomicAdd(count, 1u) == nblocks - 1u) {
+      __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      long spins = 0;
+      while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g0) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > (1l << 24)) {
+          atomicOr(&w->failed, 1);
+          ok = 0;
+          break;
+        }
+      }
+    }
+    __threadfence();
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+// Sum over the workgroup (1024 threads) of v; every thread gets it.
+__device__ __forceinline__ int block_sum(int v, int* smem) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) smem[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int t = 0;
+  for (int w = 0; w < kRebuildBlock / 64; ++w) t += smem[w];
+  return t;
+}
+
+// The counting sort of launch_binning in one launch: arrival slots (wave-
+// aggregated integer atomics, keys from the positions), barrier, scan of the
+// cell counts (each workgroup a contiguous range of cells; its offset is the
+// sum of the totals of the ranges before it), barrier, scatter, barrier,
+// per-cell sort by previous slot with the single-precision position copy.
+// The permutation equals launch_binning's.
+__global__ void __launch_bounds__(kRebuildBlock) k_rebuild(Bins b, PosSrc src, int n, Grid g,
+                                                           const int32_t* row_rank,
+                                                           unsigned long long* err, SkinGate gate,
+                                                           SkinWords* words, int32_t* in_count) {
+  __shared__ int smem[kRebuildBlock / 64];
+  __shared__ int carry_sh;
+  const bool rebuild = gate.rebuild();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    words->need[1 - gate.p] = 0;
+    if (rebuild) words->rebuilds += 1;
+  }
+  if (!rebuild || gate.failed()) return;
+  const uint32_t nb = gridDim.x;
+  const int lane = threadIdx.x & 63;
+  // 1. keys and arrival slots
+  for (int b0 = blockIdx.x * kRebuildBlock; b0 < n; b0 += nb * kRebuildBlock) {
+    const int i = b0 + threadIdx.x;
+    int k = -1;
+    if (i < n) {
+      int ijk[3];
+      const double4 p = src.pos(i);
+      if (cell_of(g, mk(p.x, p.y, p.z), ijk)) {
+        k = row_base(g, row_rank, ijk[1], ijk[2]) + ijk[0];
+      } else {
+        atomicMin(&err[kErrSim], (unsigned long long)(uint32_t)src.aux[i].id);
+        k = 0;
+      }
+      b.key[i] = k;
+      in_count[i] = 0;
+    }
+    const int kprev = __shfl_up(k, 1, 64);
+    const bool head = (lane == 0) || (k != kprev);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const int head_lane = 63 - __clzll(heads & upto);
+    const unsigned long long after = heads & ~upto;
+    const int next_head = after ? __ffsll((long long)after) - 1 : 64;
+    int base = 0;
+    if (head && k >= 0) base = atomicAdd(&b.cnt[k], next_head - lane);
+    base = __shfl(base, head_lane, 64);
+    if (i < n) b.slot[i] = base + (lane - head_lane);
+  }
+  if (!grid_barrier(words, nb)) return;
+  // 2. scan: workgroup w owns cells [w * chunk, (w + 1) * chunk)
+  const int ncell = g.ncell;
+  const int chunk = (ncell + (int)nb - 1) / (int)nb;
+  const int c0 = min((int)blockIdx.x * chunk, ncell), c1 = min(c0 + chunk, ncell);
+  int part = 0;
+  for (int c = c0 + (int)threadIdx.x; c < c1; c += kRebuildBlock) part += b.cnt[c];
+  const int total = block_sum(part, smem);
+  if (threadIdx.x == 0) b.rsum[blockIdx.x] = total;
+  if (!grid_barrier(words, nb)) return;
+  int pre = 0;
+  for (int w = threadIdx.x; w < (int)blockIdx.x; w += kRebuildBlock) pre += b.rsum[w];
+  const int offset = block_sum(pre, smem);
+  if (threadIdx.x == 0) carry_sh = offset;
+  __syncthreads();
+  for (int base = c0; base < c1; base += kRebuildBlock) {
+    const int c = base + threadIdx.x;
+    const int v = c < c1 ? b.cnt[c] : 0;
+    int tot;
+    const int ex = block_exclusive_scan<kRebuildBlock>(v, smem, tot);
+    const int carry = carry_sh;
+    if (c < c1) {
+      b.cell_start[c] = carry + ex;
+      b.cnt[c] = 0;  // ready for the next rebuild
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) carry_sh = carry + tot;
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) b.cell_start[ncell] = n;
+  if (!grid_barrier(words, nb)) return;
+  // 3. scatter
+  for (int i = blockIdx.x * kRebuildBlock + threadIdx.x; i < n; i += nb * kRebuildBlock)
+    b.perm[b.cell_start[b.key[i]] + b.slot[i]] = i;
+  if (!grid_barrier(words, nb)) return;
+  // 4. each cell in previous-slot order, single-precision position copy
+  for (int c = blockIdx.x * kRebuildBlock + threadIdx.x; c < ncell; c += nb * kRebuildBlock) {
+    const int beg = b.cell_start[c], end = b.cell_start[c + 1];
+    for (int a = beg + 1; a < end; ++a) {
+      const int v = b.perm[a];
+      int q = a - 1;
+      while (q >= beg && b.perm[q] > v) {
+        b.perm[q + 1] = b.perm[q];
+        --q;
+      }
+      b.perm[q + 1] = v;
+    }
+    for (int u = beg; u < end; ++u) {
+      const double4 p = src.pos(b.perm[u]);
+      b.pos[u] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- detect
 __global__ void __launch_bounds__(kBlock) k_detect(DetectArgs a) {
   if (!a.gate.rebuild()) return;
@@ -246,6 +396,7 @@ __global__ void __launch_bounds__(kBlock) k_detect(DetectArgs a) {
 // otherwise the screening of the existing lists for this evaluation (the force
 // kernel walks every candidate after a rebuild).
 __global__ void __launch_bounds__(kBlock) k_detect_screen(DetectArgs d, ForceArgs f) {
+  if (d.gate.failed()) return;
   const int t = logical_index(d.nblk8);
   if (t >= d.n) return;
   if (d.gate.rebuild())
@@ -257,22 +408,25 @@ __global__ void __launch_bounds__(kBlock) k_detect_screen(DetectArgs d, ForceArg
 // ---------------------------------------------------------------- force
 template <int MODE, bool RECORD, bool MEM>
 __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a) {
+  if (a.gate.failed()) return;
   __shared__ unsigned long long sh_cnt[kBlock / 64];
   const int t = logical_index(a.nblk8);
+  __shared__ StatPart sh_stat[kBlock / 64];
   int ncont = 0;
   double drift = 0.;
-  if (t < a.n) ncont = force_particle<MODE, RECORD, MEM>(a, t, drift);
+  StatAcc st;
+  if (t < a.n) ncont = force_particle<MODE, RECORD, MEM>(a, t, drift, st);
+  if (a.stat) block_stats<kBlock>(st, a.stat, sh_stat);
   if (a.words) {
-    // skin words of the copy this evaluation writes (SkinWords)
-    const unsigned long long m = wave_max_bits(drift);
-    if ((threadIdx.x & 63) == 0 && m != 0ull) atomicMax(&a.words->mstep[1 - a.gate.p], m);
-    if (t == 0) {
-      const bool rebuilt = a.gate.rebuild();
-      const int p = a.gate.p;
-      a.words->dacc[1 - p] =
-          rebuilt ? 0. : a.words->dacc[p] + __longlong_as_double((long long)a.words->mstep[p]);
-      a.words->rebuilt[1 - p] = rebuilt ? 1 : 0;
+    // skin lists: path length since the rebuild, need word of the next
+    // evaluation (SkinWords)
+    const bool rebuilt = a.gate.rebuild();
+    if (t < a.n && (MODE == MODE_MOVE || rebuilt)) {
+      const float len = skin_path(rebuilt ? 0.f : a.path[t], rebuilt, drift);
+      a.path[t] = len;
+      if (len > a.gate.limit) atomicOr(&a.words->need[1 - a.gate.p], 1);
     }
+    if (t == 0) a.words->rebuilt[1 - a.gate.p] = rebuilt ? 1 : 0;
   }
   // contact counter: wave sum, then one integer atomic per workgroup
   unsigned long long c = (unsigned long long)ncont;
@@ -282,7 +436,7 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a) {
   if (threadIdx.x == 0) {
     unsigned long long tot = 0;
     for (int w = 0; w < kBlock / 64; ++w) tot += sh_cnt[w];
-    if (tot) atomicAdd(a.contacts, tot);
+    if (tot) a.contacts[blockIdx.x] += tot;  // this workgroup's slot: no atomics
   }
 }
 
@@ -350,23 +504,67 @@ __global__ void __launch_bounds__(256) k_wall_load_final(const double4* part, in
 __global__ void __launch_bounds__(kBlock) k_move(State s, const double4* Fa, const double4* Ta,
                                                  int n, Grid g, const int32_t* row_rank,
                                                  double dtv, double dtd, int32_t* key,
-                                                 unsigned long long* err, SkinWords* words) {
+                                                 unsigned long long* err, SkinGate gate,
+                                                 float* path) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double drift = 0.;
-  if (i < n) drift = move_particle(s, Fa, Ta, i, g, row_rank, dtv, dtd, key, err);
-  if (words) {
-    const unsigned long long m = wave_max_bits(drift);
-    if ((threadIdx.x & 63) == 0 && m != 0ull) atomicMax(&words->mtmp, m);
+  if (i >= n) return;
+  const double drift = move_particle(s, Fa, Ta, i, g, row_rank, dtv, dtd, key, err);
+  if (path) {
+    // a move in place of copy p: the next evaluation reads need[p]
+    const float len = skin_path(path[i], false, drift);
+    path[i] = len;
+    if (len > gate.limit) atomicOr(const_cast<int32_t*>(&gate.w->need[gate.p]), 1);
   }
 }
 
-// After a move in place of copy p: its drift joins the bound (SkinWords).
-__global__ void k_skin_fold(SkinWords* w, int p) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  w->dacc[p] = w->dacc[p] + __longlong_as_double((long long)w->mstep[p]);
-  w->mstep[p] = w->mtmp;
-  w->mtmp = 0ull;
+// Running totals of the contact statistics from the parts of one evaluation,
+// added in workgroup order; the evaluation's time is st->time + dt_step
+// (Grains::Simulation advances the time before the step's force evaluation,
+// Base/src/Grains.cpp:290).  The largest overlap keeps the first time it was
+// reached (AppCollision.cpp:175-179).
+__global__ void __launch_bounds__(256) k_stats_final(const StatPart* part, int nparts,
+                                                    ContactStats* st, double dt_step) {
+  // thread k adds a contiguous range of parts in order, then a fixed tree
+  __shared__ StatPart sh[256];
+  const int k = threadIdx.x;
+  const int per = (nparts + 255) / 256;
+  StatPart a{0., 0., 0ull, 0ull, 0ull};
+  for (int b = k * per; b < min(nparts, (k + 1) * per); ++b) {
+    const StatPart p = part[b];
+    a.osum += p.osum;
+    a.omax = p.omax > a.omax ? p.omax : a.omax;
+    a.on += p.on;
+    a.gsum += p.gsum;
+    a.gn += p.gn;
+  }
+  sh[k] = a;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (k < h) {
+      StatPart& x = sh[k];
+      const StatPart y = sh[k + h];
+      x.osum += y.osum;
+      x.omax = y.omax > x.omax ? y.omax : x.omax;
+      x.on += y.on;
+      x.gsum += y.gsum;
+      x.gn += y.gn;
+    }
+    __syncthreads();
+  }
+  if (k == 0) {
+    const double time = st->time + dt_step;
+    st->time = time;
+    st->osum = st->osum + sh[0].osum;
+    st->on += sh[0].on;
+    st->gsum += sh[0].gsum;
+    st->gn += sh[0].gn;
+    if (sh[0].omax > st->omax) {
+      st->omax = sh[0].omax;
+      st->tmax = time;
+    }
+  }
 }
+
 
 __global__ void __launch_bounds__(kBlock) k_advance(State s, const double4* Fa, const double4* Ta,
                                                     int n, double dtv) {
@@ -470,6 +668,13 @@ void launch_binning(Bins& b, const PosSrc& src, int32_t n, const Grid& g, const 
                      b.local, b.bsum, b.perm, b.cell_start, src, b.pos, ncell, n, gate);
 }
 
+void launch_rebuild(Bins& b, const PosSrc& src, int32_t n, const Grid& g, const int32_t* row_rank,
+                    unsigned long long* err, SkinGate gate, SkinWords* words, int32_t* in_count,
+                    int32_t nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_rebuild, dim3(nblocks), dim3(kRebuildBlock), 0, st, b, src, n, g, row_rank,
+                     err, gate, words, in_count);
+}
+
 void launch_wall_loads(const ForceArgs& a, double4* part, double4* out, hipStream_t st) {
   if (a.nwalls <= 0) return;
   const int nb = a.n > 0 ? nblocks(a.n, kBlock) : 0;
@@ -522,16 +727,17 @@ void launch_force(int mode, bool record, const ForceArgs& a, hipStream_t st, hip
 
 void launch_move(const State& s, const double4* F, const double4* T, int32_t n, const Grid& g,
                  const int32_t* row_rank, double dt_vel, double dt_disp, int32_t* key,
-                 unsigned long long* err, hipStream_t st, SkinWords* words, int parity) {
+                 unsigned long long* err, hipStream_t st, SkinGate gate, float* path) {
   if (n > 0)
     hipLaunchKernelGGL(k_move, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, st, s, F, T, n, g,
-                       row_rank, dt_vel, dt_disp, key, err, words);
-  if (words) launch_skin_fold(words, parity, st);
+                       row_rank, dt_vel, dt_disp, key, err, gate, path);
 }
 
-void launch_skin_fold(SkinWords* words, int parity, hipStream_t st) {
-  hipLaunchKernelGGL(k_skin_fold, dim3(1), dim3(64), 0, st, words, parity);
+void launch_stats_final(const StatPart* part, int32_t nparts, ContactStats* st, double dt_step,
+                        hipStream_t stream) {
+  hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(256), 0, stream, part, nparts, st, dt_step);
 }
+
 
 void launch_advance(const State& s, const double4* F, const double4* T, int32_t n,
                     double dt_vel, hipStream_t st) {

@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(kPB) k_from_pack(Pack pk, PackConst k, State s
 // Candidates of particle t (slot order after the binning of a rebuild; the
 // state is read through perm then), screened and evaluated.  Returns the
 // number of pairs in contact.
-__device__ __forceinline__ int pair_particle(const PairArgs& a, int t, bool rebuild) {
+__device__ __forceinline__ int pair_particle(const PairArgs& a, int t, bool rebuild, StatAcc& st) {
   const DetectArgs& d = a.d;
   const PosSrc src{a.in.A, a.kin.rr, a.kin.aux};
   int nout = rebuild ? detect_particle(d, src, t) : d.out_count[t];
@@ -134,12 +134,16 @@ __device__ __forceinline__ int pair_particle(const PairArgs& a, int t, bool rebu
         B = mk(X.x + (-shx), X.y + (-shy), X.z + (-shz));
         rA = p2.w; rB = r; crA = ax2.crust; crB = ax.crust;
       }
-      const int rc = sphere_sphere(A, rA, crA, B, rB, crB, cp, ov, dist);
+      int rc = sphere_sphere(A, rA, crA, B, rB, crB, cp, ov, dist);
+      int nbIter = 1;  // ClosestPointSPHERESPHERE reports one iteration
+      if (rc < 0) rc = sphere_sphere_retry(A, rA, crA, B, rB, crB, cp, ov, dist, nbIter);
       if (rc < 0) {
         atomicMin(&d.err[kErrContact], first ? pack_ids(ax.id, ax2.id) : pack_ids(ax2.id, ax.id));
         continue;
       }
-      if (rc == 0 || !(dist < 0.)) continue;
+      if (rc == 0) continue;
+      if (a.stat) st.add(dist, nbIter);
+      if (!(dist < 0.)) continue;
       const Model md =
           a.models[(first ? ax.mat : ax2.mat) * a.model_stride + (first ? ax2.mat : ax.mat)];
       const v3 f = first
@@ -170,17 +174,21 @@ __device__ __forceinline__ void count_contacts(unsigned long long* counter, int 
   if (threadIdx.x == 0) {
     unsigned long long tot = 0;
     for (int w = 0; w < kPB / 64; ++w) tot += sh[w];
-    if (tot) atomicAdd(counter, tot);
+    if (tot) counter[blockIdx.x] += tot;  // this workgroup's slot: no atomics
   }
 }
 
 __global__ void __launch_bounds__(kPB) k_pair(PairArgs a) {
   __shared__ unsigned long long sh_cnt[kPB / 64];
+  __shared__ StatPart sh_stat[kPB / 64];
+  if (a.d.gate.failed()) return;
   const int t = logical_index(a.d.nblk8);
   const bool rebuild = a.d.gate.rebuild();
   int nc = 0;
-  if (t < a.d.n) nc = pair_particle(a, t, rebuild);
+  StatAcc st;
+  if (t < a.d.n) nc = pair_particle(a, t, rebuild, st);
   count_contacts(a.contacts, nc, sh_cnt);
+  if (a.stat) block_stats<kPB>(st, a.stat, sh_stat);
 }
 
 // ---------------------------------------------------------------- integrate
@@ -213,7 +221,7 @@ __device__ __forceinline__ bool wrap_axis(const IntegArgs& a, int d, double& x) 
 // Returns its wall contacts; drift = length of its displacement.
 template <int MODE, bool WALLS>
 __device__ __forceinline__ int integ_particle(const IntegArgs& a, int t, int s, bool copy_const,
-                                              double& drift) {
+                                              double& drift, StatAcc& st) {
   const double4 A4 = a.in.A[s], B4 = a.in.B[s], C4 = a.in.C[s], D4 = a.in.D[s];
   const double2 mi = a.kin.mi[s];
   const uint32_t om = a.omask[t];
@@ -261,13 +269,16 @@ __device__ __forceinline__ int integ_particle(const IntegArgs& a, int t, int s, 
         continue;
       v3 c = mk(0., 0., 0.), ov = mk(0., 0., 0.);
       double dist;
-      const int rc = wl.kind == WALL_RECTANGLE ? sphere_rect(wl, X, rr.x, c, ov, dist)
-                                               : sphere_wall(wl, X, rr.x, ax.crust, c, ov, dist);
+      int nbIter = wl.kind == WALL_RECTANGLE ? 0 : 1;  // ClosestPointRECTANGLE reports none
+      int rc = wl.kind == WALL_RECTANGLE ? sphere_rect(wl, X, rr.x, c, ov, dist)
+                                         : sphere_wall(wl, X, rr.x, ax.crust, c, ov, dist);
+      if (rc < 0) rc = sphere_wall_retry(wl, X, rr.x, ax.crust, c, ov, dist, nbIter);
       if (rc < 0) {
         atomicMin(&a.err[kErrContact], pack_ids(ax.id, -1 - wi));
         continue;
       }
       if (rc == 0 || !(dist < 0.)) continue;
+      if (a.stat) st.add(dist, nbIter);
       const Model md = a.models[ax.mat * a.model_stride + wl.material];
       // the wall's velocity at the contact point (Obstacle.cpp:128-133)
       const v3 wo = mk(wl.origin[0], wl.origin[1], wl.origin[2]);
@@ -349,6 +360,7 @@ __device__ __forceinline__ int integ_particle(const IntegArgs& a, int t, int s, 
 
 template <int MODE, bool WALLS>
 __global__ void __launch_bounds__(kPB) k_integ(IntegArgs a) {
+  if (a.gate.failed()) return;
   __shared__ unsigned long long sh_cnt[kPB / 64];
   const int t = logical_index(a.nblk8);
   const int p = a.gate.p;
@@ -356,16 +368,22 @@ __global__ void __launch_bounds__(kPB) k_integ(IntegArgs a) {
   const bool copy_const = rebuild || a.words->rebuilt[p] != 0;
   double drift = 0.;
   int nwc = 0;
-  if (t < a.n) nwc = integ_particle<MODE, WALLS>(a, t, rebuild ? a.perm[t] : t, copy_const, drift);
-  // the drift bound of the copy written here (SkinWords)
-  const unsigned long long mx = wave_max_bits(drift);
-  if ((threadIdx.x & 63) == 0 && mx != 0ull) atomicMax(&a.words->mstep[1 - p], mx);
-  if (t == 0) {
-    a.words->dacc[1 - p] =
-        rebuild ? 0. : a.words->dacc[p] + __longlong_as_double((long long)a.words->mstep[p]);
-    a.words->rebuilt[1 - p] = rebuild ? 1 : 0;
+  StatAcc st;
+  if (t < a.n) {
+    nwc = integ_particle<MODE, WALLS>(a, t, rebuild ? a.perm[t] : t, copy_const, drift, st);
+    // path length since the rebuild; need word of the next evaluation (SkinWords)
+    if (MODE == MODE_MOVE || rebuild) {
+      const float len = skin_path(rebuild ? 0.f : a.path[t], rebuild, drift);
+      a.path[t] = len;
+      if (len > a.gate.limit) atomicOr(&a.words->need[1 - p], 1);
+    }
   }
-  if (WALLS) count_contacts(a.contacts, nwc, sh_cnt);
+  if (t == 0) a.words->rebuilt[1 - p] = rebuild ? 1 : 0;
+  if (WALLS) {
+    count_contacts(a.contacts, nwc, sh_cnt);
+    __shared__ StatPart sh_stat[kPB / 64];
+    if (a.stat) block_stats<kPB>(st, a.stat, sh_stat);
+  }
 }
 
 // Kick + drift of copy pk in place (SecondOrderLeapFrog::Move via
@@ -374,7 +392,7 @@ __global__ void __launch_bounds__(kPB) k_integ(IntegArgs a) {
 __global__ void __launch_bounds__(kPB) k_move_pack(Pack pk, PackConst k, const double4* Fa,
                                                    const double4* Ta, int n, Grid g, double dtv,
                                                    double dtd, unsigned long long* err,
-                                                   SkinWords* words) {
+                                                   SkinGate gate, float* path) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double depl = 0.;
   if (i < n) {
@@ -419,9 +437,11 @@ __global__ void __launch_bounds__(kPB) k_move_pack(Pack pk, PackConst k, const d
     pk.B[i] = make_double4(V.x, V.y, V.z, q.v.y);
     pk.C[i] = make_double4(W.x, W.y, W.z, q.v.z);
     pk.D[i] = make_double4(WBF.x, WBF.y, WBF.z, q.w);
+    // a move in place of copy p: the next evaluation reads need[p]
+    const float len = skin_path(path[i], false, depl);
+    path[i] = len;
+    if (len > gate.limit) atomicOr(const_cast<int32_t*>(&gate.w->need[gate.p]), 1);
   }
-  const unsigned long long mx = wave_max_bits(depl);
-  if ((threadIdx.x & 63) == 0 && mx != 0ull) atomicMax(&words->mtmp, mx);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -472,11 +492,10 @@ void launch_integ(int mode, const IntegArgs& a, hipStream_t st, hipEvent_t start
 
 void launch_move_pack(const Pack& pk, const PackConst& k, const double4* F, const double4* T,
                       int32_t n, const Grid& g, double dt_vel, double dt_disp,
-                      unsigned long long* err, SkinWords* words, int parity, hipStream_t st) {
+                      unsigned long long* err, SkinGate gate, float* path, hipStream_t st) {
   if (n > 0)
     hipLaunchKernelGGL(k_move_pack, dim3(pblocks(n)), dim3(kPB), 0, st, pk, k, F, T, n, g, dt_vel,
-                       dt_disp, err, words);
-  launch_skin_fold(words, parity, st);
+                       dt_disp, err, gate, path);
 }
 
 }  // namespace gdem

@@ -68,7 +68,9 @@ struct Detector {
       const double gap = norm(AB) - (rA + rB);
       if (!(gap < a.skin)) return;
     } else {
-      const int rc = sphere_sphere(A, rA, crA, B, rB, crB, c, ov, dist);
+      int rc = sphere_sphere(A, rA, crA, B, rB, crB, c, ov, dist);
+      int it;
+      if (rc < 0) rc = sphere_sphere_retry(A, rA, crA, B, rB, crB, c, ov, dist, it);
       if (rc < 0) {
         atomicMin(&a.err[kErrContact], first ? pack_ids(ax.id, ax2.id) : pack_ids(ax2.id, ax.id));
         return;
@@ -197,6 +199,7 @@ struct Reducer {
   bool via_perm;  // the input copy is in the previous order (read through perm)
   int s_in;       // input slot of this particle (its memory is read there)
   int nhist;      // memory entries written for this evaluation
+  StatAcc st;     // contact statistics (pairs counted by their first body)
 
   // The memory of the contact with key in this particle's input table
   // (Component::getContactMemory, Component.cpp:634-674, without creating).
@@ -286,14 +289,16 @@ struct Reducer {
       B = mk(X.x + (-shx), X.y + (-shy), X.z + (-shz));
       rA = p2.w; rB = r; crA = ax2.crust; crB = ax.crust;
     }
-    const int rc = sphere_sphere(A, rA, crA, B, rB, crB, c, ov, dist);
-    if (cand) {
-      if (rc < 0) {
-        atomicMin(&a.err[kErrContact], first ? pack_ids(ax.id, ax2.id) : pack_ids(ax2.id, ax.id));
-        return;
-      }
-      if (rc == 0 || !(dist < 0.)) return;
+    int rc = sphere_sphere(A, rA, crA, B, rB, crB, c, ov, dist);
+    int nbIter = 1;  // ClosestPointSPHERESPHERE reports one iteration
+    if (rc < 0) rc = sphere_sphere_retry(A, rA, crA, B, rB, crB, c, ov, dist, nbIter);
+    if (rc < 0) {
+      atomicMin(&a.err[kErrContact], first ? pack_ids(ax.id, ax2.id) : pack_ids(ax2.id, ax.id));
+      return;
     }
+    if (rc == 0) return;
+    if (first && a.stat) st.add(dist, nbIter);
+    if (!(dist < 0.)) return;
     const Model md = a.models[(first ? ax.mat : ax2.mat) * a.model_stride + (first ? ax2.mat : ax.mat)];
     v3 f;
     v3 delM = mk(0., 0., 0.);
@@ -364,7 +369,8 @@ __device__ __forceinline__ void wall_load(const ForceArgs& a, int t, int32_t id,
 // sorted order).  Returns the number of contacts it is the first body of;
 // drift = length of its displacement (0 without one).
 template <int MODE, bool RECORD, bool MEM = false>
-__device__ __forceinline__ int force_particle(const ForceArgs& a, int t, double& drift) {
+__device__ __forceinline__ int force_particle(const ForceArgs& a, int t, double& drift,
+                                              StatAcc& stats) {
   // with skin lists the state moves through perm only after a rebuild;
   // otherwise it stays in its slot
   const bool skin = a.gate.w != nullptr;
@@ -433,13 +439,16 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t, double&
       continue;
     v3 c = mk(0., 0., 0.), ov = mk(0., 0., 0.);
     double dist;
-    const int rc = wl.kind == WALL_RECTANGLE ? sphere_rect(wl, R.X, R.r, c, ov, dist)
-                                             : sphere_wall(wl, R.X, R.r, R.ax.crust, c, ov, dist);
+    int nbIter = wl.kind == WALL_RECTANGLE ? 0 : 1;  // ClosestPointRECTANGLE reports none
+    int rc = wl.kind == WALL_RECTANGLE ? sphere_rect(wl, R.X, R.r, c, ov, dist)
+                                       : sphere_wall(wl, R.X, R.r, R.ax.crust, c, ov, dist);
+    if (rc < 0) rc = sphere_wall_retry(wl, R.X, R.r, R.ax.crust, c, ov, dist, nbIter);
     if (rc < 0) {
       atomicMin(&a.err[kErrContact], pack_ids(R.ax.id, -1 - wi));
       continue;
     }
     if (rc == 0 || !(dist < 0.)) continue;
+    if (a.stat) R.st.add(dist, nbIter);
     const Model md = a.models[R.ax.mat * a.model_stride + wl.material];
     // the wall's velocity at the contact point is vel + omega x (c - origin)
     // (Obstacle::getVelocityAtPoint, Component/src/Obstacle.cpp:128-133)
@@ -468,6 +477,7 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t, double&
     ++R.ncont;
     if (RECORD) R.record(R.ax.id, -1 - wi, c, ov, dist, f);
   }
+  stats = R.st;
   if (RECORD) a.rec_count[t] = R.nrec;
   if (a.wl_count) a.wl_count[t] = min(nwl, kWallLoadCap);
   if (MEM) a.hout.count[t] = min(R.nhist, kHistCap);
@@ -548,6 +558,12 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t, double&
   return R.ncont;
 }
 
+template <int MODE, bool RECORD, bool MEM = false>
+__device__ __forceinline__ int force_particle(const ForceArgs& a, int t, double& drift) {
+  StatAcc st;
+  return force_particle<MODE, RECORD, MEM>(a, t, drift, st);
+}
+
 // SecondOrderLeapFrog::Move (TimeIntegration/src/SecondOrderLeapFrog.cpp:44-67)
 // via ParticleKinematics::Move (Component/src/ParticleKinematics.cpp:62-104) of
 // the particle in storage slot i, in place; writes its new cell key.
@@ -606,14 +622,5 @@ __device__ __forceinline__ double move_particle(const State& s, const double4* F
   return depl;
 }
 
-// Largest value of v over the wave (v >= 0), as unsigned-integer bits.
-__device__ __forceinline__ unsigned long long wave_max_bits(double v) {
-  unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  for (int o = 32; o > 0; o >>= 1) {
-    const unsigned long long c = __shfl_xor(b, o, 64);
-    b = c > b ? c : b;
-  }
-  return b;
-}
 
 }  // namespace gdem

@@ -101,6 +101,35 @@ __device__ __forceinline__ void half_stencil_sweep(const Grid& g, const int32_t*
   }
 }
 
+// Reduces the per-thread statistics of a workgroup of bs threads into its
+// part (butterfly within each wave, then waves in order; every thread calls).
+template <int bs>
+__device__ __forceinline__ void block_stats(const StatAcc& a, StatPart* part, StatPart* sh) {
+  double osum = a.osum, omax = a.omax;
+  unsigned long long on = a.on, gsum = a.gsum, gn = a.gn;
+  for (int o = 32; o > 0; o >>= 1) {
+    osum += __shfl_xor(osum, o, 64);
+    const double m = __shfl_xor(omax, o, 64);
+    omax = m > omax ? m : omax;
+    on += __shfl_xor(on, o, 64);
+    gsum += __shfl_xor(gsum, o, 64);
+    gn += __shfl_xor(gn, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = StatPart{osum, omax, on, gsum, gn};
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    StatPart t = sh[0];
+    for (int w = 1; w < bs / 64; ++w) {
+      t.osum += sh[w].osum;
+      t.omax = sh[w].omax > t.omax ? sh[w].omax : t.omax;
+      t.on += sh[w].on;
+      t.gsum += sh[w].gsum;
+      t.gn += sh[w].gn;
+    }
+    part[blockIdx.x] = t;
+  }
+}
+
 // Next entry of an incoming contact list (entry f at il[f * stride], taken
 // only where bit f of mask is set) in the order (partner slot, code, list
 // position), after the entry of key `last` (-1 before the first): returns its

@@ -138,6 +138,8 @@ def load_library(path: str = LIB_PATH):
         "gdem_get_counters": (C.c_int, [vp, P(i64), P(i64), P(d), P(i64)]),
         "gdem_reset_counters": (C.c_int, [vp]),
         "gdem_get_kernel_times": (C.c_int, [vp, P(d), P(i64)]),
+        "gdem_set_contact_stats": (C.c_int, [vp, i32]),
+        "gdem_get_contact_stats": (C.c_int, [vp, P(d)]),
         "gdem_get_grid": (C.c_int, [vp, P(i32), P(d)]),
         "gdem_version": (C.c_char_p, []),
         "gdem_add_box_shape": (C.c_int, [vp, vp, P(i32)]),
@@ -195,7 +197,8 @@ EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall"
             "gdem_get_time", "gdem_move_walls", "gdem_get_wall", "gdem_write_particles_2014",
             "gdem_read_particles_2014", "gdem_set_contact_recording",
             "gdem_get_contacts", "gdem_last_error", "gdem_set_kernel_timing",
-            "gdem_get_counters", "gdem_reset_counters", "gdem_get_kernel_times", "gdem_get_grid",
+            "gdem_get_counters", "gdem_reset_counters", "gdem_get_kernel_times",
+            "gdem_set_contact_stats", "gdem_get_contact_stats", "gdem_get_grid",
             "gdem_version",
             "gdem_add_box_shape", "gdem_add_polyhedron_shape", "gdem_set_convex_particles",
             "gdem_get_basis", "gdem_batch_create", "gdem_batch_destroy", "gdem_batch_add_system",
@@ -545,6 +548,18 @@ class GdemSystem:
                     "gdem_get_counters")
         return {"force_evals": fe.value, "contacts": ct.value, "force_kernel_ms": ms.value,
                 "timed_launches": tl.value}
+
+    def set_contact_stats(self, on=True):
+        """Switch on (and reset) the contact statistics of the run."""
+        self._check(self.lib.gdem_set_contact_stats(self.h, int(bool(on))),
+                    "gdem_set_contact_stats")
+
+    def contact_stats(self):
+        """The run's contact features, as the oracle's contact_stats()."""
+        out = (C.c_double * 6)()
+        self._check(self.lib.gdem_get_contact_stats(self.h, out), "gdem_get_contact_stats")
+        return {"overlap_max": out[0], "overlap_mean": out[1], "time_overlap_max": out[2],
+                "gjk_mean": out[3], "overlaps": int(out[4]), "gjk_reports": int(out[5])}
 
     def kernel_times(self):
         """Event-timed totals (ms) and launch counts: [force / integrator kernel,

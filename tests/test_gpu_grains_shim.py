@@ -245,6 +245,7 @@ def test_driver_reports_contact_error():
     print("[case] grains_run contact error", flush=True)
     wl = workloads.plain_box(2, seed=1)
     p = wl["particles"]
-    p["x"][1] = p["x"][0] + np.array([2e-3 - 1.5e-4, 0.0, 0.0])
+    # an overlap of ten crust sums: even the retry's shrunk spheres touch
+    p["x"][1] = p["x"][0] + np.array([2e-3 - 1.05e-3, 0.0, 0.0])
     status, id0, id1, _ = run(wl, 1)
     assert status == 1 and {id0, id1} == {0, 1}

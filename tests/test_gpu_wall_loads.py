@@ -95,3 +95,37 @@ def test_loads_reproducible():
         s.close()
     for a, b in zip(out[0], out[1]):
         np.testing.assert_array_equal(a, b)
+
+
+def test_loads_switched_on_after_priming():
+    """Switched on after gdem_prime, the loads are those of the priming
+    evaluation (re-run on the unchanged state), equal to a run that switched
+    them on first; switching on twice keeps them; after a step, or with a
+    memory model, a late switch is refused."""
+    print("[case] wall loads switched on after prime", flush=True)
+    wl = case("memory_box")  # walls touched from the start
+    hooke = (1.0e5, 0.9, -1.0, 0.5, 0.0)
+    wl["models"] = [(0, 0, workloads.MODEL_HOOKE, hooke), (0, 1, workloads.MODEL_HOOKE, hooke)]
+    a = gdem.build_system(wl)
+    a.set_wall_loads(True)
+    a.prime()
+    b = gdem.build_system(wl)
+    b.prime()
+    b.set_wall_loads(True)
+    fa, ta = a.get_wall_loads()
+    fb, tb = b.get_wall_loads()
+    assert np.abs(fa).max() > 0
+    np.testing.assert_array_equal(fa, fb)
+    np.testing.assert_array_equal(ta, tb)
+    b.set_wall_loads(True)  # already on: kept
+    np.testing.assert_array_equal(b.get_wall_loads()[0], fa)
+    a.step(3, wl["dt"])
+    b.step(3, wl["dt"])
+    np.testing.assert_array_equal(a.get_state()["x"], b.get_state()["x"])
+    c = gdem.build_system(wl)
+    c.prime()
+    c.step(1, wl["dt"])
+    with pytest.raises(gdem.GdemError):
+        c.set_wall_loads(True)
+    for s in (a, b, c):
+        s.close()

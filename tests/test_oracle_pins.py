@@ -374,8 +374,9 @@ def test_elastic_frictionless_gas_conserves_energy_dissipative_loses_it(oracle_m
 
 # --------------------------------------------------------------------- errors
 def test_error_paths(oracle_mod):
-    # ContactError: overlap >= sum of crusts
-    wl = two_sphere_wl([0.01, 0.01, 0.01], [0.01 + 2e-3 - 1.2e-4, 0.01, 0.01], [0] * 3, [0] * 3)
+    # ContactError: the overlap reaches ten crust sums, where even the retry's
+    # shrunk spheres touch (an overlap of 1.2 crust sums is retried instead)
+    wl = two_sphere_wl([0.01, 0.01, 0.01], [0.01 + 2e-3 - 1.0e-3, 0.01, 0.01], [0] * 3, [0] * 3)
     o = oracle_mod.Oracle(wl)
     with pytest.raises(oracle_mod.OracleError) as ei:
         o.prime()
