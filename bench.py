@@ -142,6 +142,7 @@ def run_batch(args):
     t = time.time()
     systems = workloads.batch_systems(args.systems, seed=4 if args.seed is None else args.seed)
     ntot = sum(len(w["particles"]["radius"]) for w in systems)
+    nsub = sum(len(w["particles"]["radius"]) * w["substeps"] for w in systems)
     dt = systems[0]["dt"]
     log(f"{len(systems)} systems, {ntot} particles, generated in {time.time() - t:.1f} s")
     cache = os.path.join(os.environ.get("TMPDIR", "/tmp"), "gdem_autotune.json")
@@ -168,12 +169,13 @@ def run_batch(args):
     loop_steps = min(steps, 100)
     tot_ps, tot_t = 0, 0.0
     for wl in sample:
+        sub = wl["substeps"]
         s1 = gdem.build_system(wl)
         s1.prime()
-        s1.step(5, dt)
+        s1.step(5 * sub, dt / sub)
         s1.synchronize()
         t1 = time.perf_counter()
-        s1.step(loop_steps, dt)
+        s1.step(loop_steps * sub, dt / sub)
         s1.synchronize()
         tot_t += time.perf_counter() - t1
         tot_ps += len(wl["particles"]["radius"]) * loop_steps
@@ -186,10 +188,11 @@ def run_batch(args):
         cs = args.cpu_steps or 1000
         cps, cel = 0, 0.0
         for wl in sample:
+            sub = wl["substeps"]
             o = py_oracle.Oracle(wl)
             o.prime()
             t2 = time.perf_counter()
-            o.step(cs, dt)
+            o.step(cs * sub, dt / sub)
             cel += time.perf_counter() - t2
             cps += len(wl["particles"]["radius"]) * cs
         cpu = {"value": cps / cel, "unit": "particle-steps/s", "cores": 1, "kind": "port",
@@ -204,8 +207,11 @@ def run_batch(args):
         "config": {"workload": f"batch_{len(systems)}", "systems": len(systems),
                    "n_particles": ntot, "particles_per_system": [64, 2048],
                    "contact": "Hooke+Coulomb, per-system kn/en/mu", "dt": dt,
+                   "kn_range": [1.0e4, 1.0e6],
+                   "substeps": "per system, omega dt / substeps <= 1 (1 to 6)",
                    "block_size": tuning["block_size"], "cell_factor": tuning["cell_factor"],
                    "parallelism": "single GPU, one persistent launch"},
+        "substep_particle_steps_per_s": nsub * steps / el,
         "looped_single_system": {"value": loop_rate, "sample": f"32 systems x {loop_steps} steps"},
         "speedup_vs_looped": value / loop_rate,
         "autotune": tuning["table"],
@@ -253,27 +259,34 @@ def run_calibration(args, nsys=None):
             jvp_worst = max(jvp_worst, abs(dl[s] - want) / max(abs(want), 1e-300))
     log(f"adjoint vs forward mode: max relative error {jvp_worst:.2e}")
     eng.close()
-    # accuracy 2: central differences (h = 1e-6 relative) of the loss over a
-    # 48-step horizon of the same systems (over 400 steps the colliding packing
-    # makes the loss too rough for differences to resolve 1e-5)
-    short = gdem.GdemDiff(systems[:2], 48, dt, 10, (16, 32, 48))
-    short.set_parameters(hidden[:2], v0[:2000])
+    # accuracy 2: central differences of the loss over a 48-step horizon of 4
+    # of the same systems (over 400 steps the colliding packing makes the loss
+    # far rougher than its exact derivative at any step differences can
+    # take); the step is chosen per component (calibration.central_difference)
+    from pacific_amd import calibration
+    nfd = 4
+    short = gdem.GdemDiff(systems[:nfd], 48, dt, 10, (16, 32, 48))
+    nv = sum(len(w["particles"]["radius"]) for w in systems[:nfd])
+    short.set_parameters(hidden[:nfd], v0[:nv])
     short.set_targets(short.record())
-    short.set_parameters(theta[:2], v0[:2000])
+    short.set_parameters(theta[:nfd], v0[:nv])
     _, sg, _ = short.gradient()
-    worst = 0.0
-    for s in range(2):
+    worst_norm, worst_comp, nres = 0.0, 0.0, 0
+    for s in range(nfd):
+        fd, agree = np.zeros(3), np.zeros(3)
         for c in range(3):
             def f(h):
-                t = theta[:2].copy()
+                t = theta[:nfd].copy()
                 t[s, c] += h
                 short.set_parameters(t)
                 return short.loss()[s]
-            h = 1e-6 * theta[s, c]
-            d = (f(h) - f(-h)) / (2 * h)
-            worst = max(worst, abs(sg[s, c] - d) / max(abs(d), 1e-300))
+            fd[c], agree[c] = calibration.central_difference(f, theta[s, c])
+        short.set_parameters(theta[:nfd])
+        nw, cw, nr = calibration.gradient_error(sg[s], fd, agree, theta[s])
+        worst_norm, worst_comp, nres = max(worst_norm, nw), max(worst_comp, cw), nres + nr
     short.close()
-    log(f"adjoint vs central differences (48 steps): max relative error {worst:.2e}")
+    log(f"adjoint vs central differences (48 steps, {nfd} systems): normwise {worst_norm:.2e}, "
+        f"{nres} of {3 * nfd} components resolved, worst {worst_comp:.2e}")
     # the single-system step looped over systems, gradient by central differences
     nloop = 0 if getattr(args, "no_looped", False) else 4
     t1 = time.perf_counter()
@@ -288,8 +301,9 @@ def run_calibration(args, nsys=None):
                 w["models"] = [(0, 0, gdem.MODEL_HOOKE, hooke), (0, 1, gdem.MODEL_HOOKE, hooke)]
                 one = gdem.build_system(w)
                 last = 0
+                sub = wl["substeps"]
                 for step in obs:
-                    one.step(step - last, dt)
+                    one.step((step - last) * sub, dt / sub)
                     last = step
                     one.get_state()
                 one.close()
@@ -309,8 +323,14 @@ def run_calibration(args, nsys=None):
                    "gradient": "d loss / d (kn, en, mu) and d loss / d v0 per system",
                    "parallelism": "single GPU, one persistent launch"},
         "accuracy": {"vs_forward_mode_max_rel": jvp_worst,
-                     "vs_central_differences_max_rel": worst,
-                     "central_differences": "h = 1e-6 relative, 48-step horizon, 2 systems"},
+                     "vs_central_differences_max_rel": worst_norm,
+                     "vs_central_differences_resolved_components_max_rel": worst_comp,
+                     "central_differences_resolved_components": f"{nres} of {3 * nfd}",
+                     "central_differences": f"48-step horizon, {nfd} systems; step per "
+                                            "component among 1e-4..1e-8 relative (best "
+                                            "agreement of consecutive steps); max_rel is "
+                                            "normwise over theta_i dL/dtheta_i, resolved "
+                                            "components agree to 1e-6 between steps"},
         "looped_single_system_fd": {"value": loop_rate, "unit": "gradients/s",
                                     "sample": f"{nloop} systems, 6 runs of {nsteps} steps each"},
         "speedup_vs_looped": rate / loop_rate if loop_rate else None,

@@ -473,8 +473,9 @@ const char* gdem_version(void);
  * the reference, which runs one Grains application per system
  * (Base/src/Grains.cpp:286-452); each system here is that application's step
  * with its own domain, grid, walls and Hooke/Hertz parameters, and is
- * bit-identical to the same system advanced alone by gdem_step.  Particle
- * material is 0, wall material 1.
+ * bit-identical to the same system advanced alone by gdem_step with its
+ * lists rebuilt at every step (gdem_set_skin(ctx, 0), as the batch does).
+ * Particle material is 0, wall material 1.
  * ------------------------------------------------------------------------ */
 typedef struct gdem_batch gdem_batch;
 
@@ -490,6 +491,11 @@ typedef struct {
   const double* wall_centre;  /* nwalls x 3 */
   const double* wall_half;    /* nwalls x 3, half extents */
   double wall_crust;
+  int32_t substeps;           /* the system advances substeps steps of dt / substeps for
+                                 each step of dt (0 or 1: one); the batch's policy for
+                                 stiff contacts: gdem_step(nsteps * substeps,
+                                 dt / substeps) advances the same system alone */
+  int32_t reserved;
 } gdem_batch_system;
 
 int gdem_batch_create(int32_t device, gdem_batch** out);
@@ -536,7 +542,10 @@ int gdem_diff_create(int32_t device, gdem_diff** out);
 int gdem_diff_destroy(gdem_diff* d);
 int gdem_diff_add_system(gdem_diff* d, const gdem_batch_system* sys, int32_t* sys_id);
 /* nsteps of dt, checkpoints every checkpoint_every steps, observation steps
-   (strictly increasing, in 1..nsteps). */
+   (strictly increasing, in 1..nsteps).  A system added with substeps > 1
+   advances substeps steps of dt / substeps per step (its forward equals
+   gdem_step(nsteps * substeps, dt / substeps)); the substep count is held
+   when the parameters change, as the contact set is. */
 int gdem_diff_commit(gdem_diff* d, int32_t nsteps, double dt, int32_t checkpoint_every,
                      int32_t nobs, const int32_t* obs_steps);
 int gdem_diff_set_launch(gdem_diff* d, int32_t block_size, int32_t workgroups);

@@ -57,3 +57,45 @@ def make_engine(systems, nsteps=400, dt=1.0e-5, checkpoint_every=50, obs_steps=(
 
 def initial_velocities(systems):
     return np.concatenate([np.asarray(w["particles"]["v"], float) for w in systems])
+
+
+def central_difference(f, scale, rel_steps=(1e-4, 1e-5, 1e-6, 1e-7, 1e-8)):
+    """Central differences (f(h) - f(-h)) / 2h of a scalar function at h = 0,
+    for h = r * scale with each r of rel_steps (largest first).  The loss is
+    piecewise smooth (a contact starting or ending, or switching between
+    sticking and Coulomb sliding, puts a kink in it) and carries rounding
+    noise that a small step amplifies, so no single step suits every
+    parameter: the estimate taken is the one whose next smaller step agrees
+    with it best.  Returns (estimate, agreement), agreement being the
+    relative difference of those two estimates: how well differences resolve
+    the derivative there."""
+    ds = []
+    for r in rel_steps:
+        h = r * scale
+        ds.append((f(h) - f(-h)) / (2.0 * h))
+    best, agree = 0, float("inf")
+    for k in range(len(ds) - 1):
+        den = max(abs(ds[k]), abs(ds[k + 1]))
+        a = abs(ds[k] - ds[k + 1]) / den if den > 0 else 0.0
+        if a < agree:
+            best, agree = k, a
+    return ds[best], agree
+
+
+def gradient_error(grad, fd, agree, scale, resolved=1e-6):
+    """Errors of a gradient against central differences: the normwise
+    relative error of the scaled vectors (scale_i * grad_i against
+    scale_i * fd_i, i.e. of the changes of the loss over relative parameter
+    changes), and the largest componentwise relative error among the
+    components the differences resolve (agreement <= resolved).  Returns
+    (normwise, componentwise, number of components resolved)."""
+    g = np.asarray(grad, float) * np.asarray(scale, float)
+    d = np.asarray(fd, float) * np.asarray(scale, float)
+    den = np.linalg.norm(d)
+    normwise = float(np.linalg.norm(g - d) / den) if den > 0 else float(np.linalg.norm(g))
+    ok = np.asarray(agree) <= resolved
+    comp = 0.0
+    for gi, di, oki in zip(np.asarray(grad, float), np.asarray(fd, float), ok):
+        if oki and di != 0.0:
+            comp = max(comp, abs(gi - di) / abs(di))
+    return normwise, comp, int(ok.sum())

@@ -40,7 +40,7 @@ struct BatchSys {
   int32_t row_off;      // first row rank
   int32_t wall_off, nwalls;
   int32_t model_off;    // 2 x 2 model table: particle material 0, wall material 1
-  int32_t pad;
+  int32_t substeps;     // steps of dt / substeps per step of dt
 };
 
 struct BatchArgs {
@@ -265,7 +265,10 @@ struct BatchRunner {
       sst[1] = state_at(a.st[1], sy.off);
     }
     __syncthreads();
-    const double dt = a.dt;
+    // a stiff system advances substeps steps of dt / substeps per step
+    const int sub = sy.substeps;
+    const double dt = a.dt / (double)sub;
+    const int nsteps = a.nsteps * sub;
     ncont = 0;
     if (a.prime) {
       // keys of the initial positions (k_keys), then Link + a0
@@ -286,21 +289,39 @@ struct BatchRunner {
       force<MODE_FORCE>(c0, sst[cur ^ 1], 0., 0., true);
       cur ^= 1;
     }
-    if (a.nsteps > 0) {
+    if (nsteps > 0) {
       // step 1's kick + drift, then each force evaluation closes its step and
       // opens the next one (gdem_step)
       for (int i = threadIdx.x; i < sy.n; i += BS)
         move_particle(sst[cur], a.F + sy.off, a.T + sy.off, i, sy.grid, a.row_rank + sy.row_off,
                       0.5 * dt, dt, a.key + sy.off, err());
       __syncthreads();
-      for (int k = 0; k < a.nsteps; ++k) {
+      for (int k = 0; k < nsteps; ++k) {
         if (failed(smem)) break;
         bin(sst[cur]);
         detect(sst[cur]);
-        if (k == a.nsteps - 1) force<MODE_KICK>(sst[cur], sst[cur ^ 1], 0.5 * dt, dt, true);
+        if (k == nsteps - 1) force<MODE_KICK>(sst[cur], sst[cur ^ 1], 0.5 * dt, dt, true);
         else force<MODE_MOVE>(sst[cur], sst[cur ^ 1], 0.5 * dt, dt, false);
         cur ^= 1;
       }
+    }
+    // the host takes the state from the copy that prime + nsteps evaluations
+    // reach; a system in an even number of substeps ends in the other copy
+    // after an odd number of steps, so its state moves there (same slot
+    // order: keys, F and T stay valid)
+    const int expect = a.cur ^ ((a.prime + a.nsteps) & 1);
+    if (cur != expect) {
+      const State& src = sst[cur];
+      const State& dst = sst[expect];
+      for (int i = threadIdx.x; i < sy.n; i += BS) {
+        dst.pos_r[i] = src.pos_r[i];
+        dst.vel_m[i] = src.vel_m[i];
+        dst.om_req[i] = src.om_req[i];
+        dst.wbf_I[i] = src.wbf_I[i];
+        dst.quat[i] = src.quat[i];
+        dst.aux[i] = src.aux[i];
+      }
+      __syncthreads();
     }
     // per-system contact count: wave sums, one atomic per workgroup
     unsigned long long c = ncont;
@@ -443,7 +464,9 @@ extern "C" int gdem_batch_add_system(gdem_batch* b, const gdem_batch_system* in,
   if (in->nwalls > 0 && (!in->wall_basis || !in->wall_centre || !in->wall_half)) return GDEM_ERR_ARG;
   if (in->kind != GDEM_MODEL_HOOKE && in->kind != GDEM_MODEL_HERTZ) return GDEM_ERR_ARG;
   if (!model_params_ok(in->kind, in->pp) || !model_params_ok(in->kind, in->pw)) return GDEM_ERR_ARG;
+  if (in->substeps < 0 || in->substeps > 4096) return GDEM_ERR_ARG;
   BatchSys sy{};
+  sy.substeps = std::max<int32_t>(in->substeps, 1);
   if (!make_grid(in->cfg, sy.grid)) return GDEM_ERR_ARG;
   sy.margin = prefilter_margin(sy.grid, sy.sizef);
   for (int d = 0; d < 3; ++d) sy.gravity[d] = in->cfg.gravity[d];
@@ -499,8 +522,10 @@ extern "C" int gdem_batch_commit(gdem_batch* b) {
   const size_t ns = std::max<size_t>(b->sys.size(), 1);
   std::vector<int32_t> order(b->sys.size());
   std::iota(order.begin(), order.end(), 0);
+  // largest work first: particles times substeps
+  auto work = [&](int32_t k) { return (int64_t)b->sys[k].n * b->sys[k].substeps; };
   std::stable_sort(order.begin(), order.end(),
-                   [&](int32_t x, int32_t y) { return b->sys[x].n > b->sys[y].n; });
+                   [&](int32_t x, int32_t y) { return work(x) > work(y); });
   BHIP(upload(&b->d_sys, b->sys));
   BHIP(upload(&b->d_order, order));
   BHIP(upload(&b->d_rows, b->rows));

@@ -37,7 +37,6 @@ struct gdem_ctx {
   int32_t* d_row_rank = nullptr;
   int32_t* d_row_jk = nullptr;
   int64_t n = 0;
-  int32_t ncu = 0;  // workgroups of the one-launch rebuild (one per compute unit)
   State buf[2]{};
   int cur = 0;
   double4* d_F = nullptr;
@@ -202,11 +201,6 @@ extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
   const int64_t ncell = c->grid.ncell;
   if (hipSetDevice(c->dev) != hipSuccess) { delete c; return GDEM_ERR_HIP; }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return GDEM_ERR_HIP; }
-  if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->dev) != hipSuccess ||
-      c->ncu < 1) {
-    gdem_destroy(c);
-    return GDEM_ERR_HIP;
-  }
   std::vector<int32_t> rank = row_morton_rank(c->grid.nc);
   const int64_t nrows = (int64_t)c->grid.nc[1] * c->grid.nc[2];
   const int nb = (int)((ncell + kScanCells - 1) / kScanCells);
@@ -218,7 +212,6 @@ extern "C" int gdem_create(const gdem_config* cfg, gdem_ctx** out) {
       hipMalloc(&c->bins.cnt, sizeof(int32_t) * ncell) != hipSuccess ||
       hipMalloc(&c->bins.local, sizeof(int32_t) * ncell) != hipSuccess ||
       hipMalloc(&c->bins.bsum, sizeof(int32_t) * (nb + 1)) != hipSuccess ||
-      hipMalloc(&c->bins.rsum, sizeof(int32_t) * c->ncu) != hipSuccess ||
       hipMalloc(&c->bins.cell_start, sizeof(int32_t) * (ncell + 1)) != hipSuccess ||
       hipMalloc(&c->d_models, sizeof(c->h_models)) != hipSuccess ||
       hipMalloc(&c->d_walls, sizeof(c->h_walls)) != hipSuccess ||
@@ -359,7 +352,6 @@ extern "C" int gdem_destroy(gdem_ctx* c) {
   (void)hipFree(c->bins.key); (void)hipFree(c->bins.slot); (void)hipFree(c->bins.perm); (void)hipFree(c->bins.pos);
   (void)hipFree(c->d_out_list); (void)hipFree(c->d_in_list); (void)hipFree(c->d_out_count); (void)hipFree(c->d_in_count);
   (void)hipFree(c->bins.cnt); (void)hipFree(c->bins.local); (void)hipFree(c->bins.bsum); (void)hipFree(c->bins.cell_start);
-  (void)hipFree(c->bins.rsum);
   (void)hipFree(c->d_row_rank); (void)hipFree(c->d_row_jk); (void)hipFree(c->d_models); (void)hipFree(c->d_walls); (void)hipFree(c->d_err);
   (void)hipFree(c->d_counters); (void)hipFree(c->d_rec); (void)hipFree(c->d_rec_count); (void)hipFree(c->d_stage);
   free_cstate(c->cbuf[0]);
@@ -969,8 +961,8 @@ static int pack_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wri
   const SkinGate gate{c->d_skin, p, skin_limit(c)};
   const PackConst& kin = c->pkc[p];
   const PosSrc src{c->pk[p].A, kin.rr, kin.aux};
-  launch_rebuild(c->bins, src, (int32_t)c->n, c->grid, c->d_row_rank, c->d_err, gate, c->d_skin,
-                 c->d_in_count, c->ncu, c->stream);
+  launch_binning(c->bins, src, (int32_t)c->n, c->grid, c->d_row_rank, c->d_err, c->stream, gate,
+                 c->d_skin, c->d_in_count);
   HIP_TRY(hipGetLastError());
   const int nblk = (int)((c->n + 255) / 256);
   const int nblk8 = ((nblk + 7) / 8) * 8;
@@ -1057,8 +1049,8 @@ static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wr
   if (skin) {
     // rebuild (binning + candidate sweep) only when some particle's path
     // since the last rebuild passed skin/2
-    launch_rebuild(c->bins, canon_src(c->buf[c->cur]), (int32_t)c->n, c->grid, c->d_row_rank,
-                   c->d_err, gate, c->d_skin, c->d_in_count, c->ncu, c->stream);
+    launch_binning(c->bins, canon_src(c->buf[c->cur]), (int32_t)c->n, c->grid, c->d_row_rank,
+                   c->d_err, c->stream, gate, c->d_skin, c->d_in_count);
     HIP_TRY(hipGetLastError());
   } else if (!c->binned) {
     launch_binning(c->bins, canon_src(c->buf[c->cur]), (int32_t)c->n, c->grid, c->d_row_rank,
@@ -1424,18 +1416,6 @@ static int collect_errors(gdem_ctx* c) {
                    (int32_t)(uint32_t)(h[kErrContact] & 0xffffffffull));
   else if (h[kErrCapacity] != kNoErr)
     code = set_err(c, GDEM_ERR_CAPACITY, (int32_t)h[kErrCapacity], -1);
-  if (code == GDEM_OK && c->d_skin) {
-    // a barrier of the one-launch rebuild timed out: the evaluations since
-    // then did nothing
-    SkinWords w{};
-    HIP_TRY(hipMemcpy(&w, c->d_skin, sizeof(w), hipMemcpyDeviceToHost));
-    if (w.failed) {
-      fprintf(stderr, "gdem: a workgroup barrier of the list rebuild did not complete\n");
-      code = set_err(c, GDEM_ERR_HIP, -1, -1);
-      const int rw = reset_skin_words(c);
-      if (rw) return rw;
-    }
-  }
   if (code != GDEM_OK) {
     HIP_TRY(hipMemset(c->d_err, 0xff, sizeof(h)));
   }

@@ -46,7 +46,8 @@ struct DiffSys {
   float sizef[3];
   float margin;
   double gravity[3];
-  int32_t n, off, cell_off, cs_off, row_off, wall_off, nwalls, pad;
+  int32_t n, off, cell_off, cs_off, row_off, wall_off, nwalls;
+  int32_t substeps;  // steps of dt / substeps per step of dt
   double kn, en, mu, beta, dbeta;  // beta(en) and d beta / d en
 };
 
@@ -221,6 +222,10 @@ struct DiffRunner {
   double lsum;      // this thread's share of the loss
   double dlsum;     // tangent mode: share of the directional derivative
   v3 gth;           // this thread's share of d loss / d (kn, en, mu)
+  // a stiff system advances sub steps of dt / sub per step: nst steps of dt
+  // in all, checkpoints every Ks of them, observation k after obs[k] * sub
+  int sub = 1, nst = 0, Ks = 0;
+  double dt = 0.;
 
   __device__ __forceinline__ unsigned long long* err() const { return a.err + (size_t)s * kErrWords; }
 
@@ -434,12 +439,12 @@ struct DiffRunner {
   // First half kick and drift: v_h = v + (F/m) dt/2, w_h = w + (T/I) dt/2,
   // x' = x + v_h dt (ParticleKinematics.cpp:62-104, SecondOrderLeapFrog.cpp:44-67)
   __device__ __forceinline__ void kick_drift() {
-    const double hdt = 0.5 * a.dt;
+    const double hdt = 0.5 * dt;
     for (int i = threadIdx.x; i < sy.n; i += BS) {
       const double4 pp = P()[i];
       const v3 vh = add(ld3(V()[i]), scl(dvd(ld3(Fo()[i]), pp.y * 1.), hdt));
       const v3 wh = add(ld3(W()[i]), scl(dvd(ld3(To()[i]), pp.z), hdt));
-      const v3 dx = scl(vh, a.dt);
+      const v3 dx = scl(vh, dt);
       const double4 xo = X()[i];
       X()[i] = st3(add(mk(xo.x, xo.y, xo.z), dx), xo.w);
       V()[i] = st3(vh, 0.);
@@ -451,7 +456,7 @@ struct DiffRunner {
 
   // Closing half kick (SecondOrderLeapFrog.cpp:74-94)
   __device__ __forceinline__ void kick() {
-    const double hdt = 0.5 * a.dt;
+    const double hdt = 0.5 * dt;
     for (int i = threadIdx.x; i < sy.n; i += BS) {
       const double4 pp = P()[i];
       V()[i] = st3(add(ld3(V()[i]), scl(dvd(ld3(Fo()[i]), pp.y * 1.), hdt)), 0.);
@@ -461,8 +466,9 @@ struct DiffRunner {
   }
 
   __device__ __forceinline__ int obs_index(int step) const {
+    if (step % sub != 0) return -1;
     for (int k = 0; k < a.nobs; ++k)
-      if (a.obs[k] == step) return k;
+      if (a.obs[k] * sub == step) return k;
     return -1;
   }
 
@@ -553,13 +559,13 @@ struct DiffRunner {
 
   // tangent of the first kick and the drift (uses the tangents of F, T at n)
   __device__ __forceinline__ void tangent_kick_drift() {
-    const double hdt = 0.5 * a.dt;
+    const double hdt = 0.5 * dt;
     for (int i = threadIdx.x; i < sy.n; i += BS) {
       const double4 pp = P()[i];
       const size_t g = sy.off + i;
       const v3 tvh = add(ld3(a.av[g]), scl(ld3(a.aF[g]), hdt / pp.y));
       const v3 twh = add(ld3(a.aw[g]), scl(ld3(a.aT[g]), hdt / pp.z));
-      a.ax[g] = st3(add(ld3(a.ax[g]), scl(tvh, a.dt)), 0.);
+      a.ax[g] = st3(add(ld3(a.ax[g]), scl(tvh, dt)), 0.);
       a.av[g] = st3(tvh, 0.);
       a.aw[g] = st3(twh, 0.);
     }
@@ -567,7 +573,7 @@ struct DiffRunner {
   }
 
   __device__ __forceinline__ void tangent_kick() {
-    const double hdt = 0.5 * a.dt;
+    const double hdt = 0.5 * dt;
     for (int i = threadIdx.x; i < sy.n; i += BS) {
       const double4 pp = P()[i];
       const size_t g = sy.off + i;
@@ -696,7 +702,7 @@ struct DiffRunner {
   // Adjoint of one step n -> n+1 whose force inputs are in the state arrays;
   // adjoint arrays hold the adjoint at boundary n+1 on entry, at n on exit.
   __device__ __forceinline__ void step_adjoint(int n1, double lscale) {
-    const double hdt = 0.5 * a.dt;
+    const double hdt = 0.5 * dt;
     double4 *ax = a.ax + sy.off, *av = a.av + sy.off, *aw = a.aw + sy.off, *aF = a.aF + sy.off,
             *aT = a.aT + sy.off;
     const int k = obs_index(n1);
@@ -720,7 +726,7 @@ struct DiffRunner {
     for (int i = threadIdx.x; i < sy.n; i += BS) {
       const double4 pp = P()[i];
       const v3 xt = add(ld3(ax[i]), ld3(a.gX[sy.off + i]));
-      const v3 vt = add(add(ld3(av[i]), ld3(a.gV[sy.off + i])), scl(xt, a.dt));
+      const v3 vt = add(add(ld3(av[i]), ld3(a.gV[sy.off + i])), scl(xt, dt));
       const v3 wt = add(ld3(aw[i]), ld3(a.gW[sy.off + i]));
       ax[i] = st3(xt, 0.);
       av[i] = st3(vt, 0.);
@@ -744,6 +750,10 @@ struct DiffRunner {
     lsum = 0.;
     dlsum = 0.;
     gth = mk(0., 0., 0.);
+    sub = sy.substeps;
+    dt = a.dt / (double)sub;
+    nst = a.nsteps * sub;
+    Ks = a.K * sub;
     const double lscale = 1. / ((double)a.nobs * 3. * (double)sy.n);
     initial_state();
     bin_and_detect();
@@ -762,11 +772,11 @@ struct DiffRunner {
     forces();
     if (a.mode == DIFF_GRAD) save(a.ck_x, a.ck_v, a.ck_w, a.ck_F, a.ck_T);
     bool ok = true;
-    for (int n = 0; n < a.nsteps; ++n) {
+    for (int n = 0; n < nst; ++n) {
       if (failed()) { ok = false; break; }
       step(n + 1, -1);
-      if (a.mode == DIFF_GRAD && (n + 1) % a.K == 0 && n + 1 < a.nsteps) {
-        const size_t c = (size_t)((n + 1) / a.K) * a.ntot;
+      if (a.mode == DIFF_GRAD && (n + 1) % Ks == 0 && n + 1 < nst) {
+        const size_t c = (size_t)((n + 1) / Ks) * a.ntot;
         save(a.ck_x + c, a.ck_v + c, a.ck_w + c, a.ck_F + c, a.ck_T + c);
       }
     }
@@ -784,10 +794,10 @@ struct DiffRunner {
       a.aF[sy.off + i] = z; a.aT[sy.off + i] = z;
     }
     __syncthreads();
-    const int nwin = (a.nsteps + a.K - 1) / a.K;
+    const int nwin = (nst + Ks - 1) / Ks;
     for (int wdx = nwin - 1; wdx >= 0; --wdx) {
-      const int w0 = wdx * a.K;
-      const int len = min(a.K, a.nsteps - w0);
+      const int w0 = wdx * Ks;
+      const int len = min(Ks, nst - w0);
       const size_t c = (size_t)wdx * a.ntot;
       restore(a.ck_x + c, a.ck_v + c, a.ck_w + c, a.ck_F + c, a.ck_T + c);
       for (int j = 0; j < len; ++j) step(w0 + j + 1, j);
@@ -854,6 +864,7 @@ struct gdem_diff {
   bool committed = false;
   int32_t block = 256, workgroups = 0;
   int32_t nsteps = 0, K = 50, nobs = 0;
+  int64_t win_steps = 0;  // window slots: K times the largest substep count
   double dt = 0.;
   std::vector<int32_t> obs;
   std::vector<DiffSys> sys;
@@ -939,7 +950,9 @@ extern "C" int gdem_diff_add_system(gdem_diff* d, const gdem_batch_system* in, i
     if (in->pw[k] != in->pp[k]) return GDEM_ERR_ARG;  // one parameter set per system
   if (in->nwalls < 0 || in->nwalls > kMaxWalls) return GDEM_ERR_ARG;
   if (in->nwalls > 0 && (!in->wall_basis || !in->wall_centre || !in->wall_half)) return GDEM_ERR_ARG;
+  if (in->substeps < 0 || in->substeps > 4096) return GDEM_ERR_ARG;
   DiffSys sy{};
+  sy.substeps = std::max<int32_t>(in->substeps, 1);
   if (!make_grid(in->cfg, sy.grid)) return GDEM_ERR_ARG;
   sy.margin = prefilter_margin(sy.grid, sy.sizef);
   for (int k = 0; k < 3; ++k) sy.gravity[k] = in->cfg.gravity[k];
@@ -1005,8 +1018,12 @@ extern "C" int gdem_diff_commit(gdem_diff* d, int32_t nsteps, double dt, int32_t
   const size_t nck = (size_t)((nsteps + d->K - 1) / d->K);
   std::vector<int32_t> order(d->sys.size());
   std::iota(order.begin(), order.end(), 0);
+  // largest work first: particles times substeps
+  int32_t smax = 1;
+  for (const DiffSys& sy : d->sys) smax = std::max(smax, sy.substeps);
+  auto work = [&](int32_t k) { return (int64_t)d->sys[k].n * d->sys[k].substeps; };
   std::stable_sort(order.begin(), order.end(),
-                   [&](int32_t x, int32_t y) { return d->sys[x].n > d->sys[y].n; });
+                   [&](int32_t x, int32_t y) { return work(x) > work(y); });
   DHIP(dupload(&d->d_sys, d->sys));
   DHIP(dupload(&d->d_order, order));
   DHIP(dupload(&d->d_obs, d->obs));
@@ -1024,7 +1041,9 @@ extern "C" int gdem_diff_commit(gdem_diff* d, int32_t nsteps, double dt, int32_t
   d->d_F = d->d_w + N;
   d->d_T = d->d_F + N;
   DHIP(hipMalloc(&d->d_ck, sizeof(double4) * N * nck * 5));
-  DHIP(hipMalloc(&d->d_win, sizeof(double4) * N * (size_t)d->K * 3));
+  // window of K steps of the stiffest system: K * smax force inputs
+  DHIP(hipMalloc(&d->d_win, sizeof(double4) * N * (size_t)d->K * (size_t)smax * 3));
+  d->win_steps = (int64_t)d->K * smax;
   DHIP(hipMalloc(&d->d_target, sizeof(double4) * N * (size_t)nobs));
   DHIP(hipMemset(d->d_target, 0, sizeof(double4) * N * (size_t)nobs));
   DHIP(hipMalloc(&d->d_xobs, sizeof(double4) * N * (size_t)nobs));
@@ -1150,8 +1169,8 @@ static int diff_launch(gdem_diff* d, int mode, const double* dtheta = nullptr,
   a.ck_F = a.ck_w + N * nck;
   a.ck_T = a.ck_F + N * nck;
   a.win_x = d->d_win;
-  a.win_v = a.win_x + N * (size_t)d->K;
-  a.win_w = a.win_v + N * (size_t)d->K;
+  a.win_v = a.win_x + N * (size_t)d->win_steps;
+  a.win_w = a.win_v + N * (size_t)d->win_steps;
   a.target = d->d_target;
   a.xobs = d->d_xobs;
   a.key = d->d_key; a.slot = d->d_slot; a.perm = d->d_perm; a.cnt = d->d_cnt; a.cell_start = d->d_cs;

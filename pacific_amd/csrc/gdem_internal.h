@@ -74,8 +74,6 @@ struct SkinWords {
   int32_t need[2];
   int32_t rebuilt[2];  // rebuilt[p]: the evaluation that wrote copy p rebuilt
   int32_t rebuilds;
-  int32_t failed;      // a workgroup barrier of the rebuild did not complete
-  uint32_t bar[2];     // barrier of the one-launch rebuild: arrivals, generation
 };
 
 // Rebuild decision of the evaluation reading copy p (the same in every
@@ -86,8 +84,6 @@ struct SkinGate {
   int32_t p;
   float limit;  // skin / 2, rounded down
   __device__ __forceinline__ bool rebuild() const { return !w || w->need[p] != 0; }
-  // the evaluation stops (state left as it was) once a rebuild failed
-  __device__ __forceinline__ bool failed() const { return w && w->failed != 0; }
 };
 
 // Path length of a particle after a drift of length depl, restarted when the
@@ -351,18 +347,11 @@ struct Bins {
   float4* pos;          // sorted single-precision copy of pos_r
   int32_t* ticket;      // bsum[nbsum]: blocks of the scan pass finished
   int32_t nbsum;
-  int32_t* rsum;        // cell-range totals of the one-launch rebuild, one per workgroup
 };
 
 // Launchers (all asynchronous on the given stream).
 void launch_keys(const State& s, int32_t n, const Grid& g, const int32_t* row_rank, int32_t* key,
                  unsigned long long* err, hipStream_t st);
-// With skin lists: the whole counting sort in one launch (grid of nblocks
-// workgroups of 1024 threads, all resident), exiting at once when no rebuild
-// is due.  Does the evaluation's first-kernel duties on the skin words.
-void launch_rebuild(Bins& b, const PosSrc& src, int32_t n, const Grid& g, const int32_t* row_rank,
-                    unsigned long long* err, SkinGate gate, SkinWords* words, int32_t* in_count,
-                    int32_t nblocks, hipStream_t st);
 // Counting sort of the positions of src by cell key.  Without skin lists
 // (gate.w null) the keys were written by the previous integrator tail.  With
 // them the binning runs only when the gate says a rebuild is due; it then

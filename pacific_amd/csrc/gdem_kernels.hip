@@ -233,157 +233,6 @@ __global__ void __launch_bounds__(kBlock) k_cellsort(const int32_t* local, const
     sort_cell(local, bsum, perm, cell_start, src, pos_sorted, c, ncell, n);
 }
 
-// ---------------------------------------------------------------- one-launch rebuild
-constexpr int kRebuildBlock = 1024;
-
-// Barrier of every workgroup of a launch whose workgroups are all resident
-// (no more of them than the device holds at once; the previous launch on the
-// stream has ended).  Generation counting: the last workgroup to arrive resets
-// the count and advances the generation.  The wait is bounded, so every wave
-// always finishes: a workgroup that waited too long marks the rebuild failed
-// and returns false (the evaluation's other kernels then do nothing).
-__device__ bool grid_barrier(SkinWords* w, uint32_t nblocks) {
-  __shared__ int ok;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t* count = &w->bar[0];
-    uint32_t* gen = &w->bar[1];
-    const uint32_t g0 = __hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();  // this workgroup's writes, visible before it arrives
-    ok = 1;
-    if (atomicAdd(count, 1u) == nblocks - 1u) {
-      __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      long spins = 0;
-      while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g0) {
-        __builtin_amdgcn_s_sleep(2);
-        if (++spins > (1l << 24)) {
-          atomicOr(&w->failed, 1);
-          ok = 0;
-          break;
-        }
-      }
-    }
-    __threadfence();
-  }
-  __syncthreads();
-  return ok != 0;
-}
-
-// Sum over the workgroup (1024 threads) of v; every thread gets it.
-__device__ __forceinline__ int block_sum(int v, int* smem) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) smem[threadIdx.x >> 6] = v;
-  __syncthreads();
-  int t = 0;
-  for (int w = 0; w < kRebuildBlock / 64; ++w) t += smem[w];
-  return t;
-}
-
-// The counting sort of launch_binning in one launch: arrival slots (wave-
-// aggregated integer atomics, keys from the positions), barrier, scan of the
-// cell counts (each workgroup a contiguous range of cells; its offset is the
-// sum of the totals of the ranges before it), barrier, scatter, barrier,
-// per-cell sort by previous slot with the single-precision position copy.
-// The permutation equals launch_binning's.
-__global__ void __launch_bounds__(kRebuildBlock) k_rebuild(Bins b, PosSrc src, int n, Grid g,
-                                                           const int32_t* row_rank,
-                                                           unsigned long long* err, SkinGate gate,
-                                                           SkinWords* words, int32_t* in_count) {
-  __shared__ int smem[kRebuildBlock / 64];
-  __shared__ int carry_sh;
-  const bool rebuild = gate.rebuild();
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    words->need[1 - gate.p] = 0;
-    if (rebuild) words->rebuilds += 1;
-  }
-  if (!rebuild || gate.failed()) return;
-  const uint32_t nb = gridDim.x;
-  const int lane = threadIdx.x & 63;
-  // 1. keys and arrival slots
-  for (int b0 = blockIdx.x * kRebuildBlock; b0 < n; b0 += nb * kRebuildBlock) {
-    const int i = b0 + threadIdx.x;
-    int k = -1;
-    if (i < n) {
-      int ijk[3];
-      const double4 p = src.pos(i);
-      if (cell_of(g, mk(p.x, p.y, p.z), ijk)) {
-        k = row_base(g, row_rank, ijk[1], ijk[2]) + ijk[0];
-      } else {
-        atomicMin(&err[kErrSim], (unsigned long long)(uint32_t)src.aux[i].id);
-        k = 0;
-      }
-      b.key[i] = k;
-      in_count[i] = 0;
-    }
-    const int kprev = __shfl_up(k, 1, 64);
-    const bool head = (lane == 0) || (k != kprev);
-    const unsigned long long heads = __ballot(head);
-    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-    const int head_lane = 63 - __clzll(heads & upto);
-    const unsigned long long after = heads & ~upto;
-    const int next_head = after ? __ffsll((long long)after) - 1 : 64;
-    int base = 0;
-    if (head && k >= 0) base = atomicAdd(&b.cnt[k], next_head - lane);
-    base = __shfl(base, head_lane, 64);
-    if (i < n) b.slot[i] = base + (lane - head_lane);
-  }
-  if (!grid_barrier(words, nb)) return;
-  // 2. scan: workgroup w owns cells [w * chunk, (w + 1) * chunk)
-  const int ncell = g.ncell;
-  const int chunk = (ncell + (int)nb - 1) / (int)nb;
-  const int c0 = min((int)blockIdx.x * chunk, ncell), c1 = min(c0 + chunk, ncell);
-  int part = 0;
-  for (int c = c0 + (int)threadIdx.x; c < c1; c += kRebuildBlock) part += b.cnt[c];
-  const int total = block_sum(part, smem);
-  if (threadIdx.x == 0) b.rsum[blockIdx.x] = total;
-  if (!grid_barrier(words, nb)) return;
-  int pre = 0;
-  for (int w = threadIdx.x; w < (int)blockIdx.x; w += kRebuildBlock) pre += b.rsum[w];
-  const int offset = block_sum(pre, smem);
-  if (threadIdx.x == 0) carry_sh = offset;
-  __syncthreads();
-  for (int base = c0; base < c1; base += kRebuildBlock) {
-    const int c = base + threadIdx.x;
-    const int v = c < c1 ? b.cnt[c] : 0;
-    int tot;
-    const int ex = block_exclusive_scan<kRebuildBlock>(v, smem, tot);
-    const int carry = carry_sh;
-    if (c < c1) {
-      b.cell_start[c] = carry + ex;
-      b.cnt[c] = 0;  // ready for the next rebuild
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) carry_sh = carry + tot;
-    __syncthreads();
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) b.cell_start[ncell] = n;
-  if (!grid_barrier(words, nb)) return;
-  // 3. scatter
-  for (int i = blockIdx.x * kRebuildBlock + threadIdx.x; i < n; i += nb * kRebuildBlock)
-    b.perm[b.cell_start[b.key[i]] + b.slot[i]] = i;
-  if (!grid_barrier(words, nb)) return;
-  // 4. each cell in previous-slot order, single-precision position copy
-  for (int c = blockIdx.x * kRebuildBlock + threadIdx.x; c < ncell; c += nb * kRebuildBlock) {
-    const int beg = b.cell_start[c], end = b.cell_start[c + 1];
-    for (int a = beg + 1; a < end; ++a) {
-      const int v = b.perm[a];
-      int q = a - 1;
-      while (q >= beg && b.perm[q] > v) {
-        b.perm[q + 1] = b.perm[q];
-        --q;
-      }
-      b.perm[q + 1] = v;
-    }
-    for (int u = beg; u < end; ++u) {
-      const double4 p = src.pos(b.perm[u]);
-      b.pos[u] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
-    }
-  }
-}
-
 // ---------------------------------------------------------------- detect
 __global__ void __launch_bounds__(kBlock) k_detect(DetectArgs a) {
   if (!a.gate.rebuild()) return;
@@ -396,7 +245,6 @@ __global__ void __launch_bounds__(kBlock) k_detect(DetectArgs a) {
 // otherwise the screening of the existing lists for this evaluation (the force
 // kernel walks every candidate after a rebuild).
 __global__ void __launch_bounds__(kBlock) k_detect_screen(DetectArgs d, ForceArgs f) {
-  if (d.gate.failed()) return;
   const int t = logical_index(d.nblk8);
   if (t >= d.n) return;
   if (d.gate.rebuild())
@@ -408,7 +256,6 @@ __global__ void __launch_bounds__(kBlock) k_detect_screen(DetectArgs d, ForceArg
 // ---------------------------------------------------------------- force
 template <int MODE, bool RECORD, bool MEM>
 __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a) {
-  if (a.gate.failed()) return;
   __shared__ unsigned long long sh_cnt[kBlock / 64];
   const int t = logical_index(a.nblk8);
   __shared__ StatPart sh_stat[kBlock / 64];
@@ -666,13 +513,6 @@ void launch_binning(Bins& b, const PosSrc& src, int32_t n, const Grid& g, const 
                        b.slot, b.local, b.bsum, b.perm, n, gate);
   hipLaunchKernelGGL(k_cellsort, dim3(capped(nblocks(ncell, kBlock))), dim3(kBlock), 0, st,
                      b.local, b.bsum, b.perm, b.cell_start, src, b.pos, ncell, n, gate);
-}
-
-void launch_rebuild(Bins& b, const PosSrc& src, int32_t n, const Grid& g, const int32_t* row_rank,
-                    unsigned long long* err, SkinGate gate, SkinWords* words, int32_t* in_count,
-                    int32_t nblocks, hipStream_t st) {
-  hipLaunchKernelGGL(k_rebuild, dim3(nblocks), dim3(kRebuildBlock), 0, st, b, src, n, g, row_rank,
-                     err, gate, words, in_count);
 }
 
 void launch_wall_loads(const ForceArgs& a, double4* part, double4* out, hipStream_t st) {

@@ -181,7 +181,6 @@ __device__ __forceinline__ void count_contacts(unsigned long long* counter, int 
 __global__ void __launch_bounds__(kPB) k_pair(PairArgs a) {
   __shared__ unsigned long long sh_cnt[kPB / 64];
   __shared__ StatPart sh_stat[kPB / 64];
-  if (a.d.gate.failed()) return;
   const int t = logical_index(a.d.nblk8);
   const bool rebuild = a.d.gate.rebuild();
   int nc = 0;
@@ -360,7 +359,6 @@ __device__ __forceinline__ int integ_particle(const IntegArgs& a, int t, int s, 
 
 template <int MODE, bool WALLS>
 __global__ void __launch_bounds__(kPB) k_integ(IntegArgs a) {
-  if (a.gate.failed()) return;
   __shared__ unsigned long long sh_cnt[kPB / 64];
   const int t = logical_index(a.nblk8);
   const int p = a.gate.p;

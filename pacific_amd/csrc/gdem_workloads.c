@@ -11,18 +11,39 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <limits.h>
+#include <omp.h>
 
 #define GDW_PI 3.14159265358979323846
 
-static uint64_t splitmix64(uint64_t* s) {
-  uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+#define GDW_GAMMA 0x9E3779B97F4A7C15ULL
+
+static uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
   return z ^ (z >> 31);
 }
 
+static uint64_t splitmix64(uint64_t* s) { return mix64(*s += GDW_GAMMA); }
+
 /* uniform in [0, 1) with 53 random bits */
 static double u01(uint64_t* s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+/* The k-th draw (k >= 1) of u01 on a stream that starts at state s0, without
+   stepping the stream: SplitMix64's state advances by a constant, so every
+   draw is addressable and random sequential addition can test attempts in
+   parallel and keep the first that succeeds (the result equals the serial
+   addition's). */
+static double u01_at(uint64_t s0, uint64_t k) {
+  return (double)(mix64(s0 + k * GDW_GAMMA) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/* Attempts tested per parallel round of the additions: about twice the
+   attempts the previous particle needed, between lo and hi. */
+static long next_batch(long used, long lo, long hi) {
+  long b = 2 * used;
+  return b < lo ? lo : (b > hi ? hi : b);
+}
 
 static double normal01(uint64_t* s) {
   double u1 = u01(s), u2 = u01(s);
@@ -65,7 +86,49 @@ double gdw_radii(int n, double rmin, double rmax, double phi, uint64_t seed, dou
 
 /* Random sequential addition of n spheres (radii given) in the triply periodic
    cube [0, L)^3 with minimum-image overlap rejection; velocities N(0, sigma_v^2).
+   Attempt a (counted from 0 over the whole run) draws the centre from draws
+   3a+1..3a+3 of the stream; attempts are tested in parallel rounds and the
+   first success is kept, so the result does not depend on the thread count.
    Returns 0, or -1 if max_attempts is exhausted. */
+typedef struct {
+  const double* r;
+  const double* x;
+  const int* head;
+  const int* next;
+  double L, h;
+  int nc;
+  uint64_t s0;
+} rsa_sph;
+
+static int sphere_fits(const rsa_sph* R, int p, long a, double* c, long* cc) {
+  int ic[3];
+  for (int d = 0; d < 3; ++d) {
+    c[d] = R->L * u01_at(R->s0, (uint64_t)(3 * a + d + 1));
+    ic[d] = (int)(c[d] / R->h);
+    if (ic[d] >= R->nc) ic[d] = R->nc - 1;
+  }
+  const int nc = R->nc;
+  const double L = R->L;
+  for (int dk = -1; dk <= 1; ++dk)
+    for (int dj = -1; dj <= 1; ++dj)
+      for (int di = -1; di <= 1; ++di) {
+        int ia = (ic[0] + di + nc) % nc, ib = (ic[1] + dj + nc) % nc, ie = (ic[2] + dk + nc) % nc;
+        for (int q = R->head[((long)ie * nc + ib) * nc + ia]; q >= 0; q = R->next[q]) {
+          double d2 = 0.0;
+          for (int d = 0; d < 3; ++d) {
+            double dd = R->x[3 * q + d] - c[d];
+            if (dd > 0.5 * L) dd -= L;
+            if (dd < -0.5 * L) dd += L;
+            d2 += dd * dd;
+          }
+          double sr = R->r[p] + R->r[q];
+          if (d2 <= sr * sr * (1.0 + 1e-9)) return 0;
+        }
+      }
+  *cc = ((long)ic[2] * nc + ic[1]) * nc + ic[0];
+  return 1;
+}
+
 int gdw_rsa_periodic(int n, const double* r, double L, double sigma_v, uint64_t seed,
                      long max_attempts, double* x, double* v) {
   double rmax = 0.0;
@@ -78,43 +141,46 @@ int gdw_rsa_periodic(int n, const double* r, double L, double sigma_v, uint64_t 
   int* next = (int*)malloc(sizeof(int) * (size_t)n);
   if (!head || !next) { free(head); free(next); return -1; }
   for (long c = 0; c < ncell; ++c) head[c] = -1;
-  uint64_t st = seed * 0xD1B54A32D192ED03ULL + 3;
-  long attempts = 0;
+  rsa_sph R = {r, x, head, next, L, h, nc, seed * 0xD1B54A32D192ED03ULL + 3};
+  const long nth = omp_get_max_threads();
+  long att = 0;     /* attempts used so far */
+  long batch = 1;
   for (int p = 0; p < n; ++p) {
-    for (;;) {
-      if (++attempts > max_attempts) { free(head); free(next); return -1; }
-      double c[3];
-      int ic[3];
-      for (int d = 0; d < 3; ++d) {
-        c[d] = L * u01(&st);
-        ic[d] = (int)(c[d] / h);
-        if (ic[d] >= nc) ic[d] = nc - 1;
+    const long first = att;
+    long found = -1;
+    double c[3];
+    long cc = 0;
+    while (found < 0) {
+      if (att >= max_attempts) { free(head); free(next); return -1; }
+      const long hi = att + batch < max_attempts ? att + batch : max_attempts;
+      if (hi - att < 2 * nth) {
+        for (long a = att; a < hi; ++a)
+          if (sphere_fits(&R, p, a, c, &cc)) { found = a; break; }
+      } else {
+        long best = LONG_MAX;
+#pragma omp parallel for schedule(static) reduction(min : best)
+        for (long a = att; a < hi; ++a) {
+          double ct[3];
+          long ct_cell;
+          if (a < best && sphere_fits(&R, p, a, ct, &ct_cell)) best = a;
+        }
+        if (best != LONG_MAX) {
+          found = best;
+          sphere_fits(&R, p, found, c, &cc);
+        }
       }
-      int ok = 1;
-      for (int dk = -1; dk <= 1 && ok; ++dk)
-        for (int dj = -1; dj <= 1 && ok; ++dj)
-          for (int di = -1; di <= 1 && ok; ++di) {
-            int a = (ic[0] + di + nc) % nc, b = (ic[1] + dj + nc) % nc, e = (ic[2] + dk + nc) % nc;
-            for (int q = head[((long)e * nc + b) * nc + a]; q >= 0; q = next[q]) {
-              double d2 = 0.0;
-              for (int d = 0; d < 3; ++d) {
-                double dd = x[3 * q + d] - c[d];
-                if (dd > 0.5 * L) dd -= L;
-                if (dd < -0.5 * L) dd += L;
-                d2 += dd * dd;
-              }
-              double s = r[p] + r[q];
-              if (d2 <= s * s * (1.0 + 1e-9)) { ok = 0; break; }
-            }
-          }
-      if (!ok) continue;
-      for (int d = 0; d < 3; ++d) x[3 * p + d] = c[d];
-      long cc = ((long)ic[2] * nc + ic[1]) * nc + ic[0];
-      next[p] = head[cc];
-      head[cc] = p;
-      break;
+      if (found < 0) {
+        att = hi;
+        batch = next_batch(2 * batch, 1, 256 * nth);
+      }
     }
+    att = found + 1;
+    batch = next_batch(att - first, 1, 256 * nth);
+    for (int d = 0; d < 3; ++d) x[3 * p + d] = c[d];
+    next[p] = head[cc];
+    head[cc] = p;
   }
+  uint64_t st = R.s0 + (uint64_t)(3 * att) * GDW_GAMMA;
   for (int p = 0; p < n; ++p)
     for (int d = 0; d < 3; ++d) v[3 * p + d] = sigma_v * normal01(&st);
   free(head);
@@ -152,6 +218,7 @@ typedef struct {
   int nv, nf, ne;
   double v[8][3], f[6][3], e[6][3];
   double circ;
+  double insc;  /* radius of the largest ball about the origin inside the body */
 } gdw_poly;
 
 static void rot_apply(const double* R, const double* a, double* out) {
@@ -222,6 +289,7 @@ static void poly_box(gdw_poly* P, const double* h) {
     P->e[d][d] = 1.0;
   }
   P->circ = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+  P->insc = h[0] < h[1] ? (h[0] < h[2] ? h[0] : h[2]) : (h[1] < h[2] ? h[1] : h[2]);
 }
 
 static void poly_tri(gdw_poly* P, int nv, const double* verts, int nf, const int* tri) {
@@ -242,6 +310,14 @@ static void poly_tri(gdw_poly* P, int nv, const double* verts, int nf, const int
     const double* c = P->v[tri[3 * f + 2]];
     double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, w[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
     cross3(u, w, P->f[f]);
+  }
+  /* distance from the origin (inside) to the nearest face plane */
+  P->insc = 1e300;
+  for (int f = 0; f < nf; ++f) {
+    const double* n = P->f[f];
+    const double* a = P->v[tri[3 * f]];
+    double d = fabs(n[0] * a[0] + n[1] * a[1] + n[2] * a[2]) / sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    P->insc = d < P->insc ? d : P->insc;
   }
   /* unique edges of the triangles */
   int ne = 0, ea[12], eb[12];
@@ -271,8 +347,63 @@ static void quat_to_basis(const double* q, double* b) {
    (walls, no periodicity): type[i] = 0 box (half extents box_h), 1
    tetrahedron (tet_v, 4 triangular faces tet_f).  Orientation q (Shoemake,
    generated here), basis = R(q).  margin > 0 keeps a gap, margin < 0 allows an
-   overlap of |margin| along the least-overlap axis.  Returns 0, or -1 - p when
-   the attempts ran out while placing particle p. */
+   overlap of |margin| along the least-overlap axis.  Attempts are numbered
+   over the whole run and drawn, tested in parallel and kept as in
+   gdw_rsa_periodic.  Returns 0, or -1 - p when the attempts ran out while
+   placing particle p. */
+typedef struct {
+  const gdw_poly* P;
+  const int32_t* type;
+  const double* x;
+  const double* basis;
+  const int* head;
+  const int* next;
+  const double* lo;
+  const double* hi;
+  const int* nc;
+  double margin;
+  uint64_t s0;
+} rsa_cvx;
+
+static int convex_fits(const rsa_cvx* R, int p, long a, double* c, long* cc) {
+  const gdw_poly* A = &R->P[R->type[p] ? 1 : 0];
+  const int* nc = R->nc;
+  int ic[3];
+  for (int d = 0; d < 3; ++d) {
+    double lo = R->lo[d] + A->circ, hi = R->hi[d] - A->circ;
+    c[d] = lo + (hi - lo) * u01_at(R->s0, (uint64_t)(3 * a + d + 1));
+    ic[d] = (int)((c[d] - R->lo[d]) / ((R->hi[d] - R->lo[d]) / nc[d]));
+    if (ic[d] >= nc[d]) ic[d] = nc[d] - 1;
+  }
+  const double margin = R->margin;
+  for (int dk = -1; dk <= 1; ++dk)
+    for (int dj = -1; dj <= 1; ++dj)
+      for (int di = -1; di <= 1; ++di) {
+        int ia = ic[0] + di, ib = ic[1] + dj, ie = ic[2] + dk;
+        if (ia < 0 || ib < 0 || ie < 0 || ia >= nc[0] || ib >= nc[1] || ie >= nc[2]) continue;
+        for (int q = R->head[((long)ie * nc[1] + ib) * nc[0] + ia]; q >= 0; q = R->next[q]) {
+          const gdw_poly* B = &R->P[R->type[q] ? 1 : 0];
+          double d2 = 0.0;
+          for (int d = 0; d < 3; ++d) {
+            double dd = R->x[3 * q + d] - c[d];
+            d2 += dd * dd;
+          }
+          double sr = A->circ + B->circ + (margin > 0 ? margin : 0.0);
+          if (d2 > sr * sr) continue;
+          /* the inscribed balls overlap by more than the tolerated overlap
+             (or lie closer than the required gap): the separating-axis test
+             would reject as well (its overlap along every axis is at least
+             the balls' overlap); a relative 1e-9 keeps off its rounding */
+          double si = (A->insc + B->insc + margin) * (1.0 - 1e-9);
+          if (si > 0.0 && d2 < si * si) return 0;
+          if (!sat_separated(A, R->basis + 9 * p, c, B, R->basis + 9 * q, R->x + 3 * q, margin))
+            return 0;
+        }
+      }
+  *cc = ((long)ic[2] * nc[1] + ic[1]) * nc[0] + ic[0];
+  return 1;
+}
+
 int gdw_rsa_convex(int n, const int32_t* type, const double* box_h, const double* tet_v,
                    const int32_t* tet_f, const double* lo, const double* hi, double margin,
                    double sigma_v, uint64_t seed, long max_attempts, double* x, double* v,
@@ -296,49 +427,47 @@ int gdw_rsa_convex(int n, const int32_t* type, const double* box_h, const double
   for (long c = 0; c < ncell; ++c) head[c] = -1;
   gdw_random_quaternions(n, seed * 31 + 17, q);
   for (int i = 0; i < n; ++i) quat_to_basis(q + 4 * i, basis + 9 * i);
-  uint64_t st = seed * 0xD1B54A32D192ED03ULL + 29;
-  long attempts = 0;
+  rsa_cvx R = {P, type, x, basis, head, next, lo, hi, nc, margin,
+               seed * 0xD1B54A32D192ED03ULL + 29};
+  const long nth = omp_get_max_threads();
+  long att = 0;
+  long batch = 1;
   for (int p = 0; p < n; ++p) {
-    const gdw_poly* A = &P[type[p] ? 1 : 0];
-    for (;;) {
-      if (++attempts > max_attempts) { free(head); free(next); return -1 - p; }
-      double c[3];
-      int ic[3];
-      for (int d = 0; d < 3; ++d) {
-        double a = lo[d] + A->circ, b = hi[d] - A->circ;
-        c[d] = a + (b - a) * u01(&st);
-        ic[d] = (int)((c[d] - lo[d]) / ((hi[d] - lo[d]) / nc[d]));
-        if (ic[d] >= nc[d]) ic[d] = nc[d] - 1;
+    const long first = att;
+    long found = -1;
+    double c[3];
+    long cc = 0;
+    while (found < 0) {
+      if (att >= max_attempts) { free(head); free(next); return -1 - p; }
+      const long top = att + batch < max_attempts ? att + batch : max_attempts;
+      if (top - att < 2 * nth) {
+        for (long a = att; a < top; ++a)
+          if (convex_fits(&R, p, a, c, &cc)) { found = a; break; }
+      } else {
+        long best = LONG_MAX;
+#pragma omp parallel for schedule(static) reduction(min : best)
+        for (long a = att; a < top; ++a) {
+          double ct[3];
+          long ct_cell;
+          if (a < best && convex_fits(&R, p, a, ct, &ct_cell)) best = a;
+        }
+        if (best != LONG_MAX) {
+          found = best;
+          convex_fits(&R, p, found, c, &cc);
+        }
       }
-      int ok = 1;
-      for (int dk = -1; dk <= 1 && ok; ++dk)
-        for (int dj = -1; dj <= 1 && ok; ++dj)
-          for (int di = -1; di <= 1 && ok; ++di) {
-            int a = ic[0] + di, b = ic[1] + dj, e = ic[2] + dk;
-            if (a < 0 || b < 0 || e < 0 || a >= nc[0] || b >= nc[1] || e >= nc[2]) continue;
-            for (int qq = head[((long)e * nc[1] + b) * nc[0] + a]; qq >= 0; qq = next[qq]) {
-              const gdw_poly* B = &P[type[qq] ? 1 : 0];
-              double d2 = 0.0;
-              for (int d = 0; d < 3; ++d) {
-                double dd = x[3 * qq + d] - c[d];
-                d2 += dd * dd;
-              }
-              double s = A->circ + B->circ + (margin > 0 ? margin : 0.0);
-              if (d2 > s * s) continue;
-              if (!sat_separated(A, basis + 9 * p, c, B, basis + 9 * qq, x + 3 * qq, margin)) {
-                ok = 0;
-                break;
-              }
-            }
-          }
-      if (!ok) continue;
-      for (int d = 0; d < 3; ++d) x[3 * p + d] = c[d];
-      long cc = ((long)ic[2] * nc[1] + ic[1]) * nc[0] + ic[0];
-      next[p] = head[cc];
-      head[cc] = p;
-      break;
+      if (found < 0) {
+        att = top;
+        batch = next_batch(2 * batch, 1, 256 * nth);
+      }
     }
+    att = found + 1;
+    batch = next_batch(att - first, 1, 256 * nth);
+    for (int d = 0; d < 3; ++d) x[3 * p + d] = c[d];
+    next[p] = head[cc];
+    head[cc] = p;
   }
+  uint64_t st = R.s0 + (uint64_t)(3 * att) * GDW_GAMMA;
   for (int p = 0; p < n; ++p)
     for (int d = 0; d < 3; ++d) v[3 * p + d] = sigma_v * normal01(&st);
   free(head);

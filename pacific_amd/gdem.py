@@ -70,7 +70,8 @@ class _BatchSystem(C.Structure):
                 ("kind", C.c_int32), ("nwalls", C.c_int32),
                 ("pp", C.c_double * 5), ("pw", C.c_double * 5),
                 ("wall_basis", C.c_void_p), ("wall_centre", C.c_void_p),
-                ("wall_half", C.c_void_p), ("wall_crust", C.c_double)]
+                ("wall_half", C.c_void_p), ("wall_crust", C.c_double),
+                ("substeps", C.c_int32), ("reserved", C.c_int32)]
 
 
 _lib = None
@@ -701,6 +702,7 @@ def _batch_system(wl):
         keep += [wb, wc, wh]
         bs.wall_basis, bs.wall_centre, bs.wall_half = wb.ctypes.data, wc.ctypes.data, wh.ctypes.data
         bs.wall_crust = float(walls[0][3])
+    bs.substeps = int(wl.get("substeps", 1))
     return bs, keep
 
 

@@ -353,7 +353,7 @@ def tetrahedron_vertices(edge=TET_EDGE):
 
 
 def convex_box(n=200_000, seed=3, steps=300, phi=0.20, height=0.20, overlap=0.0, cell_factor=1.0,
-               sigma_v=0.0):
+               sigma_v=0.0, attempts_per_particle=2000):
     """n convex particles, alternating cubes (edge 2 mm) and regular tetrahedra
     (edge 2.5 mm), uniform random orientations (Shoemake), random sequential
     addition (exact separating-axis overlap test) at solid fraction phi in a
@@ -378,7 +378,7 @@ def convex_box(n=200_000, seed=3, steps=300, phi=0.20, height=0.20, overlap=0.0,
     faces = np.array(TET_FACES, np.int32).reshape(-1)
     rc = _lib_convex().gdw_rsa_convex(n, _p(types), _p(np.array(CUBE_HALF)),
                                       _p(np.ascontiguousarray(tv)), _p(faces), _p(lo), _p(hi),
-                                      -overlap, sigma_v, seed, 2000 * n + 100000, _p(x), _p(v), _p(q),
+                                      -overlap, sigma_v, seed, attempts_per_particle * n + 100000, _p(x), _p(v), _p(q),
                                       _p(basis))
     if rc != 0:
         raise RuntimeError(f"random sequential addition stopped at particle {-1 - rc} of {n}")
@@ -407,12 +407,26 @@ def _lib_convex():
 
 
 # ------------------------------------------------------------------ batched
+def stable_substeps(kn, r_min, dt, density=DENSITY, limit=1.0):
+    """Substeps of dt that keep a Hooke contact of stiffness kn between the two
+    lightest spheres (radius r_min) at omega dt / substeps <= limit, omega =
+    sqrt(kn / m*) with m* = m / 2: half the leap-frog's stability bound
+    omega dt < 2.  The policy of the batched and calibration workloads for
+    stiff systems; the oracle and gdem_step run such a system as
+    nsteps * substeps steps of dt / substeps."""
+    import math
+    m = density * 4.0 * math.pi * r_min ** 3 / 3.0
+    omega = math.sqrt(kn / (0.5 * m))
+    return max(1, math.ceil(omega * dt / limit))
+
+
 def batch_system(n, seed, aspect, kn, en, mu, cell_factor=1.0, spacing=2.3e-3, jitter=4.0e-5,
                  sigma_v=0.05):
     """One small settling system: n spheres, r ~ U[0.9, 1.1] mm, on a jittered
     cubic lattice (spacing 2.3 mm) in a closed box whose height is about
     `aspect` times its width, six walls, gravity -9.81 z, Hooke (kn, en, mu)
-    for particle-particle and particle-wall contacts, dt 1e-5 s."""
+    for particle-particle and particle-wall contacts, dt 1e-5 s, advanced in
+    stable_substeps(kn, r_min, dt) substeps per step."""
     import math
     rng = np.random.default_rng(seed)
     r = rng.uniform(0.9e-3, 1.1e-3, n)
@@ -430,17 +444,18 @@ def batch_system(n, seed, aspect, kn, en, mu, cell_factor=1.0, spacing=2.3e-3, j
     return {"name": f"batch_system_{n}_s{seed}", "origin": tuple(lo), "size": tuple(hi - lo),
             "periodic": (0, 0, 0), "cell_size": 2.0 * cell_factor * r.max(),
             "gravity": (0.0, 0.0, -9.81), "dt": 1.0e-5, "steps": 1000,
+            "substeps": stable_substeps(kn, float(r.min()), 1.0e-5),
             "models": [(0, 0, MODEL_HOOKE, hooke), (0, 1, MODEL_HOOKE, hooke)],
             "walls": box_walls(lo, hi, 1), "particles": parts}
 
 
-def batch_systems(nsys=4096, seed=4, nmin=64, nmax=2048, kn_range=(1.0e4, 1.0e5), cell_factor=1.0):
+def batch_systems(nsys=4096, seed=4, nmin=64, nmax=2048, kn_range=(1.0e4, 1.0e6), cell_factor=1.0):
     """Independent settling systems of a calibration sweep: particle count
     U{nmin..nmax}, aspect ratio U[1, 4], kn log-uniform in kn_range, en
-    U[0.5, 0.95], mu U[0.1, 0.8] per system.  The upper end of kn stays at
-    1e5 N/m: for two 0.9 mm spheres (m* = 3.8e-6 kg) the leap-frog step of
-    1e-5 s is unstable above kn = 4 m* / dt^2 = 1.5e5 N/m (omega dt > 2), and
-    systems drawn above it blow up within a few hundred steps."""
+    U[0.5, 0.95], mu U[0.1, 0.8] per system.  For two 0.9 mm spheres (m* =
+    3.8e-6 kg) the leap-frog step of 1e-5 s is unstable above kn = 4 m* /
+    dt^2 = 1.5e5 N/m (omega dt > 2); each system therefore runs
+    stable_substeps() substeps per step (up to 6 at kn = 1e6)."""
     rng = np.random.default_rng(seed)
     ns = rng.integers(nmin, nmax + 1, nsys)
     aspect = rng.uniform(1.0, 4.0, nsys)
@@ -459,7 +474,7 @@ def calibration_system(n=1000, seed=1, spacing=2.1e-3, jitter=4.0e-5, sigma_v=0.
     lattice (spacing 2.1 mm, so neighbours collide within the first
     milliseconds), v ~ N(0, sigma_v^2), in a closed box of aspect ratio about
     2, gravity -9.81 z, Hooke (kn, en, mu) = theta for particle-particle and
-    particle-wall contacts, dt 1e-5 s."""
+    particle-wall contacts, dt 1e-5 s in stable_substeps(1.5 kn) substeps."""
     import math
     nx = max(1, math.ceil((n / 2.0) ** (1.0 / 3.0)))
     nz = max(1, math.ceil(n / (nx * nx)))
@@ -475,11 +490,13 @@ def calibration_system(n=1000, seed=1, spacing=2.1e-3, jitter=4.0e-5, sigma_v=0.
     return {"name": f"calibration_{n}_s{seed}", "origin": tuple(lo), "size": tuple(hi - lo),
             "periodic": (0, 0, 0), "cell_size": 2.0 * r.max(), "gravity": (0.0, 0.0, -9.81),
             "dt": 1.0e-5, "steps": 400,
+            # room for a calibration to move kn up to 1.5 times its value
+            "substeps": stable_substeps(1.5 * float(theta[0]), float(r.min()), 1.0e-5),
             "models": [(0, 0, MODEL_HOOKE, hooke), (0, 1, MODEL_HOOKE, hooke)],
             "walls": box_walls(lo, hi, 1), "particles": _particles(x, v, r)}
 
 
-def calibration_systems(nsys=512, n=1000, seed=5, kn_range=(1.0e4, 1.0e5)):
+def calibration_systems(nsys=512, n=1000, seed=5, kn_range=(1.0e4, 1.0e6)):
     """Systems of a calibration run and their hidden parameters (kn, en, mu),
     drawn as for batch_systems: kn log-uniform in kn_range, en U[0.5, 0.95],
     mu U[0.1, 0.8]."""

@@ -1,6 +1,8 @@
 """Batched persistent launch (gdem_batch_*): every system advanced in the batch
 is bit-identical to the same system advanced alone by gdem_step, whatever the
-workgroup size, and follows the CPU oracle within the trajectory tolerance."""
+workgroup size, and follows the CPU oracle within the trajectory tolerance.
+A stiff system runs `substeps` steps of dt / substeps per step of dt in the
+batch, and nsteps * substeps steps of dt / substeps alone and in the oracle."""
 import numpy as np
 import pytest
 
@@ -22,9 +24,12 @@ def test_batch_equals_single_system_steps():
     b = gdem.build_batch(systems, block_size=256)
     b.step(60, 1e-5)
     b.synchronize()
+    assert max(wl["substeps"] for wl in systems) > 1
     for s, wl in enumerate(systems):
         single = gdem.build_system(wl)
-        single.step(60, 1e-5)
+        single.set_skin(0.0)   # lists rebuilt every step, as in the batch
+        sub = wl["substeps"]
+        single.step(60 * sub, 1e-5 / sub)
         g = single.get_state()
         h = b.get_state(s)
         for k in KEYS:
@@ -62,7 +67,8 @@ def test_batch_follows_oracle(oracle_mod):
     for s, wl in enumerate(systems):
         o = oracle_mod.Oracle(wl)
         o.prime()
-        o.step(100, 1e-5)
+        sub = wl["substeps"]
+        o.step(100 * sub, 1e-5 / sub)
         os_ = o.state()
         g = b.get_state(s)
         x0 = wl["particles"]["x"]

@@ -1,0 +1,283 @@
+"""The five benchmark workloads at their stated sizes, on the GPU, against the
+oracle (through the C ABI).
+
+* plain box, 1,000 spheres of radius 1 mm, 2,000 steps: every 100 steps up to
+  step 100 within the trajectory bar (1e-9); over the whole 2,000 steps the
+  packing collides and a one-ulp change of one velocity grows to O(1), so the
+  GPU's observables at step 2,000 are checked against the spread of the
+  oracle's own one-ulp-perturbed runs.
+* periodic Hertz, 1,000,000 spheres: after 5 steps pair set bit-exact and
+  state within 1e-9 (both the packed pair + integrator kernels and the fused
+  kernel, which agree bit for bit); the same spheres grown by 2 % (contacts
+  at the start): pair set bit-exact and F, T within 1e-12 at the priming
+  evaluation; linear momentum over 40 steps.
+* convex box, 200,000 cubes and tetrahedra: contact set at the priming
+  evaluation and after 3 steps, state within 1e-9; and as many particles with
+  small initial overlaps (contacts from step 0).
+* 4,096 batched systems in one launch (kn up to 1e6 N/m, stiff systems in
+  substeps): 16 sampled systems, every substep count among them,
+  bit-identical to gdem_step run alone with lists rebuilt every step (as the
+  batch does), 4 within the trajectory bar of the oracle.
+* 512 calibration systems of 1,000 spheres over the 400-step window: the
+  reverse sweep against forward-mode derivatives on every system.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from gdem_testutil import force_scales, normwise_ok, pair_set, rel_err
+from pacific_amd import gdem, workloads
+
+pytestmark = pytest.mark.gpu
+
+RHO = 2500.0
+KEYS = ("x", "v", "w", "q", "F", "T")
+
+
+def _masses(r):
+    return RHO * (4.0 * math.pi * r ** 3 / 3.0)
+
+
+def _observables(s, wl):
+    """Step-2,000 observables of the plain box: kinetic energy, mean height,
+    mean |v|, mean |w|."""
+    m = _masses(wl["particles"]["radius"])
+    return np.array([0.5 * (m * (s["v"] ** 2).sum(1)).sum(), s["x"][:, 2].mean(),
+                     np.linalg.norm(s["v"], axis=1).mean(), np.linalg.norm(s["w"], axis=1).mean()])
+
+
+def test_plain_box_1000_spheres_2000_steps(oracle_mod):
+    print("[case] plain box 1000 spheres r = 1 mm, 2000 steps", flush=True)
+    wl = workloads.plain_box(1000, seed=1)
+    dt = wl["dt"]
+    orc = oracle_mod.Oracle(wl)
+    orc.prime()
+    s = gdem.build_system(wl)
+    s.prime()
+    x0 = wl["particles"]["x"]
+    for k in range(1, 11):
+        orc.step(10, dt)
+        s.step(10, dt)
+        o, g = orc.state(), s.get_state()
+        assert rel_err(g["x"] - x0, o["x"] - x0) <= 1e-9, k
+        for key in ("v", "w", "q"):
+            assert rel_err(g[key], o[key]) <= 1e-9, (k, key)
+    orc.step(1900, dt)
+    s.step(1900, dt)
+    og, gg = _observables(orc.state(), wl), _observables(s.get_state(), wl)
+    # the oracle's own spread: the same run with every initial velocity moved
+    # by one ulp (three draws of the directions)
+    spread = np.zeros(4)
+    for j in range(3):
+        w2 = workloads.plain_box(1000, seed=1)
+        v = w2["particles"]["v"]
+        up = np.random.default_rng(20 + j).random(v.shape) < 0.5
+        w2["particles"]["v"] = np.nextafter(v, np.where(up, np.inf, -np.inf))
+        o2 = oracle_mod.Oracle(w2)
+        o2.prime()
+        o2.step(2000, dt)
+        spread = np.maximum(spread, np.abs(_observables(o2.state(), w2) - og))
+    print(f"[case] observables gpu {gg} oracle {og} spread {spread}", flush=True)
+    assert (np.abs(gg - og) <= 3.0 * spread + 1e-12 * np.abs(og)).all(), (gg, og, spread)
+    s.close()
+
+
+@pytest.fixture(scope="module")
+def periodic_1m():
+    print("[case] generating 1,000,000 spheres (random sequential addition)", flush=True)
+    return workloads.periodic_hertz(1_000_000, seed=2)
+
+
+def test_periodic_hertz_1m_steps(oracle_mod, periodic_1m):
+    """The workload as stated: random sequential addition leaves no overlap,
+    contacts start within the first steps.  After 5 steps the pair set equals
+    the oracle's and the state is within 1e-9 (packed and fused kernels
+    bit-identical to each other)."""
+    wl = periodic_1m
+    p = wl["particles"]
+    n = len(p["radius"])
+    dt = wl["dt"]
+    print("[case] periodic Hertz 1M: oracle, 5 steps", flush=True)
+    orc = oracle_mod.Oracle(wl, log_capacity=4 * n)
+    orc.prime()
+    fused = gdem.build_system(wl)
+    fused.set_contact_recording(32)   # recording keeps the fused kernel
+    fused.prime()
+    assert pair_set(fused.get_contacts()) == pair_set(orc.contacts())
+    orc.step(5, dt)
+    o = orc.state()
+    fused.step(5, dt)
+    packed = gdem.build_system(wl)    # gdem_step's packed pair + integrator kernels
+    packed.step(5, dt)
+    gf, gp = fused.get_state(), packed.get_state()
+    for k in KEYS:
+        assert np.array_equal(gf[k], gp[k]), k
+    oc = orc.contacts()
+    print(f"[case] contacts after 5 steps: {len(oc['id0'])}", flush=True)
+    assert len(oc["id0"]) > 10000
+    assert pair_set(fused.get_contacts()) == pair_set(oc)
+    L = np.array(wl["size"])
+    dg, do = gp["x"] - p["x"], o["x"] - p["x"]
+    dg -= L * np.round(dg / L)
+    do -= L * np.round(do / L)
+    assert rel_err(dg, do) <= 1e-9
+    for k in ("v", "w", "q"):
+        assert rel_err(gp[k], o[k]) <= 1e-9, k
+    fused.close()
+    packed.close()
+
+
+def test_periodic_hertz_1m_forces_at_prime(oracle_mod, periodic_1m):
+    """The same million spheres with radii grown by 2 % (contacts from the
+    start): pair set bit-exact and F, T within 1e-12 of the per-particle scale
+    at the priming evaluation, where both sides see the same positions."""
+    wl = dict(periodic_1m)
+    p = dict(wl["particles"])
+    p["radius"] = p["radius"] * 1.02
+    wl["particles"] = p
+    wl["cell_size"] = 2.0 * 1.2 * p["radius"].max()
+    n = len(p["radius"])
+    print("[case] periodic Hertz 1M, radii x 1.02: prime", flush=True)
+    orc = oracle_mod.Oracle(wl, log_capacity=8 * n)
+    orc.prime()
+    oc, os_ = orc.contacts(), orc.state()
+    s = gdem.build_system(wl)
+    s.set_contact_recording(32)
+    s.prime()
+    gc, gs = s.get_contacts(), s.get_state()
+    print(f"[case] contacts at prime: {len(oc['id0'])}", flush=True)
+    assert len(oc["id0"]) > 100000
+    assert pair_set(gc) == pair_set(oc)
+    sf, st = force_scales(n, oc, p["x"], np.zeros((n, 3)))
+    ok, err, bad = normwise_ok(gs["F"], os_["F"], sf, 1e-12)
+    assert ok, (np.nonzero(bad)[0][:5], err[bad][:5])
+    ok, err, bad = normwise_ok(gs["T"], os_["T"], st, 1e-12)
+    assert ok, (np.nonzero(bad)[0][:5], err[bad][:5])
+    s.close()
+
+
+def test_periodic_hertz_1m_momentum_and_capacity(periodic_1m):
+    print("[case] periodic Hertz 1M: momentum over 40 steps", flush=True)
+    wl = periodic_1m
+    m = _masses(wl["particles"]["radius"])
+    s = gdem.build_system(wl)
+    s.prime()
+    g0 = s.get_state()
+    p0 = (m[:, None] * g0["v"]).sum(0)
+    s.reset_counters()
+    s.step(40, wl["dt"])
+    g1 = s.get_state()      # raises CapacityError if a list overflowed
+    p1 = (m[:, None] * g1["v"]).sum(0)
+    scale = (m[:, None] * np.abs(g0["v"])).sum()
+    assert np.abs(p1 - p0).max() <= 1e-13 * scale
+    c = s.counters()
+    assert c["force_evals"] == 40 and c["contacts"] > 40 * 10000
+    assert abs(np.linalg.norm(g1["q"], axis=1) - 1).max() < 1e-13
+    s.close()
+
+
+@pytest.fixture(scope="module")
+def convex_200k():
+    print("[case] generating 200,000 cubes and tetrahedra (random sequential addition)",
+          flush=True)
+    return workloads.convex_box(200_000, seed=3)
+
+
+def _convex_check(oracle_mod, wl, steps, min_contacts):
+    n = len(wl["particles"]["shape"])
+    orc = oracle_mod.ConvexOracle(wl, log_capacity=8 * n)
+    orc.prime()
+    s = gdem.build_system(wl)
+    s.set_contact_recording(32)
+    s.prime()
+    assert pair_set(s.get_contacts()) == pair_set(orc.contacts())
+    assert len(orc.contacts()["id0"]) >= min_contacts
+    orc.step(steps, wl["dt"])
+    s.step(steps, wl["dt"])
+    g, o = s.get_state(), orc.state()
+    assert pair_set(s.get_contacts()) == pair_set(orc.contacts())
+    x0 = wl["particles"]["x"]
+    assert rel_err(g["x"] - x0, o["x"] - x0) <= 1e-9
+    for k in ("v", "w", "q"):
+        assert rel_err(g[k], o[k]) <= 1e-9 or np.abs(o[k]).max() == 0.0, k
+    assert rel_err(s.get_basis(), o["basis"]) <= 1e-9
+    s.close()
+
+
+def test_convex_200k_prime_and_steps(oracle_mod, convex_200k):
+    print("[case] convex 200k: prime + 3 steps", flush=True)
+    _convex_check(oracle_mod, convex_200k, 3, 0)
+
+
+def test_convex_200k_overlapping_start(oracle_mod):
+    print("[case] convex 200k with initial overlaps: prime + 3 steps", flush=True)
+    wl = workloads.convex_box(200_000, seed=7, phi=0.18, overlap=4e-5, sigma_v=0.2)
+    _convex_check(oracle_mod, wl, 3, 10000)
+
+
+def test_batch_4096_systems(oracle_mod):
+    print("[case] 4096 batched systems, 1000 steps, one launch", flush=True)
+    systems = workloads.batch_systems(4096, seed=4)
+    dt = systems[0]["dt"]
+    b = gdem.build_batch(systems)
+    b.step(1000, dt)
+    b.synchronize()
+    rng = np.random.default_rng(0)
+    subs = np.array([w["substeps"] for w in systems])
+    # the sample holds every substep count and the largest system
+    sample = [int(np.flatnonzero(subs == k)[0]) for k in np.unique(subs)]
+    sample.append(int(np.argmax([len(w["particles"]["radius"]) for w in systems])))
+    rest = [k for k in rng.permutation(len(systems)).tolist() if k not in sample]
+    sample = (sample + rest)[:16]
+    for j, s in enumerate(sample):
+        wl = systems[int(s)]
+        sub = wl["substeps"]
+        one = gdem.build_system(wl)
+        one.set_skin(0.0)   # lists rebuilt every step, as in the batch
+        one.step(1000 * sub, dt / sub)
+        g, h = one.get_state(), b.get_state(int(s))
+        for k in KEYS:
+            assert np.array_equal(g[k], h[k]), (int(s), k)
+        one.close()
+        if j < 4:
+            o = oracle_mod.Oracle(wl)
+            o.prime()
+            o.step(100 * sub, dt / sub)
+            os_ = o.state()
+            c = gdem.build_system(wl)
+            c.step(100 * sub, dt / sub)
+            gc = c.get_state()
+            x0 = wl["particles"]["x"]
+            assert rel_err(gc["x"] - x0, os_["x"] - x0) <= 1e-9, int(s)
+            assert rel_err(gc["v"], os_["v"]) <= 1e-9, int(s)
+            c.close()
+    b.close()
+
+
+def test_calibration_512_systems_reverse_equals_forward_mode():
+    print("[case] calibration 512 x 1000 spheres, 400 steps: reverse vs forward mode", flush=True)
+    systems, hidden = workloads.calibration_systems(512, n=1000, seed=5)
+    v0 = np.concatenate([w["particles"]["v"] for w in systems])
+    eng = gdem.GdemDiff(systems, 400, 1.0e-5, 50, (100, 200, 300, 400))
+    eng.set_parameters(hidden, v0)
+    eng.set_targets(eng.record())
+    theta = hidden * np.array([1.2, 0.9, 1.1])[None, :]
+    eng.set_parameters(theta, v0)
+    loss, gth, gv0 = eng.gradient()
+    rng = np.random.default_rng(7)
+    dth = rng.normal(size=theta.shape) * theta
+    dv = rng.normal(size=v0.shape)
+    loss2, dl = eng.directional(dth, dv)
+    np.testing.assert_array_equal(loss2, loss)
+    off = eng.offsets
+    worst = 0.0
+    for s in range(len(systems)):
+        a = gth[s] * dth[s]
+        b = gv0[off[s]:off[s + 1]] * dv[off[s]:off[s + 1]]
+        want = float(a.sum() + b.sum())
+        scale = float(np.abs(a).sum() + np.abs(b).sum())
+        worst = max(worst, abs(dl[s] - want) / scale)
+    print(f"[case] worst |reverse - forward| / scale = {worst:.2e}", flush=True)
+    assert worst <= 1e-10
+    eng.close()
