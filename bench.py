@@ -11,9 +11,11 @@ forces, integration).  value = particles * timed steps / seconds of the timed
 region, state resident in HBM.
 
 Prints one JSON line with the throughput, the roofline of the dominant kernel
-(the fused force + integrate kernel, timed with HIP events on its stream) and a
-CPU baseline (the oracle restatement of the reference's CPU step, one thread,
-on a bounded sample of the same workload).
+(the integrator kernel k_integ of the packed sphere step, which streams the
+state and sums the contact records; timed with HIP events on the context's
+stream), the whole step's bandwidth, the list rebuilds inside the timed window,
+and CPU baselines (the oracle restatement of the reference's CPU step on one
+thread and on every host core, on a bounded sample of the same workload).
 """
 import argparse
 import json

@@ -40,8 +40,8 @@
  *     that reaches the crust sum is first retried as
  *     ClosestPoint_ErreurHandling does, RigidBodyWithCrust.cpp:534-588, with an
  *     imposed overlap equal to the crust sum, and only an overlap the bodies
- *     shrunk by ten crusts still have is an error; the convex and the
- *     differentiated steps raise it at the crust sum),
+ *     shrunk by ten crusts still have is an error, for spheres and convex
+ *     bodies; the differentiated step raises it at the crust sum),
  *     MOTION (MotionError, Particle.cpp:447-457) or SIMULATION
  *     (SimulationError, LinkedCell.cpp:1506-1531).  gdem_last_error names the
  *     offending ids (smallest particle id among the offenders; a wall is -1-k).
@@ -446,11 +446,12 @@ int gdem_get_counters(gdem_ctx* ctx, int64_t* force_evals, int64_t* contacts,
    gdem_set_contact_stats; every force evaluation then adds its contacts'
    overlaps (fixed-order workgroup reductions, reproducible) and the GJK
    iterations the closest-point routines report (1 for a sphere contact, 2 for
-   a two-sphere contact-error retry, none for rectangles and sphere-wall
-   retries).  gdem_get_contact_stats fills out[6] = {largest overlap, mean
-   overlap, time of the largest overlap, mean GJK iterations, overlaps counted,
-   GJK reports counted}.  Sphere contexts; off by default (one extra kernel per
-   evaluation when on). */
+   a two-sphere contact-error retry, the Johnson GJK's count for convex
+   bodies, none for rectangles and sphere-wall retries).
+   gdem_get_contact_stats fills out[6] = {largest overlap, mean overlap, time
+   of the largest overlap, mean GJK iterations, overlaps counted, GJK reports
+   counted}.  Off by default (one extra kernel per evaluation when on; a
+   sphere context then steps in its canonical layout). */
 int gdem_set_contact_stats(gdem_ctx* ctx, int32_t on);
 int gdem_get_contact_stats(gdem_ctx* ctx, double* out);
 

@@ -231,6 +231,8 @@ int orc_c_get_contacts(const orc_csys* s, int capacity, int* n_out, int32_t* id0
                        double* overlap, double* force);
 int orc_c_last_error(const orc_csys* s, int* code, int32_t* id0, int32_t* id1);
 int64_t orc_c_gjk_calls(const orc_csys* s);
+/* AppCollision contact features of the run (as orc_get_contact_stats). */
+void orc_c_get_contact_stats(const orc_csys* s, double* out);
 /* one pair / one particle-wall contact on the current state, no accumulation:
    1 contact (point, overlap vector, overlap, force on the first body), 0 none,
    -1 ContactError */

@@ -73,7 +73,29 @@ struct orc_csys {
   int64_t ngjk;
   int err_code;
   int32_t err_id0, err_id1;
+  /* time of the current force evaluation; AppCollision contact features */
+  double time;
+  double ov_mean, ov_count, ov_max, ov_time_max, gjk_mean, gjk_count;
 };
+
+/* AppCollision::addToContactsFeatures (Base/src/AppCollision.cpp:160-190):
+   running mean and maximum of the overlaps, running mean of the reported GJK
+   iterations, over every closest-point result of the run. */
+static void c_contact_features(orc_csys* s, double overlap, int nbIter) {
+  if (overlap < 0.) {
+    overlap = fabs(overlap);
+    s->ov_mean = (s->ov_mean * s->ov_count + overlap) / (s->ov_count + 1.);
+    s->ov_count += 1.;
+    if (overlap > s->ov_max) {
+      s->ov_max = overlap;
+      s->ov_time_max = s->time;
+    }
+  }
+  if (nbIter) {
+    s->gjk_mean = (s->gjk_mean * s->gjk_count + (double)nbIter) / (s->gjk_count + 1.);
+    s->gjk_count += 1.;
+  }
+}
 
 static int cset_error(orc_csys* s, int code, int a, int b) {
   if (s->err_code == ORC_OK) {
@@ -640,7 +662,8 @@ typedef struct {
    scaled by (crustA + crustB - d).  Returns 1 contact candidate, 0 none,
    -1 ContactError (shrunk bodies overlap). */
 static int closest_point(const body_t* A, const body_t* B, v3* contact, v3* ov, double* dist,
-                         int64_t* ngjk) {
+                         int64_t* ngjk, int* nbIter) {
+  *nbIter = 0;
   v3 gab = sub(ld(A->origin), ld(B->origin));
   if (!(norm(gab) < A->circ + B->circ)) return 0;
   ++*ngjk;
@@ -650,6 +673,7 @@ static int closest_point(const body_t* A, const body_t* B, v3* contact, v3* ov, 
   v3 pa, pb;
   int nit = 0;
   double distance = closest_points(A->sh, B->sh, &a2w, &b2w, &pa, &pb, &nit);
+  *nbIter = nit;
   if (distance < ORC_EPSILON) return -1;
   pa = xf_apply(&a2w, pa);
   pb = xf_apply(&b2w, pb);
@@ -660,6 +684,48 @@ static int closest_point(const body_t* A, const body_t* B, v3* contact, v3* ov, 
   *ov = scl(o, A->crust + B->crust - distance);
   distance -= A->crust + B->crust;
   *dist = distance;
+  return 1;
+}
+
+/* Scale of one bounding-box axis shrunk by ten crusts, at least 0.5
+   (RigidBodyWithCrust::getTransformWithCrust(10, 0.5),
+   RigidBodyWithCrust.cpp:770-795). */
+static double retry_scale(double extent, double crust) {
+  double sc = extent < ORC_EPSILON ? 1. : (extent - 10. * crust) / extent;
+  return sc < 0.5 ? 0.5 : sc;
+}
+
+/* RigidBodyWithCrust::ClosestPoint_ErreurHandling (RigidBodyWithCrust.cpp:
+   534-588), called by Particle::InterAction (Particle.cpp:508-525) and
+   SimpleObstacle::InterAction after ClosestPoint raised the ContactError: GJK
+   between the bodies shrunk by ten crusts; contact at the midpoint of the
+   witness points, overlap vector the normalised B -> A direction times the
+   crust sum, overlap distance minus the crust sum.  -1: the shrunk bodies
+   still overlap (the error is raised). */
+static int closest_point_retry(const body_t* A, const body_t* B, v3* contact, v3* ov,
+                               double* dist, int64_t* ngjk, int* nbIter) {
+  double sa[3], sb[3];
+  for (int d = 0; d < 3; ++d) {
+    sa[d] = retry_scale(A->sh->bbox_ext[d], A->crust);
+    sb[d] = retry_scale(B->sh->bbox_ext[d], B->crust);
+  }
+  ++*ngjk;
+  xf_t a2w, b2w;
+  crust_xf(A->basis, A->origin, sa, &a2w);
+  crust_xf(B->basis, B->origin, sb, &b2w);
+  v3 pa, pb;
+  int nit = 0;
+  double distance = closest_points(A->sh, B->sh, &a2w, &b2w, &pa, &pb, &nit);
+  *nbIter = nit;
+  if (distance < ORC_EPSILON) return -1;
+  pa = xf_apply(&a2w, pa);
+  pb = xf_apply(&b2w, pb);
+  v3 ba = sub(pa, pb);
+  ba = dvd(ba, norm(ba));
+  *contact = add(dvd(pa, 2.0), dvd(pb, 2.0));
+  double imposed = 1. * (A->crust + B->crust);
+  *ov = scl(ba, imposed);
+  *dist = -imposed;
   return 1;
 }
 
@@ -796,9 +862,13 @@ static int cinteract(orc_csys* s, int p0, int p1, v3 shift) {
   body_t A = particle_body(s, p0, mk(0., 0., 0.)), B = particle_body(s, p1, shift);
   v3 c, ov;
   double dist;
-  int rc = closest_point(&A, &B, &c, &ov, &dist, &s->ngjk);
+  int nit = 0;
+  int rc = closest_point(&A, &B, &c, &ov, &dist, &s->ngjk, &nit);
+  if (rc < 0) rc = closest_point_retry(&A, &B, &c, &ov, &dist, &s->ngjk, &nit);
   if (rc < 0) return cset_error(s, ORC_ERR_CONTACT, p0, p1);
-  if (rc == 0 || !(dist < 0.)) return ORC_OK;
+  if (rc == 0) return ORC_OK;
+  c_contact_features(s, dist, nit);
+  if (!(dist < 0.)) return ORC_OK;
   const cmodel_t* m = &s->models[s->mat[p0]][s->mat[p1]];
   if (m->kind == ORC_MODEL_NONE) return cset_error(s, ORC_ERR_ARG, p0, p1);
   v3 xa = ld(A.origin), xb = ld(B.origin);
@@ -822,7 +892,9 @@ int orc_c_pair(orc_csys* s, int p0, int p1, const double* shift, double* point, 
   v3 c, ov;
   double dist;
   int64_t ng = 0;
-  int rc = closest_point(&A, &B, &c, &ov, &dist, &ng);
+  int nit = 0;
+  int rc = closest_point(&A, &B, &c, &ov, &dist, &ng, &nit);
+  if (rc < 0) rc = closest_point_retry(&A, &B, &c, &ov, &dist, &ng, &nit);
   if (rc <= 0) return rc;
   st(point, c);
   st(ovec, ov);
@@ -851,6 +923,7 @@ static int cinteract_wall(orc_csys* s, int p, int wi) {
   memset(&wsh, 0, sizeof(wsh));
   wsh.type = ORC_SHAPE_BOX;
   memcpy(wsh.ext, wl->ext, sizeof(wsh.ext));
+  memcpy(wsh.bbox_ext, wl->ext, sizeof(wsh.bbox_ext));
   body_t B;
   B.sh = &wsh;
   B.basis = wl->basis;
@@ -860,9 +933,14 @@ static int cinteract_wall(orc_csys* s, int p, int wi) {
   B.circ = wl->circ;
   v3 c, ov;
   double dist;
-  int rc = wl->rect ? crect(&A, wl, &c, &ov, &dist) : closest_point(&A, &B, &c, &ov, &dist, &s->ngjk);
+  int nit = 0;
+  int rc = wl->rect ? crect(&A, wl, &c, &ov, &dist)
+                    : closest_point(&A, &B, &c, &ov, &dist, &s->ngjk, &nit);
+  if (rc < 0) rc = closest_point_retry(&A, &B, &c, &ov, &dist, &s->ngjk, &nit);
   if (rc < 0) return cset_error(s, ORC_ERR_CONTACT, p, -1 - wi);
-  if (rc == 0 || !(dist < 0.)) return ORC_OK;
+  if (rc == 0) return ORC_OK;
+  c_contact_features(s, dist, nit);
+  if (!(dist < 0.)) return ORC_OK;
   const cmodel_t* m = &s->models[s->mat[p]][wl->material];
   if (m->kind == ORC_MODEL_NONE) return cset_error(s, ORC_ERR_ARG, p, -1 - wi);
   v3 zero = mk(0., 0., 0.);
@@ -884,6 +962,7 @@ int orc_c_wall_pair(orc_csys* s, int p, int wi, double* point, double* ovec, dou
   memset(&wsh, 0, sizeof(wsh));
   wsh.type = ORC_SHAPE_BOX;
   memcpy(wsh.ext, wl->ext, sizeof(wsh.ext));
+  memcpy(wsh.bbox_ext, wl->ext, sizeof(wsh.bbox_ext));
   body_t B;
   B.sh = &wsh;
   B.basis = wl->basis;
@@ -894,7 +973,9 @@ int orc_c_wall_pair(orc_csys* s, int p, int wi, double* point, double* ovec, dou
   v3 c, ov;
   double dist;
   int64_t ng = 0;
-  int rc = wl->rect ? crect(&A, wl, &c, &ov, &dist) : closest_point(&A, &B, &c, &ov, &dist, &ng);
+  int nit = 0;
+  int rc = wl->rect ? crect(&A, wl, &c, &ov, &dist) : closest_point(&A, &B, &c, &ov, &dist, &ng, &nit);
+  if (rc < 0) rc = closest_point_retry(&A, &B, &c, &ov, &dist, &ng, &nit);
   if (rc <= 0) return rc;
   st(point, c);
   st(ovec, ov);
@@ -1094,6 +1175,7 @@ int orc_c_prime(orc_csys* s) {
 
 int orc_c_step(orc_csys* s, int nsteps, double dt) {
   for (int k = 0; k < nsteps; ++k) {
+    s->time += dt;
     int rc = c_move(s, 0.5 * dt, dt);
     if (rc) return rc;
     rc = c_link_update(s);
@@ -1154,6 +1236,18 @@ int orc_c_last_error(const orc_csys* s, int* code, int32_t* id0, int32_t* id1) {
   if (id0) *id0 = s->err_id0;
   if (id1) *id1 = s->err_id1;
   return s->err_code;
+}
+
+/* AppCollision::getOverlapMax / getOverlapMean / getTimeOverlapMax /
+   getNbIterGJKMean (Base/src/AppCollision.cpp:205-240): out[0..3], and the
+   numbers of overlaps and of GJK reports counted, out[4..5]. */
+void orc_c_get_contact_stats(const orc_csys* s, double* out) {
+  out[0] = s->ov_max;
+  out[1] = s->ov_mean;
+  out[2] = s->ov_time_max;
+  out[3] = s->gjk_mean;
+  out[4] = s->ov_count;
+  out[5] = s->gjk_count;
 }
 
 int64_t orc_c_gjk_calls(const orc_csys* s) { return s->ngjk; }

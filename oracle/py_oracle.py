@@ -407,6 +407,7 @@ def _load_convex(lib):
         "orc_c_get_contacts": (i32, [vp, i32, P(i32), vp, vp, vp, vp, vp, vp]),
         "orc_c_last_error": (i32, [vp, P(i32), P(C.c_int32), P(C.c_int32)]),
         "orc_c_gjk_calls": (C.c_int64, [vp]),
+        "orc_c_get_contact_stats": (None, [vp, vp]),
         "orc_c_closest_points": (i32, [vp, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, P(i32), P(d)]),
         "orc_c_pair": (i32, [vp, i32, i32, vp, vp, vp, P(d), vp]),
         "orc_c_wall_pair": (i32, [vp, i32, i32, vp, vp, P(d), vp]),
@@ -528,6 +529,13 @@ class ConvexOracle:
 
     def gjk_calls(self):
         return self.lib.orc_c_gjk_calls(self.h)
+
+    def contact_stats(self):
+        """AppCollision's contact features of the run, as Oracle.contact_stats."""
+        out = np.zeros(6)
+        self.lib.orc_c_get_contact_stats(self.h, _p(out))
+        return {"overlap_max": out[0], "overlap_mean": out[1], "time_overlap_max": out[2],
+                "gjk_mean": out[3], "overlaps": int(out[4]), "gjk_reports": int(out[5])}
 
     def pair(self, p0, p1, shift=(0.0, 0.0, 0.0)):
         """(rc, point, overlap vector, overlap, force on p0) of one pair with

@@ -108,6 +108,8 @@ struct gdem_ctx {
   uint32_t* d_omask = nullptr;
   uint32_t* d_imask = nullptr;
   PairRec* d_prec = nullptr;
+  int32_t* d_queue = nullptr;   // candidates that may touch (t * 32 + entry)
+  int32_t* d_qcount = nullptr;  // its length, reset by the integrator
   // history (memory) contact models: per-particle memory tables, one per
   // state copy; allocated once a memory model is set and particles exist
   bool has_memory = false;
@@ -276,9 +278,19 @@ static void free_pack(gdem_ctx* c) {
     c->pkc[k] = PackConst{};
   }
   (void)hipFree(c->d_omask); (void)hipFree(c->d_imask); (void)hipFree(c->d_prec);
+  (void)hipFree(c->d_queue); (void)hipFree(c->d_qcount);
   c->d_omask = c->d_imask = nullptr;
   c->d_prec = nullptr;
+  c->d_queue = c->d_qcount = nullptr;
   c->packed = false;
+}
+
+// Capacity of one segment of the pair queue: the out entries of the
+// screening workgroups that append to it (workgroups b = k mod kQueueSegs).
+static int64_t queue_seg(const gdem_ctx* c) {
+  const int64_t nblk = (std::max<int64_t>(c->n, 1) + 255) / 256;
+  const int64_t nblk8 = (nblk + 7) / 8 * 8;
+  return (nblk8 + kQueueSegs - 1) / kQueueSegs * 256 * kListCap;
 }
 
 // Buffers of the packed layout for the current particle count (allocated on
@@ -299,6 +311,11 @@ static int alloc_pack(gdem_ctx* c) {
   HIP_TRY(hipMalloc(&c->d_imask, sizeof(uint32_t) * n));
   HIP_TRY(hipMalloc(&c->d_prec, sizeof(PairRec) * n * kListCap));
   HIP_TRY(hipMemset(c->d_imask, 0, sizeof(uint32_t) * n));
+  // every out entry at most once per step: a segment holds the entries of
+  // the screening workgroups (256 particles) that append to it
+  HIP_TRY(hipMalloc(&c->d_queue, sizeof(int32_t) * (size_t)queue_seg(c) * kQueueSegs));
+  HIP_TRY(hipMalloc(&c->d_qcount, sizeof(int32_t) * kQueueSegs));
+  HIP_TRY(hipMemset(c->d_qcount, 0, sizeof(int32_t) * kQueueSegs));
   return GDEM_OK;
 }
 
@@ -326,10 +343,12 @@ static int ensure_packed(gdem_ctx* c) {
 }
 
 // The packed layout serves gdem_step for spheres with skin lists, memoryless
-// contact models, static walls, and neither contact recording nor wall loads.
+// contact models, static walls, and neither contact recording, wall loads nor
+// contact statistics (whose sums need the fixed evaluation order of the
+// canonical kernels).
 static bool pack_eligible(const gdem_ctx* c) {
   return !c->convex && c->n > 0 && c->skin > 0. && !c->has_memory && c->rec_cap == 0 &&
-         !c->track_loads;
+         !c->track_loads && !c->stats_on;
 }
 
 // The next evaluation rebuilds the skin lists; the drift bound restarts.
@@ -791,7 +810,7 @@ static int add_wall_kind(gdem_ctx* c, const double* basis, const double* centre,
   // Box shape, crust scaling of its extents, circumscribed radius
   CShape wsh{};
   wsh.type = SHAPE_BOX;
-  for (int d = 0; d < 3; ++d) wsh.ext[d] = half[d];
+  for (int d = 0; d < 3; ++d) wsh.ext[d] = wsh.bbox_ext[d] = half[d];
   double wsc[3];
   crust_scale(half, crust, wsc);
   c->h_wall_shapes[c->nwalls] = wsh;
@@ -918,6 +937,7 @@ static void launch_convex_step(gdem_ctx* c, int mode, double dtv, double dtd, bo
   g.in_list = c->d_in_list;
   g.in_count = c->d_in_count;
   g.err = c->d_err;
+  g.stats = c->stats_on ? 1 : 0;
   CForceArgs a{};
   a.in = c->buf[c->cur];
   a.out = c->buf[1 - c->cur];
@@ -934,6 +954,7 @@ static void launch_convex_step(gdem_ctx* c, int mode, double dtv, double dtd, bo
   a.dt_vel = dtv;
   a.dt_disp = dtd;
   a.cand = c->d_cand;
+  a.cand_count = c->d_out_count;
   a.rec = c->d_crec;
   a.mask = c->d_mask;
   a.in_list = c->d_in_list;
@@ -948,10 +969,14 @@ static void launch_convex_step(gdem_ctx* c, int mode, double dtv, double dtd, bo
   a.rec_count = c->d_rec_count;
   a.rec_cap = c->rec_cap;
   a.nblk8 = nblk8;
+  a.stat = c->stats_on ? c->d_stat_part : nullptr;
   launch_cdetect(d, c->stream);
   // a queue of about 4 candidates per particle is covered in one pass
   launch_cgjk(g, (int32_t)std::min<int64_t>(4 * c->n, 0x7fffffff), c->stream);
   launch_cforce(mode, record, a, c->stream);
+  if (c->stats_on)
+    launch_stats_final(c->d_stat_part, nblk8, c->d_stats, mode == MODE_FORCE ? 0. : dtd,
+                       c->stream);
 }
 
 // Kernels of one evaluation in the packed layout: gated binning, pair kernel
@@ -992,7 +1017,9 @@ static int pack_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wri
   pa.models = c->d_models;
   pa.model_stride = kMaxMaterials;
   pa.contacts = c->d_blk_contacts;
-  pa.stat = c->stats_on ? c->d_stat_part : nullptr;
+  pa.queue = c->d_queue;
+  pa.qcount = c->d_qcount;
+  pa.qseg = (int32_t)std::min<int64_t>(queue_seg(c), 0x7fffffff);
   TimedLaunch* ev = c->timing ? next_event_pair(c, 1) : nullptr;
   launch_pair(pa, c->stream, ev ? ev->start : nullptr, ev ? ev->stop : nullptr);
   HIP_TRY(hipGetLastError());
@@ -1028,15 +1055,10 @@ static int pack_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wri
   a.err = c->d_err;
   a.contacts = c->d_blk_contacts;
   a.nblk8 = nblk8;
-  a.stat = c->stats_on ? c->d_stat_part + nblk8 : nullptr;
+  a.qcount = c->d_qcount;
   ev = c->timing ? next_event_pair(c, 0) : nullptr;
   launch_integ(mode, a, c->stream, ev ? ev->start : nullptr, ev ? ev->stop : nullptr);
   HIP_TRY(hipGetLastError());
-  if (c->stats_on) {
-    launch_stats_final(c->d_stat_part, c->nwalls > 0 ? 2 * nblk8 : nblk8, c->d_stats,
-                       mode == MODE_FORCE ? 0. : dtd, c->stream);
-    HIP_TRY(hipGetLastError());
-  }
   return GDEM_OK;
 }
 
@@ -2037,7 +2059,7 @@ extern "C" int gdem_reset_counters(gdem_ctx* c) {
 }
 
 extern "C" int gdem_set_contact_stats(gdem_ctx* c, int32_t on) {
-  if (!c || c->convex) return GDEM_ERR_ARG;
+  if (!c) return GDEM_ERR_ARG;
   HIP_TRY(hipSetDevice(c->dev));
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (on && !c->d_stats) HIP_TRY(hipMalloc(&c->d_stats, sizeof(ContactStats)));

@@ -389,7 +389,8 @@ struct CBody {
 // 1 = candidate (dist may be >= 0), 0 = no bounding-volume overlap,
 // -1 = ContactError (shrunk bodies overlap).
 __host__ __device__ inline int closest_point(const CBody& A, const CBody& B, v3& contact, v3& ov,
-                                             double& dist) {
+                                             double& dist, int& nbIter) {
+  nbIter = 0;
   const v3 gab = sub(A.origin, B.origin);
   if (!(norm(gab) < A.circ + B.circ)) return 0;
   const Xf a2w = crust_xf(A.basis, A.origin, A.scale);
@@ -397,6 +398,7 @@ __host__ __device__ inline int closest_point(const CBody& A, const CBody& B, v3&
   v3 pa, pb;
   int nit = 0;
   double distance = closest_points(*A.sh, *B.sh, a2w, b2w, pa, pb, nit);
+  nbIter = nit;
   if (distance < kEps) return -1;
   pa = xf_apply(a2w, pa);
   pb = xf_apply(b2w, pb);
@@ -408,6 +410,31 @@ __host__ __device__ inline int closest_point(const CBody& A, const CBody& B, v3&
   distance -= A.crust + B.crust;
   dist = distance;
   return 1;
+}
+
+// ClosestPoint_ErreurHandling (RigidBodyWithCrust.cpp:534-588) for two
+// convex bodies, after ClosestPoint found the crust-shrunk bodies overlapping:
+// Johnson GJK between the bodies shrunk by ten crusts (getTransformWithCrust
+// (10, 0.5), :770-795: per bounding-box axis (extent - 10 crust) / extent, at
+// least 0.5); contact at the midpoint of the witness points, overlap vector
+// the unit direction B -> A times the crust sum, overlap minus the crust sum.
+// -1: the shrunk bodies still overlap (the second-level ContactError).
+__host__ __device__ inline int closest_point_retry(const CBody& A, const CBody& B, v3& contact,
+                                                   v3& ov, double& dist, int& nbIter) {
+  double sa[3], sb[3];
+  for (int d = 0; d < 3; ++d) {
+    sa[d] = retry_scale(A.sh->bbox_ext[d], A.crust);
+    sb[d] = retry_scale(B.sh->bbox_ext[d], B.crust);
+  }
+  const Xf a2w = crust_xf(A.basis, A.origin, sa);
+  const Xf b2w = crust_xf(B.basis, B.origin, sb);
+  v3 pa, pb;
+  int nit = 0;
+  const double distance = closest_points(*A.sh, *B.sh, a2w, b2w, pa, pb, nit);
+  nbIter = nit;
+  if (distance < kEps) return -1;
+  return retry_contact(xf_apply(a2w, pa), xf_apply(b2w, pb), distance, A.crust, B.crust, contact,
+                       ov, dist);
 }
 
 // ClosestPointRECTANGLE with a convex particle first (RigidBodyWithCrust.cpp:

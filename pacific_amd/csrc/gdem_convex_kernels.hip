@@ -175,11 +175,19 @@ __global__ void __launch_bounds__(kCBlock) k_cgjk(CGjkArgs a) {
                     wscale, wl.crust, wsc.w};
       v3 c = mk(0., 0., 0.), ov = mk(0., 0., 0.);
       double dist = 0.;
-      const int rc = wl.kind == WALL_RECTANGLE ? convex_rect(A, wl, c, ov, dist)
-                                               : closest_point(A, B, c, ov, dist);
+      int nit = 0;  // ClosestPointRECTANGLE reports none
+      int rc = wl.kind == WALL_RECTANGLE ? convex_rect(A, wl, c, ov, dist)
+                                         : closest_point(A, B, c, ov, dist, nit);
+      if (rc < 0) rc = closest_point_retry(A, B, c, ov, dist, nit);
       if (rc < 0) {
         atomicMin(&a.err[kErrContact], pack_ids(ax1.id, -1 - u));
         continue;
+      }
+      if (a.stats) {
+        CRec& r = a.rec[item];
+        r.dist = dist;
+        r.nit = nit;
+        r.evaluated = rc != 0;
       }
       if (rc == 0 || !(dist < 0.)) continue;
       CRec& r = a.rec[item];
@@ -208,10 +216,18 @@ __global__ void __launch_bounds__(kCBlock) k_cgjk(CGjkArgs a) {
     }
     v3 c, ov;
     double dist = 0.;
-    const int rc = closest_point(A, B, c, ov, dist);
+    int nit = 0;
+    int rc = closest_point(A, B, c, ov, dist, nit);
+    if (rc < 0) rc = closest_point_retry(A, B, c, ov, dist, nit);
     if (rc < 0) {
       atomicMin(&a.err[kErrContact], first ? pack_ids(ax1.id, ax2.id) : pack_ids(ax2.id, ax1.id));
       continue;
+    }
+    if (a.stats) {
+      CRec& r = a.rec[item];
+      r.dist = dist;
+      r.nit = nit;
+      r.evaluated = rc != 0;
     }
     if (rc == 0 || !(dist < 0.)) continue;
     CRec& r = a.rec[item];
@@ -344,9 +360,20 @@ __device__ __forceinline__ void wrap_periodic(const Grid& g, v3& X) {
 template <int MODE, bool RECORD>
 __global__ void __launch_bounds__(kCBlock) k_cforce(CForceArgs a) {
   __shared__ unsigned long long sh_cnt[kCBlock / 64];
+  __shared__ StatPart sh_stat[kCBlock / 64];
   const int t = logical_index(a.nblk8);
   int ncont = 0;
+  StatAcc st;
   if (t < a.n) {
+    if (a.stat) {
+      // AppCollision::addToContactsFeatures of every closest point this
+      // particle's candidates ran (its own detections, in candidate order)
+      const int nc = min(a.cand_count[t], kCandCap);
+      for (int j = 0; j < nc; ++j) {
+        const CRec& r = a.rec[(size_t)t * kCandCap + j];
+        if (r.evaluated) st.add(r.dist, r.nit);
+      }
+    }
     const int s = a.perm[t];
     const double4 pr = a.in.pos_r[s];
     const double4 vm = a.in.vel_m[s];
@@ -467,6 +494,7 @@ __global__ void __launch_bounds__(kCBlock) k_cforce(CForceArgs a) {
     for (int w = 0; w < kCBlock / 64; ++w) tot += sh_cnt[w];
     if (tot) atomicAdd(a.contacts, tot);
   }
+  if (a.stat) block_stats<kCBlock>(st, a.stat, sh_stat);
 }
 
 // ---------------------------------------------------------------- integrator phases

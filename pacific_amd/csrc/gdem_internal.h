@@ -252,7 +252,8 @@ struct CState {
 // sets bit j of the detecting particle's mask and fills record (t, j).
 struct CRec {
   double c[3], ov[3], dist;
-  int32_t pad[2];
+  int32_t nit;        // GJK iterations the closest point reported (statistics)
+  int32_t evaluated;  // the closest point ran this step (statistics)
 };
 constexpr int kCandCap = 32;
 constexpr int kWallCode = 128;
@@ -295,6 +296,7 @@ struct CGjkArgs {
   int2* in_list;       // [n][kCandCap]: (detecting slot, candidate index)
   int32_t* in_count;   // [n], zero on entry
   unsigned long long* err;
+  int32_t stats;       // record dist and GJK iterations of every evaluated candidate
 };
 
 struct CForceArgs {
@@ -310,11 +312,13 @@ struct CForceArgs {
   double gravity[3];
   double dt_vel, dt_disp;
   const int2* cand;
+  const int32_t* cand_count;
   const CRec* rec;
   const uint32_t* mask;
   const int2* in_list;
   int32_t* in_count;
   int32_t* nwork;      // reset to zero for the next step
+  StatPart* stat;      // contact statistics, one part per workgroup (nullable)
   double4* F;
   double4* T;
   int32_t* key_out;
@@ -419,17 +423,23 @@ struct PairRec {
   double f[3], ta[3], tb[3];
 };
 
+// Segments of the pair queue (workgroup b of the screening appends to
+// segment b mod kQueueSegs).
+constexpr int kQueueSegs = 64;
+
 struct PairArgs {
   DetectArgs d;        // the candidate sweep of a rebuild (d.in unused)
   Pack in;
   PackConst kin;
-  uint32_t* omask;     // [n] out entries in contact (written)
+  uint32_t* omask;     // [n] out entries in contact (cleared by the screening)
   uint32_t* imask;     // [n] in entries in contact (atomic OR, zero on entry)
   PairRec* rec;
   const Model* models;
   int32_t model_stride;
   unsigned long long* contacts;  // contact counts, one per workgroup (added to)
-  StatPart* stat;      // contact statistics, one part per workgroup (nullable)
+  int32_t* queue;      // candidates that may touch, t * 32 + entry, in segments
+  int32_t* qcount;     // [kQueueSegs] segment lengths (zero on entry)
+  int32_t qseg;        // capacity of one segment
 };
 
 struct IntegArgs {
@@ -460,7 +470,7 @@ struct IntegArgs {
   unsigned long long* err;
   unsigned long long* contacts;  // per workgroup (walls)
   int32_t nblk8;
-  StatPart* stat;      // wall contacts' statistics, one part per workgroup (nullable)
+  int32_t* qcount;     // the pair queue's segment lengths, cleared for the next step
 };
 
 void launch_to_pack(const State& s, const Pack& pk, const PackConst& k0, const PackConst& k1,

@@ -1,27 +1,33 @@
 // gdem_pack.hip -- the sphere step in the packed layout (skin candidate
-// lists, memoryless contact models), as two streaming kernels per step:
+// lists, memoryless contact models), as three kernels per step:
 //
-//   k_pair  : one thread per particle walks the candidates it detected (each
-//             pair sits in exactly one out-list), screens them against the
-//             positions of the step, evaluates every pair that touches once
-//             with the reference's body order (RigidBodyWithCrust.cpp:596-640,
-//             HookeContactForceModel.cpp:44-114 / HertzContactForceModel.cpp:
-//             46-114) and stores its record (force on the first body and both
-//             torques) at the out-list entry; it flags the entry in its own
-//             mask and the mirror entry in the partner's mask (integer OR).
-//             On a rebuild step it first sweeps the half stencil
-//             (LinkedCell.cpp:1019-1103) to rebuild the lists.
-//   k_integ : one thread per particle sums its flagged records in a fixed
-//             order -- detected contacts by entry, then received contacts by
-//             partner slot -- (Torsor.cpp:184-189, no floating-point
-//             atomics), adds the walls, and applies the leap-frog tail
-//             (SecondOrderLeapFrog.cpp:44-94 via ParticleKinematics.cpp:
-//             62-128), writing the other state copy.
+//   k_screen  : one thread per particle walks the candidates it detected
+//               (each pair sits in exactly one out-list) and screens them
+//               against the positions of the step; the candidates that may
+//               touch go to a queue.  On a rebuild step it first sweeps the
+//               half stencil (LinkedCell.cpp:1019-1103) to rebuild the lists.
+//   k_contact : one lane per queued candidate evaluates the pair once with
+//               the reference's body order (RigidBodyWithCrust.cpp:596-640,
+//               HookeContactForceModel.cpp:44-114 / HertzContactForceModel.cpp:
+//               46-114) and stores its record (force on the first body and
+//               both torques) at the out-list entry; it flags the entry in the
+//               particle's out mask and the mirror entry in the partner's in
+//               mask (integer OR).
+//   k_integ   : one thread per particle sums its flagged records in a fixed
+//               order -- detected contacts by entry, then received contacts
+//               by partner slot -- (Torsor.cpp:184-189, no floating-point
+//               atomics), adds the walls, and applies the leap-frog tail
+//               (SecondOrderLeapFrog.cpp:44-94 via ParticleKinematics.cpp:
+//               62-128), writing the other state copy.
 //
 // The per-pair arithmetic and the order of every sum are those of the fused
-// kernel (gdem_sphere_step.h), so both give bit-identical results.
+// kernel (gdem_sphere_step.h), so both give bit-identical results.  Contact
+// statistics need fixed-order sums over the evaluations, which the queue does
+// not give; with statistics on, gdem_step runs the canonical layout.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "gdem_internal.h"
 #include "gdem_sphere_step.h"
@@ -70,10 +76,11 @@ __global__ void __launch_bounds__(kPB) k_from_pack(Pack pk, PackConst k, State s
 }
 
 // ---------------------------------------------------------------- pairs
-// Candidates of particle t (slot order after the binning of a rebuild; the
-// state is read through perm then), screened and evaluated.  Returns the
-// number of pairs in contact.
-__device__ __forceinline__ int pair_particle(const PairArgs& a, int t, bool rebuild, StatAcc& st) {
+// Screening of the candidates of particle t (slot order after the binning of
+// a rebuild; the state is read through perm then): the out entries whose
+// partner is not clearly apart (apart()), as a bit mask.  Partner positions
+// are loaded four at a time with independent loads.
+__device__ __forceinline__ uint32_t screen_particle(const PairArgs& a, int t, bool rebuild) {
   const DetectArgs& d = a.d;
   const PosSrc src{a.in.A, a.kin.rr, a.kin.aux};
   int nout = rebuild ? detect_particle(d, src, t) : d.out_count[t];
@@ -83,8 +90,6 @@ __device__ __forceinline__ int pair_particle(const PairArgs& a, int t, bool rebu
   const v3 X = mk(me.x, me.y, me.z);
   const double r = me.w;
   const int2* ol = d.out_list + t;
-  // candidates that may touch: partner positions loaded four at a time with
-  // independent loads; most candidates are clearly apart (apart())
   uint32_t may = 0;
   for (int e0 = 0; e0 < nout; e0 += 4) {
     int pp[4];
@@ -98,71 +103,73 @@ __device__ __forceinline__ int pair_particle(const PairArgs& a, int t, bool rebu
     for (int k = 0; k < 4; ++k)
       if (pp[k] >= 0 && !apart(d.grid, X, r, q[k])) may |= 1u << (e0 + k);
   }
-  uint32_t om = 0;
-  int nc = 0;
-  if (may) {
-    const double4 b = a.in.B[s], c4 = a.in.C[s];
-    const double2 rr = a.kin.rr[s], mi = a.kin.mi[s];
-    const Aux ax = a.kin.aux[s];
-    const v3 V = mk(b.x, b.y, b.z), W = mk(c4.x, c4.y, c4.z);
-    const double m = mi.x, req = rr.y;
-    for (uint32_t mm = may; mm != 0u; mm &= mm - 1u) {
-      const int e = __builtin_ctz(mm);
-      const int2 en = ol[(size_t)e * d.lstride];
-      const int u = en.x, code = en.y;
-      const int su = rebuild ? d.perm[u] : u;
-      const double4 p2 = src.pos(su);
-      const double4 b2 = a.in.B[su], c2 = a.in.C[su];
-      const double2 rr2 = a.kin.rr[su], mi2 = a.kin.mi[su];
-      const Aux ax2 = a.kin.aux[su];
-      const v3 V2 = mk(b2.x, b2.y, b2.z), W2 = mk(c2.x, c2.y, c2.z);
-      // either body may have crossed a periodic face since the list was
-      // built: the partner is taken at its nearest image
-      const int sx = image_shift(d.grid, 0, p2.x - X.x);
-      const int sy = image_shift(d.grid, 1, p2.y - X.y);
-      const int sz = image_shift(d.grid, 2, p2.z - X.z);
-      const double shx = sx * d.grid.size[0], shy = sy * d.grid.size[1],
-                   shz = sz * d.grid.size[2];
-      const bool first = (code & 64) != 0;
-      v3 A, B, cp, ov;
-      double rA, rB, crA, crB, dist;
-      if (first) {
-        A = X; B = mk(p2.x + shx, p2.y + shy, p2.z + shz);
-        rA = r; rB = p2.w; crA = ax.crust; crB = ax2.crust;
-      } else {
-        A = mk(p2.x, p2.y, p2.z);
-        B = mk(X.x + (-shx), X.y + (-shy), X.z + (-shz));
-        rA = p2.w; rB = r; crA = ax2.crust; crB = ax.crust;
-      }
-      int rc = sphere_sphere(A, rA, crA, B, rB, crB, cp, ov, dist);
-      int nbIter = 1;  // ClosestPointSPHERESPHERE reports one iteration
-      if (rc < 0) rc = sphere_sphere_retry(A, rA, crA, B, rB, crB, cp, ov, dist, nbIter);
-      if (rc < 0) {
-        atomicMin(&d.err[kErrContact], first ? pack_ids(ax.id, ax2.id) : pack_ids(ax2.id, ax.id));
-        continue;
-      }
-      if (rc == 0) continue;
-      if (a.stat) st.add(dist, nbIter);
-      if (!(dist < 0.)) continue;
-      const Model md =
-          a.models[(first ? ax.mat : ax2.mat) * a.model_stride + (first ? ax2.mat : ax.mat)];
-      const v3 f = first
-                       ? contact_force(md, cp, ov, dist, A, V, W, m, req, B, V2, W2, mi2.x, rr2.y)
-                       : contact_force(md, cp, ov, dist, A, V2, W2, mi2.x, rr2.y, B, V, W, m, req);
-      const v3 ta = cross(sub(cp, A), f);
-      const v3 tb = cross(sub(cp, B), neg(f));
-      PairRec& R = a.rec[(size_t)e * d.lstride + t];
-      R.f[0] = f.x; R.f[1] = f.y; R.f[2] = f.z;
-      R.ta[0] = ta.x; R.ta[1] = ta.y; R.ta[2] = ta.z;
-      R.tb[0] = tb.x; R.tb[1] = tb.y; R.tb[2] = tb.z;
-      om |= 1u << e;
-      const int fi = (code >> 7) & 31;  // the mirror entry in the partner's in-list
-      if (fi < kListCap) atomicOr(&a.imask[u], 1u << fi);
-      ++nc;
-    }
+  return may;
+}
+
+// One queued candidate: out entry e of particle t, evaluated once with the
+// reference's body order (RigidBodyWithCrust.cpp:596-640, the contact-error
+// retry of :534-588, HookeContactForceModel.cpp:44-114 /
+// HertzContactForceModel.cpp:46-114).  A contact stores its record at the
+// entry and flags the entry in t's out mask and the mirror entry in the
+// partner's in mask (integer OR).  Returns 1 for a contact.
+__device__ __forceinline__ int contact_pair(const PairArgs& a, int t, int e, bool rebuild) {
+  const DetectArgs& d = a.d;
+  const PosSrc src{a.in.A, a.kin.rr, a.kin.aux};
+  const int s = rebuild ? d.perm[t] : t;
+  const double4 me = src.pos(s);
+  const v3 X = mk(me.x, me.y, me.z);
+  const double r = me.w;
+  const int2 en = d.out_list[(size_t)e * d.lstride + t];
+  const int u = en.x, code = en.y;
+  const int su = rebuild ? d.perm[u] : u;
+  const double4 p2 = src.pos(su);
+  const double4 b = a.in.B[s], c4 = a.in.C[s];
+  const double2 rr = a.kin.rr[s], mi = a.kin.mi[s];
+  const Aux ax = a.kin.aux[s];
+  const double4 b2 = a.in.B[su], c2 = a.in.C[su];
+  const double2 rr2 = a.kin.rr[su], mi2 = a.kin.mi[su];
+  const Aux ax2 = a.kin.aux[su];
+  const v3 V = mk(b.x, b.y, b.z), W = mk(c4.x, c4.y, c4.z);
+  const v3 V2 = mk(b2.x, b2.y, b2.z), W2 = mk(c2.x, c2.y, c2.z);
+  // either body may have crossed a periodic face since the list was built:
+  // the partner is taken at its nearest image
+  const int sx = image_shift(d.grid, 0, p2.x - X.x);
+  const int sy = image_shift(d.grid, 1, p2.y - X.y);
+  const int sz = image_shift(d.grid, 2, p2.z - X.z);
+  const double shx = sx * d.grid.size[0], shy = sy * d.grid.size[1], shz = sz * d.grid.size[2];
+  const bool first = (code & 64) != 0;
+  v3 A, B, cp, ov;
+  double rA, rB, crA, crB, dist;
+  if (first) {
+    A = X; B = mk(p2.x + shx, p2.y + shy, p2.z + shz);
+    rA = r; rB = p2.w; crA = ax.crust; crB = ax2.crust;
+  } else {
+    A = mk(p2.x, p2.y, p2.z);
+    B = mk(X.x + (-shx), X.y + (-shy), X.z + (-shz));
+    rA = p2.w; rB = r; crA = ax2.crust; crB = ax.crust;
   }
-  a.omask[t] = om;
-  return nc;
+  int rc = sphere_sphere(A, rA, crA, B, rB, crB, cp, ov, dist);
+  int nbIter = 1;
+  if (rc < 0) rc = sphere_sphere_retry(A, rA, crA, B, rB, crB, cp, ov, dist, nbIter);
+  if (rc < 0) {
+    atomicMin(&d.err[kErrContact], first ? pack_ids(ax.id, ax2.id) : pack_ids(ax2.id, ax.id));
+    return 0;
+  }
+  if (rc == 0 || !(dist < 0.)) return 0;
+  const Model md = a.models[(first ? ax.mat : ax2.mat) * a.model_stride + (first ? ax2.mat : ax.mat)];
+  const double m = mi.x, req = rr.y;
+  const v3 f = first ? contact_force(md, cp, ov, dist, A, V, W, m, req, B, V2, W2, mi2.x, rr2.y)
+                     : contact_force(md, cp, ov, dist, A, V2, W2, mi2.x, rr2.y, B, V, W, m, req);
+  const v3 ta = cross(sub(cp, A), f);
+  const v3 tb = cross(sub(cp, B), neg(f));
+  PairRec& R = a.rec[(size_t)e * d.lstride + t];
+  R.f[0] = f.x; R.f[1] = f.y; R.f[2] = f.z;
+  R.ta[0] = ta.x; R.ta[1] = ta.y; R.ta[2] = ta.z;
+  R.tb[0] = tb.x; R.tb[1] = tb.y; R.tb[2] = tb.z;
+  atomicOr(&a.omask[t], 1u << e);
+  const int fi = (code >> 7) & 31;  // the mirror entry in the partner's in-list
+  if (fi < kListCap) atomicOr(&a.imask[u], 1u << fi);
+  return 1;
 }
 
 __device__ __forceinline__ void count_contacts(unsigned long long* counter, int nc,
@@ -178,16 +185,76 @@ __device__ __forceinline__ void count_contacts(unsigned long long* counter, int 
   }
 }
 
-__global__ void __launch_bounds__(kPB) k_pair(PairArgs a) {
-  __shared__ unsigned long long sh_cnt[kPB / 64];
-  __shared__ StatPart sh_stat[kPB / 64];
+// Candidate sweep (rebuild steps) and screening: one thread per particle;
+// the candidates that may touch are queued as t * 32 + e.  The queue has
+// kQueueSegs segments (workgroup b appends to segment b mod kQueueSegs, one
+// integer atomic per workgroup), so no counter sees more than a few hundred
+// atomics per step.  Clears the out masks for the contact kernel.
+__global__ void __launch_bounds__(kPB) k_screen(PairArgs a) {
+  __shared__ int sh_wave[kPB / 64];
+  __shared__ int sh_base;
   const int t = logical_index(a.d.nblk8);
   const bool rebuild = a.d.gate.rebuild();
+  uint32_t may = 0;
+  if (t < a.d.n) {
+    may = screen_particle(a, t, rebuild);
+    a.omask[t] = 0u;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int cnt = __builtin_popcount(may);
+  int incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) sh_wave[wid] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int w = 0; w < kPB / 64; ++w) tot += sh_wave[w];
+    sh_base = tot ? atomicAdd(&a.qcount[blockIdx.x % kQueueSegs], tot) : 0;
+  }
+  __syncthreads();
+  int pos = sh_base + incl - cnt;
+  for (int w = 0; w < wid; ++w) pos += sh_wave[w];
+  int32_t* seg = a.queue + (size_t)(blockIdx.x % kQueueSegs) * a.qseg;
+  for (uint32_t mm = may; mm != 0u; mm &= mm - 1u) {
+    if (pos < a.qseg) seg[pos] = t * 32 + __builtin_ctz(mm);
+    else atomicMin(&a.d.err[kErrCapacity], (unsigned long long)(uint32_t)t);
+    ++pos;
+  }
+}
+
+// The queued candidates, one per lane (grid-stride over the segments'
+// lengths the screening left); every contact record lands at its fixed
+// entry, so the order in which the queue was filled does not matter.
+__global__ void __launch_bounds__(kPB) k_contact(PairArgs a) {
+  __shared__ unsigned long long sh_cnt[kPB / 64];
+  __shared__ int sh_pre[kQueueSegs + 1];
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int k = 0; k < kQueueSegs; ++k) {
+      sh_pre[k] = acc;
+      acc += min(a.qcount[k], a.qseg);
+    }
+    sh_pre[kQueueSegs] = acc;
+  }
+  __syncthreads();
+  const int nq = sh_pre[kQueueSegs];
+  const bool rebuild = a.d.gate.rebuild();
   int nc = 0;
-  StatAcc st;
-  if (t < a.d.n) nc = pair_particle(a, t, rebuild, st);
+  for (int q = blockIdx.x * kPB + threadIdx.x; q < nq; q += gridDim.x * kPB) {
+    int lo = 0, hi = kQueueSegs;  // the segment k with pre[k] <= q < pre[k + 1]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (sh_pre[mid] <= q) lo = mid;
+      else hi = mid;
+    }
+    const int w = a.queue[(size_t)lo * a.qseg + (q - sh_pre[lo])];
+    nc += contact_pair(a, w >> 5, w & 31, rebuild);
+  }
   count_contacts(a.contacts, nc, sh_cnt);
-  if (a.stat) block_stats<kPB>(st, a.stat, sh_stat);
 }
 
 // ---------------------------------------------------------------- integrate
@@ -220,7 +287,7 @@ __device__ __forceinline__ bool wrap_axis(const IntegArgs& a, int d, double& x) 
 // Returns its wall contacts; drift = length of its displacement.
 template <int MODE, bool WALLS>
 __device__ __forceinline__ int integ_particle(const IntegArgs& a, int t, int s, bool copy_const,
-                                              double& drift, StatAcc& st) {
+                                              double& drift) {
   const double4 A4 = a.in.A[s], B4 = a.in.B[s], C4 = a.in.C[s], D4 = a.in.D[s];
   const double2 mi = a.kin.mi[s];
   const uint32_t om = a.omask[t];
@@ -268,7 +335,7 @@ __device__ __forceinline__ int integ_particle(const IntegArgs& a, int t, int s, 
         continue;
       v3 c = mk(0., 0., 0.), ov = mk(0., 0., 0.);
       double dist;
-      int nbIter = wl.kind == WALL_RECTANGLE ? 0 : 1;  // ClosestPointRECTANGLE reports none
+      int nbIter = 1;
       int rc = wl.kind == WALL_RECTANGLE ? sphere_rect(wl, X, rr.x, c, ov, dist)
                                          : sphere_wall(wl, X, rr.x, ax.crust, c, ov, dist);
       if (rc < 0) rc = sphere_wall_retry(wl, X, rr.x, ax.crust, c, ov, dist, nbIter);
@@ -277,7 +344,6 @@ __device__ __forceinline__ int integ_particle(const IntegArgs& a, int t, int s, 
         continue;
       }
       if (rc == 0 || !(dist < 0.)) continue;
-      if (a.stat) st.add(dist, nbIter);
       const Model md = a.models[ax.mat * a.model_stride + wl.material];
       // the wall's velocity at the contact point (Obstacle.cpp:128-133)
       const v3 wo = mk(wl.origin[0], wl.origin[1], wl.origin[2]);
@@ -366,9 +432,8 @@ __global__ void __launch_bounds__(kPB) k_integ(IntegArgs a) {
   const bool copy_const = rebuild || a.words->rebuilt[p] != 0;
   double drift = 0.;
   int nwc = 0;
-  StatAcc st;
   if (t < a.n) {
-    nwc = integ_particle<MODE, WALLS>(a, t, rebuild ? a.perm[t] : t, copy_const, drift, st);
+    nwc = integ_particle<MODE, WALLS>(a, t, rebuild ? a.perm[t] : t, copy_const, drift);
     // path length since the rebuild; need word of the next evaluation (SkinWords)
     if (MODE == MODE_MOVE || rebuild) {
       const float len = skin_path(rebuild ? 0.f : a.path[t], rebuild, drift);
@@ -377,11 +442,9 @@ __global__ void __launch_bounds__(kPB) k_integ(IntegArgs a) {
     }
   }
   if (t == 0) a.words->rebuilt[1 - p] = rebuild ? 1 : 0;
-  if (WALLS) {
-    count_contacts(a.contacts, nwc, sh_cnt);
-    __shared__ StatPart sh_stat[kPB / 64];
-    if (a.stat) block_stats<kPB>(st, a.stat, sh_stat);
-  }
+  // the queue of the next screening starts empty
+  if (blockIdx.x == 0 && threadIdx.x < kQueueSegs) a.qcount[threadIdx.x] = 0;
+  if (WALLS) count_contacts(a.contacts, nwc, sh_cnt);
 }
 
 // Kick + drift of copy pk in place (SecondOrderLeapFrog::Move via
@@ -458,10 +521,16 @@ void launch_from_pack(const Pack& pk, const PackConst& k, const State& s, int32_
 void launch_pair(const PairArgs& a, hipStream_t st, hipEvent_t start, hipEvent_t stop) {
   if (a.d.n <= 0) return;
   const dim3 grid(a.d.nblk8), block(kPB);
-  if (start)
-    hipExtLaunchKernelGGL(k_pair, grid, block, 0, st, start, stop, 0, a);
-  else
-    hipLaunchKernelGGL(k_pair, grid, block, 0, st, a);
+  // the queue holds a few candidates per hundred particles: a grid of an
+  // eighth of the particles' covers it in one pass (grid-stride beyond)
+  const dim3 cgrid(std::max(8, (a.d.nblk8 / 8 + 7) / 8 * 8));
+  if (start) {
+    hipExtLaunchKernelGGL(k_screen, grid, block, 0, st, start, nullptr, 0, a);
+    hipExtLaunchKernelGGL(k_contact, cgrid, block, 0, st, nullptr, stop, 0, a);
+  } else {
+    hipLaunchKernelGGL(k_screen, grid, block, 0, st, a);
+    hipLaunchKernelGGL(k_contact, cgrid, block, 0, st, a);
+  }
 }
 
 template <int M, bool WALLS>

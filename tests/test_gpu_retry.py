@@ -109,3 +109,86 @@ def test_statistics_of_a_settling_box(oracle_mod, skin):
     assert st_g["overlap_mean"] == pytest.approx(st_o["overlap_mean"], rel=1e-11)
     assert st_g["gjk_mean"] == 1.0 == st_o["gjk_mean"]
     g.close()
+
+
+def _cube_case(case):
+    """Two cubes driven together at 4 m/s on a soft spring (kn 1e3): the
+    overlap passes the crust sum for a few hundred steps; or one cube thrown
+    onto a box wall at 3 m/s, turned."""
+    import test_oracle_convex_retry_pins as cp
+    if case == "cubes":
+        return cp.two_cubes(-2e-7, q=(0.1, 0.2, 0.05, 0.97), v=2.0)
+    wl = cp.two_cubes(0.0)
+    p = wl["particles"]
+    for k in ("x", "v", "w", "q", "basis", "shape", "density", "crust", "material"):
+        p[k] = p[k][:1].copy()
+    q = np.array([0.3, 0.1, 0.2, 0.9])
+    q /= np.linalg.norm(q)
+    p["q"][0] = q
+    p["basis"][0] = cp._rot(q).reshape(-1)
+    p["x"][0] = (0.01, 0.01, 0.005 + np.sqrt(3) * cp.H + 2e-7)
+    p["v"][0] = (0.2, 0.0, -3.0)
+    wl["walls"] = [((1, 0, 0, 0, 1, 0, 0, 0, 1), (0.01, 0.01, 0.0025), (0.01, 0.01, 0.0025),
+                    cp.CR, 1)]
+    return wl
+
+
+@pytest.mark.parametrize("case", ["cubes", "cube_wall"])
+def test_convex_retried_contacts_follow_the_oracle(oracle_mod, case):
+    """Convex bodies pressed past the crust sum are retried with the bodies
+    shrunk by ten crusts (GJK), as the reference's run does: the contact
+    records equal the oracle's bit for bit at every 50th step and the
+    trajectories follow it within the trajectory bar."""
+    print(f"[case] convex retry {case}", flush=True)
+    wl = _cube_case(case)
+    dt = wl["dt"]
+    orc = oracle_mod.ConvexOracle(wl, log_capacity=16)
+    orc.prime()
+    g = gdem.build_system(wl)
+    g.set_contact_recording(8)
+    g.prime()
+    deepest, retried = 0.0, 0
+    for chunk in range(30):
+        orc.step(50, dt)
+        g.step(50, dt)
+        so, sg = orc.state(), g.get_state()
+        for k in ("x", "v", "w", "q"):
+            assert rel_err(sg[k], so[k]) <= 1e-9, (chunk, k)
+        oc, gc = orc.contacts(), g.get_contacts()
+        assert pair_set(gc) == pair_set(oc), chunk
+        for k in range(len(gc["id0"])):
+            assert np.array_equal(gc["point"][k], oc["point"][k]), chunk
+            assert np.array_equal(gc["ovec"][k], oc["ovec"][k]), chunk
+            assert gc["overlap"][k] == oc["overlap"][k], chunk
+            retried += int(gc["overlap"][k] == -2 * 5e-5)
+        if case == "cubes":
+            deepest = max(deepest, 2e-3 - np.linalg.norm(sg["x"][1] - sg["x"][0]))
+    assert retried > 0                  # some recorded contacts were retried ones
+    g.close()
+
+
+def test_convex_statistics_follow_the_oracle(oracle_mod):
+    """Contact statistics of convex bodies (AppCollision::addToContactsFeatures
+    over every GJK closest point): the cube pair driven through retried
+    contacts and a small dense box of cubes and tetrahedra; counts and the
+    largest overlap with its time exactly, means to rounding."""
+    print("[case] convex statistics", flush=True)
+    import test_gpu_convex_parity as cvx
+    for wl in (_cube_case("cubes"), cvx.case("convex_walls")):
+        dt = wl["dt"]
+        orc = oracle_mod.ConvexOracle(wl)
+        orc.prime()
+        g = gdem.build_system(wl)
+        g.set_contact_stats(True)
+        g.prime()
+        orc.step(300, dt)
+        g.step(300, dt)
+        sg, so = g.contact_stats(), orc.contact_stats()
+        print(f"[case] gpu {sg} oracle {so}", flush=True)
+        assert sg["overlaps"] == so["overlaps"] > 0
+        assert sg["gjk_reports"] == so["gjk_reports"] > 0
+        assert sg["overlap_max"] == so["overlap_max"]
+        assert sg["time_overlap_max"] == pytest.approx(so["time_overlap_max"], rel=1e-14)
+        assert sg["overlap_mean"] == pytest.approx(so["overlap_mean"], rel=1e-12)
+        assert sg["gjk_mean"] == pytest.approx(so["gjk_mean"], rel=1e-14)
+        g.close()
