@@ -469,6 +469,38 @@ int gdem_get_grid(gdem_ctx* ctx, int32_t* nc, double* edge);
 const char* gdem_version(void);
 
 /* ------------------------------------------------------------------------
+ * One force evaluation of a sphere system that allocates no device memory.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It evaluates what gdem_prime / gdem_compute_forces evaluate
+ * on a context holding the same particles (material 0, one contact model,
+ * no walls): binning, detection, Hooke/Hertz contact forces and torques, and
+ * the weight.  All of its buffers live in one static device array of the
+ * library (about 34 MB per device, held until the process ends).
+ *   cfg       domain, grid request, gravity, device (as gdem_create)
+ *   kind, params  GDEM_MODEL_HOOKE or GDEM_MODEL_HERTZ, as gdem_set_contact_model
+ *   pos_r, vel_m, om_req   device arrays of n x 4 doubles: (x, y, z, radius),
+ *             (v, mass), (angular velocity, equivalent-sphere radius); mass
+ *             and equivalent radius as gdem_set_particles derives them from
+ *             the density: m = rho 4 pi r^3 / 3, req = ((3 / 4 pi) vol)^(1/3)
+ *   crust     crust thickness of every particle
+ *   F, T      device arrays of n x 3 doubles (particle order), written
+ *   stream    hipStream_t (NULL: the null stream); the call waits for it
+ *   err_ids   optional host int32[2]: the ids an error names
+ * Returns GDEM_ERR_CAPACITY when n, the cell count or the row count (cells
+ * along y times cells along z) exceeds the maxima below, or when a particle
+ * detects or receives more contacts than a list holds; CONTACT and
+ * SIMULATION as gdem_last_error describes them.  Calls on one device share
+ * the static array: calls that overlap in time on one device race. */
+#define GDEM_NOALLOC_MAX_PARTICLES 32768
+#define GDEM_NOALLOC_MAX_CELLS (1 << 19)
+#define GDEM_NOALLOC_MAX_ROWS 16384
+int gdem_sphere_forces_noalloc(const gdem_config* cfg, int32_t kind, const double* params,
+                               int64_t n, const double* pos_r, const double* vel_m,
+                               const double* om_req, double crust, double* F, double* T,
+                               void* stream, int32_t* err_ids);
+
+/* ------------------------------------------------------------------------
  * Batched systems: many independent small sphere systems (calibration
  * sweeps) advanced by one persistent launch per call.  There is no batch in
  * the reference, which runs one Grains application per system

@@ -143,6 +143,8 @@ def load_library(path: str = LIB_PATH):
         "gdem_get_contact_stats": (C.c_int, [vp, P(d)]),
         "gdem_get_grid": (C.c_int, [vp, P(i32), P(d)]),
         "gdem_version": (C.c_char_p, []),
+        "gdem_sphere_forces_noalloc": (C.c_int, [P(_Config), i32, vp, i64, vp, vp, vp, d, vp, vp,
+                                                 vp, P(i32)]),
         "gdem_add_box_shape": (C.c_int, [vp, vp, P(i32)]),
         "gdem_add_polyhedron_shape": (C.c_int, [vp, i32, vp, i32, vp, vp, P(i32)]),
         "gdem_set_convex_particles": (C.c_int, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
@@ -200,7 +202,7 @@ EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall"
             "gdem_get_contacts", "gdem_last_error", "gdem_set_kernel_timing",
             "gdem_get_counters", "gdem_reset_counters", "gdem_get_kernel_times",
             "gdem_set_contact_stats", "gdem_get_contact_stats", "gdem_get_grid",
-            "gdem_version",
+            "gdem_version", "gdem_sphere_forces_noalloc",
             "gdem_add_box_shape", "gdem_add_polyhedron_shape", "gdem_set_convex_particles",
             "gdem_get_basis", "gdem_batch_create", "gdem_batch_destroy", "gdem_batch_add_system",
             "gdem_batch_commit", "gdem_batch_set_launch", "gdem_batch_prime", "gdem_batch_step",
