@@ -189,15 +189,19 @@ __device__ __forceinline__ void count_contacts(unsigned long long* counter, int 
 // the candidates that may touch are queued as t * 32 + e.  The queue has
 // kQueueSegs segments (workgroup b appends to segment b mod kQueueSegs, one
 // integer atomic per workgroup), so no counter sees more than a few hundred
-// atomics per step.  Clears the out masks for the contact kernel.
+// atomics per step.  Clears the out masks for the contact kernel.  Two
+// instantiations run every step and one of them exits at once: the sweep of
+// a rebuild needs many more registers than the screening alone, which then
+// runs at a higher occupancy.
+template <bool REBUILD>
 __global__ void __launch_bounds__(kPB) k_screen(PairArgs a) {
   __shared__ int sh_wave[kPB / 64];
   __shared__ int sh_base;
+  if (a.d.gate.rebuild() != REBUILD) return;
   const int t = logical_index(a.d.nblk8);
-  const bool rebuild = a.d.gate.rebuild();
   uint32_t may = 0;
   if (t < a.d.n) {
-    may = screen_particle(a, t, rebuild);
+    may = screen_particle(a, t, REBUILD);
     a.omask[t] = 0u;
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -525,10 +529,12 @@ void launch_pair(const PairArgs& a, hipStream_t st, hipEvent_t start, hipEvent_t
   // eighth of the particles' covers it in one pass (grid-stride beyond)
   const dim3 cgrid(std::max(8, (a.d.nblk8 / 8 + 7) / 8 * 8));
   if (start) {
-    hipExtLaunchKernelGGL(k_screen, grid, block, 0, st, start, nullptr, 0, a);
+    hipExtLaunchKernelGGL(k_screen<true>, grid, block, 0, st, start, nullptr, 0, a);
+    hipLaunchKernelGGL(k_screen<false>, grid, block, 0, st, a);
     hipExtLaunchKernelGGL(k_contact, cgrid, block, 0, st, nullptr, stop, 0, a);
   } else {
-    hipLaunchKernelGGL(k_screen, grid, block, 0, st, a);
+    hipLaunchKernelGGL(k_screen<true>, grid, block, 0, st, a);
+    hipLaunchKernelGGL(k_screen<false>, grid, block, 0, st, a);
     hipLaunchKernelGGL(k_contact, cgrid, block, 0, st, a);
   }
 }
