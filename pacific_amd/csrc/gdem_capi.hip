@@ -115,12 +115,13 @@ struct gdem_ctx {
   bool has_memory = false;
   History hist[2]{};
   double dt_mem = 0.;
-  // captured fused step (one graph per state-buffer parity)
+  // captured fused steps: [0] one step, [1] kGraphSteps steps in a row; one
+  // graph per state-buffer parity
   bool use_graph = true;
   int64_t graph_gen = 0;
-  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
-  double graph_dt[2] = {0., 0.};
-  int64_t graph_made[2] = {-1, -1};
+  hipGraphExec_t graph_exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  double graph_dt[2][2] = {{0., 0.}, {0., 0.}};
+  int64_t graph_made[2][2] = {{-1, -1}, {-1, -1}};
   // simulation time and imposed wall motions; while motions are pending,
   // gdem_step computes the walls of each of its steps on the host
   // (h_traj) and each force evaluation reads its own copy (step_walls)
@@ -383,8 +384,9 @@ extern "C" int gdem_destroy(gdem_ctx* c) {
   free_wall_loads(c);
   free_pack(c);
   for (auto& e : c->ev_pool) { (void)hipEventDestroy(e.start); (void)hipEventDestroy(e.stop); }
-  for (auto& g : c->graph_exec)
-    if (g) (void)hipGraphExecDestroy(g);
+  for (auto& row : c->graph_exec)
+    for (auto& g : row)
+      if (g) (void)hipGraphExecDestroy(g);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return GDEM_OK;
@@ -1197,21 +1199,32 @@ static int do_force(gdem_ctx* c, int mode, double dtv, double dtd, bool write_ft
   return GDEM_OK;
 }
 
-// One fused step (binning, detection, force kernel with MODE_MOVE) replayed
-// from a graph captured on first use for the current buffer parity and dt.
-// Only used without contact recording and kernel timing; the captured
-// kernels clear their own counters (no memset nodes).
-static int graph_force(gdem_ctx* c, double dt) {
+// Steps of several fused steps per graph: fewer graph launches, and the
+// work of consecutive steps is not separated by the end of a graph.
+constexpr int kGraphSteps = 16;
+
+// `steps` fused steps (binning, detection, force kernel with MODE_MOVE; 1 or
+// kGraphSteps) replayed from a graph captured on first use for the current
+// buffer parity and dt.  Only used without contact recording and kernel
+// timing; the captured kernels clear their own counters (no memset nodes).
+static int graph_force(gdem_ctx* c, double dt, int steps) {
   const int p = c->cur;
-  if (!c->graph_exec[p] || c->graph_dt[p] != dt || c->graph_made[p] != c->graph_gen) {
-    if (c->graph_exec[p]) {
-      HIP_TRY(hipGraphExecDestroy(c->graph_exec[p]));
-      c->graph_exec[p] = nullptr;
+  const int k = steps > 1 ? 1 : 0;
+  hipGraphExec_t& ge = c->graph_exec[k][p];
+  if (!ge || c->graph_dt[k][p] != dt || c->graph_made[k][p] != c->graph_gen) {
+    if (ge) {
+      HIP_TRY(hipGraphExecDestroy(ge));
+      ge = nullptr;
     }
     const bool binned = c->binned;
     c->binned = false;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = force_launches(c, MODE_MOVE, 0.5 * dt, dt, false);
+    int rc = GDEM_OK;
+    for (int s = 0; s < steps && rc == GDEM_OK; ++s) {
+      rc = force_launches(c, MODE_MOVE, 0.5 * dt, dt, false);
+      c->cur = 1 - c->cur;  // the next captured step reads the copy this one writes
+    }
+    c->cur = p;
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &graph);
     c->binned = binned;
@@ -1220,14 +1233,14 @@ static int graph_force(gdem_ctx* c, double dt) {
       return rc;
     }
     HIP_TRY(e);
-    const hipError_t ei = hipGraphInstantiate(&c->graph_exec[p], graph, nullptr, nullptr, 0);
+    const hipError_t ei = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     HIP_TRY(ei);
-    c->graph_dt[p] = dt;
-    c->graph_made[p] = c->graph_gen;
+    c->graph_dt[k][p] = dt;
+    c->graph_made[k][p] = c->graph_gen;
   }
-  HIP_TRY(hipGraphLaunch(c->graph_exec[p], c->stream));
-  after_force(c);
+  HIP_TRY(hipGraphLaunch(ge, c->stream));
+  for (int s = 0; s < steps; ++s) after_force(c);
   return GDEM_OK;
 }
 
@@ -1411,12 +1424,16 @@ extern "C" int gdem_step(gdem_ctx* c, int32_t nsteps, double dt) {
   }
   if (c->kin.any_live()) return moving_wall_steps(c, nsteps, dt);
   const bool graph = c->use_graph && !c->timing && c->rec_cap == 0;
-  for (int32_t k = 0; k < nsteps; ++k) {
-    c->time += dt;
+  for (int32_t k = 0; k < nsteps;) {
+    // the last step closes with the half kick (MODE_KICK), outside the graphs
+    const int32_t open = nsteps - 1 - k;
+    const int run = graph && open >= kGraphSteps ? kGraphSteps : 1;
+    for (int s = 0; s < run; ++s) c->time += dt;
     const bool last = (k == nsteps - 1);
-    if (!last && graph) rc = graph_force(c, dt);
+    if (!last && graph) rc = graph_force(c, dt, run);
     else rc = do_force(c, last ? MODE_KICK : MODE_MOVE, 0.5 * dt, dt, last);
     if (rc) return rc;
+    k += run;
   }
   return GDEM_OK;
 }
