@@ -74,7 +74,7 @@ def cached_convex(n, seed):
     from pacific_amd import workloads
     cache_dir = os.path.join(os.environ.get("TMPDIR", "/tmp"), "gdem_cache")
     os.makedirs(cache_dir, exist_ok=True)
-    path = os.path.join(cache_dir, f"convex_box_{n}_s{seed}.npz")
+    path = os.path.join(cache_dir, f"convex_box_{n}_s{seed}_phi025.npz")
     wl = workloads.convex_box(64, seed)  # structure; arrays replaced below
     if os.path.exists(path):
         z = np.load(path)
@@ -458,7 +458,7 @@ def main():
             "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": wl["name"], "n_particles": n, "shapes": "cube 2 mm / tetrahedron 2.5 mm",
-                       "solid_fraction": 0.20, "contact": "Hooke+Coulomb, GJK", "dt": dt,
+                       "solid_fraction": 0.25, "contact": "Hooke+Coulomb, GJK", "dt": dt,
                        "grid": nc, "parallelism": "single GPU"},
             "detect_force_us": k_ms * 1e3, "mean_contacts": k_contacts,
             "cpu_baseline": cpu,

@@ -352,16 +352,17 @@ def tetrahedron_vertices(edge=TET_EDGE):
                      [0.0, 0.0, rcirc]])
 
 
-def convex_box(n=200_000, seed=3, steps=300, phi=0.20, height=0.20, overlap=0.0, cell_factor=1.0,
-               sigma_v=0.0, attempts_per_particle=2000):
+def convex_box(n=200_000, seed=3, steps=300, phi=0.25, height=0.20, overlap=0.0, cell_factor=1.0,
+               sigma_v=0.0, attempts_per_particle=40000):
     """n convex particles, alternating cubes (edge 2 mm) and regular tetrahedra
     (edge 2.5 mm), uniform random orientations (Shoemake), random sequential
     addition (exact separating-axis overlap test) at solid fraction phi in a
-    square box of the given height with six walls; gravity -9.81 z; Hooke kn
-    2e5, en 0.8, mu 0.5; dt 2e-6.  Random sequential addition of this mix
-    saturates near 0.23 (walls included), so the default fraction is 0.20 with
-    the cross-section sized for n particles.  overlap > 0 lets neighbours
-    overlap by up to that much at the start (contacts from step 0)."""
+    square box of the given height with six walls, the cross-section sized
+    for n particles; gravity -9.81 z; Hooke kn 2e5, en 0.8, mu 0.5; dt 2e-6.
+    Near 0.25 the addition approaches its saturation slowly: 200,000 particles
+    take about 2e9 attempts (minutes on a few dozen host threads).  overlap > 0
+    lets neighbours overlap by up to that much at the start (contacts from
+    step 0)."""
     import math
     tv = tetrahedron_vertices()
     v_cube = 8.0 * CUBE_HALF[0] * CUBE_HALF[1] * CUBE_HALF[2]
