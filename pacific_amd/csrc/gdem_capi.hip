@@ -370,6 +370,7 @@ extern "C" int gdem_destroy(gdem_ctx* c) {
   free_state(c->buf[1]);
   (void)hipFree(c->d_F); (void)hipFree(c->d_T);
   (void)hipFree(c->bins.key); (void)hipFree(c->bins.slot); (void)hipFree(c->bins.perm); (void)hipFree(c->bins.pos);
+  (void)hipFree(c->bins.ids);
   (void)hipFree(c->d_out_list); (void)hipFree(c->d_in_list); (void)hipFree(c->d_out_count); (void)hipFree(c->d_in_count);
   (void)hipFree(c->bins.cnt); (void)hipFree(c->bins.local); (void)hipFree(c->bins.bsum); (void)hipFree(c->bins.cell_start);
   (void)hipFree(c->d_row_rank); (void)hipFree(c->d_row_jk); (void)hipFree(c->d_models); (void)hipFree(c->d_walls); (void)hipFree(c->d_err);
@@ -428,6 +429,7 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
   free_cstate(c->cbuf[1]);
   (void)hipFree(c->d_F); (void)hipFree(c->d_T);
   (void)hipFree(c->bins.key); (void)hipFree(c->bins.slot); (void)hipFree(c->bins.perm); (void)hipFree(c->bins.pos);
+  (void)hipFree(c->bins.ids);
   (void)hipFree(c->d_out_list); (void)hipFree(c->d_in_list); (void)hipFree(c->d_out_count); (void)hipFree(c->d_in_count);
   (void)hipFree(c->d_rec); (void)hipFree(c->d_rec_count);
   (void)hipFree(c->d_cand); (void)hipFree(c->d_crec); (void)hipFree(c->d_mask); (void)hipFree(c->d_work);
@@ -445,6 +447,7 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
   c->d_F = c->d_T = nullptr;
   c->bins.key = c->bins.slot = c->bins.perm = nullptr;
   c->bins.pos = nullptr;
+  c->bins.ids = nullptr;
   c->d_out_list = c->d_in_list = nullptr;
   c->d_out_count = c->d_in_count = nullptr;
   c->d_rec = nullptr;
@@ -490,6 +493,7 @@ static int alloc_particles(gdem_ctx* c, int64_t n, bool convex) {
   HIP_TRY(hipMalloc(&c->bins.slot, sizeof(int32_t) * nn));
   HIP_TRY(hipMalloc(&c->bins.perm, sizeof(int32_t) * nn));
   HIP_TRY(hipMalloc(&c->bins.pos, sizeof(float4) * nn));
+  HIP_TRY(hipMalloc(&c->bins.ids, sizeof(int32_t) * nn));
   HIP_TRY(hipMalloc(&c->d_in_list, sizeof(int2) * nn * in_cap));
   HIP_TRY(hipMalloc(&c->d_out_count, sizeof(int32_t) * nn));
   HIP_TRY(hipMalloc(&c->d_in_count, sizeof(int32_t) * nn));
@@ -996,6 +1000,7 @@ static int pack_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wri
   PairArgs pa{};
   DetectArgs& d = pa.d;
   d.pos = c->bins.pos;
+  d.ids = c->bins.ids;
   d.margin = prefilter_margin(c->grid, d.sizef);
   d.perm = c->bins.perm;
   d.cell_start = c->bins.cell_start;
@@ -1095,6 +1100,7 @@ static int force_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wr
   } else {
     DetectArgs d{};
     d.pos = c->bins.pos;
+    d.ids = c->bins.ids;
     d.margin = prefilter_margin(c->grid, d.sizef);
     d.in = c->buf[c->cur];
     d.perm = c->bins.perm;

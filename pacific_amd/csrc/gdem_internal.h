@@ -116,6 +116,7 @@ struct PosSrc {
 // the partner's in-list, an in entry its index in the partner's out-list).
 struct DetectArgs {
   const float4* pos;      // single-precision positions + radius, sorted order
+  const int32_t* ids;     // particle ids, sorted order (skin lists)
   float sizef[3];         // box size in single precision (periodic shifts)
   float margin;           // pre-filter margin, bounds the float rounding
   State in;               // state in storage order (aux via perm)
@@ -133,8 +134,9 @@ struct DetectArgs {
   int32_t* in_count;      // [n], zero on entry
   unsigned long long* err;
   int32_t nblk8;  // blocks padded to a multiple of 8 (XCD-aware mapping)
-  // skin lists: skin > 0 lists candidates (gap < skin) instead of contacts;
-  // the sweep runs only when the gate says a rebuild is due
+  // skin lists: skin > 0 lists candidates (every pair with gap < skin, and
+  // the few within the pre-filter margin beyond) instead of contacts; the
+  // sweep runs only when the gate says a rebuild is due
   double skin;
   SkinGate gate;
 };
@@ -349,6 +351,7 @@ struct Bins {
   int32_t* perm;        // sorted slot -> storage index
   int32_t* cell_start;  // ncell + 1
   float4* pos;          // sorted single-precision copy of pos_r
+  int32_t* ids;         // sorted particle ids (skin lists; may be null)
   int32_t* ticket;      // bsum[nbsum]: blocks of the scan pass finished
   int32_t nbsum;
 };

@@ -199,7 +199,8 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const int32_t* key, const in
 // One cell of k_cellsort.
 __device__ __forceinline__ void sort_cell(const int32_t* local, const int32_t* bsum, int32_t* perm,
                                           int32_t* cell_start, const PosSrc& src,
-                                          float4* pos_sorted, int c, int ncell, int n) {
+                                          float4* pos_sorted, int32_t* ids_sorted, int c,
+                                          int ncell, int n) {
   int beg = local[c] + bsum[c / kScanCells];
   int end = (c + 1 < ncell) ? local[c + 1] + bsum[(c + 1) / kScanCells] : n;
   for (int a = beg + 1; a < end; ++a) {
@@ -214,23 +215,27 @@ __device__ __forceinline__ void sort_cell(const int32_t* local, const int32_t* b
   cell_start[c] = beg;
   if (c == ncell - 1) cell_start[ncell] = n;
   for (int u = beg; u < end; ++u) {
-    const double4 p = src.pos(perm[u]);
+    const int s = perm[u];
+    const double4 p = src.pos(s);
     pos_sorted[u] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+    if (ids_sorted) ids_sorted[u] = src.aux[s].id;
   }
 }
 
 // Orders each cell by previous slot, which makes the permutation independent of
 // the order in which the count pass's atomics were served, then writes the
 // cell's entries of the sorted single-precision copy of position + radius
-// (read by the detection sweep's conservative pre-filter only: every contact
-// decision is made in FP64 on the full-precision state).
+// (read by the detection sweep's conservative pre-filter, which alone decides
+// skin-list candidacy: every contact decision is made in FP64 on the
+// full-precision state), and with skin lists the sorted ids.
 __global__ void __launch_bounds__(kBlock) k_cellsort(const int32_t* local, const int32_t* bsum,
                                                      int32_t* perm, int32_t* cell_start,
                                                      PosSrc src, float4* pos_sorted,
-                                                     int ncell, int n, SkinGate gate) {
+                                                     int32_t* ids_sorted, int ncell, int n,
+                                                     SkinGate gate) {
   if (!gate.rebuild()) return;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x)
-    sort_cell(local, bsum, perm, cell_start, src, pos_sorted, c, ncell, n);
+    sort_cell(local, bsum, perm, cell_start, src, pos_sorted, ids_sorted, c, ncell, n);
 }
 
 // ---------------------------------------------------------------- detect
@@ -512,7 +517,8 @@ void launch_binning(Bins& b, const PosSrc& src, int32_t n, const Grid& g, const 
     hipLaunchKernelGGL(k_scatter, dim3(capped(nblocks(n, kBlock))), dim3(kBlock), 0, st, b.key,
                        b.slot, b.local, b.bsum, b.perm, n, gate);
   hipLaunchKernelGGL(k_cellsort, dim3(capped(nblocks(ncell, kBlock))), dim3(kBlock), 0, st,
-                     b.local, b.bsum, b.perm, b.cell_start, src, b.pos, ncell, n, gate);
+                     b.local, b.bsum, b.perm, b.cell_start, src, b.pos, b.ids, ncell, n,
+                     gate);
 }
 
 void launch_wall_loads(const ForceArgs& a, double4* part, double4* out, hipStream_t st) {

@@ -27,6 +27,25 @@ struct Detector {
   __device__ __forceinline__ Detector(const DetectArgs& a_, const PosSrc& src_, int t_, float4 mef_)
       : a(a_), src(src_), t(t_), mef(mef_), have_me(false), X{}, r(0.), ax{}, nout(0) {}
 
+  // Appends partner u (entry code) to this particle's out-list and the
+  // mirror entry to u's in-list; id and id2 name the two in a capacity error.
+  __device__ __forceinline__ void append(int u, int code, int32_t id, int32_t id2) {
+    // each entry also records where its mirror sits in the partner's other
+    // list (bits 7-11; an index at or beyond the capacity is never used)
+    const int kin = atomicAdd(&a.in_count[u], 1);
+    if (nout < kListCap) {
+      a.out_list[(size_t)nout * a.lstride + t] = make_int2(u, code | (min(kin, 31) << 7));
+    } else {
+      atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)id);
+    }
+    if (kin < kListCap) {
+      a.in_list[(size_t)kin * a.lstride + u] = make_int2(t, mirror_code(code) | (min(nout, 31) << 7));
+    } else {
+      atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)id2);
+    }
+    ++nout;
+  }
+
   // One candidate u with periodic shift s (applied to u).  Cross-cell
   // candidates of the half stencil have this particle as the first body; a
   // same-cell pair is ordered by id.
@@ -38,6 +57,15 @@ struct Detector {
     const float dzf = (p2f.z + sz * a.sizef[2]) - mef.z;
     const float rsf = mef.w + p2f.w + (float)a.skin + a.margin;
     if (dxf * dxf + dyf * dyf + dzf * dzf > rsf * rsf) return;
+    if (a.skin > 0.) {
+      // candidate: the pre-filter alone decides (every pair with gap below
+      // the skin passes it, and the few within the margin beyond); the
+      // contact test of every step is made on the candidates with the same
+      // body order
+      const int32_t id = a.ids[t], id2 = a.ids[u];
+      append(u, pack_code(sx, sy, sz, same ? (id < id2) : true), id, id2);
+      return;
+    }
     if (!have_me) {
       const int s = a.perm[t];
       const double4 me = src.pos(s);
@@ -61,37 +89,15 @@ struct Detector {
       B = mk(X.x + (-shx), X.y + (-shy), X.z + (-shz));
       rA = p2.w; rB = r; crA = ax2.crust; crB = ax.crust;
     }
-    if (a.skin > 0.) {
-      // candidate: gap below the skin (the contact test of every step is
-      // made on the candidates by the force kernel, with the same body order)
-      const v3 AB = sub(B, A);
-      const double gap = norm(AB) - (rA + rB);
-      if (!(gap < a.skin)) return;
-    } else {
-      int rc = sphere_sphere(A, rA, crA, B, rB, crB, c, ov, dist);
-      int it;
-      if (rc < 0) rc = sphere_sphere_retry(A, rA, crA, B, rB, crB, c, ov, dist, it);
-      if (rc < 0) {
-        atomicMin(&a.err[kErrContact], first ? pack_ids(ax.id, ax2.id) : pack_ids(ax2.id, ax.id));
-        return;
-      }
-      if (rc == 0 || !(dist < 0.)) return;
+    int rc = sphere_sphere(A, rA, crA, B, rB, crB, c, ov, dist);
+    int it;
+    if (rc < 0) rc = sphere_sphere_retry(A, rA, crA, B, rB, crB, c, ov, dist, it);
+    if (rc < 0) {
+      atomicMin(&a.err[kErrContact], first ? pack_ids(ax.id, ax2.id) : pack_ids(ax2.id, ax.id));
+      return;
     }
-    // each entry also records where its mirror sits in the partner's other
-    // list (bits 7-11; an index at or beyond the capacity is never used)
-    const int code = pack_code(sx, sy, sz, first);
-    const int kin = atomicAdd(&a.in_count[u], 1);
-    if (nout < kListCap) {
-      a.out_list[(size_t)nout * a.lstride + t] = make_int2(u, code | (min(kin, 31) << 7));
-    } else {
-      atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax.id);
-    }
-    if (kin < kListCap) {
-      a.in_list[(size_t)kin * a.lstride + u] = make_int2(t, mirror_code(code) | (min(nout, 31) << 7));
-    } else {
-      atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)ax2.id);
-    }
-    ++nout;
+    if (rc == 0 || !(dist < 0.)) return;
+    append(u, pack_code(sx, sy, sz, first), ax.id, ax2.id);
   }
 
   __device__ __forceinline__ void range(int beg, int end, int sx, int sy, int sz, bool same) {
