@@ -61,9 +61,12 @@ struct Detector {
       // candidate: the pre-filter alone decides (every pair with gap below
       // the skin passes it, and the few within the margin beyond); the
       // contact test of every step is made on the candidates with the same
-      // body order
-      const int32_t id = a.ids[t], id2 = a.ids[u];
-      append(u, pack_code(sx, sy, sz, same ? (id < id2) : true), id, id2);
+      // body order.  The in-list positions are taken afterwards (link()).
+      const int32_t id = a.ids[t];
+      const int code = pack_code(sx, sy, sz, same ? (id < a.ids[u]) : true);
+      if (nout < kListCap) a.out_list[(size_t)nout * a.lstride + t] = make_int2(u, code);
+      else atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)id);
+      ++nout;
       return;
     }
     if (!have_me) {
@@ -100,6 +103,34 @@ struct Detector {
     append(u, pack_code(sx, sy, sz, first), ax.id, ax2.id);
   }
 
+  // Skin lists: the in-list position of every out entry the sweep stored
+  // (partner and code), from integer atomics issued four at a time, written
+  // into the entry (bits 7-11) and with the mirror entry into the partner's
+  // in-list.
+  __device__ __forceinline__ void link() {
+    const int n = min(nout, kListCap);
+    for (int e0 = 0; e0 < n; e0 += 4) {
+      int2 en[4];
+      int kin[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (e0 + k < n) en[k] = a.out_list[(size_t)(e0 + k) * a.lstride + t];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (e0 + k < n) kin[k] = atomicAdd(&a.in_count[en[k].x], 1);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (e0 + k >= n) continue;
+        const int e = e0 + k, u = en[k].x, code = en[k].y;
+        a.out_list[(size_t)e * a.lstride + t] = make_int2(u, code | (min(kin[k], 31) << 7));
+        if (kin[k] < kListCap)
+          a.in_list[(size_t)kin[k] * a.lstride + u] = make_int2(t, mirror_code(code) | (e << 7));
+        else
+          atomicMin(&a.err[kErrCapacity], (unsigned long long)(uint32_t)a.ids[u]);
+      }
+    }
+  }
+
   __device__ __forceinline__ void range(int beg, int end, int sx, int sy, int sz, bool same) {
     int u = beg;
     for (; u + 1 < end; u += 2) {
@@ -121,6 +152,7 @@ __device__ __forceinline__ int detect_particle(const DetectArgs& a, const PosSrc
   clamped_cell(a.grid, mk(me.x, me.y, me.z), ijk);
   Detector det(a, src, t, a.pos[t]);
   half_stencil_sweep(a.grid, a.row_rank, a.cell_start, t, ijk, det);
+  if (a.skin > 0.) det.link();
   a.out_count[t] = det.nout;
   return det.nout;
 }
