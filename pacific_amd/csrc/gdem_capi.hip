@@ -992,9 +992,6 @@ static int pack_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wri
   const SkinGate gate{c->d_skin, p, skin_limit(c)};
   const PackConst& kin = c->pkc[p];
   const PosSrc src{c->pk[p].A, kin.rr, kin.aux};
-  launch_binning(c->bins, src, (int32_t)c->n, c->grid, c->d_row_rank, c->d_err, c->stream, gate,
-                 c->d_skin, c->d_in_count);
-  HIP_TRY(hipGetLastError());
   const int nblk = (int)((c->n + 255) / 256);
   const int nblk8 = ((nblk + 7) / 8) * 8;
   PairArgs pa{};
@@ -1028,7 +1025,8 @@ static int pack_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wri
   pa.qcount = c->d_qcount;
   pa.qseg = (int32_t)std::min<int64_t>(queue_seg(c), 0x7fffffff);
   TimedLaunch* ev = c->timing ? next_event_pair(c, 1) : nullptr;
-  launch_pair(pa, c->stream, ev ? ev->start : nullptr, ev ? ev->stop : nullptr);
+  launch_pair(c->bins, src, c->d_skin, pa, c->stream, ev ? ev->start : nullptr,
+              ev ? ev->stop : nullptr);
   HIP_TRY(hipGetLastError());
   IntegArgs a{};
   a.in = c->pk[p];

@@ -367,6 +367,11 @@ void launch_keys(const State& s, int32_t n, const Grid& g, const int32_t* row_ra
 void launch_binning(Bins& b, const PosSrc& src, int32_t n, const Grid& g, const int32_t* row_rank,
                     unsigned long long* err, hipStream_t st, SkinGate gate = SkinGate{},
                     SkinWords* words = nullptr, int32_t* in_count = nullptr);
+// The scan and the per-cell sort of a gated binning alone (the count and
+// scatter passes then run elsewhere: gdem_binning.h).
+void launch_scan(const Bins& b, int32_t ncell, SkinGate gate, hipStream_t st);
+void launch_cellsort(const Bins& b, const PosSrc& src, int32_t ncell, int32_t n, SkinGate gate,
+                     hipStream_t st);
 void launch_detect(const DetectArgs& a, hipStream_t st);
 // With events, they record the kernel's own start and end (timing).  memory:
 // the history-model variant (a.hin / a.hout used).
@@ -480,8 +485,10 @@ void launch_to_pack(const State& s, const Pack& pk, const PackConst& k0, const P
                     int32_t n, hipStream_t st);
 void launch_from_pack(const Pack& pk, const PackConst& k, const State& s, int32_t n,
                       hipStream_t st);
-void launch_pair(const PairArgs& a, hipStream_t st, hipEvent_t start = nullptr,
-                 hipEvent_t stop = nullptr);
+// The pair stage of a packed step, the binning of a rebuild included (its
+// count and scatter passes run inside the pair kernels).
+void launch_pair(const Bins& b, const PosSrc& src, SkinWords* words, const PairArgs& a,
+                 hipStream_t st, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 void launch_integ(int mode, const IntegArgs& a, hipStream_t st, hipEvent_t start = nullptr,
                   hipEvent_t stop = nullptr);
 // Kick + drift of the packed copy in place (the opening half of gdem_step).

@@ -17,6 +17,7 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include "gdem_binning.h"
 #include "gdem_internal.h"
 #include "gdem_sphere_step.h"
 #include "gdem_sweep.h"
@@ -49,61 +50,18 @@ __global__ void __launch_bounds__(kBlock) k_keys(State s, int n, Grid g, const i
 }
 
 // ---------------------------------------------------------------- binning
-// Cell key of position p (as k_keys); -1 outside the grid.
-__device__ __forceinline__ int key_of(const Grid& g, const int32_t* row_rank, double4 p) {
-  int ijk[3];
-  if (!cell_of(g, mk(p.x, p.y, p.z), ijk)) return -1;
-  return row_base(g, row_rank, ijk[1], ijk[2]) + ijk[0];
-}
-
-// Arrival slot of each particle in its cell.  Keys arrive almost sorted (the
-// storage order is the previous step's cell order), so equal keys of
-// neighbouring lanes are merged into one atomic per run.  With skin lists
-// (words non-null) the keys are computed here from src, and block 0 does the
-// evaluation's first-kernel duties on the skin words.
+// Arrival slots of the count pass (count_pass, gdem_binning.h); with skin
+// lists (words non-null) block 0 also does the evaluation's first-kernel
+// duties on the skin words.
 __global__ void __launch_bounds__(kBlock) k_count(int32_t* key, int32_t* slot, int32_t* cnt,
                                                   int n, SkinGate gate, SkinWords* words,
                                                   int32_t* in_count, PosSrc src, Grid g,
                                                   const int32_t* row_rank,
                                                   unsigned long long* err) {
   const bool rebuild = gate.rebuild();
-  if (words && blockIdx.x == 0 && threadIdx.x == 0) {
-    words->need[1 - gate.p] = 0;
-    if (rebuild) words->rebuilds += 1;
-  }
+  if (words && blockIdx.x == 0 && threadIdx.x == 0) skin_words_duty(gate, words, rebuild);
   if (!rebuild) return;
-  const int lane = threadIdx.x & 63;
-  // grid-stride over blocks of particles: the loop bound is uniform in the
-  // block, so every lane of a wave takes part in the shuffles and ballots
-  for (int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
-    const int i = b0 + threadIdx.x;
-    int k = -1;
-    if (i < n) {
-      if (words) {
-        k = key_of(g, row_rank, src.pos(i));
-        if (k < 0) {
-          atomicMin(&err[kErrSim], (unsigned long long)(uint32_t)src.aux[i].id);
-          k = 0;
-        }
-        key[i] = k;
-      } else {
-        k = key[i];
-      }
-    }
-    const int kprev = __shfl_up(k, 1, 64);
-    const bool head = (lane == 0) || (k != kprev);
-    const unsigned long long heads = __ballot(head);
-    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-    const int head_lane = 63 - __clzll(heads & upto);
-    const unsigned long long after = heads & ~upto;
-    const int next_head = after ? __ffsll((long long)after) - 1 : 64;
-    int base = 0;
-    if (head && k >= 0) base = atomicAdd(&cnt[k], next_head - lane);
-    base = __shfl(base, head_lane, 64);
-    if (i < n) slot[i] = base + (lane - head_lane);
-    // skin lists: the rebuild's sweep appends to cleared incoming counts
-    if (in_count && i < n) in_count[i] = 0;
-  }
+  count_pass(key, slot, cnt, n, words != nullptr, in_count, src, g, row_rank, err);
 }
 
 template <int BS>
@@ -190,10 +148,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const int32_t* key, const in
                                                     const int32_t* local, const int32_t* bsum,
                                                     int32_t* perm, int n, SkinGate gate) {
   if (!gate.rebuild()) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int k = key[i];
-    perm[local[k] + bsum[k / kScanCells] + slot[i]] = i;
-  }
+  scatter_pass(key, slot, local, bsum, perm, n);
 }
 
 // One cell of k_cellsort.
@@ -518,6 +473,21 @@ void launch_binning(Bins& b, const PosSrc& src, int32_t n, const Grid& g, const 
                        b.slot, b.local, b.bsum, b.perm, n, gate);
   hipLaunchKernelGGL(k_cellsort, dim3(capped(nblocks(ncell, kBlock))), dim3(kBlock), 0, st,
                      b.local, b.bsum, b.perm, b.cell_start, src, b.pos, b.ids, ncell, n,
+                     gate);
+}
+
+// The scan and the per-cell sort of a gated binning, for launch sequences
+// that run the count and scatter passes inside kernels of their own.
+void launch_scan(const Bins& b, int32_t ncell, SkinGate gate, hipStream_t st) {
+  const int nb = nblocks(ncell, kScanCells);
+  hipLaunchKernelGGL(k_scan, dim3(nb), dim3(kScanBlock), 0, st, b.cnt, b.local, b.bsum, b.ticket,
+                     ncell, nb, gate);
+}
+
+void launch_cellsort(const Bins& b, const PosSrc& src, int32_t ncell, int32_t n, SkinGate gate,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_cellsort, dim3(std::min(nblocks(ncell, kBlock), kGateGrid)), dim3(kBlock),
+                     0, st, b.local, b.bsum, b.perm, b.cell_start, src, b.pos, b.ids, ncell, n,
                      gate);
 }
 

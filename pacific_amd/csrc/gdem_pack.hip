@@ -1,24 +1,32 @@
 // gdem_pack.hip -- the sphere step in the packed layout (skin candidate
-// lists, memoryless contact models), as three kernels per step:
+// lists, memoryless contact models).  A step that keeps its lists runs:
 //
-//   k_screen  : one thread per particle walks the candidates it detected
-//               (each pair sits in exactly one out-list) and screens them
-//               against the positions of the step; the candidates that may
-//               touch go to a queue.  On a rebuild step it first sweeps the
-//               half stencil (LinkedCell.cpp:1019-1103) to rebuild the lists.
-//   k_contact : one lane per queued candidate evaluates the pair once with
-//               the reference's body order (RigidBodyWithCrust.cpp:596-640,
-//               HookeContactForceModel.cpp:44-114 / HertzContactForceModel.cpp:
-//               46-114) and stores its record (force on the first body and
-//               both torques) at the out-list entry; it flags the entry in the
-//               particle's out mask and the mirror entry in the partner's in
-//               mask (integer OR).
-//   k_integ   : one thread per particle sums its flagged records in a fixed
-//               order -- detected contacts by entry, then received contacts
-//               by partner slot -- (Torsor.cpp:184-189, no floating-point
-//               atomics), adds the walls, and applies the leap-frog tail
-//               (SecondOrderLeapFrog.cpp:44-94 via ParticleKinematics.cpp:
-//               62-128), writing the other state copy.
+//   k_bin_screen      : one thread per particle walks the candidates it
+//                       detected (each pair sits in exactly one out-list) and
+//                       screens them against the positions of the step; the
+//                       candidates that may touch go to a queue.
+//   k_scatter_contact : one lane per queued candidate evaluates the pair once
+//                       with the reference's body order (RigidBodyWithCrust.cpp:
+//                       596-640, HookeContactForceModel.cpp:44-114 /
+//                       HertzContactForceModel.cpp:46-114) and stores its
+//                       record (force on the first body and both torques) at
+//                       the out-list entry; it flags the entry in the
+//                       particle's out mask and the mirror entry in the
+//                       partner's in mask (integer OR).
+//   k_integ           : one thread per particle sums its flagged records in a
+//                       fixed order -- detected contacts by entry, then
+//                       received contacts by partner slot -- (Torsor.cpp:
+//                       184-189, no floating-point atomics), adds the walls,
+//                       and applies the leap-frog tail (SecondOrderLeapFrog.cpp:
+//                       44-94 via ParticleKinematics.cpp:62-128), writing the
+//                       other state copy.
+//
+// A step that rebuilds its lists runs the counting sort by cell in the same
+// first kernels (count pass in k_bin_screen, scan, scatter pass in
+// k_scatter_contact, per-cell sort), then k_sweep_contact sweeps the half
+// stencil (LinkedCell.cpp:1019-1103), screens the new lists and evaluates the
+// pairs, and k_integ.  The gate word of the step picks the branch on the
+// device, so one captured graph serves both kinds of step.
 //
 // The per-pair arithmetic and the order of every sum are those of the fused
 // kernel (gdem_sphere_step.h), so both give bit-identical results.  Contact
@@ -29,6 +37,7 @@
 
 #include <algorithm>
 
+#include "gdem_binning.h"
 #include "gdem_internal.h"
 #include "gdem_sphere_step.h"
 #include "gdem_sweep.h"
@@ -185,25 +194,13 @@ __device__ __forceinline__ void count_contacts(unsigned long long* counter, int 
   }
 }
 
-// Candidate sweep (rebuild steps) and screening: one thread per particle;
-// the candidates that may touch are queued as t * 32 + e.  The queue has
-// kQueueSegs segments (workgroup b appends to segment b mod kQueueSegs, one
-// integer atomic per workgroup), so no counter sees more than a few hundred
-// atomics per step.  Clears the out masks for the contact kernel.  Two
-// instantiations run every step and one of them exits at once: the sweep of
-// a rebuild needs many more registers than the screening alone, which then
-// runs at a higher occupancy.
-template <bool REBUILD>
-__global__ void __launch_bounds__(kPB) k_screen(PairArgs a) {
-  __shared__ int sh_wave[kPB / 64];
-  __shared__ int sh_base;
-  if (a.d.gate.rebuild() != REBUILD) return;
-  const int t = logical_index(a.d.nblk8);
-  uint32_t may = 0;
-  if (t < a.d.n) {
-    may = screen_particle(a, t, REBUILD);
-    a.omask[t] = 0u;
-  }
+// Queues the candidates of particle t that may touch (bit mask may) as
+// t * 32 + e.  The queue has kQueueSegs segments (workgroup b appends to
+// segment b mod kQueueSegs, one integer atomic per workgroup), so no counter
+// sees more than a few hundred atomics per step.  Every thread of the
+// workgroup calls.
+__device__ __forceinline__ void queue_candidates(const PairArgs& a, int t, uint32_t may,
+                                                 int* sh_wave, int* sh_base) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int cnt = __builtin_popcount(may);
   int incl = cnt;
@@ -217,10 +214,10 @@ __global__ void __launch_bounds__(kPB) k_screen(PairArgs a) {
   if (threadIdx.x == 0) {
     int tot = 0;
     for (int w = 0; w < kPB / 64; ++w) tot += sh_wave[w];
-    sh_base = tot ? atomicAdd(&a.qcount[blockIdx.x % kQueueSegs], tot) : 0;
+    *sh_base = tot ? atomicAdd(&a.qcount[blockIdx.x % kQueueSegs], tot) : 0;
   }
   __syncthreads();
-  int pos = sh_base + incl - cnt;
+  int pos = *sh_base + incl - cnt;
   for (int w = 0; w < wid; ++w) pos += sh_wave[w];
   int32_t* seg = a.queue + (size_t)(blockIdx.x % kQueueSegs) * a.qseg;
   for (uint32_t mm = may; mm != 0u; mm &= mm - 1u) {
@@ -231,9 +228,9 @@ __global__ void __launch_bounds__(kPB) k_screen(PairArgs a) {
 }
 
 // The queued candidates, one per lane (grid-stride over the segments'
-// lengths the screening left); every contact record lands at its fixed
-// entry, so the order in which the queue was filled does not matter.
-__global__ void __launch_bounds__(kPB) k_contact(PairArgs a) {
+// lengths); every contact record lands at its fixed entry, so the order in
+// which the queue was filled does not matter.
+__device__ __forceinline__ void evaluate_queue(const PairArgs& a, bool rebuild) {
   __shared__ unsigned long long sh_cnt[kPB / 64];
   __shared__ int sh_pre[kQueueSegs + 1];
   static_assert(kQueueSegs == 64, "one lane of the first wave per queue segment");
@@ -251,7 +248,6 @@ __global__ void __launch_bounds__(kPB) k_contact(PairArgs a) {
   }
   __syncthreads();
   const int nq = sh_pre[kQueueSegs];
-  const bool rebuild = a.d.gate.rebuild();
   int nc = 0;
   for (int q = blockIdx.x * kPB + threadIdx.x; q < nq; q += gridDim.x * kPB) {
     int lo = 0, hi = kQueueSegs;  // the segment k with pre[k] <= q < pre[k + 1]
@@ -262,6 +258,87 @@ __global__ void __launch_bounds__(kPB) k_contact(PairArgs a) {
     }
     const int w = a.queue[(size_t)lo * a.qseg + (q - sh_pre[lo])];
     nc += contact_pair(a, w >> 5, w & 31, rebuild);
+  }
+  count_contacts(a.contacts, nc, sh_cnt);
+}
+
+// The kernels of the pair stage.  Each step launches the same five kernels;
+// a step that rebuilds the lists (gate word set) runs the counting sort by
+// cell (count, scan, scatter, per-cell sort: gdem_binning.h) and then
+// k_sweep_contact, one that does not screens its lists in k_bin_screen and
+// evaluates the queue in k_scatter_contact, and the other kernels exit at
+// once.  Sharing kernels between the two kinds of step keeps the launches
+// that do nothing to two per step.
+
+// Count pass of a rebuild, or the screening of the lists: the candidates
+// that may touch go to the queue, and the out masks are cleared for the
+// contact evaluation.  Block 0 does the skin-word duties of the evaluation.
+__global__ void __launch_bounds__(kPB) k_bin_screen(Bins b, PosSrc src, SkinWords* words,
+                                                    PairArgs a) {
+  __shared__ int sh_wave[kPB / 64];
+  __shared__ int sh_base;
+  const bool rebuild = a.d.gate.rebuild();
+  if (blockIdx.x == 0 && threadIdx.x == 0) skin_words_duty(a.d.gate, words, rebuild);
+  if (rebuild) {
+    count_pass(b.key, b.slot, b.cnt, a.d.n, true, a.d.in_count, src, a.d.grid, a.d.row_rank,
+               a.d.err);
+    return;
+  }
+  const int t = logical_index(a.d.nblk8);
+  uint32_t may = 0;
+  if (t < a.d.n) {
+    may = screen_particle(a, t, false);
+    a.omask[t] = 0u;
+  }
+  queue_candidates(a, t, may, sh_wave, &sh_base);
+}
+
+// Scatter pass of a rebuild, or the evaluation of the queued candidates.
+__global__ void __launch_bounds__(kPB) k_scatter_contact(Bins b, PairArgs a) {
+  if (a.d.gate.rebuild()) {
+    scatter_pass(b.key, b.slot, b.local, b.bsum, b.perm, a.d.n);
+    return;
+  }
+  evaluate_queue(a, false);
+}
+
+// Rebuild steps only: the candidate sweep (LinkedCell.cpp:1019-1103) and the
+// screening of the new lists; each workgroup compacts the candidates that may
+// touch into a queue in LDS (kListCap entries per particle at most) and
+// evaluates them, one per lane.
+__global__ void __launch_bounds__(kPB) k_sweep_contact(PairArgs a) {
+  __shared__ int sh_wave[kPB / 64];
+  __shared__ int32_t sh_q[kPB * kListCap];
+  __shared__ unsigned long long sh_cnt[kPB / 64];
+  if (!a.d.gate.rebuild()) return;
+  const int t = logical_index(a.d.nblk8);
+  uint32_t may = 0;
+  if (t < a.d.n) {
+    may = screen_particle(a, t, true);
+    a.omask[t] = 0u;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int cnt = __builtin_popcount(may);
+  int incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) sh_wave[wid] = incl;
+  __syncthreads();
+  int pos = incl - cnt, nq = 0;
+#pragma unroll
+  for (int w = 0; w < kPB / 64; ++w) {
+    if (w < wid) pos += sh_wave[w];
+    nq += sh_wave[w];
+  }
+  for (uint32_t mm = may; mm != 0u; mm &= mm - 1u) sh_q[pos++] = t * 32 + __builtin_ctz(mm);
+  __syncthreads();
+  int nc = 0;
+  for (int q = threadIdx.x; q < nq; q += kPB) {
+    const int w = sh_q[q];
+    nc += contact_pair(a, w >> 5, w & 31, true);
   }
   count_contacts(a.contacts, nc, sh_cnt);
 }
@@ -530,21 +607,23 @@ void launch_from_pack(const Pack& pk, const PackConst& k, const State& s, int32_
   hipLaunchKernelGGL(k_from_pack, dim3(pblocks(n)), dim3(kPB), 0, st, pk, k, s, n);
 }
 
-void launch_pair(const PairArgs& a, hipStream_t st, hipEvent_t start, hipEvent_t stop) {
+void launch_pair(const Bins& b, const PosSrc& src, SkinWords* words, const PairArgs& a,
+                 hipStream_t st, hipEvent_t start, hipEvent_t stop) {
   if (a.d.n <= 0) return;
   const dim3 grid(a.d.nblk8), block(kPB);
-  // the queue holds a few candidates per hundred particles: a grid of an
-  // eighth of the particles' covers it in one pass (grid-stride beyond)
-  const dim3 cgrid(std::max(8, (a.d.nblk8 / 8 + 7) / 8 * 8));
-  if (start) {
-    hipExtLaunchKernelGGL(k_screen<true>, grid, block, 0, st, start, nullptr, 0, a);
-    hipLaunchKernelGGL(k_screen<false>, grid, block, 0, st, a);
-    hipExtLaunchKernelGGL(k_contact, cgrid, block, 0, st, nullptr, stop, 0, a);
-  } else {
-    hipLaunchKernelGGL(k_screen<true>, grid, block, 0, st, a);
-    hipLaunchKernelGGL(k_screen<false>, grid, block, 0, st, a);
-    hipLaunchKernelGGL(k_contact, cgrid, block, 0, st, a);
-  }
+  // the queue holds a few candidates per hundred particles: an eighth of the
+  // particles' grid covers it in one pass (grid-stride beyond); the scatter
+  // pass strides over the particles
+  const int cblocks = std::max(8, (a.d.nblk8 / 8 + 7) / 8 * 8);
+  const dim3 cgrid(std::max(cblocks, std::min(pblocks(a.d.n), 1024)));
+  const int32_t ncell = a.d.grid.ncell;
+  if (start) hipExtLaunchKernelGGL(k_bin_screen, grid, block, 0, st, start, nullptr, 0, b, src, words, a);
+  else hipLaunchKernelGGL(k_bin_screen, grid, block, 0, st, b, src, words, a);
+  launch_scan(b, ncell, a.d.gate, st);
+  hipLaunchKernelGGL(k_scatter_contact, cgrid, block, 0, st, b, a);
+  launch_cellsort(b, src, ncell, a.d.n, a.d.gate, st);
+  if (start) hipExtLaunchKernelGGL(k_sweep_contact, grid, block, 0, st, nullptr, stop, 0, a);
+  else hipLaunchKernelGGL(k_sweep_contact, grid, block, 0, st, a);
 }
 
 template <int M, bool WALLS>
