@@ -110,6 +110,7 @@ struct gdem_ctx {
   PairRec* d_prec = nullptr;
   int32_t* d_queue = nullptr;   // candidates that may touch (t * 32 + entry)
   int32_t* d_qcount = nullptr;  // its length, reset by the integrator
+  int32_t* d_wsrc = nullptr;    // 1: the packed copy's world angular velocity is derived
   // history (memory) contact models: per-particle memory tables, one per
   // state copy; allocated once a memory model is set and particles exist
   bool has_memory = false;
@@ -273,16 +274,16 @@ static int ensure_wall_loads(gdem_ctx* c) {
 static void free_pack(gdem_ctx* c) {
   for (int k = 0; k < 2; ++k) {
     (void)hipFree(c->pk[k].A); (void)hipFree(c->pk[k].B); (void)hipFree(c->pk[k].C);
-    (void)hipFree(c->pk[k].D);
+    (void)hipFree(c->pk[k].D); (void)hipFree(c->pk[k].E);
     (void)hipFree(c->pkc[k].rr); (void)hipFree(c->pkc[k].mi); (void)hipFree(c->pkc[k].aux);
     c->pk[k] = Pack{};
     c->pkc[k] = PackConst{};
   }
   (void)hipFree(c->d_omask); (void)hipFree(c->d_imask); (void)hipFree(c->d_prec);
-  (void)hipFree(c->d_queue); (void)hipFree(c->d_qcount);
+  (void)hipFree(c->d_queue); (void)hipFree(c->d_qcount); (void)hipFree(c->d_wsrc);
   c->d_omask = c->d_imask = nullptr;
   c->d_prec = nullptr;
-  c->d_queue = c->d_qcount = nullptr;
+  c->d_queue = c->d_qcount = c->d_wsrc = nullptr;
   c->packed = false;
 }
 
@@ -304,6 +305,7 @@ static int alloc_pack(gdem_ctx* c) {
     HIP_TRY(hipMalloc(&c->pk[k].B, sizeof(double4) * n));
     HIP_TRY(hipMalloc(&c->pk[k].C, sizeof(double4) * n));
     HIP_TRY(hipMalloc(&c->pk[k].D, sizeof(double4) * n));
+    HIP_TRY(hipMalloc(&c->pk[k].E, sizeof(double) * n));
     HIP_TRY(hipMalloc(&c->pkc[k].rr, sizeof(double2) * n));
     HIP_TRY(hipMalloc(&c->pkc[k].mi, sizeof(double2) * n));
     HIP_TRY(hipMalloc(&c->pkc[k].aux, sizeof(Aux) * n));
@@ -317,6 +319,8 @@ static int alloc_pack(gdem_ctx* c) {
   HIP_TRY(hipMalloc(&c->d_queue, sizeof(int32_t) * (size_t)queue_seg(c) * kQueueSegs));
   HIP_TRY(hipMalloc(&c->d_qcount, sizeof(int32_t) * kQueueSegs));
   HIP_TRY(hipMemset(c->d_qcount, 0, sizeof(int32_t) * kQueueSegs));
+  HIP_TRY(hipMalloc(&c->d_wsrc, sizeof(int32_t)));
+  HIP_TRY(hipMemset(c->d_wsrc, 0, sizeof(int32_t)));
   return GDEM_OK;
 }
 
@@ -324,7 +328,8 @@ static int alloc_pack(gdem_ctx* c) {
 // call other than gdem_step reads and writes the canonical copies).
 static int ensure_canon(gdem_ctx* c) {
   if (!c->packed) return GDEM_OK;
-  launch_from_pack(c->pk[c->cur], c->pkc[c->cur], c->buf[c->cur], (int32_t)c->n, c->stream);
+  launch_from_pack(c->pk[c->cur], c->pk[1 - c->cur], c->pkc[c->cur], c->buf[c->cur], c->d_wsrc,
+                   c->bins.perm, c->d_skin, c->cur, (int32_t)c->n, c->stream);
   HIP_TRY(hipGetLastError());
   c->packed = false;
   c->graph_gen++;
@@ -336,7 +341,8 @@ static int ensure_packed(gdem_ctx* c) {
   if (c->packed) return GDEM_OK;
   const int ra = alloc_pack(c);
   if (ra) return ra;
-  launch_to_pack(c->buf[c->cur], c->pk[c->cur], c->pkc[0], c->pkc[1], (int32_t)c->n, c->stream);
+  launch_to_pack(c->buf[c->cur], c->pk[c->cur], c->pkc[0], c->pkc[1], c->d_wsrc, (int32_t)c->n,
+                 c->stream);
   HIP_TRY(hipGetLastError());
   c->packed = true;
   c->graph_gen++;
@@ -1014,6 +1020,8 @@ static int pack_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wri
   d.skin = c->skin;
   d.gate = gate;
   pa.in = c->pk[p];
+  pa.other = c->pk[1 - p];
+  pa.wsrc = c->d_wsrc;
   pa.kin = kin;
   pa.omask = c->d_omask;
   pa.imask = c->d_imask;
@@ -1031,6 +1039,7 @@ static int pack_launches(gdem_ctx* c, int mode, double dtv, double dtd, bool wri
   IntegArgs a{};
   a.in = c->pk[p];
   a.out = c->pk[1 - p];
+  a.wsrc = c->d_wsrc;
   a.kin = kin;
   a.kout = c->pkc[1 - p];
   a.perm = c->bins.perm;
@@ -1420,7 +1429,7 @@ extern "C" int gdem_step(gdem_ctx* c, int32_t nsteps, double dt) {
     note_move_step(c, dt);
     launch_move_pack(c->pk[c->cur], c->pkc[c->cur], c->d_F, c->d_T, (int32_t)c->n, c->grid,
                      0.5 * dt, dt, c->d_err, SkinGate{c->d_skin, c->cur, skin_limit(c)},
-                     c->d_path, c->stream);
+                     c->d_path, c->d_wsrc, c->stream);
     HIP_TRY(hipGetLastError());
   } else {
     rc = gdem_move(c, 0.5 * dt, dt);

@@ -414,8 +414,10 @@ void launch_set_state(const State& s, double4* F, double4* T, int32_t n, const d
 struct Pack {
   double4* A;  // x, y, z, q.x
   double4* B;  // v, q.y
-  double4* C;  // world angular velocity, q.z
-  double4* D;  // body-frame angular velocity, q.w
+  double4* C;  // world angular velocity (.w unused), stored only when the
+               // W word (wsrc) is 0; otherwise derived (world_omega)
+  double4* D;  // body-frame angular velocity, q.z
+  double* E;   // q.w
 };
 struct PackConst {
   double2* rr;  // radius, equivalent-sphere radius
@@ -438,6 +440,8 @@ constexpr int kQueueSegs = 64;
 struct PairArgs {
   DetectArgs d;        // the candidate sweep of a rebuild (d.in unused)
   Pack in;
+  Pack other;          // the other copy: the orientations before the last step
+  int32_t* wsrc;       // W word: 1 when in.C is not stored (world_omega)
   PackConst kin;
   uint32_t* omask;     // [n] out entries in contact (cleared by the screening)
   uint32_t* imask;     // [n] in entries in contact (atomic OR, zero on entry)
@@ -452,6 +456,7 @@ struct PairArgs {
 
 struct IntegArgs {
   Pack in, out;
+  int32_t* wsrc;       // W word, set for the copy this kernel writes
   PackConst kin, kout;
   const int32_t* perm;
   Grid grid;
@@ -481,10 +486,15 @@ struct IntegArgs {
   int32_t* qcount;     // the pair queue's segment lengths, cleared for the next step
 };
 
+// Both conversions keep the W word: to_pack stores the world angular
+// velocity (word 0); from_pack derives it where it is not stored, from the
+// other copy (other) at the slots perm gives when the last evaluation was a
+// rebuild (words->rebuilt[p]).
 void launch_to_pack(const State& s, const Pack& pk, const PackConst& k0, const PackConst& k1,
-                    int32_t n, hipStream_t st);
-void launch_from_pack(const Pack& pk, const PackConst& k, const State& s, int32_t n,
-                      hipStream_t st);
+                    int32_t* wsrc, int32_t n, hipStream_t st);
+void launch_from_pack(const Pack& pk, const Pack& other, const PackConst& k, const State& s,
+                      const int32_t* wsrc, const int32_t* perm, const SkinWords* words, int p,
+                      int32_t n, hipStream_t st);
 // The pair stage of a packed step, the binning of a rebuild included (its
 // count and scatter passes run inside the pair kernels).
 void launch_pair(const Bins& b, const PosSrc& src, SkinWords* words, const PairArgs& a,
@@ -494,6 +504,7 @@ void launch_integ(int mode, const IntegArgs& a, hipStream_t st, hipEvent_t start
 // Kick + drift of the packed copy in place (the opening half of gdem_step).
 void launch_move_pack(const Pack& pk, const PackConst& k, const double4* F, const double4* T,
                       int32_t n, const Grid& g, double dt_vel, double dt_disp,
-                      unsigned long long* err, SkinGate gate, float* path, hipStream_t st);
+                      unsigned long long* err, SkinGate gate, float* path, int32_t* wsrc,
+                      hipStream_t st);
 
 }  // namespace gdem

@@ -51,16 +51,29 @@ constexpr int kPB = 256;
 static inline int pblocks(int n) { return (n + kPB - 1) / kPB; }
 
 // ---------------------------------------------------------------- layout
+// World angular velocity of slot s of copy `in` where it is not stored: the
+// step that wrote the copy computed it as q (omega_body) q* with the
+// orientation q from before its update (SecondOrderLeapFrog.cpp:74-94, then
+// ParticleKinematics.cpp:62-104), which the other copy still holds at slot
+// so.  The same operations in the same order: the value is bit-identical.
+__device__ __forceinline__ v3 world_omega(const Pack& in, const Pack& other, int s, int so) {
+  const double4 d = in.D[s];
+  const quat q = qmk(mk(other.A[so].w, other.B[so].w, other.D[so].w), other.E[so]);
+  return qmulv(q, qleftvec(qconj(q), mk(d.x, d.y, d.z)));
+}
+
 __global__ void __launch_bounds__(kPB) k_to_pack(State s, Pack pk, PackConst k0, PackConst k1,
-                                                 int n) {
+                                                 int32_t* wsrc, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *wsrc = 0;  // the world angular velocity is stored
   if (i >= n) return;
   const double4 pr = s.pos_r[i], vm = s.vel_m[i], wr = s.om_req[i], wb = s.wbf_I[i],
                 q = s.quat[i];
   pk.A[i] = make_double4(pr.x, pr.y, pr.z, q.x);
   pk.B[i] = make_double4(vm.x, vm.y, vm.z, q.y);
-  pk.C[i] = make_double4(wr.x, wr.y, wr.z, q.z);
-  pk.D[i] = make_double4(wb.x, wb.y, wb.z, q.w);
+  pk.C[i] = make_double4(wr.x, wr.y, wr.z, 0.);
+  pk.D[i] = make_double4(wb.x, wb.y, wb.z, q.z);
+  pk.E[i] = q.w;
   const double2 rr = make_double2(pr.w, wr.w), mi = make_double2(vm.w, wb.w);
   const Aux ax = s.aux[i];
   k0.rr[i] = rr;
@@ -71,16 +84,25 @@ __global__ void __launch_bounds__(kPB) k_to_pack(State s, Pack pk, PackConst k0,
   k1.aux[i] = ax;
 }
 
-__global__ void __launch_bounds__(kPB) k_from_pack(Pack pk, PackConst k, State s, int n) {
+__global__ void __launch_bounds__(kPB) k_from_pack(Pack pk, Pack other, PackConst k, State s,
+                                                   const int32_t* wsrc, const int32_t* perm,
+                                                   const SkinWords* words, int p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double4 a = pk.A[i], b = pk.B[i], c = pk.C[i], d = pk.D[i];
+  const double4 a = pk.A[i], b = pk.B[i], d = pk.D[i];
   const double2 rr = k.rr[i], mi = k.mi[i];
+  v3 w;
+  if (*wsrc != 0) {
+    w = world_omega(pk, other, i, words->rebuilt[p] ? perm[i] : i);
+  } else {
+    const double4 c = pk.C[i];
+    w = mk(c.x, c.y, c.z);
+  }
   s.pos_r[i] = make_double4(a.x, a.y, a.z, rr.x);
   s.vel_m[i] = make_double4(b.x, b.y, b.z, mi.x);
-  s.om_req[i] = make_double4(c.x, c.y, c.z, rr.y);
+  s.om_req[i] = make_double4(w.x, w.y, w.z, rr.y);
   s.wbf_I[i] = make_double4(d.x, d.y, d.z, mi.y);
-  s.quat[i] = make_double4(a.w, b.w, c.w, d.w);
+  s.quat[i] = make_double4(a.w, b.w, d.w, pk.E[i]);
   s.aux[i] = k.aux[i];
 }
 
@@ -121,7 +143,8 @@ __device__ __forceinline__ uint32_t screen_particle(const PairArgs& a, int t, bo
 // HertzContactForceModel.cpp:46-114).  A contact stores its record at the
 // entry and flags the entry in t's out mask and the mirror entry in the
 // partner's in mask (integer OR).  Returns 1 for a contact.
-__device__ __forceinline__ int contact_pair(const PairArgs& a, int t, int e, bool rebuild) {
+__device__ __forceinline__ int contact_pair(const PairArgs& a, int t, int e, bool rebuild,
+                                            bool wderived) {
   const DetectArgs& d = a.d;
   const PosSrc src{a.in.A, a.kin.rr, a.kin.aux};
   const int s = rebuild ? d.perm[t] : t;
@@ -132,14 +155,26 @@ __device__ __forceinline__ int contact_pair(const PairArgs& a, int t, int e, boo
   const int u = en.x, code = en.y;
   const int su = rebuild ? d.perm[u] : u;
   const double4 p2 = src.pos(su);
-  const double4 b = a.in.B[s], c4 = a.in.C[s];
+  const double4 b = a.in.B[s];
   const double2 rr = a.kin.rr[s], mi = a.kin.mi[s];
   const Aux ax = a.kin.aux[s];
-  const double4 b2 = a.in.B[su], c2 = a.in.C[su];
+  const double4 b2 = a.in.B[su];
   const double2 rr2 = a.kin.rr[su], mi2 = a.kin.mi[su];
   const Aux ax2 = a.kin.aux[su];
-  const v3 V = mk(b.x, b.y, b.z), W = mk(c4.x, c4.y, c4.z);
-  const v3 V2 = mk(b2.x, b2.y, b2.z), W2 = mk(c2.x, c2.y, c2.z);
+  const v3 V = mk(b.x, b.y, b.z), V2 = mk(b2.x, b2.y, b2.z);
+  v3 W, W2;
+  if (wderived) {
+    // the copy's world angular velocity is not stored (never on a rebuild
+    // step: the first pair kernel of a rebuild stores it); after a rebuild
+    // the other copy is in the previous order (perm)
+    const bool moved = a.d.gate.w->rebuilt[a.d.gate.p] != 0;
+    W = world_omega(a.in, a.other, s, moved ? d.perm[s] : s);
+    W2 = world_omega(a.in, a.other, su, moved ? d.perm[su] : su);
+  } else {
+    const double4 c4 = a.in.C[s], c2 = a.in.C[su];
+    W = mk(c4.x, c4.y, c4.z);
+    W2 = mk(c2.x, c2.y, c2.z);
+  }
   // either body may have crossed a periodic face since the list was built:
   // the partner is taken at its nearest image
   const int sx = image_shift(d.grid, 0, p2.x - X.x);
@@ -230,7 +265,7 @@ __device__ __forceinline__ void queue_candidates(const PairArgs& a, int t, uint3
 // The queued candidates, one per lane (grid-stride over the segments'
 // lengths); every contact record lands at its fixed entry, so the order in
 // which the queue was filled does not matter.
-__device__ __forceinline__ void evaluate_queue(const PairArgs& a, bool rebuild) {
+__device__ __forceinline__ void evaluate_queue(const PairArgs& a, bool rebuild, bool wderived) {
   __shared__ unsigned long long sh_cnt[kPB / 64];
   __shared__ int sh_pre[kQueueSegs + 1];
   static_assert(kQueueSegs == 64, "one lane of the first wave per queue segment");
@@ -257,7 +292,7 @@ __device__ __forceinline__ void evaluate_queue(const PairArgs& a, bool rebuild) 
       else hi = mid;
     }
     const int w = a.queue[(size_t)lo * a.qseg + (q - sh_pre[lo])];
-    nc += contact_pair(a, w >> 5, w & 31, rebuild);
+    nc += contact_pair(a, w >> 5, w & 31, rebuild, wderived);
   }
   count_contacts(a.contacts, nc, sh_cnt);
 }
@@ -280,6 +315,16 @@ __global__ void __launch_bounds__(kPB) k_bin_screen(Bins b, PosSrc src, SkinWord
   const bool rebuild = a.d.gate.rebuild();
   if (blockIdx.x == 0 && threadIdx.x == 0) skin_words_duty(a.d.gate, words, rebuild);
   if (rebuild) {
+    if (*a.wsrc != 0) {
+      // the sweep reads the copy through the new order: the world angular
+      // velocity is stored first, from the orientations of the other copy at
+      // the slots of the previous order (perm still holds it here)
+      const bool moved = a.d.gate.w->rebuilt[a.d.gate.p] != 0;
+      for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.d.n; i += gridDim.x * blockDim.x) {
+        const v3 w = world_omega(a.in, a.other, i, moved ? a.d.perm[i] : i);
+        a.in.C[i] = make_double4(w.x, w.y, w.z, 0.);
+      }
+    }
     count_pass(b.key, b.slot, b.cnt, a.d.n, true, a.d.in_count, src, a.d.grid, a.d.row_rank,
                a.d.err);
     return;
@@ -296,10 +341,13 @@ __global__ void __launch_bounds__(kPB) k_bin_screen(Bins b, PosSrc src, SkinWord
 // Scatter pass of a rebuild, or the evaluation of the queued candidates.
 __global__ void __launch_bounds__(kPB) k_scatter_contact(Bins b, PairArgs a) {
   if (a.d.gate.rebuild()) {
+    // the first kernel stored the world angular velocity (no kernel before
+    // this one reads the word after it)
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.wsrc = 0;
     scatter_pass(b.key, b.slot, b.local, b.bsum, b.perm, a.d.n);
     return;
   }
-  evaluate_queue(a, false);
+  evaluate_queue(a, false, *a.wsrc != 0);
 }
 
 // Rebuild steps only: the candidate sweep (LinkedCell.cpp:1019-1103) and the
@@ -338,7 +386,7 @@ __global__ void __launch_bounds__(kPB) k_sweep_contact(PairArgs a) {
   int nc = 0;
   for (int q = threadIdx.x; q < nq; q += kPB) {
     const int w = sh_q[q];
-    nc += contact_pair(a, w >> 5, w & 31, true);
+    nc += contact_pair(a, w >> 5, w & 31, true, false);
   }
   count_contacts(a.contacts, nc, sh_cnt);
 }
@@ -374,13 +422,21 @@ __device__ __forceinline__ bool wrap_axis(const IntegArgs& a, int d, double& x) 
 template <int MODE, bool WALLS>
 __device__ __forceinline__ int integ_particle(const IntegArgs& a, int t, int s, bool copy_const,
                                               double& drift) {
-  const double4 A4 = a.in.A[s], B4 = a.in.B[s], C4 = a.in.C[s], D4 = a.in.D[s];
+  const double4 A4 = a.in.A[s], B4 = a.in.B[s], D4 = a.in.D[s];
+  const double qw = a.in.E[s];
   const double2 mi = a.kin.mi[s];
   const uint32_t om = a.omask[t];
   const uint32_t im = a.imask[t];
-  const v3 X = mk(A4.x, A4.y, A4.z), V = mk(B4.x, B4.y, B4.z), W = mk(C4.x, C4.y, C4.z);
+  const v3 X = mk(A4.x, A4.y, A4.z), V = mk(B4.x, B4.y, B4.z);
+  // the world angular velocity is read only where it is used (wall
+  // contacts, the force-only evaluation, which finds it stored)
+  v3 W = mk(0., 0., 0.);
+  if (WALLS || MODE == MODE_FORCE) {
+    const double4 C4 = a.in.C[s];
+    W = mk(C4.x, C4.y, C4.z);
+  }
   v3 WBF = mk(D4.x, D4.y, D4.z);
-  quat q = qmk(mk(A4.w, B4.w, C4.w), D4.w);
+  quat q = qmk(mk(A4.w, B4.w, D4.w), qw);
   const double m = mi.x, I = mi.y;
   // InitializeForce: weight m (1 - 0/rho) g at the centre (Particle.cpp:414-419, 1409-1416)
   const double mw = m * (1. - 0. / 1.);
@@ -491,13 +547,18 @@ __device__ __forceinline__ int integ_particle(const IntegArgs& a, int t, int s, 
       }
     }
   }
-  // streaming stores: the new copy (128 B per particle) would otherwise sit
-  // dirty in the L2s until the kernel's end (measured: 5 us per step less)
+  // streaming stores: the new copy would otherwise sit dirty in the L2s
+  // until the kernel's end (measured: 5 us per step less at 1M particles)
   typedef double nd4 __attribute__((ext_vector_type(4)));
   __builtin_nontemporal_store((nd4){Xo.x, Xo.y, Xo.z, q.v.x}, reinterpret_cast<nd4*>(a.out.A + t));
   __builtin_nontemporal_store((nd4){Vo.x, Vo.y, Vo.z, q.v.y}, reinterpret_cast<nd4*>(a.out.B + t));
-  __builtin_nontemporal_store((nd4){Wo.x, Wo.y, Wo.z, q.v.z}, reinterpret_cast<nd4*>(a.out.C + t));
-  __builtin_nontemporal_store((nd4){WBF.x, WBF.y, WBF.z, q.w}, reinterpret_cast<nd4*>(a.out.D + t));
+  __builtin_nontemporal_store((nd4){WBF.x, WBF.y, WBF.z, q.v.z}, reinterpret_cast<nd4*>(a.out.D + t));
+  __builtin_nontemporal_store(q.w, a.out.E + t);
+  // a step of the step sequence without walls leaves the world angular
+  // velocity to be derived (world_omega): 24 B per particle neither read
+  // nor written here
+  if (MODE != MODE_MOVE || WALLS)
+    __builtin_nontemporal_store((nd4){Wo.x, Wo.y, Wo.z, 0.}, reinterpret_cast<nd4*>(a.out.C + t));
   if (copy_const) {
     // a rebuild reorders the constants; the evaluation after it fills the
     // other copy
@@ -530,7 +591,10 @@ __global__ void __launch_bounds__(kPB) k_integ(IntegArgs a) {
       if (len > a.gate.limit) atomicOr(&a.words->need[1 - p], 1);
     }
   }
-  if (t == 0) a.words->rebuilt[1 - p] = rebuild ? 1 : 0;
+  if (t == 0) {
+    a.words->rebuilt[1 - p] = rebuild ? 1 : 0;
+    *a.wsrc = (MODE == MODE_MOVE && !WALLS) ? 1 : 0;
+  }
   // the queue of the next screening starts empty
   if (blockIdx.x == 0 && threadIdx.x < kQueueSegs) a.qcount[threadIdx.x] = 0;
   if (WALLS) count_contacts(a.contacts, nwc, sh_cnt);
@@ -542,15 +606,16 @@ __global__ void __launch_bounds__(kPB) k_integ(IntegArgs a) {
 __global__ void __launch_bounds__(kPB) k_move_pack(Pack pk, PackConst k, const double4* Fa,
                                                    const double4* Ta, int n, Grid g, double dtv,
                                                    double dtd, unsigned long long* err,
-                                                   SkinGate gate, float* path) {
+                                                   SkinGate gate, float* path, int32_t* wsrc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double depl = 0.;
+  if (i == 0) *wsrc = 0;  // stored below
   if (i < n) {
-    const double4 A4 = pk.A[i], B4 = pk.B[i], C4 = pk.C[i], D4 = pk.D[i];
+    const double4 A4 = pk.A[i], B4 = pk.B[i], D4 = pk.D[i];
     const double2 mi = k.mi[i];
     const double4 f4 = Fa[i], t4 = Ta[i];
     const v3 F = mk(f4.x, f4.y, f4.z), T = mk(t4.x, t4.y, t4.z);
-    quat q = qmk(mk(A4.w, B4.w, C4.w), D4.w);
+    quat q = qmk(mk(A4.w, B4.w, D4.w), pk.E[i]);
     const quat pc = qconj(q);
     const v3 tq = qmulv(pc, qleftvec(q, T));
     const double mcf = mi.x * 1.;
@@ -585,8 +650,9 @@ __global__ void __launch_bounds__(kPB) k_move_pack(Pack pk, PackConst k, const d
     if (!cell_of(g, X, ijk)) atomicMin(&err[kErrSim], (unsigned long long)(uint32_t)ax.id);
     pk.A[i] = make_double4(X.x, X.y, X.z, q.v.x);
     pk.B[i] = make_double4(V.x, V.y, V.z, q.v.y);
-    pk.C[i] = make_double4(W.x, W.y, W.z, q.v.z);
-    pk.D[i] = make_double4(WBF.x, WBF.y, WBF.z, q.w);
+    pk.C[i] = make_double4(W.x, W.y, W.z, 0.);
+    pk.D[i] = make_double4(WBF.x, WBF.y, WBF.z, q.v.z);
+    pk.E[i] = q.w;
     // a move in place of copy p: the next evaluation reads need[p]
     const float len = skin_path(path[i], false, depl);
     path[i] = len;
@@ -596,15 +662,17 @@ __global__ void __launch_bounds__(kPB) k_move_pack(Pack pk, PackConst k, const d
 
 // ---------------------------------------------------------------- launchers
 void launch_to_pack(const State& s, const Pack& pk, const PackConst& k0, const PackConst& k1,
-                    int32_t n, hipStream_t st) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_to_pack, dim3(pblocks(n)), dim3(kPB), 0, st, s, pk, k0, k1, n);
+                    int32_t* wsrc, int32_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_to_pack, dim3(pblocks(std::max(n, 1))), dim3(kPB), 0, st, s, pk, k0, k1,
+                     wsrc, n);
 }
 
-void launch_from_pack(const Pack& pk, const PackConst& k, const State& s, int32_t n,
-                      hipStream_t st) {
+void launch_from_pack(const Pack& pk, const Pack& other, const PackConst& k, const State& s,
+                      const int32_t* wsrc, const int32_t* perm, const SkinWords* words, int p,
+                      int32_t n, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_from_pack, dim3(pblocks(n)), dim3(kPB), 0, st, pk, k, s, n);
+  hipLaunchKernelGGL(k_from_pack, dim3(pblocks(n)), dim3(kPB), 0, st, pk, other, k, s, wsrc, perm,
+                     words, p, n);
 }
 
 void launch_pair(const Bins& b, const PosSrc& src, SkinWords* words, const PairArgs& a,
@@ -652,10 +720,10 @@ void launch_integ(int mode, const IntegArgs& a, hipStream_t st, hipEvent_t start
 
 void launch_move_pack(const Pack& pk, const PackConst& k, const double4* F, const double4* T,
                       int32_t n, const Grid& g, double dt_vel, double dt_disp,
-                      unsigned long long* err, SkinGate gate, float* path, hipStream_t st) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_move_pack, dim3(pblocks(n)), dim3(kPB), 0, st, pk, k, F, T, n, g, dt_vel,
-                       dt_disp, err, gate, path);
+                      unsigned long long* err, SkinGate gate, float* path, int32_t* wsrc,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_move_pack, dim3(pblocks(std::max(n, 1))), dim3(kPB), 0, st, pk, k, F, T, n,
+                     g, dt_vel, dt_disp, err, gate, path, wsrc);
 }
 
 }  // namespace gdem
