@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # written and read): the algorithmic traffic of one step
 BYTES_PER_PARTICLE_STEP = 232.0
 BYTES_PER_CONTACT = 112.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_force_periodic_hertz_1M_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_integ_periodic_hertz_1M_traffic.txt")
 
 
 def log(msg):
