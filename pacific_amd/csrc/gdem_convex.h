@@ -6,6 +6,8 @@
 // starts from the shape's first vertex with neighbours, so results do not
 // depend on call history (they differ only when two vertices tie within 1e-16).
 #pragma once
+#include <type_traits>
+
 #include "gdem_device.h"
 
 #pragma clang fp contract(off)
@@ -111,37 +113,61 @@ __host__ __device__ inline Xf crust_xf(const double* basis, v3 origin, const dou
 }
 
 // ------------------------------------------------------------ Johnson GJK
-// Geometry/src/Convex.cpp:219-430 (sub-algorithm) and :536-627 (driver)
+// Geometry/src/Convex.cpp:219-430 (sub-algorithm) and :536-627 (driver).
+// The tables are indexed by vertex slots and subset masks.  Every index here
+// is a compile-time constant: the slot of the new vertex is a template
+// argument (one instantiation per slot, picked by a switch) and the subsets
+// are enumerated by unrolled loops in the order the reference visits them,
+// so the tables live in registers instead of per-lane scratch memory.  The
+// operations on each table entry, and their order, are the reference's.
 struct Gjk {
   v3 y[4];
   double dp[4][4];
   double det[16][4];
 };
 
-__host__ __device__ inline void gjk_compute_det(unsigned bits, unsigned last, unsigned last_bit,
-                                                unsigned all_bits, Gjk& g) {
-  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1)
-    if (bits & bit) g.dp[i][last] = g.dp[last][i] = dot(g.y[i], g.y[last]);
-  g.dp[last][last] = dot(g.y[last], g.y[last]);
-  g.det[last_bit][last] = 1.;
-  for (unsigned j = 0, sj = 1; j < 4; ++j, sj <<= 1) {
-    if (bits & sj) {
-      unsigned s2 = sj | last_bit;
-      g.det[s2][j] = g.dp[last][last] - g.dp[last][j];
-      g.det[s2][last] = g.dp[j][j] - g.dp[j][last];
-      for (unsigned k = 0, sk = 1; k < j; ++k, sk <<= 1) {
-        if (bits & sk) {
-          unsigned s3 = sk | s2;
-          g.det[s3][k] = g.det[s2][j] * (g.dp[j][j] - g.dp[j][k]) +
-                         g.det[s2][last] * (g.dp[last][j] - g.dp[last][k]);
-          g.det[s3][j] = g.det[sk | last_bit][k] * (g.dp[k][k] - g.dp[k][j]) +
-                         g.det[sk | last_bit][last] * (g.dp[last][k] - g.dp[last][j]);
-          g.det[s3][last] = g.det[sk | sj][k] * (g.dp[k][k] - g.dp[k][last]) +
-                            g.det[sk | sj][j] * (g.dp[j][k] - g.dp[j][last]);
-        }
+// f(integral_constant<unsigned, K>) for K = B, B + D, ... while K != E.
+template <unsigned B, unsigned E, int D, class F>
+__host__ __device__ inline void gjk_for(F&& f) {
+  if constexpr (B != E) {
+    f(std::integral_constant<unsigned, B>{});
+    gjk_for<(unsigned)((int)B + D), E, D>(f);
+  }
+}
+
+template <unsigned LAST>
+__host__ __device__ inline void gjk_compute_det(unsigned bits, unsigned all_bits, Gjk& g) {
+  constexpr unsigned LB = 1u << LAST;
+  gjk_for<0, 4, 1>([&](auto I) {
+    constexpr unsigned i = decltype(I)::value;
+    if (bits & (1u << i)) g.dp[i][LAST] = g.dp[LAST][i] = dot(g.y[i], g.y[LAST]);
+  });
+  g.dp[LAST][LAST] = dot(g.y[LAST], g.y[LAST]);
+  g.det[LB][LAST] = 1.;
+  gjk_for<0, 4, 1>([&](auto J) {
+    constexpr unsigned j = decltype(J)::value, sj = 1u << j;
+    if constexpr (j != LAST) {
+      if (bits & sj) {
+        constexpr unsigned s2 = sj | LB;
+        g.det[s2][j] = g.dp[LAST][LAST] - g.dp[LAST][j];
+        g.det[s2][LAST] = g.dp[j][j] - g.dp[j][LAST];
+        gjk_for<0, j, 1>([&](auto K) {
+          constexpr unsigned k = decltype(K)::value, sk = 1u << k;
+          if constexpr (k != LAST) {
+            if (bits & sk) {
+              constexpr unsigned s3 = sk | s2;
+              g.det[s3][k] = g.det[s2][j] * (g.dp[j][j] - g.dp[j][k]) +
+                             g.det[s2][LAST] * (g.dp[LAST][j] - g.dp[LAST][k]);
+              g.det[s3][j] = g.det[sk | LB][k] * (g.dp[k][k] - g.dp[k][j]) +
+                             g.det[sk | LB][LAST] * (g.dp[LAST][k] - g.dp[LAST][j]);
+              g.det[s3][LAST] = g.det[sk | sj][k] * (g.dp[k][k] - g.dp[k][LAST]) +
+                                g.det[sk | sj][j] * (g.dp[j][k] - g.dp[j][LAST]);
+            }
+          }
+        });
       }
     }
-  }
+  });
   if (all_bits == 15) {
     g.det[15][0] = g.det[14][1] * (g.dp[1][1] - g.dp[1][0]) + g.det[14][2] * (g.dp[2][1] - g.dp[2][0]) +
                    g.det[14][3] * (g.dp[3][1] - g.dp[3][0]);
@@ -154,91 +180,132 @@ __host__ __device__ inline void gjk_compute_det(unsigned bits, unsigned last, un
   }
 }
 
-__host__ __device__ inline bool gjk_valid(unsigned s, unsigned all_bits, const Gjk& g) {
-  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1) {
+template <unsigned S>
+__host__ __device__ inline bool gjk_valid(unsigned all_bits, const Gjk& g) {
+  bool ok = true;
+  gjk_for<0, 4, 1>([&](auto I) {
+    constexpr unsigned i = decltype(I)::value, bit = 1u << i;
     if (all_bits & bit) {
-      if (s & bit) {
-        if (g.det[s][i] <= 1.0e-20) return false;  // EPSILON3
-      } else if (g.det[s | bit][i] > 0) {
-        return false;
+      if constexpr ((S & bit) != 0) {
+        if (g.det[S][i] <= 1.0e-20) ok = false;  // EPSILON3
+      } else {
+        if (g.det[S | bit][i] > 0) ok = false;
       }
     }
-  }
-  return true;
+  });
+  return ok;
 }
 
-__host__ __device__ inline v3 gjk_vector(unsigned bits, const Gjk& g) {
+template <unsigned S>
+__host__ __device__ inline v3 gjk_vector(const Gjk& g) {
   double sum = 0.;
   v3 v = mk(0., 0., 0.);
-  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1) {
-    if (bits & bit) {
-      sum += g.det[bits][i];
-      v = add(v, scl(g.y[i], g.det[bits][i]));
+  gjk_for<0, 4, 1>([&](auto I) {
+    constexpr unsigned i = decltype(I)::value;
+    if constexpr ((S & (1u << i)) != 0) {
+      sum += g.det[S][i];
+      v = add(v, scl(g.y[i], g.det[S][i]));
     }
-  }
+  });
   return scl(v, 1. / sum);
 }
 
-__host__ __device__ inline bool gjk_proper(unsigned s, const Gjk& g) {
-  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1)
-    if ((s & bit) && g.det[s][i] <= 1.0e-20) return false;
-  return true;
+template <unsigned S>
+__host__ __device__ inline bool gjk_proper(const Gjk& g) {
+  bool ok = true;
+  gjk_for<0, 4, 1>([&](auto I) {
+    constexpr unsigned i = decltype(I)::value;
+    if constexpr ((S & (1u << i)) != 0)
+      if (g.det[S][i] <= 1.0e-20) ok = false;
+  });
+  return ok;
 }
 
-__host__ __device__ inline bool gjk_closest(unsigned& bits, unsigned last, unsigned last_bit,
-                                            unsigned all_bits, Gjk& g, v3& v) {
-  unsigned s;
-  gjk_compute_det(bits, last, last_bit, all_bits, g);
-  for (s = bits; s; --s) {
-    if ((s & bits) == s) {
-      if (gjk_valid(s | last_bit, all_bits, g)) {
-        bits = s | last_bit;
-        v = gjk_vector(bits, g);
-        return true;
+// gjk_vector of a run-time subset.
+__host__ __device__ inline v3 gjk_vector_of(unsigned bits, const Gjk& g) {
+  v3 v = mk(0., 0., 0.);
+  gjk_for<1, 16, 1>([&](auto S) {
+    if (bits == decltype(S)::value) v = gjk_vector<decltype(S)::value>(g);
+  });
+  return v;
+}
+
+template <unsigned LAST>
+__host__ __device__ inline bool gjk_closest(unsigned& bits, unsigned all_bits, Gjk& g, v3& v) {
+  constexpr unsigned LB = 1u << LAST;
+  gjk_compute_det<LAST>(bits, all_bits, g);
+  // the subsets s of bits in decreasing order: the first with s | last valid
+  bool found = false;
+  unsigned pick = 0;
+  gjk_for<15, 0, -1>([&](auto S) {
+    constexpr unsigned s = decltype(S)::value;
+    if constexpr ((s & LB) == 0) {
+      if (!found && (s & bits) == s && gjk_valid<s | LB>(all_bits, g)) {
+        found = true;
+        pick = s | LB;
       }
     }
+  });
+  if (found) {
+    bits = pick;
+    v = gjk_vector_of(bits, g);
+    return true;
   }
-  if (gjk_valid(last_bit, all_bits, g)) {
-    bits = last_bit;
-    v = g.y[last];
+  if (gjk_valid<LB>(all_bits, g)) {
+    bits = LB;
+    v = g.y[LAST];
     return true;
   }
   double min_dist2 = __builtin_huge_val();
-  for (s = all_bits; s; --s) {
-    if ((s & all_bits) == s) {
-      if (gjk_proper(s, g)) {
-        v3 u = gjk_vector(s, g);
-        double dist2 = dot(u, u);
-        if (dist2 < min_dist2) {
-          min_dist2 = dist2;
-          bits = s;
-          v = u;
-        }
+  gjk_for<15, 0, -1>([&](auto S) {
+    constexpr unsigned s = decltype(S)::value;
+    if ((s & all_bits) == s && gjk_proper<s>(g)) {
+      const v3 u = gjk_vector<s>(g);
+      const double dist2 = dot(u, u);
+      if (dist2 < min_dist2) {
+        min_dist2 = dist2;
+        bits = s;
+        v = u;
       }
     }
-  }
+  });
   return false;
 }
 
 __host__ __device__ inline bool gjk_degenerate(unsigned all_bits, const v3* y, v3 w) {
-  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1)
-    if ((all_bits & bit) && y[i].x == w.x && y[i].y == w.y && y[i].z == w.z) return true;
-  return false;
+  bool deg = false;
+  gjk_for<0, 4, 1>([&](auto I) {
+    constexpr unsigned i = decltype(I)::value;
+    if ((all_bits & (1u << i)) && y[i].x == w.x && y[i].y == w.y && y[i].z == w.z) deg = true;
+  });
+  return deg;
+}
+
+// New vertex in slot LAST (the lowest free one), then the sub-algorithm.
+template <unsigned LAST>
+__host__ __device__ inline bool gjk_add(unsigned& bits, unsigned all_bits, Gjk& g, v3* p, v3* q,
+                                        v3 w, v3 pl, v3 ql, v3& v) {
+  g.y[LAST] = w;
+  p[LAST] = pl;
+  q[LAST] = ql;
+  return gjk_closest<LAST>(bits, all_bits, g, v);
 }
 
 // closest_points (Convex.cpp:536-627): tolerance EPSILON (GrainsExec.cpp:51),
 // no acceleration.  Returns the distance; pa, pb are the local witness points.
 __host__ __device__ inline double closest_points(const CShape& a, const CShape& b, const Xf& a2w,
                                                  const Xf& b2w, v3& pa, v3& pb, int& nbIter) {
-  unsigned bits = 0, last = 0, last_bit = 0, all_bits = 0;
+  unsigned bits = 0, all_bits = 0;
   v3 p[4], q[4];
   Gjk g;
-  for (int i = 0; i < 4; ++i) {
+  gjk_for<0, 4, 1>([&](auto I) {
+    constexpr unsigned i = decltype(I)::value;
     p[i] = q[i] = g.y[i] = mk(0., 0., 0.);
-    for (int j = 0; j < 4; ++j) g.dp[i][j] = 0.;
-  }
-  for (int s = 0; s < 16; ++s)
-    for (int j = 0; j < 4; ++j) g.det[s][j] = 0.;
+    gjk_for<0, 4, 1>([&](auto J) { g.dp[i][decltype(J)::value] = 0.; });
+  });
+  gjk_for<0, 16, 1>([&](auto S) {
+    gjk_for<0, 4, 1>([&](auto J) { g.det[decltype(S)::value][decltype(J)::value] = 0.; });
+  });
   double mu = 0.;
   int numIterations = 0;
   const double relError = kEps;
@@ -249,35 +316,44 @@ __host__ __device__ inline double closest_points(const CShape& a, const CShape& 
   double dist = norm(v);
   while (bits < 15 && dist > 1.0e-15 && numIterations < 1000) {
     ++numIterations;
-    last = 0;
-    last_bit = 1;
-    while (bits & last_bit) {
-      ++last;
-      last_bit <<= 1;
-    }
+    const unsigned last = (unsigned)__builtin_ctz(~bits);  // the lowest free slot
     const v3 d = v;
-    p[last] = support(a, vec_mat(neg(d), a2w.b));
-    q[last] = support(b, vec_mat(d, b2w.b));
-    w = sub(xf_apply(a2w, p[last]), xf_apply(b2w, q[last]));
+    // the support points of the reference's p[last] and q[last] (stored in
+    // gjk_add once the vertex is kept: a vertex that ends the loop is not)
+    const v3 pl = support(a, vec_mat(neg(d), a2w.b));
+    const v3 ql = support(b, vec_mat(d, b2w.b));
+    w = sub(xf_apply(a2w, pl), xf_apply(b2w, ql));
     mu = dist - dot(v, w) / dist;
     if (mu < dist * relError || mu < absError) break;
     if (gjk_degenerate(all_bits, g.y, w)) break;
-    g.y[last] = w;
-    all_bits = bits | last_bit;
-    if (!gjk_closest(bits, last, last_bit, all_bits, g, v)) break;
+    all_bits = bits | (1u << last);
+    bool ok;
+    switch (last) {
+      case 0: ok = gjk_add<0>(bits, all_bits, g, p, q, w, pl, ql, v); break;
+      case 1: ok = gjk_add<1>(bits, all_bits, g, p, q, w, pl, ql, v); break;
+      case 2: ok = gjk_add<2>(bits, all_bits, g, p, q, w, pl, ql, v); break;
+      default: ok = gjk_add<3>(bits, all_bits, g, p, q, w, pl, ql, v); break;
+    }
+    if (!ok) break;
     dist = norm(v);
   }
   // computePoints (Convex.cpp:314-337)
   double sum = 0.;
   pa = mk(0., 0., 0.);
   pb = mk(0., 0., 0.);
-  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1) {
-    if (bits & bit) {
-      sum += g.det[bits][i];
-      pa = add(pa, scl(p[i], g.det[bits][i]));
-      pb = add(pb, scl(q[i], g.det[bits][i]));
+  gjk_for<1, 16, 1>([&](auto S) {
+    constexpr unsigned s = decltype(S)::value;
+    if (bits == s) {
+      gjk_for<0, 4, 1>([&](auto I) {
+        constexpr unsigned i = decltype(I)::value;
+        if constexpr ((s & (1u << i)) != 0) {
+          sum += g.det[s][i];
+          pa = add(pa, scl(p[i], g.det[s][i]));
+          pb = add(pb, scl(q[i], g.det[s][i]));
+        }
+      });
     }
-  }
+  });
   const double s = 1. / sum;
   pa = scl(pa, s);
   pb = scl(pb, s);
