@@ -418,6 +418,68 @@ int gdem_read_particles_2014(gdem_ctx* ctx, const char* path, int32_t binary,
                              int32_t id_offset, int32_t wall_id_offset,
                              int32_t* geom_type_out);
 
+/* The particles' constants as gdem_set_particles received them, in id order
+   (any pointer may be NULL).  Sphere contexts. */
+int gdem_get_particles(gdem_ctx* ctx, double* radius, double* density, double* crust,
+                       int32_t* material);
+/* The contact model of material pair (a, b): kind (0: none) and its
+   parameters as gdem_set_contact_model took them (p: 8 doubles, unused ones
+   0). */
+int gdem_get_contact_model(gdem_ctx* ctx, int32_t a, int32_t b, int32_t* kind, double* p);
+/* Wall k as added (basis row-major, centre, half extents, crust, material,
+   kind GDEM_WALL_BOX or GDEM_WALL_RECTANGLE); any pointer may be NULL. */
+#define GDEM_WALL_BOX 0
+#define GDEM_WALL_RECTANGLE 1
+int gdem_get_wall_shape(gdem_ctx* ctx, int32_t k, double* basis, double* centre, double* half,
+                        double* crust, int32_t* material, int32_t* kind);
+/* Names of materials and walls as the reload files carry them (the
+   reference's material names, ContactBuilderFactory::m_materials,
+   Component/src/ContactBuilderFactory.cpp:174-187, and obstacle names,
+   Obstacle::m_name).  A name not set reads as "".  get_*: GDEM_ERR_CAPACITY
+   when cap bytes do not hold the name and its terminating zero. */
+int gdem_set_material_name(gdem_ctx* ctx, int32_t k, const char* name);
+int gdem_get_material_name(gdem_ctx* ctx, int32_t k, char* buf, int32_t cap);
+int gdem_set_wall_name(gdem_ctx* ctx, int32_t k, const char* name);
+int gdem_get_wall_name(gdem_ctx* ctx, int32_t k, char* buf, int32_t cap);
+
+/* A reload file set in the reference's format, as Grains::saveReload writes
+   it (Base/src/Grains.cpp:2543-2581) for a sphere context:
+     <root><suffix>.result          main file: "__Format2024__", "#Time t",
+                                    the contact-model file, the reference
+                                    particle types (Particle::write,
+                                    Component/src/Particle.cpp:1093-1149), the
+                                    obstacle tree (a "__AllObstacles___"
+                                    composite of box walls, SimpleObstacle::
+                                    write, Component/src/SimpleObstacle.cpp:354-
+                                    367), the particle-file count and mode
+                                    (AllComponents::write, Component/src/
+                                    AllComponents.cpp:1062-1146), "#EndTime";
+     <root>_MatContact.xml          materials and contact models
+                                    (ContactBuilderFactory::save, Component/src/
+                                    ContactBuilderFactory.cpp:270-325);
+     <root><suffix>_Particles.result[.bin]   the particle records
+                                    (gdem_write_particles_2014);
+     <root>_RFTable.txt             "t <root><suffix>" appended.
+   The reference particle types are the distinct (radius, density, crust,
+   material) of the particles in order of first appearance; type k has
+   IDnumber k.  Materials are named by gdem_set_material_name ("mat<k>" when
+   unset) and take the values 1, 2, 4, ... in index order; walls are named by
+   gdem_set_wall_name ("Obstacle<k>" when unset), obstacle k has id
+   wall_id_offset + k and a zero torsor.  binary != 0: hybrid mode (binary
+   particle file). */
+int gdem_write_reload(gdem_ctx* ctx, const char* root, const char* suffix, double time,
+                      int32_t binary, int32_t id_offset, int32_t wall_id_offset);
+/* Restart from such a file set (main file path; the contact-model file and
+   the particle file beside it, as Grains::readReload finds them): creates a
+   context on cfg (the domain is not part of a reload file) with the
+   materials (index = log2 of the value, names set), the contact models, the
+   box walls (names set) and the particles (constants from their types, state,
+   F, T and contact memory from the records), primed as the reference's reload
+   leaves it.  *time receives the file's time (may be NULL).  Sphere particles,
+   box and rectangle obstacles, one particle file. */
+int gdem_read_reload(const gdem_config* cfg, const char* main_path, int32_t id_offset,
+                     int32_t wall_id_offset, gdem_ctx** out, double* time);
+
 /* Contact list of the next force evaluations, kept for the last one.
    capacity_per_particle bounds the contacts a particle can be the first body
    of; 0 disables recording.  Records are in canonical order: sorted by (id0,

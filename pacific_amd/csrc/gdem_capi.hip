@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -49,6 +50,12 @@ struct gdem_ctx {
   Model h_models[kMaxMaterials * kMaxMaterials]{};
   Model* d_models = nullptr;
   Wall h_walls[kMaxWalls]{};
+  // host copies of what the reload files describe: the particles' constants
+  // in id order, material and wall names
+  std::vector<double> h_radius, h_density, h_crust;
+  std::vector<int32_t> h_material;
+  std::string mat_name[kMaxMaterials];
+  std::string wall_name[kMaxWalls];
   Wall* d_walls = nullptr;
   int nwalls = 0;
   unsigned long long* d_err = nullptr;
@@ -584,6 +591,11 @@ extern "C" int gdem_set_particles(gdem_ctx* c, int64_t n, const double* x, const
     c->rmax = std::max(c->rmax, rr);
     c->crust_min = i == 0 ? crust[i] : std::min(c->crust_min, crust[i]);
   }
+  c->h_radius.assign(radius, radius + n);
+  c->h_density.assign(density, density + n);
+  c->h_crust.assign(crust, crust + n);
+  c->h_material.assign((size_t)n, 0);
+  if (material) c->h_material.assign(material, material + n);
   State& s = c->buf[0];
   c->cur = 0;
   const size_t b4 = sizeof(double4) * (size_t)n;
@@ -2116,6 +2128,70 @@ extern "C" int gdem_get_contact_stats(gdem_ctx* c, double* out) {
   out[4] = (double)h.on;
   out[5] = (double)h.gn;
   return collect_errors(c);
+}
+
+extern "C" int gdem_get_particles(gdem_ctx* c, double* radius, double* density, double* crust,
+                                  int32_t* material) {
+  if (!c) return GDEM_ERR_ARG;
+  if (c->convex || c->h_radius.size() != (size_t)c->n) return GDEM_ERR_ARG;
+  const size_t n = (size_t)c->n;
+  if (radius) std::copy(c->h_radius.begin(), c->h_radius.begin() + n, radius);
+  if (density) std::copy(c->h_density.begin(), c->h_density.begin() + n, density);
+  if (crust) std::copy(c->h_crust.begin(), c->h_crust.begin() + n, crust);
+  if (material) std::copy(c->h_material.begin(), c->h_material.begin() + n, material);
+  return GDEM_OK;
+}
+
+extern "C" int gdem_get_contact_model(gdem_ctx* c, int32_t a, int32_t b, int32_t* kind,
+                                      double* p) {
+  if (!c || a < 0 || b < 0 || a >= kMaxMaterials || b >= kMaxMaterials) return GDEM_ERR_ARG;
+  const Model& m = c->h_models[a * kMaxMaterials + b];
+  if (kind) *kind = m.kind;
+  if (p)
+    for (int k = 0; k < 8; ++k) p[k] = k < model_nparams(m.kind) ? m.p[k] : 0.;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_get_wall_shape(gdem_ctx* c, int32_t k, double* basis, double* centre,
+                                   double* half, double* crust, int32_t* material, int32_t* kind) {
+  if (!c || k < 0 || k >= c->nwalls) return GDEM_ERR_ARG;
+  const Wall& w = c->h_walls[k];
+  if (basis) std::copy(w.basis, w.basis + 9, basis);
+  if (centre) std::copy(w.origin, w.origin + 3, centre);
+  if (half) std::copy(w.ext, w.ext + 3, half);
+  if (crust) *crust = w.crust;
+  if (material) *material = w.material;
+  if (kind) *kind = w.kind == WALL_RECTANGLE ? GDEM_WALL_RECTANGLE : GDEM_WALL_BOX;
+  return GDEM_OK;
+}
+
+static int copy_name(const std::string& s, char* buf, int32_t cap) {
+  if (!buf || cap <= 0) return GDEM_ERR_ARG;
+  if ((int64_t)s.size() + 1 > cap) return GDEM_ERR_CAPACITY;
+  memcpy(buf, s.c_str(), s.size() + 1);
+  return GDEM_OK;
+}
+
+extern "C" int gdem_set_material_name(gdem_ctx* c, int32_t k, const char* name) {
+  if (!c || !name || k < 0 || k >= kMaxMaterials) return GDEM_ERR_ARG;
+  c->mat_name[k] = name;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_get_material_name(gdem_ctx* c, int32_t k, char* buf, int32_t cap) {
+  if (!c || k < 0 || k >= kMaxMaterials) return GDEM_ERR_ARG;
+  return copy_name(c->mat_name[k], buf, cap);
+}
+
+extern "C" int gdem_set_wall_name(gdem_ctx* c, int32_t k, const char* name) {
+  if (!c || !name || k < 0 || k >= c->nwalls) return GDEM_ERR_ARG;
+  c->wall_name[k] = name;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_get_wall_name(gdem_ctx* c, int32_t k, char* buf, int32_t cap) {
+  if (!c || k < 0 || k >= c->nwalls) return GDEM_ERR_ARG;
+  return copy_name(c->wall_name[k], buf, cap);
 }
 
 extern "C" int gdem_get_grid(gdem_ctx* c, int32_t* nc, double* edge) {

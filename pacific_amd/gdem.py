@@ -142,6 +142,15 @@ def load_library(path: str = LIB_PATH):
         "gdem_set_contact_stats": (C.c_int, [vp, i32]),
         "gdem_get_contact_stats": (C.c_int, [vp, P(d)]),
         "gdem_get_grid": (C.c_int, [vp, P(i32), P(d)]),
+        "gdem_get_particles": (C.c_int, [vp, vp, vp, vp, vp]),
+        "gdem_get_contact_model": (C.c_int, [vp, i32, i32, P(i32), vp]),
+        "gdem_get_wall_shape": (C.c_int, [vp, i32, vp, vp, vp, P(d), P(i32), P(i32)]),
+        "gdem_set_material_name": (C.c_int, [vp, i32, C.c_char_p]),
+        "gdem_get_material_name": (C.c_int, [vp, i32, C.c_char_p, i32]),
+        "gdem_set_wall_name": (C.c_int, [vp, i32, C.c_char_p]),
+        "gdem_get_wall_name": (C.c_int, [vp, i32, C.c_char_p, i32]),
+        "gdem_write_reload": (C.c_int, [vp, C.c_char_p, C.c_char_p, d, i32, i32, i32]),
+        "gdem_read_reload": (C.c_int, [P(_Config), C.c_char_p, i32, i32, P(vp), P(d)]),
         "gdem_version": (C.c_char_p, []),
         "gdem_sphere_forces_noalloc": (C.c_int, [P(_Config), i32, vp, i64, vp, vp, vp, d, vp, vp,
                                                  vp, P(i32)]),
@@ -202,6 +211,9 @@ EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall"
             "gdem_get_contacts", "gdem_last_error", "gdem_set_kernel_timing",
             "gdem_get_counters", "gdem_reset_counters", "gdem_get_kernel_times",
             "gdem_set_contact_stats", "gdem_get_contact_stats", "gdem_get_grid",
+            "gdem_get_particles", "gdem_get_contact_model", "gdem_get_wall_shape",
+            "gdem_set_material_name", "gdem_get_material_name", "gdem_set_wall_name",
+            "gdem_get_wall_name", "gdem_write_reload", "gdem_read_reload",
             "gdem_version", "gdem_sphere_forces_noalloc",
             "gdem_add_box_shape", "gdem_add_polyhedron_shape", "gdem_set_convex_particles",
             "gdem_get_basis", "gdem_batch_create", "gdem_batch_destroy", "gdem_batch_add_system",
@@ -504,6 +516,54 @@ class GdemSystem:
                                                        int(id_offset), int(wall_id_offset), _ptr(gt)),
                     "gdem_write_particles_2014")
 
+    def set_material_name(self, k, name):
+        self._check(self.lib.gdem_set_material_name(self.h, int(k), str(name).encode()),
+                    "gdem_set_material_name")
+
+    def material_name(self, k):
+        buf = C.create_string_buffer(256)
+        self._check(self.lib.gdem_get_material_name(self.h, int(k), buf, 256), "gdem_get_material_name")
+        return buf.value.decode()
+
+    def set_wall_name(self, k, name):
+        self._check(self.lib.gdem_set_wall_name(self.h, int(k), str(name).encode()),
+                    "gdem_set_wall_name")
+
+    def wall_name(self, k):
+        buf = C.create_string_buffer(256)
+        self._check(self.lib.gdem_get_wall_name(self.h, int(k), buf, 256), "gdem_get_wall_name")
+        return buf.value.decode()
+
+    def particles(self):
+        """The particles' constants (radius, density, crust, material) in id order."""
+        r, rho, cr = np.empty(self.n), np.empty(self.n), np.empty(self.n)
+        mat = np.empty(self.n, np.int32)
+        self._check(self.lib.gdem_get_particles(self.h, _ptr(r), _ptr(rho), _ptr(cr), _ptr(mat)),
+                    "gdem_get_particles")
+        return {"radius": r, "density": rho, "crust": cr, "material": mat}
+
+    def contact_model(self, a, b):
+        kind = C.c_int32()
+        p = np.zeros(8)
+        self._check(self.lib.gdem_get_contact_model(self.h, int(a), int(b), C.byref(kind), _ptr(p)),
+                    "gdem_get_contact_model")
+        return kind.value, p
+
+    def wall_shape(self, k):
+        basis, centre, half = np.empty(9), np.empty(3), np.empty(3)
+        crust, mat, kind = C.c_double(), C.c_int32(), C.c_int32()
+        self._check(self.lib.gdem_get_wall_shape(self.h, int(k), _ptr(basis), _ptr(centre), _ptr(half),
+                                                 C.byref(crust), C.byref(mat), C.byref(kind)),
+                    "gdem_get_wall_shape")
+        return {"basis": basis, "centre": centre, "half": half, "crust": crust.value,
+                "material": mat.value, "kind": kind.value}
+
+    def write_reload(self, root, suffix, time, binary=True, id_offset=1, wall_id_offset=0):
+        """The reference's reload file set for this system (gdem_write_reload)."""
+        self._check(self.lib.gdem_write_reload(self.h, str(root).encode(), str(suffix).encode(),
+                                               float(time), int(bool(binary)), int(id_offset),
+                                               int(wall_id_offset)), "gdem_write_reload")
+
     def read_particles_2014(self, path, binary=True, id_offset=0, wall_id_offset=0):
         """Restore state and memory from a particle reload file; returns the
         records' geometric types in id order."""
@@ -580,6 +640,36 @@ class GdemSystem:
         e = (C.c_double * 3)()
         self._check(self.lib.gdem_get_grid(self.h, nc, e), "gdem_get_grid")
         return list(nc), list(e)
+
+
+def read_reload(main_path, origin, size, periodic=(0, 0, 0), cell_size=0.0,
+                gravity=(0.0, 0.0, 0.0), id_offset=1, wall_id_offset=0, device=0):
+    """A GdemSystem restarted from a reload file set in the reference's format
+    (gdem_read_reload; the domain comes from the caller).  Returns (system,
+    time)."""
+    lib = load_library()
+    cfg = _Config()
+    for d in range(3):
+        cfg.origin[d] = float(origin[d])
+        cfg.size[d] = float(size[d])
+        cfg.periodic[d] = int(bool(periodic[d]))
+        cfg.gravity[d] = float(gravity[d])
+    cfg.cell_size = float(cell_size)
+    cfg.device = int(device)
+    h = C.c_void_p()
+    t = C.c_double()
+    rc = lib.gdem_read_reload(C.byref(cfg), str(main_path).encode(), int(id_offset),
+                              int(wall_id_offset), C.byref(h), C.byref(t))
+    if rc != OK:
+        raise _ERRORS.get(rc, GdemError)(rc, "gdem_read_reload", -1, -1)
+    s = GdemSystem.__new__(GdemSystem)
+    s.lib = lib
+    s.h = h
+    n = C.c_int64()
+    nw, cv, mm = C.c_int32(), C.c_int32(), C.c_int32()
+    s._check(lib.gdem_info(h, C.byref(n), C.byref(nw), C.byref(cv), C.byref(mm)), "gdem_info")
+    s.n = n.value
+    return s, t.value
 
 
 def build_system(wl, device=0):
