@@ -168,7 +168,11 @@ int gdem_add_rectangle_wall(gdem_ctx* ctx, const double* basis, const double* ce
      kind GDEM_MODEL_HOOKE, p = {kn, en, etat, muc, kr}            (5 values)
    HertzContactForceModel(map) (HertzContactForceModel.cpp:12-22):
      kind GDEM_MODEL_HERTZ, p = {Es, en, Gs, muc, kr}              (5 values)
-   kr must be 0 (no rolling resistance) for these two.
+   kr >= 0 is the rolling resistance of these two (the moment
+   -(kr R* |FN| |w0 x (x0 - c) - w1 x (x1 - c)| / |w0 - w1|) (w0 - w1) on body
+   0, HookeContactForceModel.cpp:95-111, HertzContactForceModel.cpp:99-113):
+   sphere contexts; with kr != 0 gdem_step runs the canonical layout, and
+   convex contexts and differentiated systems refuse it.
    HookeMemoryContactForceModel(map) (HookeMemoryContactForceModel.cpp:11-29):
      kind GDEM_MODEL_HOOKE_MEMORY,
      p = {kn, en, etat, muc, kt, mur, etarpf, epst}                 (8 values)

@@ -1315,10 +1315,12 @@ static inline v3 vel_at(v3 x, v3 v, v3 w, v3 pt) { return add(v, cross(w, sub(pt
 
 /* HookeContactForceModel::performForcesCalculus (HookeContactForceModel.cpp:44-114)
    and HertzContactForceModel::performForcesCalculus (HertzContactForceModel.cpp:46-114);
-   returns delFN + delFT (kr = 0: no rolling moment). */
+   returns delFN + delFT; *delM receives the rolling-resistance moment on body 0
+   (HookeContactForceModel.cpp:95-111, HertzContactForceModel.cpp:99-113), zero
+   when kr = 0. */
 static v3 contact_force(const model_t* m, v3 pt, v3 pen, double odist, v3 x0, v3 v0,
                         v3 w0, double m0, double req0, v3 x1, v3 v1, v3 w1, double m1,
-                        double req1) {
+                        double req1, v3* delM) {
   v3 normal = dvd(pen, norm(pen));
   normal = round3(normal);
   v3 tmpV = sub(vel_at(x0, v0, w0, pt), vel_at(x1, v1, w1, pt));
@@ -1362,6 +1364,18 @@ static v3 contact_force(const model_t* m, v3 pt, v3 pen, double odist, v3 x0, v3
     if (nvt > ORC_EPSILON) tangent = dvd(neg(v_t), nvt);
     delFT = scl(tangent, fn);
   }
+  *delM = mk(0., 0., 0.);
+  if ((m->kind == ORC_MODEL_HOOKE || m->kind == ORC_MODEL_HERTZ) && m->p[4] != 0.) {
+    /* Vector3 wrel = w0 - w1; normwtrel = |w0 ^ (x0 - pt) - w1 ^ (x1 - pt)|;
+       delM = -(kr Req normFN normwtrel / normwrel) wrel when |wrel| > EPSILON */
+    v3 wrel = sub(w0, w1);
+    double normwrel = norm(wrel);
+    double normwtrel = norm(sub(cross(w0, sub(x0, pt)), cross(w1, sub(x1, pt))));
+    if (normwrel > ORC_EPSILON) {
+      double Req = 1. / (1. / req0 + 1. / req1);
+      *delM = scl(wrel, -(m->p[4] * Req * normFN * normwtrel / normwrel));
+    }
+  }
   return add(delFN, delFT);
 }
 
@@ -1369,16 +1383,19 @@ void orc_contact_force(int kind, const double* p, const double* point,
                        const double* ovec, double dist, const double* x0,
                        const double* v0, const double* w0, double m0,
                        double req0, const double* x1, const double* v1,
-                       const double* w1, double m1, double req1, double* f) {
+                       const double* w1, double m1, double req1, double* f,
+                       double* moment) {
   model_t m;
   memset(&m, 0, sizeof(m));
   m.kind = kind;
   memcpy(m.p, p, sizeof(double) * 5);
   m.beta = log(p[1]) / sqrt(ORC_PI * ORC_PI + log(p[1]) * log(p[1]));
   m.m2sqrt56 = -2. * sqrt(5. / 6.);
+  v3 dm;
   v3 r = contact_force(&m, ld(point), ld(ovec), dist, ld(x0), ld(v0), ld(w0), m0, req0,
-                       ld(x1), ld(v1), ld(w1), m1, req1);
+                       ld(x1), ld(v1), ld(w1), m1, req1, &dm);
   st(f, r);
+  if (moment) st(moment, dm);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1513,6 +1530,7 @@ static v3 memory_force(const model_t* m, v3 pt, v3 pen, double odist, v3 x0, v3 
 }
 
 static int rolling_model(const model_t* m) {
+  if (m->kind == ORC_MODEL_HOOKE || m->kind == ORC_MODEL_HERTZ) return m->p[4] != 0.;
   return m->p[m->kind == ORC_MODEL_HOOKE_MEMORY ? 5 : 4] != 0.;
 }
 
@@ -1719,7 +1737,8 @@ static int interact(orc_sys* s, int p0, int p1, v3 shift) {
   } else {
     f = contact_force(m, P, ld(ov), dist, A, ld(s->v + 3 * p0), ld(s->w + 3 * p0),
                       s->mass[p0], s->req[p0], B, ld(s->v + 3 * p1), ld(s->w + 3 * p1),
-                      s->mass[p1], s->req[p1]);
+                      s->mass[p1], s->req[p1], &delM);
+    rolling = rolling_model(m);
   }
   /* coef = 1: body 0 gets coef*(delFN+delFT); body 1 gets coef*(-delFN-delFT),
      which is -(delFN+delFT) bit for bit */
@@ -1781,7 +1800,8 @@ static int interact_wall(orc_sys* s, int p, int wi) {
     rolling = rolling_model(m);
   } else {
     f = contact_force(m, pt, ov, dist, A, ld(s->v + 3 * p), ld(s->w + 3 * p), s->mass[p],
-                      s->req[p], ld(wl->origin), wv, ww, 1.e20, wl->req);
+                      s->req[p], ld(wl->origin), wv, ww, 1.e20, wl->req, &delM);
+    rolling = rolling_model(m);
   }
   st(s->F + 3 * p, add(ld(s->F + 3 * p), f));
   st(s->T + 3 * p, add(ld(s->T + 3 * p), cross(sub(pt, A), f)));
@@ -2037,15 +2057,19 @@ static int interact_plain(orc_sys* s, int p0, int p1, v3 shift) {
   const model_t* m = &s->models[s->mat[p0]][s->mat[p1]];
   if (m->kind == ORC_MODEL_NONE) return ORC_ERR_ARG;
   v3 P = ld(pt);
+  v3 delM;
   v3 f = contact_force(m, P, ld(ov), dist, A, ld(s->v + 3 * p0), ld(s->w + 3 * p0),
                        s->mass[p0], s->req[p0], B, ld(s->v + 3 * p1), ld(s->w + 3 * p1),
-                       s->mass[p1], s->req[p1]);
+                       s->mass[p1], s->req[p1], &delM);
+  const int rolling = rolling_model(m);
   v3 f0 = scl(f, 1.);
   v3 f1 = neg(f);
   st(s->F + 3 * p0, add(ld(s->F + 3 * p0), f0));
   st(s->T + 3 * p0, add(ld(s->T + 3 * p0), cross(sub(P, A), f0)));
+  if (rolling) st(s->T + 3 * p0, add(ld(s->T + 3 * p0), scl(delM, 1.)));
   st(s->F + 3 * p1, add(ld(s->F + 3 * p1), f1));
   st(s->T + 3 * p1, add(ld(s->T + 3 * p1), cross(sub(P, B), f1)));
+  if (rolling) st(s->T + 3 * p1, add(ld(s->T + 3 * p1), scl(neg(delM), 1.)));
   return ORC_OK;
 }
 
@@ -2074,10 +2098,13 @@ static int wall_plain(orc_sys* s, int p, int wi) {
   if (rc == 0 || !(dist < 0.)) return ORC_OK;
   const model_t* m = &s->models[s->mat[p]][wl->material];
   if (m->kind == ORC_MODEL_NONE) return ORC_ERR_ARG;
+  v3 delM;
   v3 f = contact_force(m, pt, ov, dist, A, ld(s->v + 3 * p), ld(s->w + 3 * p), s->mass[p],
-                       s->req[p], ld(wl->origin), ld(wl->vel), ld(wl->omega), 1.e20, wl->req);
+                       s->req[p], ld(wl->origin), ld(wl->vel), ld(wl->omega), 1.e20, wl->req,
+                       &delM);
   st(s->F + 3 * p, add(ld(s->F + 3 * p), f));
   st(s->T + 3 * p, add(ld(s->T + 3 * p), cross(sub(pt, A), f)));
+  if (rolling_model(m)) st(s->T + 3 * p, add(ld(s->T + 3 * p), scl(delM, 1.)));
   return ORC_OK;
 }
 

@@ -187,12 +187,14 @@ int orc_sphere_box(const double* centre, double r, double cr,
                    const double* half_extent, double crb, double* point,
                    double* ovec, double* dist);
 /* Force of one contact on body 0 (body 1 receives the opposite).  kind as above;
-   body b: x, v, w (world), m, req. */
+   body b: x, v, w (world), m, req.  moment (may be NULL) receives the
+   rolling-resistance moment on body 0 (body 1 receives the opposite). */
 void orc_contact_force(int kind, const double* p, const double* point,
                        const double* ovec, double dist, const double* x0,
                        const double* v0, const double* w0, double m0,
                        double req0, const double* x1, const double* v1,
-                       const double* w1, double m1, double req1, double* f);
+                       const double* w1, double m1, double req1, double* f,
+                       double* moment);
 
 /* ---- Convex particles (boxes, polyhedra) with the Johnson GJK ---------- */
 #define ORC_SHAPE_SPHERE 0

@@ -93,7 +93,8 @@ def load():
             "orc_sphere_sphere": (i32, [vp, d, d, vp, d, d, vp, vp, P(d)]),
             "orc_sphere_box": (i32, [vp, d, d, vp, vp, vp, d, vp, vp, P(d)]),
             "orc_sphere_rectangle": (i32, [vp, d, vp, vp, vp, d, d, vp, vp, P(d)]),
-            "orc_contact_force": (None, [i32, vp, vp, vp, d, vp, vp, vp, d, d, vp, vp, vp, d, d, vp]),
+            "orc_contact_force": (None, [i32, vp, vp, vp, d, vp, vp, vp, d, d, vp, vp, vp, d, d, vp,
+                                         vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -371,13 +372,17 @@ def sphere_rectangle(centre, r, sphere_basis, basis, origin, lx, ly):
     return rc, pt, ov, d.value
 
 
-def contact_force(kind, params, point, ovec, dist, x0, v0, w0, m0, req0, x1, v1, w1, m1, req1):
+def contact_force(kind, params, point, ovec, dist, x0, v0, w0, m0, req0, x1, v1, w1, m1, req1,
+                  with_moment=False):
+    """Force on body 0; with_moment: (force, rolling-resistance moment on body 0)."""
     lib = load()
     f = np.zeros(3)
+    mo = np.zeros(3)
     lib.orc_contact_force(kind, _p(_f(params, (5,))), _p(_f(point, (3,))), _p(_f(ovec, (3,))), dist,
                           _p(_f(x0, (3,))), _p(_f(v0, (3,))), _p(_f(w0, (3,))), m0, req0,
-                          _p(_f(x1, (3,))), _p(_f(v1, (3,))), _p(_f(w1, (3,))), m1, req1, _p(f))
-    return f
+                          _p(_f(x1, (3,))), _p(_f(v1, (3,))), _p(_f(w1, (3,))), m1, req1, _p(f),
+                          _p(mo))
+    return (f, mo) if with_moment else f
 
 
 # --------------------------------------------------------------------------- convex

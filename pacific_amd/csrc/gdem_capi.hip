@@ -360,9 +360,15 @@ static int ensure_packed(gdem_ctx* c) {
 // contact models, static walls, and neither contact recording, wall loads nor
 // contact statistics (whose sums need the fixed evaluation order of the
 // canonical kernels).
+static bool any_rolling(const gdem_ctx* c) {
+  for (const Model& m : c->h_models)
+    if (model_has_rolling(m)) return true;
+  return false;
+}
+
 static bool pack_eligible(const gdem_ctx* c) {
   return !c->convex && c->n > 0 && c->skin > 0. && !c->has_memory && c->rec_cap == 0 &&
-         !c->track_loads && !c->stats_on;
+         !c->track_loads && !c->stats_on && !any_rolling(c);
 }
 
 // The next evaluation rebuilds the skin lists; the drift bound restarts.
@@ -747,6 +753,7 @@ extern "C" int gdem_set_convex_particles(gdem_ctx* c, int64_t n, const double* x
   if (n > 0 && (!x || !v || !w || !q || !basis || !shape || !density || !crust))
     return GDEM_ERR_ARG;
   if (c->has_memory) return GDEM_ERR_ARG;  // memory models: spheres only
+  if (any_rolling(c)) return GDEM_ERR_ARG;  // rolling resistance: spheres only
   for (int64_t i = 0; i < n; ++i) {
     if (shape[i] < 0 || shape[i] >= c->nshapes) return GDEM_ERR_ARG;
     if (!(density[i] > 0.) || !(crust[i] > 0.)) return GDEM_ERR_ARG;
@@ -871,6 +878,7 @@ extern "C" int gdem_set_contact_model(gdem_ctx* c, int32_t a, int32_t b, int32_t
   if (!model_params_ok(kind, p)) return GDEM_ERR_ARG;
   if (model_has_memory(kind) && c->convex) return GDEM_ERR_ARG;  // spheres only
   const Model m = make_model(kind, p);
+  if (model_has_rolling(m) && c->convex) return GDEM_ERR_ARG;  // spheres only
   c->h_models[a * kMaxMaterials + b] = m;
   c->h_models[b * kMaxMaterials + a] = m;
   bool mem = false;

@@ -115,11 +115,12 @@ struct Wall {
 __host__ __device__ __forceinline__ v3 vel_at(v3 x, v3 v, v3 w, v3 pt) { return add(v, cross(w, sub(pt, x))); }
 
 // performForcesCalculus of HookeContactForceModel (HookeContactForceModel.cpp:44-114)
-// and HertzContactForceModel (HertzContactForceModel.cpp:46-114), kr = 0.
-// Returns delFN + delFT, the force on body 0.
+// and HertzContactForceModel (HertzContactForceModel.cpp:46-114).  Returns
+// delFN + delFT, the force on body 0; delM receives the rolling-resistance
+// moment on body 0 (body 1 takes -delM), zero when kr = 0.
 __host__ __device__ __forceinline__ v3 contact_force(const Model& m, v3 pt, v3 pen, double odist, v3 x0,
                                             v3 v0, v3 w0, double m0, double req0, v3 x1, v3 v1,
-                                            v3 w1, double m1, double req1) {
+                                            v3 w1, double m1, double req1, v3& delM) {
   v3 normal = dvd(pen, norm(pen));
   normal = round3(normal);
   v3 tmpV = sub(vel_at(x0, v0, w0, pt), vel_at(x1, v1, w1, pt));
@@ -163,7 +164,29 @@ __host__ __device__ __forceinline__ v3 contact_force(const Model& m, v3 pt, v3 p
     if (nvt > kEps) tangent = dvd(neg(v_t), nvt);
     delFT = scl(tangent, fn);
   }
+  // rolling resistance moment (HookeContactForceModel.cpp:95-111,
+  // HertzContactForceModel.cpp:99-113): against the relative angular
+  // velocity, scaled by the relative rolling speed at the contact point
+  delM = mk(0., 0., 0.);
+  const double kr = m.p[4];
+  if ((m.kind == 1 || m.kind == 2) && kr != 0.) {
+    const v3 wrel = sub(w0, w1);
+    const double normwrel = norm(wrel);
+    const double normwtrel = norm(sub(cross(w0, sub(x0, pt)), cross(w1, sub(x1, pt))));
+    if (normwrel > kEps) {
+      const double Req = 1. / (1. / req0 + 1. / req1);
+      delM = scl(wrel, -(kr * Req * normFN * normwtrel / normwrel));
+    }
+  }
   return add(delFN, delFT);
+}
+
+// The force alone (callers whose models have kr = 0).
+__host__ __device__ __forceinline__ v3 contact_force(const Model& m, v3 pt, v3 pen, double odist, v3 x0,
+                                            v3 v0, v3 w0, double m0, double req0, v3 x1, v3 v1,
+                                            v3 w1, double m1, double req1) {
+  v3 delM;
+  return contact_force(m, pt, pen, odist, x0, v0, w0, m0, req0, x1, v1, w1, m1, req1, delM);
 }
 
 // ---------------------------------------------------------------- memory models

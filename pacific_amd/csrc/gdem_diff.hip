@@ -946,6 +946,7 @@ extern "C" int gdem_diff_add_system(gdem_diff* d, const gdem_batch_system* in, i
     return GDEM_ERR_ARG;
   if (in->kind != GDEM_MODEL_HOOKE || !model_params_ok(in->kind, in->pp)) return GDEM_ERR_ARG;
   if (in->pp[2] != -1.) return GDEM_ERR_ARG;  // tangential damping from en
+  if (in->pp[4] != 0.) return GDEM_ERR_ARG;  // no rolling resistance in the adjoint
   for (int k = 0; k < 5; ++k)
     if (in->pw[k] != in->pp[k]) return GDEM_ERR_ARG;  // one parameter set per system
   if (in->nwalls < 0 || in->nwalls > kMaxWalls) return GDEM_ERR_ARG;

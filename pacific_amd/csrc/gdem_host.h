@@ -128,16 +128,23 @@ inline Wall make_wall(const double* basis, const double* centre, const double* h
   return wl;
 }
 
-// Hooke / Hertz parameters the step supports: en in (0, 1], no rolling
-// resistance (kr = 0).
+// Hooke / Hertz parameters the step supports: en in (0, 1], a rolling
+// resistance kr >= 0.
 inline bool model_params_ok(int32_t kind, const double* p) {
   if (kind == GDEM_MODEL_HOOKE || kind == GDEM_MODEL_HERTZ) {
-    if (p[4] != 0.) return false;
+    if (!(p[4] >= 0.) || !std::isfinite(p[4])) return false;
     return p[1] > 0. && p[1] <= 1.;
   }
   if (kind == GDEM_MODEL_HOOKE_MEMORY) return p[1] > 0. && p[1] <= 1. && p[4] >= 0. && p[5] >= 0.;
   if (kind == GDEM_MODEL_HERTZ_MEMORY) return p[1] > 0. && p[1] <= 1. && p[0] > 0. && p[4] >= 0.;
   return false;
+}
+
+// A Hooke or Hertz model with rolling resistance (kr != 0): the sphere
+// paths that store one record per pair (packed step, differentiated
+// systems) and the convex path do not carry its moment.
+inline bool model_has_rolling(const Model& m) {
+  return (m.kind == GDEM_MODEL_HOOKE || m.kind == GDEM_MODEL_HERTZ) && m.p[4] != 0.;
 }
 
 inline bool model_has_memory(int32_t kind) {

@@ -350,10 +350,13 @@ struct Reducer {
         f = memory_force(md, ax2.id, false, c, ov, dist, A, V2, W2, vm2.w, wr2.w, B, V, W, m,
                          req, delM);
       rolling = md.p[md.kind == 3 ? 5 : 4] != 0.;
-    } else if (first) {
-      f = contact_force(md, c, ov, dist, A, V, W, m, req, B, V2, W2, vm2.w, wr2.w);
     } else {
-      f = contact_force(md, c, ov, dist, A, V2, W2, vm2.w, wr2.w, B, V, W, m, req);
+      if (first)
+        f = contact_force(md, c, ov, dist, A, V, W, m, req, B, V2, W2, vm2.w, wr2.w, delM);
+      else
+        f = contact_force(md, c, ov, dist, A, V2, W2, vm2.w, wr2.w, B, V, W, m, req, delM);
+      // addTorque(delM) / addTorque(-delM) (HookeContactForceModel.cpp:150-155)
+      rolling = md.p[4] != 0.;
     }
     if (first) {
       F = add(F, f);
@@ -507,10 +510,15 @@ __device__ __forceinline__ int force_particle(const ForceArgs& a, int t, double&
       if (a.wl_count)
         wall_load(a, t, R.ax.id, nwl, wi, c, wo, f, rolling ? neg(delM) : mk(0., 0., 0.));
     } else {
-      f = contact_force(md, c, ov, dist, R.X, R.V, R.W, R.m, R.req, wo, wv, ww, 1.e20, wl.req);
+      v3 delM;
+      f = contact_force(md, c, ov, dist, R.X, R.V, R.W, R.m, R.req, wo, wv, ww, 1.e20, wl.req,
+                        delM);
       R.F = add(R.F, f);
       R.T = add(R.T, cross(sub(c, R.X), f));
-      if (a.wl_count) wall_load(a, t, R.ax.id, nwl, wi, c, wo, f, mk(0., 0., 0.));
+      const bool rolling = md.p[4] != 0.;
+      if (rolling) R.T = add(R.T, delM);
+      if (a.wl_count)
+        wall_load(a, t, R.ax.id, nwl, wi, c, wo, f, rolling ? neg(delM) : mk(0., 0., 0.));
     }
     ++R.ncont;
     if (RECORD) R.record(R.ax.id, -1 - wi, c, ov, dist, f);
