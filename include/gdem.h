@@ -519,6 +519,19 @@ int gdem_get_counters(gdem_ctx* ctx, int64_t* force_evals, int64_t* contacts,
    counted}.  Off by default (one extra kernel per evaluation when on; a
    sphere context then steps in its canonical layout). */
 int gdem_set_contact_stats(gdem_ctx* ctx, int32_t on);
+/* Narrow phase of convex contacts, the <CollisionDetection Type="GJK"|"GJK_SV">
+   choice of the input file (Base/src/Grains.cpp:655-666, GrainsExec::
+   m_colDetGJK_SV): GDEM_GJK, the Johnson sub-algorithm (closest_points,
+   Geometry/src/Convex.cpp:536-627; the default), or GDEM_GJK_SV, the
+   signed-volume one (closest_points_GJK_SV, Geometry/src/GJK_SV.cpp:428-524),
+   both at the default tolerance EPSILON and without acceleration.  The
+   contact-error retry uses the Johnson GJK either way, as the reference's
+   ClosestPoint_ErreurHandling does (RigidBodyWithCrust.cpp:545-546).  Sphere
+   contacts are closed forms and are not affected. */
+#define GDEM_GJK 0
+#define GDEM_GJK_SV 1
+int gdem_set_narrow_phase(gdem_ctx* ctx, int32_t alg);
+int gdem_get_narrow_phase(gdem_ctx* ctx, int32_t* alg);
 int gdem_get_contact_stats(gdem_ctx* ctx, double* out);
 
 /* Event-timed totals (ms) and launch counts of two kernels since the last

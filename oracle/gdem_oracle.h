@@ -242,6 +242,8 @@ int orc_c_pair(orc_csys* s, int p0, int p1, const double* shift, double* point,
                double* ovec, double* overlap, double* force);
 int orc_c_wall_pair(orc_csys* s, int p, int wall, double* point, double* ovec,
                     double* overlap, double* force);
+/* narrow phase of the convex contacts: 0 Johnson GJK (default), 1 GJK_SV */
+int orc_c_set_narrow_phase(orc_csys* s, int sv);
 /* GJK between two shapes of this system under crust-scaled transforms:
    world witness points and distance */
 int orc_c_closest_points(const orc_csys* s, int shape_a, const double* basis_a,

@@ -1,6 +1,7 @@
 /*
  * gdem_oracle_convex.c -- CPU restatement of the Grains3D step for convex
- * particles (boxes, polyhedra) with the Johnson GJK narrow phase.
+ * particles (boxes, polyhedra) with the Johnson or the signed-volume GJK narrow
+ * phase.
  *
  * TEST INFRASTRUCTURE ONLY (see gdem_oracle.h).  Paths are relative to
  * src/Grains3D/Grains/ of the reference.  Same structure as the sphere oracle:
@@ -71,6 +72,7 @@ struct orc_csys {
   int32_t *log_id0, *log_id1;
   double *log_pt, *log_ov, *log_d, *log_f;
   int64_t ngjk;
+  int gjk_sv; /* 1: closest_points_GJK_SV, 0: Johnson closest_points */
   int err_code;
   int32_t err_id0, err_id1;
   /* time of the current force evaluation; AppCollision contact features */
@@ -354,6 +356,205 @@ static double closest_points(const shape_t* a, const shape_t* b, const xf_t* a2w
   compute_points(bits, g.det, p, q, pa, pb);
   *nbIter = numIterations;
   return dist;
+}
+
+/* ------------------------------------------------ signed-volume GJK (GJK_SV)
+   Montanari, Petrinic and Barbieri's signed-volume sub-algorithm as the
+   reference writes it (Geometry/src/GJK_SV.cpp): the simplex is reduced by
+   comparing the signs of the sub-simplex volumes (areas, lengths) with the
+   whole's, projecting onto the axis / plane of largest extent; the weights
+   are those volume ratios.  Sums keep the reference's term order. */
+static double vc(v3 a, unsigned i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+/* compareSigns (GJK_SV.cpp:9-13) */
+static unsigned same_sign(double a, double b) { return (a > 0) == (b > 0); }
+
+/* s1d (GJK_SV.cpp:18-89): segment y[i1] y[i2], i1 < i2 the two set bits */
+static void sv_s1d(const v3* y, unsigned* bits, double* lambdas) {
+  unsigned i1 = 4, i2 = 4;
+  for (unsigned i = 0; i < 4; ++i) {
+    if (*bits & (1u << i)) {
+      if (i1 == 4) i1 = i;
+      else { i2 = i; break; }
+    }
+  }
+  v3 t = sub(y[i2], y[i1]);
+  unsigned I = 0;
+  double neg_tI = -t.x;
+  if (fabs(t.y) > fabs(neg_tI)) { I = 1; neg_tI = -t.y; }
+  if (fabs(t.z) > fabs(neg_tI)) { I = 2; neg_tI = -t.z; }
+  double pI = dot(y[i2], t) / dot(t, t) * neg_tI + vc(y[i2], I);
+  double C0 = -vc(y[i2], I) + pI, C1 = vc(y[i1], I) - pI;
+  unsigned s0 = same_sign(neg_tI, C0), s1 = same_sign(neg_tI, C1);
+  if (s0 + s1 == 2) {
+    lambdas[i1] = C0 / neg_tI;
+    lambdas[i2] = C1 / neg_tI;
+  } else if (s0) {
+    *bits &= ~(1u << i2);
+    lambdas[i1] = 1.;
+  } else {
+    *bits &= ~(1u << i1);
+    lambdas[i2] = 1.;
+  }
+}
+
+/* the combination sum of lambdas[i] y[i] over the set bits, from zero */
+static v3 sv_point(unsigned bits, const v3* y, const double* lambdas) {
+  v3 r = mk(0., 0., 0.);
+  for (unsigned i = 0; i < 4; ++i)
+    if (bits & (1u << i)) r = add(r, scl(y[i], lambdas[i]));
+  return r;
+}
+
+/* s2d (GJK_SV.cpp:94-235): triangle of the three set bits */
+static void sv_s2d(const v3* y, unsigned* bits, double* lambdas) {
+  unsigned k = 0, idx[3] = {0, 0, 0};
+  for (unsigned i = 0; i < 4; ++i)
+    if (*bits & (1u << i)) idx[k++] = i;
+  const v3 a = y[idx[0]], b = y[idx[1]], c = y[idx[2]];
+  v3 n = cross(sub(b, a), sub(c, a));
+  v3 p0 = scl(n, dot(a, n) / dot(n, n));
+  /* signed area of the projection with the largest magnitude */
+  unsigned ix = 1, iy = 2;
+  double mu_max = b.y * c.z + a.y * b.z + c.y * a.z - b.y * a.z - c.y * b.z - a.y * c.z;
+  double mu = b.z * a.x + c.z * b.x + a.z * c.x - b.z * c.x - a.z * b.x - c.z * a.x;
+  if (fabs(mu) > fabs(mu_max)) { mu_max = mu; ix = 0; }
+  mu = b.x * c.y + a.x * b.y + c.x * a.y - b.x * a.y - c.x * b.y - a.x * c.y;
+  if (fabs(mu) > fabs(mu_max)) { mu_max = mu; ix = 0; iy = 1; }
+  /* areas with each vertex replaced by the projected origin: vertex k's
+     area uses the other two, (u, w) = (b, c), (c, a), (a, b) */
+  const v3 uu[3] = {b, c, a}, ww[3] = {c, a, b};
+  double C[3];
+  unsigned sg[3];
+  const double px = vc(p0, ix), py = vc(p0, iy);
+  for (int j = 0; j < 3; ++j) {
+    const double ux = vc(uu[j], ix), uy = vc(uu[j], iy), wx = vc(ww[j], ix), wy = vc(ww[j], iy);
+    C[j] = px * uy + py * wx + ux * wy - px * wy - py * ux - wx * uy;
+    sg[j] = same_sign(mu_max, C[j]);
+  }
+  if (sg[0] + sg[1] + sg[2] == 3) {
+    for (int j = 0; j < 3; ++j) lambdas[idx[j]] = C[j] / mu_max;
+    return;
+  }
+  double d = 1.e9;
+  unsigned new_bits = 0;
+  for (int j = 0; j < 3; ++j) {
+    if (sg[j]) continue;
+    unsigned used = *bits & ~(1u << idx[j]);
+    double nl[4] = {0., 0., 0., 0.};
+    sv_s1d(y, &used, nl);
+    v3 np = sv_point(used, y, nl);
+    double d_star = dot(np, np);
+    if (d_star < d) {
+      new_bits = used;
+      d = d_star;
+      for (int i = 0; i < 4; ++i) lambdas[i] = nl[i];
+    }
+  }
+  *bits = new_bits;
+}
+
+/* s3d (GJK_SV.cpp:240-325): the tetrahedron y[0..3] */
+static void sv_s3d(const v3* y, unsigned* bits, double* lambdas) {
+  const v3 a = y[0], b = y[1], c = y[2], e = y[3];
+  double C[4];
+  C[0] = e.x * c.y * b.z + c.x * b.y * e.z + b.x * e.y * c.z - b.x * c.y * e.z - c.x * e.y * b.z -
+         e.x * b.y * c.z;
+  C[1] = a.x * c.y * e.z + c.x * e.y * a.z + e.x * a.y * c.z - e.x * c.y * a.z - c.x * a.y * e.z -
+         a.x * e.y * c.z;
+  C[2] = e.x * b.y * a.z + b.x * a.y * e.z + a.x * e.y * b.z - a.x * b.y * e.z - b.x * e.y * a.z -
+         e.x * a.y * b.z;
+  C[3] = a.x * b.y * c.z + b.x * c.y * a.z + c.x * a.y * b.z - c.x * b.y * a.z - b.x * a.y * c.z -
+         a.x * c.y * b.z;
+  const double dM = C[0] + C[1] + C[2] + C[3];
+  unsigned sg[4], nsame = 0;
+  for (int j = 0; j < 4; ++j) nsame += sg[j] = same_sign(dM, C[j]);
+  if (nsame == 4) {
+    for (int i = 0; i < 4; ++i) lambdas[i] = C[i] / dM;
+    return;
+  }
+  double d = 1.e9;
+  unsigned new_bits = 0;
+  for (unsigned j = 0; j < 4; ++j) {
+    if (sg[j]) continue;
+    unsigned used = *bits & ~(1u << j);
+    double nl[4] = {0., 0., 0., 0.};
+    sv_s2d(y, &used, nl);
+    v3 np = sv_point(used, y, nl);
+    double d_star = dot(np, np);
+    if (d_star < d) {
+      new_bits = used;
+      d = d_star;
+      for (int i = 0; i < 4; ++i) lambdas[i] = nl[i];
+    }
+  }
+  *bits = new_bits;
+}
+
+/* sv_subalgorithm (GJK_SV.cpp:370-396) */
+static v3 sv_subalgorithm(const v3* y, unsigned* bits, double* lambdas) {
+  unsigned used = 0;
+  for (unsigned i = 0; i < 4; ++i) used += (*bits >> i) & 1u;
+  if (used == 1) {
+    for (unsigned i = 0; i < 4; ++i)
+      if (*bits & (1u << i)) lambdas[i] = 1.;
+  } else if (used == 2) {
+    sv_s1d(y, bits, lambdas);
+  } else if (used == 3) {
+    sv_s2d(y, bits, lambdas);
+  } else {
+    sv_s3d(y, bits, lambdas);
+  }
+  return sv_point(*bits, y, lambdas);
+}
+
+/* closest_points_GJK_SV (GJK_SV.cpp:428-524), tolerance EPSILON, no
+   acceleration.  Differs from the Johnson driver in the sub-algorithm, in the
+   degenerate test (against the current simplex only) and in the witness
+   points (the weights sum to one). */
+static double closest_points_sv(const shape_t* a, const shape_t* b, const xf_t* a2w,
+                                const xf_t* b2w, v3* pa, v3* pb, int* nbIter) {
+  unsigned bits = 0, last = 0;
+  v3 p[4], q[4], y[4];
+  double lambdas[4] = {0., 0., 0., 0.};
+  memset(p, 0, sizeof(p));
+  memset(q, 0, sizeof(q));
+  memset(y, 0, sizeof(y));
+  double mu = 0.;
+  int numIterations = 0;
+  const double relError = ORC_EPSILON;
+  const double absError = 1.e-4 * relError;
+  v3 zero = mk(0., 0., 0.);
+  v3 v = sub(xf_apply(a2w, support(a, zero)), xf_apply(b2w, support(b, zero)));
+  v3 w = v;
+  double dist = norm(v);
+  while (bits < 15 && dist > 1.0e-15 && numIterations < 1000) {
+    ++numIterations;
+    for (last = 0; last < 4; ++last)
+      if (!(bits & (1u << last))) break;
+    v3 d = v;
+    p[last] = support(a, vec_mat(neg(d), a2w->b));
+    q[last] = support(b, vec_mat(d, b2w->b));
+    w = sub(xf_apply(a2w, p[last]), xf_apply(b2w, q[last]));
+    mu = dist - dot(v, w) / dist;
+    if (mu < dist * relError || mu < absError) break;
+    if (degenerate(bits, y, w)) break;
+    y[last] = w;
+    bits |= 1u << last;
+    v = sv_subalgorithm(y, &bits, lambdas);
+    dist = norm(v);
+  }
+  /* computePoints (GJK_SV.cpp:348-365) */
+  *pa = sv_point(bits, p, lambdas);
+  *pb = sv_point(bits, q, lambdas);
+  *nbIter = numIterations;
+  return dist;
+}
+
+static double gjk_distance(int sv, const shape_t* a, const shape_t* b, const xf_t* a2w,
+                           const xf_t* b2w, v3* pa, v3* pb, int* nbIter) {
+  return sv ? closest_points_sv(a, b, a2w, b2w, pa, pb, nbIter)
+            : closest_points(a, b, a2w, b2w, pa, pb, nbIter);
 }
 
 /* ------------------------------------------------------------- shapes */
@@ -661,8 +862,8 @@ typedef struct {
    the midpoint of the witness points, overlap vector (pA - pB)/d, rounded and
    scaled by (crustA + crustB - d).  Returns 1 contact candidate, 0 none,
    -1 ContactError (shrunk bodies overlap). */
-static int closest_point(const body_t* A, const body_t* B, v3* contact, v3* ov, double* dist,
-                         int64_t* ngjk, int* nbIter) {
+static int closest_point(int sv, const body_t* A, const body_t* B, v3* contact, v3* ov,
+                         double* dist, int64_t* ngjk, int* nbIter) {
   *nbIter = 0;
   v3 gab = sub(ld(A->origin), ld(B->origin));
   if (!(norm(gab) < A->circ + B->circ)) return 0;
@@ -672,7 +873,7 @@ static int closest_point(const body_t* A, const body_t* B, v3* contact, v3* ov, 
   crust_xf(B->basis, B->origin, B->scale, &b2w);
   v3 pa, pb;
   int nit = 0;
-  double distance = closest_points(A->sh, B->sh, &a2w, &b2w, &pa, &pb, &nit);
+  double distance = gjk_distance(sv, A->sh, B->sh, &a2w, &b2w, &pa, &pb, &nit);
   *nbIter = nit;
   if (distance < ORC_EPSILON) return -1;
   pa = xf_apply(&a2w, pa);
@@ -701,7 +902,8 @@ static double retry_scale(double extent, double crust) {
    between the bodies shrunk by ten crusts; contact at the midpoint of the
    witness points, overlap vector the normalised B -> A direction times the
    crust sum, overlap distance minus the crust sum.  -1: the shrunk bodies
-   still overlap (the error is raised). */
+   still overlap (the error is raised).  It calls the Johnson closest_points
+   whichever narrow phase is selected (:545-546). */
 static int closest_point_retry(const body_t* A, const body_t* B, v3* contact, v3* ov,
                                double* dist, int64_t* ngjk, int* nbIter) {
   double sa[3], sb[3];
@@ -729,6 +931,13 @@ static int closest_point_retry(const body_t* A, const body_t* B, v3* contact, v3
   return 1;
 }
 
+/* CollisionDetection Type="GJK" / "GJK_SV" (Base/src/Grains.cpp:660-666) */
+int orc_c_set_narrow_phase(orc_csys* s, int sv) {
+  if (sv != 0 && sv != 1) return ORC_ERR_ARG;
+  s->gjk_sv = sv;
+  return ORC_OK;
+}
+
 int orc_c_closest_points(const orc_csys* s, int shape_a, const double* basis_a,
                          const double* origin_a, const double* scale_a, int shape_b,
                          const double* basis_b, const double* origin_b, const double* scale_b,
@@ -737,7 +946,8 @@ int orc_c_closest_points(const orc_csys* s, int shape_a, const double* basis_a,
   crust_xf(basis_a, origin_a, scale_a, &a2w);
   crust_xf(basis_b, origin_b, scale_b, &b2w);
   v3 A, B;
-  *dist = closest_points(&s->shapes[shape_a], &s->shapes[shape_b], &a2w, &b2w, &A, &B, nbIter);
+  *dist = gjk_distance(s->gjk_sv, &s->shapes[shape_a], &s->shapes[shape_b], &a2w, &b2w, &A, &B,
+                       nbIter);
   st(pa, xf_apply(&a2w, A));
   st(pb, xf_apply(&b2w, B));
   return ORC_OK;
@@ -863,7 +1073,7 @@ static int cinteract(orc_csys* s, int p0, int p1, v3 shift) {
   v3 c, ov;
   double dist;
   int nit = 0;
-  int rc = closest_point(&A, &B, &c, &ov, &dist, &s->ngjk, &nit);
+  int rc = closest_point(s->gjk_sv, &A, &B, &c, &ov, &dist, &s->ngjk, &nit);
   if (rc < 0) rc = closest_point_retry(&A, &B, &c, &ov, &dist, &s->ngjk, &nit);
   if (rc < 0) return cset_error(s, ORC_ERR_CONTACT, p0, p1);
   if (rc == 0) return ORC_OK;
@@ -893,7 +1103,7 @@ int orc_c_pair(orc_csys* s, int p0, int p1, const double* shift, double* point, 
   double dist;
   int64_t ng = 0;
   int nit = 0;
-  int rc = closest_point(&A, &B, &c, &ov, &dist, &ng, &nit);
+  int rc = closest_point(s->gjk_sv, &A, &B, &c, &ov, &dist, &ng, &nit);
   if (rc < 0) rc = closest_point_retry(&A, &B, &c, &ov, &dist, &ng, &nit);
   if (rc <= 0) return rc;
   st(point, c);
@@ -935,7 +1145,7 @@ static int cinteract_wall(orc_csys* s, int p, int wi) {
   double dist;
   int nit = 0;
   int rc = wl->rect ? crect(&A, wl, &c, &ov, &dist)
-                    : closest_point(&A, &B, &c, &ov, &dist, &s->ngjk, &nit);
+                    : closest_point(s->gjk_sv, &A, &B, &c, &ov, &dist, &s->ngjk, &nit);
   if (rc < 0) rc = closest_point_retry(&A, &B, &c, &ov, &dist, &s->ngjk, &nit);
   if (rc < 0) return cset_error(s, ORC_ERR_CONTACT, p, -1 - wi);
   if (rc == 0) return ORC_OK;
@@ -974,7 +1184,8 @@ int orc_c_wall_pair(orc_csys* s, int p, int wi, double* point, double* ovec, dou
   double dist;
   int64_t ng = 0;
   int nit = 0;
-  int rc = wl->rect ? crect(&A, wl, &c, &ov, &dist) : closest_point(&A, &B, &c, &ov, &dist, &ng, &nit);
+  int rc = wl->rect ? crect(&A, wl, &c, &ov, &dist)
+                     : closest_point(s->gjk_sv, &A, &B, &c, &ov, &dist, &ng, &nit);
   if (rc < 0) rc = closest_point_retry(&A, &B, &c, &ov, &dist, &ng, &nit);
   if (rc <= 0) return rc;
   st(point, c);

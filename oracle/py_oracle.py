@@ -414,6 +414,7 @@ def _load_convex(lib):
         "orc_c_gjk_calls": (C.c_int64, [vp]),
         "orc_c_get_contact_stats": (None, [vp, vp]),
         "orc_c_closest_points": (i32, [vp, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, P(i32), P(d)]),
+        "orc_c_set_narrow_phase": (i32, [vp, i32]),
         "orc_c_pair": (i32, [vp, i32, i32, vp, vp, vp, P(d), vp]),
         "orc_c_wall_pair": (i32, [vp, i32, i32, vp, vp, P(d), vp]),
     }
@@ -443,6 +444,8 @@ class ConvexOracle:
         if rc:
             raise OracleError(rc, "orc_c_create", -1, -1)
         self.h = h
+        # CollisionDetection Type (Base/src/Grains.cpp:660-666): "GJK" or "GJK_SV"
+        self.set_narrow_phase(wl.get("narrow_phase", "GJK"))
         for sh in wl["shapes"]:
             self.add_shape(sh)
         for (a, b, kind, p) in wl["models"]:
@@ -522,6 +525,11 @@ class ConvexOracle:
         inertia, bb = np.zeros(6), np.zeros(3)
         self.lib.orc_c_shape_info(self.h, k, C.byref(vol), _p(inertia), C.byref(circ), _p(bb))
         return vol.value, inertia, circ.value, bb
+
+    def set_narrow_phase(self, name):
+        if name not in ("GJK", "GJK_SV"):
+            raise ValueError(f"unknown narrow phase {name!r}")
+        self._chk(self.lib.orc_c_set_narrow_phase(self.h, int(name == "GJK_SV")), "narrow phase")
 
     def closest_points(self, sa, basis_a, origin_a, scale_a, sb, basis_b, origin_b, scale_b):
         pa, pb = np.zeros(3), np.zeros(3)

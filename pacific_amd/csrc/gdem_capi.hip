@@ -64,6 +64,7 @@ struct gdem_ctx {
   int32_t nblk_slots = 0;
   // contact statistics (AppCollision::addToContactsFeatures), when switched on
   bool stats_on = false;
+  int32_t gjk_sv = 0;  // convex narrow phase: 0 Johnson GJK, 1 signed-volume GJK
   ContactStats* d_stats = nullptr;
   StatPart* d_stat_part = nullptr;
   bool primed = false, binned = false;
@@ -970,6 +971,7 @@ static void launch_convex_step(gdem_ctx* c, int mode, double dtv, double dtd, bo
   g.in_count = c->d_in_count;
   g.err = c->d_err;
   g.stats = c->stats_on ? 1 : 0;
+  g.gjk_sv = c->gjk_sv;
   CForceArgs a{};
   a.in = c->buf[c->cur];
   a.out = c->buf[1 - c->cur];
@@ -2120,6 +2122,19 @@ extern "C" int gdem_set_contact_stats(gdem_ctx* c, int32_t on) {
   }
   c->stats_on = on != 0;
   c->graph_gen++;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_set_narrow_phase(gdem_ctx* c, int32_t alg) {
+  if (!c || (alg != GDEM_GJK && alg != GDEM_GJK_SV)) return GDEM_ERR_ARG;
+  c->gjk_sv = alg == GDEM_GJK_SV ? 1 : 0;
+  c->graph_gen++;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_get_narrow_phase(gdem_ctx* c, int32_t* alg) {
+  if (!c || !alg) return GDEM_ERR_ARG;
+  *alg = c->gjk_sv ? GDEM_GJK_SV : GDEM_GJK;
   return GDEM_OK;
 }
 

@@ -1,5 +1,6 @@
-// gdem_convex.h -- convex shapes, crust-shrunk transforms and the Johnson GJK
-// distance (FP64, reference operand order) for device and host.
+// gdem_convex.h -- convex shapes, crust-shrunk transforms and the Johnson and
+// signed-volume GJK distances (FP64, reference operand order) for device and
+// host.
 //
 // Paths are relative to src/Grains3D/Grains/.  Polyhedron::support hill-climbs
 // from a vertex it keeps between calls (Polyhedron.cpp:265-300); here each call
@@ -361,6 +362,235 @@ __host__ __device__ inline double closest_points(const CShape& a, const CShape& 
   return dist;
 }
 
+// ------------------------------------------------------------ signed-volume GJK
+// closest_points_GJK_SV (Geometry/src/GJK_SV.cpp:428-524): Montanari,
+// Petrinic and Barbieri's signed-volume sub-algorithm.  The simplex is reduced
+// by comparing the signs of the sub-simplex volumes (areas, lengths) with the
+// whole's, on the axis / plane where the simplex's projection is largest; the
+// weights are those volume ratios.  Vertex slots are compile-time template
+// arguments (the sub-simplices of a simplex are known at compile time; a switch
+// on the simplex mask picks the instantiation), so y, the weights and the
+// support points stay in registers.  Terms are summed in the reference's order.
+
+__host__ __device__ inline bool sv_same_sign(double a, double b) { return (a > 0) == (b > 0); }
+
+__host__ __device__ inline double sv_comp(v3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+// sum of lam[i] y[i] over the set bits of `bits` (among those of M), from zero
+template <unsigned M>
+__host__ __device__ inline v3 sv_point(unsigned bits, const v3* y, const double* lam) {
+  v3 r = mk(0., 0., 0.);
+  gjk_for<0, 4, 1>([&](auto I) {
+    constexpr unsigned i = decltype(I)::value;
+    if constexpr ((M & (1u << i)) != 0)
+      if (bits & (1u << i)) r = add(r, scl(y[i], lam[i]));
+  });
+  return r;
+}
+
+// s1d (GJK_SV.cpp:18-89): the segment I1 I2, I1 < I2
+template <unsigned I1, unsigned I2>
+__host__ __device__ inline void sv_s1d(const v3* y, unsigned& bits, double* lam) {
+  const v3 t = sub(y[I2], y[I1]);
+  int I = 0;
+  double neg_tI = -t.x;
+  if (fabs(t.y) > fabs(neg_tI)) { I = 1; neg_tI = -t.y; }
+  if (fabs(t.z) > fabs(neg_tI)) { I = 2; neg_tI = -t.z; }
+  const double pI = dot(y[I2], t) / dot(t, t) * neg_tI + sv_comp(y[I2], I);
+  const double C0 = -sv_comp(y[I2], I) + pI, C1 = sv_comp(y[I1], I) - pI;
+  const bool s0 = sv_same_sign(neg_tI, C0), s1 = sv_same_sign(neg_tI, C1);
+  if (s0 && s1) {
+    lam[I1] = C0 / neg_tI;
+    lam[I2] = C1 / neg_tI;
+  } else if (s0) {
+    bits &= ~(1u << I2);
+    lam[I1] = 1.;
+  } else {
+    bits &= ~(1u << I1);
+    lam[I2] = 1.;
+  }
+}
+
+// One candidate face of the fallback of s2d / s3d: the sub-simplex M's
+// closest point, kept if nearer than the best so far.
+template <unsigned M, class Sub>
+__host__ __device__ inline void sv_try(const v3* y, Sub sub_alg, double& d, unsigned& new_bits,
+                                       double* lam) {
+  unsigned used = M;
+  double nl[4] = {0., 0., 0., 0.};
+  sub_alg(used, nl);
+  const v3 np = sv_point<M>(used, y, nl);
+  const double d_star = dot(np, np);
+  if (d_star < d) {
+    new_bits = used;
+    d = d_star;
+    gjk_for<0, 4, 1>([&](auto I) { lam[decltype(I)::value] = nl[decltype(I)::value]; });
+  }
+}
+
+// s2d (GJK_SV.cpp:94-235): the triangle P0 P1 P2, P0 < P1 < P2
+template <unsigned P0, unsigned P1, unsigned P2>
+__host__ __device__ inline void sv_s2d(const v3* y, unsigned& bits, double* lam) {
+  const v3 a = y[P0], b = y[P1], c = y[P2];
+  const v3 n = cross(sub(b, a), sub(c, a));
+  const v3 p0 = scl(n, dot(a, n) / dot(n, n));
+  // the projection plane of largest signed area
+  int ix = 1, iy = 2;
+  double mu_max = b.y * c.z + a.y * b.z + c.y * a.z - b.y * a.z - c.y * b.z - a.y * c.z;
+  double mu = b.z * a.x + c.z * b.x + a.z * c.x - b.z * c.x - a.z * b.x - c.z * a.x;
+  if (fabs(mu) > fabs(mu_max)) { mu_max = mu; ix = 0; }
+  mu = b.x * c.y + a.x * b.y + c.x * a.y - b.x * a.y - c.x * b.y - a.x * c.y;
+  if (fabs(mu) > fabs(mu_max)) { mu_max = mu; ix = 0; iy = 1; }
+  // the areas with one vertex replaced by the projected origin
+  const double px = sv_comp(p0, ix), py = sv_comp(p0, iy);
+  auto area = [&](v3 u, v3 w) {
+    const double ux = sv_comp(u, ix), uy = sv_comp(u, iy), wx = sv_comp(w, ix), wy = sv_comp(w, iy);
+    return px * uy + py * wx + ux * wy - px * wy - py * ux - wx * uy;
+  };
+  const double C0 = area(b, c), C1 = area(c, a), C2 = area(a, b);
+  const bool s0 = sv_same_sign(mu_max, C0), s1 = sv_same_sign(mu_max, C1),
+             s2 = sv_same_sign(mu_max, C2);
+  if (s0 && s1 && s2) {
+    lam[P0] = C0 / mu_max;
+    lam[P1] = C1 / mu_max;
+    lam[P2] = C2 / mu_max;
+    return;
+  }
+  double d = 1.e9;
+  unsigned new_bits = 0;
+  if (!s0)
+    sv_try<(1u << P1) | (1u << P2)>(
+        y, [&](unsigned& u, double* l) { sv_s1d<P1, P2>(y, u, l); }, d, new_bits, lam);
+  if (!s1)
+    sv_try<(1u << P0) | (1u << P2)>(
+        y, [&](unsigned& u, double* l) { sv_s1d<P0, P2>(y, u, l); }, d, new_bits, lam);
+  if (!s2)
+    sv_try<(1u << P0) | (1u << P1)>(
+        y, [&](unsigned& u, double* l) { sv_s1d<P0, P1>(y, u, l); }, d, new_bits, lam);
+  bits = new_bits;
+}
+
+// s3d (GJK_SV.cpp:240-325): the tetrahedron y[0..3]
+__host__ __device__ inline void sv_s3d(const v3* y, unsigned& bits, double* lam) {
+  const v3 a = y[0], b = y[1], c = y[2], e = y[3];
+  const double C0 = e.x * c.y * b.z + c.x * b.y * e.z + b.x * e.y * c.z - b.x * c.y * e.z -
+                    c.x * e.y * b.z - e.x * b.y * c.z;
+  const double C1 = a.x * c.y * e.z + c.x * e.y * a.z + e.x * a.y * c.z - e.x * c.y * a.z -
+                    c.x * a.y * e.z - a.x * e.y * c.z;
+  const double C2 = e.x * b.y * a.z + b.x * a.y * e.z + a.x * e.y * b.z - a.x * b.y * e.z -
+                    b.x * e.y * a.z - e.x * a.y * b.z;
+  const double C3 = a.x * b.y * c.z + b.x * c.y * a.z + c.x * a.y * b.z - c.x * b.y * a.z -
+                    b.x * a.y * c.z - a.x * c.y * b.z;
+  const double dM = C0 + C1 + C2 + C3;
+  const bool s0 = sv_same_sign(dM, C0), s1 = sv_same_sign(dM, C1), s2 = sv_same_sign(dM, C2),
+             s3 = sv_same_sign(dM, C3);
+  if (s0 && s1 && s2 && s3) {
+    lam[0] = C0 / dM;
+    lam[1] = C1 / dM;
+    lam[2] = C2 / dM;
+    lam[3] = C3 / dM;
+    return;
+  }
+  double d = 1.e9;
+  unsigned new_bits = 0;
+  if (!s0) sv_try<14>(y, [&](unsigned& u, double* l) { sv_s2d<1, 2, 3>(y, u, l); }, d, new_bits, lam);
+  if (!s1) sv_try<13>(y, [&](unsigned& u, double* l) { sv_s2d<0, 2, 3>(y, u, l); }, d, new_bits, lam);
+  if (!s2) sv_try<11>(y, [&](unsigned& u, double* l) { sv_s2d<0, 1, 3>(y, u, l); }, d, new_bits, lam);
+  if (!s3) sv_try<7>(y, [&](unsigned& u, double* l) { sv_s2d<0, 1, 2>(y, u, l); }, d, new_bits, lam);
+  bits = new_bits;
+}
+
+// sv_subalgorithm (GJK_SV.cpp:370-396): bits holds the simplex with its new
+// vertex; returns the closest point of the reduced simplex.
+__host__ __device__ inline v3 sv_subalgorithm(const v3* y, unsigned& bits, double* lam) {
+  switch (bits) {
+    case 1: lam[0] = 1.; break;
+    case 2: lam[1] = 1.; break;
+    case 4: lam[2] = 1.; break;
+    case 8: lam[3] = 1.; break;
+    case 3: sv_s1d<0, 1>(y, bits, lam); break;
+    case 5: sv_s1d<0, 2>(y, bits, lam); break;
+    case 6: sv_s1d<1, 2>(y, bits, lam); break;
+    case 9: sv_s1d<0, 3>(y, bits, lam); break;
+    case 10: sv_s1d<1, 3>(y, bits, lam); break;
+    case 12: sv_s1d<2, 3>(y, bits, lam); break;
+    case 7: sv_s2d<0, 1, 2>(y, bits, lam); break;
+    case 11: sv_s2d<0, 1, 3>(y, bits, lam); break;
+    case 13: sv_s2d<0, 2, 3>(y, bits, lam); break;
+    case 14: sv_s2d<1, 2, 3>(y, bits, lam); break;
+    default: sv_s3d(y, bits, lam); break;
+  }
+  return sv_point<15>(bits, y, lam);
+}
+
+template <unsigned LAST>
+__host__ __device__ inline void sv_put(v3* y, v3* p, v3* q, v3 w, v3 pl, v3 ql) {
+  y[LAST] = w;
+  p[LAST] = pl;
+  q[LAST] = ql;
+}
+
+// closest_points_GJK_SV (GJK_SV.cpp:428-524): tolerance EPSILON, no
+// acceleration.  Against the Johnson driver: the degenerate test looks at the
+// current simplex only, and the witness points are the weighted sums (the
+// weights already sum to one).
+__host__ __device__ inline double closest_points_sv(const CShape& a, const CShape& b,
+                                                    const Xf& a2w, const Xf& b2w, v3& pa, v3& pb,
+                                                    int& nbIter) {
+  unsigned bits = 0;
+  v3 p[4], q[4], y[4];
+  double lam[4];
+  gjk_for<0, 4, 1>([&](auto I) {
+    constexpr unsigned i = decltype(I)::value;
+    p[i] = q[i] = y[i] = mk(0., 0., 0.);
+    lam[i] = 0.;
+  });
+  double mu = 0.;
+  int numIterations = 0;
+  const double relError = kEps;
+  const double absError = 1.e-4 * relError;
+  const v3 zero = mk(0., 0., 0.);
+  v3 v = sub(xf_apply(a2w, support(a, zero)), xf_apply(b2w, support(b, zero)));
+  v3 w = v;
+  double dist = norm(v);
+  while (bits < 15 && dist > 1.0e-15 && numIterations < 1000) {
+    ++numIterations;
+    const unsigned last = (unsigned)__builtin_ctz(~bits);  // the lowest free slot
+    const v3 d = v;
+    const v3 pl = support(a, vec_mat(neg(d), a2w.b));
+    const v3 ql = support(b, vec_mat(d, b2w.b));
+    w = sub(xf_apply(a2w, pl), xf_apply(b2w, ql));
+    mu = dist - dot(v, w) / dist;
+    if (mu < dist * relError || mu < absError) break;
+    if (gjk_degenerate(bits, y, w)) break;
+    switch (last) {
+      case 0: sv_put<0>(y, p, q, w, pl, ql); break;
+      case 1: sv_put<1>(y, p, q, w, pl, ql); break;
+      case 2: sv_put<2>(y, p, q, w, pl, ql); break;
+      default: sv_put<3>(y, p, q, w, pl, ql); break;
+    }
+    bits |= 1u << last;
+    v = sv_subalgorithm(y, bits, lam);
+    dist = norm(v);
+  }
+  // computePoints (GJK_SV.cpp:348-365)
+  pa = sv_point<15>(bits, p, lam);
+  pb = sv_point<15>(bits, q, lam);
+  nbIter = numIterations;
+  return dist;
+}
+
+// The narrow phase selected by CollisionDetection Type (Base/src/Grains.cpp:
+// 660-666): Johnson "GJK" (the default) or "GJK_SV".
+template <bool SV>
+__host__ __device__ inline double gjk_distance(const CShape& a, const CShape& b, const Xf& a2w,
+                                               const Xf& b2w, v3& pa, v3& pb, int& nbIter) {
+  if constexpr (SV)
+    return closest_points_sv(a, b, a2w, b2w, pa, pb, nbIter);
+  else
+    return closest_points(a, b, a2w, b2w, pa, pb, nbIter);
+}
+
 // ------------------------------------------------------------ contact-error retry
 // RigidBodyWithCrust::ClosestPoint_ErreurHandling (Geometry/src/
 // RigidBodyWithCrust.cpp:534-588), reached when an overlap reaches the sum of
@@ -461,9 +691,11 @@ struct CBody {
   double crust, circ;
 };
 
-// RigidBodyWithCrust::ClosestPoint general case (RigidBodyWithCrust.cpp:274-348).
+// RigidBodyWithCrust::ClosestPoint general case (RigidBodyWithCrust.cpp:274-348),
+// with the Johnson (SV false) or signed-volume (SV true) GJK (:305-310).
 // 1 = candidate (dist may be >= 0), 0 = no bounding-volume overlap,
 // -1 = ContactError (shrunk bodies overlap).
+template <bool SV>
 __host__ __device__ inline int closest_point(const CBody& A, const CBody& B, v3& contact, v3& ov,
                                              double& dist, int& nbIter) {
   nbIter = 0;
@@ -473,7 +705,7 @@ __host__ __device__ inline int closest_point(const CBody& A, const CBody& B, v3&
   const Xf b2w = crust_xf(B.basis, B.origin, B.scale);
   v3 pa, pb;
   int nit = 0;
-  double distance = closest_points(*A.sh, *B.sh, a2w, b2w, pa, pb, nit);
+  double distance = gjk_distance<SV>(*A.sh, *B.sh, a2w, b2w, pa, pb, nit);
   nbIter = nit;
   if (distance < kEps) return -1;
   pa = xf_apply(a2w, pa);
@@ -494,7 +726,9 @@ __host__ __device__ inline int closest_point(const CBody& A, const CBody& B, v3&
 // (10, 0.5), :770-795: per bounding-box axis (extent - 10 crust) / extent, at
 // least 0.5); contact at the midpoint of the witness points, overlap vector
 // the unit direction B -> A times the crust sum, overlap minus the crust sum.
-// -1: the shrunk bodies still overlap (the second-level ContactError).
+// -1: the shrunk bodies still overlap (the second-level ContactError).  The
+// reference calls the Johnson GJK here whichever narrow phase is selected
+// (:545-546).
 __host__ __device__ inline int closest_point_retry(const CBody& A, const CBody& B, v3& contact,
                                                    v3& ov, double& dist, int& nbIter) {
   double sa[3], sb[3];

@@ -154,6 +154,9 @@ __global__ void __launch_bounds__(kCBlock) k_cdetect(CDetectArgs a) {
 // Narrow phase of one candidate: RigidBodyWithCrust::ClosestPoint
 // (Component/src/RigidBodyWithCrust.cpp:274-348) between the crust-shrunk
 // bodies, first body A at its position, second body B moved to A's image.
+// SV: the signed-volume GJK (CollisionDetection Type="GJK_SV") instead of
+// the Johnson one.
+template <bool SV>
 __global__ void __launch_bounds__(kCBlock) k_cgjk(CGjkArgs a) {
   const int nw = *a.nwork;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nw; k += gridDim.x * blockDim.x) {
@@ -177,7 +180,7 @@ __global__ void __launch_bounds__(kCBlock) k_cgjk(CGjkArgs a) {
       double dist = 0.;
       int nit = 0;  // ClosestPointRECTANGLE reports none
       int rc = wl.kind == WALL_RECTANGLE ? convex_rect(A, wl, c, ov, dist)
-                                         : closest_point(A, B, c, ov, dist, nit);
+                                         : closest_point<SV>(A, B, c, ov, dist, nit);
       if (rc < 0) rc = closest_point_retry(A, B, c, ov, dist, nit);
       if (rc < 0) {
         atomicMin(&a.err[kErrContact], pack_ids(ax1.id, -1 - u));
@@ -217,7 +220,7 @@ __global__ void __launch_bounds__(kCBlock) k_cgjk(CGjkArgs a) {
     v3 c, ov;
     double dist = 0.;
     int nit = 0;
-    int rc = closest_point(A, B, c, ov, dist, nit);
+    int rc = closest_point<SV>(A, B, c, ov, dist, nit);
     if (rc < 0) rc = closest_point_retry(A, B, c, ov, dist, nit);
     if (rc < 0) {
       atomicMin(&a.err[kErrContact], first ? pack_ids(ax1.id, ax2.id) : pack_ids(ax2.id, ax1.id));
@@ -586,7 +589,10 @@ void launch_cgjk(const CGjkArgs& a, int32_t max_work, hipStream_t st) {
   if (max_work <= 0) return;
   // grid-stride over the queue; the grid covers a typical queue in one pass
   const int blocks = std::min(std::max(cblocks(max_work), 1), 16384);
-  hipLaunchKernelGGL(k_cgjk, dim3(blocks), dim3(kCBlock), 0, st, a);
+  if (a.gjk_sv)
+    hipLaunchKernelGGL(k_cgjk<true>, dim3(blocks), dim3(kCBlock), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_cgjk<false>, dim3(blocks), dim3(kCBlock), 0, st, a);
 }
 
 void launch_cforce(int mode, bool record, const CForceArgs& a, hipStream_t st) {

@@ -299,6 +299,7 @@ struct CGjkArgs {
   int32_t* in_count;   // [n], zero on entry
   unsigned long long* err;
   int32_t stats;       // record dist and GJK iterations of every evaluated candidate
+  int32_t gjk_sv;      // 1: signed-volume GJK, 0: Johnson
 };
 
 struct CForceArgs {

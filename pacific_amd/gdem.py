@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libgdem.so")
 
 OK, ERR_CONTACT, ERR_MOTION, ERR_SIMULATION, ERR_HIP, ERR_ARG, ERR_CAPACITY = range(7)
 MODEL_HOOKE, MODEL_HERTZ = 1, 2
+NARROW_PHASES = {"GJK": 0, "GJK_SV": 1}  # GDEM_GJK, GDEM_GJK_SV
 MODEL_HOOKE_MEMORY, MODEL_HERTZ_MEMORY = 3, 4
 # parameters per model kind (include/gdem.h)
 MODEL_NPARAMS = {MODEL_HOOKE: 5, MODEL_HERTZ: 5, MODEL_HOOKE_MEMORY: 8, MODEL_HERTZ_MEMORY: 7}
@@ -141,6 +142,8 @@ def load_library(path: str = LIB_PATH):
         "gdem_get_kernel_times": (C.c_int, [vp, P(d), P(i64)]),
         "gdem_set_contact_stats": (C.c_int, [vp, i32]),
         "gdem_get_contact_stats": (C.c_int, [vp, P(d)]),
+        "gdem_set_narrow_phase": (C.c_int, [vp, i32]),
+        "gdem_get_narrow_phase": (C.c_int, [vp, P(i32)]),
         "gdem_get_grid": (C.c_int, [vp, P(i32), P(d)]),
         "gdem_get_particles": (C.c_int, [vp, vp, vp, vp, vp]),
         "gdem_get_contact_model": (C.c_int, [vp, i32, i32, P(i32), vp]),
@@ -211,6 +214,7 @@ EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall"
             "gdem_get_contacts", "gdem_last_error", "gdem_set_kernel_timing",
             "gdem_get_counters", "gdem_reset_counters", "gdem_get_kernel_times",
             "gdem_set_contact_stats", "gdem_get_contact_stats", "gdem_get_grid",
+            "gdem_set_narrow_phase", "gdem_get_narrow_phase",
             "gdem_get_particles", "gdem_get_contact_model", "gdem_get_wall_shape",
             "gdem_set_material_name", "gdem_get_material_name", "gdem_set_wall_name",
             "gdem_get_wall_name", "gdem_write_reload", "gdem_read_reload",
@@ -617,6 +621,20 @@ class GdemSystem:
         self._check(self.lib.gdem_set_contact_stats(self.h, int(bool(on))),
                     "gdem_set_contact_stats")
 
+    def set_narrow_phase(self, name):
+        """Narrow phase of convex contacts: "GJK" (Johnson, the default) or
+        "GJK_SV" (signed volumes), as CollisionDetection Type in the
+        reference's input file."""
+        if name not in NARROW_PHASES:
+            raise ValueError(f"unknown narrow phase {name!r}")
+        self._check(self.lib.gdem_set_narrow_phase(self.h, NARROW_PHASES[name]),
+                    "gdem_set_narrow_phase")
+
+    def narrow_phase(self):
+        alg = C.c_int32()
+        self._check(self.lib.gdem_get_narrow_phase(self.h, C.byref(alg)), "gdem_get_narrow_phase")
+        return {v: k for k, v in NARROW_PHASES.items()}[alg.value]
+
     def contact_stats(self):
         """The run's contact features, as the oracle's contact_stats()."""
         out = (C.c_double * 6)()
@@ -682,6 +700,8 @@ def build_system(wl, device=0):
     for rdef in wl.get("rectangles", []):
         s.add_rectangle_wall(*rdef)
     p = wl["particles"]
+    if "narrow_phase" in wl:
+        s.set_narrow_phase(wl["narrow_phase"])
     if "shapes" in wl:
         for sh in wl["shapes"]:
             if sh[0] == "box":

@@ -1,5 +1,5 @@
-"""Parity of the HIP convex path (boxes + tetrahedra, GJK narrow phase) with
-the CPU oracle, through the C ABI.
+"""Parity of the HIP convex path (boxes + tetrahedra, GJK narrow phase, Johnson
+or signed-volume: the "_sv" cases) with the CPU oracle, through the C ABI.
 
 Bar as for spheres: pair sets bit-exact; each contact's point, overlap
 vector, overlap and force bit-identical to the oracle's per-pair evaluation
@@ -64,6 +64,11 @@ def walls_to_rectangles(wl):
 
 
 def case(name):
+    if name.endswith("_sv"):
+        # the signed-volume narrow phase (CollisionDetection Type="GJK_SV")
+        wl = case(name[:-3])
+        wl["narrow_phase"] = "GJK_SV"
+        return wl
     if name == "convex_rectangles":
         return walls_to_rectangles(case("convex_walls"))
     if name == "convex_walls":
@@ -75,7 +80,8 @@ def case(name):
     raise KeyError(name)
 
 
-CASES = ["convex_walls", "convex_dense", "convex_rectangles"]
+CASES = ["convex_walls", "convex_dense", "convex_rectangles", "convex_walls_sv",
+         "convex_dense_sv"]
 
 
 def _masses(orc, wl):
@@ -143,6 +149,31 @@ def test_convex_trajectory_100_steps(oracle_mod, name):
     assert rel_err(gb, o["basis"]) <= 1e-9
     assert pair_set(sysg.get_contacts()) == pair_set(orc.contacts())
     sysg.close()
+
+
+def test_signed_volume_narrow_phase_is_the_one_run():
+    """The GJK_SV switch reaches the kernel: its contacts differ from the
+    Johnson GJK's in the last bits (same pairs, different rounding), and the
+    context reports the choice."""
+    print("[case] convex narrow phase switch", flush=True)
+    out = {}
+    for name in ("convex_dense", "convex_dense_sv"):
+        wl = case(name)
+        s = gdem.build_system(wl)
+        assert s.narrow_phase() == ("GJK_SV" if name.endswith("_sv") else "GJK")
+        s.set_contact_recording(32)
+        s.prime()
+        out[name] = s.get_contacts()
+        s.close()
+    a, b = out["convex_dense"], out["convex_dense_sv"]
+    assert pair_set(a) == pair_set(b)
+    ka = {(int(i), int(j)): k for k, (i, j) in enumerate(zip(a["id0"], a["id1"]))}
+    differ = 0
+    for k, (i, j) in enumerate(zip(b["id0"], b["id1"])):
+        m = ka[(int(i), int(j))]
+        assert b["overlap"][k] == pytest.approx(a["overlap"][m], rel=1e-5, abs=1e-13)
+        differ += not np.array_equal(b["point"][k], a["point"][m])
+    assert differ > 0
 
 
 def test_convex_phase_calls_equal_fused_step():
