@@ -242,6 +242,13 @@ int orc_c_pair(orc_csys* s, int p0, int p1, const double* shift, double* point,
                double* ovec, double* overlap, double* force);
 int orc_c_wall_pair(orc_csys* s, int p, int wall, double* point, double* ovec,
                     double* overlap, double* force);
+/* per-particle Polyhedron support history (m_curr_vertex) on / off (after
+   orc_c_set_particles; off by default: every support call starts from the
+   first vertex with neighbours) */
+int orc_c_set_support_history(orc_csys* s, int on);
+/* one support call of a shape from a given start vertex (-1: the default) */
+int orc_c_support(const orc_csys* s, int shape, const double* dir, int start, double* point,
+                  int* vertex);
 /* narrow phase of the convex contacts: 0 Johnson GJK (default), 1 GJK_SV */
 int orc_c_set_narrow_phase(orc_csys* s, int sv);
 /* GJK between two shapes of this system under crust-scaled transforms:

@@ -73,6 +73,7 @@ struct orc_csys {
   double *log_pt, *log_ov, *log_d, *log_f;
   int64_t ngjk;
   int gjk_sv; /* 1: closest_points_GJK_SV, 0: Johnson closest_points */
+  int *hist;  /* per-particle Polyhedron::m_curr_vertex, when histories are kept */
   int err_code;
   int32_t err_id0, err_id1;
   /* time of the current force evaluation; AppCollision contact features */
@@ -118,11 +119,15 @@ static v3 support_box(const shape_t* sh, v3 v) {
 }
 
 /* Polyhedron::support (Geometry/src/Polyhedron.cpp:265-300), hill climbing on
-   the vertex adjacency, moving only on a gain of at least 1e-16 */
-static v3 support_poly(const shape_t* sh, v3 v) {
+   the vertex adjacency, moving only on a gain of at least 1e-16.  hist, when
+   given, is the body's m_curr_vertex (Polyhedron.hh:134): the climb starts
+   there and leaves it at the vertex returned, as the reference's mutable
+   member does; without it every call starts from the first vertex with
+   neighbours (the value after construction, Polyhedron.cpp:362-364). */
+static v3 support_poly(const shape_t* sh, v3 v, int* hist) {
   double nrm = norm(v);
   if (!(nrm > ORC_EPSILON)) return mk(0., 0., 0.);
-  int curr = sh->first_vertex, last = -1;
+  int curr = hist ? *hist : sh->first_vertex, last = -1;
   double h = dot(ld(sh->v[curr]), v), d = 0.;
   for (;;) {
     int i = 0, n = sh->cob_n[curr];
@@ -134,6 +139,7 @@ static v3 support_poly(const shape_t* sh, v3 v) {
     curr = sh->cob[curr][i];
     h = d;
   }
+  if (hist) *hist = curr;
   return ld(sh->v[curr]);
 }
 
@@ -147,9 +153,9 @@ static v3 support_sphere(const shape_t* sh, v3 v) {
   return mk(0., 0., 0.);
 }
 
-static v3 support(const shape_t* sh, v3 v) {
+static v3 support(const shape_t* sh, v3 v, int* hist) {
   if (sh->type == ORC_SHAPE_BOX) return support_box(sh, v);
-  if (sh->type == ORC_SHAPE_POLYHEDRON) return support_poly(sh, v);
+  if (sh->type == ORC_SHAPE_POLYHEDRON) return support_poly(sh, v, hist);
   return support_sphere(sh, v);
 }
 
@@ -318,7 +324,7 @@ static int degenerate(unsigned all_bits, const v3* y, v3 w) {
 /* closest_points (Convex.cpp:536-627), tolerance GrainsExec::m_colDetTolerance
    = EPSILON (GrainsExec.cpp:51), no acceleration */
 static double closest_points(const shape_t* a, const shape_t* b, const xf_t* a2w,
-                             const xf_t* b2w, v3* pa, v3* pb, int* nbIter) {
+                             const xf_t* b2w, v3* pa, v3* pb, int* nbIter, int* ha, int* hb) {
   unsigned bits = 0, last = 0, last_bit = 0, all_bits = 0;
   v3 p[4], q[4];
   gjk_t g;
@@ -330,7 +336,7 @@ static double closest_points(const shape_t* a, const shape_t* b, const xf_t* a2w
   const double relError = ORC_EPSILON;
   const double absError = 1.e-4 * relError;
   v3 zero = mk(0., 0., 0.);
-  v3 v = sub(xf_apply(a2w, support(a, zero)), xf_apply(b2w, support(b, zero)));
+  v3 v = sub(xf_apply(a2w, support(a, zero, ha)), xf_apply(b2w, support(b, zero, hb)));
   v3 w = v;
   double dist = norm(v);
   while (bits < 15 && dist > 1.0e-15 && numIterations < 1000) {
@@ -342,8 +348,8 @@ static double closest_points(const shape_t* a, const shape_t* b, const xf_t* a2w
       last_bit <<= 1;
     }
     v3 d = v;
-    p[last] = support(a, vec_mat(neg(d), a2w->b));
-    q[last] = support(b, vec_mat(d, b2w->b));
+    p[last] = support(a, vec_mat(neg(d), a2w->b), ha);
+    q[last] = support(b, vec_mat(d, b2w->b), hb);
     w = sub(xf_apply(a2w, p[last]), xf_apply(b2w, q[last]));
     mu = dist - dot(v, w) / dist;
     if (mu < dist * relError || mu < absError) break;
@@ -513,7 +519,8 @@ static v3 sv_subalgorithm(const v3* y, unsigned* bits, double* lambdas) {
    degenerate test (against the current simplex only) and in the witness
    points (the weights sum to one). */
 static double closest_points_sv(const shape_t* a, const shape_t* b, const xf_t* a2w,
-                                const xf_t* b2w, v3* pa, v3* pb, int* nbIter) {
+                                const xf_t* b2w, v3* pa, v3* pb, int* nbIter, int* ha,
+                                int* hb) {
   unsigned bits = 0, last = 0;
   v3 p[4], q[4], y[4];
   double lambdas[4] = {0., 0., 0., 0.};
@@ -525,7 +532,7 @@ static double closest_points_sv(const shape_t* a, const shape_t* b, const xf_t* 
   const double relError = ORC_EPSILON;
   const double absError = 1.e-4 * relError;
   v3 zero = mk(0., 0., 0.);
-  v3 v = sub(xf_apply(a2w, support(a, zero)), xf_apply(b2w, support(b, zero)));
+  v3 v = sub(xf_apply(a2w, support(a, zero, ha)), xf_apply(b2w, support(b, zero, hb)));
   v3 w = v;
   double dist = norm(v);
   while (bits < 15 && dist > 1.0e-15 && numIterations < 1000) {
@@ -533,8 +540,8 @@ static double closest_points_sv(const shape_t* a, const shape_t* b, const xf_t* 
     for (last = 0; last < 4; ++last)
       if (!(bits & (1u << last))) break;
     v3 d = v;
-    p[last] = support(a, vec_mat(neg(d), a2w->b));
-    q[last] = support(b, vec_mat(d, b2w->b));
+    p[last] = support(a, vec_mat(neg(d), a2w->b), ha);
+    q[last] = support(b, vec_mat(d, b2w->b), hb);
     w = sub(xf_apply(a2w, p[last]), xf_apply(b2w, q[last]));
     mu = dist - dot(v, w) / dist;
     if (mu < dist * relError || mu < absError) break;
@@ -551,10 +558,11 @@ static double closest_points_sv(const shape_t* a, const shape_t* b, const xf_t* 
   return dist;
 }
 
+/* ha, hb: the bodies' support histories (support_poly), or NULL */
 static double gjk_distance(int sv, const shape_t* a, const shape_t* b, const xf_t* a2w,
-                           const xf_t* b2w, v3* pa, v3* pb, int* nbIter) {
-  return sv ? closest_points_sv(a, b, a2w, b2w, pa, pb, nbIter)
-            : closest_points(a, b, a2w, b2w, pa, pb, nbIter);
+                           const xf_t* b2w, v3* pa, v3* pb, int* nbIter, int* ha, int* hb) {
+  return sv ? closest_points_sv(a, b, a2w, b2w, pa, pb, nbIter, ha, hb)
+            : closest_points(a, b, a2w, b2w, pa, pb, nbIter, ha, hb);
 }
 
 /* ------------------------------------------------------------- shapes */
@@ -563,7 +571,7 @@ static void finish_shape(shape_t* sh) {
      extent along axis d = (support(e_d).d - support(-e_d).d) / 2 */
   for (int d = 0; d < 3; ++d) {
     v3 e = mk(d == 0 ? 1. : 0., d == 1 ? 1. : 0., d == 2 ? 1. : 0.);
-    v3 sp = support(sh, e), sm = support(sh, neg(e));
+    v3 sp = support(sh, e, NULL), sm = support(sh, neg(e), NULL);
     double mx = dot(e, sp), mn = dot(e, sm);
     sh->bbox_ext[d] = (mx - mn) / 2.;
   }
@@ -601,6 +609,8 @@ static void cfree_particles(orc_csys* s) {
   free(s->x); free(s->v); free(s->w); free(s->wbf); free(s->q); free(s->basis); free(s->F);
   free(s->T); free(s->mass); free(s->inertia); free(s->req); free(s->crust); free(s->scale);
   free(s->shape); free(s->mat); free(s->next); free(s->prev); free(s->cell);
+  free(s->hist);
+  s->hist = NULL;
 }
 
 void orc_c_destroy(orc_csys* s) {
@@ -855,6 +865,7 @@ typedef struct {
   double origin[3];
   const double* scale;
   double crust, circ;
+  int* hist; /* support history (m_curr_vertex) of a polyhedron, or NULL */
 } body_t;
 
 /* RigidBodyWithCrust::ClosestPoint general case (RigidBodyWithCrust.cpp:274-348):
@@ -873,7 +884,7 @@ static int closest_point(int sv, const body_t* A, const body_t* B, v3* contact, 
   crust_xf(B->basis, B->origin, B->scale, &b2w);
   v3 pa, pb;
   int nit = 0;
-  double distance = gjk_distance(sv, A->sh, B->sh, &a2w, &b2w, &pa, &pb, &nit);
+  double distance = gjk_distance(sv, A->sh, B->sh, &a2w, &b2w, &pa, &pb, &nit, A->hist, B->hist);
   *nbIter = nit;
   if (distance < ORC_EPSILON) return -1;
   pa = xf_apply(&a2w, pa);
@@ -917,7 +928,7 @@ static int closest_point_retry(const body_t* A, const body_t* B, v3* contact, v3
   crust_xf(B->basis, B->origin, sb, &b2w);
   v3 pa, pb;
   int nit = 0;
-  double distance = closest_points(A->sh, B->sh, &a2w, &b2w, &pa, &pb, &nit);
+  double distance = closest_points(A->sh, B->sh, &a2w, &b2w, &pa, &pb, &nit, A->hist, B->hist);
   *nbIter = nit;
   if (distance < ORC_EPSILON) return -1;
   pa = xf_apply(&a2w, pa);
@@ -929,6 +940,35 @@ static int closest_point_retry(const body_t* A, const body_t* B, v3* contact, v3
   *ov = scl(ba, imposed);
   *dist = -imposed;
   return 1;
+}
+
+/* Keep each polyhedron particle's support history (Polyhedron::m_curr_vertex,
+   mutable, Polyhedron.cpp:265-300), every particle starting from its shape's
+   first vertex with neighbours as after construction or a restart; the
+   history then follows this oracle's order of closest-point calls, which is
+   the reference's order of Particle::InterAction calls. */
+int orc_c_set_support_history(orc_csys* s, int on) {
+  free(s->hist);
+  s->hist = NULL;
+  if (!on) return ORC_OK;
+  s->hist = (int*)malloc(sizeof(int) * (size_t)(s->n > 0 ? s->n : 1));
+  if (!s->hist) return ORC_ERR_ARG;
+  for (int i = 0; i < s->n; ++i) s->hist[i] = s->shapes[s->shape[i]].first_vertex;
+  return ORC_OK;
+}
+
+/* One support call on a shape of this system: from vertex `start` (a
+   polyhedron's history value; -1: the first vertex with neighbours), the
+   support point of direction dir and, for a polyhedron, the vertex reached. */
+int orc_c_support(const orc_csys* s, int shape, const double* dir, int start, double* point,
+                  int* vertex) {
+  if (shape < 0 || shape >= s->nshape) return ORC_ERR_ARG;
+  const shape_t* sh = &s->shapes[shape];
+  int h = start;
+  if (sh->type == ORC_SHAPE_POLYHEDRON && (h < 0 || h >= sh->nv)) h = sh->first_vertex;
+  st(point, support(sh, ld(dir), sh->type == ORC_SHAPE_POLYHEDRON ? &h : NULL));
+  *vertex = sh->type == ORC_SHAPE_POLYHEDRON ? h : -1;
+  return ORC_OK;
 }
 
 /* CollisionDetection Type="GJK" / "GJK_SV" (Base/src/Grains.cpp:660-666) */
@@ -947,7 +987,7 @@ int orc_c_closest_points(const orc_csys* s, int shape_a, const double* basis_a,
   crust_xf(basis_b, origin_b, scale_b, &b2w);
   v3 A, B;
   *dist = gjk_distance(s->gjk_sv, &s->shapes[shape_a], &s->shapes[shape_b], &a2w, &b2w, &A, &B,
-                       nbIter);
+                       nbIter, NULL, NULL);
   st(pa, xf_apply(&a2w, A));
   st(pb, xf_apply(&b2w, B));
   return ORC_OK;
@@ -1025,6 +1065,7 @@ static body_t particle_body(const orc_csys* s, int p, v3 shift) {
   b.scale = s->scale + 3 * p;
   b.crust = s->crust[p];
   b.circ = b.sh->circ;
+  b.hist = s->hist ? s->hist + p : NULL;
   return b;
 }
 
@@ -1043,7 +1084,7 @@ static int crect(const body_t* A, const cwall_t* wl, v3* point, v3* ov, double* 
   const double* B = A->basis;
   v3 dl = mk(B[0] * d.x + B[3] * d.y + B[6] * d.z, B[1] * d.x + B[4] * d.y + B[7] * d.z,
              B[2] * d.x + B[5] * d.y + B[8] * d.z);
-  v3 sp = support(A->sh, dl);
+  v3 sp = support(A->sh, dl, A->hist);
   /* Transform::operator() (Transform.cpp:312-317) */
   v3 pa = mk(dot(mk(B[0], B[1], B[2]), sp) + A->origin[0], dot(mk(B[3], B[4], B[5]), sp) + A->origin[1],
              dot(mk(B[6], B[7], B[8]), sp) + A->origin[2]);
@@ -1099,6 +1140,7 @@ static int cinteract(orc_csys* s, int p0, int p1, v3 shift) {
 int orc_c_pair(orc_csys* s, int p0, int p1, const double* shift, double* point, double* ovec,
                double* overlap, double* force) {
   body_t A = particle_body(s, p0, mk(0., 0., 0.)), B = particle_body(s, p1, ld(shift));
+  A.hist = B.hist = NULL; /* a diagnostic: no support history moves */
   v3 c, ov;
   double dist;
   int64_t ng = 0;
@@ -1141,6 +1183,7 @@ static int cinteract_wall(orc_csys* s, int p, int wi) {
   B.scale = wl->scale;
   B.crust = wl->crust;
   B.circ = wl->circ;
+  B.hist = NULL;
   v3 c, ov;
   double dist;
   int nit = 0;
@@ -1168,6 +1211,7 @@ int orc_c_wall_pair(orc_csys* s, int p, int wi, double* point, double* ovec, dou
                     double* force) {
   const cwall_t* wl = &s->walls[wi];
   body_t A = particle_body(s, p, mk(0., 0., 0.));
+  A.hist = NULL; /* a diagnostic: no support history moves */
   shape_t wsh;
   memset(&wsh, 0, sizeof(wsh));
   wsh.type = ORC_SHAPE_BOX;
@@ -1180,6 +1224,7 @@ int orc_c_wall_pair(orc_csys* s, int p, int wi, double* point, double* ovec, dou
   B.scale = wl->scale;
   B.crust = wl->crust;
   B.circ = wl->circ;
+  B.hist = NULL;
   v3 c, ov;
   double dist;
   int64_t ng = 0;

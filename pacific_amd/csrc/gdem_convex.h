@@ -4,8 +4,10 @@
 //
 // Paths are relative to src/Grains3D/Grains/.  Polyhedron::support hill-climbs
 // from a vertex it keeps between calls (Polyhedron.cpp:265-300); here each call
-// starts from the shape's first vertex with neighbours, so results do not
-// depend on call history (they differ only when two vertices tie within 1e-16).
+// starts from the shape's first vertex with neighbours (the value after
+// construction or a restart), so results do not depend on call history.  They
+// differ only when two vertices tie within 1e-16 along the direction, and then
+// only in which tied vertex comes back.
 #pragma once
 #include <type_traits>
 
