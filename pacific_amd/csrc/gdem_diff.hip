@@ -89,62 +89,8 @@ struct DiffArgs {
 };
 
 __device__ __forceinline__ v3 ld3(double4 a) { return mk(a.x, a.y, a.z); }
-// a.(x|y|z)[c] += g without indexing into memory
-__device__ __forceinline__ void acc3(v3& a, int c, double g) {
-  if (c == 0) a.x += g;
-  else if (c == 1) a.y += g;
-  else a.z += g;
-}
 __device__ __forceinline__ double4 st3(v3 a, double w) { return make_double4(a.x, a.y, a.z, w); }
 
-// Inputs of one particle-particle contact (A first, B second moved to A's image).
-struct PairIn {
-  v3 xA, vA, wA, xB, vB, wB;
-  double rA, crA, mA, rB, crB, mB;
-};
-struct PairAdj {
-  v3 FA, TA, FB, TB;
-};
-
-// Directional derivative of <adjoint, (F_A, T_A, F_B, T_B)> along input
-// direction s: 0..8 = (x, v, w) of A, 9..17 = (x, v, w) of B, 18..20 = (kn,
-// en, mu).
-__device__ __forceinline__ double pair_dir(const PairIn& in, const PairAdj& adj, double kn, double beta,
-                           double dbeta, double mu, int s) {
-  auto seed = [s](v3 a, int base) {
-    return mk3(Dual{a.x, s == base ? 1. : 0.}, Dual{a.y, s == base + 1 ? 1. : 0.},
-               Dual{a.z, s == base + 2 ? 1. : 0.});
-  };
-  const V3<Dual> xA = seed(in.xA, 0), vA = seed(in.vA, 3), wA = seed(in.wA, 6);
-  const V3<Dual> xB = seed(in.xB, 9), vB = seed(in.vB, 12), wB = seed(in.wB, 15);
-  HookeParams<Dual> p;
-  p.kn = Dual{kn, s == 18 ? 1. : 0.};
-  p.beta = Dual{beta, s == 19 ? dbeta : 0.};
-  p.mu = Dual{mu, s == 20 ? 1. : 0.};
-  V3<Dual> FA, TA, FB, TB;
-  if (!pair_torsors(xA, vA, wA, in.rA, in.crA, in.mA, xB, vB, wB, in.rB, in.crB, in.mB, p, FA, TA,
-                    FB, TB))
-    return 0.;
-  auto tan3 = [](V3<Dual> a) { return mk(a.x.d, a.y.d, a.z.d); };
-  return dot(adj.FA, tan3(FA)) + dot(adj.TA, tan3(TA)) + dot(adj.FB, tan3(FB)) +
-         dot(adj.TB, tan3(TB));
-}
-
-// Same for a wall contact: 0..8 = (x, v, w) of the particle, 9..11 = (kn, en, mu).
-__device__ __forceinline__ double wall_dir(const Wall& wl, v3 x, v3 v, v3 w, double r, double cr, double m,
-                           v3 aF, v3 aT, double kn, double beta, double dbeta, double mu, int s) {
-  auto seed = [s](v3 a, int base) {
-    return mk3(Dual{a.x, s == base ? 1. : 0.}, Dual{a.y, s == base + 1 ? 1. : 0.},
-               Dual{a.z, s == base + 2 ? 1. : 0.});
-  };
-  HookeParams<Dual> p;
-  p.kn = Dual{kn, s == 9 ? 1. : 0.};
-  p.beta = Dual{beta, s == 10 ? dbeta : 0.};
-  p.mu = Dual{mu, s == 11 ? 1. : 0.};
-  V3<Dual> FA, TA;
-  if (!wall_torsor(wl, seed(x, 0), seed(v, 3), seed(w, 6), r, cr, m, p, FA, TA)) return 0.;
-  return dot(aF, mk(FA.x.d, FA.y.d, FA.z.d)) + dot(aT, mk(TA.x.d, TA.y.d, TA.z.d));
-}
 
 // Tangent (Jacobian-vector product) of one contact: inputs carry tangents
 // tin, parameters (kn, beta, mu) tangents tp; returns the tangents of the
@@ -634,28 +580,18 @@ struct DiffRunner {
         v3 shift;
         const int q = decode(u, code, meA, shift);
         const PairIn in = pair_in(p, q, meA, shift);
-        // quick contact test on values before the dual passes
-        V3<double> c0, ov0;
-        double d0;
-        if (ss_contact(lift3<double>(in.xA), in.rA, in.crA, lift3<double>(in.xB), in.rB, in.crB,
-                       c0, ov0, d0) <= 0 || !(d0 < 0.))
-          return;
         const int A = meA ? p : q, B = meA ? q : p;
         PairAdj adj;
         adj.FA = ld3(aFt[A]); adj.TA = ld3(aTt[A]);
         adj.FB = ld3(aFt[B]); adj.TB = ld3(aTt[B]);
-        // columns: the 9 inputs of this particle, then (if it detected the
-        // pair) the 3 parameters
-        const int base = meA ? 0 : 9;
-        const int ncol = own ? 12 : 9;
-        for (int k = 0; k < ncol; ++k) {
-          const int sdir = k < 9 ? base + k : 18 + (k - 9);
-          const double g = pair_dir(in, adj, sy.kn, sy.beta, sy.dbeta, sy.mu, sdir);
-          if (k < 3) acc3(dx, k, g);
-          else if (k < 6) acc3(dv, k - 3, g);
-          else if (k < 9) acc3(dw, k - 6, g);
-          else acc3(gth, k - 9, g);
-        }
+        // one reverse pass gives every column: this particle's 9 inputs and,
+        // if it detected the pair, the 3 parameters
+        PairGrad pg;
+        if (!pair_vjp(in, adj, sy.kn, sy.beta, sy.dbeta, sy.mu, pg)) return;
+        dx = add(dx, meA ? pg.xA : pg.xB);
+        dv = add(dv, meA ? pg.vA : pg.vB);
+        dw = add(dw, meA ? pg.wA : pg.wB);
+        if (own) gth = add(gth, pg.th);
       };
       const int nout = min(a.out_count[sy.off + t], kListCap);
       const int2* ol = a.out_list + (size_t)sy.off * kListCap + t;
@@ -680,17 +616,13 @@ struct DiffRunner {
               fabs(wl.bb_centre[1] - x.y) <= wl.bb_ext[1] + pp.x &&
               fabs(wl.bb_centre[2] - x.z) <= wl.bb_ext[2] + pp.x))
           continue;
-        V3<double> c0, ov0;
-        double d0;
-        if (sw_contact(wl, lift3<double>(x), pp.x, pp.w, c0, ov0, d0) <= 0 || !(d0 < 0.)) continue;
-        for (int k = 0; k < 12; ++k) {
-          const double g = wall_dir(wl, x, vv, ww, pp.x, pp.w, pp.y, aF, aT, sy.kn, sy.beta, sy.dbeta,
-                                    sy.mu, k);
-          if (k < 3) acc3(dx, k, g);
-          else if (k < 6) acc3(dv, k - 3, g);
-          else if (k < 9) acc3(dw, k - 6, g);
-          else acc3(gth, k - 9, g);
-        }
+        PairGrad wg;
+        if (!wall_vjp(wl, x, vv, ww, pp.x, pp.w, pp.y, aF, aT, sy.kn, sy.beta, sy.dbeta, sy.mu, wg))
+          continue;
+        dx = add(dx, wg.xA);
+        dv = add(dv, wg.vA);
+        dw = add(dw, wg.wA);
+        gth = add(gth, wg.th);
       }
       gx[sy.off + p] = st3(dx, 0.);
       gv[sy.off + p] = st3(dv, 0.);

@@ -349,4 +349,227 @@ __host__ __device__ __forceinline__ bool wall_torsor(const Wall& wl, V3<S> xA, V
   return true;
 }
 
+// ---------------------------------------------------------------- reverse mode
+// Inputs of one particle-particle contact (A first, B second moved to A's image).
+struct PairIn {
+  v3 xA, vA, wA, xB, vB, wB;
+  double rA, crA, mA, rB, crB, mB;
+};
+// Adjoints of the contact's force and torque on each body.
+struct PairAdj {
+  v3 FA, TA, FB, TB;
+};
+// Gradient of <adjoint, (F_A, T_A, F_B, T_B)> with respect to every input
+// and to (kn, en, mu).
+struct PairGrad {
+  v3 xA, vA, wA, xB, vB, wB, th;
+};
+
+// Forward and reverse of hooke_t (kr = 0) at contact point c and overlap
+// vector ov, followed by the torsors F_A = f, T_A = (c - xA) x f, F_B = -f,
+// T_B = (c - xB) x (-f), for the adjoint adj.  Writes the velocity and angular
+// velocity gradients of both bodies, the position gradients through the lever
+// arms (g.xA, g.xB) and (kn, beta, mu) gradients in g.th; returns those of c
+// and ov in gc, gov.  A wall is body B at rest with mass 1e20 and a zero
+// adjoint (wall_torsor).
+__host__ __device__ inline void hooke_torsor_vjp(v3 c, v3 ov, const PairIn& in, const PairAdj& adj,
+                                                 double kn, double beta, double mu, PairGrad& g,
+                                                 v3& gc, v3& gov) {
+  const double np = norm(ov);
+  const v3 nr = dvd(ov, np);
+  const bool kx = !(fabs(nr.x) < kEps), ky = !(fabs(nr.y) < kEps), kz = !(fabs(nr.z) < kEps);
+  const v3 normal = mk(kx ? nr.x : 0., ky ? nr.y : 0., kz ? nr.z : 0.);
+  const v3 rA = sub(c, in.xA), rB = sub(c, in.xB);
+  const v3 tmpV = sub(add(in.vA, cross(in.wA, rA)), add(in.vB, cross(in.wB, rB)));
+  const double vns = dot(tmpV, normal);
+  const v3 vn = scl(normal, vns);
+  const v3 vt = sub(tmpV, vn);
+  const double avm = 1. / (1. / in.mA + 1. / in.mB);
+  const double sq1 = sqrt(avm * kn);
+  const double gamman = -2. * beta * sq1;
+  const v3 dFN = sub(scl(ov, kn), scl(vn, gamman));
+  const double nFN = norm(dFN);
+  const double sq2 = sqrt(kn / avm);
+  const double etat = -beta * sq2;
+  const double gammat = 2. * etat * avm;
+  const v3 dFT0 = scl(vt, -gammat);
+  const double fn = mu * nFN;
+  const bool slip = fn < norm(dFT0);
+  const double nvt = norm(vt);
+  const bool has_t = nvt > kEps;
+  const v3 tangent = slip && has_t ? dvd(neg(vt), nvt) : mk(0., 0., 0.);
+  const v3 f = add(dFN, slip ? scl(tangent, fn) : dFT0);
+  // torsors
+  const v3 gf = sub(add(sub(adj.FA, adj.FB), cross(adj.TA, rA)), cross(adj.TB, rB));
+  v3 grA = cross(f, adj.TA), grB = cross(adj.TB, f);
+  v3 gvt = mk(0., 0., 0.), gdFN = gf;
+  double gkn = 0., gbeta = 0., gmu = 0.;
+  if (slip) {
+    // f = dFN + tangent * mu |dFN|
+    const double gfn = dot(gf, tangent);
+    if (has_t) {
+      const v3 gt = scl(gf, fn);
+      gvt = add(scl(gt, -1. / nvt), scl(vt, dot(gt, vt) / (nvt * nvt * nvt)));
+    }
+    gmu = gfn * nFN;
+    if (nFN > 0.) gdFN = add(gdFN, scl(dFN, gfn * mu / nFN));
+  } else {
+    // f = dFN + vt * (-2 avm (-beta sqrt(kn / avm)))
+    gvt = scl(gf, -gammat);
+    const double getat = 2. * avm * -dot(gf, vt);
+    gbeta = -sq2 * getat;
+    if (sq2 > 0.) gkn = -beta * getat * (0.5 / sq2) / avm;
+  }
+  // dFN = ov kn - vn (-2 beta sqrt(avm kn))
+  gkn += dot(gdFN, ov);
+  const double ggamman = -dot(gdFN, vn);
+  gbeta += -2. * sq1 * ggamman;
+  if (sq1 > 0.) gkn += -2. * beta * ggamman * (0.5 / sq1) * avm;
+  gov = scl(gdFN, kn);
+  const v3 gvn = sub(scl(gdFN, -gamman), gvt);
+  // vt = tmpV - vn, vn = normal (tmpV . normal)
+  const double gvns = dot(gvn, normal);
+  const v3 gtmpV = add(gvt, scl(normal, gvns));
+  v3 gnormal = add(scl(gvn, vns), scl(tmpV, gvns));
+  gnormal = mk(kx ? gnormal.x : 0., ky ? gnormal.y : 0., kz ? gnormal.z : 0.);
+  // normal = ov / |ov|
+  gov = add(gov, sub(dvd(gnormal, np), scl(ov, dot(gnormal, ov) / (np * np * np))));
+  // tmpV = vA + wA x rA - vB - wB x rB
+  g.vA = gtmpV;
+  g.vB = neg(gtmpV);
+  g.wA = cross(rA, gtmpV);
+  g.wB = neg(cross(rB, gtmpV));
+  grA = add(grA, cross(gtmpV, in.wA));
+  grB = sub(grB, cross(gtmpV, in.wB));
+  // rA = c - xA, rB = c - xB
+  gc = add(grA, grB);
+  g.xA = neg(grA);
+  g.xB = neg(grB);
+  g.th = mk(gkn, gbeta, gmu);
+}
+
+// Reverse-mode derivative of one pair contact (pair_torsors over ss_contact
+// and hooke_t, kr = 0), written out by hand: the forward values once, then
+// each operation's transpose in reverse order, with the branch decisions
+// (contact, Coulomb sliding, zero tangential velocity, rounded normal
+// components) of the forward evaluation.  One call gives all 21 partial
+// derivatives that pair_torsors over Dual gives one at a time.  dbeta is
+// d beta / d en.  False when not in contact.
+__host__ __device__ inline bool pair_vjp(const PairIn& in, const PairAdj& adj, double kn,
+                                         double beta, double dbeta, double mu, PairGrad& g) {
+  // ss_contact
+  const v3 AB = sub(in.xB, in.xA);
+  const double nAB = norm(AB);
+  const double dist = nAB - (in.rA + in.rB);
+  if (dist > 0. || -dist >= in.crA + in.crB || !(dist < 0.)) return false;
+  const double sc = in.rA + 0.5 * dist;
+  const v3 c = add(in.xA, dvd(scl(AB, sc), nAB));
+  const v3 ov = scl(dvd(AB, nAB), dist);
+  v3 gc, gov;
+  hooke_torsor_vjp(c, ov, in, adj, kn, beta, mu, g, gc, gov);
+  // c = xA + AB sc / |AB|, ov = AB dist / |AB|, sc = rA + dist / 2,
+  // dist = |AB| - (rA + rB)
+  const double n2 = nAB * nAB, n3 = n2 * nAB;
+  const double gcAB = dot(gc, AB), govAB = dot(gov, AB);
+  const v3 gAB = add(add(scl(gc, sc / nAB), scl(AB, gcAB * 0.5 / n2 - gcAB * sc / n3)),
+                     add(scl(gov, dist / nAB), scl(AB, govAB / n2 - govAB * dist / n3)));
+  g.xA = sub(add(g.xA, gc), gAB);
+  g.xB = add(g.xB, gAB);
+  g.th.y *= dbeta;
+  return true;
+}
+
+// Transpose of a row-major 3x3 matrix applied to a vector.
+__host__ __device__ inline v3 mat_t_vec(const double* b, v3 u) {
+  return mk(b[0] * u.x + b[3] * u.y + b[6] * u.z, b[1] * u.x + b[4] * u.y + b[7] * u.z,
+            b[2] * u.x + b[5] * u.y + b[8] * u.z);
+}
+
+// Reverse-mode derivative of a particle-wall contact (wall_torsor over
+// sw_contact, box_sphere_t and hooke_t): gradients of <(aF, aT), (F, T)>
+// with respect to the particle's x, v, w (g.xA, g.vA, g.wA) and (kn, en, mu)
+// (g.th).  False when not in contact.
+__host__ __device__ inline bool wall_vjp(const Wall& wl, v3 x, v3 v, v3 w, double r, double cr,
+                                         double m, v3 aF, v3 aT, double kn, double beta,
+                                         double dbeta, double mu, PairGrad& g) {
+  // sw_contact with box_sphere_t, values only, remembering the region
+  const v3 lg = xform(wl.inv_basis, wl.inv_origin, x);
+  const double ex = wl.ext[0], ey = wl.ext[1], ez = wl.ext[2];
+  const int sx = lg.x > ex ? 1 : (lg.x < -ex ? -1 : 0);
+  const int sy = lg.y > ey ? 1 : (lg.y < -ey ? -1 : 0);
+  const int sz = lg.z > ez ? 1 : (lg.z < -ez ? -1 : 0);
+  const int nout = (sx != 0) + (sy != 0) + (sz != 0);
+  if (nout == 0) return false;  // centre inside the box: overlap -10000, a contact error
+  // corner of the region (the box corner, or the edge's corner point)
+  const v3 corner = mk(sx < 0 ? -ex : ex, sy < 0 ? -ey : ey, sz < 0 ? -ez : ez);
+  double overlap, nd = 0., kk = 0.;
+  v3 cp, d = mk(0., 0., 0.);
+  int face = -1;
+  if (nout == 1) {
+    face = sx ? 0 : (sy ? 1 : 2);
+    const int sg = sx + sy + sz;
+    const double gc = face == 0 ? lg.x : (face == 1 ? lg.y : lg.z);
+    const double e = face == 0 ? ex : (face == 1 ? ey : ez);
+    nd = sg > 0 ? gc - e : (-e) - gc;
+    if (!(nd < r)) return false;
+    overlap = nd - r;
+    const double cf = sg > 0 ? e + 0.5 * overlap : -e - 0.5 * overlap;
+    cp = mk(face == 0 ? cf : lg.x, face == 1 ? cf : lg.y, face == 2 ? cf : lg.z);
+  } else {
+    // an edge keeps the coordinate inside the box's range, a corner none
+    const v3 pr = mk(sx ? lg.x : corner.x, sy ? lg.y : corner.y, sz ? lg.z : corner.z);
+    d = sub(nout == 3 ? lg : pr, corner);
+    nd = norm(d);
+    if (!(nd < r)) return false;
+    overlap = nd - r;
+    kk = 1. + 0.5 * overlap / nd;
+    cp = sub(lg, scl(d, kk));
+  }
+  if (!(overlap < 0.) || -overlap >= cr + wl.crust) return false;
+  const v3 contact = xform(wl.basis, wl.origin, cp);
+  const v3 AB = sub(contact, x);
+  const double nAB = norm(AB);
+  const v3 ov = scl(AB, overlap / nAB);
+  PairIn in;
+  in.xA = x; in.vA = v; in.wA = w; in.rA = r; in.crA = cr; in.mA = m;
+  in.xB = mk(wl.origin[0], wl.origin[1], wl.origin[2]);
+  in.vB = in.wB = mk(0., 0., 0.);
+  in.rB = 0.; in.crB = wl.crust; in.mB = 1.e20;
+  PairAdj adj;
+  adj.FA = aF; adj.TA = aT;
+  adj.FB = adj.TB = mk(0., 0., 0.);
+  v3 gc, gov;
+  hooke_torsor_vjp(contact, ov, in, adj, kn, beta, mu, g, gc, gov);
+  // ov = AB overlap / |AB|, AB = contact - x
+  const double govAB = dot(gov, AB);
+  const double n3 = nAB * nAB * nAB;
+  const v3 gAB = sub(scl(gov, overlap / nAB), scl(AB, govAB * overlap / n3));
+  double gover = govAB / nAB;
+  const v3 gcontact = add(gc, gAB);
+  v3 gx = sub(g.xA, gAB);
+  // contact = basis cp + origin
+  const v3 gcp = mat_t_vec(wl.basis, gcontact);
+  v3 glg;
+  if (face >= 0) {
+    // cp: the face coordinate e + overlap / 2 (or -e - overlap / 2), the
+    // others the centre's; overlap = nd - r with nd = +-(g - e)
+    const int sg = sx + sy + sz;
+    const double gcf = face == 0 ? gcp.x : (face == 1 ? gcp.y : gcp.z);
+    gover += sg > 0 ? 0.5 * gcf : -0.5 * gcf;
+    const double gnd = sg > 0 ? gover : -gover;
+    glg = mk(face == 0 ? gnd : gcp.x, face == 1 ? gnd : gcp.y, face == 2 ? gnd : gcp.z);
+  } else {
+    // cp = g - d (1 + overlap / (2 nd)), overlap = nd - r, nd = |d|
+    const double gk = -dot(gcp, d);
+    gover += gk * 0.5 / nd;
+    const double gnd = gover + gk * (-0.5 * overlap / (nd * nd));
+    const v3 gd = add(scl(gcp, -kk), scl(d, gnd / nd));
+    glg = add(gcp, nout == 3 ? gd : mk(sx ? gd.x : 0., sy ? gd.y : 0., sz ? gd.z : 0.));
+  }
+  gx = add(gx, mat_t_vec(wl.inv_basis, glg));
+  g.xA = gx;
+  g.th.y *= dbeta;
+  return true;
+}
+
 }  // namespace gdem
