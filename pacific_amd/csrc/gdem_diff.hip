@@ -157,7 +157,10 @@ __device__ __forceinline__ int block_scan_excl_d(int v, int* smem, int& total) {
   return prefix + x - v;
 }
 
-template <int BS>
+// TAN: the forward-mode (DIFF_TANGENT) instantiation; the others carry no
+// tangent code, which keeps their register allocation to what the gradient
+// needs.
+template <int BS, bool TAN>
 struct DiffRunner {
   const DiffArgs& a;
   const DiffSys& sy;
@@ -531,7 +534,7 @@ struct DiffRunner {
 
   // One forward step n -> n+1; optionally stores the force inputs in window slot j.
   __device__ __forceinline__ void step(int n1, int j) {
-    const bool tan = a.mode == DIFF_TANGENT;
+    constexpr bool tan = TAN;
     if (tan) tangent_kick_drift();
     kick_drift();
     bin_and_detect();
@@ -689,7 +692,7 @@ struct DiffRunner {
     const double lscale = 1. / ((double)a.nobs * 3. * (double)sy.n);
     initial_state();
     bin_and_detect();
-    if (a.mode == DIFF_TANGENT) {
+    if constexpr (TAN) {
       // tangents of the initial state: positions fixed, velocities along the
       // given direction (in gv0), angular velocities fixed
       for (int i = threadIdx.x; i < sy.n; i += BS) {
@@ -715,7 +718,7 @@ struct DiffRunner {
     if (ok && failed()) ok = false;
     const double L = block_sum<BS>(lsum, red) * lscale;
     if (threadIdx.x == 0) a.loss[s] = ok ? L : __builtin_nan("");
-    if (a.mode == DIFF_TANGENT) {
+    if constexpr (TAN) {
       const double dL = block_sum<BS>(dlsum, red) * lscale;
       if (threadIdx.x == 0) a.dloss[s] = ok ? dL : __builtin_nan("");
     }
@@ -762,7 +765,7 @@ struct DiffRunner {
   }
 };
 
-template <int BS>
+template <int BS, bool TAN>
 __global__ void __launch_bounds__(BS) k_diff(DiffArgs a) {
   __shared__ int s_item;
   __shared__ int smem[BS / 64 > 1 ? BS / 64 : 2];
@@ -778,7 +781,7 @@ __global__ void __launch_bounds__(BS) k_diff(DiffArgs a) {
     const int s = a.order[item];
     if (threadIdx.x == 0) ssy = a.sys[s];
     __syncthreads();
-    DiffRunner<BS> r{a, ssy, s, smem, red, &sd, 0., 0., mk(0., 0., 0.)};
+    DiffRunner<BS, TAN> r{a, ssy, s, smem, red, &sd, 0., 0., mk(0., 0., 0.)};
     r.run();
     __syncthreads();
   }
@@ -1131,10 +1134,18 @@ static int diff_launch(gdem_diff* d, int mode, const double* dtheta = nullptr,
   (void)hipGetLastError();  // an earlier failed call must not be reported as this launch's
   hipLaunchKernelGGL(k_diff_zero, dim3(1), dim3(1), 0, d->stream, d->d_next);
   switch (d->block) {
-    case 64: hipLaunchKernelGGL(k_diff<64>, dim3(wg), dim3(64), 0, d->stream, a); break;
-    case 128: hipLaunchKernelGGL(k_diff<128>, dim3(wg), dim3(128), 0, d->stream, a); break;
-    case 512: hipLaunchKernelGGL(k_diff<512>, dim3(wg), dim3(512), 0, d->stream, a); break;
-    default: hipLaunchKernelGGL(k_diff<256>, dim3(wg), dim3(256), 0, d->stream, a); break;
+#define GDEM_LAUNCH_DIFF(B)                                                        \
+  do {                                                                             \
+    if (mode == DIFF_TANGENT)                                                      \
+      hipLaunchKernelGGL((k_diff<B, true>), dim3(wg), dim3(B), 0, d->stream, a);  \
+    else                                                                           \
+      hipLaunchKernelGGL((k_diff<B, false>), dim3(wg), dim3(B), 0, d->stream, a); \
+  } while (0)
+    case 64: GDEM_LAUNCH_DIFF(64); break;
+    case 128: GDEM_LAUNCH_DIFF(128); break;
+    case 512: GDEM_LAUNCH_DIFF(512); break;
+    default: GDEM_LAUNCH_DIFF(256); break;
+#undef GDEM_LAUNCH_DIFF
   }
   DHIP(hipGetLastError());
   DHIP(hipStreamSynchronize(d->stream));
