@@ -360,6 +360,22 @@ int gdem_set_skin(gdem_ctx* ctx, double skin);
    the particles or the skin were last set.  No counterpart in the reference,
    whose lists are rebuilt at every step. */
 int gdem_get_skin(gdem_ctx* ctx, double* skin, int64_t* rebuilds);
+/* Capacity of the lists, which the reference (std::list, unbounded) does not
+   have.  Each particle's lists hold 24 entries per direction (convex: 32
+   candidates); a build that overflows raises GDEM_ERR_CAPACITY.  With skin
+   lists on, gdem_step ends by measuring the longest list of the last build
+   and, when it comes within 4 entries of the capacity, halves the skin
+   (switching the lists off below 0.05 r_max) and rebuilds at the next
+   evaluation, so a packing that densifies keeps running; the results do not
+   depend on the skin.  The check synchronises the context's stream.
+   gdem_set_skin_adapt(ctx, 0) switches it off (on by default).
+   gdem_get_list_occupancy gives the longest list of the last build (skin
+   lists: candidates within the skin; otherwise the outgoing contacts of the
+   last evaluation; convex: circumscribed-sphere candidates), the capacity and
+   the number of halvings so far. */
+int gdem_set_skin_adapt(gdem_ctx* ctx, int32_t on);
+int gdem_get_list_occupancy(gdem_ctx* ctx, int32_t* peak, int32_t* capacity,
+                            int32_t* skin_halvings);
 
 /* Waits for the stream; returns the first deferred device error. */
 int gdem_synchronize(gdem_ctx* ctx);

@@ -104,6 +104,11 @@ struct gdem_ctx {
   // skin lists (spheres): requested skin (< 0: automatic), effective skin,
   // device words and reference positions
   double skin_req = -1., skin = 0., rmax = 0., crust_min = 0.;
+  // skin lists: halve the skin when a list build comes within kListHeadroom
+  // entries of the capacity (checked at the end of each gdem_step)
+  bool skin_adapt = true;
+  int32_t skin_halvings = 0;
+  int32_t* d_peak = nullptr;
   SkinWords* d_skin = nullptr;
   float* d_path = nullptr;  // path length of each particle since the last rebuild
   unsigned long long* d_cmask = nullptr;
@@ -162,6 +167,7 @@ struct gdem_ctx {
   } while (0)
 
 static int collect_errors(gdem_ctx* c);
+static int adapt_skin(gdem_ctx* c);
 
 static int set_err(gdem_ctx* c, int code, int id0, int id1) {
   c->err_code = code;
@@ -391,6 +397,7 @@ extern "C" int gdem_destroy(gdem_ctx* c) {
   (void)hipFree(c->d_F); (void)hipFree(c->d_T);
   (void)hipFree(c->bins.key); (void)hipFree(c->bins.slot); (void)hipFree(c->bins.perm); (void)hipFree(c->bins.pos);
   (void)hipFree(c->bins.ids);
+  (void)hipFree(c->d_peak);
   (void)hipFree(c->d_out_list); (void)hipFree(c->d_in_list); (void)hipFree(c->d_out_count); (void)hipFree(c->d_in_count);
   (void)hipFree(c->bins.cnt); (void)hipFree(c->bins.local); (void)hipFree(c->bins.bsum); (void)hipFree(c->bins.cell_start);
   (void)hipFree(c->d_row_rank); (void)hipFree(c->d_row_jk); (void)hipFree(c->d_models); (void)hipFree(c->d_walls); (void)hipFree(c->d_err);
@@ -1470,6 +1477,54 @@ extern "C" int gdem_step(gdem_ctx* c, int32_t nsteps, double dt) {
     if (rc) return rc;
     k += run;
   }
+  return adapt_skin(c);
+}
+
+// Largest list length of any particle at the last list build (synchronises).
+static int list_peak(gdem_ctx* c, int32_t* peak) {
+  *peak = 0;
+  if (!c->d_out_count || !c->d_in_count || c->n <= 0) return GDEM_OK;
+  if (!c->d_peak) HIP_TRY(hipMalloc(&c->d_peak, sizeof(int32_t)));
+  HIP_TRY(hipMemsetAsync(c->d_peak, 0, sizeof(int32_t), c->stream));
+  launch_list_peak(c->d_out_count, c->d_in_count, (int32_t)c->n, c->d_peak, c->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(peak, c->d_peak, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GDEM_OK;
+}
+
+// Degrade before the skin lists overflow: when the last list build left a
+// particle within kListHeadroom entries of the capacity, halve the skin (the
+// lists then hold fewer candidates; below 5 % of the largest radius the skin
+// is switched off and lists hold contacts only) and rebuild at the next
+// evaluation.  The physics does not depend on the skin.
+static int adapt_skin(gdem_ctx* c) {
+  if (!c->skin_adapt || !(c->skin > 0.) || c->convex) return GDEM_OK;
+  int32_t peak = 0;
+  int rc = list_peak(c, &peak);
+  if (rc) return rc;
+  if (peak <= kListCap - kListHeadroom) return GDEM_OK;
+  // a list that overflowed has raised GDEM_ERR_CAPACITY: report it
+  rc = collect_errors(c);
+  if (rc) return rc;
+  c->skin_halvings++;
+  return gdem_set_skin(c, 0.5 * c->skin);
+}
+
+extern "C" int gdem_set_skin_adapt(gdem_ctx* c, int32_t on) {
+  if (!c) return GDEM_ERR_ARG;
+  c->skin_adapt = on != 0;
+  return GDEM_OK;
+}
+
+extern "C" int gdem_get_list_occupancy(gdem_ctx* c, int32_t* peak, int32_t* capacity,
+                                       int32_t* skin_halvings) {
+  if (!c || !peak) return GDEM_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->dev));
+  const int rc = list_peak(c, peak);
+  if (rc) return rc;
+  if (capacity) *capacity = c->convex ? kCandCap : kListCap;
+  if (skin_halvings) *skin_halvings = c->skin_halvings;
   return GDEM_OK;
 }
 

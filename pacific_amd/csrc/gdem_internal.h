@@ -45,6 +45,8 @@ struct ContactRec {
 // Maximum contacts one particle can detect (outgoing) or receive (incoming)
 // per step; exceeding it raises GDEM_ERR_CAPACITY.
 constexpr int kListCap = 24;
+// skin lists: entries kept free before the skin is halved
+constexpr int kListHeadroom = 4;
 
 // Cells per block of the binning scan (1,024 threads x 4 cells); the
 // block totals bsum[] are indexed by cell / kScanCells.
@@ -371,6 +373,8 @@ void launch_binning(Bins& b, const PosSrc& src, int32_t n, const Grid& g, const 
 // The scan and the per-cell sort of a gated binning alone (the count and
 // scatter passes then run elsewhere: gdem_binning.h).
 void launch_scan(const Bins& b, int32_t ncell, SkinGate gate, hipStream_t st);
+void launch_list_peak(const int32_t* out, const int32_t* in, int32_t n, int32_t* peak,
+                      hipStream_t st);
 void launch_cellsort(const Bins& b, const PosSrc& src, int32_t ncell, int32_t n, SkinGate gate,
                      hipStream_t st);
 void launch_detect(const DetectArgs& a, hipStream_t st);

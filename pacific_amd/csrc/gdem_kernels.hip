@@ -478,6 +478,30 @@ void launch_binning(Bins& b, const PosSrc& src, int32_t n, const Grid& g, const 
 
 // The scan and the per-cell sort of a gated binning, for launch sequences
 // that run the count and scatter passes inside kernels of their own.
+// Largest list length of any particle (its outgoing or incoming entries at
+// the last list build): a fixed-size tree per workgroup, then one atomic.
+__global__ void __launch_bounds__(256) k_list_peak(const int32_t* out, const int32_t* in, int32_t n,
+                                                   int32_t* peak) {
+  __shared__ int32_t red[256];
+  int32_t m = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    m = max(m, max(out[i], in[i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(peak, red[0]);
+}
+
+void launch_list_peak(const int32_t* out, const int32_t* in, int32_t n, int32_t* peak,
+                      hipStream_t st) {
+  if (n <= 0) return;
+  const int blocks = std::min((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_list_peak, dim3(blocks), dim3(256), 0, st, out, in, n, peak);
+}
+
 void launch_scan(const Bins& b, int32_t ncell, SkinGate gate, hipStream_t st) {
   const int nb = nblocks(ncell, kScanCells);
   hipLaunchKernelGGL(k_scan, dim3(nb), dim3(kScanBlock), 0, st, b.cnt, b.local, b.bsum, b.ticket,

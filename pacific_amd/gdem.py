@@ -143,6 +143,8 @@ def load_library(path: str = LIB_PATH):
         "gdem_set_contact_stats": (C.c_int, [vp, i32]),
         "gdem_get_contact_stats": (C.c_int, [vp, P(d)]),
         "gdem_set_narrow_phase": (C.c_int, [vp, i32]),
+        "gdem_set_skin_adapt": (C.c_int, [vp, i32]),
+        "gdem_get_list_occupancy": (C.c_int, [vp, P(i32), P(i32), P(i32)]),
         "gdem_get_narrow_phase": (C.c_int, [vp, P(i32)]),
         "gdem_get_grid": (C.c_int, [vp, P(i32), P(d)]),
         "gdem_get_particles": (C.c_int, [vp, vp, vp, vp, vp]),
@@ -214,7 +216,8 @@ EXPORTED = ["gdem_create", "gdem_destroy", "gdem_set_particles", "gdem_add_wall"
             "gdem_get_contacts", "gdem_last_error", "gdem_set_kernel_timing",
             "gdem_get_counters", "gdem_reset_counters", "gdem_get_kernel_times",
             "gdem_set_contact_stats", "gdem_get_contact_stats", "gdem_get_grid",
-            "gdem_set_narrow_phase", "gdem_get_narrow_phase",
+            "gdem_set_narrow_phase", "gdem_get_narrow_phase", "gdem_set_skin_adapt",
+            "gdem_get_list_occupancy",
             "gdem_get_particles", "gdem_get_contact_model", "gdem_get_wall_shape",
             "gdem_set_material_name", "gdem_get_material_name", "gdem_set_wall_name",
             "gdem_get_wall_name", "gdem_write_reload", "gdem_read_reload",
@@ -620,6 +623,18 @@ class GdemSystem:
         """Switch on (and reset) the contact statistics of the run."""
         self._check(self.lib.gdem_set_contact_stats(self.h, int(bool(on))),
                     "gdem_set_contact_stats")
+
+    def set_skin_adapt(self, on=True):
+        """Halve the skin when a list build nears the list capacity (default on)."""
+        self._check(self.lib.gdem_set_skin_adapt(self.h, int(bool(on))), "gdem_set_skin_adapt")
+
+    def list_occupancy(self):
+        """Longest list of the last build, the list capacity and the skin
+        halvings so far."""
+        peak, cap, halv = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.lib.gdem_get_list_occupancy(self.h, C.byref(peak), C.byref(cap),
+                                                     C.byref(halv)), "gdem_get_list_occupancy")
+        return {"peak": peak.value, "capacity": cap.value, "skin_halvings": halv.value}
 
     def set_narrow_phase(self, name):
         """Narrow phase of convex contacts: "GJK" (Johnson, the default) or
