@@ -358,6 +358,8 @@ def main():
                     help="skip the batched calibration-gradient part of the default line")
     ap.add_argument("--no-looped", action="store_true",
                     help="calibration: skip the looped single-system baseline")
+    ap.add_argument("--narrow-phase", choices=["GJK", "GJK_SV"], default="GJK",
+                    help="convex_box: Johnson (default) or signed-volume GJK")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "the DEM step runs on one GPU: it is not sharded across "
@@ -380,6 +382,7 @@ def main():
         args.cpu_steps = 5 if convex else 20
     if convex:
         wl = cached_convex(args.n or 200_000, 3 if args.seed is None else args.seed)
+        wl["narrow_phase"] = args.narrow_phase
     else:
         wl = cached_workload(args.n or 1_000_000, 2 if args.seed is None else args.seed)
     n = len(wl["particles"]["x"])
@@ -458,7 +461,8 @@ def main():
             "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": wl["name"], "n_particles": n, "shapes": "cube 2 mm / tetrahedron 2.5 mm",
-                       "solid_fraction": 0.25, "contact": "Hooke+Coulomb, GJK", "dt": dt,
+                       "solid_fraction": 0.25, "dt": dt,
+                       "contact": f"Hooke+Coulomb, {wl.get('narrow_phase', 'GJK')}",
                        "grid": nc, "parallelism": "single GPU"},
             "detect_force_us": k_ms * 1e3, "mean_contacts": k_contacts,
             "cpu_baseline": cpu,
